@@ -1,0 +1,131 @@
+/*
+ * mumblebee_hip.h — C ABI of the MI355X conic interior-point solve path.
+ *
+ * The entry points replace, one for one, the Julia calls of Mumblebee.jl's IPM solve path
+ * (paths relative to the Mumblebee.jl repository root):
+ *
+ *   mb_problem_create        IPMProblem(Q, B, f, g, μ, K, s; compress)   src/IPM/src/ipm/problem.jl:64-127
+ *   mb_problem_symbolic*     symbkkt → CliqueTrees.symbolic               src/IPM/src/kkt/kkt.jl:23-53
+ *   mb_settings_default      IPMSettings{Float64}() defaults              src/IPM/src/ipm/settings/ipm.jl:1-18
+ *   mb_solver_create         init(prob, settings; p0, d0, y0)             src/IPM/src/ipm/solver/ipm.jl:431-491
+ *   mb_solver_step           step!(solver)                                src/IPM/src/ipm/solver/ipm.jl:577-774
+ *   mb_solver_solve          solve!(solver)                               src/IPM/src/ipm/solver/solver.jl:119-150
+ *   mb_solver_result         result(solver, status) → IPMResult           src/IPM/src/ipm/solver/ipm.jl:30-74
+ *   mb_solver_history        IPMHistory rows                              src/IPM/src/ipm/history/ipm.jl:1-24
+ *   mb_solver_kkt_factor     initkkt!(kkt, H; δ) → (ok, ρ)                src/IPM/src/kkt/uzawa.jl:73-115
+ *
+ * Sparse inputs are CSC with 0-based int64 indices, rows sorted within each column (stored zeros
+ * are kept, as in Julia's SparseMatrixCSC). Cone kinds: 0 nonnegative, 1 free ("cofree"),
+ * 2 second-order, 3 PSD (svec), 4 exponential, 5 power (cone_param = alpha).
+ *
+ * Errors: functions returning a pointer return NULL, functions returning int return a negative
+ * value; mb_last_error() describes the last failure of the calling thread. Nothing is thrown
+ * across the ABI. Handles are not re-entrant: one host thread per solver at a time.
+ *
+ * Status codes follow the reference's @enum order: IPMStatus CONTINUE=0, OPTIMAL=1,
+ * NEAR_OPTIMAL=2, STALLED=3, NUMERICAL_FAILURE=4, ITERATION_LIMIT=5, PRIMAL_INFEASIBLE=6,
+ * DUAL_INFEASIBLE=7, ILL_POSED=8, NEAR_PRIMAL_INFEASIBLE=9, NEAR_DUAL_INFEASIBLE=10,
+ * NEAR_ILL_POSED=11 (src/IPM/src/ipm/ipm.jl:1); KKTStatus KKT_CONTINUE=0, KKT_SOLVED=1,
+ * KKT_STAGNATED=2, KKT_ITMAX=3, KKT_NUMERICAL_FAILURE=4, KKT_IRMAX=5 (src/IPM/src/kkt/kkt.jl:1).
+ */
+#ifndef MUMBLEBEE_HIP_H
+#define MUMBLEBEE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mb_problem mb_problem;
+typedef struct mb_solver mb_solver;
+
+/* IPMSettings (src/IPM/src/ipm/settings/ipm.jl:1-18) plus build-specific knobs. */
+typedef struct mb_settings {
+  int32_t verbose;
+  double step_frac;
+  double feas_tol;
+  double gap_tol;
+  int32_t max_iter;
+  double near_factor;
+  double stall_tol;
+  double refine_stall_tol;
+  int32_t scale_max_iter;
+  int32_t refine_max_iter;
+  int32_t newton_max_iter;
+  double aug_tol;
+  double infeas_abs;
+  double infeas_rel;
+  int32_t amalgamate; /* relaxed supernode amalgamation (1 default) */
+  int32_t precision;  /* 0: FP64 factor */
+} mb_settings;
+
+/* Fill the reference defaults. */
+void mb_settings_default(mb_settings* s);
+
+/* IPMProblem(Q, B, f, g, μ, K, s; compress) — problem.jl:64-127. */
+mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, const int64_t* Q_rowval,
+                              const double* Q_nzval, const int64_t* B_colptr, const int64_t* B_rowval,
+                              const double* B_nzval, const double* f, const double* g, double mu,
+                              int64_t ncone, const int32_t* cone_kind, const double* cone_param,
+                              const int64_t* cone_size, double compress);
+void mb_problem_destroy(mb_problem* p);
+
+/* dims[0..6] = nvtx, nout, B narc, B nbnz, Q narc, Q nbnz, n */
+int mb_problem_dims(const mb_problem* p, int64_t* dims);
+
+/* Block-sparse Q and B in the layout of block_sparse_matrix.jl:10-158 (0-based), permuted f, g,
+ * row / column permutations P1, P2 (new -> old), per-vertex cone kinds. Any pointer may be NULL. */
+int mb_problem_export(const mb_problem* p, int64_t* P1, int64_t* P2, int32_t* vertex_kind, int64_t* B_xsrc,
+                      int64_t* B_xcol, int64_t* B_xrow, int64_t* B_xblk, int64_t* B_tgt, double* B_val,
+                      int64_t* Q_xsrc, int64_t* Q_xblk, int64_t* Q_tgt, double* Q_val, double* f, double* g);
+
+/* Symbolic analysis of δH + BᵀB (kkt.jl:23-53 with the build's own ordering).
+ * dims[0..2] = nvtx, nfront, total separator length. */
+int mb_problem_symbolic_dims(mb_problem* p, int32_t amalgamate, int64_t* dims);
+/* vperm[nvtx] (new -> old vertex), front_vtx[nfront+1], sep_ptr[nfront+1], sep[...] (new vertex
+ * indices), parent[nfront], flops[2] = {exact, amalgamated}. */
+int mb_problem_symbolic(mb_problem* p, int32_t amalgamate, int64_t* vperm, int64_t* front_vtx, int64_t* sep_ptr,
+                        int64_t* sep, int64_t* parent, double* flops);
+
+/* init(prob, settings; p0, d0, y0): symbolic analysis, equilibration, device allocation,
+ * starting point. p0 / d0 / y0 may be NULL (user space, length n / n / m). */
+mb_solver* mb_solver_create(const mb_problem* p, const mb_settings* s, const double* p0, const double* d0,
+                            const double* y0);
+void mb_solver_destroy(mb_solver* s);
+
+/* One IPM iteration; returns the IPMStatus (CONTINUE = 0 to keep going). */
+int mb_solver_step(mb_solver* s);
+/* Iterate until the status leaves CONTINUE; returns it. */
+int mb_solver_solve(mb_solver* s);
+
+/* IPMResult: p, d (length n), y (length m) in user space; info[0..7] = mu, pres, dres, pobj,
+ * dobj, niter, nsolve, status. Any pointer may be NULL. */
+int mb_solver_result(mb_solver* s, double* p, double* d, double* y, double* info);
+
+int64_t mb_solver_niter(const mb_solver* s);
+/* rows[niter][17] = pres dres pobj dobj step piter ppass pstat citer cpass cstat dmin dmax rho
+ * delta mu chi (history/ipm.jl:1-24) */
+int mb_solver_history(const mb_solver* s, double* rows);
+
+/* stats[0..6] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
+ * total factorization ms, factorization calls, last factorization ms */
+int mb_solver_stats(const mb_solver* s, double* stats);
+
+/* initkkt!(kkt, H; δ) on the solver's current H (uzawa.jl:73-115): returns 1 on success,
+ * 0 on a failed factorization, negative on error; *rho receives the shift used. */
+int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);
+
+/* Debug: the current factor as a dense n×n column-major lower triangle (small n only). */
+int mb_solver_dense_factor(mb_solver* s, double* L);
+
+/* Lane-layout probe of v_mfma_f64_16x16x4f64 (256 doubles out). */
+int mb_probe_mfma_f64(double* out);
+
+const char* mb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MUMBLEBEE_HIP_H */

@@ -1,0 +1,300 @@
+// extern "C" boundary (include/mumblebee_hip.h). Catches every exception; errors are
+// reported through return codes and mb_last_error().
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/mumblebee_hip.h"
+#include "solver.h"
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& fn) {
+  try {
+    return fn();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -1;
+  }
+}
+}  // namespace
+
+struct mb_problem {
+  mb::Problem P;
+  int64_t n = 0, m = 0;
+  int sym_amalg = -1;
+  mb::Symbolic sym;
+};
+
+struct mb_solver {
+  std::unique_ptr<mb::Solver> S;
+};
+
+static mb::CSC make_csc(int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval, const double* nzval) {
+  mb::CSC A;
+  A.m = m;
+  A.n = n;
+  A.colptr.assign(colptr, colptr + n + 1);
+  int64_t nz = A.colptr[n];
+  A.rowval.assign(rowval, rowval + nz);
+  A.nzval.assign(nzval, nzval + nz);
+  for (int64_t j = 0; j < n; ++j) {
+    if (A.colptr[j + 1] < A.colptr[j]) throw std::invalid_argument("CSC colptr not monotone");
+    for (int64_t p = A.colptr[j]; p < A.colptr[j + 1]; ++p) {
+      if (A.rowval[p] < 0 || A.rowval[p] >= m) throw std::invalid_argument("CSC row index out of range");
+      if (p > A.colptr[j] && A.rowval[p] <= A.rowval[p - 1]) throw std::invalid_argument("CSC rows not sorted");
+    }
+  }
+  return A;
+}
+
+static void ensure_symbolic(mb_problem* p, int amalg) {
+  if (p->sym_amalg == amalg) return;
+  std::vector<int64_t> w;
+  std::vector<std::vector<int64_t>> adj;
+  mb::vertex_graph(p->P.B, p->P.Q, w, adj);
+  p->sym = mb::analyse(w, adj, amalg != 0);
+  p->sym_amalg = amalg;
+}
+
+extern "C" {
+
+void mb_settings_default(mb_settings* s) {
+  mb::Settings d;
+  s->verbose = d.verbose;
+  s->step_frac = d.step_frac;
+  s->feas_tol = d.feas_tol;
+  s->gap_tol = d.gap_tol;
+  s->max_iter = d.max_iter;
+  s->near_factor = d.near_factor;
+  s->stall_tol = d.stall_tol;
+  s->refine_stall_tol = d.refine_stall_tol;
+  s->scale_max_iter = d.scale_max_iter;
+  s->refine_max_iter = d.refine_max_iter;
+  s->newton_max_iter = d.newton_max_iter;
+  s->aug_tol = d.aug_tol;
+  s->infeas_abs = d.infeas_abs;
+  s->infeas_rel = d.infeas_rel;
+  s->amalgamate = d.amalgamate;
+  s->precision = 0;
+}
+
+mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, const int64_t* Q_rowval,
+                              const double* Q_nzval, const int64_t* B_colptr, const int64_t* B_rowval,
+                              const double* B_nzval, const double* f, const double* g, double mu, int64_t ncone,
+                              const int32_t* cone_kind, const double* cone_param, const int64_t* cone_size,
+                              double compress) {
+  mb_problem* out = nullptr;
+  guard([&]() {
+    if (n < 0 || m < 0 || ncone < 0) throw std::invalid_argument("negative dimension");
+    mb::CSC Q = make_csc(n, n, Q_colptr, Q_rowval, Q_nzval);
+    mb::CSC B = make_csc(m, n, B_colptr, B_rowval, B_nzval);
+    std::vector<mb::Cone> K(ncone);
+    std::vector<int64_t> s(cone_size, cone_size + ncone);
+    for (int64_t k = 0; k < ncone; ++k) {
+      if (cone_kind[k] < 0 || cone_kind[k] > 5) throw std::invalid_argument("unknown cone kind");
+      if (cone_size[k] < 0) throw std::invalid_argument("negative cone size");
+      K[k].kind = cone_kind[k];
+      K[k].param = cone_param ? cone_param[k] : 0.0;
+    }
+    std::vector<double> fv(f, f + n), gv(g, g + m);
+    auto p = std::make_unique<mb_problem>();
+    p->P = mb::make_problem(Q, B, fv, gv, mu, K, s, compress);
+    p->n = n;
+    p->m = m;
+    out = p.release();
+    return 0;
+  });
+  return out;
+}
+
+void mb_problem_destroy(mb_problem* p) { delete p; }
+
+int mb_problem_dims(const mb_problem* p, int64_t* dims) {
+  return guard([&]() {
+    dims[0] = p->P.B.nvtx;
+    dims[1] = p->P.B.nout;
+    dims[2] = p->P.B.narc();
+    dims[3] = p->P.B.nbnz();
+    dims[4] = p->P.Q.narc();
+    dims[5] = p->P.Q.nbnz();
+    dims[6] = p->n;
+    return 0;
+  });
+}
+
+int mb_problem_export(const mb_problem* p, int64_t* P1, int64_t* P2, int32_t* vertex_kind, int64_t* B_xsrc,
+                      int64_t* B_xcol, int64_t* B_xrow, int64_t* B_xblk, int64_t* B_tgt, double* B_val,
+                      int64_t* Q_xsrc, int64_t* Q_xblk, int64_t* Q_tgt, double* Q_val, double* f, double* g) {
+  return guard([&]() {
+    const mb::Problem& P = p->P;
+    auto cp = [](auto* dst, const auto& v) {
+      if (dst) std::copy(v.begin(), v.end(), dst);
+    };
+    cp(P1, P.P1);
+    cp(P2, P.P2);
+    if (vertex_kind)
+      for (size_t v = 0; v < P.K.size(); ++v) vertex_kind[v] = P.K[v].kind;
+    cp(B_xsrc, P.B.xsrc);
+    cp(B_xcol, P.B.xcol);
+    cp(B_xrow, P.B.xrow);
+    cp(B_xblk, P.B.xblk);
+    cp(B_tgt, P.B.tgt);
+    cp(B_val, P.B.val);
+    cp(Q_xsrc, P.Q.xsrc);
+    cp(Q_xblk, P.Q.xblk);
+    cp(Q_tgt, P.Q.tgt);
+    cp(Q_val, P.Q.val);
+    cp(f, P.f);
+    cp(g, P.g);
+    return 0;
+  });
+}
+
+int mb_problem_symbolic_dims(mb_problem* p, int32_t amalgamate, int64_t* dims) {
+  return guard([&]() {
+    ensure_symbolic(p, amalgamate);
+    int64_t tot = 0;
+    for (auto& s : p->sym.front_sep) tot += (int64_t)s.size();
+    dims[0] = (int64_t)p->sym.vperm.size();
+    dims[1] = p->sym.nfront();
+    dims[2] = tot;
+    return 0;
+  });
+}
+
+int mb_problem_symbolic(mb_problem* p, int32_t amalgamate, int64_t* vperm, int64_t* front_vtx, int64_t* sep_ptr,
+                        int64_t* sep, int64_t* parent, double* flops) {
+  return guard([&]() {
+    ensure_symbolic(p, amalgamate);
+    const mb::Symbolic& S = p->sym;
+    if (vperm) std::copy(S.vperm.begin(), S.vperm.end(), vperm);
+    if (front_vtx) std::copy(S.front_vtx.begin(), S.front_vtx.end(), front_vtx);
+    int64_t k = 0;
+    for (int64_t f = 0; f < S.nfront(); ++f) {
+      if (sep_ptr) sep_ptr[f] = k;
+      for (int64_t u : S.front_sep[f]) {
+        if (sep) sep[k] = u;
+        ++k;
+      }
+    }
+    if (sep_ptr) sep_ptr[S.nfront()] = k;
+    if (parent) std::copy(S.front_parent.begin(), S.front_parent.end(), parent);
+    if (flops) {
+      flops[0] = S.exact_flops;
+      double t = 0.0;
+      for (int64_t f = 0; f < S.nfront(); ++f) {
+        double r = (double)(S.xcol[S.front_vtx[f + 1]] - S.xcol[S.front_vtx[f]]);
+        double s = 0.0;
+        for (int64_t u : S.front_sep[f]) s += (double)(S.xcol[u + 1] - S.xcol[u]);
+        t += r * r * r / 3.0 + s * r * r + s * s * r;
+      }
+      flops[1] = t;
+    }
+    return 0;
+  });
+}
+
+mb_solver* mb_solver_create(const mb_problem* p, const mb_settings* s, const double* p0, const double* d0,
+                            const double* y0) {
+  mb_solver* out = nullptr;
+  guard([&]() {
+    mb::Settings st;
+    if (s) {
+      st.verbose = s->verbose;
+      st.step_frac = s->step_frac;
+      st.feas_tol = s->feas_tol;
+      st.gap_tol = s->gap_tol;
+      st.max_iter = s->max_iter;
+      st.near_factor = s->near_factor;
+      st.stall_tol = s->stall_tol;
+      st.refine_stall_tol = s->refine_stall_tol;
+      st.scale_max_iter = s->scale_max_iter;
+      st.refine_max_iter = s->refine_max_iter;
+      st.newton_max_iter = s->newton_max_iter;
+      st.aug_tol = s->aug_tol;
+      st.infeas_abs = s->infeas_abs;
+      st.infeas_rel = s->infeas_rel;
+      st.amalgamate = s->amalgamate;
+      if (s->precision != 0) throw std::invalid_argument("precision mode not available");
+    }
+    auto h = std::make_unique<mb_solver>();
+    h->S = std::make_unique<mb::Solver>(p->P, st, p0, d0, y0);
+    out = h.release();
+    return 0;
+  });
+  return out;
+}
+
+void mb_solver_destroy(mb_solver* s) { delete s; }
+
+int mb_solver_step(mb_solver* s) { return guard([&]() { return s->S->step(); }); }
+int mb_solver_solve(mb_solver* s) { return guard([&]() { return s->S->solve(); }); }
+
+int mb_solver_result(mb_solver* s, double* p, double* d, double* y, double* info) {
+  return guard([&]() {
+    s->S->result(p, d, y, info);
+    return 0;
+  });
+}
+
+int64_t mb_solver_niter(const mb_solver* s) { return (int64_t)s->S->history().size(); }
+
+int mb_solver_history(const mb_solver* s, double* rows) {
+  return guard([&]() {
+    size_t k = 0;
+    for (const mb::HistRow& h : s->S->history()) {
+      double r[17] = {h.pres, h.dres, h.pobj, h.dobj, h.step, (double)h.piter, (double)h.ppass, (double)h.pstat,
+                      (double)h.citer, (double)h.cpass, (double)h.cstat, h.dmin, h.dmax, h.rho, h.delta, h.mu,
+                      h.chi};
+      std::memcpy(rows + 17 * k, r, sizeof(r));
+      ++k;
+    }
+    return 0;
+  });
+}
+
+int mb_solver_stats(const mb_solver* s, double* st) {
+  return guard([&]() {
+    st[0] = s->S->exact_flops();
+    st[1] = s->S->front_flops();
+    st[2] = s->S->nfront();
+    st[3] = s->S->nlevel();
+    st[4] = s->S->total_factor_ms();
+    st[5] = s->S->factor_calls();
+    st[6] = s->S->last_factor_ms();
+    return 0;
+  });
+}
+
+int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho) {
+  return guard([&]() { return s->S->kkt_factor(delta, rho); });
+}
+
+int mb_solver_dense_factor(mb_solver* s, double* L) {
+  return guard([&]() {
+    s->S->dense_factor(L);
+    return 0;
+  });
+}
+
+int mb_probe_mfma_f64(double* out) {
+  return guard([&]() {
+    double* d = nullptr;
+    if (hipMalloc(&d, 256 * sizeof(double)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    mb::launch_probe_mfma(nullptr, d);
+    hipError_t e = hipMemcpy(out, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+    return 0;
+  });
+}
+
+const char* mb_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

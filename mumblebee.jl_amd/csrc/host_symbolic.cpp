@@ -1,0 +1,302 @@
+// Symbolic analysis of δH + BᵀB on the cone-block vertex graph.
+//
+// The vertex graph restates `weightedgraph` (src/IPM/src/utils.jl:135-145): vertex weights
+// are block widths, edges are pattern(Q) ∪ linegraph(B). The reference obtains the ordering,
+// elimination tree and fronts from CliqueTrees.jl `symbolic` (kkt.jl:36), whose source is not
+// available; this file defines the build's own: weighted exact-external-degree minimum degree
+// on the quotient graph (ties to the smallest vertex), elimination-tree postorder, fundamental
+// supernodes, then relaxed amalgamation so that fronts are dense enough for MFMA tiles.
+#include <algorithm>
+#include <set>
+#include <stdexcept>
+
+#include "host.h"
+
+namespace mb {
+
+void vertex_graph(const BSM& B, const BSM& Q, std::vector<i64>& weights,
+                  std::vector<std::vector<i64>>& adj) {
+  const i64 N = B.nvtx;
+  weights.resize(N);
+  for (i64 v = 0; v < N; ++v) weights[v] = B.ncols(v);
+  std::vector<std::vector<i64>> rows(B.nout);
+  for (i64 v = 0; v < N; ++v)
+    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) rows[B.tgt[e]].push_back(v);
+  std::vector<std::vector<i64>> a(N);
+  for (i64 v = 0; v < Q.nvtx; ++v)
+    for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) {
+      i64 u = Q.tgt[e];
+      if (u != v) {
+        a[u].push_back(v);
+        a[v].push_back(u);
+      }
+    }
+  for (auto& lst : rows)
+    for (i64 x : lst)
+      for (i64 y : lst)
+        if (x != y) a[x].push_back(y);
+  for (auto& l : a) {
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+  }
+  adj.swap(a);
+}
+
+namespace {
+
+struct MinDegree {
+  i64 N;
+  const std::vector<i64>& w;
+  std::vector<std::vector<i64>> A, E, L;  // L indexed by element (= eliminated vertex)
+  std::vector<i64> mark, emark;
+  i64 stamp = 0, estamp = 0;
+  std::vector<i64> deg;
+  std::vector<char> alive;
+
+  MinDegree(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj)
+      : N((i64)weights.size()), w(weights), A(adj), E(weights.size()), L(weights.size()),
+        mark(weights.size(), 0), emark(weights.size(), 0), deg(weights.size(), 0),
+        alive(weights.size(), 1) {}
+
+  // reach(i) collected into out (excluding i); returns weighted size
+  i64 reach(i64 i, std::vector<i64>* out) {
+    ++stamp;
+    mark[i] = stamp;
+    i64 d = 0;
+    for (i64 j : A[i])
+      if (mark[j] != stamp) {
+        mark[j] = stamp;
+        d += w[j];
+        if (out) out->push_back(j);
+      }
+    for (i64 e : E[i])
+      for (i64 j : L[e])
+        if (mark[j] != stamp) {
+          mark[j] = stamp;
+          d += w[j];
+          if (out) out->push_back(j);
+        }
+    return d;
+  }
+
+  void run(std::vector<i64>& order, std::vector<std::vector<i64>>& strct) {
+    order.clear();
+    strct.assign(N, {});
+    i64 W = 0;
+    for (i64 x : w) W += x;
+    std::set<std::pair<i64, i64>> pq;
+    for (i64 i = 0; i < N; ++i) {
+      deg[i] = reach(i, nullptr);
+      pq.insert({deg[i], i});
+    }
+    std::vector<i64> Lp;
+    while (!pq.empty()) {
+      i64 p = pq.begin()->second;
+      if (deg[p] + w[p] == W) {
+        // p reaches every remaining variable: after it the rest is a clique, where minimum
+        // degree picks by (-weight, index)
+        pq.erase(pq.begin());
+        std::vector<i64> rest;
+        for (auto& it : pq) rest.push_back(it.second);
+        std::sort(rest.begin(), rest.end());
+        order.push_back(p);
+        strct[p] = rest;
+        std::stable_sort(rest.begin(), rest.end(), [&](i64 a, i64 b) {
+          if (w[a] != w[b]) return w[a] > w[b];
+          return a < b;
+        });
+        for (size_t k = 0; k < rest.size(); ++k) {
+          order.push_back(rest[k]);
+          std::vector<i64> s(rest.begin() + k + 1, rest.end());
+          std::sort(s.begin(), s.end());
+          strct[rest[k]] = s;
+        }
+        break;
+      }
+      pq.erase(pq.begin());
+      Lp.clear();
+      reach(p, &Lp);
+      std::sort(Lp.begin(), Lp.end());
+      strct[p] = Lp;
+      // mark L_p members and E_p elements
+      ++stamp;
+      for (i64 j : Lp) mark[j] = stamp;
+      mark[p] = stamp;
+      ++estamp;
+      for (i64 e : E[p]) emark[e] = estamp;
+      for (i64 i : Lp) {
+        auto& Ai = A[i];
+        Ai.erase(std::remove_if(Ai.begin(), Ai.end(), [&](i64 j) { return mark[j] == stamp; }), Ai.end());
+        auto& Ei = E[i];
+        Ei.erase(std::remove_if(Ei.begin(), Ei.end(), [&](i64 e) { return emark[e] == estamp; }), Ei.end());
+        Ei.push_back(p);
+      }
+      for (i64 e : E[p]) std::vector<i64>().swap(L[e]);
+      L[p] = Lp;
+      std::vector<i64>().swap(A[p]);
+      std::vector<i64>().swap(E[p]);
+      alive[p] = 0;
+      W -= w[p];
+      order.push_back(p);
+      for (i64 i : Lp) {
+        pq.erase({deg[i], i});
+        deg[i] = reach(i, nullptr);
+        pq.insert({deg[i], i});
+      }
+    }
+  }
+};
+
+}  // namespace
+
+Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj,
+                 bool amalgamate) {
+  const i64 N = (i64)weights.size();
+  std::vector<i64> order;
+  std::vector<std::vector<i64>> strct;
+  {
+    MinDegree md(weights, adj);
+    md.run(order, strct);
+  }
+  std::vector<i64> pos(N);
+  for (i64 k = 0; k < N; ++k) pos[order[k]] = k;
+  std::vector<i64> parent(N, -1);
+  for (i64 v = 0; v < N; ++v) {
+    i64 best = -1;
+    for (i64 u : strct[v])
+      if (best < 0 || pos[u] < pos[best]) best = u;
+    parent[v] = best;
+  }
+  std::vector<std::vector<i64>> children(N);
+  std::vector<i64> roots;
+  for (i64 k = 0; k < N; ++k) {
+    i64 v = order[k];
+    if (parent[v] >= 0) children[parent[v]].push_back(v);
+    else roots.push_back(v);
+  }
+  std::vector<i64> post;
+  post.reserve(N);
+  for (i64 r : roots) {
+    std::vector<std::pair<i64, size_t>> st{{r, 0}};
+    while (!st.empty()) {
+      auto& top = st.back();
+      i64 v = top.first;
+      if (top.second < children[v].size()) {
+        i64 c = children[v][top.second++];
+        st.push_back({c, 0});
+      } else {
+        post.push_back(v);
+        st.pop_back();
+      }
+    }
+  }
+  std::vector<i64> newpos(N);
+  for (i64 k = 0; k < N; ++k) newpos[post[k]] = k;
+  std::vector<i64> npar(N), nchild(N, 0);
+  std::vector<std::vector<i64>> nstr(N);
+  for (i64 k = 0; k < N; ++k) {
+    i64 v = post[k];
+    npar[k] = parent[v] >= 0 ? newpos[parent[v]] : -1;
+    for (i64 u : strct[v]) nstr[k].push_back(newpos[u]);
+    std::sort(nstr[k].begin(), nstr[k].end());
+  }
+  for (i64 k = 0; k < N; ++k)
+    if (npar[k] >= 0) nchild[npar[k]]++;
+  // fundamental supernodes
+  std::vector<i64> fv{0};
+  for (i64 k = 0; k + 1 < N; ++k) {
+    bool merge = npar[k] == k + 1 && nchild[k + 1] == 1 && nstr[k].size() == nstr[k + 1].size() + 1;
+    if (!merge) fv.push_back(k + 1);
+  }
+  if (N > 0) fv.push_back(N);
+  const i64 F = (i64)fv.size() - 1;
+  std::vector<i64> vfront(N);
+  for (i64 f = 0; f < F; ++f)
+    for (i64 k = fv[f]; k < fv[f + 1]; ++k) vfront[k] = f;
+  std::vector<i64> wpost(N), xc(N + 1, 0);
+  for (i64 k = 0; k < N; ++k) {
+    wpost[k] = weights[post[k]];
+    xc[k + 1] = xc[k] + wpost[k];
+  }
+  std::vector<i64> fpar(F), r(F), s(F);
+  std::vector<std::vector<i64>> fsep(F);
+  double flops = 0.0;
+  for (i64 f = 0; f < F; ++f) {
+    fsep[f] = nstr[fv[f + 1] - 1];
+    fpar[f] = fsep[f].empty() ? -1 : vfront[fsep[f][0]];
+    r[f] = xc[fv[f + 1]] - xc[fv[f]];
+    s[f] = 0;
+    for (i64 u : fsep[f]) s[f] += wpost[u];
+    double rr = (double)r[f], ss = (double)s[f];
+    flops += rr * rr * rr / 3.0 + ss * rr * rr + ss * ss * rr;
+  }
+  // relaxed amalgamation (merge a child front into its parent when the explicit zeros this
+  // adds stay below a size-dependent fraction)
+  std::vector<i64> rep(F);
+  for (i64 f = 0; f < F; ++f) rep[f] = f;
+  std::vector<std::vector<i64>> members(F);
+  for (i64 f = 0; f < F; ++f) members[f] = {f};
+  if (amalgamate) {
+    std::vector<std::vector<i64>> fch(F);
+    for (i64 f = 0; f < F; ++f)
+      if (fpar[f] >= 0) fch[fpar[f]].push_back(f);
+    std::vector<double> z(F, 0.0);
+    std::vector<i64> rr(r);
+    for (i64 f = 0; f < F; ++f) {
+      std::vector<i64> mem;
+      for (i64 c : fch[f]) {
+        double rc = (double)rr[c], rf = (double)rr[f];
+        double rt = rc + rf, st = (double)s[f];
+        double zz = z[c] + z[f] + rc * (rf + st - (double)s[c]);
+        double tot = rt * (rt + 1.0) / 2.0 + rt * st;
+        bool ok = rt <= 4 || (rt <= 16 && zz <= 0.8 * tot) || (rt <= 48 && zz <= 0.1 * tot) ||
+                  zz <= 0.05 * tot;
+        if (ok) {
+          rr[f] = (i64)rt;
+          z[f] = zz;
+          rep[c] = f;
+          mem.insert(mem.end(), members[c].begin(), members[c].end());
+          members[c].clear();
+        }
+      }
+      mem.push_back(f);
+      members[f] = mem;
+    }
+  }
+  auto find = [&](i64 f) {
+    while (rep[f] != f) f = rep[f];
+    return f;
+  };
+  // new vertex order: representatives in increasing index, each listing its members' vertices
+  std::vector<i64> reps;
+  for (i64 f = 0; f < F; ++f)
+    if (rep[f] == f) reps.push_back(f);
+  const i64 NF = (i64)reps.size();
+  std::vector<i64> repidx(F, -1);
+  for (i64 k = 0; k < NF; ++k) repidx[reps[k]] = k;
+  Symbolic S;
+  S.front_vtx.push_back(0);
+  std::vector<i64> vnew(N);  // postorder vertex -> final vertex index
+  i64 cnt = 0;
+  for (i64 R : reps) {
+    for (i64 f : members[R])
+      for (i64 k = fv[f]; k < fv[f + 1]; ++k) {
+        vnew[k] = cnt++;
+        S.vperm.push_back(post[k]);
+      }
+    S.front_vtx.push_back(cnt);
+  }
+  S.xcol.assign(N + 1, 0);
+  for (i64 k = 0; k < N; ++k) S.xcol[k + 1] = S.xcol[k] + weights[S.vperm[k]];
+  for (i64 R : reps) {
+    std::vector<i64> sep;
+    for (i64 u : fsep[R]) sep.push_back(vnew[u]);
+    std::sort(sep.begin(), sep.end());
+    S.front_sep.push_back(sep);
+    S.front_parent.push_back(fpar[R] >= 0 ? repidx[find(fpar[R])] : -1);
+  }
+  S.exact_flops = flops;
+  return S;
+}
+
+}  // namespace mb

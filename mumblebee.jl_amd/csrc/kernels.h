@@ -1,0 +1,84 @@
+// Launch wrappers of the HIP kernels and the per-instance device table they read.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "device.h"
+#include "kinds.h"
+
+namespace mb {
+
+// n-vector kinds (one arena of NV_COUNT × n doubles per instance)
+enum NVKind : int {
+  NV_P = 0, NV_D, NV_F, NV_DF, NV_FR, NV_QP, NV_DPA, NV_DDA, NV_DP, NV_DD, NV_PSCL,
+  NV_KF, NV_KX, NV_CW, NV_T0, NV_T1, NV_T2, NV_COUNT
+};
+// m-vector kinds
+enum MVKind : int {
+  MV_Y = 0, MV_G, MV_DG, MV_DYA, MV_DY, MV_YSCL, MV_KG, MV_KY, MV_CP, MV_T0, MV_T1, MV_COUNT
+};
+
+struct InstDev {
+  double* nv;       // NV_COUNT × n
+  double* mv;       // MV_COUNT × m
+  double* sc;       // NSCALAR
+  double* scratch;  // 3n doubles for cone correctors
+  double* F;        // frontal storage base
+  CSRDev B, Bt, Q;  // B (m rows), Bᵀ (n rows), Q (n rows)
+  int n, m;
+};
+
+struct Tile {
+  int inst;
+  int start;
+};
+
+// coefficient value: c (slot < 0), c·sc[slot], or c/sc[slot] (inv)
+struct Coef {
+  double c;
+  int slot;
+  int inv;
+};
+inline Coef K(double c) { return Coef{c, -1, 0}; }
+inline Coef KS(double c, int slot) { return Coef{c, slot, 0}; }
+inline Coef KI(double c, int slot) { return Coef{c, slot, 1}; }
+
+void launch_axpby(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kz, Coef a,
+                  Coef b);
+void launch_axpbypcz(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kw, int kz,
+                     Coef a, Coef b, Coef c);
+void launch_vmul(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kz, int mode);
+void launch_dot(hipStream_t, const InstDev*, const Tile*, int nt, const int* tile_ofs, int ninst, int space,
+                int kx, int ky, int mode, int kw0, int kw1, double* partial, int slot, int op);
+void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
+void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky, Coef a);
+void launch_cone_scale(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
+                       double* out);
+void launch_cone_hnorm2(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
+                        double* out, int slot);
+void launch_cone_corr0(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr);
+void launch_cone_corr(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr, int kdp, int kdd);
+void launch_cone_maxstep(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
+                         int kdp, int kdd, double step_frac, double* out);
+void launch_zerofree(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx);
+void launch_fill(hipStream_t, double* x, size_t n, double v);
+void launch_assemble_union(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* lval,
+                           const double* qval, int64_t nent);
+void launch_assemble_cones(hipStream_t, const InstDev*, const ConeDesc*, const int* cone_front, const FrontDesc*,
+                           int nc);
+void launch_add_diag(hipStream_t, const FrontDesc*, int nf, const InstDev*);
+void launch_front_norm2(hipStream_t, const FrontDesc*, int nf, double* partial);
+void launch_extend_add(hipStream_t, const FrontDesc*, const int* items, const int* item_tile0, int nitem,
+                       int ntile);
+void launch_panel(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*);
+void launch_update(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*);
+void launch_fsolve(hipStream_t, const FrontDesc*, const int* items, int nitem, const int* childlist,
+                   const InstDev*, int kv);
+void launch_bsolve(hipStream_t, const FrontDesc*, const int* items, int nitem, const InstDev*, int kv);
+void launch_cg_scalars(hipStream_t, const InstDev*, int ninst, int op);
+void launch_probe_mfma(hipStream_t, double* out);
+void launch_set_scalar(hipStream_t, double* sc, int slot, double v);
+
+}  // namespace mb

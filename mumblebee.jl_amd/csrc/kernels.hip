@@ -1,0 +1,1026 @@
+// HIP kernels for the MI355X conic IPM solve path (gfx950, 64-wide wavefronts).
+//
+//  * vector algebra over per-instance tile lists, deterministic two-pass reductions;
+//  * CSR sparse matrix-vector products for B, Bᵀ and Q;
+//  * cone kernels (nonnegative, free, second-order): NT scaling blocks, correctors, max steps;
+//  * assembly of δH + BᵀB into dense frontal matrices;
+//  * batched blocked right-looking partial Cholesky of frontal matrices (FP64 MFMA trailing
+//    update), extend-add of child update matrices;
+//  * per-front forward / backward triangular solves.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace mb {
+
+#define TILE 256
+
+__device__ __forceinline__ double coefval(const Coef& c, const double* sc) {
+  if (c.slot < 0) return c.c;
+  double v = sc[c.slot];
+  return c.inv ? c.c / v : c.c * v;
+}
+
+__device__ __forceinline__ double* nvec(const InstDev& I, int kind) { return I.nv + (size_t)kind * I.n; }
+__device__ __forceinline__ double* mvec(const InstDev& I, int kind) { return I.mv + (size_t)kind * I.m; }
+
+__device__ __forceinline__ double* vecof(const InstDev& I, int space, int kind) {
+  return space == 0 ? nvec(I, kind) : mvec(I, kind);
+}
+
+__device__ __forceinline__ int spacelen(const InstDev& I, int space) { return space == 0 ? I.n : I.m; }
+
+// ------------------------------------------------------------------------------------------
+// vector algebra
+// ------------------------------------------------------------------------------------------
+
+// z = a x + b y   (x, y, z of one space; kinds may alias)
+__global__ void k_axpby(const InstDev* insts, const Tile* tiles, int space, int kx, int ky, int kz,
+                        Coef a, Coef b) {
+  Tile t = tiles[blockIdx.x];
+  const InstDev& I = insts[t.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  int i = t.start + threadIdx.x;
+  int n = spacelen(I, space);
+  if (i >= n) return;
+  double av = coefval(a, I.sc), bv = coefval(b, I.sc);
+  const double* x = vecof(I, space, kx);
+  const double* y = vecof(I, space, ky);
+  double* z = vecof(I, space, kz);
+  double yv = (bv == 0.0) ? 0.0 : bv * y[i];
+  z[i] = av * x[i] + yv;
+}
+
+// z = a x + b y + c w
+__global__ void k_axpbypcz(const InstDev* insts, const Tile* tiles, int space, int kx, int ky, int kw,
+                           int kz, Coef a, Coef b, Coef c) {
+  Tile t = tiles[blockIdx.x];
+  const InstDev& I = insts[t.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  int i = t.start + threadIdx.x;
+  int n = spacelen(I, space);
+  if (i >= n) return;
+  double av = coefval(a, I.sc), bv = coefval(b, I.sc), cv = coefval(c, I.sc);
+  const double* x = vecof(I, space, kx);
+  const double* y = vecof(I, space, ky);
+  const double* w = vecof(I, space, kw);
+  double* z = vecof(I, space, kz);
+  double acc = av * x[i];
+  if (bv != 0.0) acc += bv * y[i];
+  if (cv != 0.0) acc += cv * w[i];
+  z[i] = acc;
+}
+
+// z = x ⊙ y or x ⊘ y (mode 0 multiply, 1 divide)
+__global__ void k_vmul(const InstDev* insts, const Tile* tiles, int space, int kx, int ky, int kz, int mode) {
+  Tile t = tiles[blockIdx.x];
+  const InstDev& I = insts[t.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  int i = t.start + threadIdx.x;
+  if (i >= spacelen(I, space)) return;
+  const double* x = vecof(I, space, kx);
+  const double* y = vecof(I, space, ky);
+  double* z = vecof(I, space, kz);
+  z[i] = mode == 0 ? x[i] * y[i] : x[i] / y[i];
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+  return s;
+}
+
+// partial[tile] = Σ x⊙y (mode 0), Σ (x/s)² with s = kind ky (mode 1), Σ x (mode 2),
+// Σ (x + τ dx)(y + τ dy) with τ = S_TAU (mode 3: x=P, y=D, extra kinds in kw0/kw1)
+__global__ void k_dot_partial(const InstDev* insts, const Tile* tiles, int space, int kx, int ky,
+                              int mode, int kw0, int kw1, double* partial) {
+  __shared__ double red[TILE / 64];
+  Tile t = tiles[blockIdx.x];
+  const InstDev& I = insts[t.inst];
+  int i = t.start + threadIdx.x;
+  double v = 0.0;
+  if (I.sc[S_ACTIVE] != 0.0 && i < spacelen(I, space)) {
+    const double* x = vecof(I, space, kx);
+    if (mode == 0) {
+      v = x[i] * vecof(I, space, ky)[i];
+    } else if (mode == 1) {
+      double q = x[i] / vecof(I, space, ky)[i];
+      v = q * q;
+    } else if (mode == 2) {
+      v = x[i];
+    } else {
+      double tau = I.sc[S_TAU];
+      double a = x[i] + tau * vecof(I, space, kw0)[i];
+      double b = vecof(I, space, ky)[i] + tau * vecof(I, space, kw1)[i];
+      v = a * b;
+    }
+  }
+  double s = block_sum(v, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// sc[slot] = op(Σ partial over the instance's tiles), one block per instance, fixed order
+__global__ void k_reduce_slot(const InstDev* insts, const int* tile_ofs, const double* partial, int slot,
+                              int op) {
+  __shared__ double red[TILE / 64];
+  int inst = blockIdx.x;
+  const InstDev& I = insts[inst];
+  int a = tile_ofs[inst], b = tile_ofs[inst + 1];
+  double v = 0.0;
+  for (int k = a + (int)threadIdx.x; k < b; k += blockDim.x) v += partial[k];
+  double s = block_sum(v, red);
+  if (threadIdx.x == 0 && I.sc[S_ACTIVE] != 0.0) I.sc[slot] = op == 1 ? sqrt(s) : s;
+}
+
+// ------------------------------------------------------------------------------------------
+// sparse matrix-vector products: y = a A x + b y with A ∈ {B (m rows), Bᵀ (n rows), Q (n rows)}
+// ------------------------------------------------------------------------------------------
+
+__global__ void k_spmv(const InstDev* insts, const Tile* tiles, int which, int kx, int ky, Coef a, Coef b) {
+  Tile t = tiles[blockIdx.x];
+  const InstDev& I = insts[t.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const CSRDev& A = which == 0 ? I.B : (which == 1 ? I.Bt : I.Q);
+  int xspace = which == 0 ? 0 : (which == 1 ? 1 : 0);
+  int yspace = which == 0 ? 1 : 0;
+  int i = t.start + threadIdx.x;
+  if (i >= A.nrow) return;
+  const double* x = vecof(I, xspace, kx);
+  double* y = vecof(I, yspace, ky);
+  double acc = 0.0;
+  for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) acc += A.val[k] * x[A.col[k]];
+  double av = coefval(a, I.sc), bv = coefval(b, I.sc);
+  y[i] = av * acc + (bv == 0.0 ? 0.0 : bv * y[i]);
+}
+
+// y[cols] += a D_v x[cols] over all cones (the cone-block part of H x)
+__global__ void k_cone_matvec(const InstDev* insts, const ConeDesc* cones, int ncone, int kx, int ky, Coef a) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const double* x = nvec(I, kx) + C.col0;
+  double* y = nvec(I, ky) + C.col0;
+  double av = coefval(a, I.sc);
+  if (C.kind == CONE_POS) {
+    for (int i = 0; i < C.n; ++i) y[i] += av * C.D[i] * x[i];
+  } else if (C.kind == CONE_SOC) {
+    for (int i = 0; i < C.n; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < C.n; ++j) acc += C.D[i + (size_t)j * C.n] * x[j];
+      y[i] += av * acc;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// error-free transformations (src/IPM/src/utils.jl:27-53)
+// ------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ void twosum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void twoprod(double a, double b, double& p, double& e) {
+  p = a * b;
+  e = fma(a, b, -p);
+}
+
+__device__ double cdot(const double* p, const double* d, int n) {
+  double s = 0.0, c = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double pr, e, e2;
+    twoprod(p[i], d[i], pr, e);
+    twosum(s, pr, s, e2);
+    c += e + e2;
+  }
+  return s + c;
+}
+
+// compensated J-inner product x₁y₁ − Σ xᵢyᵢ as (hi, lo) (soc.jl:48-61)
+__device__ void socdot(const double* x, const double* y, int n, double& hi, double& lo) {
+  double s, c;
+  twoprod(x[0], y[0], s, c);
+  for (int i = 1; i < n; ++i) {
+    double p, e, e2;
+    twoprod(x[i], y[i], p, e);
+    twosum(s, -p, s, e2);
+    c += -e + e2;
+  }
+  twosum(s, c, hi, lo);
+}
+
+// √H x (inverse=0) or √H y = x (inverse=1), in place (soc.jl:82-110)
+__device__ void socroot(double* x, const double* w, double beta, int n, int inverse) {
+  double sigma = inverse ? 1.0 : -1.0;
+  double alpha = inverse ? beta : 1.0 / beta;
+  double w1p1 = w[0] + 1.0;
+  double wpdx = w1p1 * x[0];
+  for (int i = 1; i < n; ++i) wpdx = fma(sigma * w[i], x[i], wpdx);
+  wpdx /= w1p1;
+  x[0] = alpha * fma(wpdx, w1p1, -x[0]);
+  for (int i = 1; i < n; ++i) x[i] = alpha * fma(sigma * wpdx, w[i], x[i]);
+}
+
+// ------------------------------------------------------------------------------------------
+// cone kernels: one thread per cone (SOC caches: β, pdet, ddet, pdetlo, ddetlo, w[n])
+// ------------------------------------------------------------------------------------------
+
+// D ← cone scaling block; per-cone (flag, spsd) into out[2c], out[2c+1]
+__global__ void k_cone_scale(const InstDev* insts, const ConeDesc* cones, int ncone, double* out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const double* p = nvec(I, NV_P) + C.col0;
+  const double* d = nvec(I, NV_D) + C.col0;
+  int n = C.n;
+  double flag = 1.0, spsd = 0.0;
+  if (C.kind == CONE_POS) {
+    // pos.jl:75-84
+    for (int i = 0; i < n; ++i) {
+      C.D[i] = d[i] / p[i];
+      spsd += 1.0 / (p[i] * d[i]);
+    }
+  } else if (C.kind == CONE_SOC) {
+    // soc.jl:192-263
+    double* cache = C.cache;
+    double* w = cache + 5;
+    double pdet, pdetlo, ddet, ddetlo;
+    socdot(p, p, n, pdet, pdetlo);
+    socdot(d, d, n, ddet, ddetlo);
+    cache[1] = pdet;
+    cache[2] = ddet;
+    cache[3] = pdetlo;
+    cache[4] = ddetlo;
+    bool ok = pdet > 0 && p[0] > 0 && ddet > 0 && d[0] > 0;
+    double beta = 0.0;
+    if (ok) {
+      double pdot = cdot(p, d, n);
+      double spdet = sqrt(pdet), sddet = sqrt(ddet);
+      double kappa = sqrt(2.0 * fma(spdet, sddet, pdot));
+      beta = sqrt(spdet / sddet);
+      w[0] = fma(d[0], beta, p[0] / beta) / kappa;
+      for (int i = 1; i < n; ++i) w[i] = fma(-d[i], beta, p[i] / beta) / kappa;
+      double eta = 1.0 / (beta * beta), eta2 = 2.0 * eta;
+      double* D = C.D;
+      double w1 = w[0];
+      D[0] = fma(eta2, w1 * w1, -eta);
+      double nw1 = -eta2 * w1;
+      for (int i = 1; i < n; ++i) D[i] = D[(size_t)i * n] = nw1 * w[i];
+      for (int j = 1; j < n; ++j) {
+        double wj = w[j];
+        D[j + (size_t)j * n] = fma(eta2, wj * wj, eta);
+        double ew = eta2 * wj;
+        for (int i = j + 1; i < n; ++i) D[i + (size_t)j * n] = D[j + (size_t)i * n] = ew * w[i];
+      }
+      spsd = 4.0 * pdot / (pdet * ddet);
+    } else {
+      flag = 0.0;
+    }
+    cache[0] = beta;
+  }
+  out[2 * c] = flag;
+  out[2 * c + 1] = spsd;
+}
+
+// ‖Q_vv + D_v‖² per cone into out[c] (the diagonal-block part of ‖H‖)
+__global__ void k_cone_hnorm2(const InstDev* insts, const ConeDesc* cones, int ncone, double* out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  int n = C.n;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double h = C.Qd[i + (size_t)j * n];
+      if (C.kind == CONE_POS && i == j) h += C.D[i];
+      else if (C.kind == CONE_SOC) h += C.D[i + (size_t)j * n];
+      s += h * h;
+    }
+  out[c] = s;
+}
+
+// per-instance sum of out[c] into sc[slot]
+__global__ void k_cone_sum(const InstDev* insts, const int* cone_ofs, const double* out, int slot) {
+  int inst = blockIdx.x;
+  const InstDev& I = insts[inst];
+  if (threadIdx.x != 0 || I.sc[S_ACTIVE] == 0.0) return;
+  double s = 0.0;
+  for (int c = cone_ofs[inst]; c < cone_ofs[inst + 1]; ++c) s += out[c];
+  I.sc[slot] = s;
+}
+
+// per-instance reduction of cone outputs: S_FLAGOK = all(flag), S_SPSD = Σ spsd (mode 0);
+// S_TAU = min(step) (mode 1, out[c] per cone)
+__global__ void k_cone_reduce(const InstDev* insts, const int* cone_ofs, const double* out, int mode) {
+  int inst = blockIdx.x;
+  const InstDev& I = insts[inst];
+  if (threadIdx.x != 0 || I.sc[S_ACTIVE] == 0.0) return;
+  int a = cone_ofs[inst], b = cone_ofs[inst + 1];
+  if (mode == 0) {
+    double ok = 1.0, s = 0.0;
+    for (int c = a; c < b; ++c) {
+      ok = (ok != 0.0 && out[2 * c] != 0.0) ? 1.0 : 0.0;
+      s += out[2 * c + 1];
+    }
+    I.sc[S_FLAGOK] = ok;
+    I.sc[S_SPSD] = s;
+  } else {
+    double tau = 1.0;
+    for (int c = a; c < b; ++c) tau = fmin(tau, out[c]);
+    I.sc[S_TAU] = tau;
+  }
+}
+
+// predictor right-hand side r = −d + σμ(−f′(p)) per cone (corr0!: pos.jl:131-137,
+// noc.jl:303-306, soc.jl:362-379); writes kind kr
+__global__ void k_cone_corr0(const InstDev* insts, const ConeDesc* cones, int ncone, int kr) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const double* p = nvec(I, NV_P) + C.col0;
+  const double* d = nvec(I, NV_D) + C.col0;
+  double* r = nvec(I, kr) + C.col0;
+  double smu = I.sc[S_SMU];
+  int n = C.n;
+  if (C.kind == CONE_POS) {
+    for (int i = 0; i < n; ++i) r[i] = smu / p[i] - d[i];
+  } else if (C.kind == CONE_FREE) {
+    for (int i = 0; i < n; ++i) r[i] = 0.0;
+  } else if (C.kind == CONE_SOC) {
+    const double* cache = C.cache;
+    double beta = cache[0], pdet = cache[1];
+    const double* w = cache + 5;
+    double ib = 1.0 / beta;
+    double eta = ib * ib;
+    double wp, wplo;
+    socdot(w, p, n, wp, wplo);
+    double t = 2.0 * eta * wp;
+    double c1 = 2.0 * smu / pdet + eta;
+    r[0] = fma(c1, p[0], -t * w[0]);
+    for (int i = 1; i < n; ++i) r[i] = fma(t, w[i], -c1 * p[i]);
+  }
+}
+
+// Mehrotra corrector r = −d + (σμ e − Δp∘Δd)/p (corr!: pos.jl:107-120, noc.jl:285-297,
+// soc.jl:282-351). Needs 3n doubles of scratch per SOC cone: scratch + 3*col0.
+__global__ void k_cone_corr(const InstDev* insts, const ConeDesc* cones, int ncone, int kr, int kdp, int kdd) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const double* p = nvec(I, NV_P) + C.col0;
+  const double* d = nvec(I, NV_D) + C.col0;
+  const double* dp0 = nvec(I, kdp) + C.col0;
+  const double* dd0 = nvec(I, kdd) + C.col0;
+  double* r = nvec(I, kr) + C.col0;
+  double smu = I.sc[S_SMU];
+  int n = C.n;
+  if (C.kind == CONE_POS) {
+    for (int i = 0; i < n; ++i) r[i] = (smu - dp0[i] * dd0[i]) / p[i] - d[i];
+  } else if (C.kind == CONE_FREE) {
+    for (int i = 0; i < n; ++i) r[i] = 0.0;
+  } else if (C.kind == CONE_SOC) {
+    const double* cache = C.cache;
+    double beta = cache[0], pdet = cache[1];
+    const double* w = cache + 5;
+    double* dp = I.scratch + 3 * (size_t)C.col0;
+    double* dd = dp + n;
+    double* v = dd + n;
+    for (int i = 0; i < n; ++i) {
+      dp[i] = dp0[i];
+      dd[i] = dd0[i];
+    }
+    socroot(dp, w, beta, n, 0);
+    socroot(dd, w, beta, n, 1);
+    double dp1 = dp[0], dd1 = dd[0];
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += dp[i] * dd[i];
+    r[0] = acc;
+    for (int i = 1; i < n; ++i) r[i] = fma(dd1, dp[i], dp1 * dd[i]);
+    for (int i = 0; i < n; ++i) v[i] = p[i];
+    socroot(v, w, beta, n, 0);
+    double vdet, vdetlo, vr, vrlo;
+    socdot(v, v, n, vdet, vdetlo);
+    double v1 = v[0];
+    socdot(v, r, n, vr, vrlo);
+    double r1 = vr / vdet;
+    r[0] = r1;
+    double iv1 = 1.0 / v1;
+    for (int i = 1; i < n; ++i) r[i] = fma(-r1, v[i], r[i]) * iv1;
+    for (int i = 0; i < n; ++i) r[i] += v[i];
+    socroot(r, w, beta, n, 0);
+    double gamma = 2.0 * smu / pdet;
+    r[0] = fma(gamma, p[0], -r[0]);
+    for (int i = 1; i < n; ++i) r[i] = fma(-gamma, p[i], -r[i]);
+  }
+}
+
+__device__ double posmaxstep(const double* x, const double* dx, int n) {
+  double tau = 1.0;
+  for (int i = 0; i < n; ++i)
+    if (dx[i] < 0) tau = fmin(tau, -x[i] / dx[i]);
+  return tau;
+}
+
+// soc.jl:391-470
+__device__ double socmaxstep(const double* x, const double* dx, int n, double c, double clo) {
+  const double eps = 2.220446049250313e-16;
+  double a, alo, b, blo;
+  socdot(dx, dx, n, a, alo);
+  socdot(x, dx, n, b, blo);
+  b *= 2.0;
+  blo *= 2.0;
+  double tau = 1.0;
+  if (fabs(a) > eps * fmax(fabs(b), fabs(c))) {
+    double bb, bblo, ac, aclo, s0, s0lo;
+    twoprod(b, b, bb, bblo);
+    bblo = fma(2.0 * b, blo, bblo);
+    twoprod(4.0 * a, c, ac, aclo);
+    aclo = fma(4.0 * a, clo, fma(4.0 * alo, c, aclo));
+    twosum(bb, -ac, s0, s0lo);
+    double disc = s0 + ((bblo - aclo) + s0lo);
+    if (disc > -(eps * fmax(bb, fabs(ac)))) {
+      double s = sqrt(fmax(disc, 0.0));
+      double q = -(b + copysign(s, b)) / 2.0;
+      double tau1 = b >= 0 ? q / a : c / q;
+      if (tau1 > 0) tau = fmin(tau, tau1);
+    }
+  } else if (b < 0.0) {
+    tau = fmin(tau, -c / b);
+  }
+  if (dx[0] < 0) tau = fmin(tau, -x[0] / dx[0]);
+  return tau;
+}
+
+// per-cone max step (cone.jl:455-476, × step_frac except free cones) into out[c]
+__global__ void k_cone_maxstep(const InstDev* insts, const ConeDesc* cones, int ncone, int kdp, int kdd,
+                               double step_frac, double* out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const double* p = nvec(I, NV_P) + C.col0;
+  const double* d = nvec(I, NV_D) + C.col0;
+  const double* dp = nvec(I, kdp) + C.col0;
+  const double* dd = nvec(I, kdd) + C.col0;
+  double tp = 1.0, td = 1.0;
+  if (C.kind == CONE_POS) {
+    tp = posmaxstep(p, dp, C.n);
+    td = posmaxstep(d, dd, C.n);
+  } else if (C.kind == CONE_SOC) {
+    tp = socmaxstep(p, dp, C.n, C.cache[1], C.cache[3]);
+    td = socmaxstep(d, dd, C.n, C.cache[2], C.cache[4]);
+  }
+  if (C.kind != CONE_FREE) {
+    tp *= step_frac;
+    td *= step_frac;
+  }
+  out[c] = fmin(tp, td);
+}
+
+// zero the Δd entries of free cones (solver/ipm.jl:551-559)
+__global__ void k_zerofree(const InstDev* insts, const ConeDesc* cones, int ncone, int kx) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0 || C.kind != CONE_FREE) return;
+  double* x = nvec(I, kx) + C.col0;
+  for (int i = 0; i < C.n; ++i) x[i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------
+// assembly: F ← 0; F[pos] = L + δ Q on the union pattern; F[diag blocks] += δ D
+// ------------------------------------------------------------------------------------------
+
+__global__ void k_fill(double* x, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_assemble_union(const InstDev* insts, int inst, const int64_t* pos, const double* lval,
+                                 const double* qval, int64_t nent) {
+  const InstDev& I = insts[inst];
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nent) return;
+  double delta = I.sc[S_DELTA];
+  I.F[pos[k]] = lval[k] + delta * qval[k];
+}
+
+// cone blocks: one thread per (cone, column); lower triangle of the block lands in the
+// front holding the cone's columns
+__global__ void k_assemble_cones(const InstDev* insts, const ConeDesc* cones, const int* cone_front,
+                                 const FrontDesc* fronts, int ncone) {
+  int c = blockIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0 || C.kind == CONE_FREE) return;
+  const FrontDesc& Fd = fronts[cone_front[c]];
+  double delta = I.sc[S_DELTA];
+  int off = C.col0 - Fd.res0;
+  for (int j = threadIdx.x; j < C.n; j += blockDim.x) {
+    double* col = Fd.F + (size_t)(off + j) * Fd.ld + off;
+    if (C.kind == CONE_POS) {
+      col[j] += delta * C.D[j];
+    } else {
+      for (int i = j; i < C.n; ++i) col[i] += delta * C.D[i + (size_t)j * C.n];
+    }
+  }
+}
+
+// F[diag] += ρ over res columns of every front of the instance; also ‖tril F‖² partials
+__global__ void k_add_diag(const FrontDesc* fronts, int nfront, const InstDev* insts) {
+  int f = blockIdx.x;
+  if (f >= nfront) return;
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  double rho = I.sc[S_RHO];
+  for (int j = threadIdx.x; j < Fd.r; j += blockDim.x) Fd.F[(size_t)j * Fd.ld + j] += rho;
+}
+
+// Σ F² over the res columns' lower part of every front (one block per front) → partial[f]
+__global__ void k_front_norm2(const FrontDesc* fronts, int nfront, double* partial) {
+  __shared__ double red[TILE / 64];
+  int f = blockIdx.x;
+  const FrontDesc& Fd = fronts[f];
+  double v = 0.0;
+  size_t tot = (size_t)Fd.ld * Fd.r;
+  for (size_t k = threadIdx.x; k < tot; k += blockDim.x) {
+    size_t j = k / Fd.ld, i = k % Fd.ld;
+    if (i >= j) {
+      double x = Fd.F[k];
+      v += x * x;
+    }
+  }
+  double s = block_sum(v, red);
+  if (threadIdx.x == 0) partial[f] = s;
+}
+
+// ------------------------------------------------------------------------------------------
+// extend-add: parent F += child's trailing update block, at relidx positions (atomics:
+// siblings of one parent run concurrently)
+// ------------------------------------------------------------------------------------------
+
+__global__ void k_extend_add(const FrontDesc* fronts, const int* items, const int* item_tile0, int nitem) {
+  // binary search the item owning this block
+  int b = blockIdx.x;
+  int lo = 0, hi = nitem - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (item_tile0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  int child = items[lo];
+  int tile = b - item_tile0[lo];
+  const FrontDesc& C = fronts[child];
+  const FrontDesc& P = fronts[C.parent];
+  int s = C.s;
+  int T = (s + NB - 1) / NB;
+  // lower tile pair (ti ≥ tj) from linear index
+  int ti = (int)((sqrt(8.0 * tile + 1.0) - 1.0) / 2.0);
+  while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+  while (ti * (ti + 1) / 2 > tile) --ti;
+  int tj = tile - ti * (ti + 1) / 2;
+  if (ti >= T) return;
+  const double* U = C.F + (size_t)C.r * C.ld + C.r;
+  int lr = threadIdx.x & 63;
+  int i = ti * NB + lr;
+  if (i >= s) return;
+  int pi = C.relidx[i];
+  for (int jj = threadIdx.x >> 6; jj < NB; jj += blockDim.x >> 6) {
+    int j = tj * NB + jj;
+    if (j > i || j >= s) continue;
+    int pj = C.relidx[j];
+    double v = U[i + (size_t)j * C.ld];
+    atomicAdd(&P.F[pi + (size_t)pj * P.ld], v);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// blocked partial Cholesky of frontal matrices
+// ------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ int find_item(const StepDesc* steps, int nstep, int b) {
+  int lo = 0, hi = nstep - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (steps[mid].tile0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Factor the w×w diagonal block held in LDS (column-major, stride NB+1). Returns false on a
+// nonpositive or non-finite pivot (the block is then garbage).
+__device__ bool lds_potrf(double* A, int w) {
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const int ld = NB + 1;
+  for (int j = 0; j < w; ++j) {
+    double piv = A[j + j * ld];
+    if (!(piv > 0.0) || !isfinite(piv)) {
+      if (threadIdx.x == 0) bad = 1;
+      piv = 1.0;
+    }
+    double l = sqrt(piv);
+    __syncthreads();
+    int i = j + 1 + (int)threadIdx.x;
+    if (threadIdx.x == 0) A[j + j * ld] = l;
+    if (i < w) A[i + j * ld] /= l;
+    __syncthreads();
+    // rank-1 update of the trailing lower triangle
+    int rem = w - j - 1;
+    for (int t = threadIdx.x; t < rem * rem; t += blockDim.x) {
+      int ii = j + 1 + t % rem, kk = j + 1 + t / rem;
+      if (ii >= kk) A[ii + kk * ld] -= A[ii + j * ld] * A[kk + j * ld];
+    }
+    __syncthreads();
+  }
+  return bad == 0;
+}
+
+// Panel step: block 0 of each item factors the diagonal block and writes it; blocks 1..ntile
+// factor it redundantly and solve their 64-row tile below: X = A L⁻ᵀ.
+__global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const StepDesc* steps, int nstep,
+                                               const InstDev* insts) {
+  __shared__ double Ld[NB * (NB + 1)];
+  __shared__ double X[NB * (NB + 1)];
+  const int ld = NB + 1;
+  int it = find_item(steps, nstep, blockIdx.x);
+  StepDesc S = steps[it];
+  int tile = blockIdx.x - S.tile0;  // 0 = diag, 1.. = row tiles below kb+w
+  const FrontDesc& Fd = fronts[S.front];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  double* F = Fd.F;
+  int LD = Fd.ld, kb = S.kb, w = S.w;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    int i = t % NB, j = t / NB;
+    Ld[i + j * ld] = (i < w && j < w && i >= j) ? F[(size_t)(kb + i) + (size_t)(kb + j) * LD] : 0.0;
+  }
+  __syncthreads();
+  bool ok = lds_potrf(Ld, w);
+  if (tile == 0) {
+    if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+      int i = t % NB, j = t / NB;
+      if (i < w && j < w && i >= j) F[(size_t)(kb + i) + (size_t)(kb + j) * LD] = Ld[i + j * ld];
+    }
+    return;
+  }
+  int r0 = kb + w + (tile - 1) * NB;
+  int nr = min(NB, LD - r0);
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    int i = t % NB, j = t / NB;
+    X[i + j * ld] = (i < nr && j < w) ? F[(size_t)(r0 + i) + (size_t)(kb + j) * LD] : 0.0;
+  }
+  __syncthreads();
+  // column sweep: X[:, j] /= L[j,j]; X[:, k] -= X[:, j] L[k, j] for k > j
+  for (int j = 0; j < w; ++j) {
+    if (threadIdx.x < NB) X[threadIdx.x + j * ld] /= Ld[j + j * ld];
+    __syncthreads();
+    int rem = w - j - 1;
+    for (int t = threadIdx.x; t < rem * NB; t += blockDim.x) {
+      int i = t % NB, k = j + 1 + t / NB;
+      X[i + k * ld] -= X[i + j * ld] * Ld[k + j * ld];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    int i = t % NB, j = t / NB;
+    if (i < nr && j < w) F[(size_t)(r0 + i) + (size_t)(kb + j) * LD] = X[i + j * ld];
+  }
+}
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Trailing update C[ti, tj] -= X[ti] X[tj]ᵀ over lower tile pairs of [kb+w, ld)², FP64 MFMA
+// (v_mfma_f64_16x16x4f64). Four waves, each a 32×32 quadrant = 2×2 MFMA tiles.
+__global__ void __launch_bounds__(256) k_update(const FrontDesc* fronts, const StepDesc* steps, int nstep,
+                                                const InstDev* insts) {
+  __shared__ double As[NB * (NB + 1)];
+  __shared__ double Bs[NB * (NB + 1)];
+  const int ld = NB + 1;
+  int it = find_item(steps, nstep, blockIdx.x);
+  StepDesc S = steps[it];
+  int tile = blockIdx.x - S.tile0;
+  const FrontDesc& Fd = fronts[S.front];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  int ti = (int)((sqrt(8.0 * tile + 1.0) - 1.0) / 2.0);
+  while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+  while (ti * (ti + 1) / 2 > tile) --ti;
+  int tj = tile - ti * (ti + 1) / 2;
+  double* F = Fd.F;
+  int LD = Fd.ld, kb = S.kb, w = S.w;
+  int base = kb + w;
+  int ri = base + ti * NB, rj = base + tj * NB;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    int i = t % NB, k = t / NB;
+    bool kin = k < w;
+    As[i + k * ld] = (kin && ri + i < LD) ? F[(size_t)(ri + i) + (size_t)(kb + k) * LD] : 0.0;
+    Bs[i + k * ld] = (kin && rj + i < LD) ? F[(size_t)(rj + i) + (size_t)(kb + k) * LD] : 0.0;
+  }
+  __syncthreads();
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  int li = lane & 15, lk = lane >> 4;
+  int kmax = (w + 3) & ~3;
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    double av[2], bv[2];
+    for (int a = 0; a < 2; ++a) av[a] = As[(qi + a * 16 + li) + (k0 + lk) * ld];
+    for (int b = 0; b < 2; ++b) bv[b] = Bs[(qj + b * 16 + li) + (k0 + lk) * ld];
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+  // D layout: lane holds rows 4*(lane/16)+t, column lane%16 of each 16×16 tile
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int t = 0; t < 4; ++t) {
+        int i = ri + qi + a * 16 + 4 * (lane >> 4) + t;
+        int j = rj + qj + b * 16 + (lane & 15);
+        if (i < LD && j < LD && i >= j) F[(size_t)i + (size_t)j * LD] -= acc[a][b][t];
+      }
+}
+
+// ------------------------------------------------------------------------------------------
+// triangular solves, one block per front per level
+// ------------------------------------------------------------------------------------------
+
+// forward: v[res] ← L11⁻¹ (v[res] − child contributions); uvec ← child pass-through + L21 x
+__global__ void __launch_bounds__(256) k_fsolve(const FrontDesc* fronts, const int* items, const int* childlist,
+                                                const InstDev* insts, int kv) {
+  __shared__ double xs[NB];
+  int f = items[blockIdx.x];
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  double* v = nvec(I, kv) + Fd.res0;
+  double* u = Fd.uvec;
+  int r = Fd.r, s = Fd.s, LD = Fd.ld;
+  const double* F = Fd.F;
+  for (int k = threadIdx.x; k < s; k += blockDim.x) u[k] = 0.0;
+  __syncthreads();
+  for (int c = 0; c < Fd.nchild; ++c) {
+    const FrontDesc& C = fronts[childlist[Fd.child0 + c]];
+    for (int k = threadIdx.x; k < C.s; k += blockDim.x) {
+      int p = C.relidx[k];
+      if (p < r) v[p] -= C.uvec[k];
+      else u[p - r] += C.uvec[k];
+    }
+    __syncthreads();
+  }
+  for (int jb = 0; jb < r; jb += NB) {
+    int w = min(NB, r - jb);
+    // diagonal block solve by wave 0: lane i holds row i of the block in registers
+    if (threadIdx.x < 64) {
+      int lane = threadIdx.x;
+      double Lr[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        Lr[j] = (lane < w && j <= lane) ? F[(size_t)(jb + lane) + (size_t)(jb + j) * LD] : 0.0;
+      double xi = lane < w ? v[jb + lane] : 0.0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < w) {
+          double xj = __shfl(xi / Lr[j], j, 64);
+          if (lane == j) xi = xj;
+          if (lane > j) xi -= Lr[j] * xj;
+        }
+      }
+      if (lane < w) {
+        v[jb + lane] = xi;
+        xs[lane] = xi;
+      }
+    }
+    __syncthreads();
+    // update the rest of res and the sep part
+    for (int i = jb + w + (int)threadIdx.x; i < r + s; i += blockDim.x) {
+      double acc = 0.0;
+      for (int j = 0; j < w; ++j) acc += F[(size_t)i + (size_t)(jb + j) * LD] * xs[j];
+      if (i < r) v[i] -= acc;
+      else u[i - r] += acc;
+    }
+    __syncthreads();
+  }
+}
+
+// backward: v[res] ← L11⁻ᵀ (v[res] − L21ᵀ v[sep])
+__global__ void __launch_bounds__(256) k_bsolve(const FrontDesc* fronts, const int* items, const InstDev* insts,
+                                                int kv) {
+  __shared__ double xs[NB];
+  __shared__ double red[256];
+  int f = items[blockIdx.x];
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  double* vall = nvec(I, kv);
+  double* v = vall + Fd.res0;
+  int r = Fd.r, s = Fd.s, LD = Fd.ld;
+  const double* F = Fd.F;
+  // v[res] -= L21ᵀ v[sep]: one wave per column
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (s > 0) {
+    for (int j = wave; j < r; j += blockDim.x >> 6) {
+      double acc = 0.0;
+      for (int k = lane; k < s; k += 64) acc += F[(size_t)(r + k) + (size_t)j * LD] * vall[Fd.sepcols[k]];
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) v[j] -= acc;
+    }
+  }
+  __syncthreads();
+  int nblk = (r + NB - 1) / NB;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    int jb = bi * NB;
+    int w = min(NB, r - jb);
+    if (threadIdx.x < 64) {
+      // solve L_bbᵀ x = v_b, bottom-up: lane i holds column i of the block in registers
+      double Lc[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        Lc[j] = (lane < w && j < w && j >= lane) ? F[(size_t)(jb + j) + (size_t)(jb + lane) * LD] : 0.0;
+      double xi = lane < w ? v[jb + lane] : 0.0;
+#pragma unroll
+      for (int j = NB - 1; j >= 0; --j) {
+        if (j < w) {
+          // lane j divides by its own diagonal Lc[j] = L[j,j]
+          double xj = __shfl(xi / Lc[j], j, 64);
+          if (lane == j) xi = xj;
+          if (lane < j) xi -= Lc[j] * xj;
+        }
+      }
+      if (lane < w) {
+        v[jb + lane] = xi;
+        xs[lane] = xi;
+      }
+    }
+    __syncthreads();
+    // v[0:jb] -= L[jb:jb+w, 0:jb]ᵀ x_b: one wave per column
+    for (int j = wave; j < jb; j += blockDim.x >> 6) {
+      double acc = lane < w ? F[(size_t)(jb + lane) + (size_t)j * LD] * xs[lane] : 0.0;
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) v[j] -= acc;
+    }
+    __syncthreads();
+  }
+  (void)red;
+}
+
+// ------------------------------------------------------------------------------------------
+// scalar kernels (one thread per instance)
+// ------------------------------------------------------------------------------------------
+
+// CG coefficient updates (cg.jl:108-160). op 0: α = ν²/ε² (ε² in S_CG_EP2), sets S_CG_ALPHA;
+// op 1: β = ν²_new/ν²_old with new ν² in S_NORM0, sets S_CG_BETA, S_CG_NU2, S_CG_NP2
+__global__ void k_cg_scalars(const InstDev* insts, int ninst, int op) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst) return;
+  double* sc = insts[k].sc;
+  if (sc[S_ACTIVE] == 0.0) return;
+  if (op == 0) {
+    sc[S_CG_ALPHA] = sc[S_CG_NU2] / sc[S_CG_EP2];
+  } else {
+    double pnu2 = sc[S_CG_NU2];
+    double nu2 = sc[S_NORM0];
+    double beta = nu2 / pnu2;
+    sc[S_CG_BETA] = beta;
+    sc[S_CG_NU2] = nu2;
+    sc[S_CG_NP2] = nu2 + beta * beta * sc[S_CG_NP2];
+  }
+}
+
+// layout probe: D = A·B for one v_mfma_f64_16x16x4f64 with A[i][k] = i + 100 k,
+// B[k][j] = j + 1000 k; every lane writes its 4 outputs to out[lane*4 + t]
+__global__ void k_probe_mfma(double* out) {
+  int lane = threadIdx.x & 63;
+  double a = (double)((lane & 15) + 100 * (lane >> 4));
+  double b = (double)((lane & 15) + 1000 * (lane >> 4));
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int t = 0; t < 4; ++t) out[lane * 4 + t] = acc[t];
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side launch wrappers
+// ------------------------------------------------------------------------------------------
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+void launch_axpby(hipStream_t st, const InstDev* I, const Tile* T, int nt, int space, int kx, int ky, int kz,
+                  Coef a, Coef b) {
+  if (nt) k_axpby<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, kz, a, b);
+}
+void launch_axpbypcz(hipStream_t st, const InstDev* I, const Tile* T, int nt, int space, int kx, int ky, int kw,
+                     int kz, Coef a, Coef b, Coef c) {
+  if (nt) k_axpbypcz<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, kw, kz, a, b, c);
+}
+void launch_vmul(hipStream_t st, const InstDev* I, const Tile* T, int nt, int space, int kx, int ky, int kz,
+                 int mode) {
+  if (nt) k_vmul<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, kz, mode);
+}
+void launch_dot(hipStream_t st, const InstDev* I, const Tile* T, int nt, const int* tile_ofs, int ninst,
+                int space, int kx, int ky, int mode, int kw0, int kw1, double* partial, int slot, int op) {
+  if (nt) k_dot_partial<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, mode, kw0, kw1, partial);
+  k_reduce_slot<<<ninst, TILE, 0, st>>>(I, tile_ofs, partial, slot, op);
+}
+void launch_spmv(hipStream_t st, const InstDev* I, const Tile* T, int nt, int which, int kx, int ky, Coef a,
+                 Coef b) {
+  if (nt) k_spmv<<<nt, TILE, 0, st>>>(I, T, which, kx, ky, a, b);
+}
+void launch_cone_matvec(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx, int ky, Coef a) {
+  if (nc) k_cone_matvec<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx, ky, a);
+}
+void launch_cone_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
+                       int ninst, double* out) {
+  if (nc) k_cone_scale<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
+  k_cone_reduce<<<ninst, 64, 0, st>>>(I, cone_ofs, out, 0);
+}
+void launch_cone_hnorm2(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
+                        int ninst, double* out, int slot) {
+  if (nc) k_cone_hnorm2<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
+  k_cone_sum<<<ninst, 64, 0, st>>>(I, cone_ofs, out, slot);
+}
+void launch_cone_corr0(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kr) {
+  if (nc) k_cone_corr0<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kr);
+}
+void launch_cone_corr(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kr, int kdp, int kdd) {
+  if (nc) k_cone_corr<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kr, kdp, kdd);
+}
+void launch_cone_maxstep(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
+                         int ninst, int kdp, int kdd, double step_frac, double* out) {
+  if (nc) k_cone_maxstep<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kdp, kdd, step_frac, out);
+  k_cone_reduce<<<ninst, 64, 0, st>>>(I, cone_ofs, out, 1);
+}
+void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx) {
+  if (nc) k_zerofree<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx);
+}
+void launch_fill(hipStream_t st, double* x, size_t n, double v) {
+  if (n) k_fill<<<cdiv((long)n, TILE), TILE, 0, st>>>(x, n, v);
+}
+void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* lval,
+                           const double* qval, int64_t nent) {
+  if (nent) k_assemble_union<<<cdiv(nent, TILE), TILE, 0, st>>>(I, inst, pos, lval, qval, nent);
+}
+void launch_assemble_cones(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* cone_front,
+                           const FrontDesc* F, int nc) {
+  if (nc) k_assemble_cones<<<nc, 64, 0, st>>>(I, C, cone_front, F, nc);
+}
+void launch_add_diag(hipStream_t st, const FrontDesc* F, int nf, const InstDev* I) {
+  if (nf) k_add_diag<<<nf, TILE, 0, st>>>(F, nf, I);
+}
+void launch_front_norm2(hipStream_t st, const FrontDesc* F, int nf, double* partial) {
+  if (nf) k_front_norm2<<<nf, TILE, 0, st>>>(F, nf, partial);
+}
+void launch_extend_add(hipStream_t st, const FrontDesc* F, const int* items, const int* item_tile0, int nitem,
+                       int ntile) {
+  if (ntile) k_extend_add<<<ntile, TILE, 0, st>>>(F, items, item_tile0, nitem);
+}
+void launch_panel(hipStream_t st, const FrontDesc* F, const StepDesc* S, int nstep, int nblk, const InstDev* I) {
+  if (nblk) k_panel<<<nblk, 256, 0, st>>>(F, S, nstep, I);
+}
+void launch_update(hipStream_t st, const FrontDesc* F, const StepDesc* S, int nstep, int nblk, const InstDev* I) {
+  if (nblk) k_update<<<nblk, 256, 0, st>>>(F, S, nstep, I);
+}
+void launch_fsolve(hipStream_t st, const FrontDesc* F, const int* items, int nitem, const int* childlist,
+                   const InstDev* I, int kv) {
+  if (nitem) k_fsolve<<<nitem, 256, 0, st>>>(F, items, childlist, I, kv);
+}
+void launch_bsolve(hipStream_t st, const FrontDesc* F, const int* items, int nitem, const InstDev* I, int kv) {
+  if (nitem) k_bsolve<<<nitem, 256, 0, st>>>(F, items, I, kv);
+}
+void launch_cg_scalars(hipStream_t st, const InstDev* I, int ninst, int op) {
+  k_cg_scalars<<<cdiv(ninst, 64), 64, 0, st>>>(I, ninst, op);
+}
+void launch_probe_mfma(hipStream_t st, double* out) { k_probe_mfma<<<1, 64, 0, st>>>(out); }
+
+__global__ void k_set_scalar(double* sc, int slot, double v) { sc[slot] = v; }
+void launch_set_scalar(hipStream_t st, double* sc, int slot, double v) { k_set_scalar<<<1, 1, 0, st>>>(sc, slot, v); }
+
+}  // namespace mb

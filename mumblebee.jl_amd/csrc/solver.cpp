@@ -1,0 +1,1209 @@
+// IPM driver over device-resident state (see solver.h).
+#include "solver.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+
+namespace mb {
+
+static const double EPS = std::numeric_limits<double>::epsilon();
+static const double NaN = std::numeric_limits<double>::quiet_NaN();
+static const double FORCING_FRAC = 0.1;  // solver/solver.jl:3
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") +           \
+                                                   hipGetErrorString(e_) + " at " #x);     \
+  } while (0)
+
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+void DevBuf::alloc(size_t b) {
+  if (p) {
+    HIPCHK(hipFree(p));
+    p = nullptr;
+  }
+  bytes = b;
+  if (b) HIPCHK(hipMalloc(&p, b));
+}
+template <class T>
+void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
+  alloc(std::max<size_t>(v.size() * sizeof(T), sizeof(T)));
+  // synchronous: callers pass temporaries
+  HIPCHK(hipStreamSynchronize(st));
+  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------------------------------------
+// Ruiz equilibration (scaling.jl:52-184)
+// ------------------------------------------------------------------------------------------
+void equilibrate(BSM& B, BSM& Q, std::vector<double>& f, std::vector<double>& g, int itmax, double tol,
+                 std::vector<double>& pscl, std::vector<double>& yscl) {
+  const i64 nrow = B.nrow(), nvtx = B.nvtx;
+  pscl.assign(B.ncol(), 1.0);
+  yscl.assign(nrow, 1.0);
+  std::vector<double> vscl(nvtx, 1.0), vnrm(nvtx), rnrm(nrow), vswp(nvtx, 1.0), rswp(nrow, 1.0);
+  for (int it = 0; it < itmax; ++it) {
+    std::fill(rnrm.begin(), rnrm.end(), 0.0);
+    for (i64 v = 0; v < nvtx; ++v) {
+      double nv = 0.0;
+      for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e)
+        for (i64 b = Q.xblk[e]; b < Q.xblk[e + 1]; ++b) nv = std::max(nv, std::fabs(Q.val[b]));
+      i64 nc = B.ncols(v);
+      for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
+        i64 u = B.tgt[e], nr = B.nrows(u), r0 = B.xrow[u];
+        for (i64 jl = 0; jl < nc; ++jl)
+          for (i64 il = 0; il < nr; ++il) {
+            double a = std::fabs(B.val[B.xblk[e] + jl * nr + il]);
+            nv = std::max(nv, a);
+            rnrm[r0 + il] = std::max(rnrm[r0 + il], a);
+          }
+      }
+      vnrm[v] = nv;
+    }
+    double d = 0.0;
+    for (i64 v = 0; v < nvtx; ++v) {
+      double n = std::sqrt(vnrm[v]);
+      d = std::max(d, std::fabs(1.0 - n));
+      if (n != 0.0) {
+        vswp[v] = 1.0 / n;
+        vscl[v] *= vswp[v];
+      }
+    }
+    for (i64 i = 0; i < nrow; ++i) {
+      double n = std::sqrt(rnrm[i]);
+      d = std::max(d, std::fabs(1.0 - n));
+      if (n != 0.0) {
+        rswp[i] = 1.0 / n;
+        yscl[i] *= rswp[i];
+      }
+    }
+    for (i64 v = 0; v < nvtx; ++v) {
+      double sv = vswp[v];
+      i64 nc = B.ncols(v);
+      for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
+        i64 u = B.tgt[e], nr = B.nrows(u), r0 = B.xrow[u];
+        for (i64 jl = 0; jl < nc; ++jl)
+          for (i64 il = 0; il < nr; ++il) B.val[B.xblk[e] + jl * nr + il] *= rswp[r0 + il] * sv;
+      }
+      for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) {
+        double s = vswp[Q.tgt[e]] * sv;
+        for (i64 b = Q.xblk[e]; b < Q.xblk[e + 1]; ++b) Q.val[b] *= s;
+      }
+    }
+    if (d < tol) break;
+  }
+  for (i64 v = 0; v < nvtx; ++v)
+    for (i64 j = B.xcol[v]; j < B.xcol[v + 1]; ++j) {
+      pscl[j] = vscl[v];
+      f[j] *= vscl[v];
+    }
+  for (i64 i = 0; i < nrow; ++i) g[i] *= yscl[i];
+}
+
+static double norm2(const std::vector<double>& v) {
+  double s = 0.0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+
+static double scalenorm(const std::vector<double>& v, const std::vector<double>& s) {
+  double o = 0.0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    double q = v[i] / s[i];
+    o += q * q;
+  }
+  return std::sqrt(o);
+}
+
+static int cone_degree(const Cone& c, i64 n) {
+  switch (c.kind) {
+    case CONE_POS: return (int)n;
+    case CONE_FREE: return 0;
+    case CONE_SOC: return 2;
+    case CONE_PSD: return (int)((std::sqrt(1.0 + 8.0 * (double)n) - 1.0) / 2.0 + 0.5);
+    default: return 3;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// construction
+// ------------------------------------------------------------------------------------------
+
+Solver::Solver(const Problem& prob, const Settings& st, const double* p0, const double* d0, const double* y0)
+    : set_(st) {
+  for (const Cone& c : prob.K)
+    if (c.kind != CONE_POS && c.kind != CONE_FREE && c.kind != CONE_SOC)
+      throw std::invalid_argument("cone kind not supported by this build's device path");
+  n_ = prob.B.ncol();
+  m_ = prob.B.nrow();
+  nu_ = 0;
+  for (i64 v = 0; v < prob.B.nvtx; ++v) nu_ += cone_degree(prob.K[v], prob.B.ncols(v));
+  mu_t_ = prob.mu;
+  // symbkkt (kkt.jl:23-53)
+  std::vector<i64> w;
+  std::vector<std::vector<i64>> adj;
+  vertex_graph(prob.B, prob.Q, w, adj);
+  sym_ = analyse(w, adj, set_.amalgamate != 0);
+  const std::vector<i64>& vp = sym_.vperm;
+  P_.Q = symselectvtxs(prob.Q, vp);
+  P_.B = selectvtxs(prob.B, vp);
+  std::vector<i64> colp;
+  for (i64 v : vp)
+    for (i64 j = prob.B.xcol[v]; j < prob.B.xcol[v + 1]; ++j) colp.push_back(j);
+  P_.f.resize(n_);
+  P_.P2.resize(n_);
+  for (i64 j = 0; j < n_; ++j) {
+    P_.f[j] = prob.f[colp[j]];
+    P_.P2[j] = prob.P2[colp[j]];
+  }
+  P_.g = prob.g;
+  P_.P1 = prob.P1;
+  for (i64 v : vp) P_.K.push_back(prob.K[v]);
+  P_.mu = prob.mu;
+  if (set_.scale_max_iter > 0) {
+    equilibrate(P_.B, P_.Q, P_.f, P_.g, set_.scale_max_iter, 1e-3, pscl_, yscl_);
+  } else {
+    pscl_.assign(n_, 1.0);
+    yscl_.assign(m_, 1.0);
+  }
+  nB_ = norm2(P_.B.val);
+  nH_ = norm2(P_.Q.val);
+  nf_ = norm2(P_.f);
+  ng_ = norm2(P_.g);
+  sg_ = scalenorm(P_.g, yscl_);
+  sf_ = scalenorm(P_.f, pscl_);
+  HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  HIPCHK(hipHostMalloc((void**)&hsc_, NSCALAR * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipEventCreate(&ev0_));
+  HIPCHK(hipEventCreate(&ev1_));
+  build_device();
+  build_schedule();
+  reinit(p0, d0, y0);
+}
+
+Solver::~Solver() {
+  if (st_) (void)hipStreamSynchronize(st_);
+  if (hsc_) (void)hipHostFree(hsc_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+static void bsm_to_csr(const BSM& A, bool transpose, std::vector<int>& ptr, std::vector<int>& col,
+                       std::vector<double>& val) {
+  // entries (i, j, x) of A; CSR over rows i (or over columns j if transpose)
+  i64 nr = transpose ? A.ncol() : A.nrow();
+  std::vector<i64> cnt(nr + 1, 0);
+  auto visit = [&](auto&& fn) {
+    for (i64 v = 0; v < A.nvtx; ++v) {
+      i64 nc = A.ncols(v);
+      for (i64 e = A.xsrc[v]; e < A.xsrc[v + 1]; ++e) {
+        i64 u = A.tgt[e], nrw = A.nrows(u);
+        for (i64 jl = 0; jl < nc; ++jl)
+          for (i64 il = 0; il < nrw; ++il) {
+            double x = A.val[A.xblk[e] + jl * nrw + il];
+            if (x != 0.0) fn(A.xrow[u] + il, A.xcol[v] + jl, x);
+          }
+      }
+    }
+  };
+  visit([&](i64 i, i64 j, double) { cnt[(transpose ? j : i) + 1]++; });
+  for (i64 r = 0; r < nr; ++r) cnt[r + 1] += cnt[r];
+  ptr.assign(cnt.begin(), cnt.end());
+  col.resize(cnt[nr]);
+  val.resize(cnt[nr]);
+  std::vector<i64> nxt(cnt.begin(), cnt.end() - 1);
+  visit([&](i64 i, i64 j, double x) {
+    i64 r = transpose ? j : i, c = transpose ? i : j;
+    i64 q = nxt[r]++;
+    col[q] = (int)c;
+    val[q] = x;
+  });
+}
+
+void Solver::build_device() {
+  const i64 N = P_.B.nvtx;
+  const i64 NF = sym_.nfront();
+  // fronts
+  fr_.assign(NF, FrontDesc{});
+  std::vector<i64> fofs(NF + 1, 0);
+  std::vector<std::vector<int>> sepcols(NF);
+  std::vector<int> colfront(n_);
+  for (i64 f = 0; f < NF; ++f) {
+    i64 res0 = sym_.xcol[sym_.front_vtx[f]];
+    i64 r = sym_.xcol[sym_.front_vtx[f + 1]] - res0;
+    for (i64 u : sym_.front_sep[f])
+      for (i64 j = sym_.xcol[u]; j < sym_.xcol[u + 1]; ++j) sepcols[f].push_back((int)j);
+    i64 s = (i64)sepcols[f].size();
+    FrontDesc& F = fr_[f];
+    F.r = (int)r;
+    F.s = (int)s;
+    F.ld = (int)(r + s);
+    F.res0 = (int)res0;
+    F.parent = (int)sym_.front_parent[f];
+    F.inst = 0;
+    fofs[f + 1] = fofs[f] + (r + s) * (r + s);
+    for (i64 j = res0; j < res0 + r; ++j) colfront[j] = (int)f;
+  }
+  fsize_ = fofs[NF];
+  // position of (i, j), i ≥ j, in the frontal storage
+  auto fpos = [&](i64 i, i64 j) -> i64 {
+    int f = colfront[j];
+    const FrontDesc& F = fr_[f];
+    i64 jl = j - F.res0, il;
+    if (i < F.res0 + F.r) {
+      il = i - F.res0;
+    } else {
+      auto& sc = sepcols[f];
+      auto it = std::lower_bound(sc.begin(), sc.end(), (int)i);
+      if (it == sc.end() || *it != i) throw std::logic_error("symbolic structure misses an entry of M");
+      il = F.r + (it - sc.begin());
+    }
+    return fofs[f] + il + jl * F.ld;
+  };
+  // children lists, relidx, sepcols
+  std::vector<std::vector<int>> children(NF);
+  for (i64 f = 0; f < NF; ++f)
+    if (fr_[f].parent >= 0) children[fr_[f].parent].push_back((int)f);
+  std::vector<int> childlist, relidx, allsep;
+  std::vector<i64> relofs(NF), sepofs(NF), uofs(NF + 1, 0);
+  for (i64 f = 0; f < NF; ++f) {
+    fr_[f].child0 = (int)childlist.size();
+    fr_[f].nchild = (int)children[f].size();
+    for (int c : children[f]) childlist.push_back(c);
+    sepofs[f] = (i64)allsep.size();
+    allsep.insert(allsep.end(), sepcols[f].begin(), sepcols[f].end());
+    relofs[f] = (i64)relidx.size();
+    int p = fr_[f].parent;
+    for (int col : sepcols[f]) {
+      if (p < 0) throw std::logic_error("root front with a separator");
+      const FrontDesc& Pf = fr_[p];
+      if (col >= Pf.res0 && col < Pf.res0 + Pf.r) {
+        relidx.push_back(col - Pf.res0);
+      } else {
+        auto& sc = sepcols[p];
+        auto it = std::lower_bound(sc.begin(), sc.end(), col);
+        if (it == sc.end() || *it != col) throw std::logic_error("separator not contained in the parent");
+        relidx.push_back(Pf.r + (int)(it - sc.begin()));
+      }
+    }
+    uofs[f + 1] = uofs[f] + fr_[f].s;
+  }
+  // union pattern of L = BᵀB and Q (lower triangle)
+  BSM L = gram(P_.B);
+  std::vector<int64_t> upos;
+  std::vector<double> ul, uq;
+  for (i64 v = 0; v < N; ++v) {
+    i64 nc = P_.B.ncols(v), c0 = P_.B.xcol[v];
+    i64 a = L.xsrc[v], ae = L.xsrc[v + 1], b = P_.Q.xsrc[v], be = P_.Q.xsrc[v + 1];
+    while (a < ae || b < be) {
+      i64 ua = a < ae ? L.tgt[a] : std::numeric_limits<i64>::max();
+      i64 ub = b < be ? P_.Q.tgt[b] : std::numeric_limits<i64>::max();
+      i64 u = std::min(ua, ub);
+      const double* lb = ua == u ? L.val.data() + L.xblk[a] : nullptr;
+      const double* qb = ub == u ? P_.Q.val.data() + P_.Q.xblk[b] : nullptr;
+      i64 nr = P_.B.ncols(u), r0 = P_.B.xcol[u];
+      if (r0 + nr - 1 >= c0) {
+        for (i64 jl = 0; jl < nc; ++jl)
+          for (i64 il = 0; il < nr; ++il) {
+            i64 i = r0 + il, j = c0 + jl;
+            if (i < j) continue;
+            double lv = lb ? lb[jl * nr + il] : 0.0, qv = qb ? qb[jl * nr + il] : 0.0;
+            if (lv == 0.0 && qv == 0.0) continue;
+            upos.push_back(fpos(i, j));
+            ul.push_back(lv);
+            uq.push_back(qv);
+          }
+      }
+      if (ua == u) ++a;
+      if (ub == u) ++b;
+    }
+  }
+  nunion_ = (i64)upos.size();
+  // ‖Q‖² off the diagonal blocks
+  qoff2_ = 0.0;
+  for (i64 v = 0; v < N; ++v)
+    for (i64 e = P_.Q.xsrc[v]; e < P_.Q.xsrc[v + 1]; ++e)
+      if (P_.Q.tgt[e] != v)
+        for (i64 b = P_.Q.xblk[e]; b < P_.Q.xblk[e + 1]; ++b) qoff2_ += P_.Q.val[b] * P_.Q.val[b];
+  // cones
+  ncone_ = (int)N;
+  std::vector<i64> dofs(N + 1, 0), cofs(N + 1, 0), qofs(N + 1, 0);
+  for (i64 v = 0; v < N; ++v) {
+    i64 nc = P_.B.ncols(v);
+    int k = P_.K[v].kind;
+    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : (k == CONE_SOC ? nc * nc : 0));
+    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : 0);
+    qofs[v + 1] = qofs[v] + nc * nc;
+  }
+  std::vector<double> qdiag(qofs[N], 0.0);
+  for (i64 v = 0; v < N; ++v)
+    for (i64 e = P_.Q.xsrc[v]; e < P_.Q.xsrc[v + 1]; ++e)
+      if (P_.Q.tgt[e] == v) std::copy(P_.Q.val.begin() + P_.Q.xblk[e], P_.Q.val.begin() + P_.Q.xblk[e + 1],
+                                      qdiag.begin() + qofs[v]);
+  cone_front_.assign(N, 0);
+  for (i64 f = 0; f < NF; ++f)
+    for (i64 v = sym_.front_vtx[f]; v < sym_.front_vtx[f + 1]; ++v) cone_front_[v] = (int)f;
+  // device allocations
+  d_nv_.alloc(sizeof(double) * NV_COUNT * std::max<i64>(n_, 1));
+  d_mv_.alloc(sizeof(double) * MV_COUNT * std::max<i64>(m_, 1));
+  d_sc_.alloc(sizeof(double) * NSCALAR);
+  d_scr_.alloc(sizeof(double) * 3 * std::max<i64>(n_, 1));
+  d_F_.alloc(sizeof(double) * std::max<i64>(fsize_, 1));
+  d_D_.alloc(sizeof(double) * std::max<i64>(dofs[N], 1));
+  d_cache_.alloc(sizeof(double) * std::max<i64>(cofs[N], 1));
+  d_qdiag_.upload(qdiag, st_);
+  d_uvec_.alloc(sizeof(double) * std::max<i64>(uofs[NF], 1));
+  HIPCHK(hipMemsetAsync(d_nv_.p, 0, d_nv_.bytes, st_));
+  HIPCHK(hipMemsetAsync(d_mv_.p, 0, d_mv_.bytes, st_));
+  HIPCHK(hipMemsetAsync(d_sc_.p, 0, d_sc_.bytes, st_));
+  HIPCHK(hipMemsetAsync(d_cache_.p, 0, d_cache_.bytes, st_));
+  HIPCHK(hipMemsetAsync(d_D_.p, 0, d_D_.bytes, st_));
+  d_sepcols_.upload(allsep, st_);
+  d_relidx_.upload(relidx, st_);
+  d_childlist_.upload(childlist, st_);
+  for (i64 f = 0; f < NF; ++f) {
+    fr_[f].F = d_F_.as<double>() + fofs[f];
+    fr_[f].F32 = nullptr;
+    fr_[f].sepcols = d_sepcols_.as<int>() + sepofs[f];
+    fr_[f].relidx = d_relidx_.as<int>() + relofs[f];
+    fr_[f].uvec = d_uvec_.as<double>() + uofs[f];
+  }
+  d_fr_.upload(fr_, st_);
+  cones_.assign(N, ConeDesc{});
+  for (i64 v = 0; v < N; ++v) {
+    ConeDesc& C = cones_[v];
+    C.kind = P_.K[v].kind;
+    C.inst = 0;
+    C.col0 = (int)P_.B.xcol[v];
+    C.n = (int)P_.B.ncols(v);
+    C.param = P_.K[v].param;
+    C.D = d_D_.as<double>() + dofs[v];
+    C.cache = d_cache_.as<double>() + cofs[v];
+    C.Qd = d_qdiag_.as<double>() + qofs[v];
+  }
+  d_cones_.upload(cones_, st_);
+  d_cone_front_.upload(cone_front_, st_);
+  d_cone_ofs_.upload(std::vector<int>{0, (int)N}, st_);
+  // sparse matrices
+  std::vector<int> p, c;
+  std::vector<double> x;
+  bsm_to_csr(P_.B, false, p, c, x);
+  d_Bp_.upload(p, st_);
+  d_Bc_.upload(c, st_);
+  d_Bv_.upload(x, st_);
+  bsm_to_csr(P_.B, true, p, c, x);
+  d_Btp_.upload(p, st_);
+  d_Btc_.upload(c, st_);
+  d_Btv_.upload(x, st_);
+  bsm_to_csr(P_.Q, false, p, c, x);
+  d_Qp_.upload(p, st_);
+  d_Qc_.upload(c, st_);
+  d_Qv_.upload(x, st_);
+  d_upos_.upload(upos, st_);
+  d_ul_.upload(ul, st_);
+  d_uq_.upload(uq, st_);
+  // tiles
+  std::vector<Tile> tn, tm;
+  for (i64 s = 0; s < n_; s += 256) tn.push_back(Tile{0, (int)s});
+  for (i64 s = 0; s < m_; s += 256) tm.push_back(Tile{0, (int)s});
+  ntile_n_ = (int)tn.size();
+  ntile_m_ = (int)tm.size();
+  d_ntiles_.upload(tn, st_);
+  d_mtiles_.upload(tm, st_);
+  d_ntofs_.upload(std::vector<int>{0, ntile_n_}, st_);
+  d_mtofs_.upload(std::vector<int>{0, ntile_m_}, st_);
+  d_partial_.alloc(sizeof(double) * (std::max<i64>({(i64)ntile_n_, (i64)ntile_m_, NF, 1}) + 16));
+  d_coneout_.alloc(sizeof(double) * 3 * std::max<i64>(N, 1));
+  // instance table
+  idev_.nv = d_nv_.as<double>();
+  idev_.mv = d_mv_.as<double>();
+  idev_.sc = d_sc_.as<double>();
+  idev_.scratch = d_scr_.as<double>();
+  idev_.F = d_F_.as<double>();
+  idev_.B = CSRDev{d_Bp_.as<int>(), d_Bc_.as<int>(), d_Bv_.as<double>(), (int)m_};
+  idev_.Bt = CSRDev{d_Btp_.as<int>(), d_Btc_.as<int>(), d_Btv_.as<double>(), (int)n_};
+  idev_.Q = CSRDev{d_Qp_.as<int>(), d_Qc_.as<int>(), d_Qv_.as<double>(), (int)n_};
+  idev_.n = (int)n_;
+  idev_.m = (int)m_;
+  d_inst_.upload(std::vector<InstDev>{idev_}, st_);
+  // constant vectors: f, g, pscl, yscl
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_F * n_, P_.f.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_PSCL * n_, pscl_.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_G * m_, P_.g.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_YSCL * m_, yscl_.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  set_scalar(S_ACTIVE, 1.0);
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+void Solver::build_schedule() {
+  const int NF = (int)fr_.size();
+  std::vector<int> h(NF, 0);
+  for (int f = 0; f < NF; ++f)
+    if (fr_[f].parent >= 0) h[fr_[f].parent] = std::max(h[fr_[f].parent], h[f] + 1);
+  int H = 0;
+  for (int f = 0; f < NF; ++f) H = std::max(H, h[f] + 1);
+  levels_.assign(H, Level{});
+  for (int f = 0; f < NF; ++f) levels_[h[f]].fronts.push_back(f);
+  std::vector<StepDesc> steps;
+  std::vector<int> ea_items, ea_tile0, solve_items;
+  for (int l = 0; l < H; ++l) {
+    Level& L = levels_[l];
+    L.items_off = (int)solve_items.size();
+    solve_items.insert(solve_items.end(), L.fronts.begin(), L.fronts.end());
+    // extend-add items: children of this level's fronts
+    L.ea_off = (int)ea_items.size();
+    int tiles = 0;
+    for (int f : L.fronts)
+      for (int c = 0; c < NF; ++c)
+        if (fr_[c].parent == f && fr_[c].s > 0) {
+          int T = (fr_[c].s + NB - 1) / NB;
+          ea_items.push_back(c);
+          ea_tile0.push_back(tiles);
+          tiles += T * (T + 1) / 2;
+        }
+    L.ea_n = (int)ea_items.size() - L.ea_off;
+    L.ea_tiles = tiles;
+    int kmax = 0;
+    for (int f : L.fronts) kmax = std::max(kmax, (fr_[f].r + NB - 1) / NB);
+    for (int k = 0; k < kmax; ++k) {
+      int pb = 0, ub = 0;
+      std::vector<StepDesc> pan, upd;
+      for (int f : L.fronts) {
+        int kb = k * NB;
+        if (kb >= fr_[f].r) continue;
+        int w = std::min(NB, fr_[f].r - kb);
+        int T = (fr_[f].ld - kb - w + NB - 1) / NB;
+        pan.push_back(StepDesc{f, kb, w, T, pb});
+        pb += 1 + T;
+        if (T > 0) {
+          upd.push_back(StepDesc{f, kb, w, T, ub});
+          ub += T * (T + 1) / 2;
+        }
+      }
+      L.pan_off.push_back((int)steps.size());
+      L.pan_n.push_back((int)pan.size());
+      L.pan_blk.push_back(pb);
+      steps.insert(steps.end(), pan.begin(), pan.end());
+      L.upd_off.push_back((int)steps.size());
+      L.upd_n.push_back((int)upd.size());
+      L.upd_blk.push_back(ub);
+      steps.insert(steps.end(), upd.begin(), upd.end());
+    }
+  }
+  d_steps_.upload(steps, st_);
+  d_ea_items_.upload(ea_items, st_);
+  d_ea_tile0_.upload(ea_tile0, st_);
+  d_solve_items_.upload(solve_items, st_);
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+double Solver::front_flops() const {
+  double t = 0.0;
+  for (const FrontDesc& F : fr_) {
+    double r = F.r, s = F.s;
+    t += r * r * r / 3.0 + s * r * r + s * s * r;
+  }
+  return t;
+}
+
+// ------------------------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------------------------
+
+void Solver::sync_scalars() {
+  HIPCHK(hipMemcpyAsync(hsc_, d_sc_.p, NSCALAR * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+void Solver::set_scalar(int slot, double v) {
+  hsc_[slot] = v;
+  // the value travels as a kernel argument: no host buffer outlives the call
+  launch_set_scalar(st_, d_sc_.as<double>(), slot, v);
+}
+
+void Solver::axpby(int space, int kx, int ky, int kz, Coef a, Coef b) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  if (space == 0) launch_axpby(st_, I, d_ntiles_.as<Tile>(), ntile_n_, 0, kx, ky, kz, a, b);
+  else launch_axpby(st_, I, d_mtiles_.as<Tile>(), ntile_m_, 1, kx, ky, kz, a, b);
+}
+
+void Solver::axpbypcz(int space, int kx, int ky, int kw, int kz, Coef a, Coef b, Coef c) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  if (space == 0) launch_axpbypcz(st_, I, d_ntiles_.as<Tile>(), ntile_n_, 0, kx, ky, kw, kz, a, b, c);
+  else launch_axpbypcz(st_, I, d_mtiles_.as<Tile>(), ntile_m_, 1, kx, ky, kw, kz, a, b, c);
+}
+
+void Solver::dot(int space, int kx, int ky, int mode, int slot, int op, int kw0, int kw1) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  if (space == 0)
+    launch_dot(st_, I, d_ntiles_.as<Tile>(), ntile_n_, d_ntofs_.as<int>(), 1, 0, kx, ky, mode, kw0, kw1,
+               d_partial_.as<double>(), slot, op);
+  else
+    launch_dot(st_, I, d_mtiles_.as<Tile>(), ntile_m_, d_mtofs_.as<int>(), 1, 1, kx, ky, mode, kw0, kw1,
+               d_partial_.as<double>(), slot, op);
+}
+
+void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  if (which == 0) launch_spmv(st_, I, d_mtiles_.as<Tile>(), ntile_m_, 0, kx, ky, a, b);
+  else launch_spmv(st_, I, d_ntiles_.as<Tile>(), ntile_n_, which, kx, ky, a, b);
+}
+
+// ky = H kx = Q kx + D kx
+void Solver::hmul(int kx, int ky) {
+  spmv(2, kx, ky, K(1.0), K(0.0));
+  launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, ky, K(1.0));
+}
+
+void Solver::lsolve(int kv) {
+  for (const Level& L : levels_)
+    launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, (int)L.fronts.size(),
+                  d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv);
+}
+
+void Solver::usolve(int kv) {
+  for (int l = (int)levels_.size() - 1; l >= 0; --l) {
+    const Level& L = levels_[l];
+    launch_bsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, (int)L.fronts.size(),
+                  d_inst_.as<InstDev>(), kv);
+  }
+}
+
+void Solver::run_factor() {
+  const FrontDesc* F = d_fr_.as<FrontDesc>();
+  const StepDesc* S = d_steps_.as<StepDesc>();
+  const InstDev* I = d_inst_.as<InstDev>();
+  for (const Level& L : levels_) {
+    launch_extend_add(st_, F, d_ea_items_.as<int>() + L.ea_off, d_ea_tile0_.as<int>() + L.ea_off, L.ea_n,
+                      L.ea_tiles);
+    for (size_t k = 0; k < L.pan_off.size(); ++k) {
+      launch_panel(st_, F, S + L.pan_off[k], L.pan_n[k], L.pan_blk[k], I);
+      launch_update(st_, F, S + L.upd_off[k], L.upd_n[k], L.upd_blk[k], I);
+    }
+  }
+}
+
+// initkkt! (uzawa.jl:73-115): F Fᵀ = δH + BᵀB, on failure retry with ρ = eps ‖δH + BᵀB‖ I
+bool Solver::factor(double delta, double& rho) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  auto assemble = [&]() {
+    launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
+    launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_);
+    launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(),
+                          ncone_);
+  };
+  set_scalar(S_DELTA, delta);
+  set_scalar(S_FAIL, 0.0);
+  assemble();
+  HIPCHK(hipEventRecord(ev0_, st_));
+  run_factor();
+  HIPCHK(hipEventRecord(ev1_, st_));
+  sync_scalars();
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  fac_ms_ = ms;
+  fac_ms_tot_ += ms;
+  fac_calls_ += 1;
+  rho = 0.0;
+  if (hsc_[S_FAIL] == 0.0) return true;
+  assemble();
+  launch_front_norm2(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), d_partial_.as<double>());
+  std::vector<double> part(fr_.size());
+  HIPCHK(hipMemcpyAsync(part.data(), d_partial_.p, part.size() * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  double s = 0.0;
+  for (double x : part) s += x;
+  rho = EPS * std::sqrt(s);
+  set_scalar(S_RHO, rho);
+  set_scalar(S_FAIL, 0.0);
+  launch_add_diag(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), I);
+  run_factor();
+  sync_scalars();
+  return hsc_[S_FAIL] == 0.0;
+}
+
+int Solver::kkt_factor(double delta, double* rho) {
+  double r = 0.0;
+  bool ok = factor(delta, r);
+  if (rho) *rho = r;
+  return ok ? 1 : 0;
+}
+
+double Solver::maxsteps(int kdp, int kdd, double step_frac) {
+  launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
+                      kdd, step_frac, d_coneout_.as<double>());
+  sync_scalars();
+  return hsc_[S_TAU];
+}
+
+// ------------------------------------------------------------------------------------------
+// starting point (solver/ipm.jl:321-429)
+// ------------------------------------------------------------------------------------------
+
+static void host_identity(double* x, const Cone& c, i64 n) {
+  if (c.kind == CONE_POS) {
+    for (i64 i = 0; i < n; ++i) x[i] = 1.0;
+  } else if (c.kind == CONE_FREE) {
+    for (i64 i = 0; i < n; ++i) x[i] = 0.0;
+  } else if (c.kind == CONE_SOC) {
+    for (i64 i = 0; i < n; ++i) x[i] = 0.0;
+    x[0] = std::sqrt(2.0);
+  }
+}
+
+static bool host_shadow(double* s, const double* x, const Cone& c, i64 n) {
+  if (c.kind == CONE_POS) {
+    for (i64 i = 0; i < n; ++i) s[i] = 1.0 / x[i];
+    return true;
+  }
+  if (c.kind == CONE_FREE) {
+    for (i64 i = 0; i < n; ++i) s[i] = 0.0;
+    return true;
+  }
+  // SOC: s = 2 Jx / det(x) with the compensated determinant (soc.jl:488-504)
+  double xdet = x[0] * x[0];
+  double corr = std::fma(x[0], x[0], -xdet);
+  for (i64 i = 1; i < n; ++i) {
+    double p = x[i] * x[i], e = std::fma(x[i], x[i], -p);
+    double sum = xdet - p, bb = sum - xdet;
+    double e2 = (xdet - (sum - bb)) + (-p - bb);
+    xdet = sum;
+    corr += -e + e2;
+  }
+  xdet = xdet + corr;
+  bool ok = xdet > 0 && x[0] > 0;
+  if (ok) {
+    double gm = 2.0 / xdet;
+    s[0] = gm * x[0];
+    for (i64 i = 1; i < n; ++i) s[i] = -gm * x[i];
+  }
+  return ok;
+}
+
+static void bsm_gemv(const BSM& A, const std::vector<double>& x, std::vector<double>& y) {
+  y.assign(A.nrow(), 0.0);
+  for (i64 v = 0; v < A.nvtx; ++v) {
+    i64 nc = A.ncols(v);
+    for (i64 e = A.xsrc[v]; e < A.xsrc[v + 1]; ++e) {
+      i64 u = A.tgt[e], nr = A.nrows(u);
+      for (i64 jl = 0; jl < nc; ++jl)
+        for (i64 il = 0; il < nr; ++il) y[A.xrow[u] + il] += A.val[A.xblk[e] + jl * nr + il] * x[A.xcol[v] + jl];
+    }
+  }
+}
+
+void Solver::reinit(const double* p0, const double* d0, const double* y0) {
+  std::vector<double> p(n_), d(n_), y(m_, 0.0);
+  const i64 N = P_.B.nvtx;
+  if (!p0 && !d0) {
+    for (i64 v = 0; v < N; ++v) {
+      host_identity(&p[P_.B.xcol[v]], P_.K[v], P_.B.ncols(v));
+      host_identity(&d[P_.B.xcol[v]], P_.K[v], P_.B.ncols(v));
+    }
+    std::vector<double> z, w;
+    bsm_gemv(P_.B, p, z);
+    bsm_gemv(P_.Q, p, w);
+    double np = norm2(p), nz = norm2(z), nw = norm2(w);
+    double sp = nz > EPS * np ? std::max(1.0, norm2(P_.g) / nz) : 1.0;
+    double sd = np > EPS ? std::max(1.0, (norm2(P_.f) + sp * nw) / np) : 1.0;
+    for (auto& x : p) x *= sp;
+    for (auto& x : d) x *= sd;
+    if (y0)
+      for (i64 i = 0; i < m_; ++i) y[i] = y0[P_.P1[i]] / yscl_[i];
+  } else {
+    if (p0)
+      for (i64 j = 0; j < n_; ++j) p[j] = p0[P_.P2[j]];
+    if (d0)
+      for (i64 j = 0; j < n_; ++j) d[j] = d0[P_.P2[j]];
+    for (i64 v = 0; v < N; ++v) {
+      i64 a = P_.B.xcol[v], nc = P_.B.ncols(v);
+      if (!d0) host_shadow(&d[a], &p[a], P_.K[v], nc);
+      else if (!p0) host_shadow(&p[a], &d[a], P_.K[v], nc);
+    }
+    if (y0)
+      for (i64 i = 0; i < m_; ++i) y[i] = y0[P_.P1[i]];
+    for (i64 j = 0; j < n_; ++j) {
+      p[j] /= pscl_[j];
+      d[j] *= pscl_[j];
+    }
+    for (i64 i = 0; i < m_; ++i) y[i] /= yscl_[i];
+  }
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_P * n_, p.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_D * n_, d.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_Y * m_, y.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  hist_.clear();
+  nH_ = norm2(P_.Q.val);
+  status_ = CONTINUE;
+}
+
+// ------------------------------------------------------------------------------------------
+// KKT: Uzawa refinement + dual CG (uzawa.jl:319-507, cg.jl:69-167)
+// ------------------------------------------------------------------------------------------
+
+int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& status) {
+  // cg.jl:69-167 with p = MV_CP, w = NV_CW
+  int niter = 0;
+  dot(1, ku, ku, 0, S_R8, 0);
+  if (stops) dot(1, ku, MV_YSCL, 1, S_R9, 1);
+  sync_scalars();
+  double nu2 = hsc_[S_R8], nu = std::sqrt(nu2), np2 = nu2;
+  double chist0 = nu;
+  std::vector<double>& ch = cghist_;
+  ch.assign(1, chist0);
+  axpby(1, ku, ku, ky, K(0.0), K(0.0));  // y ← 0
+  status = 1;                             // CG_SOLVED
+  auto stopf = [&](double scaled) { return stops && scaled / (1.0 + sg_) <= set_.feas_tol; };
+  if (nu > atol && !stopf(hsc_[S_R9])) {
+    status = 3;  // CG_ITMAX
+    if (itmax > 0) {
+      axpby(1, ku, ku, MV_CP, K(1.0), K(0.0));  // p ← u
+      status = 0;
+      set_scalar(S_CG_NU2, nu2);
+      set_scalar(S_CG_NP2, np2);
+      while (status == 0) {
+        niter += 1;
+        spmv(1, MV_CP, NV_CW, K(1.0), K(0.0));  // w = Bᵀ p
+        lsolve(NV_CW);
+        dot(0, NV_CW, NV_CW, 0, S_CG_EP2, 0);
+        sync_scalars();
+        double ep2 = hsc_[S_CG_EP2];
+        if (ep2 <= EPS * np2) {
+          status = 2;  // CG_NUMERICAL_FAILURE
+        } else {
+          usolve(NV_CW);
+          launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, 0);  // α = ν²/ε²
+          axpby(1, MV_CP, ky, ky, KS(1.0, S_CG_ALPHA), K(1.0));  // y += α p
+          axpby(0, NV_CW, kx, kx, KS(1.0, S_CG_ALPHA), K(1.0));  // x += α w
+          spmv(0, NV_CW, ku, KS(-1.0, S_CG_ALPHA), K(1.0));       // u −= α B w
+          dot(1, ku, ku, 0, S_NORM0, 0);
+          if (stops) dot(1, ku, MV_YSCL, 1, S_R9, 1);
+          sync_scalars();
+          double pnu2 = nu2;
+          nu2 = hsc_[S_NORM0];
+          nu = std::sqrt(nu2);
+          ch.push_back(nu);
+          if (nu <= atol || 1.0 + nu <= 1.0 || stopf(hsc_[S_R9])) {
+            status = 1;
+            set_scalar(S_CG_NU2, nu2);
+          } else if (niter >= itmax) {
+            status = 3;
+          } else {
+            double beta = nu2 / pnu2;
+            launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, 1);  // β, ν², np² on device
+            axpby(1, ku, MV_CP, MV_CP, K(1.0), KS(1.0, S_CG_BETA));
+            np2 = nu2 + beta * beta * np2;
+          }
+        }
+      }
+    }
+  }
+  return niter;
+}
+
+Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, double ftol, double gtol, int irmin,
+                                bool stops) {
+  const double stall = set_.refine_stall_tol;
+  const int irmax = set_.refine_max_iter, cgmax = set_.newton_max_iter;
+  int nir = 0;
+  std::vector<double> hist;
+  auto residual = [&]() {
+    // KF = f − H x + Bᵀ y ; KG = g − B x
+    hmul(kx, NV_T0);
+    spmv(1, ky, NV_T1, K(1.0), K(0.0));
+    axpbypcz(0, kf, NV_T0, NV_T1, NV_KF, K(1.0), K(-1.0), K(1.0));
+    axpby(1, kg, kg, MV_KG, K(1.0), K(0.0));
+    spmv(0, kx, MV_KG, K(-1.0), K(1.0));
+    dot(0, NV_KF, NV_KF, 0, S_R10, 1);
+    if (stops) dot(0, NV_KF, NV_PSCL, 1, S_R11, 1);
+    sync_scalars();
+  };
+  if (warm) {
+    residual();
+  } else {
+    axpby(0, kf, kf, NV_KF, K(1.0), K(0.0));
+    axpby(1, kg, kg, MV_KG, K(1.0), K(0.0));
+    axpby(0, kx, kx, kx, K(0.0), K(0.0));
+    axpby(1, ky, ky, ky, K(0.0), K(0.0));
+    dot(0, NV_KF, NV_KF, 0, S_R10, 1);
+    if (stops) dot(0, NV_KF, NV_PSCL, 1, S_R11, 1);
+    sync_scalars();
+  }
+  auto fstop = [&]() { return stops && hsc_[S_R11] / (1.0 + sf_) <= set_.feas_tol; };
+  double fres = hsc_[S_R10], fprv = fres;
+  hist.push_back(fres);
+  int irstat;
+  if (nir >= irmin && (fres <= ftol || fstop())) irstat = KKT_SOLVED;
+  else if (nir >= irmax) irstat = KKT_IRMAX;
+  else irstat = KKT_CONTINUE;
+  while (irstat == KKT_CONTINUE) {
+    nir += 1;
+    spmv(1, MV_KG, NV_T0, K(1.0), K(0.0));                          // T0 = Bᵀ δg
+    axpby(0, NV_KF, NV_T0, NV_KX, KS(1.0, S_DELTA), K(1.0));        // δx = δ δf + Bᵀ δg
+    lsolve(NV_KX);
+    usolve(NV_KX);
+    spmv(0, NV_KX, MV_KG, K(-1.0), K(1.0));                         // δg −= B δx
+    axpby(0, NV_KX, kx, kx, K(1.0), K(1.0));                        // x += δx
+    axpby(1, MV_KG, ky, ky, KS(1.0, S_ALPHA), K(1.0));              // y += α δg
+    residual();
+    fres = hsc_[S_R10];
+    hist.push_back(fres);
+    if (nir >= irmin && (fres <= ftol || fstop())) irstat = KKT_SOLVED;
+    else if (nir > 1 && fres > (1 - stall) * fprv) irstat = KKT_STAGNATED;
+    else if (nir >= irmax) irstat = KKT_IRMAX;
+    else irstat = KKT_CONTINUE;
+    fprv = fres;
+  }
+  axpby(1, MV_KG, ky, ky, KS(-1.0, S_ALPHA), K(1.0));  // y −= α δg
+  int cgstat = 1;
+  int ncg = cg(kx, MV_KY, MV_KG, gtol, cgmax, stops, cgstat);
+  axpbypcz(1, MV_KY, MV_KG, ky, ky, KS(1.0, S_ALPHA), KS(1.0, S_ALPHA), K(1.0));  // y += α δy + α δg
+  KKTOut o;
+  o.ncg = ncg;
+  o.nir = nir;
+  if (cgstat == 2) o.status = KKT_NUMERICAL_FAILURE;
+  else if (irstat == KKT_STAGNATED) o.status = KKT_STAGNATED;
+  else if (irstat == KKT_IRMAX || cgstat == 3) o.status = KKT_ITMAX;
+  else o.status = KKT_SOLVED;
+  // kktwindow (uzawa.jl:199-227)
+  o.dmin = o.dmax = NaN;
+  double delta = hsc_[S_DELTA];
+  if (nir >= 1) {
+    double logd = std::log(delta), logfres = std::log(hist[1]), loggtol = std::log(gtol),
+           logftol = std::log(ftol);
+    int i = 0;
+    double logdmax = -std::numeric_limits<double>::infinity();
+    double logdmin = logd + logfres - logftol;
+    while (logdmin > logdmax && i < ncg + 1) {
+      i += 1;
+      double loggres = std::log(cghist_[i - 1]);
+      logdmax = logd + (loggtol - loggres) / i;
+    }
+    o.dmin = std::exp(logdmin);
+    o.dmax = std::exp(logdmax);
+  }
+  return o;
+}
+
+// ------------------------------------------------------------------------------------------
+// step! (solver/ipm.jl:577-774)
+// ------------------------------------------------------------------------------------------
+
+double Solver::getaug() {
+  // solver.jl:40-52, history.jl:9-25
+  if (nB_ == 0.0 || nH_ == 0.0) return 1.0;
+  if (hist_.empty()) return set_.aug_tol * nB_ * nB_ / nH_;
+  const HistRow& h = hist_.back();
+  double delta = h.delta;
+  if (h.ppass > 1) {
+    if (std::isfinite(h.dmin)) delta = std::max(delta, h.dmin);
+    delta *= 10;
+  } else if (std::isfinite(h.dmin) && std::isfinite(h.dmax) && h.dmin > 0 && h.dmax > 0) {
+    delta = std::sqrt(h.dmin) * std::sqrt(h.dmax);
+  }
+  return delta;
+}
+
+bool Solver::isstalled() const {
+  const int window = 5;
+  double tol = set_.stall_tol;
+  int n = (int)hist_.size();
+  if (n <= window) return false;
+  for (int i = n - window; i < n; ++i) {
+    const HistRow &h = hist_[i], &hp = hist_[i - 1];
+    if (h.mu < (1 - tol) * hp.mu || h.pres < (1 - tol) * hp.pres || h.dres < (1 - tol) * hp.dres) return false;
+  }
+  return true;
+}
+
+bool Solver::isopt(double mu, double mus, double pobj, double dobj, double pres, double dres, double fac) const {
+  double feas = fac * set_.feas_tol, gap = fac * set_.gap_tol;
+  if (!(pres < feas && dres < feas)) return false;
+  double tol = gap * std::max(1.0, std::min(std::fabs(pobj), std::fabs(dobj)));
+  if (mu_t_ > 0 && nu_ > 0) return nu_ * (mu - mu_t_ + mu_t_ * std::log1p(mus * mu_t_ - 1.0)) <= tol;
+  return pobj - dobj < tol;
+}
+
+// statistics of the current iterate for the termination and certificate tests
+void Solver::compute_stats() {
+  // R0 gy, R1 ‖y‖, R2 ‖Qp‖, R3 ‖d‖, R4 ‖Qp − d − Bᵀy‖, R5 fp, R6 ‖p‖, R7 ‖Bp‖
+  dot(1, MV_G, MV_Y, 0, S_R0, 0);
+  dot(1, MV_Y, MV_Y, 0, S_R1, 1);
+  dot(0, NV_QP, NV_QP, 0, S_R2, 1);
+  dot(0, NV_D, NV_D, 0, S_R3, 1);
+  spmv(1, MV_Y, NV_T1, K(1.0), K(0.0));
+  axpbypcz(0, NV_QP, NV_D, NV_T1, NV_T2, K(1.0), K(-1.0), K(-1.0));
+  dot(0, NV_T2, NV_T2, 0, S_R4, 1);
+  dot(0, NV_F, NV_P, 0, S_R5, 0);
+  dot(0, NV_P, NV_P, 0, S_R6, 1);
+  spmv(0, NV_P, MV_T0, K(1.0), K(0.0));
+  dot(1, MV_T0, MV_T0, 0, S_R7, 1);
+  sync_scalars();
+}
+
+// solver/ipm.jl:502-517
+bool Solver::isprimalinfeasible(double rtol, double atol) {
+  double gy = hsc_[S_R0], ny = hsc_[S_R1];
+  bool flag = gy > atol * ny * (1 + ng_);
+  if (flag) flag = std::max(hsc_[S_R2], hsc_[S_R4]) < rtol * gy * (1 + hsc_[S_R3] / ny);
+  return flag;
+}
+
+// solver/ipm.jl:528-540
+bool Solver::isdualinfeasible(double rtol, double atol) {
+  double fp = hsc_[S_R5], np = hsc_[S_R6];
+  bool flag = fp > atol * np * (1 + nf_);
+  if (flag) flag = std::max(hsc_[S_R7], hsc_[S_R2]) < rtol * std::fabs(fp);
+  return flag;
+}
+
+int Solver::nearstatus(int status, double mu, double mus, double pobj, double dobj, double pres, double dres) {
+  double f = set_.near_factor;
+  if (isopt(mu, mus, pobj, dobj, pres, dres, f)) return NEAR_OPTIMAL;
+  if (isprimalinfeasible(f * set_.infeas_rel, set_.infeas_abs)) return NEAR_PRIMAL_INFEASIBLE;
+  if (isdualinfeasible(f * set_.infeas_rel, set_.infeas_abs)) return NEAR_DUAL_INFEASIBLE;
+  return status;
+}
+
+int Solver::step() {
+  int status = CONTINUE;
+  HistRow row{NaN, NaN, NaN, NaN, 0.0, 0, 0, KKT_SOLVED, 0, 0, KKT_SOLVED, NaN, NaN, NaN, 0.0, NaN, NaN};
+  const double mut = mu_t_;
+  double delta = getaug();
+  row.delta = delta;
+  set_scalar(S_DELTA, delta);
+  set_scalar(S_ALPHA, 1.0 / delta);
+  // pᵀd, scale!, residuals!
+  dot(0, NV_P, NV_D, 0, S_TMP0, 0);
+  launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
+                    d_coneout_.as<double>());
+  launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
+                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1);
+  spmv(2, NV_P, NV_QP, K(1.0), K(0.0));                                  // Qp
+  spmv(1, MV_Y, NV_T0, K(1.0), K(0.0));                                  // Bᵀy
+  axpbypcz(0, NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0));        // Δf = d + f − Qp
+  axpby(0, NV_T0, NV_DF, NV_DF, K(1.0), K(1.0));                         //      + Bᵀy
+  axpby(1, MV_G, MV_G, MV_DG, K(1.0), K(0.0));                           // Δg = g
+  spmv(0, NV_P, MV_DG, K(-1.0), K(1.0));                                 //      − Bp
+  dot(1, MV_DG, MV_YSCL, 1, S_NORM0, 1);
+  dot(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
+  dot(0, NV_P, NV_QP, 0, S_NORM2, 0);
+  dot(0, NV_F, NV_P, 0, S_NORM3, 0);
+  dot(1, MV_G, MV_Y, 0, S_NORM4, 0);
+  sync_scalars();
+  double pd = hsc_[S_TMP0];
+  bool flag = hsc_[S_FLAGOK] != 0.0;
+  double spsd = hsc_[S_SPSD];
+  nH_ = std::sqrt(qoff2_ + hsc_[S_TMP1]);
+  if (!flag) {
+    status = NUMERICAL_FAILURE;
+  } else {
+    double nu1 = std::max(nu_, 1);
+    double mu = pd / nu1, mus = spsd / nu1, chi = mu * mus - 1;
+    double pres = hsc_[S_NORM0] / (1 + sg_), dres = hsc_[S_NORM1] / (1 + sf_);
+    double pQp = hsc_[S_NORM2];
+    double pobj = pQp / 2 - hsc_[S_NORM3];
+    double dobj = hsc_[S_NORM4] - pQp / 2;
+    row.mu = mu;
+    row.chi = chi;
+    row.pres = pres;
+    row.dres = dres;
+    row.pobj = pobj;
+    row.dobj = dobj;
+    compute_stats();
+    if (isopt(mu, mus, pobj, dobj, pres, dres, 1.0)) {
+      status = OPTIMAL;
+    } else if (isprimalinfeasible(set_.infeas_rel, set_.infeas_abs)) {
+      status = PRIMAL_INFEASIBLE;
+    } else if (isdualinfeasible(set_.infeas_rel, set_.infeas_abs)) {
+      status = DUAL_INFEASIBLE;
+    } else if ((int)hist_.size() >= set_.max_iter) {
+      status = nearstatus(ITERATION_LIMIT, mu, mus, pobj, dobj, pres, dres);
+    } else if (isstalled()) {
+      status = nearstatus(STALLED, mu, mus, pobj, dobj, pres, dres);
+    } else if (nu_ == 0) {
+      double rho;
+      bool ok = factor(delta, rho);
+      row.rho = rho;
+      if (!ok) {
+        status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
+      } else {
+        // solveexact! (solver/ipm.jl:284-315)
+        axpby(0, NV_DF, NV_DF, NV_FR, K(1.0), K(0.0));
+        KKTOut o = solvekkt(NV_DPA, MV_DYA, NV_FR, MV_DG, false, EPS, EPS, 0, true);
+        row.piter = o.ncg;
+        row.ppass = o.nir;
+        row.pstat = o.status;
+        row.dmin = o.dmin;
+        row.dmax = o.dmax;
+        axpby(0, NV_DPA, NV_P, NV_P, K(1.0), K(1.0));
+        axpby(1, MV_DYA, MV_Y, MV_Y, K(1.0), K(1.0));
+      }
+    } else if (!(mu > 0)) {
+      status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
+    } else {
+      double rho;
+      bool ok = factor(delta, rho);
+      row.rho = rho;
+      if (!ok) {
+        status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
+      } else {
+        double mu1 = hist_.empty() ? mu : hist_[0].mu;
+        double tol;
+        if (mut > 0) tol = FORCING_FRAC * std::min(std::fabs(mu - mut) / std::fabs(mu1 - mut), 1.0);
+        else tol = FORCING_FRAC * mu / mu1;
+        double ftol = tol * (1 + nf_), gtol = tol * (1 + ng_);
+        // predictor (solver/ipm.jl:111-163)
+        if (mut > 0) {
+          set_scalar(S_SMU, mut);
+          launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
+        } else {
+          axpby(0, NV_D, NV_D, NV_FR, K(-1.0), K(0.0));
+        }
+        axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
+        KKTOut po = solvekkt(NV_DPA, MV_DYA, NV_FR, MV_DG, false, ftol, gtol, 1, false);
+        // Δda = Q Δpa − Bᵀ Δya − Δf
+        spmv(1, MV_DYA, NV_T0, K(1.0), K(0.0));
+        spmv(2, NV_DPA, NV_DDA, K(1.0), K(0.0));
+        axpbypcz(0, NV_T0, NV_DF, NV_DDA, NV_DDA, K(-1.0), K(-1.0), K(1.0));
+        launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DDA);
+        // corrector (solver/ipm.jl:181-272)
+        double taua = maxsteps(NV_DPA, NV_DDA, 1.0);
+        dot(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA);
+        sync_scalars();
+        double smu = hsc_[S_TMP2];
+        double mua = smu / nu_;
+        if (mut > 0) {
+          double dmu = mu - mut;
+          if (dmu > 0) {
+            double dmua = std::min(std::max(mua - mut, 0.0), dmu);
+            smu = mut + std::min(std::max(dmua * (dmua / dmu) * (dmua / dmu), 0.0), dmu);
+          } else {
+            smu = mut;
+          }
+        } else {
+          smu = std::min(std::max(mua * (mua / mu) * (mua / mu), 0.0), mu);
+        }
+        (void)taua;
+        set_scalar(S_SMU, smu);
+        launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
+        axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
+        axpby(0, NV_DPA, NV_DPA, NV_DP, K(1.0), K(0.0));
+        axpby(1, MV_DYA, MV_DYA, MV_DY, K(1.0), K(0.0));
+        KKTOut co = solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, true, ftol, gtol, 1, false);
+        spmv(1, MV_DY, NV_T0, K(1.0), K(0.0));
+        spmv(2, NV_DP, NV_DD, K(1.0), K(0.0));
+        axpbypcz(0, NV_T0, NV_DF, NV_DD, NV_DD, K(-1.0), K(-1.0), K(1.0));
+        launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DD);
+        row.piter = po.ncg;
+        row.ppass = po.nir;
+        row.pstat = po.status;
+        row.dmin = po.dmin;
+        row.dmax = po.dmax;
+        row.citer = co.ncg;
+        row.cpass = co.nir;
+        row.cstat = co.status;
+        if (po.status != KKT_SOLVED || co.status != KKT_SOLVED) {
+          compute_stats();
+          status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
+        } else {
+          double stp = maxsteps(NV_DP, NV_DD, set_.step_frac);
+          row.step = stp;
+          set_scalar(S_STEP, stp);
+          axpby(0, NV_DP, NV_P, NV_P, KS(1.0, S_STEP), K(1.0));
+          axpby(0, NV_DD, NV_D, NV_D, KS(1.0, S_STEP), K(1.0));
+          axpby(1, MV_DY, MV_Y, MV_Y, KS(1.0, S_STEP), K(1.0));
+        }
+      }
+    }
+  }
+  hist_.push_back(row);
+  status_ = status;
+  return status;
+}
+
+int Solver::solve() {
+  int status = CONTINUE;
+  while (status == CONTINUE) status = step();
+  return status;
+}
+
+// result (solver/ipm.jl:30-74)
+void Solver::result(double* p, double* d, double* y, double* info) {
+  std::vector<double> hp(n_), hd(n_), hy(m_);
+  HIPCHK(hipMemcpyAsync(hp.data(), idev_.nv + (size_t)NV_P * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(hd.data(), idev_.nv + (size_t)NV_D * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(hy.data(), idev_.mv + (size_t)MV_Y * m_, m_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  double mu = NaN, pres = NaN, dres = NaN, pobj = NaN, dobj = NaN;
+  int s = status_;
+  if (!(s == PRIMAL_INFEASIBLE || s == NEAR_PRIMAL_INFEASIBLE || s == DUAL_INFEASIBLE ||
+        s == NEAR_DUAL_INFEASIBLE || hist_.empty())) {
+    spmv(2, NV_P, NV_QP, K(1.0), K(0.0));
+    spmv(1, MV_Y, NV_T0, K(1.0), K(0.0));
+    axpbypcz(0, NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0));
+    axpby(0, NV_T0, NV_DF, NV_DF, K(1.0), K(1.0));
+    axpby(1, MV_G, MV_G, MV_DG, K(1.0), K(0.0));
+    spmv(0, NV_P, MV_DG, K(-1.0), K(1.0));
+    dot(1, MV_DG, MV_YSCL, 1, S_NORM0, 1);
+    dot(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
+    dot(0, NV_P, NV_QP, 0, S_NORM2, 0);
+    dot(0, NV_F, NV_P, 0, S_NORM3, 0);
+    dot(1, MV_G, MV_Y, 0, S_NORM4, 0);
+    dot(0, NV_P, NV_D, 0, S_TMP0, 0);
+    sync_scalars();
+    double pQp = hsc_[S_NORM2];
+    pobj = pQp / 2 - hsc_[S_NORM3];
+    dobj = hsc_[S_NORM4] - pQp / 2;
+    mu = nu_ == 0 ? NaN : hsc_[S_TMP0] / nu_;
+    pres = hsc_[S_NORM0] / (1 + sg_);
+    dres = hsc_[S_NORM1] / (1 + sf_);
+  }
+  for (i64 j = 0; j < n_; ++j) {
+    if (p) p[P_.P2[j]] = hp[j] * pscl_[j];
+    if (d) d[P_.P2[j]] = hd[j] / pscl_[j];
+  }
+  if (y)
+    for (i64 i = 0; i < m_; ++i) y[P_.P1[i]] = hy[i] * yscl_[i];
+  if (info) {
+    int nsolve = 0;
+    for (const HistRow& h : hist_) nsolve += h.piter + h.ppass + h.citer + h.cpass;
+    info[0] = mu;
+    info[1] = pres;
+    info[2] = dres;
+    info[3] = pobj;
+    info[4] = dobj;
+    info[5] = (double)hist_.size();
+    info[6] = (double)nsolve;
+    info[7] = (double)status_;
+  }
+}
+
+void Solver::dense_factor(double* out) {
+  std::vector<double> F(fsize_);
+  HIPCHK(hipMemcpyAsync(F.data(), d_F_.p, fsize_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  std::memset(out, 0, sizeof(double) * n_ * n_);
+  size_t ofs = 0;
+  for (size_t f = 0; f < fr_.size(); ++f) {
+    const FrontDesc& Fd = fr_[f];
+    std::vector<int> rows;
+    for (int i = 0; i < Fd.r; ++i) rows.push_back(Fd.res0 + i);
+    std::vector<int> sep(Fd.s);
+    if (Fd.s) HIPCHK(hipMemcpy(sep.data(), Fd.sepcols, Fd.s * 4, hipMemcpyDeviceToHost));
+    rows.insert(rows.end(), sep.begin(), sep.end());
+    for (int j = 0; j < Fd.r; ++j)
+      for (int i = j; i < Fd.ld; ++i) out[(size_t)rows[i] + (size_t)rows[j] * n_] = F[ofs + i + (size_t)j * Fd.ld];
+    ofs += (size_t)Fd.ld * Fd.ld;
+  }
+}
+
+}  // namespace mb

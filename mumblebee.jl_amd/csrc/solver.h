@@ -1,0 +1,167 @@
+// The IPM driver over device-resident state: restates src/IPM/src/ipm/solver/ipm.jl (step!,
+// predictor, corrector, termination) and src/IPM/src/kkt/uzawa.jl / cg.jl (Uzawa refinement,
+// dual CG) with every vector operation, cone kernel, factorization and triangular solve on the
+// GPU; only scalars cross to the host.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "host.h"
+#include "kernels.h"
+
+namespace mb {
+
+struct Settings {
+  int verbose = 0;
+  double step_frac = 0.99;
+  double feas_tol = 1e-8;
+  double gap_tol = 1e-8;
+  int max_iter = 100;
+  double near_factor = 1000.0;
+  double stall_tol = 1e-3;
+  double refine_stall_tol = 0.5;
+  int scale_max_iter = 10;
+  int refine_max_iter = 10;
+  int newton_max_iter = 100;
+  double aug_tol = 1e-7;
+  double infeas_abs = 1e-8;
+  double infeas_rel = 1e-8;
+  int amalgamate = 1;
+};
+
+struct HistRow {
+  double pres, dres, pobj, dobj, step;
+  int piter, ppass, pstat, citer, cpass, cstat;
+  double dmin, dmax, rho, delta, mu, chi;
+};
+
+enum IPMStatus {
+  CONTINUE = 0, OPTIMAL, NEAR_OPTIMAL, STALLED, NUMERICAL_FAILURE, ITERATION_LIMIT, PRIMAL_INFEASIBLE,
+  DUAL_INFEASIBLE, ILL_POSED, NEAR_PRIMAL_INFEASIBLE, NEAR_DUAL_INFEASIBLE, NEAR_ILL_POSED
+};
+enum KKTStatus { KKT_CONTINUE = 0, KKT_SOLVED, KKT_STAGNATED, KKT_ITMAX, KKT_NUMERICAL_FAILURE, KKT_IRMAX };
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void alloc(size_t b);
+  template <class T>
+  T* as() const { return (T*)p; }
+  template <class T>
+  void upload(const std::vector<T>& v, hipStream_t st);
+};
+
+struct Level {
+  std::vector<int> fronts;   // fronts at this level
+  // extend-add items (children of this level's fronts) and their tile prefix
+  int ea_off = 0, ea_n = 0, ea_tiles = 0;
+  // panel / update steps: (offset into step table, count, blocks) per panel step k
+  std::vector<int> pan_off, pan_n, pan_blk, upd_off, upd_n, upd_blk;
+  int items_off = 0;         // fronts of the level in the solve item table
+};
+
+class Solver {
+ public:
+  Solver(const Problem& prob, const Settings& st, const double* p0, const double* d0, const double* y0);
+  ~Solver();
+  int step();
+  int solve();
+  void result(double* p, double* d, double* y, double* info);  // info: mu pres dres pobj dobj niter nsolve status
+  const std::vector<HistRow>& history() const { return hist_; }
+  // diagnostics
+  double exact_flops() const { return sym_.exact_flops; }
+  double front_flops() const;
+  int nfront() const { return (int)fr_.size(); }
+  int nlevel() const { return (int)levels_.size(); }
+  double last_factor_ms() const { return fac_ms_; }
+  double total_factor_ms() const { return fac_ms_tot_; }
+  int factor_calls() const { return fac_calls_; }
+  void dense_factor(double* out);  // n×n lower factor (debug; small n)
+  int kkt_factor(double delta, double* rho);  // initkkt! on the current H
+  hipStream_t stream() const { return st_; }
+  int n() const { return (int)n_; }
+  int m() const { return (int)m_; }
+  int last_status() const { return status_; }
+
+ private:
+  // setup
+  void build_device();
+  void build_schedule();
+  // device ops
+  void sync_scalars();
+  void set_scalar(int slot, double v);
+  void axpby(int space, int kx, int ky, int kz, Coef a, Coef b);
+  void axpbypcz(int space, int kx, int ky, int kw, int kz, Coef a, Coef b, Coef c);
+  void dot(int space, int kx, int ky, int mode, int slot, int op, int kw0 = 0, int kw1 = 0);
+  void spmv(int which, int kx, int ky, Coef a, Coef b);
+  void hmul(int kx, int ky);  // ky = H kx
+  void lsolve(int kv);
+  void usolve(int kv);
+  bool factor(double delta, double& rho);
+  void run_factor();
+  double maxsteps(int kdp, int kdd, double step_frac);
+  void reinit(const double* p0, const double* d0, const double* y0);
+  // KKT
+  struct KKTOut {
+    int ncg, nir, status;
+    double dmin, dmax;
+  };
+  KKTOut solvekkt(int kx, int ky, int kf, int kg, bool warm, double ftol, double gtol, int irmin, bool stops);
+  int cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& status);
+  double getaug();
+  bool isstalled() const;
+  int nearstatus(int status, double mu, double mus, double pobj, double dobj, double pres, double dres);
+  bool isopt(double mu, double mus, double pobj, double dobj, double pres, double dres, double factor) const;
+  bool isprimalinfeasible(double rtol, double atol);
+  bool isdualinfeasible(double rtol, double atol);
+  void compute_stats();
+
+  Settings set_;
+  Problem P_;          // permuted + scaled
+  Symbolic sym_;
+  int64_t n_ = 0, m_ = 0;
+  int nu_ = 0;
+  double mu_t_ = 0.0;
+  std::vector<double> pscl_, yscl_;
+  double nB_ = 0, nf_ = 0, ng_ = 0, sf_ = 0, sg_ = 0, nH_ = 0, qoff2_ = 0;
+  std::vector<HistRow> hist_;
+  std::vector<double> cghist_;
+  int status_ = CONTINUE;
+
+  // host copies of device descriptors
+  std::vector<FrontDesc> fr_;
+  std::vector<ConeDesc> cones_;
+  std::vector<Level> levels_;
+  std::vector<int> cone_front_;
+  int ncone_ = 0;
+  int64_t fsize_ = 0;
+
+  hipStream_t st_ = nullptr;
+  double* hsc_ = nullptr;  // pinned host scalar mirror
+  InstDev idev_{};
+  // device buffers
+  DevBuf d_inst_, d_nv_, d_mv_, d_sc_, d_scr_, d_F_, d_fr_, d_cones_, d_cone_front_, d_cone_ofs_;
+  DevBuf d_Bp_, d_Bc_, d_Bv_, d_Btp_, d_Btc_, d_Btv_, d_Qp_, d_Qc_, d_Qv_;
+  DevBuf d_upos_, d_ul_, d_uq_, d_D_, d_cache_, d_uvec_, d_sepcols_, d_relidx_, d_childlist_;
+  DevBuf d_steps_, d_ea_items_, d_ea_tile0_, d_solve_items_, d_ntiles_, d_mtiles_, d_ntofs_, d_mtofs_;
+  DevBuf d_partial_, d_coneout_, d_qdiag_;
+  int ntile_n_ = 0, ntile_m_ = 0;
+  int64_t nunion_ = 0;
+  double fac_ms_ = 0.0, fac_ms_tot_ = 0.0;
+  int fac_calls_ = 0;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+// restatement of equilibrate! (src/IPM/src/scaling/scaling.jl:122-184) on host BSMs
+void equilibrate(BSM& B, BSM& Q, std::vector<double>& f, std::vector<double>& g, int itmax, double tol,
+                 std::vector<double>& pscl, std::vector<double>& yscl);
+
+}  // namespace mb

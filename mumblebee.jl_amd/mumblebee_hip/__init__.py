@@ -1,0 +1,388 @@
+"""mumblebee_hip — the MI355X conic IPM solve path behind Mumblebee.jl's own interface.
+
+This module mirrors the reference's Julia API for the solve path (same names, argument meaning
+and error behaviour) over the C ABI in include/mumblebee_hip.h:
+
+    IPMProblem(Q, B, f, g, μ, K, s; compress)   src/IPM/src/ipm/problem.jl:64-127
+    IPMSettings(...)                            src/IPM/src/ipm/settings/ipm.jl:1-18
+    init(prob, settings; p0, d0, y0)            src/IPM/src/ipm/solver/ipm.jl:485-491
+    step(solver)  (step!)                       src/IPM/src/ipm/solver/ipm.jl:577-774
+    solve(prob, settings) / solve_(solver)      src/IPM/src/ipm/solver/ipm.jl:798-804
+    IPMResult, IPMHistory rows, IPMStatus       result/ipm.jl, history/ipm.jl, ipm/ipm.jl:1
+
+All numerics run in the HIP library; there is no CPU fallback. Loading fails loudly when the
+library has not been built.
+"""
+import ctypes
+import enum
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libmumblebee_hip.so")
+_lib = None
+
+
+class IPMStatus(enum.IntEnum):
+    CONTINUE = 0
+    OPTIMAL = 1
+    NEAR_OPTIMAL = 2
+    STALLED = 3
+    NUMERICAL_FAILURE = 4
+    ITERATION_LIMIT = 5
+    PRIMAL_INFEASIBLE = 6
+    DUAL_INFEASIBLE = 7
+    ILL_POSED = 8
+    NEAR_PRIMAL_INFEASIBLE = 9
+    NEAR_DUAL_INFEASIBLE = 10
+    NEAR_ILL_POSED = 11
+
+
+class KKTStatus(enum.IntEnum):
+    KKT_CONTINUE = 0
+    KKT_SOLVED = 1
+    KKT_STAGNATED = 2
+    KKT_ITMAX = 3
+    KKT_NUMERICAL_FAILURE = 4
+    KKT_IRMAX = 5
+
+
+# cones (src/IPM/src/cone/*.jl); kind codes are the C ABI's
+class AbstractCone:
+    kind = -1
+    param = 0.0
+
+
+class PositiveCone(AbstractCone):
+    kind = 0
+
+
+class CofreeCone(AbstractCone):
+    kind = 1
+
+
+class SecondOrderCone(AbstractCone):
+    kind = 2
+
+
+class SemidefiniteCone(AbstractCone):
+    kind = 3
+
+
+class ExponentialCone(AbstractCone):
+    kind = 4
+
+
+class PowerCone(AbstractCone):
+    kind = 5
+
+    def __init__(self, alpha):
+        self.param = float(alpha)
+
+
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+class _Settings(ctypes.Structure):
+    _fields_ = [("verbose", ctypes.c_int32), ("step_frac", ctypes.c_double), ("feas_tol", ctypes.c_double),
+                ("gap_tol", ctypes.c_double), ("max_iter", ctypes.c_int32), ("near_factor", ctypes.c_double),
+                ("stall_tol", ctypes.c_double), ("refine_stall_tol", ctypes.c_double),
+                ("scale_max_iter", ctypes.c_int32), ("refine_max_iter", ctypes.c_int32),
+                ("newton_max_iter", ctypes.c_int32), ("aug_tol", ctypes.c_double),
+                ("infeas_abs", ctypes.c_double), ("infeas_rel", ctypes.c_double),
+                ("amalgamate", ctypes.c_int32), ("precision", ctypes.c_int32)]
+
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "mb_settings_default": (None, [ctypes.POINTER(_Settings)]),
+    "mb_problem_create": (ctypes.c_void_p, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _DP, _I64P, _I64P, _DP,
+                                            _DP, _DP, ctypes.c_double, ctypes.c_int64, _I32P, _DP, _I64P,
+                                            ctypes.c_double]),
+    "mb_problem_destroy": (None, [ctypes.c_void_p]),
+    "mb_problem_dims": (ctypes.c_int, [ctypes.c_void_p, _I64P]),
+    "mb_problem_export": (ctypes.c_int, [ctypes.c_void_p, _I64P, _I64P, _I32P, _I64P, _I64P, _I64P, _I64P, _I64P,
+                                         _DP, _I64P, _I64P, _I64P, _DP, _DP, _DP]),
+    "mb_problem_symbolic_dims": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I64P]),
+    "mb_problem_symbolic": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I64P, _I64P, _I64P, _I64P, _I64P,
+                                           _DP]),
+    "mb_solver_create": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.POINTER(_Settings), _DP, _DP, _DP]),
+    "mb_solver_destroy": (None, [ctypes.c_void_p]),
+    "mb_solver_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "mb_solver_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "mb_solver_result": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP, _DP]),
+    "mb_solver_niter": (ctypes.c_int64, [ctypes.c_void_p]),
+    "mb_solver_history": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
+    "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
+    "mb_last_error": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Load libmumblebee_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build the HIP library first (__graft_entry__.build())")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _err(what):
+    msg = lib().mb_last_error()
+    return RuntimeError(f"{what}: {msg.decode() if msg else 'unknown error'}")
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype)) if a is not None else None
+
+
+def _csc(A, m, n):
+    import scipy.sparse as sp
+    A = sp.csc_matrix(A, shape=(m, n), copy=True)
+    A.sort_indices()
+    return (np.ascontiguousarray(A.indptr, dtype=np.int64), np.ascontiguousarray(A.indices, dtype=np.int64),
+            np.ascontiguousarray(A.data, dtype=np.float64))
+
+
+@dataclass
+class IPMSettings:
+    """IPMSettings{Float64} (settings/ipm.jl:1-18); `amalgamate` and `precision` are this
+    build's knobs (relaxed supernode amalgamation; 0 = FP64 factor)."""
+    verbose: int = 0
+    step_frac: float = 0.99
+    feas_tol: float = 1e-8
+    gap_tol: float = 1e-8
+    max_iter: int = 100
+    near_factor: float = 1000.0
+    stall_tol: float = 1e-3
+    refine_stall_tol: float = 0.5
+    scale_max_iter: int = 10
+    refine_max_iter: int = 10
+    newton_max_iter: int = 100
+    aug_tol: float = 1e-7
+    infeas_abs: float = 1e-8
+    infeas_rel: float = 1e-8
+    amalgamate: int = 1
+    precision: int = 0
+
+    def _c(self):
+        s = _Settings()
+        for k in _Settings._fields_:
+            setattr(s, k[0], getattr(self, k[0]))
+        return s
+
+
+class IPMProblem:
+    """IPMProblem(Q, B, f, g, μ, K, s; compress=1.0) (problem.jl:64-127).
+
+    Q (n×n) and B (m×n) are scipy sparse matrices (stored zeros are kept); f, g dense; K a list
+    of cone objects with block sizes s (Σs = n)."""
+
+    def __init__(self, Q, B, f, g, mu, K, s, compress=1.0):
+        m, n = B.shape
+        self.n, self.m = int(n), int(m)
+        self._Q = _csc(Q, n, n)
+        self._B = _csc(B, m, n)
+        self._f = np.ascontiguousarray(f, dtype=np.float64)
+        self._g = np.ascontiguousarray(g, dtype=np.float64)
+        if self._f.shape != (n,) or self._g.shape != (m,):
+            raise ValueError("IPMProblem: f and g must have lengths n and m")
+        kinds = np.array([k.kind for k in K], dtype=np.int32)
+        params = np.array([k.param for k in K], dtype=np.float64)
+        sizes = np.array(s, dtype=np.int64)
+        self.K = list(K)
+        h = lib().mb_problem_create(n, m, _p(self._Q[0], ctypes.c_int64), _p(self._Q[1], ctypes.c_int64),
+                                    _p(self._Q[2], ctypes.c_double), _p(self._B[0], ctypes.c_int64),
+                                    _p(self._B[1], ctypes.c_int64), _p(self._B[2], ctypes.c_double),
+                                    _p(self._f, ctypes.c_double), _p(self._g, ctypes.c_double), float(mu),
+                                    len(K), _p(kinds, ctypes.c_int32), _p(params, ctypes.c_double),
+                                    _p(sizes, ctypes.c_int64), float(compress))
+        if not h:
+            raise _err("IPMProblem")
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.mb_problem_destroy(h)
+            self._h = None
+
+    def dims(self):
+        d = np.zeros(7, dtype=np.int64)
+        if lib().mb_problem_dims(self._h, _p(d, ctypes.c_int64)) != 0:
+            raise _err("dims")
+        return dict(nvtx=int(d[0]), nout=int(d[1]), narcB=int(d[2]), nbnzB=int(d[3]), narcQ=int(d[4]),
+                    nbnzQ=int(d[5]), n=int(d[6]))
+
+    def export(self):
+        """The constructed block-sparse problem (0-based), as plain arrays."""
+        d = self.dims()
+        N, U = d["nvtx"], d["nout"]
+        out = dict(P1=np.zeros(self.m, np.int64), P2=np.zeros(self.n, np.int64), kind=np.zeros(N, np.int32),
+                   B_xsrc=np.zeros(N + 1, np.int64), B_xcol=np.zeros(N + 1, np.int64),
+                   B_xrow=np.zeros(U + 1, np.int64), B_xblk=np.zeros(d["narcB"] + 1, np.int64),
+                   B_tgt=np.zeros(d["narcB"], np.int64), B_val=np.zeros(d["nbnzB"]),
+                   Q_xsrc=np.zeros(N + 1, np.int64), Q_xblk=np.zeros(d["narcQ"] + 1, np.int64),
+                   Q_tgt=np.zeros(d["narcQ"], np.int64), Q_val=np.zeros(d["nbnzQ"]), f=np.zeros(self.n),
+                   g=np.zeros(self.m))
+        i = ctypes.c_int64
+        r = lib().mb_problem_export(self._h, _p(out["P1"], i), _p(out["P2"], i), _p(out["kind"], ctypes.c_int32),
+                                    _p(out["B_xsrc"], i), _p(out["B_xcol"], i), _p(out["B_xrow"], i),
+                                    _p(out["B_xblk"], i), _p(out["B_tgt"], i), _p(out["B_val"], ctypes.c_double),
+                                    _p(out["Q_xsrc"], i), _p(out["Q_xblk"], i), _p(out["Q_tgt"], i),
+                                    _p(out["Q_val"], ctypes.c_double), _p(out["f"], ctypes.c_double),
+                                    _p(out["g"], ctypes.c_double))
+        if r != 0:
+            raise _err("export")
+        return out
+
+    def symbolic(self, amalgamate=True):
+        """The build's symbolic analysis (ordering, fronts) of this problem's KKT graph."""
+        d = np.zeros(3, dtype=np.int64)
+        if lib().mb_problem_symbolic_dims(self._h, int(amalgamate), _p(d, ctypes.c_int64)) != 0:
+            raise _err("symbolic")
+        N, F, S = (int(x) for x in d)
+        vperm = np.zeros(N, np.int64)
+        fv = np.zeros(F + 1, np.int64)
+        sp = np.zeros(F + 1, np.int64)
+        sep = np.zeros(max(S, 1), np.int64)
+        par = np.zeros(max(F, 1), np.int64)
+        fl = np.zeros(2)
+        i = ctypes.c_int64
+        if lib().mb_problem_symbolic(self._h, int(amalgamate), _p(vperm, i), _p(fv, i), _p(sp, i), _p(sep, i),
+                                     _p(par, i), _p(fl, ctypes.c_double)) != 0:
+            raise _err("symbolic")
+        return dict(vperm=vperm, front_vtx=fv, sep_ptr=sp, sep=sep[:S], parent=par[:F], exact_flops=fl[0],
+                    front_flops=fl[1])
+
+
+HIST_FIELDS = ["pres", "dres", "pobj", "dobj", "step", "piter", "ppass", "pstat", "citer", "cpass", "cstat",
+               "dmin", "dmax", "rho", "delta", "mu", "chi"]
+
+
+@dataclass
+class IPMResult:
+    """IPMResult (result/ipm.jl:1-15)."""
+    p: np.ndarray
+    d: np.ndarray
+    y: np.ndarray
+    status: IPMStatus
+    niter: int
+    nsolve: int
+    history: list = field(repr=False)
+    mu: float
+    pres: float
+    dres: float
+    pobj: float
+    dobj: float
+
+
+class IPMSolver:
+    """IPMSolver (solver/ipm.jl:1-28): device-resident iterate, scaling, factor."""
+
+    def __init__(self, prob, settings=None, p0=None, d0=None, y0=None):
+        settings = settings or IPMSettings()
+        self.prob = prob
+        self.settings = settings
+        cs = settings._c()
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (p0, d0, y0)]
+        h = lib().mb_solver_create(prob._h, ctypes.byref(cs), *(_p(a, ctypes.c_double) for a in arr))
+        if not h:
+            raise _err("init")
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.mb_solver_destroy(h)
+            self._h = None
+
+    def step(self):
+        r = lib().mb_solver_step(self._h)
+        if r < 0:
+            raise _err("step!")
+        return IPMStatus(r)
+
+    def solve(self):
+        r = lib().mb_solver_solve(self._h)
+        if r < 0:
+            raise _err("solve!")
+        return self.result()
+
+    def history(self):
+        n = int(lib().mb_solver_niter(self._h))
+        rows = np.zeros((max(n, 1), 17))
+        if lib().mb_solver_history(self._h, _p(rows, ctypes.c_double)) != 0:
+            raise _err("history")
+        out = []
+        for k in range(n):
+            row = dict(zip(HIST_FIELDS, rows[k].tolist()))
+            for key in ("piter", "ppass", "pstat", "citer", "cpass", "cstat"):
+                row[key] = int(row[key])
+            out.append(row)
+        return out
+
+    def stats(self):
+        st = np.zeros(7)
+        if lib().mb_solver_stats(self._h, _p(st, ctypes.c_double)) != 0:
+            raise _err("stats")
+        return dict(exact_flops=st[0], front_flops=st[1], nfront=int(st[2]), nlevel=int(st[3]),
+                    factor_ms_total=st[4], factor_calls=int(st[5]), factor_ms_last=st[6])
+
+    def kkt_factor(self, delta):
+        rho = ctypes.c_double(0.0)
+        r = lib().mb_solver_kkt_factor(self._h, float(delta), ctypes.byref(rho))
+        if r < 0:
+            raise _err("initkkt!")
+        return bool(r), rho.value
+
+    def dense_factor(self):
+        n = self.prob.n
+        L = np.zeros((n, n), order="F")
+        if lib().mb_solver_dense_factor(self._h, L.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) != 0:
+            raise _err("dense_factor")
+        return L
+
+    def result(self):
+        n, m = self.prob.n, self.prob.m
+        p, d, y, info = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(8)
+        if lib().mb_solver_result(self._h, _p(p, ctypes.c_double), _p(d, ctypes.c_double), _p(y, ctypes.c_double),
+                                  _p(info, ctypes.c_double)) != 0:
+            raise _err("result")
+        return IPMResult(p, d, y, IPMStatus(int(info[7])), int(info[5]), int(info[6]), self.history(),
+                         info[0], info[1], info[2], info[3], info[4])
+
+
+def init(prob, settings=None, p0=None, d0=None, y0=None):
+    """CommonSolve.init(prob, settings; p0, d0, y0) (solver/ipm.jl:485-491)."""
+    return IPMSolver(prob, settings, p0, d0, y0)
+
+
+def step(solver):
+    """step!(solver) (solver/ipm.jl:577-774)."""
+    return solver.step()
+
+
+def solve(prob, settings=None, p0=None, d0=None, y0=None):
+    """CommonSolve.solve(prob, settings; p0, d0, y0) (solver/ipm.jl:798-804)."""
+    return init(prob, settings, p0, d0, y0).solve()
+
+
+def probe_mfma_f64():
+    out = np.zeros(256)
+    if lib().mb_probe_mfma_f64(_p(out, ctypes.c_double)) != 0:
+        raise _err("probe")
+    return out
