@@ -112,6 +112,16 @@ int mb_solver_history(const mb_solver* s, double* rows);
  * total factorization ms, factorization calls, last factorization ms */
 int mb_solver_stats(const mb_solver* s, double* stats);
 
+/* reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429): new starting point, empty history;
+ * the symbolic factor and device buffers stay resident. */
+int mb_solver_reinit(mb_solver* s, const double* p0, const double* d0, const double* y0);
+
+/* Live kernel timing with HIP events on the solver's stream. mask bit k enables kind k:
+ * 0 factorization, 1 trailing update, 2 panel, 3 forward solve, 4 backward solve, 5 extend-add.
+ * out[4k..4k+3] = total ms, launches, algorithmic flops, algorithmic bytes of kind k. */
+int mb_solver_set_profile(mb_solver* s, int32_t mask);
+int mb_solver_profile(mb_solver* s, double* out);
+
 /* initkkt!(kkt, H; δ) on the solver's current H (uzawa.jl:73-115): returns 1 on success,
  * 0 on a failed factorization, negative on error; *rho receives the shift used. */
 int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);

@@ -272,6 +272,28 @@ int mb_solver_stats(const mb_solver* s, double* st) {
   });
 }
 
+int mb_solver_reinit(mb_solver* s, const double* p0, const double* d0, const double* y0) {
+  return guard([&]() {
+    s->S->reinit_public(p0, d0, y0);
+    return 0;
+  });
+}
+
+int mb_solver_set_profile(mb_solver* s, int32_t mask) {
+  return guard([&]() {
+    s->S->set_profile(mask);
+    return 0;
+  });
+}
+
+int mb_solver_profile(mb_solver* s, double* out) {
+  return guard([&]() {
+    s->S->prof_flush();
+    s->S->prof_get(out);
+    return 0;
+  });
+}
+
 int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho) {
   return guard([&]() { return s->S->kkt_factor(delta, rho); });
 }

@@ -760,11 +760,11 @@ __global__ void __launch_bounds__(256) k_update(const FrontDesc* fronts, const S
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
   }
-  // D layout: lane holds rows 4*(lane/16)+t, column lane%16 of each 16×16 tile
+  // D layout of v_mfma_f64_16x16x4f64: lane holds rows (lane/16) + 4t, column lane%16
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int t = 0; t < 4; ++t) {
-        int i = ri + qi + a * 16 + 4 * (lane >> 4) + t;
+        int i = ri + qi + a * 16 + (lane >> 4) + 4 * t;
         int j = rj + qj + b * 16 + (lane & 15);
         if (i < LD && j < LD && i >= j) F[(size_t)i + (size_t)j * LD] -= acc[a][b][t];
       }

@@ -193,6 +193,7 @@ Solver::~Solver() {
   if (hsc_) (void)hipHostFree(hsc_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -452,6 +453,9 @@ void Solver::build_schedule() {
   for (int f = 0; f < NF; ++f) H = std::max(H, h[f] + 1);
   levels_.assign(H, Level{});
   for (int f = 0; f < NF; ++f) levels_[h[f]].fronts.push_back(f);
+  std::vector<std::vector<int>> kids(NF);
+  for (int c = 0; c < NF; ++c)
+    if (fr_[c].parent >= 0) kids[fr_[c].parent].push_back(c);
   std::vector<StepDesc> steps;
   std::vector<int> ea_items, ea_tile0, solve_items;
   for (int l = 0; l < H; ++l) {
@@ -462,8 +466,8 @@ void Solver::build_schedule() {
     L.ea_off = (int)ea_items.size();
     int tiles = 0;
     for (int f : L.fronts)
-      for (int c = 0; c < NF; ++c)
-        if (fr_[c].parent == f && fr_[c].s > 0) {
+      for (int c : kids[f])
+        if (fr_[c].s > 0) {
           int T = (fr_[c].s + NB - 1) / NB;
           ea_items.push_back(c);
           ea_tile0.push_back(tiles);
@@ -471,16 +475,24 @@ void Solver::build_schedule() {
         }
     L.ea_n = (int)ea_items.size() - L.ea_off;
     L.ea_tiles = tiles;
+    for (int f : L.fronts) {
+      double r = fr_[f].r, s = fr_[f].s;
+      L.solve_bytes += (r * (r + 1) / 2 + s * r) * 8.0;
+    }
     int kmax = 0;
     for (int f : L.fronts) kmax = std::max(kmax, (fr_[f].r + NB - 1) / NB);
     for (int k = 0; k < kmax; ++k) {
       int pb = 0, ub = 0;
       std::vector<StepDesc> pan, upd;
+      double pfl = 0.0, ufl = 0.0;
       for (int f : L.fronts) {
         int kb = k * NB;
         if (kb >= fr_[f].r) continue;
         int w = std::min(NB, fr_[f].r - kb);
         int T = (fr_[f].ld - kb - w + NB - 1) / NB;
+        double rem = (double)(fr_[f].ld - kb - w), ww = (double)w;
+        pfl += ww * ww * ww / 3.0 + rem * ww * ww;
+        ufl += (rem * rem + rem) * ww;
         pan.push_back(StepDesc{f, kb, w, T, pb});
         pb += 1 + T;
         if (T > 0) {
@@ -488,6 +500,8 @@ void Solver::build_schedule() {
           ub += T * (T + 1) / 2;
         }
       }
+      L.pan_fl.push_back(pfl);
+      L.upd_fl.push_back(ufl);
       L.pan_off.push_back((int)steps.size());
       L.pan_n.push_back((int)pan.size());
       L.pan_blk.push_back(pb);
@@ -521,6 +535,7 @@ double Solver::front_flops() const {
 void Solver::sync_scalars() {
   HIPCHK(hipMemcpyAsync(hsc_, d_sc_.p, NSCALAR * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
+  prof_flush();
 }
 
 void Solver::set_scalar(int slot, double v) {
@@ -564,16 +579,21 @@ void Solver::hmul(int kx, int ky) {
 }
 
 void Solver::lsolve(int kv) {
-  for (const Level& L : levels_)
+  for (const Level& L : levels_) {
+    int ps = prof_begin(PROF_FSOLVE);
     launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, (int)L.fronts.size(),
                   d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv);
+    prof_end(ps, PROF_FSOLVE, 0.0, L.solve_bytes);
+  }
 }
 
 void Solver::usolve(int kv) {
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
+    int ps = prof_begin(PROF_BSOLVE);
     launch_bsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, (int)L.fronts.size(),
                   d_inst_.as<InstDev>(), kv);
+    prof_end(ps, PROF_BSOLVE, 0.0, L.solve_bytes);
   }
 }
 
@@ -581,13 +601,64 @@ void Solver::run_factor() {
   const FrontDesc* F = d_fr_.as<FrontDesc>();
   const StepDesc* S = d_steps_.as<StepDesc>();
   const InstDev* I = d_inst_.as<InstDev>();
+  int pf = prof_begin(PROF_FACTOR);
   for (const Level& L : levels_) {
+    int pe = prof_begin(PROF_EXTEND);
     launch_extend_add(st_, F, d_ea_items_.as<int>() + L.ea_off, d_ea_tile0_.as<int>() + L.ea_off, L.ea_n,
                       L.ea_tiles);
+    prof_end(pe, PROF_EXTEND, 0.0, 0.0);
     for (size_t k = 0; k < L.pan_off.size(); ++k) {
+      int pp = prof_begin(PROF_PANEL);
       launch_panel(st_, F, S + L.pan_off[k], L.pan_n[k], L.pan_blk[k], I);
+      prof_end(pp, PROF_PANEL, L.pan_fl[k], 0.0);
+      if (L.upd_blk[k] == 0) continue;
+      int pu = prof_begin(PROF_UPDATE);
       launch_update(st_, F, S + L.upd_off[k], L.upd_n[k], L.upd_blk[k], I);
+      prof_end(pu, PROF_UPDATE, L.upd_fl[k], 0.0);
     }
+  }
+  prof_end(pf, PROF_FACTOR, sym_.exact_flops, 0.0);
+}
+
+int Solver::prof_begin(int kind) {
+  if (!(prof_mask_ & (1 << kind))) return -1;
+  int slot = prof_next_++;
+  while ((int)evpool_.size() < 2 * prof_next_) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    evpool_.push_back(e);
+  }
+  HIPCHK(hipEventRecord(evpool_[2 * slot], st_));
+  return slot;
+}
+
+void Solver::prof_end(int slot, int kind, double flops, double bytes) {
+  if (slot < 0) return;
+  HIPCHK(hipEventRecord(evpool_[2 * slot + 1], st_));
+  pending_.push_back(ProfRec{slot, kind, flops, bytes});
+}
+
+void Solver::prof_flush() {
+  if (pending_.empty()) return;
+  HIPCHK(hipStreamSynchronize(st_));
+  for (const ProfRec& r : pending_) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, evpool_[2 * r.ev], evpool_[2 * r.ev + 1]));
+    prof_ms_[r.kind] += ms;
+    prof_n_[r.kind] += 1;
+    prof_fl_[r.kind] += r.flops;
+    prof_by_[r.kind] += r.bytes;
+  }
+  pending_.clear();
+  prof_next_ = 0;
+}
+
+void Solver::prof_get(double* out) const {
+  for (int k = 0; k < PROF_NKIND; ++k) {
+    out[4 * k] = prof_ms_[k];
+    out[4 * k + 1] = prof_n_[k];
+    out[4 * k + 2] = prof_fl_[k];
+    out[4 * k + 3] = prof_by_[k];
   }
 }
 

@@ -45,6 +45,8 @@ enum IPMStatus {
 };
 enum KKTStatus { KKT_CONTINUE = 0, KKT_SOLVED, KKT_STAGNATED, KKT_ITMAX, KKT_NUMERICAL_FAILURE, KKT_IRMAX };
 
+enum ProfKind { PROF_FACTOR = 0, PROF_UPDATE, PROF_PANEL, PROF_FSOLVE, PROF_BSOLVE, PROF_EXTEND, PROF_NKIND };
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -65,7 +67,9 @@ struct Level {
   int ea_off = 0, ea_n = 0, ea_tiles = 0;
   // panel / update steps: (offset into step table, count, blocks) per panel step k
   std::vector<int> pan_off, pan_n, pan_blk, upd_off, upd_n, upd_blk;
+  std::vector<double> pan_fl, upd_fl;  // algorithmic flops of each launch
   int items_off = 0;         // fronts of the level in the solve item table
+  double solve_bytes = 0.0;  // factor bytes one triangular sweep reads at this level
 };
 
 class Solver {
@@ -87,6 +91,11 @@ class Solver {
   void dense_factor(double* out);  // n×n lower factor (debug; small n)
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
   hipStream_t stream() const { return st_; }
+  // live per-kernel timing with HIP events on the solver's stream (kinds: PROF_*)
+  void set_profile(int mask) { prof_mask_ = mask; }
+  void prof_flush();
+  void prof_get(double* out) const;  // per kind: total ms, launches, algorithmic flops, bytes
+  void reinit_public(const double* p0, const double* d0, const double* y0) { reinit(p0, d0, y0); }
   int n() const { return (int)n_; }
   int m() const { return (int)m_; }
   int last_status() const { return status_; }
@@ -158,6 +167,20 @@ class Solver {
   double fac_ms_ = 0.0, fac_ms_tot_ = 0.0;
   int fac_calls_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  // profiling
+  struct ProfRec {
+    int ev;      // index of the event pair in the pool
+    int kind;
+    double flops, bytes;
+  };
+  int prof_mask_ = 0;
+  std::vector<hipEvent_t> evpool_;
+  std::vector<ProfRec> pending_;
+  double prof_ms_[PROF_NKIND] = {0}, prof_n_[PROF_NKIND] = {0}, prof_fl_[PROF_NKIND] = {0},
+         prof_by_[PROF_NKIND] = {0};
+  int prof_next_ = 0;
+  int prof_begin(int kind);
+  void prof_end(int slot, int kind, double flops, double bytes);
 };
 
 // restatement of equilibrate! (src/IPM/src/scaling/scaling.jl:122-184) on host BSMs

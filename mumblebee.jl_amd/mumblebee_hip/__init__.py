@@ -118,6 +118,9 @@ EXPORTS = {
     "mb_solver_niter": (ctypes.c_int64, [ctypes.c_void_p]),
     "mb_solver_history": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_solver_reinit": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP]),
+    "mb_solver_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mb_solver_profile": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
@@ -341,6 +344,28 @@ class IPMSolver:
             raise _err("stats")
         return dict(exact_flops=st[0], front_flops=st[1], nfront=int(st[2]), nlevel=int(st[3]),
                     factor_ms_total=st[4], factor_calls=int(st[5]), factor_ms_last=st[6])
+
+    def reinit(self, p0=None, d0=None, y0=None):
+        """reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429)."""
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (p0, d0, y0)]
+        if lib().mb_solver_reinit(self._h, *(_p(a, ctypes.c_double) for a in arr)) != 0:
+            raise _err("reinit!")
+
+    PROF_KINDS = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add"]
+
+    def set_profile(self, kinds):
+        mask = 0
+        for k in kinds:
+            mask |= 1 << self.PROF_KINDS.index(k)
+        if lib().mb_solver_set_profile(self._h, mask) != 0:
+            raise _err("profile")
+
+    def profile(self):
+        out = np.zeros(4 * len(self.PROF_KINDS))
+        if lib().mb_solver_profile(self._h, _p(out, ctypes.c_double)) != 0:
+            raise _err("profile")
+        return {k: dict(ms=out[4 * i], launches=int(out[4 * i + 1]), flops=out[4 * i + 2], bytes=out[4 * i + 3])
+                for i, k in enumerate(self.PROF_KINDS)}
 
     def kkt_factor(self, delta):
         rho = ctypes.c_double(0.0)

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 def test_mfma_f64_lane_layout():
     """v_mfma_f64_16x16x4f64: A[i][k] from lane i+16k, B[k][j] from lane j+16k, D[i][j] in
-    lane j+16(i//4), element i%4 — the layout the trailing-update kernel assumes."""
+    lane j+16(i%4), element i//4 — the layout the trailing-update kernel assumes."""
     out = mh.probe_mfma_f64().reshape(64, 4)
     A = np.array([[i + 100 * k for k in range(4)] for i in range(16)], dtype=float)
     B = np.array([[j + 1000 * k for j in range(16)] for k in range(4)], dtype=float)
@@ -18,7 +18,7 @@ def test_mfma_f64_lane_layout():
     exp = np.zeros((64, 4))
     for lane in range(64):
         for t in range(4):
-            exp[lane, t] = D[4 * (lane // 16) + t, lane % 16]
+            exp[lane, t] = D[(lane // 16) + 4 * t, lane % 16]
     if not np.array_equal(out, exp):
         print("actual lane outputs:\n", out)
     assert np.array_equal(out, exp)
@@ -33,10 +33,15 @@ def test_factor_matches_dense_cholesky(case):
     osolver = O.Solver(oracle_problem(d))
     delta = 0.37
     ok, rho = solver.kkt_factor(delta)
-    assert ok and rho == 0.0
     M = osolver.kkt.L + delta * osolver.Qd
-    if not np.all(np.linalg.eigvalsh(M) > 0):
-        pytest.skip("δQ + BᵀB singular for this case")
+    # the ρ retry (uzawa.jl:97-112) is taken exactly when the unshifted factorization fails;
+    # with Q = 0 (SOCP cases) BᵀB is singular, so it must be
+    ook, orho = osolver.kkt.init(osolver.Qd, delta)
+    assert ok and ook
+    assert (rho == 0.0) == (orho == 0.0)
+    if rho > 0:
+        assert abs(rho - orho) <= 1e-12 * orho
+        M = M + rho * np.eye(M.shape[0])
     Lref = np.linalg.cholesky(M)
     L = np.tril(solver.dense_factor())
     err = np.linalg.norm(L - Lref) / np.linalg.norm(Lref)

@@ -1,0 +1,96 @@
+"""Pins the oracle's IPM driver by solver-independent certificates, an exact LP solver
+(HiGHS via scipy.optimize.linprog) and forced failure paths."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.optimize import linprog
+
+from oracle import ipm as O
+from oracle.problem import CSC, make_problem
+from mumblebee_hip import generators as G
+from helpers import kkt_certificate, oracle_problem
+
+
+def _lp(seed, n=60, m=25):
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=0.2, random_state=rng, data_rvs=rng.standard_normal, format="csc")
+    x0 = rng.uniform(0.1, 1.0, n)
+    y0 = rng.standard_normal(m)
+    c = A.T @ y0 + rng.uniform(0.1, 1.0, n)
+    return dict(Q=sp.csc_matrix((n, n)), B=A, f=-c, g=A @ x0, K=[("pos",)], s=[n])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_lp_objective_matches_highs(seed):
+    d = _lp(seed)
+    r = O.solve(oracle_problem(d))
+    assert r.status == O.OPTIMAL
+    ref = linprog(-d["f"], A_eq=d["B"].toarray(), b_eq=d["g"], bounds=(0, None), method="highs")
+    assert ref.status == 0
+    # min ½pᵀQp − fᵀp with Q = 0 is min cᵀp, c = −f
+    assert abs(r.pobj - ref.fun) <= 1e-7 * max(1.0, abs(ref.fun))
+
+
+@pytest.mark.parametrize("gen,kw", [(G.qp, dict(n=150, m=70, seed=21)),
+                                    (G.socp, dict(n=300, m=140, ncone=12, seed=22, adens=0.03))])
+def test_certificates(gen, kw):
+    d = gen(**kw)
+    r = O.solve(oracle_problem(d))
+    assert r.status == O.OPTIMAL
+    rp, rd = kkt_certificate(d, r.p, r.d, r.y)
+    assert rp < 1e-7 and rd < 1e-7
+    # cone membership of p and d
+    j = 0
+    for k, n in zip(d["K"], d["s"]):
+        pv, dv = r.p[j:j + n], r.d[j:j + n]
+        if k[0] == "pos":
+            assert pv.min() > -1e-9 and dv.min() > -1e-9
+        elif k[0] == "soc":
+            assert pv[0] - np.linalg.norm(pv[1:]) > -1e-8 and dv[0] - np.linalg.norm(dv[1:]) > -1e-8
+        j += n
+    # objective = dual objective (zero gap)
+    assert abs(r.pobj - r.dobj) <= 1e-7 * max(1.0, abs(r.pobj))
+    # complementarity pᵀd ≈ 0 relative to the objective scale
+    assert abs(r.p @ r.d) <= 1e-6 * max(1.0, abs(r.pobj))
+
+
+def test_equality_qp_closed_form():
+    """A QP whose bounds are inactive at the optimum equals the closed-form KKT solution."""
+    rng = np.random.default_rng(7)
+    n, m = 20, 6
+    M = rng.standard_normal((n, n))
+    Q = M @ M.T + n * np.eye(n)
+    A = rng.standard_normal((m, n))
+    xs = rng.uniform(1.0, 2.0, n)
+    ys = rng.standard_normal(m)
+    # choose f so that (xs, ys) satisfies Q x − f − Aᵀy = 0 with x > 0 (d = 0)
+    f = Q @ xs - A.T @ ys
+    d = dict(Q=sp.csc_matrix(Q), B=sp.csc_matrix(A), f=f, g=A @ xs, K=[("pos",)], s=[n])
+    r = O.solve(oracle_problem(d))
+    assert r.status == O.OPTIMAL
+    np.testing.assert_allclose(r.p, xs, rtol=1e-6)
+    np.testing.assert_allclose(r.y, ys, rtol=1e-5, atol=1e-6)
+
+
+def test_iteration_limit():
+    d = G.qp(n=80, m=30, seed=3)
+    r = O.solve(oracle_problem(d), O.Settings(max_iter=1))
+    assert r.status in (O.ITERATION_LIMIT, O.NEAR_OPTIMAL)
+    assert r.niter == 2
+
+
+def test_primal_infeasible():
+    n = 6
+    A = np.ones((1, n))
+    d = dict(Q=sp.csc_matrix((n, n)), B=sp.csc_matrix(A), f=-np.ones(n), g=np.array([-1.0]), K=[("pos",)], s=[n])
+    r = O.solve(oracle_problem(d))
+    assert r.status in (O.PRIMAL_INFEASIBLE, O.NEAR_PRIMAL_INFEASIBLE)
+
+
+def test_dual_infeasible():
+    n = 4
+    A = np.array([[1.0, -1.0, 0.0, 0.0]])
+    d = dict(Q=sp.csc_matrix((n, n)), B=sp.csc_matrix(A), f=np.array([1.0, 1.0, 0.0, 0.0]) * 1.0,
+             g=np.array([0.0]), K=[("pos",)], s=[n])
+    r = O.solve(oracle_problem(d))
+    assert r.status in (O.DUAL_INFEASIBLE, O.NEAR_DUAL_INFEASIBLE)
