@@ -56,8 +56,9 @@ typedef struct mb_settings {
   double aug_tol;
   double infeas_abs;
   double infeas_rel;
-  int32_t amalgamate; /* relaxed supernode amalgamation (1 default) */
-  int32_t precision;  /* 0: FP64 factor */
+  int32_t amalgamate;   /* relaxed supernode amalgamation (1 default) */
+  int32_t precision;    /* 0: FP64 factor */
+  int32_t big_front_ld; /* fronts of this order or larger take the persistent kernels (512) */
 } mb_settings;
 
 /* Fill the reference defaults. */
@@ -117,10 +118,12 @@ int mb_solver_stats(const mb_solver* s, double* stats);
 int mb_solver_reinit(mb_solver* s, const double* p0, const double* d0, const double* y0);
 
 /* Live kernel timing with HIP events on the solver's stream. mask bit k enables kind k:
- * 0 factorization, 1 trailing update, 2 panel, 3 forward solve, 4 backward solve, 5 extend-add.
- * out[4k..4k+3] = total ms, launches, algorithmic flops, algorithmic bytes of kind k. */
+ * 0 factorization, 1 trailing update, 2 panel, 3 forward solve, 4 backward solve, 5 extend-add,
+ * 6 persistent large-front factorization.
+ * out[4k..4k+3] = total ms, launches, algorithmic flops, algorithmic bytes of kind k; at most
+ * len doubles are written; returns the number of kinds. */
 int mb_solver_set_profile(mb_solver* s, int32_t mask);
-int mb_solver_profile(mb_solver* s, double* out);
+int mb_solver_profile(mb_solver* s, double* out, int32_t len);
 
 /* initkkt!(kkt, H; δ) on the solver's current H (uzawa.jl:73-115): returns 1 on success,
  * 0 on a failed factorization, negative on error; *rho receives the shift used. */

@@ -82,6 +82,7 @@ void mb_settings_default(mb_settings* s) {
   s->infeas_rel = d.infeas_rel;
   s->amalgamate = d.amalgamate;
   s->precision = 0;
+  s->big_front_ld = d.big_front_ld;
 }
 
 mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, const int64_t* Q_rowval,
@@ -221,6 +222,7 @@ mb_solver* mb_solver_create(const mb_problem* p, const mb_settings* s, const dou
       st.infeas_abs = s->infeas_abs;
       st.infeas_rel = s->infeas_rel;
       st.amalgamate = s->amalgamate;
+      st.big_front_ld = s->big_front_ld > 0 ? s->big_front_ld : mb::BIG_LD;
       if (s->precision != 0) throw std::invalid_argument("precision mode not available");
     }
     auto h = std::make_unique<mb_solver>();
@@ -286,11 +288,13 @@ int mb_solver_set_profile(mb_solver* s, int32_t mask) {
   });
 }
 
-int mb_solver_profile(mb_solver* s, double* out) {
+int mb_solver_profile(mb_solver* s, double* out, int32_t len) {
   return guard([&]() {
     s->S->prof_flush();
-    s->S->prof_get(out);
-    return 0;
+    double tmp[4 * mb::PROF_NKIND];
+    s->S->prof_get(tmp);
+    for (int k = 0; k < std::min<int>(len, 4 * mb::PROF_NKIND); ++k) out[k] = tmp[k];
+    return (int)mb::PROF_NKIND;
   });
 }
 

@@ -242,21 +242,35 @@ Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<
       if (fpar[f] >= 0) fch[fpar[f]].push_back(f);
     std::vector<double> z(F, 0.0);
     std::vector<i64> rr(r);
+    std::vector<std::vector<i64>> kept(F);  // children that stay separate fronts
     for (i64 f = 0; f < F; ++f) {
       std::vector<i64> mem;
-      for (i64 c : fch[f]) {
+      // worklist: original children, then the kept children of every child merged into f
+      std::vector<i64> queue(fch[f].begin(), fch[f].end());
+      for (size_t qi = 0; qi < queue.size(); ++qi) {
+        i64 c = queue[qi];
         double rc = (double)rr[c], rf = (double)rr[f];
         double rt = rc + rf, st = (double)s[f];
         double zz = z[c] + z[f] + rc * (rf + st - (double)s[c]);
         double tot = rt * (rt + 1.0) / 2.0 + rt * st;
         bool ok = rt <= 4 || (rt <= 16 && zz <= 0.8 * tot) || (rt <= 48 && zz <= 0.1 * tot) ||
                   zz <= 0.05 * tot;
+        // GPU rule: a child of at most one 64-column panel whose separator is at least half the
+        // parent's order merges: its own update is memory bound, and merging saves its frontal
+        // matrix, its extend-add and a launch, for at most ~4x the child's own flops
+        ok = ok || (rc <= 64 && 2.0 * (double)s[c] >= rf + st);
+        // and any merge that costs at most 2 % more flops than the two fronts apart
+        auto fl = [](double r, double s) { return r * r * r / 3.0 + s * r * r + s * s * r; };
+        ok = ok || fl(rt, st) <= 1.02 * (fl(rc, (double)s[c]) + fl(rf, st));
         if (ok) {
           rr[f] = (i64)rt;
           z[f] = zz;
           rep[c] = f;
           mem.insert(mem.end(), members[c].begin(), members[c].end());
           members[c].clear();
+          queue.insert(queue.end(), kept[c].begin(), kept[c].end());
+        } else {
+          kept[f].push_back(c);
         }
       }
       mem.push_back(f);

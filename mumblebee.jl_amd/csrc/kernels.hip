@@ -317,35 +317,53 @@ __global__ void k_cone_hnorm2(const InstDev* insts, const ConeDesc* cones, int n
   out[c] = s;
 }
 
-// per-instance sum of out[c] into sc[slot]
+// per-instance sum of out[c] into sc[slot] (one block per instance, fixed-order tree)
 __global__ void k_cone_sum(const InstDev* insts, const int* cone_ofs, const double* out, int slot) {
+  __shared__ double red[TILE / 64];
   int inst = blockIdx.x;
   const InstDev& I = insts[inst];
-  if (threadIdx.x != 0 || I.sc[S_ACTIVE] == 0.0) return;
   double s = 0.0;
-  for (int c = cone_ofs[inst]; c < cone_ofs[inst + 1]; ++c) s += out[c];
-  I.sc[slot] = s;
+  for (int c = cone_ofs[inst] + (int)threadIdx.x; c < cone_ofs[inst + 1]; c += blockDim.x) s += out[c];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0 && I.sc[S_ACTIVE] != 0.0) I.sc[slot] = s;
+}
+
+__device__ __forceinline__ double block_min(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double m = v;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmin(m, red[k]);
+  return m;
 }
 
 // per-instance reduction of cone outputs: S_FLAGOK = all(flag), S_SPSD = Σ spsd (mode 0);
 // S_TAU = min(step) (mode 1, out[c] per cone)
 __global__ void k_cone_reduce(const InstDev* insts, const int* cone_ofs, const double* out, int mode) {
+  __shared__ double red[TILE / 64];
   int inst = blockIdx.x;
   const InstDev& I = insts[inst];
-  if (threadIdx.x != 0 || I.sc[S_ACTIVE] == 0.0) return;
   int a = cone_ofs[inst], b = cone_ofs[inst + 1];
   if (mode == 0) {
-    double ok = 1.0, s = 0.0;
-    for (int c = a; c < b; ++c) {
-      ok = (ok != 0.0 && out[2 * c] != 0.0) ? 1.0 : 0.0;
+    double bad = 0.0, s = 0.0;
+    for (int c = a + (int)threadIdx.x; c < b; c += blockDim.x) {
+      bad += out[2 * c] == 0.0 ? 1.0 : 0.0;
       s += out[2 * c + 1];
     }
-    I.sc[S_FLAGOK] = ok;
-    I.sc[S_SPSD] = s;
+    s = block_sum(s, red);
+    bad = block_sum(bad, red);
+    if (threadIdx.x == 0 && I.sc[S_ACTIVE] != 0.0) {
+      I.sc[S_FLAGOK] = bad == 0.0 ? 1.0 : 0.0;
+      I.sc[S_SPSD] = s;
+    }
   } else {
     double tau = 1.0;
-    for (int c = a; c < b; ++c) tau = fmin(tau, out[c]);
-    I.sc[S_TAU] = tau;
+    for (int c = a + (int)threadIdx.x; c < b; c += blockDim.x) tau = fmin(tau, out[c]);
+    tau = block_min(tau, red);
+    if (threadIdx.x == 0 && I.sc[S_ACTIVE] != 0.0) I.sc[S_TAU] = tau;
   }
 }
 
@@ -960,12 +978,12 @@ void launch_cone_matvec(hipStream_t st, const InstDev* I, const ConeDesc* C, int
 void launch_cone_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
                        int ninst, double* out) {
   if (nc) k_cone_scale<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
-  k_cone_reduce<<<ninst, 64, 0, st>>>(I, cone_ofs, out, 0);
+  k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 0);
 }
 void launch_cone_hnorm2(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
                         int ninst, double* out, int slot) {
   if (nc) k_cone_hnorm2<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
-  k_cone_sum<<<ninst, 64, 0, st>>>(I, cone_ofs, out, slot);
+  k_cone_sum<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, slot);
 }
 void launch_cone_corr0(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kr) {
   if (nc) k_cone_corr0<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kr);
@@ -976,7 +994,7 @@ void launch_cone_corr(hipStream_t st, const InstDev* I, const ConeDesc* C, int n
 void launch_cone_maxstep(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
                          int ninst, int kdp, int kdd, double step_frac, double* out) {
   if (nc) k_cone_maxstep<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kdp, kdd, step_frac, out);
-  k_cone_reduce<<<ninst, 64, 0, st>>>(I, cone_ofs, out, 1);
+  k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 1);
 }
 void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx) {
   if (nc) k_zerofree<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx);

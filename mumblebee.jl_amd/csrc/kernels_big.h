@@ -1,0 +1,34 @@
+// Persistent tile-DAG factorization and pipelined triangular solves for large fronts.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace mb {
+
+constexpr int BIG_LD = 512;  // fronts with ld ≥ BIG_LD take the persistent kernels
+
+struct BigFront {
+  double* F;            // frontal storage (ld × ld, column-major)
+  int ld, r, s;
+  int T, Tr;            // tiles (res tiles first, then separator tiles from column r)
+  const int* tb;        // T+1 tile boundaries
+  int* state;           // T(T+1)/2 tile progress counters
+  int* ticket;          // [0] task ticket, [1] abort flag
+  int* abortf;          // = ticket + 1
+  const int4* tasks;    // (type, i, j, k): 0 POTRF, 1 TRSM, 2 UPDATE
+  int ntask;
+  int nstate;
+  int* sflag;           // T solve flags (epoch-valued)
+  long long* sticket;   // 64-bit monotone ticket of the solve kernels
+  int inst;
+};
+
+void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid);
+void launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
+                       const InstDev* I, int kv, int epoch, long long base);
+void launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
+                       int epoch, long long base);
+
+}  // namespace mb

@@ -458,10 +458,83 @@ void Solver::build_schedule() {
     if (fr_[c].parent >= 0) kids[fr_[c].parent].push_back(c);
   std::vector<StepDesc> steps;
   std::vector<int> ea_items, ea_tile0, solve_items;
+  // large fronts: persistent tile-DAG factorization and pipelined solves
+  int nbig = 0;
+  for (int f = 0; f < NF; ++f)
+    if (fr_[f].ld >= set_.big_front_ld) ++nbig;
+  d_bigtick_.alloc(sizeof(int) * 2 * std::max(nbig, 1));
+  d_bigstick_.alloc(sizeof(long long) * std::max(nbig, 1));
+  HIPCHK(hipMemset(d_bigtick_.p, 0, d_bigtick_.bytes));
+  HIPCHK(hipMemset(d_bigstick_.p, 0, d_bigstick_.bytes));
+  {
+    int dev = 0, ncu = 256;
+    HIPCHK(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 256;
+    dag_grid_ = ncu;  // the DAG kernel runs at one workgroup per CU (register bound)
+  }
   for (int l = 0; l < H; ++l) {
     Level& L = levels_[l];
+    std::vector<int> small;
+    for (int f : L.fronts) {
+      if (fr_[f].ld < set_.big_front_ld) {
+        small.push_back(f);
+        continue;
+      }
+      auto bh = std::make_unique<BigHost>();
+      const FrontDesc& Fd = fr_[f];
+      std::vector<int> tb;
+      for (int c = 0; c < Fd.r; c += NB) tb.push_back(c);
+      for (int c = Fd.r; c < Fd.ld; c += NB) tb.push_back(c);
+      tb.push_back(Fd.ld);
+      int T = (int)tb.size() - 1, Tr = (Fd.r + NB - 1) / NB;
+      std::vector<int4> tasks;
+      auto tri = [](int i, int j) { return i * (i + 1) / 2 + j; };
+      (void)tri;
+      if (Tr > 0) {
+        tasks.push_back(int4{0, 0, 0, 0});
+        for (int i = 1; i < T; ++i) tasks.push_back(int4{1, i, 0, 0});
+      }
+      for (int k = 0; k < Tr; ++k) {
+        if (k + 1 < T)
+          for (int i = k + 1; i < T; ++i) tasks.push_back(int4{2, i, k + 1, k});
+        if (k + 1 < Tr) {
+          tasks.push_back(int4{0, k + 1, k + 1, k + 1});
+          for (int i = k + 2; i < T; ++i) tasks.push_back(int4{1, i, 0, k + 1});
+        }
+        for (int j = k + 2; j < T; ++j)
+          for (int i = j; i < T; ++i) tasks.push_back(int4{2, i, j, k});
+      }
+      bh->tb.upload(tb, st_);
+      bh->tasks.upload(tasks, st_);
+      bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
+      bh->sflag.alloc(sizeof(int) * std::max(1, T));
+      HIPCHK(hipMemset(bh->sflag.p, 0, bh->sflag.bytes));
+      BigFront& B = bh->B;
+      B.F = Fd.F;
+      B.ld = Fd.ld;
+      B.r = Fd.r;
+      B.s = Fd.s;
+      B.T = T;
+      B.Tr = Tr;
+      B.tb = bh->tb.as<int>();
+      B.state = bh->state.as<int>();
+      B.nstate = T * (T + 1) / 2;
+      B.ticket = d_bigtick_.as<int>() + 2 * big_.size();
+      B.abortf = B.ticket + 1;
+      B.tasks = bh->tasks.as<int4>();
+      B.ntask = (int)tasks.size();
+      B.sflag = bh->sflag.as<int>();
+      B.sticket = d_bigstick_.as<long long>() + big_.size();
+      B.inst = Fd.inst;
+      bh->front = f;
+      double r = Fd.r, s = Fd.s;
+      L.big_flops += r * r * r / 3.0 + s * r * r + s * s * r;
+      L.big.push_back((int)big_.size());
+      big_.push_back(std::move(bh));
+    }
     L.items_off = (int)solve_items.size();
-    solve_items.insert(solve_items.end(), L.fronts.begin(), L.fronts.end());
+    L.nsmall = (int)small.size();
+    solve_items.insert(solve_items.end(), small.begin(), small.end());
     // extend-add items: children of this level's fronts
     L.ea_off = (int)ea_items.size();
     int tiles = 0;
@@ -480,12 +553,12 @@ void Solver::build_schedule() {
       L.solve_bytes += (r * (r + 1) / 2 + s * r) * 8.0;
     }
     int kmax = 0;
-    for (int f : L.fronts) kmax = std::max(kmax, (fr_[f].r + NB - 1) / NB);
+    for (int f : small) kmax = std::max(kmax, (fr_[f].r + NB - 1) / NB);
     for (int k = 0; k < kmax; ++k) {
       int pb = 0, ub = 0;
       std::vector<StepDesc> pan, upd;
       double pfl = 0.0, ufl = 0.0;
-      for (int f : L.fronts) {
+      for (int f : small) {
         int kb = k * NB;
         if (kb >= fr_[f].r) continue;
         int w = std::min(NB, fr_[f].r - kb);
@@ -581,8 +654,15 @@ void Solver::hmul(int kx, int ky) {
 void Solver::lsolve(int kv) {
   for (const Level& L : levels_) {
     int ps = prof_begin(PROF_FSOLVE);
-    launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, (int)L.fronts.size(),
+    launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, L.nsmall,
                   d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv);
+    for (int b : L.big) {
+      BigHost& bh = *big_[b];
+      ++epoch_;
+      launch_big_fsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_childlist_.as<int>(), d_inst_.as<InstDev>(),
+                        kv, epoch_, bh.sbase);
+      bh.sbase += bh.B.T;
+    }
     prof_end(ps, PROF_FSOLVE, 0.0, L.solve_bytes);
   }
 }
@@ -591,10 +671,25 @@ void Solver::usolve(int kv) {
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
     int ps = prof_begin(PROF_BSOLVE);
-    launch_bsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, (int)L.fronts.size(),
+    launch_bsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, L.nsmall,
                   d_inst_.as<InstDev>(), kv);
+    for (int b : L.big) {
+      BigHost& bh = *big_[b];
+      ++epoch_;
+      launch_big_bsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_inst_.as<InstDev>(), kv, epoch_, bh.sbase);
+      bh.sbase += bh.B.Tr;
+    }
     prof_end(ps, PROF_BSOLVE, 0.0, L.solve_bytes);
   }
+}
+
+// a persistent kernel that hit its wait limit raised its abort flag: fail loudly
+void Solver::check_abort() {
+  if (big_.empty()) return;
+  std::vector<int> t(2 * big_.size());
+  HIPCHK(hipMemcpy(t.data(), d_bigtick_.p, t.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t b = 0; b < big_.size(); ++b)
+    if (t[2 * b + 1] != 0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
 }
 
 void Solver::run_factor() {
@@ -615,6 +710,11 @@ void Solver::run_factor() {
       int pu = prof_begin(PROF_UPDATE);
       launch_update(st_, F, S + L.upd_off[k], L.upd_n[k], L.upd_blk[k], I);
       prof_end(pu, PROF_UPDATE, L.upd_fl[k], 0.0);
+    }
+    if (!L.big.empty()) {
+      int pd = prof_begin(PROF_DAG);
+      for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_);
+      prof_end(pd, PROF_DAG, L.big_flops, 0.0);
     }
   }
   prof_end(pf, PROF_FACTOR, sym_.exact_flops, 0.0);
@@ -678,6 +778,7 @@ bool Solver::factor(double delta, double& rho) {
   run_factor();
   HIPCHK(hipEventRecord(ev1_, st_));
   sync_scalars();
+  check_abort();
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
   fac_ms_ = ms;
@@ -698,6 +799,7 @@ bool Solver::factor(double delta, double& rho) {
   launch_add_diag(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), I);
   run_factor();
   sync_scalars();
+  check_abort();
   return hsc_[S_FAIL] == 0.0;
 }
 
@@ -936,6 +1038,7 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   axpby(1, MV_KG, ky, ky, KS(-1.0, S_ALPHA), K(1.0));  // y −= α δg
   int cgstat = 1;
   int ncg = cg(kx, MV_KY, MV_KG, gtol, cgmax, stops, cgstat);
+  check_abort();
   axpbypcz(1, MV_KY, MV_KG, ky, ky, KS(1.0, S_ALPHA), KS(1.0, S_ALPHA), K(1.0));  // y += α δy + α δg
   KKTOut o;
   o.ncg = ncg;

@@ -12,6 +12,7 @@
 #include "device.h"
 #include "host.h"
 #include "kernels.h"
+#include "kernels_big.h"
 
 namespace mb {
 
@@ -31,6 +32,7 @@ struct Settings {
   double infeas_abs = 1e-8;
   double infeas_rel = 1e-8;
   int amalgamate = 1;
+  int big_front_ld = BIG_LD;  // fronts with ld ≥ this take the persistent kernels
 };
 
 struct HistRow {
@@ -45,7 +47,9 @@ enum IPMStatus {
 };
 enum KKTStatus { KKT_CONTINUE = 0, KKT_SOLVED, KKT_STAGNATED, KKT_ITMAX, KKT_NUMERICAL_FAILURE, KKT_IRMAX };
 
-enum ProfKind { PROF_FACTOR = 0, PROF_UPDATE, PROF_PANEL, PROF_FSOLVE, PROF_BSOLVE, PROF_EXTEND, PROF_NKIND };
+enum ProfKind {
+  PROF_FACTOR = 0, PROF_UPDATE, PROF_PANEL, PROF_FSOLVE, PROF_BSOLVE, PROF_EXTEND, PROF_DAG, PROF_NKIND
+};
 
 struct DevBuf {
   void* p = nullptr;
@@ -68,8 +72,11 @@ struct Level {
   // panel / update steps: (offset into step table, count, blocks) per panel step k
   std::vector<int> pan_off, pan_n, pan_blk, upd_off, upd_n, upd_blk;
   std::vector<double> pan_fl, upd_fl;  // algorithmic flops of each launch
-  int items_off = 0;         // fronts of the level in the solve item table
+  int items_off = 0;         // small fronts of the level in the solve item table
+  int nsmall = 0;
+  std::vector<int> big;      // indices into Solver::big_ of this level's large fronts
   double solve_bytes = 0.0;  // factor bytes one triangular sweep reads at this level
+  double big_flops = 0.0;    // algorithmic flops of this level's large fronts
 };
 
 class Solver {
@@ -181,6 +188,18 @@ class Solver {
   int prof_next_ = 0;
   int prof_begin(int kind);
   void prof_end(int slot, int kind, double flops, double bytes);
+  // large fronts
+  struct BigHost {
+    BigFront B;
+    int front;
+    long long sbase = 0;
+    DevBuf tb, state, tasks, sflag;
+  };
+  std::vector<std::unique_ptr<BigHost>> big_;
+  DevBuf d_bigtick_, d_bigstick_;
+  int epoch_ = 0;
+  int dag_grid_ = 512;
+  void check_abort();
 };
 
 // restatement of equilibrate! (src/IPM/src/scaling/scaling.jl:122-184) on host BSMs

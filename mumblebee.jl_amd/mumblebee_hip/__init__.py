@@ -13,9 +13,11 @@ and error behaviour) over the C ABI in include/mumblebee_hip.h:
 All numerics run in the HIP library; there is no CPU fallback. Loading fails loudly when the
 library has not been built.
 """
+import atexit
 import ctypes
 import enum
 import os
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -23,6 +25,17 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libmumblebee_hip.so")
 _lib = None
+_live_solvers = weakref.WeakSet()
+_live_problems = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    """Free device state while the HIP runtime is still up (before its own exit handlers)."""
+    for s in list(_live_solvers):
+        s.close()
+    for p in list(_live_problems):
+        p.close()
 
 
 class IPMStatus(enum.IntEnum):
@@ -94,7 +107,8 @@ class _Settings(ctypes.Structure):
                 ("scale_max_iter", ctypes.c_int32), ("refine_max_iter", ctypes.c_int32),
                 ("newton_max_iter", ctypes.c_int32), ("aug_tol", ctypes.c_double),
                 ("infeas_abs", ctypes.c_double), ("infeas_rel", ctypes.c_double),
-                ("amalgamate", ctypes.c_int32), ("precision", ctypes.c_int32)]
+                ("amalgamate", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("big_front_ld", ctypes.c_int32)]
 
 
 EXPORTS = {
@@ -120,7 +134,7 @@ EXPORTS = {
     "mb_solver_stats": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_solver_reinit": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP]),
     "mb_solver_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
-    "mb_solver_profile": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_solver_profile": (ctypes.c_int, [ctypes.c_void_p, _DP, ctypes.c_int32]),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
@@ -180,6 +194,7 @@ class IPMSettings:
     infeas_rel: float = 1e-8
     amalgamate: int = 1
     precision: int = 0
+    big_front_ld: int = 512
 
     def _c(self):
         s = _Settings()
@@ -216,12 +231,16 @@ class IPMProblem:
         if not h:
             raise _err("IPMProblem")
         self._h = ctypes.c_void_p(h)
+        _live_problems.add(self)
 
-    def __del__(self):
+    def close(self):
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None:
             _lib.mb_problem_destroy(h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        self.close()
 
     def dims(self):
         d = np.zeros(7, dtype=np.int64)
@@ -306,12 +325,16 @@ class IPMSolver:
         if not h:
             raise _err("init")
         self._h = ctypes.c_void_p(h)
+        _live_solvers.add(self)
 
-    def __del__(self):
+    def close(self):
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None:
             _lib.mb_solver_destroy(h)
-            self._h = None
+        self._h = None
+
+    def __del__(self):
+        self.close()
 
     def step(self):
         r = lib().mb_solver_step(self._h)
@@ -351,7 +374,7 @@ class IPMSolver:
         if lib().mb_solver_reinit(self._h, *(_p(a, ctypes.c_double) for a in arr)) != 0:
             raise _err("reinit!")
 
-    PROF_KINDS = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add"]
+    PROF_KINDS = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
 
     def set_profile(self, kinds):
         mask = 0
@@ -362,7 +385,7 @@ class IPMSolver:
 
     def profile(self):
         out = np.zeros(4 * len(self.PROF_KINDS))
-        if lib().mb_solver_profile(self._h, _p(out, ctypes.c_double)) != 0:
+        if lib().mb_solver_profile(self._h, _p(out, ctypes.c_double), len(out)) < 0:
             raise _err("profile")
         return {k: dict(ms=out[4 * i], launches=int(out[4 * i + 1]), flops=out[4 * i + 2], bytes=out[4 * i + 3])
                 for i, k in enumerate(self.PROF_KINDS)}

@@ -146,7 +146,9 @@ def analyse(weights, adj, amalgamate=True):
     """Ordering → elimination tree → postorder → fundamental supernodes → relaxed
     amalgamation (a child front merges into its parent when the explicit zeros this adds stay
     below 80 % / 10 % / 5 % of the merged front for ≤16 / ≤48 / larger column counts, or
-    always when the merged front has ≤4 columns)."""
+    always when the merged front has ≤4 columns, or when the child has ≤64 columns and its
+    separator is at least half the parent's order, or when the merged front costs at most 2 %
+    more flops than the two apart)."""
     N = len(weights)
     order, struct = mindegree(weights, adj)
     pos = np.empty(N, dtype=np.int64)
@@ -214,21 +216,35 @@ def analyse(weights, adj, amalgamate=True):
                 fch[fpar[f]].append(f)
         z = [0.0] * nf
         rr = list(r)
+        kept = [[] for _ in range(nf)]  # children that stay separate fronts
         for f in range(nf):
             mem = []
-            for c in fch[f]:
+            # worklist: original children, then the kept children of every child merged into f
+            queue = list(fch[f])
+            qi = 0
+            while qi < len(queue):
+                c = queue[qi]
+                qi += 1
                 rc, rf = float(rr[c]), float(rr[f])
                 rt, st = rc + rf, float(s[f])
                 zz = z[c] + z[f] + rc * (rf + st - float(s[c]))
                 tot = rt * (rt + 1.0) / 2.0 + rt * st
                 ok = (rt <= 4 or (rt <= 16 and zz <= 0.8 * tot) or (rt <= 48 and zz <= 0.1 * tot)
                       or zz <= 0.05 * tot)
+                # the GPU rule: a child of at most one panel with a large separator merges
+                ok = ok or (rc <= 64 and 2.0 * float(s[c]) >= rf + st)
+                # and any merge that costs at most 2 % more flops than the two fronts apart
+                fl = lambda a, b: a * a * a / 3.0 + b * a * a + b * b * a  # noqa: E731
+                ok = ok or fl(rt, st) <= 1.02 * (fl(rc, float(s[c])) + fl(rf, st))
                 if ok:
                     rr[f] = int(rt)
                     z[f] = zz
                     rep[c] = f
                     mem.extend(members[c])
                     members[c] = []
+                    queue.extend(kept[c])
+                else:
+                    kept[f].append(c)
             mem.append(f)
             members[f] = mem
 
