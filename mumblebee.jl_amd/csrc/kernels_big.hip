@@ -24,11 +24,21 @@ namespace mb {
 #define LDT (NB + 1)
 constexpr long SPIN_LIMIT = 1L << 24;
 
-__device__ __forceinline__ int ld_acquire(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Cross-workgroup data moves through agent-scope relaxed atomics: on this part they are
+// coherent loads/stores at the device coherence point (sc1), so no L2 writeback or
+// invalidate is needed. A flag is stored only after the producer's coherent stores completed
+// (s_waitcnt), and consumers read the data with coherent loads after observing the flag.
+__device__ __forceinline__ int ld_flag(const int* p) {
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_flag(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double cload(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void cstore(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // thread 0 waits until *p >= v; returns false on abort / spin limit (then sets abort)
@@ -37,13 +47,13 @@ __device__ __forceinline__ bool wait_geq(const int* p, int v, int* abortf) {
   if (threadIdx.x == 0) {
     long spins = 0;
     int good = 1;
-    while (ld_acquire(p) < v) {
-      if (ld_acquire(abortf) != 0) {
+    while (ld_flag(p) < v) {
+      if (ld_flag(abortf) != 0) {
         good = 0;
         break;
       }
       if (++spins > SPIN_LIMIT) {
-        st_release(abortf, 1);
+        st_flag(abortf, 1);
         good = 0;
         break;
       }
@@ -52,17 +62,16 @@ __device__ __forceinline__ bool wait_geq(const int* p, int v, int* abortf) {
     ok = good;
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   bool r = ok != 0;
   __syncthreads();  // nobody may overwrite ok before every thread has read it
   return r;
 }
 
-// all threads' prior global stores are made visible, then thread 0 publishes
+// every thread's coherent stores have completed, then thread 0 publishes
 __device__ __forceinline__ void publish(int* p, int v) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (threadIdx.x == 0) st_release(p, v);
+  if (threadIdx.x == 0) st_flag(p, v);
 }
 
 __device__ __forceinline__ int tri_index(int i, int j) { return i * (i + 1) / 2 + j; }
@@ -75,7 +84,7 @@ __device__ __forceinline__ int tri_index(int i, int j) { return i * (i + 1) / 2 
 __device__ __forceinline__ void load_tile(double* S, const double* F, int LD, int r0, int c0, int h, int w) {
   for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
     int i = t % NB, j = t / NB;
-    S[i + j * LDT] = (i < h && j < w) ? F[(size_t)(r0 + i) + (size_t)(c0 + j) * LD] : 0.0;
+    S[i + j * LDT] = (i < h && j < w) ? cload(&F[(size_t)(r0 + i) + (size_t)(c0 + j) * LD]) : 0.0;
   }
 }
 
@@ -96,7 +105,7 @@ __device__ __forceinline__ bool potrf64_wave0(double* F, int LD, int c0, int w) 
     double row[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k)
-      row[k] = (i < w && k < w && k <= i) ? F[(size_t)(c0 + i) + (size_t)(c0 + k) * LD] : (i == k ? 1.0 : 0.0);
+      row[k] = (i < w && k < w && k <= i) ? cload(&F[(size_t)(c0 + i) + (size_t)(c0 + k) * LD]) : (i == k ? 1.0 : 0.0);
     int bad = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -115,7 +124,7 @@ __device__ __forceinline__ bool potrf64_wave0(double* F, int LD, int c0, int w) 
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k)
-      if (i < w && k < w && k <= i) F[(size_t)(c0 + i) + (size_t)(c0 + k) * LD] = row[k];
+      if (i < w && k < w && k <= i) cstore(&F[(size_t)(c0 + i) + (size_t)(c0 + k) * LD], row[k]);
     if (i == 0) okflag = !bad;
   }
   __syncthreads();
@@ -131,7 +140,7 @@ __device__ __forceinline__ void trsm64(double* F, int LD, int ri, int h, int ck,
                                        double* Linv) {
   for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
     int a = t % NB, b = t / NB;  // L[a][b], column-major read
-    Lt[a * LROW + b] = (a < w && b < w && b <= a) ? F[(size_t)(ck + a) + (size_t)(ck + b) * LD] : (a == b ? 1.0 : 0.0);
+    Lt[a * LROW + b] = (a < w && b < w && b <= a) ? cload(&F[(size_t)(ck + a) + (size_t)(ck + b) * LD]) : (a == b ? 1.0 : 0.0);
   }
   load_tile(Xs, F, LD, ri, ck, h, w);
   __syncthreads();
@@ -155,7 +164,7 @@ __device__ __forceinline__ void trsm64(double* F, int LD, int ri, int h, int ck,
   __syncthreads();
   for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
     int a = t % NB, b = t / NB;
-    if (a < h && b < w) F[(size_t)(ri + a) + (size_t)(ck + b) * LD] = Xs[a + b * LDT];
+    if (a < h && b < w) cstore(&F[(size_t)(ri + a) + (size_t)(ck + b) * LD], Xs[a + b * LDT]);
   }
 }
 
@@ -183,7 +192,10 @@ __device__ __forceinline__ void mfma_update(double* F, int LD, const double* As,
       for (int t = 0; t < 4; ++t) {
         int i = qi + a * 16 + (lane >> 4) + 4 * t;
         int j = qj + b * 16 + (lane & 15);
-        if (i < hi && j < hj && (!diag || i >= j)) F[(size_t)(ri + i) + (size_t)(rj + j) * LD] -= acc[a][b][t];
+        if (i < hi && j < hj && (!diag || i >= j)) {
+          double* q = &F[(size_t)(ri + i) + (size_t)(rj + j) * LD];
+          cstore(q, cload(q) - acc[a][b][t]);
+        }
       }
 }
 
@@ -203,7 +215,7 @@ __global__ void __launch_bounds__(256) k_dag_factor(BigFront B, const InstDev* i
   for (;;) {
     if (threadIdx.x == 0) {
       int t = atomicAdd(B.ticket, 1);
-      if (t < B.ntask && ld_acquire(B.abortf) != 0) t = B.ntask;
+      if (t < B.ntask && ld_flag(B.abortf) != 0) t = B.ntask;
       task_s = t;
     }
     __syncthreads();
@@ -296,7 +308,7 @@ __global__ void __launch_bounds__(256) k_big_fsolve(BigFront B, const FrontDesc*
   for (int j = 0; j < jmax; ++j) {
     if (!wait_geq(&B.sflag[j], epoch, B.abortf)) return;
     int c0 = B.tb[j], w = B.tb[j + 1] - c0;
-    if (threadIdx.x < w) xs[threadIdx.x] = v[c0 + threadIdx.x];
+    if (threadIdx.x < w) xs[threadIdx.x] = cload(&v[c0 + threadIdx.x]);
     __syncthreads();
     if (row < h)
       for (int c = q; c < w; c += 4) acc += F[(size_t)(r0 + row) + (size_t)(c0 + c) * LD] * xs[c];
@@ -322,7 +334,7 @@ __global__ void __launch_bounds__(256) k_big_fsolve(BigFront B, const FrontDesc*
           if (lane > j) xi -= Lr[j] * xj;
         }
       }
-      if (lane < h) v[r0 + lane] = xi;
+      if (lane < h) cstore(&v[r0 + lane], xi);
     }
     publish(&B.sflag[b], epoch);
   } else {
@@ -371,7 +383,7 @@ __global__ void __launch_bounds__(256) k_big_bsolve(BigFront B, const FrontDesc*
     if (!wait_geq(&B.sflag[j], epoch, B.abortf)) return;
     int r0 = B.tb[j], h = B.tb[j + 1] - r0;
     load_tile(S, F, LD, r0, c0, h, w);
-    if (threadIdx.x < h) xs[threadIdx.x] = v[r0 + threadIdx.x];
+    if (threadIdx.x < h) xs[threadIdx.x] = cload(&v[r0 + threadIdx.x]);
     __syncthreads();
     if (col < w)
       for (int rr = q; rr < h; rr += 4) acc += S[rr + col * LDT] * xs[rr];
@@ -391,7 +403,7 @@ __global__ void __launch_bounds__(256) k_big_bsolve(BigFront B, const FrontDesc*
       if (lane == j) xi = xj;
       if (lane < j) xi -= S[j + lane * LDT] * xj;
     }
-    if (lane < w) v[c0 + lane] = xi;
+    if (lane < w) cstore(&v[c0 + lane], xi);
   }
   publish(&B.sflag[b], epoch);
 }
