@@ -121,9 +121,16 @@ int mb_solver_reinit(mb_solver* s, const double* p0, const double* d0, const dou
  * 0 factorization, 1 trailing update, 2 panel, 3 forward solve, 4 backward solve, 5 extend-add,
  * 6 persistent large-front factorization.
  * out[4k..4k+3] = total ms, launches, algorithmic flops, algorithmic bytes of kind k; at most
- * len doubles are written; returns the number of kinds. */
+ * len doubles are written; returns the number of kinds. Mask bit 16 records the task timeline
+ * of the persistent large-front factorization. */
 int mb_solver_set_profile(mb_solver* s, int32_t mask);
 int mb_solver_profile(mb_solver* s, double* out, int32_t len);
+
+/* Task timeline of the last factorization of large front `front` (in schedule order): four
+ * 100 MHz wall-clock stamps per task (claimed, updates done, last input ready, published).
+ * Copies at most len values; returns 4 × the task count (0 if there is no such front).
+ * Diagnostic only; no counterpart in the reference. */
+int mb_solver_dag_trace(mb_solver* s, int32_t front, int64_t* out, int32_t len);
 
 /* initkkt!(kkt, H; δ) on the solver's current H (uzawa.jl:73-115): returns 1 on success,
  * 0 on a failed factorization, negative on error; *rho receives the shift used. */

@@ -298,6 +298,10 @@ int mb_solver_profile(mb_solver* s, double* out, int32_t len) {
   });
 }
 
+int mb_solver_dag_trace(mb_solver* s, int32_t front, int64_t* out, int32_t len) {
+  return guard([&]() { return s->S->get_trace(front, (long long*)out, len); });
+}
+
 int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho) {
   return guard([&]() { return s->S->kkt_factor(delta, rho); });
 }

@@ -14,18 +14,20 @@ struct BigFront {
   int ld, r, s;
   int T, Tr;            // tiles (res tiles first, then separator tiles from column r)
   const int* tb;        // T+1 tile boundaries
-  int* state;           // T(T+1)/2 tile progress counters
+  int* state;           // T(T+1)/2 tile flags: the factorization epoch once the tile is final
   int* ticket;          // [0] task ticket, [1] abort flag
   int* abortf;          // = ticket + 1
-  const int4* tasks;    // (type, i, j, k): 0 POTRF, 1 TRSM, 2 UPDATE
+  const int4* tasks;    // (type, i, j, kmax) in column order: 0 diagonal, 1 factor, 2 update matrix
   int ntask;
   int nstate;
+  double* Dinv;         // Tr inverted diagonal blocks, 64×64 column-major each
+  unsigned long long* trace;  // optional: 4 timestamps per task (100 MHz wall clock), or null
   int* sflag;           // T solve flags (epoch-valued)
   long long* sticket;   // 64-bit monotone ticket of the solve kernels
   int inst;
 };
 
-void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid);
+void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch);
 void launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
                        const InstDev* I, int kv, int epoch, long long base);
 void launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,

@@ -1,15 +1,19 @@
 // Persistent kernels for large frontal matrices (ld ≥ BIG_LD).
 //
-// Factorization: a tile DAG executed by a persistent grid. Tasks are claimed in a fixed
-// topological order from an atomic ticket counter: POTRF(k), TRSM(i,k), UPDATE(i,j,k). A task
-// waits only on tasks with smaller tickets, which are already held by running workgroups, so the
-// grid always makes progress. Each lower tile (i, j) carries a progress counter: the number of
-// updates applied, then k+1 once it is final in column k. Waits are bounded: a workgroup that
+// Factorization: a left-looking tile Cholesky executed by a persistent grid. Task (i, j), i ≥ j,
+// owns the lower tile (i, j) for its whole life: it holds the tile in FP64 MFMA accumulators,
+// subtracts L_ik L_jkᵀ for every factor column k < min(j, Tr) as soon as both tiles are final,
+// and then either factors it and inverts the factor (i == j < Tr), multiplies it by the inverted
+// diagonal block (i > j, j < Tr), or stores it (update-matrix columns j ≥ Tr). Tasks are claimed
+// from an atomic ticket in column order, diagonal first, so a task waits only on tasks with
+// smaller tickets, which running workgroups already hold: the grid always makes progress. A
+// final tile publishes the factorization epoch in its flag. Waits are bounded: a workgroup that
 // spins past the limit raises the abort flag and every workgroup leaves.
 //
 // Triangular solves: one workgroup per 64-row tile, ticket-ordered; tile b waits for the
-// solution blocks it needs (published with release/acquire flags) and accumulates their
-// contributions while later blocks are still being solved.
+// solution blocks it needs and accumulates their contributions while later blocks are still
+// being solved. The tile it needs last and its inverted diagonal block are staged in LDS before
+// it starts waiting, so each step of the dependency chain is two 64×64 products.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -21,7 +25,9 @@
 
 namespace mb {
 
-#define LDT (NB + 1)
+#define LDT (NB + 1)           // column stride of a column-major tile in LDS
+constexpr int LROW = NB + 2;   // row stride of a row-major tile in LDS (16-byte aligned rows)
+constexpr int NT = 256;        // threads per workgroup of every kernel here
 constexpr long SPIN_LIMIT = 1L << 24;
 
 // Cross-workgroup data moves through agent-scope relaxed atomics: on this part they are
@@ -76,15 +82,104 @@ __device__ __forceinline__ void publish(int* p, int v) {
 
 __device__ __forceinline__ int tri_index(int i, int j) { return i * (i + 1) / 2 + j; }
 
+// task timeline (optional): claim, updates done, inputs of the last step ready, published
+__device__ __forceinline__ void stamp(unsigned long long* trace, int t, int slot) {
+  if (trace != nullptr && threadIdx.x == 0) trace[4 * (size_t)t + slot] = (unsigned long long)wall_clock64();
+}
+
 // ------------------------------------------------------------------------------------------
-// tile tasks (256 threads)
+// tile movement (256 threads; every load is issued unconditionally from a clamped address so
+// that a thread's 16 loads are in flight together)
 // ------------------------------------------------------------------------------------------
 
-// load an h×w tile at (r0, c0) of F (leading dimension LD) into LDS (stride LDT), zero padded
+// h×w tile at (r0, c0) of F (leading dimension LD) → LDS column-major (stride LDT), zero padded.
+// COH: coherent loads (the tile was written by another workgroup of the running kernel).
+template <bool COH>
 __device__ __forceinline__ void load_tile(double* S, const double* F, int LD, int r0, int c0, int h, int w) {
-  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+  const size_t base = (size_t)r0 + (size_t)c0 * LD;
+#pragma unroll
+  for (int u = 0; u < NB * NB / NT; ++u) {
+    int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
-    S[i + j * LDT] = (i < h && j < w) ? cload(&F[(size_t)(r0 + i) + (size_t)(c0 + j) * LD]) : 0.0;
+    bool ok = i < h && j < w;
+    size_t off = ok ? base + (size_t)i + (size_t)j * LD : base;
+    double v = COH ? cload(F + off) : F[off];
+    S[i + j * LDT] = ok ? v : 0.0;
+  }
+}
+
+// LDS column-major tile → h×w tile at (r0, c0) of F with coherent stores (lower part only if lower)
+__device__ __forceinline__ void store_tile(double* F, int LD, int r0, int c0, int h, int w, const double* S,
+                                           bool lower) {
+  const size_t base = (size_t)r0 + (size_t)c0 * LD;
+#pragma unroll
+  for (int u = 0; u < NB * NB / NT; ++u) {
+    int t = threadIdx.x + u * NT;
+    int i = t % NB, j = t / NB;
+    if (i < h && j < w && (!lower || i >= j)) cstore(F + base + (size_t)i + (size_t)j * LD, S[i + j * LDT]);
+  }
+}
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// The 64×64 accumulator tile of a workgroup: wave w holds the 32×32 quadrant (qi, qj) =
+// (32·(w>>1), 32·(w&1)) as 2×2 blocks of v_mfma_f64_16x16x4f64 outputs; element t of block
+// (a, b) in lane l is row qi + 16a + (l>>4) + 4t, column qj + 16b + (l&15).
+struct Acc {
+  d4 v[2][2];
+};
+
+__device__ __forceinline__ void acc_from_lds(Acc& A, const double* S) {
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        A.v[a][b][t] = S[(qi + a * 16 + (lane >> 4) + 4 * t) + (qj + b * 16 + (lane & 15)) * LDT];
+}
+
+__device__ __forceinline__ void acc_to_lds(const Acc& A, double* S) {
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        S[(qi + a * 16 + (lane >> 4) + 4 * t) + (qj + b * 16 + (lane & 15)) * LDT] = A.v[a][b][t];
+}
+
+__device__ __forceinline__ void acc_zero(Acc& A) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) A.v[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// A ← A ± X Yᵀ for 64×64 LDS tiles X, Y (column-major, zero padded past column w)
+template <bool NEG>
+__device__ __forceinline__ void mfma_acc(Acc& A, const double* X, const double* Y, int w) {
+  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
+  int li = lane & 15, lk = lane >> 4;
+  int kmax = (w + 3) & ~3;
+  for (int k0 = 0; k0 < kmax; k0 += 4) {
+    double xv[2], yv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      double x = X[(qi + a * 16 + li) + (k0 + lk) * LDT];
+      xv[a] = NEG ? -x : x;
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) yv[b] = Y[(qj + b * 16 + li) + (k0 + lk) * LDT];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) A.v[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[a], yv[b], A.v[a][b], 0, 0, 0);
   }
 }
 
@@ -95,117 +190,170 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Cholesky of the w×w diagonal tile at (c0, c0) of F, in place, by wave 0: lane i holds row i
-// in registers (identity padded past w); column j's pivot and row j are broadcast with
-// v_readlane, fully unrolled so every register index is static. Returns the pivot status.
-__device__ __forceinline__ bool potrf64_wave0(double* F, int LD, int c0, int w) {
+// 1/√d to full FP64 precision: the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    double e = __builtin_fma(-d * y, y, 1.0);  // 1 − d y²
+    y = __builtin_fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+constexpr int HB = NB / 2;  // half tile: the diagonal-block kernels work on 32-column panels
+
+// Right-looking factorization of the 32-column panel [c0, c0+32) of the diagonal tile held
+// column-major in S (rows c0..63), by wave 0: lane i holds row i of the panel in registers,
+// fully unrolled so every register index is static. Step j takes the pivot d from lane c0+j
+// (v_readlane) and scales column j by 1/√d (on the pivot lane this gives √d), writes it to S,
+// then updates the panel's trailing columns with independent FMAs: the next column from a
+// v_readlane of its one entry (it feeds the next pivot), the others from S read back as LDS
+// broadcasts (a wave's LDS accesses complete in program order). Lanes above the diagonal
+// compute values that only ever reach entries above the diagonal; the caller clears those.
+// Returns 1 on a non-positive pivot (then taken as 1).
+__device__ __forceinline__ int panel32(double* S, double* Lv, int c0) {
+  const int i = threadIdx.x;
+  double row[HB];
+#pragma unroll
+  for (int k = 0; k < HB; ++k) row[k] = S[i + (c0 + k) * LDT];
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < HB; ++j) {
+    double d = readlane_d(row[j], c0 + j);
+    if (!(d > 0.0) || !isfinite(d)) {
+      bad = 1;  // the factor is discarded: the caller retries with a diagonal shift
+      d = 1.0;
+    }
+    const double r = rsqrt_nr(d);
+    const double lij = row[j] * r;
+    S[i + (c0 + j) * LDT] = lij;
+    Lv[c0 + j] = r;  // the same value from every lane: no branch in the unrolled body
+    if (j + 1 < HB) {
+      row[j + 1] -= lij * readlane_d(lij, c0 + j + 1);
+#pragma unroll
+      for (int k = j + 2; k < HB; ++k) row[k] -= lij * S[(c0 + k) + (c0 + j) * LDT];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // bounds the live range of the broadcast reads
+  }
+  return bad;
+}
+
+// Cholesky of the diagonal tile of order h held column-major in S (zero padded past h), as a
+// 2×2 block algorithm: panel [0, 32), then C ← C − B Bᵀ on the trailing 32×32 block by all
+// threads, then panel [32, 64). L is left in S (column-major, zero upper part, identity
+// padded), 1/L_ii in Lv. Returns false on a non-positive pivot (taken as 1).
+__device__ __forceinline__ bool potrf64(double* S, double* Lv, int h) {
   __shared__ int okflag;
-  if (threadIdx.x < 64) {
-    const int i = threadIdx.x;
-    double row[NB];
+  if (threadIdx.x >= h && threadIdx.x < NB) S[threadIdx.x * (LDT + 1)] = 1.0;
+  __syncthreads();
+  int bad = 0;
+  if (threadIdx.x < NB) bad = panel32(S, Lv, 0);
+  __syncthreads();
+  {
+    const int r = HB + (threadIdx.x & (HB - 1)), c = HB + 4 * (threadIdx.x >> 5);
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+    for (int k = 0; k < HB; ++k) {
+      const double b = S[r + k * LDT];
 #pragma unroll
-    for (int k = 0; k < NB; ++k)
-      row[k] = (i < w && k < w && k <= i) ? cload(&F[(size_t)(c0 + i) + (size_t)(c0 + k) * LD]) : (i == k ? 1.0 : 0.0);
-    int bad = 0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      double s = row[j];
-#pragma unroll
-      for (int k = 0; k < j; ++k) s -= row[k] * readlane_d(row[k], j);
-      double d = readlane_d(s, j);
-      if (!(d > 0.0) || !isfinite(d)) {
-        bad = 1;
-        d = 1.0;
-      }
-      double l = sqrt(d);
-      double inv = 1.0 / l;
-      row[j] = (i == j) ? l : ((i > j) ? s * inv : row[j]);
-      __builtin_amdgcn_sched_barrier(0);
+      for (int u = 0; u < 4; ++u) a[u] += b * S[(c + u) + k * LDT];
     }
 #pragma unroll
-    for (int k = 0; k < NB; ++k)
-      if (i < w && k < w && k <= i) cstore(&F[(size_t)(c0 + i) + (size_t)(c0 + k) * LD], row[k]);
-    if (i == 0) okflag = !bad;
+    for (int u = 0; u < 4; ++u) S[r + (c + u) * LDT] -= a[u];
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) {
+    bad |= panel32(S, Lv, HB);
+    if (threadIdx.x == 0) okflag = !bad;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NB * NB; t += NT) {
+    const int r = t % NB, c = t / NB;
+    if (r < c) S[r + c * LDT] = 0.0;
   }
   __syncthreads();
   return okflag != 0;
 }
 
-constexpr int LROW = NB + 2;  // row stride of a transposed factor tile in LDS (16-byte aligned rows)
-
-// X = A L⁻ᵀ for the h×w tile at (ri, ck): L (w×w, factored, identity padded) staged row-major
-// in LDS and X staged column-major in LDS by all threads (coalesced); wave 0 then holds row i of
-// X in registers (lane i), fully unrolled, and the tile goes back through LDS.
-__device__ __forceinline__ void trsm64(double* F, int LD, int ri, int h, int ck, int w, double* Lt, double* Xs,
-                                       double* Linv) {
-  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-    int a = t % NB, b = t / NB;  // L[a][b], column-major read
-    Lt[a * LROW + b] = (a < w && b < w && b <= a) ? cload(&F[(size_t)(ck + a) + (size_t)(ck + b) * LD]) : (a == b ? 1.0 : 0.0);
+// X = L⁻¹ (column-major, stride LDT) for the factor L in S (identity padded) with 1/L_ii in
+// Lv, by recursive doubling: wave 0 inverts the eight 8×8 diagonal blocks (lane 8q+c forms
+// column c of block q right-looking: x_i = b_i / L_ii, then b_m −= L_mi x_i for m > i), then
+// neighbouring blocks of order b = 8, 16, 32 combine as [A 0; B C]⁻¹ = [A⁻¹ 0; −C⁻¹ B A⁻¹ C⁻¹]
+// with all threads forming Y = B A⁻¹ and then −C⁻¹ Y.
+__device__ __forceinline__ void trinv64(const double* S, const double* Lv, double* X) {
+  for (int t = threadIdx.x; t < NB * NB; t += NT) {
+    const int r = t % NB, c = t / NB;
+    if (r < c && (r >> 3) != (c >> 3)) X[r + c * LDT] = 0.0;
   }
-  load_tile(Xs, F, LD, ri, ck, h, w);
-  __syncthreads();
-  if (threadIdx.x < NB) Linv[threadIdx.x] = 1.0 / Lt[threadIdx.x * LROW + threadIdx.x];
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const int i = threadIdx.x;
-    double x[NB];
+  if (threadIdx.x < NB) {
+    const int o = threadIdx.x & ~7, c = threadIdx.x & 7;
+    double x[8];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) x[k] = Xs[i + k * LDT];
+    for (int m = 0; m < 8; ++m) x[m] = (m == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      double s = x[j];
+    for (int i = 0; i < 8; ++i) {
+      x[i] *= Lv[o + i];
 #pragma unroll
-      for (int k = 0; k < j; ++k) s -= x[k] * Lt[j * LROW + k];
-      x[j] = s * Linv[j];
+      for (int m = i + 1; m < 8; ++m) x[m] -= S[(o + m) + (o + i) * LDT] * x[i];
     }
 #pragma unroll
-    for (int k = 0; k < NB; ++k) Xs[i + k * LDT] = x[k];
+    for (int m = 0; m < 8; ++m) X[(o + m) + (o + c) * LDT] = x[m];
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-    int a = t % NB, b = t / NB;
-    if (a < h && b < w) cstore(&F[(size_t)(ri + a) + (size_t)(ck + b) * LD], Xs[a + b * LDT]);
-  }
-}
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// C (hi×hj at (ri, rj) in F) -= A Bᵀ with A (hi×w), B (hj×w) in LDS; lower part only if diag
-__device__ __forceinline__ void mfma_update(double* F, int LD, const double* As, const double* Bs, int ri, int rj, int hi,
-                            int hj, int w, bool diag) {
-  int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
-  d4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  int li = lane & 15, lk = lane >> 4;
-  int kmax = (w + 3) & ~3;
-  for (int k0 = 0; k0 < kmax; k0 += 4) {
-    double av[2], bv[2];
-    for (int a = 0; a < 2; ++a) av[a] = As[(qi + a * 16 + li) + (k0 + lk) * LDT];
-    for (int b = 0; b < 2; ++b) bv[b] = Bs[(qj + b * 16 + li) + (k0 + lk) * LDT];
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
-  }
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int t = 0; t < 4; ++t) {
-        int i = qi + a * 16 + (lane >> 4) + 4 * t;
-        int j = qj + b * 16 + (lane & 15);
-        if (i < hi && j < hj && (!diag || i >= j)) {
-          double* q = &F[(size_t)(ri + i) + (size_t)(rj + j) * LD];
-          cstore(q, cload(q) - acc[a][b][t]);
-        }
+  for (int b = 8; b < NB; b *= 2) {
+    const int bb = b * b, ne = (NB / (2 * b)) * bb;  // elements of all lower-left blocks
+    double z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // Y = B A⁻¹ (A⁻¹ is lower triangular)
+      const int e = threadIdx.x + u * NT;
+      if (e < ne) {
+        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
+        double y = 0.0;
+        for (int k = cc; k < b; ++k) y += S[(o + b + rr) + (o + k) * LDT] * X[(o + k) + (o + cc) * LDT];
+        z[u] = y;
       }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = threadIdx.x + u * NT;
+      if (e < ne) {
+        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
+        X[(o + b + rr) + (o + cc) * LDT] = z[u];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {  // −C⁻¹ Y (C⁻¹ is lower triangular)
+      const int e = threadIdx.x + u * NT;
+      if (e < ne) {
+        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
+        double y = 0.0;
+        for (int k = 0; k <= rr; ++k) y -= X[(o + b + rr) + (o + b + k) * LDT] * X[(o + b + k) + (o + cc) * LDT];
+        z[u] = y;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = threadIdx.x + u * NT;
+      if (e < ne) {
+        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
+        X[(o + b + rr) + (o + cc) * LDT] = z[u];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------------------------------
-// the DAG factorization kernel
+// the left-looking tile factorization kernel
 // ------------------------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_dag_factor(BigFront B, const InstDev* insts) {
-  __shared__ double S0[NB * LROW];  // also the row-major factor tile of TRSM
-  __shared__ double S1[NB * LDT];
+__global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* insts, int epoch) {
+  __shared__ __align__(16) double SA[NB * LROW];
+  __shared__ __align__(16) double SB[NB * LROW];
+  __shared__ __align__(16) double SC[NB * LROW];
   __shared__ double Lv[NB];
   __shared__ int task_s;
   const InstDev& I = insts[B.inst];
@@ -219,37 +367,70 @@ __global__ void __launch_bounds__(256) k_dag_factor(BigFront B, const InstDev* i
       task_s = t;
     }
     __syncthreads();
-    int t = task_s;
+    const int t = task_s;
     __syncthreads();
     if (t >= B.ntask) break;
-    int4 task = B.tasks[t];
-    int type = task.x, i = task.y, j = task.z, k = task.w;
-    int r0i = B.tb[i], hi = B.tb[i + 1] - r0i;
-    int c0k = B.tb[k], wk = B.tb[k + 1] - c0k;
-    if (type == 0) {
-      // POTRF(k): tile (k, k) has all k updates
-      if (!wait_geq(&B.state[tri_index(k, k)], k, B.abortf)) break;
-      bool ok = potrf64_wave0(F, LD, c0k, wk);
-      if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-      publish(&B.state[tri_index(k, k)], k + 1);
-    } else if (type == 1) {
-      // TRSM(i, k)
-      if (!wait_geq(&B.state[tri_index(k, k)], k + 1, B.abortf)) break;
-      if (!wait_geq(&B.state[tri_index(i, k)], k, B.abortf)) break;
-      trsm64(F, LD, r0i, hi, c0k, wk, S0, S1, Lv);
-      publish(&B.state[tri_index(i, k)], k + 1);
-    } else {
-      // UPDATE(i, j, k): C_ij -= L_ik L_jkᵀ
-      int r0j = B.tb[j], hj = B.tb[j + 1] - r0j;
-      if (!wait_geq(&B.state[tri_index(i, k)], k + 1, B.abortf)) break;
-      if (j != i && !wait_geq(&B.state[tri_index(j, k)], k + 1, B.abortf)) break;
-      if (!wait_geq(&B.state[tri_index(i, j)], k, B.abortf)) break;
-      load_tile(S0, F, LD, r0i, c0k, hi, wk);
-      load_tile(S1, F, LD, r0j, c0k, hj, wk);
+    stamp(B.trace, t, 0);
+    const int4 task = B.tasks[t];
+    const int type = task.x, i = task.y, j = task.z, kmax = task.w;
+    const int r0i = B.tb[i], hi = B.tb[i + 1] - r0i;
+    const int r0j = B.tb[j], hj = B.tb[j + 1] - r0j;
+    // the assembled tile (written by earlier launches; only this task writes it here)
+    load_tile<false>(SA, F, LD, r0i, r0j, hi, hj);
+    __syncthreads();
+    Acc C;
+    acc_from_lds(C, SA);
+    __syncthreads();
+    bool alive = true;
+    for (int k = 0; k < kmax; ++k) {
+      if (!wait_geq(&B.state[tri_index(i, k)], epoch, B.abortf) ||
+          (i != j && !wait_geq(&B.state[tri_index(j, k)], epoch, B.abortf))) {
+        alive = false;
+        break;
+      }
+      const int c0k = B.tb[k], wk = B.tb[k + 1] - c0k;
+      load_tile<true>(SA, F, LD, r0i, c0k, hi, wk);
+      if (i != j) load_tile<true>(SB, F, LD, r0j, c0k, hj, wk);
       __syncthreads();
-      mfma_update(F, LD, S0, S1, r0i, r0j, hi, hj, wk, i == j);
-      publish(&B.state[tri_index(i, j)], k + 1);
+      mfma_acc<true>(C, SA, i != j ? SB : SA, wk);
+      __syncthreads();
     }
+    if (!alive) break;
+    stamp(B.trace, t, 1);
+    if (type == 0) {
+      // L_jj → F (read only by later launches), L_jj⁻¹ → Dinv (read by this column's tiles)
+      acc_to_lds(C, SA);
+      __syncthreads();
+      bool ok = potrf64(SA, Lv, hj);
+      if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
+      stamp(B.trace, t, 2);
+      store_tile(F, LD, r0j, r0j, hj, hj, SA, true);
+      trinv64(SA, Lv, SC);
+      store_tile(B.Dinv + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
+      publish(&B.state[tri_index(j, j)], epoch);
+    } else if (type == 1) {
+      // L_ij = C_ij L_jj⁻ᵀ
+      acc_to_lds(C, SA);
+      if (!wait_geq(&B.state[tri_index(j, j)], epoch, B.abortf)) break;
+      stamp(B.trace, t, 2);
+      load_tile<true>(SB, B.Dinv + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
+      __syncthreads();
+      Acc X;
+      acc_zero(X);
+      mfma_acc<false>(X, SA, SB, NB);
+      __syncthreads();
+      acc_to_lds(X, SA);
+      __syncthreads();
+      store_tile(F, LD, r0i, r0j, hi, hj, SA, false);
+      publish(&B.state[tri_index(i, j)], epoch);
+    } else {
+      // update-matrix tile: final once every factor column has been subtracted
+      acc_to_lds(C, SA);
+      __syncthreads();
+      stamp(B.trace, t, 2);
+      store_tile(F, LD, r0i, r0j, hi, hj, SA, i == j);
+    }
+    stamp(B.trace, t, 3);
   }
 }
 
@@ -283,10 +464,13 @@ __global__ void k_big_gather(const FrontDesc* fronts, int f, const int* childlis
   }
 }
 
-// forward: tile b (ticket order) accumulates L[b, j] x_j for published j < min(b, Tr), then
-// solves its diagonal block (res tiles) or adds into uvec (sep tiles)
-__global__ void __launch_bounds__(256) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
-                                                    int kv, int epoch, long long base) {
+// forward: tile b (ticket order) accumulates L[b, j] x_j for published j < min(b, Tr); a
+// factor tile then forms x_b = L_bb⁻¹ (v_b − Σ) and publishes it, a separator tile adds Σ into
+// the front's update vector
+__global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
+                                                   int kv, int epoch, long long base) {
+  __shared__ double Sinv[NB * LDT];
+  __shared__ double Sc[NB * LDT];
   __shared__ double xs[NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
@@ -296,60 +480,58 @@ __global__ void __launch_bounds__(256) k_big_fsolve(BigFront B, const FrontDesc*
   if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
   __syncthreads();
   if (I.sc[S_ACTIVE] == 0.0) return;
-  int b = (int)tk;
+  const int b = (int)tk;
   if (b < 0 || b >= B.T) return;
   double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
   const double* F = B.F;
   const int LD = B.ld;
-  int r0 = B.tb[b], h = B.tb[b + 1] - r0;
-  int row = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int r0 = B.tb[b], h = B.tb[b + 1] - r0;
+  const int row = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int jmax = b < B.Tr ? b : B.Tr;
+  const bool diag = b < B.Tr;
+  // stage the last step's operands before waiting
+  if (diag) load_tile<false>(Sinv, B.Dinv + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  if (jmax > 0) load_tile<false>(Sc, F, LD, r0, B.tb[jmax - 1], h, B.tb[jmax] - B.tb[jmax - 1]);
   double acc = 0.0;
-  int jmax = b < B.Tr ? b : B.Tr;
   for (int j = 0; j < jmax; ++j) {
     if (!wait_geq(&B.sflag[j], epoch, B.abortf)) return;
-    int c0 = B.tb[j], w = B.tb[j + 1] - c0;
-    if (threadIdx.x < w) xs[threadIdx.x] = cload(&v[c0 + threadIdx.x]);
+    const int c0 = B.tb[j], w = B.tb[j + 1] - c0;
+    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < w ? cload(&v[c0 + threadIdx.x]) : 0.0;
     __syncthreads();
-    if (row < h)
-      for (int c = q; c < w; c += 4) acc += F[(size_t)(r0 + row) + (size_t)(c0 + c) * LD] * xs[c];
+    if (j == jmax - 1) {
+#pragma unroll
+      for (int c = q; c < NB; c += 4) acc += Sc[row + c * LDT] * xs[c];
+    } else if (row < h) {
+      const double* col = F + (size_t)(r0 + row) + (size_t)c0 * LD;
+      for (int c = q; c < w; c += 4) acc += col[(size_t)c * LD] * xs[c];
+    }
     __syncthreads();
   }
   part[q][row] = acc;
   __syncthreads();
-  if (b < B.Tr) {
-    if (threadIdx.x < 64) {
-      int lane = threadIdx.x;
-      double t = 0.0;
-      if (lane < h) t = v[r0 + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-      double Lr[NB];
+  const double sum = part[0][row] + part[1][row] + part[2][row] + part[3][row];
+  if (diag) {
+    if (threadIdx.x < NB) xs[row] = row < h ? v[r0 + row] - sum : 0.0;
+    __syncthreads();
+    double y = 0.0;
 #pragma unroll
-      for (int j = 0; j < NB; ++j)
-        Lr[j] = (lane < h && j <= lane) ? F[(size_t)(r0 + lane) + (size_t)(r0 + j) * LD] : 0.0;
-      double xi = t;
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        if (j < h) {
-          double xj = __shfl(xi / Lr[j], j, 64);
-          if (lane == j) xi = xj;
-          if (lane > j) xi -= Lr[j] * xj;
-        }
-      }
-      if (lane < h) cstore(&v[r0 + lane], xi);
-    }
+    for (int c = q; c < NB; c += 4) y += Sinv[row + c * LDT] * xs[c];
+    part[q][row] = y;
+    __syncthreads();
+    if (threadIdx.x < NB && row < h) cstore(&v[r0 + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
     publish(&B.sflag[b], epoch);
-  } else {
-    if (threadIdx.x < h) {
-      int lane = threadIdx.x;
-      Fd.uvec[r0 - Fd.r + lane] += part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-    }
+  } else if (threadIdx.x < h) {
+    Fd.uvec[r0 - Fd.r + row] += sum;
   }
 }
 
-// backward: column tile b (ticket order descending) accumulates L[j, b]ᵀ x_j for j > b and
-// L21[:, b]ᵀ x_sep, then solves L_bbᵀ x_b
-__global__ void __launch_bounds__(256) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
-                                                    int kv, int epoch, long long base) {
+// backward: column tile b (ticket order descending) accumulates L21[:, b]ᵀ x_sep and
+// L[j, b]ᵀ x_j for j > b, then forms x_b = L_bb⁻ᵀ (v_b − Σ)
+__global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
+                                                   int kv, int epoch, long long base) {
   __shared__ double S[NB * LDT];
+  __shared__ double Sinv[NB * LDT];
+  __shared__ double Sc[NB * LDT];
   __shared__ double xs[NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
@@ -358,53 +540,52 @@ __global__ void __launch_bounds__(256) k_big_bsolve(BigFront B, const FrontDesc*
   if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
   __syncthreads();
   if (I.sc[S_ACTIVE] == 0.0) return;
-  int tkt = (int)tk;
+  const int tkt = (int)tk;
   if (tkt < 0 || tkt >= B.Tr) return;
-  int b = B.Tr - 1 - tkt;
+  const int b = B.Tr - 1 - tkt;
   double* vall = I.nv + (size_t)kv * I.n;
   double* v = vall + Fd.res0;
   const double* F = B.F;
   const int LD = B.ld;
-  int c0 = B.tb[b], w = B.tb[b + 1] - c0;
-  int col = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
+  const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
+  load_tile<false>(Sinv, B.Dinv + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  if (b + 1 < B.Tr) load_tile<false>(Sc, F, LD, B.tb[b + 1], c0, B.tb[b + 2] - B.tb[b + 1], w);
   double acc = 0.0;
   // separator part (rows ≥ r; x_sep is final), 64-row chunks through LDS so that the global
   // reads run down the columns
   for (int rs0 = Fd.r; rs0 < LD; rs0 += NB) {
-    int h = min(NB, LD - rs0);
-    load_tile(S, F, LD, rs0, c0, h, w);
-    if (threadIdx.x < h) xs[threadIdx.x] = vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]];
+    const int h = min(NB, LD - rs0);
+    load_tile<false>(S, F, LD, rs0, c0, h, w);
+    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
     __syncthreads();
-    if (col < w)
-      for (int rr = q; rr < h; rr += 4) acc += S[rr + col * LDT] * xs[rr];
+#pragma unroll
+    for (int rr = q; rr < NB; rr += 4) acc += S[rr + col * LDT] * xs[rr];
     __syncthreads();
   }
   for (int j = B.Tr - 1; j > b; --j) {
     if (!wait_geq(&B.sflag[j], epoch, B.abortf)) return;
-    int r0 = B.tb[j], h = B.tb[j + 1] - r0;
-    load_tile(S, F, LD, r0, c0, h, w);
-    if (threadIdx.x < h) xs[threadIdx.x] = cload(&v[r0 + threadIdx.x]);
+    const int r0 = B.tb[j], h = B.tb[j + 1] - r0;
+    if (j != b + 1) load_tile<false>(S, F, LD, r0, c0, h, w);
+    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? cload(&v[r0 + threadIdx.x]) : 0.0;
     __syncthreads();
-    if (col < w)
-      for (int rr = q; rr < h; rr += 4) acc += S[rr + col * LDT] * xs[rr];
+    const double* T = (j == b + 1) ? Sc : S;
+#pragma unroll
+    for (int rr = q; rr < NB; rr += 4) acc += T[rr + col * LDT] * xs[rr];
     __syncthreads();
   }
   part[q][col] = acc;
-  load_tile(S, F, LD, c0, c0, w, w);
   __syncthreads();
-  if (threadIdx.x < 64) {
-    int lane = threadIdx.x;
-    double xi = 0.0;
-    if (lane < w) xi = v[c0 + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-    // solve L_bbᵀ x = t bottom-up: lane i holds column i of the block (from LDS)
-    for (int j = w - 1; j >= 0; --j) {
-      double dj = S[j + j * LDT];
-      double xj = __shfl(xi / dj, j, 64);
-      if (lane == j) xi = xj;
-      if (lane < j) xi -= S[j + lane * LDT] * xj;
-    }
-    if (lane < w) cstore(&v[c0 + lane], xi);
-  }
+  if (threadIdx.x < NB)
+    xs[col] = col < w ? v[c0 + col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : 0.0;
+  __syncthreads();
+  // x_b[c] = Σ_r (L⁻¹)[r][c] t[r]
+  double y = 0.0;
+#pragma unroll
+  for (int rr = q; rr < NB; rr += 4) y += Sinv[rr + col * LDT] * xs[rr];
+  part[q][col] = y;
+  __syncthreads();
+  if (threadIdx.x < NB && col < w) cstore(&v[c0 + col], part[0][col] + part[1][col] + part[2][col] + part[3][col]);
   publish(&B.sflag[b], epoch);
 }
 
@@ -412,21 +593,20 @@ __global__ void __launch_bounds__(256) k_big_bsolve(BigFront B, const FrontDesc*
 // launch wrappers
 // ------------------------------------------------------------------------------------------
 
-void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid) {
-  k_fill_int<<<(B.nstate + 255) / 256, 256, 0, st>>>(B.state, B.nstate, 0);
+void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch) {
   k_fill_int<<<1, 64, 0, st>>>(B.ticket, 2, 0);  // ticket and abort flag
-  k_dag_factor<<<grid, 256, 0, st>>>(B, I);
+  k_dag_factor<<<grid, NT, 0, st>>>(B, I, epoch);
 }
 
 void launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
                        const InstDev* I, int kv, int epoch, long long base) {
   k_big_gather<<<1, 256, 0, st>>>(F, f, childlist, I, kv);
-  k_big_fsolve<<<B.T, 256, 0, st>>>(B, F, f, I, kv, epoch, base);
+  k_big_fsolve<<<B.T, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
 }
 
 void launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
                        int epoch, long long base) {
-  k_big_bsolve<<<B.Tr, 256, 0, st>>>(B, F, f, I, kv, epoch, base);
+  k_big_bsolve<<<B.Tr, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
 }
 
 }  // namespace mb

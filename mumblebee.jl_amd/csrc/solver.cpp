@@ -487,26 +487,17 @@ void Solver::build_schedule() {
       for (int c = Fd.r; c < Fd.ld; c += NB) tb.push_back(c);
       tb.push_back(Fd.ld);
       int T = (int)tb.size() - 1, Tr = (Fd.r + NB - 1) / NB;
+      // one task per lower tile, column by column, diagonal first (the ticket order is a
+      // topological order of the left-looking dependencies)
       std::vector<int4> tasks;
-      auto tri = [](int i, int j) { return i * (i + 1) / 2 + j; };
-      (void)tri;
-      if (Tr > 0) {
-        tasks.push_back(int4{0, 0, 0, 0});
-        for (int i = 1; i < T; ++i) tasks.push_back(int4{1, i, 0, 0});
-      }
-      for (int k = 0; k < Tr; ++k) {
-        if (k + 1 < T)
-          for (int i = k + 1; i < T; ++i) tasks.push_back(int4{2, i, k + 1, k});
-        if (k + 1 < Tr) {
-          tasks.push_back(int4{0, k + 1, k + 1, k + 1});
-          for (int i = k + 2; i < T; ++i) tasks.push_back(int4{1, i, 0, k + 1});
-        }
-        for (int j = k + 2; j < T; ++j)
-          for (int i = j; i < T; ++i) tasks.push_back(int4{2, i, j, k});
-      }
+      for (int j = 0; j < T; ++j)
+        for (int i = j; i < T; ++i)
+          tasks.push_back(int4{j < Tr ? (i == j ? 0 : 1) : 2, i, j, std::min(j, Tr)});
       bh->tb.upload(tb, st_);
       bh->tasks.upload(tasks, st_);
       bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
+      HIPCHK(hipMemset(bh->state.p, 0, bh->state.bytes));
+      bh->dinv.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       bh->sflag.alloc(sizeof(int) * std::max(1, T));
       HIPCHK(hipMemset(bh->sflag.p, 0, bh->sflag.bytes));
       BigFront& B = bh->B;
@@ -523,6 +514,8 @@ void Solver::build_schedule() {
       B.abortf = B.ticket + 1;
       B.tasks = bh->tasks.as<int4>();
       B.ntask = (int)tasks.size();
+      B.Dinv = bh->dinv.as<double>();
+      B.trace = nullptr;
       B.sflag = bh->sflag.as<int>();
       B.sticket = d_bigstick_.as<long long>() + big_.size();
       B.inst = Fd.inst;
@@ -696,6 +689,7 @@ void Solver::run_factor() {
   const FrontDesc* F = d_fr_.as<FrontDesc>();
   const StepDesc* S = d_steps_.as<StepDesc>();
   const InstDev* I = d_inst_.as<InstDev>();
+  ++fac_epoch_;
   int pf = prof_begin(PROF_FACTOR);
   for (const Level& L : levels_) {
     int pe = prof_begin(PROF_EXTEND);
@@ -713,7 +707,7 @@ void Solver::run_factor() {
     }
     if (!L.big.empty()) {
       int pd = prof_begin(PROF_DAG);
-      for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_);
+      for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_, fac_epoch_);
       prof_end(pd, PROF_DAG, L.big_flops, 0.0);
     }
   }
@@ -751,6 +745,26 @@ void Solver::prof_flush() {
   }
   pending_.clear();
   prof_next_ = 0;
+}
+
+void Solver::set_trace(bool on) {
+  for (auto& bh : big_) {
+    if (on && bh->trace.p == nullptr) {
+      bh->trace.alloc(sizeof(unsigned long long) * 4 * std::max(1, bh->B.ntask));
+      HIPCHK(hipMemset(bh->trace.p, 0, bh->trace.bytes));
+    }
+    bh->B.trace = on ? bh->trace.as<unsigned long long>() : nullptr;
+  }
+}
+
+int Solver::get_trace(int b, long long* out, int len) {
+  if (b < 0 || b >= (int)big_.size()) return 0;
+  BigHost& bh = *big_[b];
+  int n = 4 * bh.B.ntask;
+  if (bh.trace.p == nullptr || out == nullptr) return n;
+  HIPCHK(hipStreamSynchronize(st_));
+  HIPCHK(hipMemcpy(out, bh.trace.p, sizeof(long long) * std::min(n, len), hipMemcpyDeviceToHost));
+  return n;
 }
 
 void Solver::prof_get(double* out) const {

@@ -50,6 +50,7 @@ enum KKTStatus { KKT_CONTINUE = 0, KKT_SOLVED, KKT_STAGNATED, KKT_ITMAX, KKT_NUM
 enum ProfKind {
   PROF_FACTOR = 0, PROF_UPDATE, PROF_PANEL, PROF_FSOLVE, PROF_BSOLVE, PROF_EXTEND, PROF_DAG, PROF_NKIND
 };
+constexpr int PROF_TRACE_BIT = 1 << 16;  // profile mask bit: record the large-front task timeline
 
 struct DevBuf {
   void* p = nullptr;
@@ -99,7 +100,13 @@ class Solver {
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
   hipStream_t stream() const { return st_; }
   // live per-kernel timing with HIP events on the solver's stream (kinds: PROF_*)
-  void set_profile(int mask) { prof_mask_ = mask; }
+  void set_profile(int mask) {
+    prof_mask_ = mask;
+    set_trace((mask & PROF_TRACE_BIT) != 0);
+  }
+  // per-task timeline of the persistent factorization of large front b (4 stamps per task)
+  void set_trace(bool on);
+  int get_trace(int b, long long* out, int len);
   void prof_flush();
   void prof_get(double* out) const;  // per kind: total ms, launches, algorithmic flops, bytes
   void reinit_public(const double* p0, const double* d0, const double* y0) { reinit(p0, d0, y0); }
@@ -193,12 +200,13 @@ class Solver {
     BigFront B;
     int front;
     long long sbase = 0;
-    DevBuf tb, state, tasks, sflag;
+    DevBuf tb, state, tasks, sflag, dinv, trace;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
   DevBuf d_bigtick_, d_bigstick_;
   int epoch_ = 0;
   int dag_grid_ = 512;
+  int fac_epoch_ = 0;  // factorization counter: a large front's tile flags hold it once final
   void check_abort();
 };
 

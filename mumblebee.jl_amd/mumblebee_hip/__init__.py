@@ -135,6 +135,7 @@ EXPORTS = {
     "mb_solver_reinit": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP]),
     "mb_solver_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mb_solver_profile": (ctypes.c_int, [ctypes.c_void_p, _DP, ctypes.c_int32]),
+    "mb_solver_dag_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I64P, ctypes.c_int32]),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
@@ -376,12 +377,26 @@ class IPMSolver:
 
     PROF_KINDS = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
 
-    def set_profile(self, kinds):
+    def set_profile(self, kinds, trace=False):
         mask = 0
         for k in kinds:
             mask |= 1 << self.PROF_KINDS.index(k)
+        if trace:
+            mask |= 1 << 16
         if lib().mb_solver_set_profile(self._h, mask) != 0:
             raise _err("profile")
+
+    def dag_trace(self, front=0):
+        """Task timeline of the last persistent factorization of large front `front`: an
+        (ntask, 4) array of 100 MHz wall-clock stamps (claimed, updates done, last input ready,
+        published), in schedule order."""
+        n = lib().mb_solver_dag_trace(self._h, int(front), None, 0)
+        if n < 0:
+            raise _err("trace")
+        out = np.zeros(max(n, 0), dtype=np.int64)
+        if n > 0 and lib().mb_solver_dag_trace(self._h, int(front), _p(out, ctypes.c_int64), n) < 0:
+            raise _err("trace")
+        return out.reshape(-1, 4)
 
     def profile(self):
         out = np.zeros(4 * len(self.PROF_KINDS))
