@@ -87,12 +87,14 @@ struct SegDesc {    // a contiguous range of one instance's vector
   int tile0;        // prefix offset of tiles
 };
 
-// Sparse matrix (CSR) on device: y_i = Σ val[k] x[col[k]].
+// Sparse matrix (CSR) on device: y_i = Σ val[k] x[col[k]]. `lanes` (a power of two ≤ 64)
+// lanes work on one row, so a 256-thread workgroup covers 256/lanes consecutive rows.
 struct CSRDev {
   const int* ptr;
   const int* col;
   const double* val;
   int nrow;
+  int lanes;
 };
 
 }  // namespace mb

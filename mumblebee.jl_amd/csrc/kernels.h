@@ -34,6 +34,7 @@ struct Tile {
   int inst;
   int start;
 };
+constexpr int TILE_THREADS = 256;  // threads of a tile-list workgroup
 
 // coefficient value: c (slot < 0), c·sc[slot], or c/sc[slot] (inv)
 struct Coef {

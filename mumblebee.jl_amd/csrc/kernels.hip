@@ -17,7 +17,7 @@
 
 namespace mb {
 
-#define TILE 256
+#define TILE TILE_THREADS
 
 __device__ __forceinline__ double coefval(const Coef& c, const double* sc) {
   if (c.slot < 0) return c.c;
@@ -146,21 +146,32 @@ __global__ void k_reduce_slot(const InstDev* insts, const int* tile_ofs, const d
 // sparse matrix-vector products: y = a A x + b y with A ∈ {B (m rows), Bᵀ (n rows), Q (n rows)}
 // ------------------------------------------------------------------------------------------
 
-__global__ void k_spmv(const InstDev* insts, const Tile* tiles, int which, int kx, int ky, Coef a, Coef b) {
-  Tile t = tiles[blockIdx.x];
+// Each tile is 256/G consecutive rows of one instance's matrix; G = A.lanes lanes share a row,
+// summing a lane-strided slice of it, and a fixed xor tree combines the G partial sums (the
+// result does not depend on timing). Every lane runs the same trip count of the row loop so
+// the shuffles see whole groups.
+__global__ void __launch_bounds__(TILE) k_spmv(const InstDev* insts, const Tile* tiles, int which, int kx, int ky,
+                                               Coef a, Coef b) {
+  const Tile t = tiles[blockIdx.x];
   const InstDev& I = insts[t.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
   const CSRDev& A = which == 0 ? I.B : (which == 1 ? I.Bt : I.Q);
-  int xspace = which == 0 ? 0 : (which == 1 ? 1 : 0);
-  int yspace = which == 0 ? 1 : 0;
-  int i = t.start + threadIdx.x;
-  if (i >= A.nrow) return;
+  const int xspace = which == 0 ? 0 : (which == 1 ? 1 : 0);
+  const int yspace = which == 0 ? 1 : 0;
+  const int G = A.lanes;
+  const int lane = threadIdx.x & (G - 1);
+  const int i = t.start + (int)threadIdx.x / G;
+  const bool ok = i < A.nrow;
   const double* x = vecof(I, xspace, kx);
   double* y = vecof(I, yspace, ky);
   double acc = 0.0;
-  for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) acc += A.val[k] * x[A.col[k]];
-  double av = coefval(a, I.sc), bv = coefval(b, I.sc);
-  y[i] = av * acc + (bv == 0.0 ? 0.0 : bv * y[i]);
+  const int k0 = ok ? A.ptr[i] : 0, k1 = ok ? A.ptr[i + 1] : 0;
+  for (int k = k0 + lane; k < k1; k += G) acc += A.val[k] * x[A.col[k]];
+  for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (ok && lane == 0) {
+    const double av = coefval(a, I.sc), bv = coefval(b, I.sc);
+    y[i] = av * acc + (bv == 0.0 ? 0.0 : bv * y[i]);
+  }
 }
 
 // y[cols] += a D_v x[cols] over all cones (the cone-block part of H x)

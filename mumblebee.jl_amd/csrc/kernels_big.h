@@ -17,10 +17,12 @@ struct BigFront {
   int* state;           // T(T+1)/2 tile flags: the factorization epoch once the tile is final
   int* ticket;          // [0] task ticket, [1] abort flag
   int* abortf;          // = ticket + 1
-  const int4* tasks;    // (type, i, j, kmax) in column order: 0 diagonal, 1 factor, 2 update matrix
+  const int4* tasks;    // (type, i, j, kmax) in column order: 0 diagonal, 1 factor tile,
+                        // 2 update-matrix tile, 3 sub-diagonal tile (staged for the diagonal)
   int ntask;
   int nstate;
   double* Dinv;         // Tr inverted diagonal blocks, 64×64 column-major each
+  int* stage;           // Tr flags: sub-diagonal tile (j+1, j) accumulated (epoch-valued)
   unsigned long long* trace;  // optional: 4 timestamps per task (100 MHz wall clock), or null
   int* sflag;           // T solve flags (epoch-valued)
   long long* sticket;   // 64-bit monotone ticket of the solve kernels

@@ -1,14 +1,17 @@
 // Persistent kernels for large frontal matrices (ld ≥ BIG_LD).
 //
 // Factorization: a left-looking tile Cholesky executed by a persistent grid. Task (i, j), i ≥ j,
-// owns the lower tile (i, j) for its whole life: it holds the tile in FP64 MFMA accumulators,
-// subtracts L_ik L_jkᵀ for every factor column k < min(j, Tr) as soon as both tiles are final,
-// and then either factors it and inverts the factor (i == j < Tr), multiplies it by the inverted
-// diagonal block (i > j, j < Tr), or stores it (update-matrix columns j ≥ Tr). Tasks are claimed
-// from an atomic ticket in column order, diagonal first, so a task waits only on tasks with
-// smaller tickets, which running workgroups already hold: the grid always makes progress. A
-// final tile publishes the factorization epoch in its flag. Waits are bounded: a workgroup that
-// spins past the limit raises the abort flag and every workgroup leaves.
+// owns the lower tile (i, j): it holds the tile in FP64 MFMA accumulators and subtracts
+// L_ik L_jkᵀ for every factor column k < min(j, Tr) as soon as both tiles are final. Then:
+//   diagonal (j < Tr): first forms L_{j,j−1} = C_{j,j−1} L_{j−1,j−1}⁻ᵀ itself from the staged
+//     sub-diagonal tile and subtracts it, then factors the tile, inverts the factor and publishes;
+//   sub-diagonal (i = j+1 < Tr): stages its accumulated tile for the next diagonal task;
+//   other factor tiles: multiply by the inverted diagonal block L_jj⁻¹ (MFMA) and publish;
+//   update-matrix tiles (j ≥ Tr): store.
+// Tasks are claimed from an atomic ticket in column order, diagonal first, so a task waits only
+// on tasks with smaller tickets, which running workgroups already hold: the grid always makes
+// progress. Final tiles publish the factorization epoch in their flag. Waits are bounded: a
+// workgroup that spins past the limit raises the abort flag and every workgroup leaves.
 //
 // Triangular solves: one workgroup per 64-row tile, ticket-ordered; tile b waits for the
 // solution blocks it needs and accumulates their contributions while later blocks are still
@@ -201,149 +204,156 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
   return y;
 }
 
-constexpr int HB = NB / 2;  // half tile: the diagonal-block kernels work on 32-column panels
+// ------------------------------------------------------------------------------------------
+// Cholesky factor and inverse of a 64×64 diagonal tile, recursively: 16×16 diagonal blocks in
+// wave-0 registers, combined as [A11 ·; A21 A22] → L11, X11 = L11⁻¹; L21 = A21 X11ᵀ;
+// A22 −= L21 L21ᵀ; L22, X22; X21 = −X22 (L21 X11), the products on MFMA 16×16 tiles.
+// ------------------------------------------------------------------------------------------
 
-// Right-looking factorization of the 32-column panel [c0, c0+32) of the diagonal tile held
-// column-major in S (rows c0..63), by wave 0: lane i holds row i of the panel in registers,
-// fully unrolled so every register index is static. Step j takes the pivot d from lane c0+j
-// (v_readlane) and scales column j by 1/√d (on the pivot lane this gives √d), writes it to S,
-// then updates the panel's trailing columns with independent FMAs: the next column from a
-// v_readlane of its one entry (it feeds the next pivot), the others from S read back as LDS
-// broadcasts (a wave's LDS accesses complete in program order). Lanes above the diagonal
-// compute values that only ever reach entries above the diagonal; the caller clears those.
-// Returns 1 on a non-positive pivot (then taken as 1).
-__device__ __forceinline__ int panel32(double* S, double* Lv, int c0) {
-  const int i = threadIdx.x;
-  double row[HB];
+// 16×16 MFMA tile: acc ± Σ_{k<K} A[ra+i][ca+k] · op(B)[k][j] from LDS (stride LDT), with
+// op(B)[k][j] = B[rb+j][cb+k] if BT, else B[rb+k][cb+j]
+template <int K, bool BT, bool NEG>
+__device__ __forceinline__ d4 mfma16(d4 acc, const double* A, int ra, int ca, const double* Bm, int rb, int cb) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int k = 0; k < HB; ++k) row[k] = S[i + (c0 + k) * LDT];
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const double a = A[(ra + li) + (ca + k0 + lk) * LDT];
+    const double b = BT ? Bm[(rb + li) + (cb + k0 + lk) * LDT] : Bm[(rb + k0 + lk) + (cb + li) * LDT];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? -a : a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ d4 tile16_load(const double* C, int rc, int cc) {
+  const int lane = threadIdx.x & 63;
+  d4 v;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) v[t] = C[(rc + (lane >> 4) + 4 * t) + (cc + (lane & 15)) * LDT];
+  return v;
+}
+
+__device__ __forceinline__ void tile16_store(double* C, int rc, int cc, d4 v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) C[(rc + (lane >> 4) + 4 * t) + (cc + (lane & 15)) * LDT] = v[t];
+}
+
+// 16×16 diagonal block at (o, o) of S, by wave 0: lane i (i < 16) holds row o+i in registers
+// (lanes 16..63 repeat rows 0..15 and write nothing). Step j takes the pivot from lane j,
+// scales column j by 1/√d and updates the trailing columns with entries broadcast by
+// v_readlane. Then lane c forms column c of L⁻¹ right-looking (x_i /= L_ii, x_m −= L_mi x_i),
+// with L's entries again broadcast from the row registers. L → S and L⁻¹ → X (entries above
+// the diagonal are left for the caller to clear). Returns 1 on a non-positive pivot (taken as 1).
+__device__ __forceinline__ int chol_inv16(double* S, double* X, int o) {
+  const int lane = threadIdx.x, i = lane & 15;
+  double row[16], rinv[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) row[k] = S[(o + i) + (o + k) * LDT];
   int bad = 0;
 #pragma unroll
-  for (int j = 0; j < HB; ++j) {
-    double d = readlane_d(row[j], c0 + j);
+  for (int j = 0; j < 16; ++j) {
+    double d = readlane_d(row[j], j);
     if (!(d > 0.0) || !isfinite(d)) {
       bad = 1;  // the factor is discarded: the caller retries with a diagonal shift
       d = 1.0;
     }
     const double r = rsqrt_nr(d);
+    rinv[j] = r;
     const double lij = row[j] * r;
-    S[i + (c0 + j) * LDT] = lij;
-    Lv[c0 + j] = r;  // the same value from every lane: no branch in the unrolled body
-    if (j + 1 < HB) {
-      row[j + 1] -= lij * readlane_d(lij, c0 + j + 1);
+    row[j] = lij;
 #pragma unroll
-      for (int k = j + 2; k < HB; ++k) row[k] -= lij * S[(c0 + k) + (c0 + j) * LDT];
+    for (int k = j + 1; k < 16; ++k) row[k] -= lij * readlane_d(lij, k);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  double x[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) x[m] = (m == i) ? 1.0 : 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    x[c] *= rinv[c];
+#pragma unroll
+    for (int m = c + 1; m < 16; ++m) x[m] -= readlane_d(row[c], m) * x[c];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      S[(o + i) + (o + k) * LDT] = row[k];
+      X[(o + k) + (o + i) * LDT] = x[k];
     }
-    __builtin_amdgcn_sched_barrier(0);  // bounds the live range of the broadcast reads
   }
   return bad;
 }
 
-// Cholesky of the diagonal tile of order h held column-major in S (zero padded past h), as a
-// 2×2 block algorithm: panel [0, 32), then C ← C − B Bᵀ on the trailing 32×32 block by all
-// threads, then panel [32, 64). L is left in S (column-major, zero upper part, identity
-// padded), 1/L_ii in Lv. Returns false on a non-positive pivot (taken as 1).
-__device__ __forceinline__ bool potrf64(double* S, double* Lv, int h) {
+// With L11, X11 final for the b×b block at (o, o): L21 = A21 X11ᵀ into S, then
+// A22 −= L21 L21ᵀ. One 16×16 output tile per wave.
+template <int b>
+__device__ __forceinline__ void comb_lower(double* S, const double* X, int o) {
+  constexpr int nt = (b / 16) * (b / 16);
+  const int w = threadIdx.x >> 6, ti = 16 * (w % (b / 16)), tj = 16 * (w / (b / 16));
+  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  if (w < nt) acc = mfma16<b, true, false>(acc, S, o + b + ti, o, X, o + tj, o);
+  __syncthreads();
+  if (w < nt) tile16_store(S, o + b + ti, o + tj, acc);
+  __syncthreads();
+  if (w < nt) {
+    acc = tile16_load(S, o + b + ti, o + b + tj);
+    acc = mfma16<b, true, true>(acc, S, o + b + ti, o, S, o + b + tj, o);
+    tile16_store(S, o + b + ti, o + b + tj, acc);
+  }
+  __syncthreads();
+}
+
+// With X11, L21 and X22 final: X21 = −X22 (L21 X11), through X21's place as the temporary.
+template <int b>
+__device__ __forceinline__ void comb_inverse(const double* S, double* X, int o) {
+  constexpr int nt = (b / 16) * (b / 16);
+  const int w = threadIdx.x >> 6, ti = 16 * (w % (b / 16)), tj = 16 * (w / (b / 16));
+  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  if (w < nt) acc = mfma16<b, false, false>(acc, S, o + b + ti, o, X, o, o + tj);
+  __syncthreads();
+  if (w < nt) tile16_store(X, o + b + ti, o + tj, acc);
+  __syncthreads();
+  acc = d4{0.0, 0.0, 0.0, 0.0};
+  if (w < nt) acc = mfma16<b, false, true>(acc, X, o + b + ti, o + b, X, o + b, o + tj);
+  __syncthreads();
+  if (w < nt) tile16_store(X, o + b + ti, o + tj, acc);
+  __syncthreads();
+}
+
+__device__ __forceinline__ int chol_inv32(double* S, double* X, int o) {
+  int bad = 0;
+  if (threadIdx.x < 64) bad = chol_inv16(S, X, o);
+  __syncthreads();
+  comb_lower<16>(S, X, o);
+  if (threadIdx.x < 64) bad |= chol_inv16(S, X, o + 16);
+  __syncthreads();
+  comb_inverse<16>(S, X, o);
+  return bad;
+}
+
+// Cholesky factor L (in S: column-major, zero upper part, identity padded past h) and its
+// inverse X of the diagonal tile of order h held in S (zero padded past h). Returns false on a
+// non-positive pivot.
+__device__ __forceinline__ bool chol_inv64(double* S, double* X, int h) {
   __shared__ int okflag;
   if (threadIdx.x >= h && threadIdx.x < NB) S[threadIdx.x * (LDT + 1)] = 1.0;
+  for (int t = threadIdx.x; t < NB * NB; t += NT) X[(t % NB) + (t / NB) * LDT] = 0.0;  // the products read X's upper blocks
   __syncthreads();
-  int bad = 0;
-  if (threadIdx.x < NB) bad = panel32(S, Lv, 0);
-  __syncthreads();
-  {
-    const int r = HB + (threadIdx.x & (HB - 1)), c = HB + 4 * (threadIdx.x >> 5);
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-    for (int k = 0; k < HB; ++k) {
-      const double b = S[r + k * LDT];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) a[u] += b * S[(c + u) + k * LDT];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) S[r + (c + u) * LDT] -= a[u];
-  }
-  __syncthreads();
-  if (threadIdx.x < NB) {
-    bad |= panel32(S, Lv, HB);
-    if (threadIdx.x == 0) okflag = !bad;
-  }
-  __syncthreads();
+  int bad = chol_inv32(S, X, 0);
+  comb_lower<32>(S, X, 0);
+  bad |= chol_inv32(S, X, 32);
+  comb_inverse<32>(S, X, 0);
+  if (threadIdx.x == 0) okflag = !bad;
   for (int t = threadIdx.x; t < NB * NB; t += NT) {
     const int r = t % NB, c = t / NB;
-    if (r < c) S[r + c * LDT] = 0.0;
+    if (r < c) {
+      S[r + c * LDT] = 0.0;
+      X[r + c * LDT] = 0.0;
+    }
   }
   __syncthreads();
   return okflag != 0;
-}
-
-// X = L⁻¹ (column-major, stride LDT) for the factor L in S (identity padded) with 1/L_ii in
-// Lv, by recursive doubling: wave 0 inverts the eight 8×8 diagonal blocks (lane 8q+c forms
-// column c of block q right-looking: x_i = b_i / L_ii, then b_m −= L_mi x_i for m > i), then
-// neighbouring blocks of order b = 8, 16, 32 combine as [A 0; B C]⁻¹ = [A⁻¹ 0; −C⁻¹ B A⁻¹ C⁻¹]
-// with all threads forming Y = B A⁻¹ and then −C⁻¹ Y.
-__device__ __forceinline__ void trinv64(const double* S, const double* Lv, double* X) {
-  for (int t = threadIdx.x; t < NB * NB; t += NT) {
-    const int r = t % NB, c = t / NB;
-    if (r < c && (r >> 3) != (c >> 3)) X[r + c * LDT] = 0.0;
-  }
-  if (threadIdx.x < NB) {
-    const int o = threadIdx.x & ~7, c = threadIdx.x & 7;
-    double x[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) x[m] = (m == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      x[i] *= Lv[o + i];
-#pragma unroll
-      for (int m = i + 1; m < 8; ++m) x[m] -= S[(o + m) + (o + i) * LDT] * x[i];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) X[(o + m) + (o + c) * LDT] = x[m];
-  }
-  __syncthreads();
-  for (int b = 8; b < NB; b *= 2) {
-    const int bb = b * b, ne = (NB / (2 * b)) * bb;  // elements of all lower-left blocks
-    double z[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {  // Y = B A⁻¹ (A⁻¹ is lower triangular)
-      const int e = threadIdx.x + u * NT;
-      if (e < ne) {
-        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
-        double y = 0.0;
-        for (int k = cc; k < b; ++k) y += S[(o + b + rr) + (o + k) * LDT] * X[(o + k) + (o + cc) * LDT];
-        z[u] = y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = threadIdx.x + u * NT;
-      if (e < ne) {
-        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
-        X[(o + b + rr) + (o + cc) * LDT] = z[u];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {  // −C⁻¹ Y (C⁻¹ is lower triangular)
-      const int e = threadIdx.x + u * NT;
-      if (e < ne) {
-        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
-        double y = 0.0;
-        for (int k = 0; k <= rr; ++k) y -= X[(o + b + rr) + (o + b + k) * LDT] * X[(o + b + k) + (o + cc) * LDT];
-        z[u] = y;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = threadIdx.x + u * NT;
-      if (e < ne) {
-        const int o = 2 * b * (e / bb), rr = (e % bb) % b, cc = (e % bb) / b;
-        X[(o + b + rr) + (o + cc) * LDT] = z[u];
-      }
-    }
-    __syncthreads();
-  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -354,7 +364,6 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
   __shared__ __align__(16) double SA[NB * LROW];
   __shared__ __align__(16) double SB[NB * LROW];
   __shared__ __align__(16) double SC[NB * LROW];
-  __shared__ double Lv[NB];
   __shared__ int task_s;
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
@@ -398,16 +407,41 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     if (!alive) break;
     stamp(B.trace, t, 1);
     if (type == 0) {
-      // L_jj → F (read only by later launches), L_jj⁻¹ → Dinv (read by this column's tiles)
+      if (j > 0) {
+        // L_{j,j−1} = C_{j,j−1} L_{j−1,j−1}⁻ᵀ from the staged tile (fetched before the inverse is
+        // ready), then C_jj −= L_{j,j−1} L_{j,j−1}ᵀ
+        const int c0p = B.tb[j - 1], wp = B.tb[j] - c0p;
+        if (!wait_geq(&B.stage[j - 1], epoch, B.abortf)) break;
+        load_tile<true>(SA, F, LD, r0j, c0p, hj, wp);
+        if (!wait_geq(&B.state[tri_index(j - 1, j - 1)], epoch, B.abortf)) break;
+        load_tile<true>(SB, B.Dinv + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
+        __syncthreads();
+        Acc X;
+        acc_zero(X);
+        mfma_acc<false>(X, SA, SB, NB);
+        __syncthreads();
+        acc_to_lds(X, SA);
+        __syncthreads();
+        store_tile(F, LD, r0j, c0p, hj, wp, SA, false);
+        mfma_acc<true>(C, SA, SA, wp);
+        publish(&B.state[tri_index(j, j - 1)], epoch);  // the stores drained during the update
+      }
+      stamp(B.trace, t, 2);
+      // factor and invert; L_jj⁻¹ → Dinv and publish, then L_jj → F (read only by later launches)
       acc_to_lds(C, SA);
       __syncthreads();
-      bool ok = potrf64(SA, Lv, hj);
+      bool ok = chol_inv64(SA, SC, hj);
       if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-      stamp(B.trace, t, 2);
-      store_tile(F, LD, r0j, r0j, hj, hj, SA, true);
-      trinv64(SA, Lv, SC);
       store_tile(B.Dinv + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
       publish(&B.state[tri_index(j, j)], epoch);
+      store_tile(F, LD, r0j, r0j, hj, hj, SA, true);
+    } else if (type == 3) {
+      // sub-diagonal tile: hand the accumulated tile to the next diagonal task
+      acc_to_lds(C, SA);
+      __syncthreads();
+      stamp(B.trace, t, 2);
+      store_tile(F, LD, r0i, r0j, hi, hj, SA, false);
+      publish(&B.stage[j], epoch);
     } else if (type == 1) {
       // L_ij = C_ij L_jj⁻ᵀ
       acc_to_lds(C, SA);

@@ -396,18 +396,33 @@ void Solver::build_device() {
   // sparse matrices
   std::vector<int> p, c;
   std::vector<double> x;
+  int lanes[3];
+  // lanes per row: the largest power of two ≤ the mean row length (1..64); tiles of 256/lanes rows
+  auto plan = [&](int which, i64 nrow) {
+    double avg = nrow > 0 ? (double)c.size() / (double)nrow : 0.0;
+    int G = 1;
+    while (G < 64 && 2.0 * G <= avg) G *= 2;
+    lanes[which] = G;
+    std::vector<Tile> t;
+    for (i64 s = 0; s < nrow; s += TILE_THREADS / G) t.push_back(Tile{0, (int)s});
+    nsptile_[which] = (int)t.size();
+    if (!t.empty()) d_sptiles_[which].upload(t, st_);
+  };
   bsm_to_csr(P_.B, false, p, c, x);
   d_Bp_.upload(p, st_);
   d_Bc_.upload(c, st_);
   d_Bv_.upload(x, st_);
+  plan(0, m_);
   bsm_to_csr(P_.B, true, p, c, x);
   d_Btp_.upload(p, st_);
   d_Btc_.upload(c, st_);
   d_Btv_.upload(x, st_);
+  plan(1, n_);
   bsm_to_csr(P_.Q, false, p, c, x);
   d_Qp_.upload(p, st_);
   d_Qc_.upload(c, st_);
   d_Qv_.upload(x, st_);
+  plan(2, n_);
   d_upos_.upload(upos, st_);
   d_ul_.upload(ul, st_);
   d_uq_.upload(uq, st_);
@@ -429,9 +444,9 @@ void Solver::build_device() {
   idev_.sc = d_sc_.as<double>();
   idev_.scratch = d_scr_.as<double>();
   idev_.F = d_F_.as<double>();
-  idev_.B = CSRDev{d_Bp_.as<int>(), d_Bc_.as<int>(), d_Bv_.as<double>(), (int)m_};
-  idev_.Bt = CSRDev{d_Btp_.as<int>(), d_Btc_.as<int>(), d_Btv_.as<double>(), (int)n_};
-  idev_.Q = CSRDev{d_Qp_.as<int>(), d_Qc_.as<int>(), d_Qv_.as<double>(), (int)n_};
+  idev_.B = CSRDev{d_Bp_.as<int>(), d_Bc_.as<int>(), d_Bv_.as<double>(), (int)m_, lanes[0]};
+  idev_.Bt = CSRDev{d_Btp_.as<int>(), d_Btc_.as<int>(), d_Btv_.as<double>(), (int)n_, lanes[1]};
+  idev_.Q = CSRDev{d_Qp_.as<int>(), d_Qc_.as<int>(), d_Qv_.as<double>(), (int)n_, lanes[2]};
   idev_.n = (int)n_;
   idev_.m = (int)m_;
   d_inst_.upload(std::vector<InstDev>{idev_}, st_);
@@ -488,16 +503,24 @@ void Solver::build_schedule() {
       tb.push_back(Fd.ld);
       int T = (int)tb.size() - 1, Tr = (Fd.r + NB - 1) / NB;
       // one task per lower tile, column by column, diagonal first (the ticket order is a
-      // topological order of the left-looking dependencies)
+      // topological order of the left-looking dependencies). A diagonal task forms the
+      // sub-diagonal factor tile of the previous column itself, so its own update loop stops
+      // one column early and that tile's task only accumulates and stages it.
       std::vector<int4> tasks;
       for (int j = 0; j < T; ++j)
-        for (int i = j; i < T; ++i)
-          tasks.push_back(int4{j < Tr ? (i == j ? 0 : 1) : 2, i, j, std::min(j, Tr)});
+        for (int i = j; i < T; ++i) {
+          if (j >= Tr) tasks.push_back(int4{2, i, j, Tr});
+          else if (i == j) tasks.push_back(int4{0, i, j, std::max(j - 1, 0)});
+          else if (i == j + 1 && i < Tr) tasks.push_back(int4{3, i, j, j});
+          else tasks.push_back(int4{1, i, j, j});
+        }
       bh->tb.upload(tb, st_);
       bh->tasks.upload(tasks, st_);
       bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
       HIPCHK(hipMemset(bh->state.p, 0, bh->state.bytes));
       bh->dinv.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
+      bh->stage.alloc(sizeof(int) * std::max(1, Tr));
+      HIPCHK(hipMemset(bh->stage.p, 0, bh->stage.bytes));
       bh->sflag.alloc(sizeof(int) * std::max(1, T));
       HIPCHK(hipMemset(bh->sflag.p, 0, bh->sflag.bytes));
       BigFront& B = bh->B;
@@ -515,6 +538,7 @@ void Solver::build_schedule() {
       B.tasks = bh->tasks.as<int4>();
       B.ntask = (int)tasks.size();
       B.Dinv = bh->dinv.as<double>();
+      B.stage = bh->stage.as<int>();
       B.trace = nullptr;
       B.sflag = bh->sflag.as<int>();
       B.sticket = d_bigstick_.as<long long>() + big_.size();
@@ -634,8 +658,7 @@ void Solver::dot(int space, int kx, int ky, int mode, int slot, int op, int kw0,
 
 void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
   const InstDev* I = d_inst_.as<InstDev>();
-  if (which == 0) launch_spmv(st_, I, d_mtiles_.as<Tile>(), ntile_m_, 0, kx, ky, a, b);
-  else launch_spmv(st_, I, d_ntiles_.as<Tile>(), ntile_n_, which, kx, ky, a, b);
+  launch_spmv(st_, I, d_sptiles_[which].as<Tile>(), nsptile_[which], which, kx, ky, a, b);
 }
 
 // ky = H kx = Q kx + D kx

@@ -177,6 +177,8 @@ class Solver {
   DevBuf d_steps_, d_ea_items_, d_ea_tile0_, d_solve_items_, d_ntiles_, d_mtiles_, d_ntofs_, d_mtofs_;
   DevBuf d_partial_, d_coneout_, d_qdiag_;
   int ntile_n_ = 0, ntile_m_ = 0;
+  DevBuf d_sptiles_[3];  // row tiles of B, Bᵀ, Q for the group-per-row spmv
+  int nsptile_[3] = {0, 0, 0};
   int64_t nunion_ = 0;
   double fac_ms_ = 0.0, fac_ms_tot_ = 0.0;
   int fac_calls_ = 0;
@@ -200,7 +202,7 @@ class Solver {
     BigFront B;
     int front;
     long long sbase = 0;
-    DevBuf tb, state, tasks, sflag, dinv, trace;
+    DevBuf tb, state, tasks, sflag, dinv, stage, trace;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
   DevBuf d_bigtick_, d_bigstick_;

@@ -56,19 +56,43 @@ def _check_factor(L, M):
         assert np.linalg.norm(L - Lref) / np.linalg.norm(Lref) < 1e-11
 
 
+def _regularized(d, shift=0.5):
+    """The same problem with Q + shift·I: δQ + BᵀB is then positive definite (with Q = 0 and
+    m < n, BᵀB is singular and its ρ-shifted factor is rounding noise past the rank)."""
+    import scipy.sparse as sp
+    d = dict(d)
+    d["Q"] = (d["Q"] + shift * sp.identity(d["Q"].shape[0], format="csc")).tocsc()
+    return d
+
+
 @pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_factor_persistent_kernels_match_dense_cholesky(case):
     """Every front of order ≥ 64 through the tile-DAG kernel (fronts with separators included)."""
     from oracle import ipm as O
-    d = small_cases()[case]
+    d = _regularized(small_cases()[case])
     solver = mh.init(hip_problem(d), mh.IPMSettings(big_front_ld=64))
     osolver = O.Solver(oracle_problem(d))
     delta = 0.37
     ok, rho = solver.kkt_factor(delta)
     ook, orho = osolver.kkt.init(osolver.Qd, delta)
-    assert ok and ook and (rho == 0.0) == (orho == 0.0)
-    M = osolver.kkt.L + delta * osolver.Qd + rho * np.eye(osolver.Qd.shape[0])
+    assert ok and ook and rho == 0.0 and orho == 0.0
+    M = osolver.kkt.L + delta * osolver.Qd
+    assert np.linalg.eigvalsh(M).min() > 1e-8 * np.linalg.norm(M, 2)
     _check_factor(np.tril(solver.dense_factor()), M)
+
+
+@pytest.mark.parametrize("case", [2, 3])
+def test_factor_persistent_kernels_singular_takes_rho_retry(case):
+    """SOCP cases with Q = 0: BᵀB (rank ≤ m < n) is singular, so the unshifted factorization
+    fails on both sides and the retry shift is ρ = eps·‖F‖ (uzawa.jl:97-112), the same value."""
+    from oracle import ipm as O
+    d = small_cases()[case]
+    solver = mh.init(hip_problem(d), mh.IPMSettings(big_front_ld=64))
+    osolver = O.Solver(oracle_problem(d))
+    ok, rho = solver.kkt_factor(0.37)
+    ook, orho = osolver.kkt.init(osolver.Qd, 0.37)
+    assert rho > 0.0 and orho > 0.0
+    assert abs(rho - orho) <= 1e-12 * orho
 
 
 def test_factor_large_front():
