@@ -6,7 +6,8 @@ xcol[v]:xcol[v+1], out-vertex u owns rows xrow[u]:xrow[u+1], arc e (a block (u, 
 values xblk[e]:xblk[e+1], arcs of a vertex are sorted by target (block_sparse_matrix.jl:703-711).
 0-based here.
 
-Pinned by: dense numpy products on random small inputs (tests/test_oracle_bsm.py).
+Pinned by: dense numpy products on random small inputs
+(tests/test_oracle_problem.py::test_bsm_products_against_dense).
 """
 import numpy as np
 

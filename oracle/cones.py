@@ -1,10 +1,11 @@
 """Cone kernels (oracle restatement of src/IPM/src/cone/{pos,noc,soc}.jl and the
 error-free transformations of src/IPM/src/utils.jl:27-53).
 
-Pinned by (tests/test_oracle_cones.py):
+Pinned by (tests/test_oracle_cones_kkt.py):
   - NT invariant H·p = d for the scaling block added by `scale` (pos and SOC), H ≻ 0;
-  - −f′(e) = e at the identity; shadow maps invert (socshadow twice is the identity);
-  - socroot(√H) applied twice equals H; the corrector reduces to the closed-form
+  - −f′(e) = e at the identity; −f′ is an involution on the interior (shadow twice);
+  - √H (socroot) applied twice equals the NT block H, and its inverse form undoes it;
+  - the corrector reduces to the closed-form
     Jordan-product formula; max step against brute-force bisection on membership;
   - twoprod / twosum exactness checked with fractions.Fraction.
 """

@@ -2,10 +2,11 @@
 history/history.jl, settings/ipm.jl).
 
 Pinned by (tests/test_oracle_ipm.py): solver-independent optimality certificates at 1e-8
-(primal/dual residuals, gap, cone membership) on seeded QPs and SOCPs; agreement with a
-direct dense KKT Newton solve of small equality QPs (closed form); forced failure paths
+(primal/dual residuals, gap, cone membership) on seeded QPs and SOCPs; LP optima against
+scipy's HiGHS; the closed-form optimum of small equality QPs; forced failure paths
 (max_iter=1 → ITERATION_LIMIT, infeasible Bp=g, p≥0 → PRIMAL_INFEASIBLE, unbounded
-objective → DUAL_INFEASIBLE, newton_max_iter=0 → non-SOLVED KKT status).
+objective → DUAL_INFEASIBLE). The KKT status semantics are pinned at the solvekkt level
+(tests/test_oracle_cones_kkt.py).
 """
 import math
 from dataclasses import dataclass

@@ -7,9 +7,10 @@ the same mathematical object, so the factor is pinned by LLᵀ = M to rounding, 
 reference's accumulation order (which is unpinned). The Uzawa / CG control logic is restated
 line by line: stopping rules, stall test, irmin/irmax, cgmax, statuses, kktwindow.
 
-Pinned by (tests/test_oracle_kkt.py): the returned (x, y) satisfy the KKT system to the
+Pinned by (tests/test_oracle_cones_kkt.py): the returned (x, y) satisfy the KKT system to the
 requested tolerances on random SPD inputs; against a direct dense solve of the saddle-point
-system; the status/counter semantics on forced cases (irmax=0, cgmax=0, stagnation).
+system; the status/counter semantics on forced cases (irmax=0, cgmax=0, stall=1 stagnation,
+failed factorization → ρ retry).
 """
 import math
 

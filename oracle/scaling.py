@@ -2,7 +2,7 @@
 
 Pinned by: the invariant that after the sweeps every row / vertex-block ∞-norm of the scaled
 data is within tol of 1 or itmax sweeps ran (scaling.jl:143-169), and that scale/unscale
-round-trips (tests/test_oracle_scaling.py).
+round-trips as a diagonal congruence (tests/test_oracle_scaling.py).
 """
 import numpy as np
 

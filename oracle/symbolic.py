@@ -14,8 +14,8 @@ smallest vertex index, followed by an elimination-tree postorder and fundamental
 supernodes) and the C++ host restatement must match it bit for bit. Structural invariants
 (etree parent after child, fronts partition the columns into contiguous ranges, each
 separator is contained in the ancestors' columns, the factor pattern covers the pattern of
-M) are checked in tests/test_oracle_symbolic.py; the fill it produces is checked against
-brute-force symbolic elimination of the permuted graph.
+M) are checked in tests/test_host_parity.py::test_symbolic_structure_invariants, which also
+checks the fronts against brute-force symbolic elimination of the permuted graph.
 """
 import numpy as np
 

@@ -180,3 +180,74 @@ def test_factor_failure_takes_the_rho_retry():
     H2 = np.diag([1.0, 1.0, 1.0, 1e-300]) - np.diag([0, 0, 0, 2e-300])
     ok2, rho2 = solver.init(H2, 1.0)
     assert rho2 > 0 and ok2
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_socroot_squared_is_the_nt_scaling(seed):
+    # √H applied twice equals the scaling block H that socscale adds (soc.jl:82-110 vs
+    # soc.jl:192-263), and the inverse form undoes the forward one
+    rng = np.random.default_rng(seed)
+    n = 7
+    p, d = soc_point(rng, n), soc_point(rng, n)
+    cache = C.SocCache(n)
+    H = np.zeros((n, n))
+    flag, _ = C.socscale(H, cache, p, d)
+    assert flag
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        x = C.socroot(C.socroot(e.copy(), cache.w, cache.beta, False), cache.w, cache.beta, False)
+        np.testing.assert_allclose(x, H[:, i], rtol=1e-12, atol=1e-12)
+    x = rng.standard_normal(n)
+    y = C.socroot(C.socroot(x.copy(), cache.w, cache.beta, False), cache.w, cache.beta, True)
+    np.testing.assert_allclose(y, x, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("cone", [("pos",), ("soc",)])
+def test_shadow_is_an_involution(cone):
+    # −f′ maps the interior onto itself and −f′(−f′(x)) = x for −Σ log xᵢ and −log det x
+    rng = np.random.default_rng(7)
+    n = 6
+    x = rng.random(n) + 0.1 if cone[0] == "pos" else soc_point(rng, n)
+    s, t = np.zeros(n), np.zeros(n)
+    assert C.shadow(cone, s, x)
+    assert C.shadow(cone, t, s)
+    np.testing.assert_allclose(t, x, rtol=1e-13)
+
+
+def _kkt_instance(seed, n=30, m=12):
+    rng = np.random.default_rng(seed)
+    Bd = rng.standard_normal((m, n))
+    G0 = rng.standard_normal((n, n))
+    H = G0 @ G0.T / n + np.eye(n)
+    solver = UzawaSolver(_dense_bsm(Bd))
+    ok, rho = solver.init(H, 0.05)
+    assert ok and rho == 0.0
+    return solver, H, Bd, rng.standard_normal(n), rng.standard_normal(m)
+
+
+def test_solvekkt_irmax_zero_skips_refinement():
+    # nir ≥ irmax before the first pass: no refinement, the dual CG still runs, and the
+    # status is ITMAX (uzawa.jl:384-390, 480-488)
+    from oracle.kkt import KKT_ITMAX
+    solver, H, Bd, f, g = _kkt_instance(1)
+    x, y = np.zeros(len(f)), np.zeros(len(g))
+    ncg, nir, st, _, _ = solvekkt(solver, H, Bd, x, y, f, g, False, 1e-12, 1e-12, 0.5, 0, 100)
+    assert nir == 0 and ncg > 0 and st == KKT_ITMAX
+
+
+def test_solvekkt_cgmax_zero_reports_itmax():
+    from oracle.kkt import KKT_ITMAX
+    solver, H, Bd, f, g = _kkt_instance(2)
+    x, y = np.zeros(len(f)), np.zeros(len(g))
+    ncg, nir, st, _, _ = solvekkt(solver, H, Bd, x, y, f, g, False, 1e-12, 1e-12, 0.5, 10, 0, 1)
+    assert ncg == 0 and nir >= 1 and st == KKT_ITMAX
+
+
+def test_solvekkt_stall_stops_the_second_pass():
+    # stall = 1 demands fres ≤ 0·fprv, so the second pass is declared stagnated unless solved
+    from oracle.kkt import KKT_STAGNATED
+    solver, H, Bd, f, g = _kkt_instance(3)
+    x, y = np.zeros(len(f)), np.zeros(len(g))
+    ncg, nir, st, _, _ = solvekkt(solver, H, Bd, x, y, f, g, False, 0.0, 0.0, 1.0, 10, 100)
+    assert nir == 2 and st == KKT_STAGNATED

@@ -94,3 +94,4 @@ def test_dual_infeasible():
              g=np.array([0.0]), K=[("pos",)], s=[n])
     r = O.solve(oracle_problem(d))
     assert r.status in (O.DUAL_INFEASIBLE, O.NEAR_DUAL_INFEASIBLE)
+
