@@ -3,21 +3,28 @@
 A step is one full IPM solve to OPTIMAL at gap_tol = feas_tol = 1e-8 of one seeded instance of
 the 2000-variable sparse QP workload (n=2000, m=1000, nonnegative orthant, Q = PᵀP + 1e-3 I with
 P at 0.5 % density, B at 1 % density). Every timed instance is distinct (its own seed) and
-resident in HBM — problem data, symbolic factor and buffers set up before the timed region —
-and is solved from the cold starting point. value = solves/s over the K timed solves; p50/p99
-latency over the same solves.
+resident in HBM — problem data, symbolic factor and buffers set up before the timed region — and
+is solved from the cold starting point. `--slots S` instances are solved at a time, each from its
+own host thread on its own HIP stream, so their kernels overlap on the GPU; the persistent
+factorization of each takes 1/S of the CUs. value = solves/s over the K timed solves (whole job);
+p50/p99 latency over the same solves; solo_ms = one instance alone on the whole GPU.
 
-roofline: the FP64 MFMA trailing-update kernel of the frontal Cholesky, timed live with HIP
-events on the solver's stream over the timed region; achieved = its algorithmic flops
-Σ (N_rem² + N_rem)·w per launch ÷ its measured time.
+roofline: the dominant kernel, the persistent FP64 MFMA Cholesky of the 2000×2000 front,
+timed live with HIP events on each solver's stream over the timed region; achieved = its
+algorithmic flops r³/3 per launch ÷ its measured average launch time (under the S-way sharing).
 
 cpu_baseline: the oracle (numpy + LAPACK restatement of the reference path) on the host cores,
 one instance.
+
+`--serve`: a served stream instead — requests of mixed kind and size (QP and SOCP, n log-uniform
+in [50, 4000]) arrive as a Poisson process and go through the continuous-batching loop
+(IPMServer); latency = finish − arrival, including host analysis and device setup.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -28,6 +35,7 @@ import numpy as np  # noqa: E402
 
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
+METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP/s"
 
 
 def workload(seed, n, m):
@@ -56,46 +64,63 @@ def cpu_baseline(n, m, seed):
                        f"{O.STATUS_NAMES[res.status]}, {res.niter} iterations)")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=2000)
-    ap.add_argument("--m", type=int, default=1000)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-    import mumblebee_hip as mh
-    K, W = args.steps, args.warmup
-    solvers = []
+def run_concurrent(solvers, slots):
+    """Solve every solver from `slots` host threads; returns (elapsed s, latencies s, results)."""
+    lat = [0.0] * len(solvers)
+    res = [None] * len(solvers)
+    nxt = [0]
+    lock = threading.Lock()
+
+    def worker():
+        while True:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= len(solvers):
+                return
+            a = time.perf_counter()
+            res[i] = solvers[i].solve()
+            lat[i] = time.perf_counter() - a
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker) for _ in range(max(1, slots))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    return time.perf_counter() - t0, lat, res
+
+
+def bench_resident(args, mh):
+    K, W, S = args.steps, args.warmup, max(1, args.slots)
+    ncu = mh.device_info()["cus"]
+    settings = mh.IPMSettings(dag_grid=max(16, ncu // S))
+    kinds = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
     t_setup = time.perf_counter()
+    solvers = []
     for k in range(W + K):
         d = workload(1000 + k, args.n, args.m)
         prob = mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, [mh.PositiveCone()], d["s"])
-        solvers.append(mh.init(prob, mh.IPMSettings()))
-    t_setup = time.perf_counter() - t_setup
-    for s in solvers[:W]:
-        r = s.solve()
-        assert r.status == mh.IPMStatus.OPTIMAL, r.status
+        solvers.append(mh.init(prob, settings))
+    t_setup = (time.perf_counter() - t_setup) / (W + K)
+    # one instance alone on the whole GPU (latency reference; not part of the timed job)
+    d = workload(999, args.n, args.m)
+    solo = mh.init(mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, [mh.PositiveCone()], d["s"]))
+    solo.solve()
+    solo.reinit()
+    a = time.perf_counter()
+    r = solo.solve()
+    solo_ms = (time.perf_counter() - a) * 1e3
+    assert r.status == mh.IPMStatus.OPTIMAL, r.status
+    solo.close()
+    if W:
+        _, _, wres = run_concurrent(solvers[:W], S)
+        assert all(x.status == mh.IPMStatus.OPTIMAL for x in wres), [x.status for x in wres]
     timed = solvers[W:]
     for s in timed:
-        s.set_profile(["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"])
-    lat = []
-    niters = []
-    statuses = []
-    t0 = time.perf_counter()
-    for s in timed:
-        a = time.perf_counter()
-        r = s.solve()
-        lat.append(time.perf_counter() - a)
-        niters.append(r.niter)
-        statuses.append(r.status)
-    elapsed = time.perf_counter() - t0
-    assert all(st == mh.IPMStatus.OPTIMAL for st in statuses), statuses
+        s.set_profile(kinds)
+    elapsed, lat, res = run_concurrent(timed, S)
+    assert all(x.status == mh.IPMStatus.OPTIMAL for x in res), [x.status for x in res]
     prof = {}
     for s in timed:
         for k, v in s.profile().items():
@@ -104,14 +129,12 @@ def main():
                 acc[key] += v[key]
     st0 = timed[0].stats()
     fac = prof["factor"]
-    # the dominant factorization kernel: persistent large-front DAG or batched trailing update
     dom_name = "dag" if prof["dag"]["ms"] >= prof["update"]["ms"] else "update"
-    upd = prof[dom_name]
-    achieved = upd["flops"] / (upd["ms"] * 1e-3) / 1e12 if upd["ms"] > 0 else 0.0
-    fac_tflops = fac["flops"] / (fac["ms"] * 1e-3) / 1e12 if fac["ms"] > 0 else 0.0
+    dom = prof[dom_name]
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else 0.0
     lat_ms = np.array(lat) * 1e3
     line = {
-        "metric": "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP/s",
+        "metric": METRIC,
         "value": K / elapsed,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -126,22 +149,22 @@ def main():
         "config": {
             "workload": f"sparse convex QP n={args.n} m={args.m} on the nonnegative orthant, "
                         f"Q=PᵀP+1e-3I (P 0.5% dense), B 1% dense, N(0,1); {K} distinct seeded instances, "
-                        f"resident, cold start, gap_tol=feas_tol=1e-8",
+                        f"resident, cold start, gap_tol=feas_tol=1e-8, {S} solved at a time",
             "precision": "fp64",
+            "slots": S,
+            "dag_grid": settings.dag_grid,
             "p50_ms": float(np.percentile(lat_ms, 50)),
             "p99_ms": float(np.percentile(lat_ms, 99)),
-            "iterations_mean": float(np.mean(niters)),
-            "factor_tflops": fac_tflops,
+            "solo_ms": solo_ms,
+            "iterations_mean": float(np.mean([x.niter for x in res])),
+            "factor_tflops_aggregate": fac["flops"] / elapsed / 1e12,
             "factor_ms_per_call": fac["ms"] / max(1, fac["launches"]),
             "factor_exact_flops": st0["exact_flops"],
             "factor_front_flops": st0["front_flops"],
             "nfront": st0["nfront"],
-            "setup_s_per_instance": t_setup / (W + K),
+            "setup_s_per_instance": t_setup,
             "kernel_ms": {k: v["ms"] for k, v in prof.items()},
             "kernel_launches": {k: v["launches"] for k, v in prof.items()},
-            "solve_sweep_GBps": ((prof["fsolve"]["bytes"] + prof["bsolve"]["bytes"]) /
-                                 ((prof["fsolve"]["ms"] + prof["bsolve"]["ms"]) * 1e-3) / 1e9
-                                 if prof["fsolve"]["ms"] > 0 else 0.0),
         },
         "roofline": {
             "bound": "mfma",
@@ -150,14 +173,78 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_MFMA,
             "traffic": None,
-            "kernel": ("k_dag_factor (persistent tile-DAG FP64 MFMA Cholesky of the large fronts)"
+            "kernel": ("k_dag_factor (persistent left-looking tile FP64 MFMA Cholesky of the large fronts)"
                        if dom_name == "dag" else "k_update (FP64 MFMA trailing update of the frontal Cholesky)"),
-            "avg_launch_us": upd["ms"] / max(1, upd["launches"]) * 1e3,
-            "flops_per_launch": upd["flops"] / max(1, upd["launches"]),
+            "avg_launch_us": dom["ms"] / max(1, dom["launches"]) * 1e3,
+            "flops_per_launch": dom["flops"] / max(1, dom["launches"]),
         },
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.n, args.m, 1000 + W)
+    return line
+
+
+def bench_serve(args, mh):
+    """A Poisson stream of mixed requests through IPMServer."""
+    from mumblebee_hip import generators as G
+    rng = np.random.default_rng(7)
+    K = args.steps
+    reqs = []
+    for k in range(K):
+        n = int(G.log_uniform_n(rng, 50, args.serve_nmax))
+        if rng.random() < 0.7:
+            d = G.qp(n=n, m=max(1, n // 2), seed=5000 + k)
+        else:
+            d = G.socp(n=n, m=max(1, n // 2), ncone=max(1, n // 40), seed=5000 + k, adens=min(0.2, 20.0 / n))
+        cones = [{"pos": mh.PositiveCone, "free": mh.CofreeCone, "soc": mh.SecondOrderCone}[c[0]]() for c in d["K"]]
+        reqs.append(mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, cones, d["s"]))
+    gaps = rng.exponential(1.0 / args.rate, K)
+    server = mh.IPMServer(nslots=args.slots)
+    t0 = mh.clock()
+    arrivals, tickets = [], []
+    for k in range(K):
+        target = t0 + float(np.sum(gaps[:k + 1]))
+        while mh.clock() < target:
+            time.sleep(min(0.002, max(0.0, target - mh.clock())))
+        arrivals.append(mh.clock())
+        tickets.append(server.submit(reqs[k]))
+    out = [server.wait(t) for t in tickets]
+    st = server.stats()
+    server.close()
+    ok = sum(r.status == mh.IPMStatus.OPTIMAL for r, _ in out)
+    fin = np.array([t["finished"] for _, t in out])
+    lat_ms = (fin - np.array([t["submitted"] for _, t in out])) * 1e3
+    elapsed = float(fin.max() - t0)
+    return {
+        "metric": METRIC, "value": K / elapsed, "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"served stream: {K} requests, 70% QP / 30% SOCP, n log-uniform in "
+                               f"[50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, {args.slots} slots",
+                   "optimal": ok, "p50_ms": float(np.percentile(lat_ms, 50)),
+                   "p99_ms": float(np.percentile(lat_ms, 99)), "peak_reserved_gb": st["peak_reserved_bytes"] / 2 ** 30},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--n", type=int, default=2000)
+    ap.add_argument("--m", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", action="store_true")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--serve-nmax", type=int, default=4000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import mumblebee_hip as mh
+    line = bench_serve(args, mh) if args.serve else bench_resident(args, mh)
     print(json.dumps(line))
 
 

@@ -59,6 +59,7 @@ typedef struct mb_settings {
   int32_t amalgamate;   /* relaxed supernode amalgamation (1 default) */
   int32_t precision;    /* 0: FP64 factor */
   int32_t big_front_ld; /* fronts of this order or larger take the persistent kernels (512) */
+  int32_t dag_grid;     /* workgroups of the persistent factorization (0: one per CU) */
 } mb_settings;
 
 /* Fill the reference defaults. */
@@ -141,6 +142,36 @@ int mb_solver_dense_factor(mb_solver* s, double* L);
 
 /* Lane-layout probe of v_mfma_f64_16x16x4f64 (256 doubles out). */
 int mb_probe_mfma_f64(double* out);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving: a continuous-batching request loop on one GPU. No counterpart in the reference; it
+ * serves many independent solve(prob, settings) calls (solver/solver.jl:119-150) at once.
+ * nslots slots (host threads, one HIP stream each) take requests in arrival order; a request is
+ * admitted once a slot is free and its device-memory bound fits mem_budget_bytes (≤ 0: 90 % of
+ * the free device memory) beside the running ones; its kernels then overlap on the device with
+ * the running solves. dag_grid ≤ 0 gives each slot 1/nslots of the CUs for the persistent
+ * factorization. Results are copied to host memory when a solve ends and freed on release.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct mb_server mb_server;
+mb_server* mb_server_create(int32_t nslots, double mem_budget_bytes, int32_t dag_grid);
+/* Finishes the queued and running requests, then frees the server. */
+void mb_server_destroy(mb_server* s);
+/* Queue solve(prob, settings); the problem must stay alive until its result is collected.
+ * Returns a ticket ≥ 0, or −1. */
+int64_t mb_server_submit(mb_server* s, const mb_problem* p, const mb_settings* st);
+/* Wait up to timeout_s (< 0: no limit) for a ticket. Returns its IPMStatus, −2 on timeout, −1 on
+ * error. p, d (length n), y (length m), info[8] as in mb_solver_result; times[4] = submitted,
+ * admitted and finished (mb_clock seconds), device bytes used. Any pointer may be NULL. */
+int mb_server_wait(mb_server* s, int64_t ticket, double timeout_s, double* p, double* d, double* y, double* info,
+                   double* times);
+/* Forget a collected ticket. */
+int mb_server_release(mb_server* s, int64_t ticket);
+/* out[5] = queued, running, completed, device bytes reserved now, peak bytes reserved. */
+int mb_server_stats(mb_server* s, double* out);
+/* Seconds on the steady clock the server's timestamps use. */
+double mb_clock(void);
+/* out[3] = compute units, free and total device bytes of the current device. */
+int mb_device_info(double* out);
 
 const char* mb_last_error(void);
 

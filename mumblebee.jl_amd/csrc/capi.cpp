@@ -6,6 +6,7 @@
 
 #include "../../include/mumblebee_hip.h"
 #include "solver.h"
+#include "server.h"
 
 namespace {
 thread_local std::string g_err;
@@ -62,6 +63,30 @@ static void ensure_symbolic(mb_problem* p, int amalg) {
   p->sym_amalg = amalg;
 }
 
+static mb::Settings to_settings(const mb_settings* s) {
+  mb::Settings st;
+  if (!s) return st;
+  st.verbose = s->verbose;
+  st.step_frac = s->step_frac;
+  st.feas_tol = s->feas_tol;
+  st.gap_tol = s->gap_tol;
+  st.max_iter = s->max_iter;
+  st.near_factor = s->near_factor;
+  st.stall_tol = s->stall_tol;
+  st.refine_stall_tol = s->refine_stall_tol;
+  st.scale_max_iter = s->scale_max_iter;
+  st.refine_max_iter = s->refine_max_iter;
+  st.newton_max_iter = s->newton_max_iter;
+  st.aug_tol = s->aug_tol;
+  st.infeas_abs = s->infeas_abs;
+  st.infeas_rel = s->infeas_rel;
+  st.amalgamate = s->amalgamate;
+  st.big_front_ld = s->big_front_ld > 0 ? s->big_front_ld : mb::BIG_LD;
+  st.dag_grid = s->dag_grid;
+  if (s->precision != 0) throw std::invalid_argument("precision mode not available");
+  return st;
+}
+
 extern "C" {
 
 void mb_settings_default(mb_settings* s) {
@@ -83,6 +108,7 @@ void mb_settings_default(mb_settings* s) {
   s->amalgamate = d.amalgamate;
   s->precision = 0;
   s->big_front_ld = d.big_front_ld;
+  s->dag_grid = d.dag_grid;
 }
 
 mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, const int64_t* Q_rowval,
@@ -205,26 +231,7 @@ mb_solver* mb_solver_create(const mb_problem* p, const mb_settings* s, const dou
                             const double* y0) {
   mb_solver* out = nullptr;
   guard([&]() {
-    mb::Settings st;
-    if (s) {
-      st.verbose = s->verbose;
-      st.step_frac = s->step_frac;
-      st.feas_tol = s->feas_tol;
-      st.gap_tol = s->gap_tol;
-      st.max_iter = s->max_iter;
-      st.near_factor = s->near_factor;
-      st.stall_tol = s->stall_tol;
-      st.refine_stall_tol = s->refine_stall_tol;
-      st.scale_max_iter = s->scale_max_iter;
-      st.refine_max_iter = s->refine_max_iter;
-      st.newton_max_iter = s->newton_max_iter;
-      st.aug_tol = s->aug_tol;
-      st.infeas_abs = s->infeas_abs;
-      st.infeas_rel = s->infeas_rel;
-      st.amalgamate = s->amalgamate;
-      st.big_front_ld = s->big_front_ld > 0 ? s->big_front_ld : mb::BIG_LD;
-      if (s->precision != 0) throw std::invalid_argument("precision mode not available");
-    }
+    mb::Settings st = to_settings(s);
     auto h = std::make_unique<mb_solver>();
     h->S = std::make_unique<mb::Solver>(p->P, st, p0, d0, y0);
     out = h.release();
@@ -326,5 +333,88 @@ int mb_probe_mfma_f64(double* out) {
 }
 
 const char* mb_last_error(void) { return g_err.c_str(); }
+
+
+// ------------------------------------------------------------------------------------------
+// serving
+// ------------------------------------------------------------------------------------------
+
+struct mb_server {
+  std::unique_ptr<mb::Server> S;
+};
+
+mb_server* mb_server_create(int32_t nslots, double mem_budget_bytes, int32_t dag_grid) {
+  mb_server* out = nullptr;
+  guard([&]() {
+    auto h = std::make_unique<mb_server>();
+    h->S = std::make_unique<mb::Server>(nslots, mem_budget_bytes, dag_grid);
+    out = h.release();
+    return 0;
+  });
+  return out;
+}
+
+void mb_server_destroy(mb_server* s) { delete s; }
+
+int64_t mb_server_submit(mb_server* s, const mb_problem* p, const mb_settings* st) {
+  int64_t id = -1;
+  guard([&]() {
+    if (!p) throw std::invalid_argument("null problem");
+    id = s->S->submit(&p->P, to_settings(st));
+    return 0;
+  });
+  return id;
+}
+
+int mb_server_wait(mb_server* s, int64_t ticket, double timeout_s, double* p, double* d, double* y, double* info,
+                   double* times) {
+  return guard([&]() {
+    auto job = s->S->wait(ticket, timeout_s);
+    if (!job) return -2;
+    if (job->status < 0) throw std::runtime_error(job->error.empty() ? "solve failed" : job->error);
+    if (p) std::copy(job->p.begin(), job->p.end(), p);
+    if (d) std::copy(job->d.begin(), job->d.end(), d);
+    if (y) std::copy(job->y.begin(), job->y.end(), y);
+    if (info) std::copy(job->info, job->info + 8, info);
+    if (times) {
+      times[0] = job->t_submit;
+      times[1] = job->t_admit;
+      times[2] = job->t_end;
+      times[3] = job->bytes;
+    }
+    return job->status;
+  });
+}
+
+int mb_server_release(mb_server* s, int64_t ticket) {
+  return guard([&]() {
+    s->S->release(ticket);
+    return 0;
+  });
+}
+
+int mb_server_stats(mb_server* s, double* out) {
+  return guard([&]() {
+    s->S->stats(out);
+    return 0;
+  });
+}
+
+double mb_clock(void) { return mb::Server::now(); }
+
+int mb_device_info(double* out) {
+  return guard([&]() {
+    int dev = 0, ncu = 0;
+    size_t free_b = 0, total_b = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+      throw std::runtime_error("no HIP device");
+    out[0] = ncu;
+    out[1] = (double)free_b;
+    out[2] = (double)total_b;
+    return 0;
+  });
+}
 
 }  // extern "C"

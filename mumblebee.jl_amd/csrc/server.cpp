@@ -1,0 +1,140 @@
+#include "server.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+
+namespace mb {
+
+double Server::now() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+Server::Server(int nslots, double mem_budget_bytes, int dag_grid)
+    : nslots_(std::max(1, nslots)), dag_grid_(dag_grid), budget_(mem_budget_bytes) {
+  if (budget_ <= 0.0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
+    budget_ = 0.9 * (double)free_b;
+  }
+  if (dag_grid_ <= 0) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+    dag_grid_ = std::max(16, ncu / nslots_);
+  }
+  for (int s = 0; s < nslots_; ++s) slots_.emplace_back(&Server::slot_main, this, s);
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  cv_mem_.notify_all();
+  for (auto& t : slots_) t.join();
+}
+
+int64_t Server::submit(const Problem* prob, const Settings& st) {
+  auto job = std::make_shared<ServeJob>();
+  job->prob = prob;
+  job->set = st;
+  job->set.dag_grid = dag_grid_;
+  job->t_submit = now();
+  int64_t id;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (stop_) throw std::runtime_error("server is shutting down");
+    id = job->id = next_id_++;
+    jobs_[id] = job;
+    queue_.push_back(job);
+  }
+  cv_work_.notify_one();
+  return id;
+}
+
+std::shared_ptr<ServeJob> Server::wait(int64_t id, double timeout_s) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto it = jobs_.find(id);
+  if (it == jobs_.end()) throw std::invalid_argument("unknown ticket");
+  auto job = it->second;
+  auto ready = [&] { return job->done; };
+  if (timeout_s < 0) cv_done_.wait(lk, ready);
+  else if (!cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), ready))
+    return nullptr;
+  return job;
+}
+
+void Server::release(int64_t id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  jobs_.erase(id);
+}
+
+void Server::stats(double* out) {
+  std::lock_guard<std::mutex> lk(mu_);
+  out[0] = (double)queue_.size();
+  out[1] = (double)running_;
+  out[2] = (double)completed_;
+  out[3] = reserved_;
+  out[4] = peak_;
+}
+
+void Server::slot_main(int slot) {
+  (void)slot;
+  for (;;) {
+    std::shared_ptr<ServeJob> job;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_work_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+      if (queue_.empty()) return;  // stop_ with nothing left
+      job = queue_.front();
+      queue_.pop_front();
+      ++running_;
+    }
+    double reserve = 0.0;
+    try {
+      Solver S(*job->prob, job->set, Solver::Deferred{});
+      reserve = S.estimate_device_bytes();
+      {
+        // admission: the bound must fit what the running solves leave of the budget; a request
+        // larger than the whole budget runs alone
+        std::unique_lock<std::mutex> lk(mu_);
+        if (reserve > budget_ && running_ > 1)
+          cv_mem_.wait(lk, [&] { return reserved_ == 0.0; });
+        else
+          cv_mem_.wait(lk, [&] { return reserved_ + reserve <= budget_; });
+        reserved_ += reserve;
+        peak_ = std::max(peak_, reserved_);
+      }
+      job->t_admit = now();
+      S.device_setup(nullptr, nullptr, nullptr);
+      job->bytes = (double)S.device_bytes();
+      job->status = S.solve();
+      job->p.resize(S.n());
+      job->d.resize(S.n());
+      job->y.resize(S.m());
+      S.result(job->p.data(), job->d.data(), job->y.data(), job->info);
+    } catch (const std::exception& e) {
+      job->error = e.what();
+      job->status = -1;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      reserved_ -= reserve;
+      if (reserved_ < 1.0) reserved_ = 0.0;
+      --running_;
+      ++completed_;
+      job->t_end = now();
+      job->done = true;
+    }
+    cv_mem_.notify_all();
+    cv_done_.notify_all();
+  }
+}
+
+}  // namespace mb

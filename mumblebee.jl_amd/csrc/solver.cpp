@@ -21,6 +21,9 @@ static const double FORCING_FRAC = 0.1;  // solver/solver.jl:3
                                                    hipGetErrorString(e_) + " at " #x);     \
   } while (0)
 
+// device bytes allocated through DevBuf by the solver being built on this thread
+static thread_local size_t* g_dev_counter = nullptr;
+
 DevBuf::~DevBuf() {
   if (p) (void)hipFree(p);
 }
@@ -31,6 +34,7 @@ void DevBuf::alloc(size_t b) {
   }
   bytes = b;
   if (b) HIPCHK(hipMalloc(&p, b));
+  if (g_dev_counter) *g_dev_counter += b;
 }
 template <class T>
 void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
@@ -138,6 +142,13 @@ static int cone_degree(const Cone& c, i64 n) {
 
 Solver::Solver(const Problem& prob, const Settings& st, const double* p0, const double* d0, const double* y0)
     : set_(st) {
+  host_setup(prob);
+  device_setup(p0, d0, y0);
+}
+
+Solver::Solver(const Problem& prob, const Settings& st, Deferred) : set_(st) { host_setup(prob); }
+
+void Solver::host_setup(const Problem& prob) {
   for (const Cone& c : prob.K)
     if (c.kind != CONE_POS && c.kind != CONE_FREE && c.kind != CONE_SOC)
       throw std::invalid_argument("cone kind not supported by this build's device path");
@@ -179,12 +190,45 @@ Solver::Solver(const Problem& prob, const Settings& st, const double* p0, const 
   ng_ = norm2(P_.g);
   sg_ = scalenorm(P_.g, yscl_);
   sf_ = scalenorm(P_.f, pscl_);
-  HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-  HIPCHK(hipHostMalloc((void**)&hsc_, NSCALAR * sizeof(double), hipHostMallocDefault));
-  HIPCHK(hipEventCreate(&ev0_));
-  HIPCHK(hipEventCreate(&ev1_));
-  build_device();
-  build_schedule();
+}
+
+// An upper bound of what device_setup allocates, from the symbolic structure alone: frontal
+// storage, inverted diagonal blocks, vector arenas, CSR copies of B, Bᵀ, Q, cone blocks, and the
+// assembly map (at most one entry per frontal position).
+double Solver::estimate_device_bytes() const {
+  double fr = 0.0, dinv = 0.0;
+  for (i64 f = 0; f < sym_.nfront(); ++f) {
+    double r = (double)(sym_.xcol[sym_.front_vtx[f + 1]] - sym_.xcol[sym_.front_vtx[f]]);
+    double sc = 0.0;
+    for (i64 u : sym_.front_sep[f]) sc += (double)(sym_.xcol[u + 1] - sym_.xcol[u]);
+    fr += (r + sc) * (r + sc);
+    dinv += std::ceil(r / NB) * NB * NB;
+  }
+  double cone = 0.0;
+  for (i64 v = 0; v < P_.B.nvtx; ++v) {
+    double c = (double)P_.B.ncols(v);
+    cone += c * c;
+  }
+  double nnz = 2.0 * (double)P_.B.val.size() + (double)P_.Q.val.size();
+  double b = 8.0 * fr * 1.5 + 8.0 * dinv + 8.0 * (NV_COUNT * (double)n_ + MV_COUNT * (double)m_) + 12.0 * nnz +
+             8.0 * 3.0 * cone + 24.0 * (double)(n_ + m_);
+  return 1.1 * b + (4 << 20);
+}
+
+void Solver::device_setup(const double* p0, const double* d0, const double* y0) {
+  g_dev_counter = &dev_bytes_;
+  try {
+    HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void**)&hsc_, NSCALAR * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipEventCreate(&ev0_));
+    HIPCHK(hipEventCreate(&ev1_));
+    build_device();
+    build_schedule();
+  } catch (...) {
+    g_dev_counter = nullptr;
+    throw;
+  }
+  g_dev_counter = nullptr;
   reinit(p0, d0, y0);
 }
 
@@ -485,7 +529,9 @@ void Solver::build_schedule() {
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 256;
-    dag_grid_ = ncu;  // the DAG kernel runs at one workgroup per CU (register bound)
+    // the DAG kernel runs at most one workgroup per CU (register bound); a serving slot may
+    // take a share of the CUs so that several instances' factorizations run side by side
+    dag_grid_ = set_.dag_grid > 0 ? std::min(set_.dag_grid, ncu) : ncu;
   }
   for (int l = 0; l < H; ++l) {
     Level& L = levels_[l];

@@ -33,6 +33,7 @@ struct Settings {
   double infeas_rel = 1e-8;
   int amalgamate = 1;
   int big_front_ld = BIG_LD;  // fronts with ld ≥ this take the persistent kernels
+  int dag_grid = 0;           // workgroups of the persistent factorization (0: one per CU)
 };
 
 struct HistRow {
@@ -82,7 +83,13 @@ struct Level {
 
 class Solver {
  public:
+  struct Deferred {};
   Solver(const Problem& prob, const Settings& st, const double* p0, const double* d0, const double* y0);
+  // host analysis only (symbolic factor, permutation, equilibration); device_setup() completes it
+  Solver(const Problem& prob, const Settings& st, Deferred);
+  void device_setup(const double* p0, const double* d0, const double* y0);
+  double estimate_device_bytes() const;     // upper bound of device_setup's allocations
+  size_t device_bytes() const { return dev_bytes_; }  // what device_setup allocated
   ~Solver();
   int step();
   int solve();
@@ -116,6 +123,7 @@ class Solver {
 
  private:
   // setup
+  void host_setup(const Problem& prob);
   void build_device();
   void build_schedule();
   // device ops
@@ -208,7 +216,8 @@ class Solver {
   DevBuf d_bigtick_, d_bigstick_;
   int epoch_ = 0;
   int dag_grid_ = 512;
-  int fac_epoch_ = 0;  // factorization counter: a large front's tile flags hold it once final
+  int fac_epoch_ = 0;     // factorization counter: a large front's tile flags hold it once final
+  size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
   void check_abort();
 };
 

@@ -27,11 +27,14 @@ LIBPATH = os.path.join(_HERE, "libmumblebee_hip.so")
 _lib = None
 _live_solvers = weakref.WeakSet()
 _live_problems = weakref.WeakSet()
+_live_servers = weakref.WeakSet()
 
 
 @atexit.register
 def _release_all():
     """Free device state while the HIP runtime is still up (before its own exit handlers)."""
+    for s in list(_live_servers):
+        s.close()
     for s in list(_live_solvers):
         s.close()
     for p in list(_live_problems):
@@ -108,7 +111,7 @@ class _Settings(ctypes.Structure):
                 ("newton_max_iter", ctypes.c_int32), ("aug_tol", ctypes.c_double),
                 ("infeas_abs", ctypes.c_double), ("infeas_rel", ctypes.c_double),
                 ("amalgamate", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("big_front_ld", ctypes.c_int32)]
+                ("big_front_ld", ctypes.c_int32), ("dag_grid", ctypes.c_int32)]
 
 
 EXPORTS = {
@@ -136,6 +139,14 @@ EXPORTS = {
     "mb_solver_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mb_solver_profile": (ctypes.c_int, [ctypes.c_void_p, _DP, ctypes.c_int32]),
     "mb_solver_dag_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I64P, ctypes.c_int32]),
+    "mb_server_create": (ctypes.c_void_p, [ctypes.c_int32, ctypes.c_double, ctypes.c_int32]),
+    "mb_server_destroy": (None, [ctypes.c_void_p]),
+    "mb_server_submit": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(_Settings)]),
+    "mb_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, _DP, _DP, _DP, _DP, _DP]),
+    "mb_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "mb_server_stats": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_clock": (ctypes.c_double, []),
+    "mb_device_info": (ctypes.c_int, [_DP]),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
@@ -196,6 +207,7 @@ class IPMSettings:
     amalgamate: int = 1
     precision: int = 0
     big_front_ld: int = 512
+    dag_grid: int = 0
 
     def _c(self):
         s = _Settings()
@@ -442,6 +454,82 @@ def step(solver):
 def solve(prob, settings=None, p0=None, d0=None, y0=None):
     """CommonSolve.solve(prob, settings; p0, d0, y0) (solver/ipm.jl:798-804)."""
     return init(prob, settings, p0, d0, y0).solve()
+
+
+class IPMServer:
+    """Continuous-batching request loop on one GPU (no counterpart in the reference; each request
+    is a solve(prob, settings), solver/solver.jl:119-150). `nslots` slots, each a host thread with
+    its own HIP stream, take requests in arrival order and admit them as slots and device memory
+    (`mem_budget_gb`, default 90 % of the free memory) free up; their kernels overlap on the GPU.
+    `dag_grid` workgroups per slot for the persistent factorization (0: the CUs split evenly)."""
+
+    def __init__(self, nslots=8, mem_budget_gb=0.0, dag_grid=0):
+        h = lib().mb_server_create(int(nslots), float(mem_budget_gb) * 2.0 ** 30, int(dag_grid))
+        if not h:
+            raise _err("server")
+        self._h = ctypes.c_void_p(h)
+        self._probs = {}
+        _live_servers.add(self)
+
+    def submit(self, prob, settings=None):
+        """Queue a solve; returns its ticket. The problem is kept alive until collected."""
+        cs = (settings or IPMSettings())._c()
+        t = lib().mb_server_submit(self._h, prob._h, ctypes.byref(cs))
+        if t < 0:
+            raise _err("submit")
+        self._probs[int(t)] = prob
+        return int(t)
+
+    def wait(self, ticket, timeout=None):
+        """Block until the ticket's solve ends (or `timeout` seconds pass: returns None). Returns
+        (IPMResult without history, dict of submitted/admitted/finished clock times and device
+        bytes)."""
+        prob = self._probs[ticket]
+        n, m = prob.n, prob.m
+        p, d, y, info, tm = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(8), np.zeros(4)
+        r = lib().mb_server_wait(self._h, int(ticket), -1.0 if timeout is None else float(timeout),
+                                 _p(p, ctypes.c_double), _p(d, ctypes.c_double), _p(y, ctypes.c_double),
+                                 _p(info, ctypes.c_double), _p(tm, ctypes.c_double))
+        if r == -2:
+            return None
+        if r < 0:
+            err = _err("serve")
+            self.release(ticket)
+            raise err
+        self.release(ticket)
+        res = IPMResult(p, d, y, IPMStatus(int(info[7])), int(info[5]), int(info[6]), [],
+                        info[0], info[1], info[2], info[3], info[4])
+        return res, dict(submitted=tm[0], admitted=tm[1], finished=tm[2], device_bytes=tm[3])
+
+    def release(self, ticket):
+        lib().mb_server_release(self._h, int(ticket))
+        self._probs.pop(int(ticket), None)
+
+    def stats(self):
+        out = np.zeros(5)
+        if lib().mb_server_stats(self._h, _p(out, ctypes.c_double)) != 0:
+            raise _err("stats")
+        return dict(queued=int(out[0]), running=int(out[1]), completed=int(out[2]), reserved_bytes=out[3],
+                    peak_reserved_bytes=out[4])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mb_server_destroy(self._h)
+            self._h = None
+            self._probs.clear()
+
+
+def clock():
+    """Seconds on the steady clock of the server's timestamps."""
+    return lib().mb_clock()
+
+
+def device_info():
+    """Compute units, free and total bytes of the current device."""
+    out = np.zeros(3)
+    if lib().mb_device_info(_p(out, ctypes.c_double)) != 0:
+        raise _err("device_info")
+    return dict(cus=int(out[0]), free_bytes=out[1], total_bytes=out[2])
 
 
 def probe_mfma_f64():

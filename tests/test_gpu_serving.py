@@ -1,0 +1,82 @@
+"""Serving: requests solved through the continuous-batching loop give the same results as solo
+solves, admission keeps the reserved device memory within the budget, and a failing request
+reports its error without disturbing the others."""
+import numpy as np
+import pytest
+
+import mumblebee_hip as mh
+from mumblebee_hip import generators as G
+from helpers import hip_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    return [
+        G.qp(n=600, m=300, seed=101),
+        G.socp(n=500, m=250, ncone=20, seed=102, adens=0.02),
+        G.qp(n=150, m=60, seed=103, pdens=0.03, adens=0.05),
+        G.qp(n=1100, m=500, seed=104),
+        G.socp(n=300, m=120, ncone=10, seed=105, adens=0.03),
+        G.qp(n=400, m=200, seed=106),
+    ]
+
+
+def _close(a, b):
+    assert a.status == b.status == mh.IPMStatus.OPTIMAL
+    assert abs(a.niter - b.niter) <= 1
+    assert rel(a.pobj, b.pobj) < 1e-8 and rel(a.dobj, b.dobj) < 1e-8
+    assert np.linalg.norm(a.p - b.p) <= 1e-6 * (1.0 + np.linalg.norm(b.p))
+    assert np.linalg.norm(a.y - b.y) <= 1e-6 * (1.0 + np.linalg.norm(b.y))
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix():
+    data = _mix()
+    probs = [hip_problem(d) for d in data]
+    solo = [mh.solve(p) for p in probs]
+    server = mh.IPMServer(nslots=4)
+    try:
+        tickets = [server.submit(p) for p in probs]
+        served = [server.wait(t) for t in tickets]
+        st = server.stats()
+    finally:
+        server.close()
+    assert st["completed"] == len(probs) and st["running"] == 0 and st["queued"] == 0
+    for (res, times), ref in zip(served, solo):
+        _close(res, ref)
+        assert times["submitted"] <= times["admitted"] <= times["finished"]
+        assert times["device_bytes"] > 0
+
+
+def test_admission_keeps_reservations_within_the_budget():
+    data = [G.qp(n=700, m=300, seed=200 + k) for k in range(6)]
+    probs = [hip_problem(d) for d in data]
+    budget_gb = 0.03  # about two of these instances at a time
+    server = mh.IPMServer(nslots=6, mem_budget_gb=budget_gb)
+    try:
+        tickets = [server.submit(p) for p in probs]
+        out = [server.wait(t) for t in tickets]
+        st = server.stats()
+    finally:
+        server.close()
+    assert all(r.status == mh.IPMStatus.OPTIMAL for r, _ in out)
+    assert 0 < st["peak_reserved_bytes"] <= budget_gb * 2.0 ** 30
+    # admission made some requests wait for memory rather than for a slot
+    waits = sorted(t["admitted"] - t["submitted"] for _, t in out)
+    assert waits[-1] > waits[0]
+
+
+def test_a_failing_request_reports_its_error():
+    bad = G.qp(n=105, m=40, seed=300)  # one 14×14 PSD block in svec form (14·15/2 = 105)
+    bad_prob = mh.IPMProblem(bad["Q"], bad["B"], bad["f"], bad["g"], 0.0, [mh.SemidefiniteCone()], [105])
+    good = hip_problem(G.qp(n=200, m=80, seed=301))
+    server = mh.IPMServer(nslots=2)
+    try:
+        tb = server.submit(bad_prob)
+        tg = server.submit(good)
+        with pytest.raises(RuntimeError, match="cone kind"):
+            server.wait(tb)
+        res, _ = server.wait(tg)
+    finally:
+        server.close()
+    assert res.status == mh.IPMStatus.OPTIMAL
