@@ -7,6 +7,7 @@
 // on the quotient graph (ties to the smallest vertex), elimination-tree postorder, fundamental
 // supernodes, then relaxed amalgamation so that fronts are dense enough for MFMA tiles.
 #include <algorithm>
+#include <cstdint>
 #include <set>
 #include <stdexcept>
 
@@ -147,6 +148,87 @@ struct MinDegree {
   }
 };
 
+// The same ordering with the reach sets kept explicitly as bitsets. Eliminating p changes the
+// reach of exactly the vertices i ∈ L_p = reach(p), to (reach(i) ∪ L_p) \ {i, p}; every other
+// reach is unchanged (a vertex outside L_p shares no element or edge with p). So each exact
+// external degree is updated by the weights of the few new members instead of recomputing a
+// union over the quotient graph, and the elimination decisions are identical.
+struct MinDegreeBits {
+  i64 N, nw;
+  const std::vector<i64>& w;
+  std::vector<uint64_t> R;  // N rows of nw words
+  std::vector<i64> deg;
+
+  MinDegreeBits(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj)
+      : N((i64)weights.size()), nw(((i64)weights.size() + 63) / 64), w(weights),
+        R((size_t)((i64)weights.size() * (((i64)weights.size() + 63) / 64)), 0), deg(weights.size(), 0) {
+    for (i64 i = 0; i < N; ++i)
+      for (i64 j : adj[i]) {
+        R[(size_t)(i * nw + j / 64)] |= 1ull << (j % 64);
+        deg[i] += w[j];
+      }
+  }
+
+  void members(i64 i, std::vector<i64>& out) const {
+    out.clear();
+    const uint64_t* r = &R[(size_t)(i * nw)];
+    for (i64 k = 0; k < nw; ++k)
+      for (uint64_t b = r[k]; b; b &= b - 1) out.push_back(k * 64 + __builtin_ctzll(b));
+  }
+
+  void run(std::vector<i64>& order, std::vector<std::vector<i64>>& strct) {
+    order.clear();
+    strct.assign(N, {});
+    i64 W = 0;
+    for (i64 x : w) W += x;
+    std::set<std::pair<i64, i64>> pq;
+    for (i64 i = 0; i < N; ++i) pq.insert({deg[i], i});
+    std::vector<i64> Lp;
+    std::vector<uint64_t> lp(nw);
+    while (!pq.empty()) {
+      i64 p = pq.begin()->second;
+      pq.erase(pq.begin());
+      members(p, Lp);  // sorted
+      if (deg[p] + w[p] == W) {
+        // p reaches every remaining variable: after it the rest is a clique, where minimum
+        // degree picks by (-weight, index)
+        order.push_back(p);
+        strct[p] = Lp;
+        std::vector<i64> rest = Lp;
+        std::stable_sort(rest.begin(), rest.end(), [&](i64 a, i64 b) {
+          if (w[a] != w[b]) return w[a] > w[b];
+          return a < b;
+        });
+        for (size_t k = 0; k < rest.size(); ++k) {
+          order.push_back(rest[k]);
+          std::vector<i64> s(rest.begin() + k + 1, rest.end());
+          std::sort(s.begin(), s.end());
+          strct[rest[k]] = s;
+        }
+        break;
+      }
+      strct[p] = Lp;
+      order.push_back(p);
+      W -= w[p];
+      std::copy(&R[(size_t)(p * nw)], &R[(size_t)(p * nw)] + nw, lp.begin());
+      for (i64 i : Lp) {
+        uint64_t* r = &R[(size_t)(i * nw)];
+        i64 d = deg[i] - w[p];
+        for (i64 k = 0; k < nw; ++k) {
+          uint64_t nb = lp[k] & ~r[k];
+          if (k == i / 64) nb &= ~(1ull << (i % 64));
+          for (uint64_t b = nb; b; b &= b - 1) d += w[k * 64 + __builtin_ctzll(b)];
+          r[k] |= nb;
+        }
+        r[p / 64] &= ~(1ull << (p % 64));
+        pq.erase({deg[i], i});
+        deg[i] = d;
+        pq.insert({deg[i], i});
+      }
+    }
+  }
+};
+
 }  // namespace
 
 Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj,
@@ -154,7 +236,10 @@ Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<
   const i64 N = (i64)weights.size();
   std::vector<i64> order;
   std::vector<std::vector<i64>> strct;
-  {
+  if (N <= 16384) {  // N² bits stay within 32 MB
+    MinDegreeBits md(weights, adj);
+    md.run(order, strct);
+  } else {
     MinDegree md(weights, adj);
     md.run(order, strct);
   }
