@@ -55,14 +55,26 @@ void launch_dot(hipStream_t, const InstDev*, const Tile*, int nt, const int* til
                 int kx, int ky, int mode, int kw0, int kw1, double* partial, int slot, int op);
 void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
 void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky, Coef a);
+// psd / dense: lists of the PSD cones and of the cones with dense scaling blocks (SOC, PSD)
 void launch_cone_scale(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                       double* out);
+                       double* out, const int* psd, int npsd);
 void launch_cone_hnorm2(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                        double* out, int slot);
+                        double* out, int slot, const int* dense, int ndense);
 void launch_cone_corr0(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr);
 void launch_cone_corr(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr, int kdp, int kdd);
 void launch_cone_maxstep(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                         int kdp, int kdd, double step_frac, double* out);
+                         int kdp, int kdd, double step_frac, double* out, const int* psd, int npsd);
+
+// PSD cones (order d ≤ PSD_MAX) and dense-block cones: one workgroup per listed cone (psd.hip)
+constexpr int PSD_MAX = 48;
+void launch_psd_scale(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, double* out);
+void launch_psd_corr0(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kr);
+void launch_psd_corr(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kr, int kdp, int kdd);
+void launch_psd_maxstep(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kdp, int kdd,
+                        double step_frac, double* out);
+void launch_cone_matvec_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kx, int ky,
+                              Coef a);
+void launch_cone_hnorm2_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, double* out);
 void launch_zerofree(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx);
 void launch_fill(hipStream_t, double* x, size_t n, double v);
 void launch_assemble_union(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* lval,

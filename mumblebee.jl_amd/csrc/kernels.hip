@@ -184,15 +184,8 @@ __global__ void k_cone_matvec(const InstDev* insts, const ConeDesc* cones, int n
   const double* x = nvec(I, kx) + C.col0;
   double* y = nvec(I, ky) + C.col0;
   double av = coefval(a, I.sc);
-  if (C.kind == CONE_POS) {
+  if (C.kind == CONE_POS)  // dense blocks (second-order, PSD): k_cone_matvec_dense
     for (int i = 0; i < C.n; ++i) y[i] += av * C.D[i] * x[i];
-  } else if (C.kind == CONE_SOC) {
-    for (int i = 0; i < C.n; ++i) {
-      double acc = 0.0;
-      for (int j = 0; j < C.n; ++j) acc += C.D[i + (size_t)j * C.n] * x[j];
-      y[i] += av * acc;
-    }
-  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -257,6 +250,7 @@ __global__ void k_cone_scale(const InstDev* insts, const ConeDesc* cones, int nc
   ConeDesc C = cones[c];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
+  if (C.kind == CONE_PSD) return;  // k_psd_scale
   const double* p = nvec(I, NV_P) + C.col0;
   const double* d = nvec(I, NV_D) + C.col0;
   int n = C.n;
@@ -316,13 +310,13 @@ __global__ void k_cone_hnorm2(const InstDev* insts, const ConeDesc* cones, int n
   ConeDesc C = cones[c];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
+  if (C.kind == CONE_SOC || C.kind == CONE_PSD) return;  // k_cone_hnorm2_dense
   int n = C.n;
   double s = 0.0;
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) {
       double h = C.Qd[i + (size_t)j * n];
       if (C.kind == CONE_POS && i == j) h += C.D[i];
-      else if (C.kind == CONE_SOC) h += C.D[i + (size_t)j * n];
       s += h * h;
     }
   out[c] = s;
@@ -386,6 +380,7 @@ __global__ void k_cone_corr0(const InstDev* insts, const ConeDesc* cones, int nc
   ConeDesc C = cones[c];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
+  if (C.kind == CONE_PSD) return;  // k_psd_corr0
   const double* p = nvec(I, NV_P) + C.col0;
   const double* d = nvec(I, NV_D) + C.col0;
   double* r = nvec(I, kr) + C.col0;
@@ -512,6 +507,7 @@ __global__ void k_cone_maxstep(const InstDev* insts, const ConeDesc* cones, int 
   if (I.sc[S_ACTIVE] == 0.0) return;
   const double* p = nvec(I, NV_P) + C.col0;
   const double* d = nvec(I, NV_D) + C.col0;
+  if (C.kind == CONE_PSD) return;  // k_psd_maxstep
   const double* dp = nvec(I, kdp) + C.col0;
   const double* dd = nvec(I, kdd) + C.col0;
   double tp = 1.0, td = 1.0;
@@ -987,13 +983,15 @@ void launch_cone_matvec(hipStream_t st, const InstDev* I, const ConeDesc* C, int
   if (nc) k_cone_matvec<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx, ky, a);
 }
 void launch_cone_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                       int ninst, double* out) {
+                       int ninst, double* out, const int* psd, int npsd) {
   if (nc) k_cone_scale<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
+  launch_psd_scale(st, I, C, psd, npsd, out);
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 0);
 }
 void launch_cone_hnorm2(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                        int ninst, double* out, int slot) {
+                        int ninst, double* out, int slot, const int* dense, int ndense) {
   if (nc) k_cone_hnorm2<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
+  launch_cone_hnorm2_dense(st, I, C, dense, ndense, out);
   k_cone_sum<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, slot);
 }
 void launch_cone_corr0(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kr) {
@@ -1003,8 +1001,9 @@ void launch_cone_corr(hipStream_t st, const InstDev* I, const ConeDesc* C, int n
   if (nc) k_cone_corr<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kr, kdp, kdd);
 }
 void launch_cone_maxstep(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                         int ninst, int kdp, int kdd, double step_frac, double* out) {
+                         int ninst, int kdp, int kdd, double step_frac, double* out, const int* psd, int npsd) {
   if (nc) k_cone_maxstep<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kdp, kdd, step_frac, out);
+  launch_psd_maxstep(st, I, C, psd, npsd, kdp, kdd, step_frac, out);
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 1);
 }
 void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx) {

@@ -150,8 +150,14 @@ Solver::Solver(const Problem& prob, const Settings& st, Deferred) : set_(st) { h
 
 void Solver::host_setup(const Problem& prob) {
   for (const Cone& c : prob.K)
-    if (c.kind != CONE_POS && c.kind != CONE_FREE && c.kind != CONE_SOC)
+    if (c.kind != CONE_POS && c.kind != CONE_FREE && c.kind != CONE_SOC && c.kind != CONE_PSD)
       throw std::invalid_argument("cone kind not supported by this build's device path");
+  for (i64 v = 0; v < prob.B.nvtx; ++v)
+    if (prob.K[v].kind == CONE_PSD) {
+      i64 nc = prob.B.ncols(v), d = cone_degree(prob.K[v], nc);
+      if (d * (d + 1) / 2 != nc) throw std::invalid_argument("PSD cone size is not d(d+1)/2");
+      if (d > PSD_MAX) throw std::invalid_argument("PSD cone larger than the device path's block limit");
+    }
   n_ = prob.B.ncol();
   m_ = prob.B.nrow();
   nu_ = 0;
@@ -384,8 +390,9 @@ void Solver::build_device() {
   for (i64 v = 0; v < N; ++v) {
     i64 nc = P_.B.ncols(v);
     int k = P_.K[v].kind;
-    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : (k == CONE_SOC ? nc * nc : 0));
-    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : 0);
+    const i64 d = k == CONE_PSD ? cone_degree(P_.K[v], nc) : 0;
+    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : ((k == CONE_SOC || k == CONE_PSD) ? nc * nc : 0));
+    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : (k == CONE_PSD ? 2 * d * d + d : 0));
     qofs[v + 1] = qofs[v] + nc * nc;
   }
   std::vector<double> qdiag(qofs[N], 0.0);
@@ -435,6 +442,17 @@ void Solver::build_device() {
     C.Qd = d_qdiag_.as<double>() + qofs[v];
   }
   d_cones_.upload(cones_, st_);
+  {
+    std::vector<int> psd, dense;
+    for (i64 v = 0; v < N; ++v) {
+      if (cones_[v].kind == CONE_PSD) psd.push_back((int)v);
+      if (cones_[v].kind == CONE_PSD || cones_[v].kind == CONE_SOC) dense.push_back((int)v);
+    }
+    npsd_ = (int)psd.size();
+    ndense_ = (int)dense.size();
+    if (npsd_) d_psdlist_.upload(psd, st_);
+    if (ndense_) d_denselist_.upload(dense, st_);
+  }
   d_cone_front_.upload(cone_front_, st_);
   d_cone_ofs_.upload(std::vector<int>{0, (int)N}, st_);
   // sparse matrices
@@ -711,6 +729,8 @@ void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
 void Solver::hmul(int kx, int ky) {
   spmv(2, kx, ky, K(1.0), K(0.0));
   launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, ky, K(1.0));
+  launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_, kx,
+                           ky, K(1.0));
 }
 
 void Solver::lsolve(int kv) {
@@ -895,7 +915,7 @@ int Solver::kkt_factor(double delta, double* rho) {
 
 double Solver::maxsteps(int kdp, int kdd, double step_frac) {
   launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
-                      kdd, step_frac, d_coneout_.as<double>());
+                      kdd, step_frac, d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
   sync_scalars();
   return hsc_[S_TAU];
 }
@@ -912,7 +932,51 @@ static void host_identity(double* x, const Cone& c, i64 n) {
   } else if (c.kind == CONE_SOC) {
     for (i64 i = 0; i < n; ++i) x[i] = 0.0;
     x[0] = std::sqrt(2.0);
+  } else if (c.kind == CONE_PSD) {  // svec(I) (sdp.jl:92-111)
+    i64 d = cone_degree(c, n), k = 0;
+    for (i64 i = 0; i < n; ++i) x[i] = 0.0;
+    for (i64 j = 0; j < d; ++j) {
+      x[k] = 1.0;
+      k += d - j;
+    }
   }
+}
+
+// X⁻¹ in svec form through the Cholesky factor of X (sdp.jl:666-678); false if X is not
+// positive definite
+static bool psd_shadow(double* s, const double* x, i64 n, i64 d) {
+  const double r2 = std::sqrt(2.0);
+  std::vector<double> L(d * d, 0.0), Z(d * d, 0.0);
+  for (i64 j = 0, k = 0; j < d; ++j)
+    for (i64 i = j; i < d; ++i, ++k) L[i + j * d] = i == j ? x[k] : x[k] / r2;
+  for (i64 j = 0; j < d; ++j) {
+    double p = L[j + j * d];
+    for (i64 k = 0; k < j; ++k) p -= L[j + k * d] * L[j + k * d];
+    if (!(p > 0.0)) return false;
+    L[j + j * d] = std::sqrt(p);
+    for (i64 i = j + 1; i < d; ++i) {
+      double a = L[i + j * d];
+      for (i64 k = 0; k < j; ++k) a -= L[i + k * d] * L[j + k * d];
+      L[i + j * d] = a / L[j + j * d];
+    }
+  }
+  for (i64 c = 0; c < d; ++c) {  // Z = L⁻ᵀ L⁻¹ e_c
+    std::vector<double> y(d, 0.0);
+    for (i64 i = 0; i < d; ++i) {
+      double a = i == c ? 1.0 : 0.0;
+      for (i64 k = 0; k < i; ++k) a -= L[i + k * d] * y[k];
+      y[i] = a / L[i + i * d];
+    }
+    for (i64 i = d - 1; i >= 0; --i) {
+      double a = y[i];
+      for (i64 k = i + 1; k < d; ++k) a -= L[k + i * d] * Z[k + c * d];
+      Z[i + c * d] = a / L[i + i * d];
+    }
+  }
+  for (i64 j = 0, k = 0; j < d; ++j)
+    for (i64 i = j; i < d; ++i, ++k) s[k] = i == j ? Z[i + j * d] : r2 * Z[i + j * d];
+  (void)n;
+  return true;
 }
 
 static bool host_shadow(double* s, const double* x, const Cone& c, i64 n) {
@@ -924,6 +988,7 @@ static bool host_shadow(double* s, const double* x, const Cone& c, i64 n) {
     for (i64 i = 0; i < n; ++i) s[i] = 0.0;
     return true;
   }
+  if (c.kind == CONE_PSD) return psd_shadow(s, x, n, cone_degree(c, n));
   // SOC: s = 2 Jx / det(x) with the compensated determinant (soc.jl:488-504)
   double xdet = x[0] * x[0];
   double corr = std::fma(x[0], x[0], -xdet);
@@ -1241,9 +1306,9 @@ int Solver::step() {
   // pᵀd, scale!, residuals!
   dot(0, NV_P, NV_D, 0, S_TMP0, 0);
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                    d_coneout_.as<double>());
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1);
+                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_);
   spmv(2, NV_P, NV_QP, K(1.0), K(0.0));                                  // Qp
   spmv(1, MV_Y, NV_T0, K(1.0), K(0.0));                                  // Bᵀy
   axpbypcz(0, NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0));        // Δf = d + f − Qp
@@ -1322,6 +1387,7 @@ int Solver::step() {
         if (mut > 0) {
           set_scalar(S_SMU, mut);
           launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
+          launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
         } else {
           axpby(0, NV_D, NV_D, NV_FR, K(-1.0), K(0.0));
         }
@@ -1352,6 +1418,8 @@ int Solver::step() {
         (void)taua;
         set_scalar(S_SMU, smu);
         launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
+        launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
+                        NV_DDA);
         axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
         axpby(0, NV_DPA, NV_DPA, NV_DP, K(1.0), K(0.0));
         axpby(1, MV_DYA, MV_DYA, MV_DY, K(1.0), K(0.0));

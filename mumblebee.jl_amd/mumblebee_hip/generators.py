@@ -18,6 +18,17 @@ Workloads:
             objective is made dual-feasible (an objective drawn N(0,1) makes the problem
             unbounded with high probability when n = 2m): f = −(Bᵀy0 + s0) with y0 ~ N(0,1)
             and s0 a point of int K* built from N(0,1) draws.
+  maxcut    max-cut SDP relaxation min ⟨C, X⟩ s.t. diag(X) = 1, X ⪰ 0 on a graph of nv
+            vertices: the band of half-width `band` plus extra random edges with probability
+            `extra`, weights U(0,1), C = −Laplacian/4. X is decomposed chordally into
+            overlapping windows of `clique` consecutive vertices (consecutive windows share
+            `band` vertices), one PSD cone per window in svec form (lower triangle column by
+            column, off-diagonal ×√2), with diag(X)=1 rows on each vertex's first window and
+            consensus rows X_k[i,j] = X_{k+1}[i,j] on every overlap entry; each C entry goes to
+            the first window containing it. The windows are a chordal cover of the band; extra
+            edges are drawn only between vertices that share a window (uniformly random
+            long-range edges make the chordal extension of a 2000-vertex band nearly dense).
+            X = I is strictly feasible.
 """
 import math
 
@@ -96,3 +107,66 @@ def socp(n=10000, m=5000, ncone=500, seed=0, adens=0.002):
 
 def log_uniform_n(rng, lo=50, hi=4000):
     return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+
+def maxcut(nv=2000, band=20, clique=48, extra=0.001, seed=0):
+    rng = np.random.default_rng(seed)
+    stride = clique - band
+    starts = list(range(0, max(1, nv - band), stride))
+    if starts[-1] + clique < nv:
+        starts.append(nv - clique)
+    wins = [(a, min(nv, a + clique)) for a in starts]
+    # graph: band plus in-window extra edges, weights U(0, 1)
+    W = {}
+    for i in range(nv):
+        for j in range(i + 1, min(nv, i + band + 1)):
+            W[(j, i)] = rng.uniform(0.0, 1.0)
+    for a, b in wins:
+        for i in range(a, b):
+            for j in range(max(i + band + 1, a), b):
+                if (j, i) not in W and rng.random() < extra:
+                    W[(j, i)] = rng.uniform(0.0, 1.0)
+    deg = np.zeros(nv)
+    for (j, i), w in W.items():
+        deg[i] += w
+        deg[j] += w
+    Cent = {(i, i): -deg[i] / 4.0 for i in range(nv)}  # C = −L/4, L = diag(deg) − W
+    for (j, i), w in W.items():
+        Cent[(j, i)] = w / 4.0
+    # one PSD cone per window
+    sizes = [(b - a) * (b - a + 1) // 2 for a, b in wins]
+    col0 = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    n = int(col0[-1])
+
+    def pos(k, i, j):  # global column of X_k[i, j], i ≥ j (global vertex indices)
+        a, b = wins[k]
+        d = b - a
+        ii, jj = i - a, j - a
+        return int(col0[k] + jj * d - jj * (jj - 1) // 2 + (ii - jj))
+
+    r2 = math.sqrt(2.0)
+    f = np.zeros(n)
+    owner = {}
+    for k, (a, b) in enumerate(wins):
+        for j in range(a, b):
+            for i in range(j, b):
+                if (i, j) in Cent and (i, j) not in owner:
+                    owner[(i, j)] = k
+                    f[pos(k, i, j)] = -(Cent[(i, j)] if i == j else r2 * Cent[(i, j)])
+    rows, cols, vals, g = [], [], [], []
+    seen = set()
+    for k, (a, b) in enumerate(wins):  # diag(X) = 1 on the first window of each vertex
+        for i in range(a, b):
+            if i not in seen:
+                seen.add(i)
+                rows.append(len(g)); cols.append(pos(k, i, i)); vals.append(1.0); g.append(1.0)
+    for k in range(len(wins) - 1):  # consensus on the overlap of consecutive windows
+        lo, hi = wins[k + 1][0], wins[k][1]
+        for j in range(lo, hi):
+            for i in range(j, hi):
+                r = len(g)
+                rows += [r, r]; cols += [pos(k, i, j), pos(k + 1, i, j)]; vals += [1.0, -1.0]; g.append(0.0)
+    B = sp.csc_matrix((vals, (rows, cols)), shape=(len(g), n))
+    B.sort_indices()
+    return dict(Q=sp.csc_matrix((n, n)), B=B, f=f, g=np.array(g), K=[("psd",)] * len(wins), s=sizes,
+                name=f"maxcut_nv{nv}_w{band}_c{clique}_s{seed}")

@@ -96,8 +96,123 @@ def identity(x, cone):
     elif k == "soc":
         x[:] = 0.0
         x[0] = math.sqrt(2.0)
+    elif k == "psd":
+        x[:] = svec(np.eye(triroot(len(x))))  # sdp.jl:92-111
     else:
         raise NotImplementedError(k)
+
+
+# ---------------------------------------------------------------------------------------
+# PSD cone in svec coordinates (sdp.jl; smat/svec/skron: cone/utils.jl:475-664)
+# ---------------------------------------------------------------------------------------
+
+_R2 = math.sqrt(2.0)
+
+
+def svec(M):
+    """Lower triangle column by column, off-diagonal entries times √2 (utils.jl:570-597)."""
+    d = M.shape[0]
+    out = []
+    for j in range(d):
+        out.append(M[j, j])
+        out.extend(_R2 * M[j + 1:, j])
+    return np.array(out, dtype=np.float64)
+
+
+def smat(v):
+    """Inverse of svec, symmetric (utils.jl:631-664 then symmetrize!)."""
+    d = triroot(len(v))
+    M = np.zeros((d, d))
+    k = 0
+    for j in range(d):
+        M[j, j] = v[k]
+        M[j + 1:, j] = v[k + 1:k + d - j] / _R2
+        k += d - j
+    return np.tril(M) + np.tril(M, -1).T
+
+
+def svec_index(d):
+    """(i, j) with i ≥ j of each svec position."""
+    return [(i, j) for j in range(d) for i in range(j, d)]
+
+
+def skron(A):
+    """H = A ⊗ₛ A in svec coordinates (utils.jl:475-513): for i ≥ j, k ≥ l,
+    H[(i,j),(k,l)] = c_ij c_kl (A_ik A_jl + A_il A_jk) / 2 with c = 1 on the diagonal, √2 off it."""
+    d = A.shape[0]
+    idx = svec_index(d)
+    I = np.array([a for a, _ in idx])
+    J = np.array([b for _, b in idx])
+    c = np.where(I == J, 1.0, _R2)
+    H = (A[I][:, I] * A[J][:, J] + A[I][:, J] * A[J][:, I]) / 2.0
+    return H * c[:, None] * c[None, :]
+
+
+class PsdCache:
+    """R = LP⁻ᵀU, S = LP U, s = Σ with LPᵀLD = UΣVᵀ (sdp.jl:10-30)."""
+    __slots__ = ("R", "S", "s")
+
+    def __init__(self, d):
+        self.R = np.zeros((d, d))
+        self.S = np.zeros((d, d))
+        self.s = np.zeros(d)
+
+
+def psdscale(H, cache, p, d):
+    """sdpscaledynamic! (sdp.jl:212-262): Cholesky of P = smat p and D = smat d, the SVD of
+    LPᵀ LD (the reference runs one-sided Jacobi, eig.jl:1-174; LAPACK's SVD gives the same
+    factorization up to the order and signs of the singular triplets, which leave W and the
+    cached products' uses invariant), then W = R Σ Rᵀ and H += W ⊗ₛ W."""
+    try:
+        LP = np.linalg.cholesky(smat(p))
+        LD = np.linalg.cholesky(smat(d))
+    except np.linalg.LinAlgError:
+        return False, 0.0
+    U, sv, _ = np.linalg.svd(LP.T @ LD)
+    if not np.all(sv > 0):
+        return False, 0.0
+    import scipy.linalg as sla
+    cache.S[:, :] = LP @ U
+    cache.R[:, :] = sla.solve_triangular(LP.T, U, lower=False)
+    cache.s[:] = sv
+    Y = cache.R * np.sqrt(sv)[None, :]
+    H += skron(Y @ Y.T)
+    return True, float(np.sum(1.0 / sv ** 2))
+
+
+def psdcorr(r, cache, dp, dd, smu):
+    """sdpcorrdynamic! (sdp.jl:374-433): A = Rᵀ ΔP R, B = Sᵀ ΔD S, X = A B,
+    Ŵ_ij = −(s_i X_ij + s_j X_ji)/(s_i + s_j) off the diagonal, σμ − s_j² − X_jj on it,
+    r = svec(R Ŵ Rᵀ)."""
+    R, S, sv = cache.R, cache.S, cache.s
+    A = R.T @ smat(dp) @ R
+    B = S.T @ smat(dd) @ S
+    X = A @ B
+    W = -(sv[:, None] * X + sv[None, :] * X.T) / (sv[:, None] + sv[None, :])
+    W[np.diag_indices(len(sv))] = smu - sv ** 2 - np.diag(X)
+    r[:] = svec(R @ W @ R.T)
+
+
+def psdcorr0(r, cache, smu):
+    """sdpcorr0dynamic! (sdp.jl:490-509): r = svec(R (σμ I − Σ²) Rᵀ)."""
+    R, sv = cache.R, cache.s
+    r[:] = svec((R * (smu - sv ** 2)[None, :]) @ R.T)
+
+
+def psdmaxstep(F, dx, scale):
+    """sdpmaxstepdynamic (sdp.jl:583-630): M = Fᵀ smat(Δx) F (divided by s_i s_j for the dual);
+    τ = 1 if I + M is positive definite, else 1/max(1, −λmin(M)) (the reference finds λmin by
+    Sturm bisection on a tridiagonal form, eig.jl:180-589; here LAPACK's eigvalsh)."""
+    M = F.T @ smat(dx) @ F
+    if scale is not None:
+        M = M / (scale[:, None] * scale[None, :])
+    try:
+        np.linalg.cholesky(np.eye(M.shape[0]) + M)
+        return 1.0
+    except np.linalg.LinAlgError:
+        pass
+    lam = float(np.linalg.eigvalsh((M + M.T) / 2.0)[0])
+    return 1.0 / max(1.0, -lam)
 
 
 class SocCache:
@@ -112,6 +227,8 @@ class SocCache:
 def make_cache(cone, n):
     if cone[0] == "soc":
         return SocCache(n)
+    if cone[0] == "psd":
+        return PsdCache(triroot(n))
     return None
 
 
@@ -130,6 +247,8 @@ def scale(cone, H, p, d, cache):
         return True, 0.0
     if k == "soc":
         return socscale(H, cache, p, d)
+    if k == "psd":
+        return psdscale(H, cache, p, d)
     raise NotImplementedError(k)
 
 
@@ -195,6 +314,8 @@ def corr0(cone, r, p, d, smu, cache):
         c1 = 2.0 * smu / c.pdet + eta
         r[0] = c1 * p[0] - t * c.w[0]
         r[1:] = t * c.w[1:] - c1 * p[1:]
+    elif k == "psd":
+        psdcorr0(r, cache, smu)
     else:
         raise NotImplementedError(k)
 
@@ -207,6 +328,8 @@ def corr(cone, r, p, d, dp, dd, smu, cache):
         r[:] = 0.0
     elif k == "soc":
         soccorr(r, cache.w, cache.beta, p, cache.pdet, dp, dd, smu)
+    elif k == "psd":
+        psdcorr(r, cache, dp, dd, smu)
     else:
         raise NotImplementedError(k)
 
@@ -280,6 +403,8 @@ def maxsteps(cone, p, dp, d, dd, cache):
     if k == "soc":
         return (socmaxstep(p, dp, cache.pdet, cache.pdetlo),
                 socmaxstep(d, dd, cache.ddet, cache.ddetlo))
+    if k == "psd":
+        return psdmaxstep(cache.R, dp, None), psdmaxstep(cache.S, dd, cache.s)
     raise NotImplementedError(k)
 
 
@@ -304,4 +429,13 @@ def shadow(cone, s, x):
             s[0] = g * x[0]
             s[1:] = -g * x[1:]
         return flag
+    if k == "psd":
+        # sdpshadowdynamic! (sdp.jl:666-678): −f′(X) = X⁻¹ through the Cholesky factor
+        try:
+            Lc = np.linalg.cholesky(smat(x))
+        except np.linalg.LinAlgError:
+            return False
+        Li = np.linalg.inv(Lc)
+        s[:] = svec(Li.T @ Li)
+        return True
     raise NotImplementedError(k)

@@ -67,14 +67,15 @@ def test_admission_keeps_reservations_within_the_budget():
 
 
 def test_a_failing_request_reports_its_error():
-    bad = G.qp(n=105, m=40, seed=300)  # one 14×14 PSD block in svec form (14·15/2 = 105)
-    bad_prob = mh.IPMProblem(bad["Q"], bad["B"], bad["f"], bad["g"], 0.0, [mh.SemidefiniteCone()], [105])
+    # one 50×50 PSD block in svec form (50·51/2 = 1275), above the device path's 48×48 limit
+    bad = G.qp(n=1275, m=40, seed=300)
+    bad_prob = mh.IPMProblem(bad["Q"], bad["B"], bad["f"], bad["g"], 0.0, [mh.SemidefiniteCone()], [1275])
     good = hip_problem(G.qp(n=200, m=80, seed=301))
     server = mh.IPMServer(nslots=2)
     try:
         tb = server.submit(bad_prob)
         tg = server.submit(good)
-        with pytest.raises(RuntimeError, match="cone kind"):
+        with pytest.raises(RuntimeError, match="block limit"):
             server.wait(tb)
         res, _ = server.wait(tg)
     finally:
