@@ -243,6 +243,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); slots beyond that
+    # would share a queue and run their kernels one after another, so give each slot its own
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, args.slots))))
     import mumblebee_hip as mh
     line = bench_serve(args, mh) if args.serve else bench_resident(args, mh)
     print(json.dumps(line))

@@ -95,31 +95,57 @@ __device__ __forceinline__ void stamp(unsigned long long* trace, int t, int slot
 // that a thread's 16 loads are in flight together)
 // ------------------------------------------------------------------------------------------
 
+// Tiles move through raw buffer instructions over a resource spanning exactly the tile's
+// extent in F: a lane outside the h×w tile gets an offset past that extent, which the hardware
+// range check turns into a zero load or a dropped store. With the device-coherent cache policy
+// (sc1, the same policy as the agent-scope atomics above) the loads are ordinary loads to the
+// compiler, so a thread's 16 loads are in flight together instead of one after another.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int CPOL_SC1 = 16;           // buffer cache policy bit: coherent at device scope
+constexpr unsigned OFF_OUT = 0x80000000u;  // an offset past every tile extent
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double* F, int LD, int r0, int c0, int h, int w) {
+  const double* base = F + (size_t)r0 + (size_t)c0 * LD;
+  const unsigned bytes = (unsigned)(((size_t)(w - 1) * LD + h) * sizeof(double));
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, (int)bytes, 0x00020000);
+}
+
 // h×w tile at (r0, c0) of F (leading dimension LD) → LDS column-major (stride LDT), zero padded.
 // COH: coherent loads (the tile was written by another workgroup of the running kernel).
 template <bool COH>
 __device__ __forceinline__ void load_tile(double* S, const double* F, int LD, int r0, int c0, int h, int w) {
-  const size_t base = (size_t)r0 + (size_t)c0 * LD;
+  const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
+  double v[NB * NB / NT];
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
-    bool ok = i < h && j < w;
-    size_t off = ok ? base + (size_t)i + (size_t)j * LD : base;
-    double v = COH ? cload(F + off) : F[off];
-    S[i + j * LDT] = ok ? v : 0.0;
+    unsigned off = i < h && j < w ? (unsigned)((i + (size_t)j * LD) * sizeof(double)) : OFF_OUT;
+    v[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+  }
+#pragma unroll
+  for (int u = 0; u < NB * NB / NT; ++u) {
+    int t = threadIdx.x + u * NT;
+    S[(t % NB) + (t / NB) * LDT] = v[u];
   }
 }
 
 // LDS column-major tile → h×w tile at (r0, c0) of F with coherent stores (lower part only if lower)
 __device__ __forceinline__ void store_tile(double* F, int LD, int r0, int c0, int h, int w, const double* S,
                                            bool lower) {
-  const size_t base = (size_t)r0 + (size_t)c0 * LD;
+  const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
+  double v[NB * NB / NT];
+#pragma unroll
+  for (int u = 0; u < NB * NB / NT; ++u) {
+    int t = threadIdx.x + u * NT;
+    v[u] = S[(t % NB) + (t / NB) * LDT];
+  }
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
-    if (i < h && j < w && (!lower || i >= j)) cstore(F + base + (size_t)i + (size_t)j * LD, S[i + j * LDT]);
+    unsigned off = i < h && j < w && (!lower || i >= j) ? (unsigned)((i + (size_t)j * LD) * sizeof(double)) : OFF_OUT;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[u]), rs, (int)off, 0, CPOL_SC1);
   }
 }
 
