@@ -33,10 +33,11 @@ constexpr int LROW = NB + 2;   // row stride of a row-major tile in LDS (16-byte
 constexpr int NT = 256;        // threads per workgroup of every kernel here
 constexpr long SPIN_LIMIT = 1L << 24;
 
-// Cross-workgroup data moves through agent-scope relaxed atomics: on this part they are
-// coherent loads/stores at the device coherence point (sc1), so no L2 writeback or
-// invalidate is needed. A flag is stored only after the producer's coherent stores completed
-// (s_waitcnt), and consumers read the data with coherent loads after observing the flag.
+// Cross-workgroup data moves with the device-coherent cache policy (sc1): flags and single
+// values through agent-scope relaxed atomics, tiles through buffer loads/stores carrying the
+// same policy, so no L2 writeback or invalidate is needed. A flag is stored only after the
+// producer's coherent stores completed (s_waitcnt), and consumers read the data with coherent
+// loads after observing the flag.
 __device__ __forceinline__ int ld_flag(const int* p) {
   return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -91,8 +92,7 @@ __device__ __forceinline__ void stamp(unsigned long long* trace, int t, int slot
 }
 
 // ------------------------------------------------------------------------------------------
-// tile movement (256 threads; every load is issued unconditionally from a clamped address so
-// that a thread's 16 loads are in flight together)
+// tile movement (256 threads, 16 elements each)
 // ------------------------------------------------------------------------------------------
 
 // Tiles move through raw buffer instructions over a resource spanning exactly the tile's
@@ -110,24 +110,39 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double* F, int
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, (int)bytes, 0x00020000);
 }
 
-// h×w tile at (r0, c0) of F (leading dimension LD) → LDS column-major (stride LDT), zero padded.
+// A thread's share of a tile in flight: element t = threadIdx.x + u·NT is (t % NB, t / NB).
+struct TileRegs {
+  double v[NB * NB / NT];
+};
+
+// issue the loads of the h×w tile at (r0, c0) of F (leading dimension LD), zero padded.
 // COH: coherent loads (the tile was written by another workgroup of the running kernel).
 template <bool COH>
-__device__ __forceinline__ void load_tile(double* S, const double* F, int LD, int r0, int c0, int h, int w) {
+__device__ __forceinline__ void fetch_tile(TileRegs& R, const double* F, int LD, int r0, int c0, int h, int w) {
   const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
-  double v[NB * NB / NT];
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
     unsigned off = i < h && j < w ? (unsigned)((i + (size_t)j * LD) * sizeof(double)) : OFF_OUT;
-    v[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+    R.v[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
   }
+}
+
+// fetched tile → LDS column-major (stride LDT)
+__device__ __forceinline__ void put_tile(double* S, const TileRegs& R) {
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
-    S[(t % NB) + (t / NB) * LDT] = v[u];
+    S[(t % NB) + (t / NB) * LDT] = R.v[u];
   }
+}
+
+template <bool COH>
+__device__ __forceinline__ void load_tile(double* S, const double* F, int LD, int r0, int c0, int h, int w) {
+  TileRegs R;
+  fetch_tile<COH>(R, F, LD, r0, c0, h, w);
+  put_tile(S, R);
 }
 
 // LDS column-major tile → h×w tile at (r0, c0) of F with coherent stores (lower part only if lower)
@@ -390,6 +405,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
   __shared__ __align__(16) double SA[NB * LROW];
   __shared__ __align__(16) double SB[NB * LROW];
   __shared__ __align__(16) double SC[NB * LROW];
+  __shared__ __align__(16) double SD[NB * LROW];
   __shared__ int task_s;
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
@@ -416,20 +432,37 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     Acc C;
     acc_from_lds(C, SA);
     __syncthreads();
+    // C −= Σ_k L_ik L_jkᵀ, software pipelined: the tiles of column k+1 are in flight while the
+    // MFMAs of column k run from the other LDS buffer pair
     bool alive = true;
-    for (int k = 0; k < kmax; ++k) {
-      if (!wait_geq(&B.state[tri_index(i, k)], epoch, B.abortf) ||
-          (i != j && !wait_geq(&B.state[tri_index(j, k)], epoch, B.abortf))) {
-        alive = false;
+    TileRegs RA, RB;
+    if (kmax > 0) {
+      if (!wait_geq(&B.state[tri_index(i, 0)], epoch, B.abortf) ||
+          (i != j && !wait_geq(&B.state[tri_index(j, 0)], epoch, B.abortf)))
         break;
-      }
-      const int c0k = B.tb[k], wk = B.tb[k + 1] - c0k;
-      load_tile<true>(SA, F, LD, r0i, c0k, hi, wk);
-      if (i != j) load_tile<true>(SB, F, LD, r0j, c0k, hj, wk);
-      __syncthreads();
-      mfma_acc<true>(C, SA, i != j ? SB : SA, wk);
-      __syncthreads();
+      const int c00 = B.tb[0], w0 = B.tb[1] - c00;
+      fetch_tile<true>(RA, F, LD, r0i, c00, hi, w0);
+      if (i != j) fetch_tile<true>(RB, F, LD, r0j, c00, hj, w0);
     }
+    for (int k = 0; k < kmax; ++k) {
+      double* XA = (k & 1) ? SC : SA;
+      double* XB = (k & 1) ? SD : SB;
+      put_tile(XA, RA);
+      if (i != j) put_tile(XB, RB);
+      __syncthreads();  // also: the buffer pair written next was last read before this point
+      if (k + 1 < kmax) {
+        if (!wait_geq(&B.state[tri_index(i, k + 1)], epoch, B.abortf) ||
+            (i != j && !wait_geq(&B.state[tri_index(j, k + 1)], epoch, B.abortf))) {
+          alive = false;
+          break;
+        }
+        const int c0n = B.tb[k + 1], wn = B.tb[k + 2] - c0n;
+        fetch_tile<true>(RA, F, LD, r0i, c0n, hi, wn);
+        if (i != j) fetch_tile<true>(RB, F, LD, r0j, c0n, hj, wn);
+      }
+      mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
+    }
+    __syncthreads();
     if (!alive) break;
     stamp(B.trace, t, 1);
     if (type == 0) {
