@@ -36,6 +36,9 @@ import numpy as np  # noqa: E402
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
 METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP/s"
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+# scaled by a 256 MiB device copy (profiles/k_dag_factor_qp2000_fp64.md); keyed by (n, m)
+TRAFFIC_BYTES = {(2000, 1000): 144.0e6}
 
 
 def workload(seed, n, m):
@@ -172,7 +175,7 @@ def bench_resident(args, mh):
             "peak": PEAK_FP64_MFMA,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_MFMA,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES.get((args.n, args.m)) if dom_name == "dag" else None,
             "kernel": ("k_dag_factor (persistent left-looking tile FP64 MFMA Cholesky of the large fronts)"
                        if dom_name == "dag" else "k_update (FP64 MFMA trailing update of the frontal Cholesky)"),
             "avg_launch_us": dom["ms"] / max(1, dom["launches"]) * 1e3,
@@ -226,6 +229,20 @@ def bench_serve(args, mh):
     }
 
 
+def copy_calibration(mib):
+    """A plain device copy of known size (mib MiB read, mib MiB written, 3 launches): the reference
+    the HBM traffic counters of the solve kernels are scaled by."""
+    import torch
+    n = mib * 2 ** 20 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,7 +255,11 @@ def main():
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-nmax", type=int, default=4000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
