@@ -76,6 +76,8 @@ void launch_cone_matvec_dense(hipStream_t, const InstDev*, const ConeDesc*, cons
                               Coef a);
 void launch_cone_hnorm2_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, double* out);
 void launch_zerofree(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx);
+// the shadow-solve seed of every listed exponential / power cone (initcache!)
+void launch_td_initcache(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n);
 void launch_fill(hipStream_t, double* x, size_t n, double v);
 void launch_assemble_union(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent);

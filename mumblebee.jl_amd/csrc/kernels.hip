@@ -14,6 +14,7 @@
 
 #include "device.h"
 #include "kernels.h"
+#include "tdc.h"
 
 namespace mb {
 
@@ -184,8 +185,13 @@ __global__ void k_cone_matvec(const InstDev* insts, const ConeDesc* cones, int n
   const double* x = nvec(I, kx) + C.col0;
   double* y = nvec(I, ky) + C.col0;
   double av = coefval(a, I.sc);
-  if (C.kind == CONE_POS)  // dense blocks (second-order, PSD): k_cone_matvec_dense
+  if (C.kind == CONE_POS) {  // dense blocks (second-order, PSD): k_cone_matvec_dense
     for (int i = 0; i < C.n; ++i) y[i] += av * C.D[i] * x[i];
+  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
+    const double* D = C.D;
+    const double x0 = x[0], x1 = x[1], x2 = x[2];
+    for (int i = 0; i < 3; ++i) y[i] += av * (D[i] * x0 + D[i + 3] * x1 + D[i + 6] * x2);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -298,6 +304,9 @@ __global__ void k_cone_scale(const InstDev* insts, const ConeDesc* cones, int nc
       flag = 0.0;
     }
     cache[0] = beta;
+  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
+    // tdscale! (tdc.jl:185-270)
+    flag = tdc::scale(C.kind, C.param, p, d, C.cache, C.D, spsd) ? 1.0 : 0.0;
   }
   out[2 * c] = flag;
   out[2 * c + 1] = spsd;
@@ -317,6 +326,7 @@ __global__ void k_cone_hnorm2(const InstDev* insts, const ConeDesc* cones, int n
     for (int i = 0; i < n; ++i) {
       double h = C.Qd[i + (size_t)j * n];
       if (C.kind == CONE_POS && i == j) h += C.D[i];
+      if (C.kind == CONE_EXP || C.kind == CONE_POW) h += C.D[i + (size_t)j * n];
       s += h * h;
     }
   out[c] = s;
@@ -402,6 +412,8 @@ __global__ void k_cone_corr0(const InstDev* insts, const ConeDesc* cones, int nc
     double c1 = 2.0 * smu / pdet + eta;
     r[0] = fma(c1, p[0], -t * w[0]);
     for (int i = 1; i < n; ++i) r[i] = fma(t, w[i], -c1 * p[i]);
+  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
+    tdc::corr0(r, d, C.cache, smu);
   }
 }
 
@@ -457,6 +469,8 @@ __global__ void k_cone_corr(const InstDev* insts, const ConeDesc* cones, int nco
     double gamma = 2.0 * smu / pdet;
     r[0] = fma(gamma, p[0], -r[0]);
     for (int i = 1; i < n; ++i) r[i] = fma(-gamma, p[i], -r[i]);
+  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
+    tdc::corr(C.kind, C.param, r, p, d, dp0, dd0, C.cache, smu);
   }
 }
 
@@ -517,12 +531,28 @@ __global__ void k_cone_maxstep(const InstDev* insts, const ConeDesc* cones, int 
   } else if (C.kind == CONE_SOC) {
     tp = socmaxstep(p, dp, C.n, C.cache[1], C.cache[3]);
     td = socmaxstep(d, dd, C.n, C.cache[2], C.cache[4]);
+  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
+    tdc::maxsteps(C.kind, C.param, p, dp, d, dd, C.cache, tp, td);
   }
   if (C.kind != CONE_FREE) {
     tp *= step_frac;
     td *= step_frac;
   }
   out[c] = fmin(tp, td);
+}
+
+// shadow-solve seed of the listed exponential / power cones (initcache!: exp.jl:27-30,
+// pow.jl:35-38)
+__global__ void k_td_initcache(const InstDev* insts, const ConeDesc* cones, const int* list, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const ConeDesc C = cones[list[t]];
+  if (insts[C.inst].sc[S_ACTIVE] == 0.0) return;
+  C.cache[tdc::C_SEED] = tdc::seed0(C.kind);
+}
+
+void launch_td_initcache(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* list, int n) {
+  if (n) k_td_initcache<<<(n + 255) / 256, 256, 0, st>>>(I, C, list, n);
 }
 
 // zero the Δd entries of free cones (solver/ipm.jl:551-559)

@@ -1,5 +1,6 @@
 // IPM driver over device-resident state (see solver.h).
 #include "solver.h"
+#include "tdc.h"
 
 #include <algorithm>
 #include <cmath>
@@ -150,14 +151,20 @@ Solver::Solver(const Problem& prob, const Settings& st, Deferred) : set_(st) { h
 
 void Solver::host_setup(const Problem& prob) {
   for (const Cone& c : prob.K)
-    if (c.kind != CONE_POS && c.kind != CONE_FREE && c.kind != CONE_SOC && c.kind != CONE_PSD)
+    if (c.kind < CONE_POS || c.kind > CONE_POW)
       throw std::invalid_argument("cone kind not supported by this build's device path");
-  for (i64 v = 0; v < prob.B.nvtx; ++v)
-    if (prob.K[v].kind == CONE_PSD) {
+  for (i64 v = 0; v < prob.B.nvtx; ++v) {
+    const int k = prob.K[v].kind;
+    if ((k == CONE_EXP || k == CONE_POW) && prob.B.ncols(v) != 3)
+      throw std::invalid_argument("exponential and power cones have 3 columns");
+    if (k == CONE_POW && !(prob.K[v].param > 0.0 && prob.K[v].param < 1.0))
+      throw std::invalid_argument("power cone exponent outside (0, 1)");
+    if (k == CONE_PSD) {
       i64 nc = prob.B.ncols(v), d = cone_degree(prob.K[v], nc);
       if (d * (d + 1) / 2 != nc) throw std::invalid_argument("PSD cone size is not d(d+1)/2");
       if (d > PSD_MAX) throw std::invalid_argument("PSD cone larger than the device path's block limit");
     }
+  }
   n_ = prob.B.ncol();
   m_ = prob.B.nrow();
   nu_ = 0;
@@ -213,7 +220,7 @@ double Solver::estimate_device_bytes() const {
   double cone = 0.0;
   for (i64 v = 0; v < P_.B.nvtx; ++v) {
     double c = (double)P_.B.ncols(v);
-    cone += c * c;
+    cone += c * c + (double)tdc::CACHE;
   }
   double nnz = 2.0 * (double)P_.B.val.size() + (double)P_.Q.val.size();
   double b = 8.0 * fr * 1.5 + 8.0 * dinv + 8.0 * (NV_COUNT * (double)n_ + MV_COUNT * (double)m_) + 12.0 * nnz +
@@ -391,8 +398,9 @@ void Solver::build_device() {
     i64 nc = P_.B.ncols(v);
     int k = P_.K[v].kind;
     const i64 d = k == CONE_PSD ? cone_degree(P_.K[v], nc) : 0;
-    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : ((k == CONE_SOC || k == CONE_PSD) ? nc * nc : 0));
-    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : (k == CONE_PSD ? 2 * d * d + d : 0));
+    const bool td = k == CONE_EXP || k == CONE_POW;
+    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : (k == CONE_FREE ? 0 : nc * nc));
+    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : (k == CONE_PSD ? 2 * d * d + d : (td ? tdc::CACHE : 0)));
     qofs[v + 1] = qofs[v] + nc * nc;
   }
   std::vector<double> qdiag(qofs[N], 0.0);
@@ -443,15 +451,19 @@ void Solver::build_device() {
   }
   d_cones_.upload(cones_, st_);
   {
-    std::vector<int> psd, dense;
+    std::vector<int> psd, dense, td;
     for (i64 v = 0; v < N; ++v) {
-      if (cones_[v].kind == CONE_PSD) psd.push_back((int)v);
-      if (cones_[v].kind == CONE_PSD || cones_[v].kind == CONE_SOC) dense.push_back((int)v);
+      const int k = cones_[v].kind;
+      if (k == CONE_PSD) psd.push_back((int)v);
+      if (k == CONE_PSD || k == CONE_SOC) dense.push_back((int)v);
+      if (k == CONE_EXP || k == CONE_POW) td.push_back((int)v);
     }
     npsd_ = (int)psd.size();
     ndense_ = (int)dense.size();
+    ntd_ = (int)td.size();
     if (npsd_) d_psdlist_.upload(psd, st_);
     if (ndense_) d_denselist_.upload(dense, st_);
+    if (ntd_) d_tdlist_.upload(td, st_);
   }
   d_cone_front_.upload(cone_front_, st_);
   d_cone_ofs_.upload(std::vector<int>{0, (int)N}, st_);
@@ -939,6 +951,8 @@ static void host_identity(double* x, const Cone& c, i64 n) {
       x[k] = 1.0;
       k += d - j;
     }
+  } else {  // exp.jl:15-25, pow.jl:27-33
+    tdc::identity(c.kind, c.param, x);
   }
 }
 
@@ -979,7 +993,15 @@ static bool psd_shadow(double* s, const double* x, i64 n, i64 d) {
   return true;
 }
 
-static bool host_shadow(double* s, const double* x, const Cone& c, i64 n) {
+// s = −f'(x) (dualshadow!); for the exponential and power cones, whose barriers are not
+// self-concordant-dual to themselves, primal = true solves −f'(s) = x instead (primalshadow!)
+static bool host_shadow(double* s, const double* x, const Cone& c, i64 n, bool primal) {
+  if (c.kind == CONE_EXP || c.kind == CONE_POW) {
+    double cache[tdc::CACHE] = {0.0};
+    if (primal) tdc::primal_shadow(c.kind, c.param, s, x, cache);
+    else tdc::dual_shadow(c.kind, c.param, s, x, cache);
+    return true;
+  }
   if (c.kind == CONE_POS) {
     for (i64 i = 0; i < n; ++i) s[i] = 1.0 / x[i];
     return true;
@@ -1046,8 +1068,8 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
       for (i64 j = 0; j < n_; ++j) d[j] = d0[P_.P2[j]];
     for (i64 v = 0; v < N; ++v) {
       i64 a = P_.B.xcol[v], nc = P_.B.ncols(v);
-      if (!d0) host_shadow(&d[a], &p[a], P_.K[v], nc);
-      else if (!p0) host_shadow(&p[a], &d[a], P_.K[v], nc);
+      if (!d0) host_shadow(&d[a], &p[a], P_.K[v], nc, false);
+      else if (!p0) host_shadow(&p[a], &d[a], P_.K[v], nc, true);
     }
     if (y0)
       for (i64 i = 0; i < m_; ++i) y[i] = y0[P_.P1[i]];
@@ -1060,6 +1082,7 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
   HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_P * n_, p.data(), n_ * 8, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_D * n_, d.data(), n_ * 8, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_Y * m_, y.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  launch_td_initcache(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_tdlist_.as<int>(), ntd_);  // initcache!
   HIPCHK(hipStreamSynchronize(st_));
   hist_.clear();
   nH_ = norm2(P_.Q.val);

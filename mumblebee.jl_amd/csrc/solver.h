@@ -187,7 +187,8 @@ class Solver {
   int ntile_n_ = 0, ntile_m_ = 0;
   DevBuf d_sptiles_[3];  // row tiles of B, Bᵀ, Q for the group-per-row spmv
   DevBuf d_psdlist_, d_denselist_;  // PSD cones; cones with dense scaling blocks (SOC, PSD)
-  int npsd_ = 0, ndense_ = 0;
+  DevBuf d_tdlist_;                 // exponential and power cones
+  int npsd_ = 0, ndense_ = 0, ntd_ = 0;
   int nsptile_[3] = {0, 0, 0};
   int64_t nunion_ = 0;
   double fac_ms_ = 0.0, fac_ms_tot_ = 0.0;

@@ -28,6 +28,14 @@ Workloads:
             the first window containing it. The windows are a chordal cover of the band; extra
             edges are drawn only between vertices that share a window (uniformly random
             long-range edges make the chordal extension of a 2000-vertex band nearly dense).
+  exppow    entropy on a probability simplex plus weighted geometric means under budgets:
+            `nexp` exponential cones (u_i, x_i, t_i) with u_i = 1, Σ x_i = 1 and t_i ≤ x_i
+            log(1/x_i); `npow` power cones (a_j, b_j, c_j) with α_j ~ U(0.2, 0.8), a_j + b_j = 2
+            and |c_j| ≤ a_j^α b_j^(1−α); `m` further sparse equalities (density `dens`, N(0,1))
+            over the x, a, b coordinates with g = B x0 at the strictly interior x0 (x_i = 1/nexp,
+            a_j = b_j = 1). Objective: maximise Σ t_i + Σ c_j + εᵀ(x, a, b), ε ~ 0.1 N(0,1),
+            bounded because x, a, b range over compact sets. With m = 0 and ε = 0 the optimum
+            is log(nexp) + Σ_j (2α_j)^α_j (2(1−α_j))^(1−α_j).
             X = I is strictly feasible.
 """
 import math
@@ -170,3 +178,41 @@ def maxcut(nv=2000, band=20, clique=48, extra=0.001, seed=0):
     B.sort_indices()
     return dict(Q=sp.csc_matrix((n, n)), B=B, f=f, g=np.array(g), K=[("psd",)] * len(wins), s=sizes,
                 name=f"maxcut_nv{nv}_w{band}_c{clique}_s{seed}")
+
+
+def exppow(nexp=2000, npow=500, m=499, seed=0, dens=0.005, eps=0.1):
+    rng = np.random.default_rng(seed)
+    n = 3 * (nexp + npow)
+    alphas = rng.uniform(0.2, 0.8, npow)
+    rows, cols, vals = [], [], []
+    r = 0
+    for i in range(nexp):  # u_i = 1
+        rows.append(r), cols.append(3 * i), vals.append(1.0)
+        r += 1
+    for i in range(nexp):  # Σ x_i = 1
+        rows.append(r), cols.append(3 * i + 1), vals.append(1.0)
+    r += 1
+    for j in range(npow):  # a_j + b_j = 2
+        c0 = 3 * (nexp + j)
+        rows += [r, r]
+        cols += [c0, c0 + 1]
+        vals += [1.0, 1.0]
+        r += 1
+    xab = np.array([3 * i + 1 for i in range(nexp)] + [3 * (nexp + j) + k for j in range(npow) for k in (0, 1)])
+    for _ in range(m):  # sparse equalities over x, a, b; at least two entries per row
+        k = max(2, rng.binomial(len(xab), dens))
+        for c in rng.choice(xab, size=k, replace=False):
+            rows.append(r), cols.append(int(c)), vals.append(float(rng.standard_normal()))
+        r += 1
+    B = sp.csc_matrix((vals, (rows, cols)), shape=(r, n))
+    x0 = np.zeros(n)
+    x0[0:3 * nexp:3] = 1.0
+    x0[1:3 * nexp:3] = 1.0 / max(nexp, 1)
+    x0[3 * nexp::3] = 1.0
+    x0[3 * nexp + 1::3] = 1.0
+    g = B @ x0
+    f = np.zeros(n)
+    f[2::3] = 1.0  # t_i and c_j (objective −fᵀp)
+    f[xab] = eps * rng.standard_normal(len(xab))
+    K = [("exp",)] * nexp + [("pow", float(a)) for a in alphas]
+    return dict(Q=sp.csc_matrix((n, n)), B=B, f=f, g=g, K=K, s=[3] * (nexp + npow), alphas=alphas)

@@ -98,6 +98,9 @@ def identity(x, cone):
         x[0] = math.sqrt(2.0)
     elif k == "psd":
         x[:] = svec(np.eye(triroot(len(x))))  # sdp.jl:92-111
+    elif k in ("exp", "pow"):
+        from . import tdc
+        x[:] = tdc.identity(cone)  # exp.jl:15-25, pow.jl:27-33
     else:
         raise NotImplementedError(k)
 
@@ -229,6 +232,9 @@ def make_cache(cone, n):
         return SocCache(n)
     if cone[0] == "psd":
         return PsdCache(triroot(n))
+    if cone[0] in ("exp", "pow"):
+        from . import tdc
+        return tdc.TdCache(cone[0])
     return None
 
 
@@ -249,6 +255,9 @@ def scale(cone, H, p, d, cache):
         return socscale(H, cache, p, d)
     if k == "psd":
         return psdscale(H, cache, p, d)
+    if k in ("exp", "pow"):
+        from . import tdc
+        return tdc.scale(cone, H, p, d, cache)
     raise NotImplementedError(k)
 
 
@@ -316,6 +325,9 @@ def corr0(cone, r, p, d, smu, cache):
         r[1:] = t * c.w[1:] - c1 * p[1:]
     elif k == "psd":
         psdcorr0(r, cache, smu)
+    elif k in ("exp", "pow"):
+        from . import tdc
+        tdc.corr0(r, d, cache, smu)
     else:
         raise NotImplementedError(k)
 
@@ -330,6 +342,9 @@ def corr(cone, r, p, d, dp, dd, smu, cache):
         soccorr(r, cache.w, cache.beta, p, cache.pdet, dp, dd, smu)
     elif k == "psd":
         psdcorr(r, cache, dp, dd, smu)
+    elif k in ("exp", "pow"):
+        from . import tdc
+        tdc.corr(cone, r, p, d, dp, dd, cache, smu)
     else:
         raise NotImplementedError(k)
 
@@ -405,6 +420,9 @@ def maxsteps(cone, p, dp, d, dd, cache):
                 socmaxstep(d, dd, cache.ddet, cache.ddetlo))
     if k == "psd":
         return psdmaxstep(cache.R, dp, None), psdmaxstep(cache.S, dd, cache.s)
+    if k in ("exp", "pow"):
+        from . import tdc
+        return tdc.maxsteps(cone, p, dp, d, dd, cache)
     raise NotImplementedError(k)
 
 
@@ -412,9 +430,13 @@ def maxsteps(cone, p, dp, d, dd, cache):
 # shadows (pos.jl:173-187, noc.jl:320-328, soc.jl:476-504)
 # ---------------------------------------------------------------------------------------
 
-def shadow(cone, s, x):
-    """s = −f′(x); returns flag (interior)."""
+def shadow(cone, s, x, primal=False):
+    """s = −f′(x); returns flag (interior). For the exponential and power cones `primal`
+    selects the inverse map: s with −f′(s) = x (primalshadow!, tdc.jl:471-474)."""
     k = cone[0]
+    if k in ("exp", "pow"):
+        from . import tdc
+        return tdc.primal_shadow(cone, s, x) if primal else tdc.dual_shadow(cone, s, x)
     if k == "pos":
         s[:] = 1.0 / x
         return True

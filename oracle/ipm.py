@@ -145,11 +145,15 @@ class Solver:
                 if d0 is None:
                     C.shadow(self.K[v], self.d[a:b], self.p[a:b])
                 elif p0 is None:
-                    C.shadow(self.K[v], self.p[a:b], self.d[a:b])
+                    C.shadow(self.K[v], self.p[a:b], self.d[a:b], primal=True)
             self.y[:] = 0.0 if y0 is None else np.asarray(y0)[self.P1]
             self.p /= self.pscl
             self.d *= self.pscl
             self.y /= self.yscl
+        for v in range(self.B.nvtx):  # initcache! (solver/ipm.jl:422-424)
+            if self.K[v][0] in ("exp", "pow"):
+                from . import tdc
+                self.caches[v].seed = tdc.seed0(self.K[v][0])
         self.hist = []
 
     # ---------------------------------------------------------------------------------

@@ -1,0 +1,109 @@
+"""Exponential and power cones on the device: whole solves against closed-form optima and
+against the oracle (src/IPM/src/cone/tdc/*.jl).
+
+The bar is the objective and the residuals. Near μ ≈ 1e-9 the reference's line search (nflog3)
+refuses steps that stay within a few noise bands (65536·eps) of a cone boundary. How many
+iterations that tail takes, and whether the gap test is met before the stall detector ends the
+solve (OPTIMAL vs NEAR_OPTIMAL), then depends on the last bits of the iterates. Those bits
+differ between the device's and the oracle's log/exp and double-double log, so statuses are
+compared as the set {OPTIMAL, NEAR_OPTIMAL}, and iteration counts are not compared."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import mumblebee_hip as mh
+from mumblebee_hip import generators as G
+from helpers import hip_problem, oracle_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+GOOD = (mh.IPMStatus.OPTIMAL, mh.IPMStatus.NEAR_OPTIMAL)
+
+
+def _print_case(**kw):
+    print("case", kw, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle(data, **kw):
+    from oracle import ipm as O
+    return O.Solver(oracle_problem(data), **kw).solve()
+
+
+@pytest.mark.parametrize("k", [1, 3, 10, 200])
+def test_entropy_on_the_simplex(k):
+    _print_case(test="entropy", k=k)
+    data = G.exppow(nexp=k, npow=0, m=0, eps=0.0, seed=0)
+    res = mh.solve(hip_problem(data))
+    assert res.status in GOOD
+    assert res.pobj == pytest.approx(-math.log(k), rel=1e-8, abs=1e-8)
+
+
+def test_softmax_entropy_closed_form():
+    """max Σ −x log x + εᵀx on the simplex = log Σ e^ε."""
+    _print_case(test="softmax")
+    data = G.exppow(nexp=20, npow=0, m=0, eps=0.1, seed=1)
+    eps = data["f"][1:60:3]
+    res = mh.solve(hip_problem(data))
+    assert res.status in GOOD
+    assert res.pobj == pytest.approx(-math.log(np.exp(eps).sum()), rel=1e-8)
+
+
+def test_weighted_geometric_means_under_budgets():
+    _print_case(test="geomean")
+    data = G.exppow(nexp=0, npow=12, m=0, eps=0.0, seed=2)
+    opt = -sum((2 * a) ** a * (2 * (1 - a)) ** (1 - a) for a in data["alphas"])
+    res = mh.solve(hip_problem(data))
+    assert res.status == mh.IPMStatus.OPTIMAL  # power cones alone do not reach the band
+    assert res.pobj == pytest.approx(opt, rel=1e-8)
+
+
+@pytest.mark.parametrize("nexp,npow,m,seed", [(0, 10, 0, 1), (1, 1, 0, 1), (1, 1, 0, 2), (5, 5, 0, 1),
+                                              (5, 5, 0, 2), (5, 5, 0, 3), (20, 10, 3, 3)])
+def test_mixed_cones_match_oracle(nexp, npow, m, seed):
+    _print_case(test="mixed", nexp=nexp, npow=npow, m=m, seed=seed)
+    data = G.exppow(nexp=nexp, npow=npow, m=m, eps=0.1, seed=seed)
+    ref = _oracle(data)
+    res = mh.solve(hip_problem(data))
+    assert int(ref.status) in GOOD and res.status in GOOD
+    assert rel(res.pobj, ref.pobj) < 1e-8
+    assert rel(res.dobj, ref.dobj) < 1e-8
+    assert res.pres < 1e-8 and res.dres < 1e-8
+
+
+def test_line_search_band_stall_is_reproduced():
+    """Below μ ≈ 1e-9 a cone whose log-form membership is within a few noise bands of its
+    boundary gets a zero step (nflog3); such problems end NEAR_OPTIMAL in both paths."""
+    _print_case(test="band stall")
+    data = G.exppow(nexp=20, npow=10, m=8, eps=0.1, seed=1)
+    ref = _oracle(data)
+    res = mh.solve(hip_problem(data))
+    assert int(ref.status) == int(mh.IPMStatus.NEAR_OPTIMAL)
+    assert res.status in GOOD
+    assert rel(res.pobj, ref.pobj) < 1e-8
+
+
+@pytest.mark.parametrize("which", ["p", "d"])
+def test_warm_start_completes_the_missing_side_by_shadows(which):
+    """reinit with only p0 (d = −f′(p)) or only d0 (p solves −f′(p) = d), as the oracle."""
+    _print_case(test="warm", which=which)
+    data = G.exppow(nexp=5, npow=5, m=0, eps=0.1, seed=2)
+    base = _oracle(data)
+    p0 = base.p * 0.9 + 0.1 * np.tile([1.0, 0.5, -0.5], len(base.p) // 3)
+    for i in range(len(p0) // 3):  # keep the start interior
+        a, b = p0[3 * i], p0[3 * i + 1]
+        if i < 5:
+            p0[3 * i + 2] = min(p0[3 * i + 2], b * math.log(a / b) - 0.2)
+        else:
+            p0[3 * i + 2] = 0.0
+    ident = np.concatenate([[1.2909282315382298, 0.8051015526498357, -0.8278379086082098]] * 5 +
+                           [[math.sqrt(1 + a), math.sqrt(2 - a), 0.0] for a in data["alphas"]])
+    kw = dict(p0=p0) if which == "p" else dict(d0=base.d + 0.1 * ident)  # dual-cone point + interior
+    ref = _oracle(data, **kw)
+    s = mh.init(hip_problem(data), **kw)
+    res = s.solve()
+    s.close()
+    assert int(ref.status) in GOOD and res.status in GOOD
+    assert rel(res.pobj, ref.pobj) < 1e-8
