@@ -57,9 +57,11 @@ typedef struct mb_settings {
   double infeas_abs;
   double infeas_rel;
   int32_t amalgamate;   /* relaxed supernode amalgamation (1 default) */
-  int32_t precision;    /* 0: FP64 factor */
+  int32_t precision;    /* 0: FP64 factor; 1: mixed — large fronts factored in FP32 on FP32 MFMA,
+                           FP64 Uzawa/CG refinement, FP64 factor once μ < mixed_mu */
   int32_t big_front_ld; /* fronts of this order or larger take the persistent kernels (512) */
   int32_t dag_grid;     /* workgroups of the persistent factorization (0: one per CU) */
+  double mixed_mu;      /* precision 1: μ below which the factor returns to FP64 (1e-5) */
 } mb_settings;
 
 /* Fill the reference defaults. */
@@ -110,8 +112,9 @@ int64_t mb_solver_niter(const mb_solver* s);
  * delta mu chi (history/ipm.jl:1-24) */
 int mb_solver_history(const mb_solver* s, double* rows);
 
-/* stats[0..6] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
- * total factorization ms, factorization calls, last factorization ms */
+/* stats[0..7] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
+ * total factorization ms, factorization calls, last factorization ms, factorizations whose
+ * large fronts were FP32 (precision 1) */
 int mb_solver_stats(const mb_solver* s, double* stats);
 
 /* reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429): new starting point, empty history;
@@ -140,8 +143,10 @@ int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);
 /* Debug: the current factor as a dense n×n column-major lower triangle (small n only). */
 int mb_solver_dense_factor(mb_solver* s, double* L);
 
-/* Lane-layout probe of v_mfma_f64_16x16x4f64 (256 doubles out). */
+/* Lane-layout probes of v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 (256 doubles out:
+ * element t of lane l at out[4l + t]). */
 int mb_probe_mfma_f64(double* out);
+int mb_probe_mfma_f32(double* out);
 
 /* ------------------------------------------------------------------------------------------
  * Serving: a continuous-batching request loop on one GPU. No counterpart in the reference; it

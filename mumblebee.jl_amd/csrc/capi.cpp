@@ -83,7 +83,9 @@ static mb::Settings to_settings(const mb_settings* s) {
   st.amalgamate = s->amalgamate;
   st.big_front_ld = s->big_front_ld > 0 ? s->big_front_ld : mb::BIG_LD;
   st.dag_grid = s->dag_grid;
-  if (s->precision != 0) throw std::invalid_argument("precision mode not available");
+  if (s->precision != 0 && s->precision != 1) throw std::invalid_argument("precision must be 0 or 1");
+  st.precision = s->precision;
+  st.mixed_mu = s->mixed_mu > 0 ? s->mixed_mu : mb::Settings().mixed_mu;
   return st;
 }
 
@@ -106,9 +108,10 @@ void mb_settings_default(mb_settings* s) {
   s->infeas_abs = d.infeas_abs;
   s->infeas_rel = d.infeas_rel;
   s->amalgamate = d.amalgamate;
-  s->precision = 0;
+  s->precision = d.precision;
   s->big_front_ld = d.big_front_ld;
   s->dag_grid = d.dag_grid;
+  s->mixed_mu = d.mixed_mu;
 }
 
 mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, const int64_t* Q_rowval,
@@ -277,6 +280,7 @@ int mb_solver_stats(const mb_solver* s, double* st) {
     st[4] = s->S->total_factor_ms();
     st[5] = s->S->factor_calls();
     st[6] = s->S->last_factor_ms();
+    st[7] = s->S->factor_calls_fp32();
     return 0;
   });
 }
@@ -320,17 +324,20 @@ int mb_solver_dense_factor(mb_solver* s, double* L) {
   });
 }
 
-int mb_probe_mfma_f64(double* out) {
+static int probe_mfma(double* out, bool fp32) {
   return guard([&]() {
     double* d = nullptr;
     if (hipMalloc(&d, 256 * sizeof(double)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
-    mb::launch_probe_mfma(nullptr, d);
+    mb::launch_probe_mfma(nullptr, d, fp32);
     hipError_t e = hipMemcpy(out, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
     return 0;
   });
 }
+
+int mb_probe_mfma_f64(double* out) { return probe_mfma(out, false); }
+int mb_probe_mfma_f32(double* out) { return probe_mfma(out, true); }
 
 const char* mb_last_error(void) { return g_err.c_str(); }
 

@@ -93,7 +93,7 @@ void launch_fsolve(hipStream_t, const FrontDesc*, const int* items, int nitem, c
                    const InstDev*, int kv);
 void launch_bsolve(hipStream_t, const FrontDesc*, const int* items, int nitem, const InstDev*, int kv);
 void launch_cg_scalars(hipStream_t, const InstDev*, int ninst, int op);
-void launch_probe_mfma(hipStream_t, double* out);
+void launch_probe_mfma(hipStream_t, double* out, bool fp32);
 void launch_set_scalar(hipStream_t, double* sc, int slot, double v);
 
 }  // namespace mb

@@ -982,6 +982,17 @@ __global__ void k_probe_mfma(double* out) {
   for (int t = 0; t < 4; ++t) out[lane * 4 + t] = acc[t];
 }
 
+// the same operands through v_mfma_f32_16x16x4f32 (every product and sum is exact in FP32)
+__global__ void k_probe_mfma_f32(double* out) {
+  int lane = threadIdx.x & 63;
+  float a = (float)((lane & 15) + 100 * (lane >> 4));
+  float b = (float)((lane & 15) + 1000 * (lane >> 4));
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  for (int t = 0; t < 4; ++t) out[lane * 4 + t] = (double)acc[t];
+}
+
 // ------------------------------------------------------------------------------------------
 // host-side launch wrappers
 // ------------------------------------------------------------------------------------------
@@ -1076,7 +1087,10 @@ void launch_bsolve(hipStream_t st, const FrontDesc* F, const int* items, int nit
 void launch_cg_scalars(hipStream_t st, const InstDev* I, int ninst, int op) {
   k_cg_scalars<<<cdiv(ninst, 64), 64, 0, st>>>(I, ninst, op);
 }
-void launch_probe_mfma(hipStream_t st, double* out) { k_probe_mfma<<<1, 64, 0, st>>>(out); }
+void launch_probe_mfma(hipStream_t st, double* out, bool fp32) {
+  if (fp32) k_probe_mfma_f32<<<1, 64, 0, st>>>(out);
+  else k_probe_mfma<<<1, 64, 0, st>>>(out);
+}
 
 __global__ void k_set_scalar(double* sc, int slot, double v) { sc[slot] = v; }
 void launch_set_scalar(hipStream_t st, double* sc, int slot, double v) { k_set_scalar<<<1, 1, 0, st>>>(sc, slot, v); }

@@ -22,6 +22,8 @@ struct BigFront {
   int ntask;
   int nstate;
   double* Dinv;         // Tr inverted diagonal blocks, 64×64 column-major each
+  float* F32;           // mixed precision: the FP32 factor (ld × r, same indexing as F), or null
+  float* Dinv32;        // mixed precision: FP32 inverted diagonal blocks, or null
   int* stage;           // Tr flags: sub-diagonal tile (j+1, j) accumulated (epoch-valued)
   unsigned long long* trace;  // optional: 4 timestamps per task (100 MHz wall clock), or null
   int* sflag;           // T solve flags (epoch-valued)
@@ -29,10 +31,12 @@ struct BigFront {
   int inst;
 };
 
-void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch);
+// fp32: the factor in FP32 on FP32 MFMA (B.F32 / B.Dinv32 must be allocated); the solves read
+// the factor of the same precision and compute in FP64
+void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32);
 void launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
-                       const InstDev* I, int kv, int epoch, long long base);
+                       const InstDev* I, int kv, int epoch, long long base, bool fp32);
 void launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
-                       int epoch, long long base);
+                       int epoch, long long base, bool fp32);
 
 }  // namespace mb

@@ -104,76 +104,118 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int CPOL_SC1 = 16;           // buffer cache policy bit: coherent at device scope
 constexpr unsigned OFF_OUT = 0x80000000u;  // an offset past every tile extent
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double* F, int LD, int r0, int c0, int h, int w) {
-  const double* base = F + (size_t)r0 + (size_t)c0 * LD;
-  const unsigned bytes = (unsigned)(((size_t)(w - 1) * LD + h) * sizeof(double));
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, (int)bytes, 0x00020000);
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const T* F, int LD, int r0, int c0, int h, int w) {
+  const T* base = F + (size_t)r0 + (size_t)c0 * LD;
+  const unsigned bytes = (unsigned)(((size_t)(w - 1) * LD + h) * sizeof(T));
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), 0, (int)bytes, 0x00020000);
+}
+
+template <bool COH>
+__device__ __forceinline__ void buf_load(double& v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
+  v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+}
+template <bool COH>
+__device__ __forceinline__ void buf_load(float& v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
+  v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, (int)off, 0, CPOL_SC1);
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)off, 0, CPOL_SC1);
 }
 
 // A thread's share of a tile in flight: element t = threadIdx.x + u·NT is (t % NB, t / NB).
+template <class T>
 struct TileRegs {
-  double v[NB * NB / NT];
+  T v[NB * NB / NT];
 };
 
 // issue the loads of the h×w tile at (r0, c0) of F (leading dimension LD), zero padded.
 // COH: coherent loads (the tile was written by another workgroup of the running kernel).
-template <bool COH>
-__device__ __forceinline__ void fetch_tile(TileRegs& R, const double* F, int LD, int r0, int c0, int h, int w) {
+template <bool COH, class T>
+__device__ __forceinline__ void fetch_tile(TileRegs<T>& R, const T* F, int LD, int r0, int c0, int h, int w) {
   const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
-    unsigned off = i < h && j < w ? (unsigned)((i + (size_t)j * LD) * sizeof(double)) : OFF_OUT;
-    R.v[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+    unsigned off = i < h && j < w ? (unsigned)((i + (size_t)j * LD) * sizeof(T)) : OFF_OUT;
+    buf_load<COH>(R.v[u], rs, off);
   }
 }
 
-// fetched tile → LDS column-major (stride LDT)
-__device__ __forceinline__ void put_tile(double* S, const TileRegs& R) {
+// fetched tile → LDS column-major (stride LDT), converted to the LDS element type
+template <class TD, class TS>
+__device__ __forceinline__ void put_tile(TD* S, const TileRegs<TS>& R) {
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
-    S[(t % NB) + (t / NB) * LDT] = R.v[u];
+    S[(t % NB) + (t / NB) * LDT] = (TD)R.v[u];
   }
 }
 
-template <bool COH>
-__device__ __forceinline__ void load_tile(double* S, const double* F, int LD, int r0, int c0, int h, int w) {
-  TileRegs R;
+template <bool COH, class TD, class TS>
+__device__ __forceinline__ void load_tile(TD* S, const TS* F, int LD, int r0, int c0, int h, int w) {
+  TileRegs<TS> R;
   fetch_tile<COH>(R, F, LD, r0, c0, h, w);
   put_tile(S, R);
 }
 
-// LDS column-major tile → h×w tile at (r0, c0) of F with coherent stores (lower part only if lower)
-__device__ __forceinline__ void store_tile(double* F, int LD, int r0, int c0, int h, int w, const double* S,
-                                           bool lower) {
+// LDS column-major tile → h×w tile at (r0, c0) of F with coherent stores (lower part only if
+// lower), converted to F's element type
+template <class TD, class TS>
+__device__ __forceinline__ void store_tile(TD* F, int LD, int r0, int c0, int h, int w, const TS* S, bool lower) {
   const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
-  double v[NB * NB / NT];
+  TD v[NB * NB / NT];
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
-    v[u] = S[(t % NB) + (t / NB) * LDT];
+    v[u] = (TD)S[(t % NB) + (t / NB) * LDT];
   }
 #pragma unroll
   for (int u = 0; u < NB * NB / NT; ++u) {
     int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
-    unsigned off = i < h && j < w && (!lower || i >= j) ? (unsigned)((i + (size_t)j * LD) * sizeof(double)) : OFF_OUT;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[u]), rs, (int)off, 0, CPOL_SC1);
+    unsigned off = i < h && j < w && (!lower || i >= j) ? (unsigned)((i + (size_t)j * LD) * sizeof(TD)) : OFF_OUT;
+    buf_store(v[u], rs, off);
   }
 }
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// The 64×64 accumulator tile of a workgroup: wave w holds the 32×32 quadrant (qi, qj) =
-// (32·(w>>1), 32·(w&1)) as 2×2 blocks of v_mfma_f64_16x16x4f64 outputs; element t of block
-// (a, b) in lane l is row qi + 16a + (l>>4) + 4t, column qj + 16b + (l&15).
-struct Acc {
-  d4 v[2][2];
+// 16×16×4 MFMA of the element type. Both take A[i][k] from lane i + 16k and B[k][j] from lane
+// j + 16k; they differ in where the 16×16 result lands: element t of lane l is row
+// (l>>4) + 4t for FP64 and row 4(l>>4) + t for FP32, column l & 15 for both (the lane-layout
+// probes mb_probe_mfma_f64 / _f32 check this on the device).
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int t) { return (lane >> 4) + 4 * t; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int t) { return 4 * (lane >> 4) + t; }
 };
 
-__device__ __forceinline__ void acc_from_lds(Acc& A, const double* S) {
+// The 64×64 accumulator tile of a workgroup: wave w holds the 32×32 quadrant (qi, qj) =
+// (32·(w>>1), 32·(w&1)) as 2×2 blocks of 16×16 MFMA outputs; element t of block (a, b) in
+// lane l is row qi + 16a + Mfma<T>::row(l, t), column qj + 16b + (l&15).
+template <class T>
+struct Acc {
+  typename Mfma<T>::v4 v[2][2];
+};
+
+template <class T>
+__device__ __forceinline__ void acc_from_lds(Acc<T>& A, const T* S) {
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
 #pragma unroll
@@ -182,10 +224,11 @@ __device__ __forceinline__ void acc_from_lds(Acc& A, const double* S) {
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        A.v[a][b][t] = S[(qi + a * 16 + (lane >> 4) + 4 * t) + (qj + b * 16 + (lane & 15)) * LDT];
+        A.v[a][b][t] = S[(qi + a * 16 + Mfma<T>::row(lane, t)) + (qj + b * 16 + (lane & 15)) * LDT];
 }
 
-__device__ __forceinline__ void acc_to_lds(const Acc& A, double* S) {
+template <class T>
+__device__ __forceinline__ void acc_to_lds(const Acc<T>& A, T* S) {
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
 #pragma unroll
@@ -194,28 +237,29 @@ __device__ __forceinline__ void acc_to_lds(const Acc& A, double* S) {
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int t = 0; t < 4; ++t)
-        S[(qi + a * 16 + (lane >> 4) + 4 * t) + (qj + b * 16 + (lane & 15)) * LDT] = A.v[a][b][t];
+        S[(qi + a * 16 + Mfma<T>::row(lane, t)) + (qj + b * 16 + (lane & 15)) * LDT] = A.v[a][b][t];
 }
 
-__device__ __forceinline__ void acc_zero(Acc& A) {
+template <class T>
+__device__ __forceinline__ void acc_zero(Acc<T>& A) {
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) A.v[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 2; ++b) A.v[a][b] = typename Mfma<T>::v4{0, 0, 0, 0};
 }
 
 // A ← A ± X Yᵀ for 64×64 LDS tiles X, Y (column-major, zero padded past column w)
-template <bool NEG>
-__device__ __forceinline__ void mfma_acc(Acc& A, const double* X, const double* Y, int w) {
+template <bool NEG, class T>
+__device__ __forceinline__ void mfma_acc(Acc<T>& A, const T* X, const T* Y, int w) {
   int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int qi = (wave >> 1) * 32, qj = (wave & 1) * 32;
   int li = lane & 15, lk = lane >> 4;
   int kmax = (w + 3) & ~3;
   for (int k0 = 0; k0 < kmax; k0 += 4) {
-    double xv[2], yv[2];
+    T xv[2], yv[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-      double x = X[(qi + a * 16 + li) + (k0 + lk) * LDT];
+      T x = X[(qi + a * 16 + li) + (k0 + lk) * LDT];
       xv[a] = NEG ? -x : x;
     }
 #pragma unroll
@@ -223,18 +267,22 @@ __device__ __forceinline__ void mfma_acc(Acc& A, const double* X, const double* 
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) A.v[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[a], yv[b], A.v[a][b], 0, 0, 0);
+      for (int b = 0; b < 2; ++b) A.v[a][b] = Mfma<T>::op(xv[a], yv[b], A.v[a][b]);
   }
 }
 
-__device__ __forceinline__ double readlane_d(double v, int lane) {
+__device__ __forceinline__ double readlane(double v, int lane) {
   long long b = __double_as_longlong(v);
   int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
   int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ __forceinline__ float readlane(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
 
-// 1/√d to full FP64 precision: the hardware estimate refined by two Newton steps
+// 1/√d to full precision: the hardware estimate refined by Newton steps (two for FP64, one for
+// FP32)
 __device__ __forceinline__ double rsqrt_nr(double d) {
   double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
@@ -243,6 +291,11 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
     y = __builtin_fma(0.5 * y, e, y);
   }
   return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float d) {
+  float y = __builtin_amdgcn_rsqf(d);
+  float e = __builtin_fmaf(-d * y, y, 1.0f);
+  return __builtin_fmaf(0.5f * y, e, y);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -253,30 +306,33 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 
 // 16×16 MFMA tile: acc ± Σ_{k<K} A[ra+i][ca+k] · op(B)[k][j] from LDS (stride LDT), with
 // op(B)[k][j] = B[rb+j][cb+k] if BT, else B[rb+k][cb+j]
-template <int K, bool BT, bool NEG>
-__device__ __forceinline__ d4 mfma16(d4 acc, const double* A, int ra, int ca, const double* Bm, int rb, int cb) {
+template <int K, bool BT, bool NEG, class T>
+__device__ __forceinline__ typename Mfma<T>::v4 mfma16(typename Mfma<T>::v4 acc, const T* A, int ra, int ca,
+                                                       const T* Bm, int rb, int cb) {
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int k0 = 0; k0 < K; k0 += 4) {
-    const double a = A[(ra + li) + (ca + k0 + lk) * LDT];
-    const double b = BT ? Bm[(rb + li) + (cb + k0 + lk) * LDT] : Bm[(rb + k0 + lk) + (cb + li) * LDT];
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(NEG ? -a : a, b, acc, 0, 0, 0);
+    const T a = A[(ra + li) + (ca + k0 + lk) * LDT];
+    const T b = BT ? Bm[(rb + li) + (cb + k0 + lk) * LDT] : Bm[(rb + k0 + lk) + (cb + li) * LDT];
+    acc = Mfma<T>::op(NEG ? -a : a, b, acc);
   }
   return acc;
 }
 
-__device__ __forceinline__ d4 tile16_load(const double* C, int rc, int cc) {
+template <class T>
+__device__ __forceinline__ typename Mfma<T>::v4 tile16_load(const T* C, int rc, int cc) {
   const int lane = threadIdx.x & 63;
-  d4 v;
+  typename Mfma<T>::v4 v;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) v[t] = C[(rc + (lane >> 4) + 4 * t) + (cc + (lane & 15)) * LDT];
+  for (int t = 0; t < 4; ++t) v[t] = C[(rc + Mfma<T>::row(lane, t)) + (cc + (lane & 15)) * LDT];
   return v;
 }
 
-__device__ __forceinline__ void tile16_store(double* C, int rc, int cc, d4 v) {
+template <class T>
+__device__ __forceinline__ void tile16_store(T* C, int rc, int cc, typename Mfma<T>::v4 v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) C[(rc + (lane >> 4) + 4 * t) + (cc + (lane & 15)) * LDT] = v[t];
+  for (int t = 0; t < 4; ++t) C[(rc + Mfma<T>::row(lane, t)) + (cc + (lane & 15)) * LDT] = v[t];
 }
 
 // 16×16 diagonal block at (o, o) of S, by wave 0: lane i (i < 16) holds row o+i in registers
@@ -285,35 +341,36 @@ __device__ __forceinline__ void tile16_store(double* C, int rc, int cc, d4 v) {
 // v_readlane. Then lane c forms column c of L⁻¹ right-looking (x_i /= L_ii, x_m −= L_mi x_i),
 // with L's entries again broadcast from the row registers. L → S and L⁻¹ → X (entries above
 // the diagonal are left for the caller to clear). Returns 1 on a non-positive pivot (taken as 1).
-__device__ __forceinline__ int chol_inv16(double* S, double* X, int o) {
+template <class T>
+__device__ __forceinline__ int chol_inv16(T* S, T* X, int o) {
   const int lane = threadIdx.x, i = lane & 15;
-  double row[16], rinv[16];
+  T row[16], rinv[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) row[k] = S[(o + i) + (o + k) * LDT];
   int bad = 0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    double d = readlane_d(row[j], j);
-    if (!(d > 0.0) || !isfinite(d)) {
+    T d = readlane(row[j], j);
+    if (!(d > T(0)) || !isfinite(d)) {
       bad = 1;  // the factor is discarded: the caller retries with a diagonal shift
-      d = 1.0;
+      d = T(1);
     }
-    const double r = rsqrt_nr(d);
+    const T r = rsqrt_nr(d);
     rinv[j] = r;
-    const double lij = row[j] * r;
+    const T lij = row[j] * r;
     row[j] = lij;
 #pragma unroll
-    for (int k = j + 1; k < 16; ++k) row[k] -= lij * readlane_d(lij, k);
+    for (int k = j + 1; k < 16; ++k) row[k] -= lij * readlane(lij, k);
     __builtin_amdgcn_sched_barrier(0);
   }
-  double x[16];
+  T x[16];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) x[m] = (m == i) ? 1.0 : 0.0;
+  for (int m = 0; m < 16; ++m) x[m] = (m == i) ? T(1) : T(0);
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
     x[c] *= rinv[c];
 #pragma unroll
-    for (int m = c + 1; m < 16; ++m) x[m] -= readlane_d(row[c], m) * x[c];
+    for (int m = c + 1; m < 16; ++m) x[m] -= readlane(row[c], m) * x[c];
     __builtin_amdgcn_sched_barrier(0);
   }
   if (lane < 16) {
@@ -328,11 +385,11 @@ __device__ __forceinline__ int chol_inv16(double* S, double* X, int o) {
 
 // With L11, X11 final for the b×b block at (o, o): L21 = A21 X11ᵀ into S, then
 // A22 −= L21 L21ᵀ. One 16×16 output tile per wave.
-template <int b>
-__device__ __forceinline__ void comb_lower(double* S, const double* X, int o) {
+template <int b, class T>
+__device__ __forceinline__ void comb_lower(T* S, const T* X, int o) {
   constexpr int nt = (b / 16) * (b / 16);
   const int w = threadIdx.x >> 6, ti = 16 * (w % (b / 16)), tj = 16 * (w / (b / 16));
-  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  typename Mfma<T>::v4 acc = {0, 0, 0, 0};
   if (w < nt) acc = mfma16<b, true, false>(acc, S, o + b + ti, o, X, o + tj, o);
   __syncthreads();
   if (w < nt) tile16_store(S, o + b + ti, o + tj, acc);
@@ -346,23 +403,24 @@ __device__ __forceinline__ void comb_lower(double* S, const double* X, int o) {
 }
 
 // With X11, L21 and X22 final: X21 = −X22 (L21 X11), through X21's place as the temporary.
-template <int b>
-__device__ __forceinline__ void comb_inverse(const double* S, double* X, int o) {
+template <int b, class T>
+__device__ __forceinline__ void comb_inverse(const T* S, T* X, int o) {
   constexpr int nt = (b / 16) * (b / 16);
   const int w = threadIdx.x >> 6, ti = 16 * (w % (b / 16)), tj = 16 * (w / (b / 16));
-  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  typename Mfma<T>::v4 acc = {0, 0, 0, 0};
   if (w < nt) acc = mfma16<b, false, false>(acc, S, o + b + ti, o, X, o, o + tj);
   __syncthreads();
   if (w < nt) tile16_store(X, o + b + ti, o + tj, acc);
   __syncthreads();
-  acc = d4{0.0, 0.0, 0.0, 0.0};
+  acc = typename Mfma<T>::v4{0, 0, 0, 0};
   if (w < nt) acc = mfma16<b, false, true>(acc, X, o + b + ti, o + b, X, o + b, o + tj);
   __syncthreads();
   if (w < nt) tile16_store(X, o + b + ti, o + tj, acc);
   __syncthreads();
 }
 
-__device__ __forceinline__ int chol_inv32(double* S, double* X, int o) {
+template <class T>
+__device__ __forceinline__ int chol_inv32(T* S, T* X, int o) {
   int bad = 0;
   if (threadIdx.x < 64) bad = chol_inv16(S, X, o);
   __syncthreads();
@@ -376,10 +434,11 @@ __device__ __forceinline__ int chol_inv32(double* S, double* X, int o) {
 // Cholesky factor L (in S: column-major, zero upper part, identity padded past h) and its
 // inverse X of the diagonal tile of order h held in S (zero padded past h). Returns false on a
 // non-positive pivot.
-__device__ __forceinline__ bool chol_inv64(double* S, double* X, int h) {
+template <class T>
+__device__ __forceinline__ bool chol_inv64(T* S, T* X, int h) {
   __shared__ int okflag;
-  if (threadIdx.x >= h && threadIdx.x < NB) S[threadIdx.x * (LDT + 1)] = 1.0;
-  for (int t = threadIdx.x; t < NB * NB; t += NT) X[(t % NB) + (t / NB) * LDT] = 0.0;  // the products read X's upper blocks
+  if (threadIdx.x >= h && threadIdx.x < NB) S[threadIdx.x * (LDT + 1)] = T(1);
+  for (int t = threadIdx.x; t < NB * NB; t += NT) X[(t % NB) + (t / NB) * LDT] = T(0);  // the products read X's upper blocks
   __syncthreads();
   int bad = chol_inv32(S, X, 0);
   comb_lower<32>(S, X, 0);
@@ -389,8 +448,8 @@ __device__ __forceinline__ bool chol_inv64(double* S, double* X, int h) {
   for (int t = threadIdx.x; t < NB * NB; t += NT) {
     const int r = t % NB, c = t / NB;
     if (r < c) {
-      S[r + c * LDT] = 0.0;
-      X[r + c * LDT] = 0.0;
+      S[r + c * LDT] = T(0);
+      X[r + c * LDT] = T(0);
     }
   }
   __syncthreads();
@@ -401,15 +460,35 @@ __device__ __forceinline__ bool chol_inv64(double* S, double* X, int h) {
 // the left-looking tile factorization kernel
 // ------------------------------------------------------------------------------------------
 
+// T: element type of the factor (FP64, or FP32 for the mixed-precision form). The assembled
+// front is always FP64 (B.F) and the update-matrix tiles go back to it in FP64 for the parent's
+// extend-add; the factor tiles, the staged sub-diagonal tiles and the inverted diagonal blocks
+// live in B.F (FP64) or B.F32 / B.Dinv32 (FP32).
+template <class T>
+__device__ __forceinline__ T* factor_of(const BigFront& B);
+template <>
+__device__ __forceinline__ double* factor_of<double>(const BigFront& B) { return B.F; }
+template <>
+__device__ __forceinline__ float* factor_of<float>(const BigFront& B) { return B.F32; }
+template <class T>
+__device__ __forceinline__ T* dinv_of(const BigFront& B);
+template <>
+__device__ __forceinline__ double* dinv_of<double>(const BigFront& B) { return B.Dinv; }
+template <>
+__device__ __forceinline__ float* dinv_of<float>(const BigFront& B) { return B.Dinv32; }
+
+template <class T>
 __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* insts, int epoch) {
-  __shared__ __align__(16) double SA[NB * LROW];
-  __shared__ __align__(16) double SB[NB * LROW];
-  __shared__ __align__(16) double SC[NB * LROW];
-  __shared__ __align__(16) double SD[NB * LROW];
+  __shared__ __align__(16) T SA[NB * LROW];
+  __shared__ __align__(16) T SB[NB * LROW];
+  __shared__ __align__(16) T SC[NB * LROW];
+  __shared__ __align__(16) T SD[NB * LROW];
   __shared__ int task_s;
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
   double* F = B.F;
+  T* Lf = factor_of<T>(B);
+  T* Di = dinv_of<T>(B);
   const int LD = B.ld;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -429,24 +508,24 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     // the assembled tile (written by earlier launches; only this task writes it here)
     load_tile<false>(SA, F, LD, r0i, r0j, hi, hj);
     __syncthreads();
-    Acc C;
+    Acc<T> C;
     acc_from_lds(C, SA);
     __syncthreads();
     // C −= Σ_k L_ik L_jkᵀ, software pipelined: the tiles of column k+1 are in flight while the
     // MFMAs of column k run from the other LDS buffer pair
     bool alive = true;
-    TileRegs RA, RB;
+    TileRegs<T> RA, RB;
     if (kmax > 0) {
       if (!wait_geq(&B.state[tri_index(i, 0)], epoch, B.abortf) ||
           (i != j && !wait_geq(&B.state[tri_index(j, 0)], epoch, B.abortf)))
         break;
       const int c00 = B.tb[0], w0 = B.tb[1] - c00;
-      fetch_tile<true>(RA, F, LD, r0i, c00, hi, w0);
-      if (i != j) fetch_tile<true>(RB, F, LD, r0j, c00, hj, w0);
+      fetch_tile<true>(RA, Lf, LD, r0i, c00, hi, w0);
+      if (i != j) fetch_tile<true>(RB, Lf, LD, r0j, c00, hj, w0);
     }
     for (int k = 0; k < kmax; ++k) {
-      double* XA = (k & 1) ? SC : SA;
-      double* XB = (k & 1) ? SD : SB;
+      T* XA = (k & 1) ? SC : SA;
+      T* XB = (k & 1) ? SD : SB;
       put_tile(XA, RA);
       if (i != j) put_tile(XB, RB);
       __syncthreads();  // also: the buffer pair written next was last read before this point
@@ -457,8 +536,8 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
           break;
         }
         const int c0n = B.tb[k + 1], wn = B.tb[k + 2] - c0n;
-        fetch_tile<true>(RA, F, LD, r0i, c0n, hi, wn);
-        if (i != j) fetch_tile<true>(RB, F, LD, r0j, c0n, hj, wn);
+        fetch_tile<true>(RA, Lf, LD, r0i, c0n, hi, wn);
+        if (i != j) fetch_tile<true>(RB, Lf, LD, r0j, c0n, hj, wn);
       }
       mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
     }
@@ -471,17 +550,17 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
         // ready), then C_jj −= L_{j,j−1} L_{j,j−1}ᵀ
         const int c0p = B.tb[j - 1], wp = B.tb[j] - c0p;
         if (!wait_geq(&B.stage[j - 1], epoch, B.abortf)) break;
-        load_tile<true>(SA, F, LD, r0j, c0p, hj, wp);
+        load_tile<true>(SA, Lf, LD, r0j, c0p, hj, wp);
         if (!wait_geq(&B.state[tri_index(j - 1, j - 1)], epoch, B.abortf)) break;
-        load_tile<true>(SB, B.Dinv + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
+        load_tile<true>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
         __syncthreads();
-        Acc X;
+        Acc<T> X;
         acc_zero(X);
         mfma_acc<false>(X, SA, SB, NB);
         __syncthreads();
         acc_to_lds(X, SA);
         __syncthreads();
-        store_tile(F, LD, r0j, c0p, hj, wp, SA, false);
+        store_tile(Lf, LD, r0j, c0p, hj, wp, SA, false);
         mfma_acc<true>(C, SA, SA, wp);
         publish(&B.state[tri_index(j, j - 1)], epoch);  // the stores drained during the update
       }
@@ -491,33 +570,34 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
       __syncthreads();
       bool ok = chol_inv64(SA, SC, hj);
       if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-      store_tile(B.Dinv + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
+      store_tile(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
       publish(&B.state[tri_index(j, j)], epoch);
-      store_tile(F, LD, r0j, r0j, hj, hj, SA, true);
+      store_tile(Lf, LD, r0j, r0j, hj, hj, SA, true);
     } else if (type == 3) {
       // sub-diagonal tile: hand the accumulated tile to the next diagonal task
       acc_to_lds(C, SA);
       __syncthreads();
       stamp(B.trace, t, 2);
-      store_tile(F, LD, r0i, r0j, hi, hj, SA, false);
+      store_tile(Lf, LD, r0i, r0j, hi, hj, SA, false);
       publish(&B.stage[j], epoch);
     } else if (type == 1) {
       // L_ij = C_ij L_jj⁻ᵀ
       acc_to_lds(C, SA);
       if (!wait_geq(&B.state[tri_index(j, j)], epoch, B.abortf)) break;
       stamp(B.trace, t, 2);
-      load_tile<true>(SB, B.Dinv + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
+      load_tile<true>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
       __syncthreads();
-      Acc X;
+      Acc<T> X;
       acc_zero(X);
       mfma_acc<false>(X, SA, SB, NB);
       __syncthreads();
       acc_to_lds(X, SA);
       __syncthreads();
-      store_tile(F, LD, r0i, r0j, hi, hj, SA, false);
+      store_tile(Lf, LD, r0i, r0j, hi, hj, SA, false);
       publish(&B.state[tri_index(i, j)], epoch);
     } else {
-      // update-matrix tile: final once every factor column has been subtracted
+      // update-matrix tile: final once every factor column has been subtracted (FP64 for the
+      // parent's extend-add)
       acc_to_lds(C, SA);
       __syncthreads();
       stamp(B.trace, t, 2);
@@ -559,7 +639,8 @@ __global__ void k_big_gather(const FrontDesc* fronts, int f, const int* childlis
 
 // forward: tile b (ticket order) accumulates L[b, j] x_j for published j < min(b, Tr); a
 // factor tile then forms x_b = L_bb⁻¹ (v_b − Σ) and publishes it, a separator tile adds Σ into
-// the front's update vector
+// the front's update vector. TF: element type of the stored factor; the arithmetic is FP64.
+template <class TF>
 __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
   __shared__ double Sinv[NB * LDT];
@@ -576,14 +657,14 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   const int b = (int)tk;
   if (b < 0 || b >= B.T) return;
   double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
-  const double* F = B.F;
+  const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
   const int r0 = B.tb[b], h = B.tb[b + 1] - r0;
   const int row = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int jmax = b < B.Tr ? b : B.Tr;
   const bool diag = b < B.Tr;
   // stage the last step's operands before waiting
-  if (diag) load_tile<false>(Sinv, B.Dinv + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  if (diag) load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
   if (jmax > 0) load_tile<false>(Sc, F, LD, r0, B.tb[jmax - 1], h, B.tb[jmax] - B.tb[jmax - 1]);
   double acc = 0.0;
   for (int j = 0; j < jmax; ++j) {
@@ -595,8 +676,8 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
 #pragma unroll
       for (int c = q; c < NB; c += 4) acc += Sc[row + c * LDT] * xs[c];
     } else if (row < h) {
-      const double* col = F + (size_t)(r0 + row) + (size_t)c0 * LD;
-      for (int c = q; c < w; c += 4) acc += col[(size_t)c * LD] * xs[c];
+      const TF* col = F + (size_t)(r0 + row) + (size_t)c0 * LD;
+      for (int c = q; c < w; c += 4) acc += (double)col[(size_t)c * LD] * xs[c];
     }
     __syncthreads();
   }
@@ -620,6 +701,7 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
 
 // backward: column tile b (ticket order descending) accumulates L21[:, b]ᵀ x_sep and
 // L[j, b]ᵀ x_j for j > b, then forms x_b = L_bb⁻ᵀ (v_b − Σ)
+template <class TF>
 __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
   __shared__ double S[NB * LDT];
@@ -638,11 +720,11 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   const int b = B.Tr - 1 - tkt;
   double* vall = I.nv + (size_t)kv * I.n;
   double* v = vall + Fd.res0;
-  const double* F = B.F;
+  const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
   const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
-  load_tile<false>(Sinv, B.Dinv + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
   if (b + 1 < B.Tr) load_tile<false>(Sc, F, LD, B.tb[b + 1], c0, B.tb[b + 2] - B.tb[b + 1], w);
   double acc = 0.0;
   // separator part (rows ≥ r; x_sep is final), 64-row chunks through LDS so that the global
@@ -686,20 +768,23 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
 // launch wrappers
 // ------------------------------------------------------------------------------------------
 
-void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch) {
+void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
   k_fill_int<<<1, 64, 0, st>>>(B.ticket, 2, 0);  // ticket and abort flag
-  k_dag_factor<<<grid, NT, 0, st>>>(B, I, epoch);
+  if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
+  else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
 }
 
 void launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
-                       const InstDev* I, int kv, int epoch, long long base) {
+                       const InstDev* I, int kv, int epoch, long long base, bool fp32) {
   k_big_gather<<<1, 256, 0, st>>>(F, f, childlist, I, kv);
-  k_big_fsolve<<<B.T, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  if (fp32) k_big_fsolve<float><<<B.T, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  else k_big_fsolve<double><<<B.T, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
 }
 
 void launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
-                       int epoch, long long base) {
-  k_big_bsolve<<<B.Tr, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+                       int epoch, long long base, bool fp32) {
+  if (fp32) k_big_bsolve<float><<<B.Tr, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  else k_big_bsolve<double><<<B.Tr, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
 }
 
 }  // namespace mb

@@ -209,13 +209,15 @@ void Solver::host_setup(const Problem& prob) {
 // storage, inverted diagonal blocks, vector arenas, CSR copies of B, Bᵀ, Q, cone blocks, and the
 // assembly map (at most one entry per frontal position).
 double Solver::estimate_device_bytes() const {
-  double fr = 0.0, dinv = 0.0;
+  double fr = 0.0, dinv = 0.0, mixed = 0.0;
   for (i64 f = 0; f < sym_.nfront(); ++f) {
     double r = (double)(sym_.xcol[sym_.front_vtx[f + 1]] - sym_.xcol[sym_.front_vtx[f]]);
     double sc = 0.0;
     for (i64 u : sym_.front_sep[f]) sc += (double)(sym_.xcol[u + 1] - sym_.xcol[u]);
     fr += (r + sc) * (r + sc);
     dinv += std::ceil(r / NB) * NB * NB;
+    if (set_.precision == 1 && r + sc >= set_.big_front_ld)  // FP32 factor and inverses, in doubles
+      mixed += 0.5 * ((r + sc) * r + std::ceil(r / NB) * NB * NB);
   }
   double cone = 0.0;
   for (i64 v = 0; v < P_.B.nvtx; ++v) {
@@ -223,7 +225,7 @@ double Solver::estimate_device_bytes() const {
     cone += c * c + (double)tdc::CACHE;
   }
   double nnz = 2.0 * (double)P_.B.val.size() + (double)P_.Q.val.size();
-  double b = 8.0 * fr * 1.5 + 8.0 * dinv + 8.0 * (NV_COUNT * (double)n_ + MV_COUNT * (double)m_) + 12.0 * nnz +
+  double b = 8.0 * fr * 1.5 + 8.0 * dinv + 8.0 * mixed + 8.0 * (NV_COUNT * (double)n_ + MV_COUNT * (double)m_) + 12.0 * nnz +
              8.0 * 3.0 * cone + 24.0 * (double)(n_ + m_);
   return 1.1 * b + (4 << 20);
 }
@@ -595,6 +597,10 @@ void Solver::build_schedule() {
       bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
       HIPCHK(hipMemset(bh->state.p, 0, bh->state.bytes));
       bh->dinv.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
+      if (set_.precision == 1) {  // FP32 factor columns (ld × r) and inverted diagonal blocks
+        bh->f32.alloc(sizeof(float) * (size_t)Fd.ld * std::max(1, Fd.r));
+        bh->dinv32.alloc(sizeof(float) * NB * NB * std::max(1, Tr));
+      }
       bh->stage.alloc(sizeof(int) * std::max(1, Tr));
       HIPCHK(hipMemset(bh->stage.p, 0, bh->stage.bytes));
       bh->sflag.alloc(sizeof(int) * std::max(1, T));
@@ -614,6 +620,8 @@ void Solver::build_schedule() {
       B.tasks = bh->tasks.as<int4>();
       B.ntask = (int)tasks.size();
       B.Dinv = bh->dinv.as<double>();
+      B.F32 = bh->f32.as<float>();
+      B.Dinv32 = bh->dinv32.as<float>();
       B.stage = bh->stage.as<int>();
       B.trace = nullptr;
       B.sflag = bh->sflag.as<int>();
@@ -754,7 +762,7 @@ void Solver::lsolve(int kv) {
       BigHost& bh = *big_[b];
       ++epoch_;
       launch_big_fsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_childlist_.as<int>(), d_inst_.as<InstDev>(),
-                        kv, epoch_, bh.sbase);
+                        kv, epoch_, bh.sbase, fac_fp32_);
       bh.sbase += bh.B.T;
     }
     prof_end(ps, PROF_FSOLVE, 0.0, L.solve_bytes);
@@ -770,7 +778,8 @@ void Solver::usolve(int kv) {
     for (int b : L.big) {
       BigHost& bh = *big_[b];
       ++epoch_;
-      launch_big_bsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_inst_.as<InstDev>(), kv, epoch_, bh.sbase);
+      launch_big_bsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_inst_.as<InstDev>(), kv, epoch_, bh.sbase,
+                        fac_fp32_);
       bh.sbase += bh.B.Tr;
     }
     prof_end(ps, PROF_BSOLVE, 0.0, L.solve_bytes);
@@ -786,7 +795,7 @@ void Solver::check_abort() {
     if (t[2 * b + 1] != 0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
 }
 
-void Solver::run_factor() {
+void Solver::run_factor(bool fp32) {
   const FrontDesc* F = d_fr_.as<FrontDesc>();
   const StepDesc* S = d_steps_.as<StepDesc>();
   const InstDev* I = d_inst_.as<InstDev>();
@@ -808,7 +817,7 @@ void Solver::run_factor() {
     }
     if (!L.big.empty()) {
       int pd = prof_begin(PROF_DAG);
-      for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_, fac_epoch_);
+      for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_, fac_epoch_, fp32);
       prof_end(pd, PROF_DAG, L.big_flops, 0.0);
     }
   }
@@ -889,11 +898,25 @@ bool Solver::factor(double delta, double& rho) {
   set_scalar(S_DELTA, delta);
   set_scalar(S_FAIL, 0.0);
   assemble();
+  // mixed precision: the large fronts in FP32 while μ is above the switch point; an FP32
+  // breakdown (non-positive pivot) is redone in FP64 before any diagonal shift
+  fac_fp32_ = set_.precision == 1 && !mixed_off_ && !big_.empty() && cur_mu_ > set_.mixed_mu;
   HIPCHK(hipEventRecord(ev0_, st_));
-  run_factor();
+  run_factor(fac_fp32_);
   HIPCHK(hipEventRecord(ev1_, st_));
   sync_scalars();
   check_abort();
+  if (fac_fp32_ && hsc_[S_FAIL] != 0.0) {
+    fac_fp32_ = false;
+    set_scalar(S_FAIL, 0.0);
+    assemble();
+    HIPCHK(hipEventRecord(ev0_, st_));
+    run_factor(false);
+    HIPCHK(hipEventRecord(ev1_, st_));
+    sync_scalars();
+    check_abort();
+  }
+  if (fac_fp32_) ++fac_fp32_calls_;
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
   fac_ms_ = ms;
@@ -912,7 +935,7 @@ bool Solver::factor(double delta, double& rho) {
   set_scalar(S_RHO, rho);
   set_scalar(S_FAIL, 0.0);
   launch_add_diag(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), I);
-  run_factor();
+  run_factor(false);
   sync_scalars();
   check_abort();
   return hsc_[S_FAIL] == 0.0;
@@ -1087,6 +1110,7 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
   hist_.clear();
   nH_ = norm2(P_.Q.val);
   status_ = CONTINUE;
+  mixed_off_ = false;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1353,6 +1377,7 @@ int Solver::step() {
   } else {
     double nu1 = std::max(nu_, 1);
     double mu = pd / nu1, mus = spsd / nu1, chi = mu * mus - 1;
+    cur_mu_ = mu;
     double pres = hsc_[S_NORM0] / (1 + sg_), dres = hsc_[S_NORM1] / (1 + sf_);
     double pQp = hsc_[S_NORM2];
     double pobj = pQp / 2 - hsc_[S_NORM3];
@@ -1401,74 +1426,86 @@ int Solver::step() {
       if (!ok) {
         status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
       } else {
-        double mu1 = hist_.empty() ? mu : hist_[0].mu;
-        double tol;
-        if (mut > 0) tol = FORCING_FRAC * std::min(std::fabs(mu - mut) / std::fabs(mu1 - mut), 1.0);
-        else tol = FORCING_FRAC * mu / mu1;
-        double ftol = tol * (1 + nf_), gtol = tol * (1 + ng_);
-        // predictor (solver/ipm.jl:111-163)
-        if (mut > 0) {
-          set_scalar(S_SMU, mut);
-          launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
-          launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
-        } else {
-          axpby(0, NV_D, NV_D, NV_FR, K(-1.0), K(0.0));
-        }
-        axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
-        KKTOut po = solvekkt(NV_DPA, MV_DYA, NV_FR, MV_DG, false, ftol, gtol, 1, false);
-        // Δda = Q Δpa − Bᵀ Δya − Δf
-        spmv(1, MV_DYA, NV_T0, K(1.0), K(0.0));
-        spmv(2, NV_DPA, NV_DDA, K(1.0), K(0.0));
-        axpbypcz(0, NV_T0, NV_DF, NV_DDA, NV_DDA, K(-1.0), K(-1.0), K(1.0));
-        launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DDA);
-        // corrector (solver/ipm.jl:181-272)
-        double taua = maxsteps(NV_DPA, NV_DDA, 1.0);
-        dot(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA);
-        sync_scalars();
-        double smu = hsc_[S_TMP2];
-        double mua = smu / nu_;
-        if (mut > 0) {
-          double dmu = mu - mut;
-          if (dmu > 0) {
-            double dmua = std::min(std::max(mua - mut, 0.0), dmu);
-            smu = mut + std::min(std::max(dmua * (dmua / dmu) * (dmua / dmu), 0.0), dmu);
+        // attempt 0 with the factor of this iteration; with mixed precision a KKT solve that the
+        // FP32 factor cannot refine to tolerance is repeated once on an FP64 factor, which then
+        // stays for the rest of the solve (μ only decreases)
+        for (int attempt = 0;; ++attempt) {
+          double mu1 = hist_.empty() ? mu : hist_[0].mu;
+          double tol;
+          if (mut > 0) tol = FORCING_FRAC * std::min(std::fabs(mu - mut) / std::fabs(mu1 - mut), 1.0);
+          else tol = FORCING_FRAC * mu / mu1;
+          double ftol = tol * (1 + nf_), gtol = tol * (1 + ng_);
+          // predictor (solver/ipm.jl:111-163)
+          if (mut > 0) {
+            set_scalar(S_SMU, mut);
+            launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
+            launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
           } else {
-            smu = mut;
+            axpby(0, NV_D, NV_D, NV_FR, K(-1.0), K(0.0));
           }
-        } else {
-          smu = std::min(std::max(mua * (mua / mu) * (mua / mu), 0.0), mu);
-        }
-        (void)taua;
-        set_scalar(S_SMU, smu);
-        launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
-        launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
-                        NV_DDA);
-        axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
-        axpby(0, NV_DPA, NV_DPA, NV_DP, K(1.0), K(0.0));
-        axpby(1, MV_DYA, MV_DYA, MV_DY, K(1.0), K(0.0));
-        KKTOut co = solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, true, ftol, gtol, 1, false);
-        spmv(1, MV_DY, NV_T0, K(1.0), K(0.0));
-        spmv(2, NV_DP, NV_DD, K(1.0), K(0.0));
-        axpbypcz(0, NV_T0, NV_DF, NV_DD, NV_DD, K(-1.0), K(-1.0), K(1.0));
-        launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DD);
-        row.piter = po.ncg;
-        row.ppass = po.nir;
-        row.pstat = po.status;
-        row.dmin = po.dmin;
-        row.dmax = po.dmax;
-        row.citer = co.ncg;
-        row.cpass = co.nir;
-        row.cstat = co.status;
-        if (po.status != KKT_SOLVED || co.status != KKT_SOLVED) {
-          compute_stats();
-          status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
-        } else {
-          double stp = maxsteps(NV_DP, NV_DD, set_.step_frac);
-          row.step = stp;
-          set_scalar(S_STEP, stp);
-          axpby(0, NV_DP, NV_P, NV_P, KS(1.0, S_STEP), K(1.0));
-          axpby(0, NV_DD, NV_D, NV_D, KS(1.0, S_STEP), K(1.0));
-          axpby(1, MV_DY, MV_Y, MV_Y, KS(1.0, S_STEP), K(1.0));
+          axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
+          KKTOut po = solvekkt(NV_DPA, MV_DYA, NV_FR, MV_DG, false, ftol, gtol, 1, false);
+          // Δda = Q Δpa − Bᵀ Δya − Δf
+          spmv(1, MV_DYA, NV_T0, K(1.0), K(0.0));
+          spmv(2, NV_DPA, NV_DDA, K(1.0), K(0.0));
+          axpbypcz(0, NV_T0, NV_DF, NV_DDA, NV_DDA, K(-1.0), K(-1.0), K(1.0));
+          launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DDA);
+          // corrector (solver/ipm.jl:181-272)
+          double taua = maxsteps(NV_DPA, NV_DDA, 1.0);
+          dot(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA);
+          sync_scalars();
+          double smu = hsc_[S_TMP2];
+          double mua = smu / nu_;
+          if (mut > 0) {
+            double dmu = mu - mut;
+            if (dmu > 0) {
+              double dmua = std::min(std::max(mua - mut, 0.0), dmu);
+              smu = mut + std::min(std::max(dmua * (dmua / dmu) * (dmua / dmu), 0.0), dmu);
+            } else {
+              smu = mut;
+            }
+          } else {
+            smu = std::min(std::max(mua * (mua / mu) * (mua / mu), 0.0), mu);
+          }
+          (void)taua;
+          set_scalar(S_SMU, smu);
+          launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
+          launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
+                          NV_DDA);
+          axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
+          axpby(0, NV_DPA, NV_DPA, NV_DP, K(1.0), K(0.0));
+          axpby(1, MV_DYA, MV_DYA, MV_DY, K(1.0), K(0.0));
+          KKTOut co = solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, true, ftol, gtol, 1, false);
+          spmv(1, MV_DY, NV_T0, K(1.0), K(0.0));
+          spmv(2, NV_DP, NV_DD, K(1.0), K(0.0));
+          axpbypcz(0, NV_T0, NV_DF, NV_DD, NV_DD, K(-1.0), K(-1.0), K(1.0));
+          launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DD);
+          row.piter = po.ncg;
+          row.ppass = po.nir;
+          row.pstat = po.status;
+          row.dmin = po.dmin;
+          row.dmax = po.dmax;
+          row.citer = co.ncg;
+          row.cpass = co.nir;
+          row.cstat = co.status;
+          if (attempt == 0 && fac_fp32_ && (po.status != KKT_SOLVED || co.status != KKT_SOLVED)) {
+            mixed_off_ = true;
+            ok = factor(delta, rho);
+            row.rho = rho;
+            if (ok) continue;
+          }
+          if (po.status != KKT_SOLVED || co.status != KKT_SOLVED) {
+            compute_stats();
+            status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
+          } else {
+            double stp = maxsteps(NV_DP, NV_DD, set_.step_frac);
+            row.step = stp;
+            set_scalar(S_STEP, stp);
+            axpby(0, NV_DP, NV_P, NV_P, KS(1.0, S_STEP), K(1.0));
+            axpby(0, NV_DD, NV_D, NV_D, KS(1.0, S_STEP), K(1.0));
+            axpby(1, MV_DY, MV_Y, MV_Y, KS(1.0, S_STEP), K(1.0));
+          }
+          break;
         }
       }
     }

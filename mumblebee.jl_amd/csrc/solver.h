@@ -34,6 +34,8 @@ struct Settings {
   int amalgamate = 1;
   int big_front_ld = BIG_LD;  // fronts with ld ≥ this take the persistent kernels
   int dag_grid = 0;           // workgroups of the persistent factorization (0: one per CU)
+  int precision = 0;          // 1: large fronts factored in FP32, refined in FP64
+  double mixed_mu = 1e-5;     // precision 1: μ below which the factor returns to FP64
 };
 
 struct HistRow {
@@ -103,6 +105,7 @@ class Solver {
   double last_factor_ms() const { return fac_ms_; }
   double total_factor_ms() const { return fac_ms_tot_; }
   int factor_calls() const { return fac_calls_; }
+  int factor_calls_fp32() const { return fac_fp32_calls_; }
   void dense_factor(double* out);  // n×n lower factor (debug; small n)
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
   hipStream_t stream() const { return st_; }
@@ -137,7 +140,7 @@ class Solver {
   void lsolve(int kv);
   void usolve(int kv);
   bool factor(double delta, double& rho);
-  void run_factor();
+  void run_factor(bool fp32);
   double maxsteps(int kdp, int kdd, double step_frac);
   void reinit(const double* p0, const double* d0, const double* y0);
   // KKT
@@ -213,13 +216,17 @@ class Solver {
     BigFront B;
     int front;
     long long sbase = 0;
-    DevBuf tb, state, tasks, sflag, dinv, stage, trace;
+    DevBuf tb, state, tasks, sflag, dinv, stage, trace, f32, dinv32;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
   DevBuf d_bigtick_, d_bigstick_;
   int epoch_ = 0;
   int dag_grid_ = 512;
   int fac_epoch_ = 0;     // factorization counter: a large front's tile flags hold it once final
+  bool fac_fp32_ = false; // the current factor of the large fronts is FP32
+  bool mixed_off_ = false;  // mixed precision fell back to FP64 for the rest of this solve
+  double cur_mu_ = 0.0;   // μ of the iteration being factored (selects the factor precision)
+  int fac_fp32_calls_ = 0;
   size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
   void check_abort();
 };

@@ -111,7 +111,7 @@ class _Settings(ctypes.Structure):
                 ("newton_max_iter", ctypes.c_int32), ("aug_tol", ctypes.c_double),
                 ("infeas_abs", ctypes.c_double), ("infeas_rel", ctypes.c_double),
                 ("amalgamate", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("big_front_ld", ctypes.c_int32), ("dag_grid", ctypes.c_int32)]
+                ("big_front_ld", ctypes.c_int32), ("dag_grid", ctypes.c_int32), ("mixed_mu", ctypes.c_double)]
 
 
 EXPORTS = {
@@ -150,6 +150,7 @@ EXPORTS = {
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
+    "mb_probe_mfma_f32": (ctypes.c_int, [_DP]),
     "mb_last_error": (ctypes.c_char_p, []),
 }
 
@@ -188,8 +189,10 @@ def _csc(A, m, n):
 
 @dataclass
 class IPMSettings:
-    """IPMSettings{Float64} (settings/ipm.jl:1-18); `amalgamate` and `precision` are this
-    build's knobs (relaxed supernode amalgamation; 0 = FP64 factor)."""
+    """IPMSettings{Float64} (settings/ipm.jl:1-18) plus this build's knobs: `amalgamate`
+    (relaxed supernode amalgamation), `precision` (0: FP64 factor; 1: the large fronts factored
+    in FP32 on FP32 MFMA with FP64 Uzawa/CG refinement, back to FP64 once μ < `mixed_mu`),
+    `big_front_ld`, `dag_grid`."""
     verbose: int = 0
     step_frac: float = 0.99
     feas_tol: float = 1e-8
@@ -208,6 +211,7 @@ class IPMSettings:
     precision: int = 0
     big_front_ld: int = 512
     dag_grid: int = 0
+    mixed_mu: float = 1e-5
 
     def _c(self):
         s = _Settings()
@@ -375,11 +379,12 @@ class IPMSolver:
         return out
 
     def stats(self):
-        st = np.zeros(7)
+        st = np.zeros(8)
         if lib().mb_solver_stats(self._h, _p(st, ctypes.c_double)) != 0:
             raise _err("stats")
         return dict(exact_flops=st[0], front_flops=st[1], nfront=int(st[2]), nlevel=int(st[3]),
-                    factor_ms_total=st[4], factor_calls=int(st[5]), factor_ms_last=st[6])
+                    factor_ms_total=st[4], factor_calls=int(st[5]), factor_ms_last=st[6],
+                    factor_calls_fp32=int(st[7]))
 
     def reinit(self, p0=None, d0=None, y0=None):
         """reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429)."""
@@ -535,5 +540,12 @@ def device_info():
 def probe_mfma_f64():
     out = np.zeros(256)
     if lib().mb_probe_mfma_f64(_p(out, ctypes.c_double)) != 0:
+        raise _err("probe")
+    return out
+
+
+def probe_mfma_f32():
+    out = np.zeros(256)
+    if lib().mb_probe_mfma_f32(_p(out, ctypes.c_double)) != 0:
         raise _err("probe")
     return out
