@@ -41,12 +41,41 @@ METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP
 TRAFFIC_BYTES = {(2000, 1000): 144.0e6}
 
 
-def workload(seed, n, m):
+WORKLOADS = {
+    "qp": "sparse convex QP n={n} m={m} on the nonnegative orthant, Q=PᵀP+1e-3I (P 0.5% dense), B 1% dense, N(0,1)",
+    "socp": "SOCP n=10000 m=5000: 500 second-order cones (sizes U{{3..50}} scaled to 9000 columns) + nonneg, "
+            "B 0.2% dense, N(0,1), strictly feasible g, dual-feasible f",
+    "maxcut": "max-cut SDP relaxation, 2000 vertices, band 20 + 0.1% extra edges inside windows, C = -L/4, "
+              "chordal windows of 48 vertices (71 PSD cones of order 48, diag and consensus rows)",
+    "exppow": "2000 exponential cones (entropy on the simplex) + 500 power cones (alpha U(0.2,0.8)) with budgets, "
+              "499 sparse equalities at 0.5% density, strictly feasible g",
+}
+
+
+def workload(kind, seed, n, m):
     from mumblebee_hip import generators as G
-    return G.qp(n=n, m=m, seed=seed)
+    if kind == "qp":
+        return G.qp(n=n, m=m, seed=seed)
+    if kind == "socp":
+        return G.socp(seed=seed)
+    if kind == "maxcut":
+        return G.maxcut(nv=2000, band=20, clique=48, seed=seed)
+    if kind == "exppow":
+        return G.exppow(seed=seed)
+    raise ValueError(kind)
 
 
-def cpu_baseline(n, m, seed):
+def hip_problem(mh, d):
+    kinds = {"pos": mh.PositiveCone, "free": mh.CofreeCone, "soc": mh.SecondOrderCone,
+             "psd": mh.SemidefiniteCone, "exp": mh.ExponentialCone}
+    cones = [mh.PowerCone(c[1]) if c[0] == "pow" else kinds[c[0]]() for c in d["K"]]
+    return mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, cones, d["s"])
+
+
+def cpu_baseline(kind, n, m, seed, gpu_iters, budget_s=30.0):
+    """The oracle (numpy + LAPACK) on one instance: a whole solve when it ends within the budget,
+    otherwise the IPM iterations done within it, converted to solves/s at the GPU's iterations
+    per solve."""
     from oracle import ipm as O
     from oracle.problem import CSC, make_problem
     try:
@@ -54,21 +83,35 @@ def cpu_baseline(n, m, seed):
         cores = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
     except Exception:
         cores = os.cpu_count() or 1
-    d = workload(seed, n, m)
+    d = workload(kind, seed, n, m)
     t0 = time.perf_counter()
     prob = make_problem(CSC.from_scipy(d["Q"]), CSC.from_scipy(d["B"]), d["f"], d["g"], 0.0, d["K"], d["s"])
-    solver = O.Solver(prob, order="identity")
+    solver = O.Solver(prob, order="identity" if kind == "qp" else "mindegree")
     t1 = time.perf_counter()
-    res = solver.solve()
+    it, st = 0, 0
+    while st == 0 and time.perf_counter() - t1 < budget_s:
+        st = solver.step()
+        it += 1
     t2 = time.perf_counter()
-    return dict(value=1.0 / (t2 - t1), unit="solves/s", cores=int(cores), kind="port",
-                sample=f"1 solve of the QP n={n} m={m} seed={seed} by the numpy/LAPACK oracle "
-                       f"(solve {t2 - t1:.2f} s after {t1 - t0:.2f} s setup; status "
-                       f"{O.STATUS_NAMES[res.status]}, {res.niter} iterations)")
+    if st != 0:
+        return dict(value=1.0 / (t2 - t1), unit="solves/s", cores=int(cores), kind="port",
+                    sample=f"1 solve of the {kind} instance seed={seed} by the numpy/LAPACK oracle "
+                           f"(solve {t2 - t1:.2f} s after {t1 - t0:.2f} s setup; status {O.STATUS_NAMES[st]}, "
+                           f"{it} iterations)")
+    per_it = (t2 - t1) / it
+    return dict(value=1.0 / (per_it * gpu_iters), unit="solves/s", cores=int(cores), kind="port",
+                sample=f"{it} IPM iterations of the {kind} instance seed={seed} by the numpy/LAPACK oracle in "
+                       f"{t2 - t1:.1f} s (after {t1 - t0:.1f} s setup), scaled to {gpu_iters:.1f} iterations "
+                       f"per solve")
 
 
-def run_concurrent(solvers, slots):
-    """Solve every solver from `slots` host threads; returns (elapsed s, latencies s, results)."""
+def note(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_concurrent(solvers, slots, tag=None):
+    """Solve every solver from `slots` host threads; returns (elapsed s, latencies s, results).
+    With a tag, each finished solve is noted on stderr."""
     lat = [0.0] * len(solvers)
     res = [None] * len(solvers)
     nxt = [0]
@@ -84,6 +127,8 @@ def run_concurrent(solvers, slots):
             a = time.perf_counter()
             res[i] = solvers[i].solve()
             lat[i] = time.perf_counter() - a
+            if tag:
+                note(f"{tag}: solve {i} {res[i].status.name} in {lat[i]:.2f} s ({res[i].niter} iterations)")
 
     t0 = time.perf_counter()
     threads = [threading.Thread(target=worker) for _ in range(max(1, slots))]
@@ -94,48 +139,68 @@ def run_concurrent(solvers, slots):
     return time.perf_counter() - t0, lat, res
 
 
-def bench_resident(args, mh):
+def _resident_run(args, mh, precision):
+    """Set up W + K resident instances at the given precision, solve the W untimed and then the
+    K timed ones `--slots` at a time; returns the measurements and the timed results."""
     K, W, S = args.steps, args.warmup, max(1, args.slots)
     ncu = mh.device_info()["cus"]
-    settings = mh.IPMSettings(dag_grid=max(16, ncu // S))
+    settings = mh.IPMSettings(dag_grid=max(16, ncu // S) if S > 1 else 0, precision=precision)
     kinds = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
+    # exponential cones can end NEAR_OPTIMAL: the reference's line search stalls within a few
+    # noise bands of a cone boundary (a documented property of the path, not of this build)
+    good = (mh.IPMStatus.OPTIMAL, mh.IPMStatus.NEAR_OPTIMAL) if args.workload == "exppow" else (mh.IPMStatus.OPTIMAL,)
     t_setup = time.perf_counter()
     solvers = []
     for k in range(W + K):
-        d = workload(1000 + k, args.n, args.m)
-        prob = mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, [mh.PositiveCone()], d["s"])
-        solvers.append(mh.init(prob, settings))
+        solvers.append(mh.init(hip_problem(mh, workload(args.workload, 1000 + k, args.n, args.m)), settings))
     t_setup = (time.perf_counter() - t_setup) / (W + K)
+    tag = None if args.workload == "qp" else f"{args.workload} precision {precision}"
+    note(f"{args.workload} precision {precision}: {W + K} instances set up ({t_setup:.2f} s each)")
     # one instance alone on the whole GPU (latency reference; not part of the timed job)
-    d = workload(999, args.n, args.m)
-    solo = mh.init(mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, [mh.PositiveCone()], d["s"]))
+    solo = mh.init(hip_problem(mh, workload(args.workload, 999, args.n, args.m)), mh.IPMSettings(precision=precision))
     solo.solve()
     solo.reinit()
     a = time.perf_counter()
     r = solo.solve()
     solo_ms = (time.perf_counter() - a) * 1e3
-    assert r.status == mh.IPMStatus.OPTIMAL, r.status
+    assert r.status in good, r.status
     solo.close()
+    note(f"{args.workload} precision {precision}: solo solve {solo_ms:.1f} ms")
     if W:
-        _, _, wres = run_concurrent(solvers[:W], S)
-        assert all(x.status == mh.IPMStatus.OPTIMAL for x in wres), [x.status for x in wres]
+        _, _, wres = run_concurrent(solvers[:W], S, tag)
+        assert all(x.status in good for x in wres), [x.status for x in wres]
+        for s in solvers[:W]:
+            s.close()
     timed = solvers[W:]
     for s in timed:
         s.set_profile(kinds)
-    elapsed, lat, res = run_concurrent(timed, S)
-    assert all(x.status == mh.IPMStatus.OPTIMAL for x in res), [x.status for x in res]
+    elapsed, lat, res = run_concurrent(timed, S, tag)
+    note(f"{args.workload} precision {precision}: {K} timed solves in {elapsed:.2f} s")
+    assert all(x.status in good for x in res), [x.status for x in res]
     prof = {}
     for s in timed:
         for k, v in s.profile().items():
             acc = prof.setdefault(k, dict(ms=0.0, launches=0, flops=0.0, bytes=0.0))
             for key in acc:
                 acc[key] += v[key]
-    st0 = timed[0].stats()
+    stats = [s.stats() for s in timed]
+    for s in timed:
+        s.close()
+    lat_ms = np.array(lat) * 1e3
+    return dict(settings=settings, elapsed=elapsed, lat_ms=lat_ms, res=res, prof=prof, stats=stats,
+                solo_ms=solo_ms, t_setup=t_setup)
+
+
+def bench_resident(args, mh):
+    K, W, S = args.steps, args.warmup, max(1, args.slots)
+    plain = _resident_run(args, mh, 0)
+    mixed = _resident_run(args, mh, 1) if not args.no_mixed else None
+    settings, elapsed, lat_ms, res, prof = (plain[k] for k in ("settings", "elapsed", "lat_ms", "res", "prof"))
+    st0 = plain["stats"][0]
     fac = prof["factor"]
     dom_name = "dag" if prof["dag"]["ms"] >= prof["update"]["ms"] else "update"
     dom = prof[dom_name]
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else 0.0
-    lat_ms = np.array(lat) * 1e3
     line = {
         "metric": METRIC,
         "value": K / elapsed,
@@ -150,22 +215,24 @@ def bench_resident(args, mh):
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"sparse convex QP n={args.n} m={args.m} on the nonnegative orthant, "
-                        f"Q=PᵀP+1e-3I (P 0.5% dense), B 1% dense, N(0,1); {K} distinct seeded instances, "
-                        f"resident, cold start, gap_tol=feas_tol=1e-8, {S} solved at a time",
-            "precision": "fp64",
+            "workload": WORKLOADS[args.workload].format(n=args.n, m=args.m) +
+                        f"; {K} distinct seeded instances, resident, cold start, gap_tol=feas_tol=1e-8, "
+                        f"{S} solved at a time",
+            "statuses": {mh.IPMStatus(int(x.status)).name: int(sum(int(y.status) == int(x.status) for y in res))
+                         for x in res},
+            "precision": "fp64 (plain); the mixed form is reported under mixed",
             "slots": S,
             "dag_grid": settings.dag_grid,
             "p50_ms": float(np.percentile(lat_ms, 50)),
             "p99_ms": float(np.percentile(lat_ms, 99)),
-            "solo_ms": solo_ms,
+            "solo_ms": plain["solo_ms"],
             "iterations_mean": float(np.mean([x.niter for x in res])),
             "factor_tflops_aggregate": fac["flops"] / elapsed / 1e12,
             "factor_ms_per_call": fac["ms"] / max(1, fac["launches"]),
             "factor_exact_flops": st0["exact_flops"],
             "factor_front_flops": st0["front_flops"],
             "nfront": st0["nfront"],
-            "setup_s_per_instance": t_setup,
+            "setup_s_per_instance": plain["t_setup"],
             "kernel_ms": {k: v["ms"] for k, v in prof.items()},
             "kernel_launches": {k: v["launches"] for k, v in prof.items()},
         },
@@ -175,15 +242,37 @@ def bench_resident(args, mh):
             "peak": PEAK_FP64_MFMA,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_MFMA,
-            "traffic": TRAFFIC_BYTES.get((args.n, args.m)) if dom_name == "dag" else None,
+            "traffic": (TRAFFIC_BYTES.get((args.n, args.m)) if dom_name == "dag" and args.workload == "qp"
+                        else None),
             "kernel": ("k_dag_factor (persistent left-looking tile FP64 MFMA Cholesky of the large fronts)"
                        if dom_name == "dag" else "k_update (FP64 MFMA trailing update of the frontal Cholesky)"),
             "avg_launch_us": dom["ms"] / max(1, dom["launches"]) * 1e3,
             "flops_per_launch": dom["flops"] / max(1, dom["launches"]),
         },
     }
+    if mixed is not None:
+        # the same seeded instances: per-result error of the mixed form against the FP64 results
+        err = [max(abs(a.pobj - b.pobj) / max(1.0, abs(b.pobj)), abs(a.dobj - b.dobj) / max(1.0, abs(b.dobj)))
+               for a, b in zip(mixed["res"], res)]
+        perr = [float(np.linalg.norm(a.p - b.p) / max(1e-300, np.linalg.norm(b.p))) for a, b in zip(mixed["res"], res)]
+        mp = mixed["prof"]
+        line["config"]["mixed"] = {
+            "solves_per_s": K / mixed["elapsed"],
+            "vs_plain": (K / mixed["elapsed"]) / (K / elapsed),
+            "p50_ms": float(np.percentile(mixed["lat_ms"], 50)),
+            "p99_ms": float(np.percentile(mixed["lat_ms"], 99)),
+            "solo_ms": mixed["solo_ms"],
+            "iterations_mean": float(np.mean([x.niter for x in mixed["res"]])),
+            "fp32_factor_share": float(np.sum([s["factor_calls_fp32"] for s in mixed["stats"]]) /
+                                       max(1, np.sum([s["factor_calls"] for s in mixed["stats"]]))),
+            "factor_ms_per_call": mp["factor"]["ms"] / max(1, mp["factor"]["launches"]),
+            "max_rel_obj_err_vs_fp64": float(np.max(err)),
+            "max_rel_p_err_vs_fp64": float(np.max(perr)),
+            "kernel_ms": {k: v["ms"] for k, v in mp.items()},
+        }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.n, args.m, 1000 + W)
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.n, args.m, 1000 + W,
+                                            float(np.mean([x.niter for x in res])))
     return line
 
 
@@ -246,12 +335,14 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--n", type=int, default=2000)
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="qp")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-nmax", type=int, default=4000)

@@ -1108,6 +1108,7 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
   launch_td_initcache(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_tdlist_.as<int>(), ntd_);  // initcache!
   HIPCHK(hipStreamSynchronize(st_));
   hist_.clear();
+  hist_fp32_.clear();
   nH_ = norm2(P_.Q.val);
   status_ = CONTINUE;
   mixed_off_ = false;
@@ -1272,7 +1273,11 @@ double Solver::getaug() {
   if (hist_.empty()) return set_.aug_tol * nB_ * nB_ / nH_;
   const HistRow& h = hist_.back();
   double delta = h.delta;
-  if (h.ppass > 1) {
+  // mixed precision: after an FP32-factor iteration a second refinement pass reflects the FP32
+  // rounding floor rather than too large a δ, so only the window applies (the window then keeps
+  // δ large enough for the FP32 factorization to stay positive definite)
+  const bool fp32 = !hist_fp32_.empty() && hist_fp32_.back();
+  if (h.ppass > 1 && !fp32) {
     if (std::isfinite(h.dmin)) delta = std::max(delta, h.dmin);
     delta *= 10;
   } else if (std::isfinite(h.dmin) && std::isfinite(h.dmax) && h.dmin > 0 && h.dmax > 0) {
@@ -1494,6 +1499,8 @@ int Solver::step() {
             row.rho = rho;
             if (ok) continue;
           }
+          // once the FP32 factor needs a second refinement pass, FP64 for the rest of the solve
+          if (fac_fp32_ && po.nir >= 2) mixed_off_ = true;
           if (po.status != KKT_SOLVED || co.status != KKT_SOLVED) {
             compute_stats();
             status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
@@ -1511,6 +1518,7 @@ int Solver::step() {
     }
   }
   hist_.push_back(row);
+  hist_fp32_.push_back(fac_fp32_ ? 1 : 0);
   status_ = status;
   return status;
 }

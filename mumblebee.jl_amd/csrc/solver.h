@@ -225,6 +225,7 @@ class Solver {
   int fac_epoch_ = 0;     // factorization counter: a large front's tile flags hold it once final
   bool fac_fp32_ = false; // the current factor of the large fronts is FP32
   bool mixed_off_ = false;  // mixed precision fell back to FP64 for the rest of this solve
+  std::vector<char> hist_fp32_;  // per history row: the iteration's large fronts were FP32
   double cur_mu_ = 0.0;   // μ of the iteration being factored (selects the factor precision)
   int fac_fp32_calls_ = 0;
   size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
