@@ -637,6 +637,39 @@ __global__ void k_big_gather(const FrontDesc* fronts, int f, const int* childlis
   }
 }
 
+constexpr int KB = 8;  // solution blocks a solve workgroup takes at once
+
+// The solve chains: thread 0 waits (bounded) until block j is published; lanes 0..n−1 of wave 0
+// read the flags of blocks j, j+step, …, and the count of consecutive published blocks from j
+// is returned to every thread (0 on abort). Taking every block that is already there at once
+// spreads a workgroup's per-block overhead (flag and x round trips, barriers) over the batch, so
+// a workgroup that lags the chain catches up instead of pacing it.
+__device__ __forceinline__ int take_published(const int* flags, int j, int step, int n, int epoch, int* abortf) {
+  __shared__ int cnt;
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    bool ok = lane < n && ld_flag(&flags[j + step * lane]) >= epoch;
+    if (lane == 0 && !ok) {
+      long spins = 0;
+      while (!ok) {
+        if (ld_flag(abortf) != 0) break;
+        if (++spins > SPIN_LIMIT) {
+          st_flag(abortf, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        ok = ld_flag(&flags[j]) >= epoch;
+      }
+    }
+    const unsigned long long m = __ballot(ok);
+    if (lane == 0) cnt = (m & 1ull) ? __builtin_ctzll(~m) : 0;  // leading published blocks
+  }
+  __syncthreads();
+  const int r = cnt;
+  __syncthreads();  // nobody may overwrite cnt before every thread has read it
+  return r;
+}
+
 // forward: tile b (ticket order) accumulates L[b, j] x_j for published j < min(b, Tr); a
 // factor tile then forms x_b = L_bb⁻¹ (v_b − Σ) and publishes it, a separator tile adds Σ into
 // the front's update vector. TF: element type of the stored factor; the arithmetic is FP64.
@@ -645,7 +678,7 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
                                                    int kv, int epoch, long long base) {
   __shared__ double Sinv[NB * LDT];
   __shared__ double Sc[NB * LDT];
-  __shared__ double xs[NB];
+  __shared__ double xs[KB * NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -667,19 +700,27 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   if (diag) load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
   if (jmax > 0) load_tile<false>(Sc, F, LD, r0, B.tb[jmax - 1], h, B.tb[jmax] - B.tb[jmax - 1]);
   double acc = 0.0;
-  for (int j = 0; j < jmax; ++j) {
-    if (!wait_geq(&B.sflag[j], epoch, B.abortf)) return;
-    const int c0 = B.tb[j], w = B.tb[j + 1] - c0;
-    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < w ? cload(&v[c0 + threadIdx.x]) : 0.0;
+  for (int j = 0; j < jmax;) {
+    const int n = take_published(B.sflag, j, 1, min(KB, jmax - j), epoch, B.abortf);
+    if (n == 0) return;
+    const int c0 = B.tb[j], wtot = B.tb[j + n] - c0;
+    for (int t = threadIdx.x; t < KB * NB; t += NT) xs[t] = t < wtot ? cload(&v[c0 + t]) : 0.0;
     __syncthreads();
-    if (j == jmax - 1) {
-#pragma unroll
-      for (int c = q; c < NB; c += 4) acc += Sc[row + c * LDT] * xs[c];
-    } else if (row < h) {
+    // the batch's columns straight from the factor, except block jmax−1 (staged in LDS)
+    const bool last = j + n == jmax;
+    const int wg = last ? B.tb[jmax - 1] - c0 : wtot;
+    if (row < h) {
       const TF* col = F + (size_t)(r0 + row) + (size_t)c0 * LD;
-      for (int c = q; c < w; c += 4) acc += (double)col[(size_t)c * LD] * xs[c];
+#pragma unroll 4
+      for (int c = q; c < wg; c += 4) acc += (double)col[(size_t)c * LD] * xs[c];
+    }
+    if (last) {
+      const int o = B.tb[jmax - 1] - c0;
+#pragma unroll
+      for (int c = q; c < NB; c += 4) acc += Sc[row + c * LDT] * xs[o + c];
     }
     __syncthreads();
+    j += n;
   }
   part[q][row] = acc;
   __syncthreads();
@@ -704,10 +745,10 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
-  __shared__ double S[NB * LDT];
+  __shared__ double S[2][NB * LDT];  // double buffer: the next tile loads while this one is used
   __shared__ double Sinv[NB * LDT];
   __shared__ double Sc[NB * LDT];
-  __shared__ double xs[NB];
+  __shared__ double xs[KB * NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -727,27 +768,48 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
   if (b + 1 < B.Tr) load_tile<false>(Sc, F, LD, B.tb[b + 1], c0, B.tb[b + 2] - B.tb[b + 1], w);
   double acc = 0.0;
-  // separator part (rows ≥ r; x_sep is final), 64-row chunks through LDS so that the global
-  // reads run down the columns
+  int cur = 0;
+  TileRegs<TF> R;
+  // separator part (rows ≥ r; x_sep is final): 64-row chunks through LDS so that the global reads
+  // run down the columns, the next chunk in flight while this one is multiplied
+  if (Fd.r < LD) fetch_tile<false>(R, F, LD, Fd.r, c0, min(NB, LD - Fd.r), w);
   for (int rs0 = Fd.r; rs0 < LD; rs0 += NB) {
     const int h = min(NB, LD - rs0);
-    load_tile<false>(S, F, LD, rs0, c0, h, w);
+    put_tile(S[cur], R);
     if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
     __syncthreads();
+    if (rs0 + NB < LD) fetch_tile<false>(R, F, LD, rs0 + NB, c0, min(NB, LD - rs0 - NB), w);
 #pragma unroll
-    for (int rr = q; rr < NB; rr += 4) acc += S[rr + col * LDT] * xs[rr];
+    for (int rr = q; rr < NB; rr += 4) acc += S[cur][rr + col * LDT] * xs[rr];
     __syncthreads();
+    cur ^= 1;
   }
-  for (int j = B.Tr - 1; j > b; --j) {
-    if (!wait_geq(&B.sflag[j], epoch, B.abortf)) return;
-    const int r0 = B.tb[j], h = B.tb[j + 1] - r0;
-    if (j != b + 1) load_tile<false>(S, F, LD, r0, c0, h, w);
-    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? cload(&v[r0 + threadIdx.x]) : 0.0;
+  for (int j = B.Tr - 1; j > b;) {
+    // blocks j, j−1, …, j−n+1 (all published), their rows [rlo, rhi); tile b+1 is staged (Sc),
+    // the others stream through the double buffer
+    const int n = take_published(B.sflag, j, -1, min(KB, j - b), epoch, B.abortf);
+    if (n == 0) return;
+    const int klo = j - n + 1, rlo = B.tb[klo], rhi = B.tb[j + 1];
+    for (int t = threadIdx.x; t < KB * NB; t += NT) xs[t] = t < rhi - rlo ? cload(&v[rlo + t]) : 0.0;
+    int kn = klo == b + 1 ? klo + 1 : klo;  // next tile to fetch
+    if (kn <= j) fetch_tile<false>(R, F, LD, B.tb[kn], c0, B.tb[kn + 1] - B.tb[kn], w);
     __syncthreads();
-    const double* T = (j == b + 1) ? Sc : S;
+    for (int k = klo; k <= j; ++k) {
+      const double* T = Sc;
+      if (k != b + 1) {
+        put_tile(S[cur], R);
+        __syncthreads();
+        T = S[cur];
+        cur ^= 1;
+        kn = k + 1;
+        if (kn <= j) fetch_tile<false>(R, F, LD, B.tb[kn], c0, B.tb[kn + 1] - B.tb[kn], w);
+      }
+      const double* x = xs + (B.tb[k] - rlo);
 #pragma unroll
-    for (int rr = q; rr < NB; rr += 4) acc += T[rr + col * LDT] * xs[rr];
+      for (int rr = q; rr < NB; rr += 4) acc += T[rr + col * LDT] * x[rr];
+    }
     __syncthreads();
+    j -= n;
   }
   part[q][col] = acc;
   __syncthreads();
