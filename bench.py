@@ -284,14 +284,17 @@ def bench_serve(args, mh):
     reqs = []
     for k in range(K):
         n = int(G.log_uniform_n(rng, 50, args.serve_nmax))
-        if rng.random() < 0.7:
+        u = rng.random()
+        if u < 0.6:
             d = G.qp(n=n, m=max(1, n // 2), seed=5000 + k)
-        else:
+        elif u < 0.9:
             d = G.socp(n=n, m=max(1, n // 2), ncone=max(1, n // 40), seed=5000 + k, adens=min(0.2, 20.0 / n))
-        cones = [{"pos": mh.PositiveCone, "free": mh.CofreeCone, "soc": mh.SecondOrderCone}[c[0]]() for c in d["K"]]
-        reqs.append(mh.IPMProblem(d["Q"], d["B"], d["f"], d["g"], 0.0, cones, d["s"]))
+        else:  # a small max-cut SDP: windows of 10 vertices (PSD cones of order 10, 55 columns each)
+            d = G.maxcut(nv=max(12, n // 25), band=4, clique=10, seed=5000 + k)
+        reqs.append(hip_problem(mh, d))
     gaps = rng.exponential(1.0 / args.rate, K)
     server = mh.IPMServer(nslots=args.slots)
+    settings = mh.IPMSettings(precision=args.precision)
     t0 = mh.clock()
     arrivals, tickets = [], []
     for k in range(K):
@@ -299,7 +302,8 @@ def bench_serve(args, mh):
         while mh.clock() < target:
             time.sleep(min(0.002, max(0.0, target - mh.clock())))
         arrivals.append(mh.clock())
-        tickets.append(server.submit(reqs[k]))
+        tickets.append(server.submit(reqs[k], settings))
+    note(f"served stream: {K} requests submitted")
     out = [server.wait(t) for t in tickets]
     st = server.stats()
     server.close()
@@ -311,8 +315,10 @@ def bench_serve(args, mh):
         "metric": METRIC, "value": K / elapsed, "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": 0,
         "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"served stream: {K} requests, 70% QP / 30% SOCP, n log-uniform in "
-                               f"[50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, {args.slots} slots",
+        "config": {"workload": f"served stream: {K} requests, 60% QP / 30% SOCP / 10% small max-cut SDP, n "
+                               f"log-uniform in [50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, "
+                               f"{args.slots} slots",
+                   "precision": "mixed" if args.precision else "fp64",
                    "optimal": ok, "p50_ms": float(np.percentile(lat_ms, 50)),
                    "p99_ms": float(np.percentile(lat_ms, 99)), "peak_reserved_gb": st["peak_reserved_bytes"] / 2 ** 30},
     }
@@ -343,6 +349,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="qp")
+    ap.add_argument("--precision", type=int, default=0, choices=[0], help="--serve: FP64 factors only")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-nmax", type=int, default=4000)
