@@ -13,6 +13,10 @@
  *   mb_solver_result         result(solver, status) → IPMResult           src/IPM/src/ipm/solver/ipm.jl:30-74
  *   mb_solver_history        IPMHistory rows                              src/IPM/src/ipm/history/ipm.jl:1-24
  *   mb_solver_kkt_factor     initkkt!(kkt, H; δ) → (ok, ρ)                src/IPM/src/kkt/uzawa.jl:73-115
+ *   mb_cone_scale            scale!(sched, K, H, Q, caches, p, d, …)      src/IPM/src/cone/cone.jl:146-247
+ *   mb_cone_corr0            initpredictor! / corr0!                      src/IPM/src/cone/cone.jl:361-444
+ *   mb_cone_corr             initcorrector! / corr!                       src/IPM/src/cone/cone.jl:261-352
+ *   mb_cone_maxsteps         maxsteps(sched, K, p, d, Δp, Δd, …)          src/IPM/src/cone/cone.jl:455-553
  *
  * Sparse inputs are CSC with 0-based int64 indices, rows sorted within each column (stored zeros
  * are kept, as in Julia's SparseMatrixCSC). Cone kinds: 0 nonnegative, 1 free ("cofree"),
@@ -139,6 +143,30 @@ int mb_solver_dag_trace(mb_solver* s, int32_t front, int64_t* out, int32_t len);
 /* initkkt!(kkt, H; δ) on the solver's current H (uzawa.jl:73-115): returns 1 on success,
  * 0 on a failed factorization, negative on error; *rho receives the shift used. */
 int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);
+
+/* Cone fan-outs on the solver's iterate in its internal coordinates (columns permuted by the
+ * problem construction and scaled by the equilibration, as the reference's IPMSolver holds p and
+ * d). All vectors have length n.
+ * mb_cone_scale: sets the iterate to (p, d), forms every cone's scaling block and cache; returns
+ * 1 if every cone accepted the point (the reference's all(flags)), 0 if not, −1 on error; spsd ←
+ * Σ⟨p*, d*⟩. mb_cone_blocks: the blocks D_v in cone order — n_v entries for a nonnegative cone
+ * (the diagonal), none for a free cone, n_v² column-major for the others; mb_cone_blocks_size
+ * gives the total. mb_cone_corr0: r = the predictor right-hand side −d + σμ(−f′(p)) per cone.
+ * mb_cone_corr: r = the Mehrotra corrector for directions (dp, dd). mb_cone_maxsteps: step ←
+ * the largest τ ≤ 1 keeping both iterates interior, × step_frac except on free cones. */
+int64_t mb_solver_ncone(const mb_solver* s);
+/* The internal layout: P2[j] = the caller's column of internal column j (n entries); cone v owns
+ * internal columns xcol[v]..xcol[v+1]−1 and has kind[v], param[v] (ncone entries, xcol ncone+1);
+ * colscale[j] = the equilibration factor (internal p = p_caller[P2[j]] / colscale[j], internal d =
+ * d_caller[P2[j]] · colscale[j]). */
+int mb_solver_layout(const mb_solver* s, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,
+                     double* colscale);
+int mb_cone_scale(mb_solver* s, const double* p, const double* d, double* spsd);
+int64_t mb_cone_blocks_size(const mb_solver* s);
+int mb_cone_blocks(mb_solver* s, double* out);
+int mb_cone_corr0(mb_solver* s, double sigma_mu, double* r);
+int mb_cone_corr(mb_solver* s, const double* dp, const double* dd, double sigma_mu, double* r);
+int mb_cone_maxsteps(mb_solver* s, const double* dp, const double* dd, double step_frac, double* step);
 
 /* Debug: the current factor as a dense n×n column-major lower triangle (small n only). */
 int mb_solver_dense_factor(mb_solver* s, double* L);

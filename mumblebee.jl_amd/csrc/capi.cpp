@@ -317,6 +317,59 @@ int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho) {
   return guard([&]() { return s->S->kkt_factor(delta, rho); });
 }
 
+int64_t mb_solver_ncone(const mb_solver* s) { return (int64_t)s->S->internal().K.size(); }
+
+int mb_solver_layout(const mb_solver* s, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,
+                     double* colscale) {
+  return guard([&]() {
+    const mb::Problem& P = s->S->internal();
+    for (size_t j = 0; j < P.P2.size(); ++j) P2[j] = P.P2[j];
+    for (size_t v = 0; v < P.K.size(); ++v) {
+      xcol[v] = P.B.xcol[v];
+      kind[v] = P.K[v].kind;
+      param[v] = P.K[v].param;
+    }
+    xcol[P.K.size()] = P.B.xcol[P.K.size()];
+    const std::vector<double>& sc = s->S->col_scale();
+    for (size_t j = 0; j < sc.size(); ++j) colscale[j] = sc[j];
+    return 0;
+  });
+}
+
+int mb_cone_scale(mb_solver* s, const double* p, const double* d, double* spsd) {
+  return guard([&]() { return s->S->cone_scale(p, d, spsd) ? 1 : 0; });
+}
+
+int64_t mb_cone_blocks_size(const mb_solver* s) { return (int64_t)s->S->cone_blocks_size(); }
+
+int mb_cone_blocks(mb_solver* s, double* out) {
+  return guard([&]() {
+    s->S->cone_blocks(out);
+    return 0;
+  });
+}
+
+int mb_cone_corr0(mb_solver* s, double sigma_mu, double* r) {
+  return guard([&]() {
+    s->S->cone_corr0(sigma_mu, r);
+    return 0;
+  });
+}
+
+int mb_cone_corr(mb_solver* s, const double* dp, const double* dd, double sigma_mu, double* r) {
+  return guard([&]() {
+    s->S->cone_corr(dp, dd, sigma_mu, r);
+    return 0;
+  });
+}
+
+int mb_cone_maxsteps(mb_solver* s, const double* dp, const double* dd, double step_frac, double* step) {
+  return guard([&]() {
+    *step = s->S->cone_maxsteps(dp, dd, step_frac);
+    return 0;
+  });
+}
+
 int mb_solver_dense_factor(mb_solver* s, double* L) {
   return guard([&]() {
     s->S->dense_factor(L);

@@ -948,6 +948,51 @@ int Solver::kkt_factor(double delta, double* rho) {
   return ok ? 1 : 0;
 }
 
+// ------------------------------------------------------------------------------------------
+// cone fan-outs through the C ABI (the plugin-level boundary: scale!, initpredictor!,
+// initcorrector!, maxsteps of cone.jl:146-553), on host vectors in internal coordinates
+// ------------------------------------------------------------------------------------------
+
+bool Solver::cone_scale(const double* p, const double* d, double* spsd) {
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_P * n_, p, n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_D * n_, d, n_ * 8, hipMemcpyHostToDevice, st_));
+  launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
+  sync_scalars();
+  if (spsd) *spsd = hsc_[S_SPSD];
+  return hsc_[S_FLAGOK] != 0.0;
+}
+
+void Solver::cone_blocks(double* out) {
+  HIPCHK(hipMemcpyAsync(out, d_D_.p, d_D_.bytes, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+void Solver::cone_corr0(double smu, double* r) {
+  set_scalar(S_SMU, smu);
+  launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
+  launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
+  HIPCHK(hipMemcpyAsync(r, idev_.nv + (size_t)NV_FR * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+void Solver::cone_corr(const double* dp, const double* dd, double smu, double* r) {
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_DPA * n_, dp, n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_DDA * n_, dd, n_ * 8, hipMemcpyHostToDevice, st_));
+  set_scalar(S_SMU, smu);
+  launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
+  launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
+                  NV_DDA);
+  HIPCHK(hipMemcpyAsync(r, idev_.nv + (size_t)NV_FR * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+double Solver::cone_maxsteps(const double* dp, const double* dd, double step_frac) {
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_DP * n_, dp, n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_DD * n_, dd, n_ * 8, hipMemcpyHostToDevice, st_));
+  return maxsteps(NV_DP, NV_DD, step_frac);
+}
+
 double Solver::maxsteps(int kdp, int kdd, double step_frac) {
   launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
                       kdd, step_frac, d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);

@@ -108,6 +108,16 @@ class Solver {
   int factor_calls_fp32() const { return fac_fp32_calls_; }
   void dense_factor(double* out);  // n×n lower factor (debug; small n)
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
+  // cone fan-outs on the iterate, in the solver's internal (permuted, scaled) coordinates
+  // (cone.jl:146-553); scale replaces the iterate by (p, d)
+  bool cone_scale(const double* p, const double* d, double* spsd);
+  void cone_blocks(double* out);  // D_v per cone: n_v entries (nonneg), none (free), n_v² (others)
+  size_t cone_blocks_size() const { return d_D_.bytes / sizeof(double); }
+  const Problem& internal() const { return P_; }  // the permuted problem (before equilibration)
+  const std::vector<double>& col_scale() const { return pscl_; }
+  void cone_corr0(double smu, double* r);
+  void cone_corr(const double* dp, const double* dd, double smu, double* r);
+  double cone_maxsteps(const double* dp, const double* dd, double step_frac);
   hipStream_t stream() const { return st_; }
   // live per-kernel timing with HIP events on the solver's stream (kinds: PROF_*)
   void set_profile(int mask) {

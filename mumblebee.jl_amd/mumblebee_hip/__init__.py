@@ -150,6 +150,16 @@ EXPORTS = {
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
+    "mb_solver_ncone": (ctypes.c_int64, [ctypes.c_void_p]),
+    "mb_solver_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), _DP,
+                                         _DP]),
+    "mb_cone_scale": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP]),
+    "mb_cone_blocks_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "mb_cone_blocks": (ctypes.c_int, [ctypes.c_void_p, _DP]),
+    "mb_cone_corr0": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
+    "mb_cone_corr": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, ctypes.c_double, _DP]),
+    "mb_cone_maxsteps": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, ctypes.c_double, _DP]),
     "mb_probe_mfma_f32": (ctypes.c_int, [_DP]),
     "mb_last_error": (ctypes.c_char_p, []),
 }
@@ -428,6 +438,67 @@ class IPMSolver:
         if r < 0:
             raise _err("initkkt!")
         return bool(r), rho.value
+
+    # -- cone fan-outs (cone.jl:146-553) on vectors in the solver's internal coordinates --
+
+    def layout(self):
+        """Internal column order and cones: P2 (internal → caller column), xcol, kind, param,
+        colscale (the equilibration factors)."""
+        n, N = self.prob.n, int(lib().mb_solver_ncone(self._h))
+        P2, xcol = np.zeros(n, np.int64), np.zeros(N + 1, np.int64)
+        kind, param, cs = np.zeros(N, np.int32), np.zeros(N), np.zeros(n)
+        i = ctypes.c_int64
+        if lib().mb_solver_layout(self._h, _p(P2, i), _p(xcol, i), _p(kind, ctypes.c_int32),
+                                  _p(param, ctypes.c_double), _p(cs, ctypes.c_double)) != 0:
+            raise _err("layout")
+        return dict(P2=P2, xcol=xcol, kind=kind, param=param, colscale=cs)
+
+    def _vec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.prob.n,):
+            raise ValueError("cone fan-out vectors must have length n")
+        return x
+
+    def cone_scale(self, p, d):
+        """scale! (cone.jl:146-247): sets the iterate to (p, d); returns (all flags, Σ spsd)."""
+        p, d = self._vec(p), self._vec(d)
+        spsd = ctypes.c_double(0.0)
+        r = lib().mb_cone_scale(self._h, _p(p, ctypes.c_double), _p(d, ctypes.c_double), ctypes.byref(spsd))
+        if r < 0:
+            raise _err("scale!")
+        return bool(r), spsd.value
+
+    def cone_blocks(self):
+        """The scaling blocks in cone order: diagonal (nonneg), nothing (free), n_v² column-major."""
+        out = np.zeros(int(lib().mb_cone_blocks_size(self._h)))
+        if lib().mb_cone_blocks(self._h, _p(out, ctypes.c_double)) != 0:
+            raise _err("cone blocks")
+        return out
+
+    def cone_corr0(self, sigma_mu):
+        """initpredictor! right-hand side (cone.jl:361-444) at the scaled iterate."""
+        r = np.zeros(self.prob.n)
+        if lib().mb_cone_corr0(self._h, float(sigma_mu), _p(r, ctypes.c_double)) != 0:
+            raise _err("corr0!")
+        return r
+
+    def cone_corr(self, dp, dd, sigma_mu):
+        """initcorrector! right-hand side (cone.jl:261-352) for directions (dp, dd)."""
+        dp, dd = self._vec(dp), self._vec(dd)
+        r = np.zeros(self.prob.n)
+        if lib().mb_cone_corr(self._h, _p(dp, ctypes.c_double), _p(dd, ctypes.c_double), float(sigma_mu),
+                              _p(r, ctypes.c_double)) != 0:
+            raise _err("corr!")
+        return r
+
+    def cone_maxsteps(self, dp, dd, step_frac=1.0):
+        """maxsteps (cone.jl:455-553): the largest step keeping both iterates interior."""
+        dp, dd = self._vec(dp), self._vec(dd)
+        st = ctypes.c_double(0.0)
+        if lib().mb_cone_maxsteps(self._h, _p(dp, ctypes.c_double), _p(dd, ctypes.c_double), float(step_frac),
+                                  ctypes.byref(st)) != 0:
+            raise _err("maxsteps")
+        return st.value
 
     def dense_factor(self):
         n = self.prob.n
