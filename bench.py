@@ -349,7 +349,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="qp")
-    ap.add_argument("--precision", type=int, default=0, choices=[0], help="--serve: FP64 factors only")
+    ap.add_argument("--precision", type=int, default=0, choices=[0, 1], help="--serve: 0 FP64 factors, 1 mixed")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-nmax", type=int, default=4000)

@@ -190,7 +190,7 @@ mb_server* mb_server_create(int32_t nslots, double mem_budget_bytes, int32_t dag
 /* Finishes the queued and running requests, then frees the server. */
 void mb_server_destroy(mb_server* s);
 /* Queue solve(prob, settings); the problem must stay alive until its result is collected.
- * Returns a ticket ≥ 0, or −1. The serving loop runs FP64 factors only (settings precision 0). */
+ * Returns a ticket ≥ 0, or −1. */
 int64_t mb_server_submit(mb_server* s, const mb_problem* p, const mb_settings* st);
 /* Wait up to timeout_s (< 0: no limit) for a ticket. Returns its IPMStatus, −2 on timeout, −1 on
  * error. p, d (length n), y (length m), info[8] as in mb_solver_result; times[4] = submitted,

@@ -41,9 +41,6 @@ Server::~Server() {
 }
 
 int64_t Server::submit(const Problem* prob, const Settings& st) {
-  // a served stream with FP32 factors once stopped on a persistent kernel's bounded wait; until
-  // that is understood the serving loop runs FP64 factors only
-  if (st.precision != 0) throw std::invalid_argument("the serving loop runs FP64 factors only (precision 0)");
   auto job = std::make_shared<ServeJob>();
   job->prob = prob;
   job->set = st;

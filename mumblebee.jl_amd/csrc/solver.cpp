@@ -37,12 +37,15 @@ void DevBuf::alloc(size_t b) {
   if (b) HIPCHK(hipMalloc(&p, b));
   if (g_dev_counter) *g_dev_counter += b;
 }
+// Every copy and fill of a solver's buffers goes on the solver's own stream. The solver's stream
+// is non-blocking, so work on the null stream (hipMemcpy, hipMemset: both may return before the
+// device has finished them) is not ordered before the solver's kernels.
 template <class T>
 void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
+  HIPCHK(hipStreamSynchronize(st));  // the old buffer is no longer in use
   alloc(std::max<size_t>(v.size() * sizeof(T), sizeof(T)));
-  // synchronous: callers pass temporaries
-  HIPCHK(hipStreamSynchronize(st));
-  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // synchronous: callers pass temporaries
 }
 
 // ------------------------------------------------------------------------------------------
@@ -555,8 +558,8 @@ void Solver::build_schedule() {
     if (fr_[f].ld >= set_.big_front_ld) ++nbig;
   d_bigtick_.alloc(sizeof(int) * 2 * std::max(nbig, 1));
   d_bigstick_.alloc(sizeof(long long) * std::max(nbig, 1));
-  HIPCHK(hipMemset(d_bigtick_.p, 0, d_bigtick_.bytes));
-  HIPCHK(hipMemset(d_bigstick_.p, 0, d_bigstick_.bytes));
+  HIPCHK(hipMemsetAsync(d_bigtick_.p, 0, d_bigtick_.bytes, st_));
+  HIPCHK(hipMemsetAsync(d_bigstick_.p, 0, d_bigstick_.bytes, st_));
   {
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
@@ -595,16 +598,16 @@ void Solver::build_schedule() {
       bh->tb.upload(tb, st_);
       bh->tasks.upload(tasks, st_);
       bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
-      HIPCHK(hipMemset(bh->state.p, 0, bh->state.bytes));
+      HIPCHK(hipMemsetAsync(bh->state.p, 0, bh->state.bytes, st_));
       bh->dinv.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       if (set_.precision == 1) {  // FP32 factor columns (ld × r) and inverted diagonal blocks
         bh->f32.alloc(sizeof(float) * (size_t)Fd.ld * std::max(1, Fd.r));
         bh->dinv32.alloc(sizeof(float) * NB * NB * std::max(1, Tr));
       }
       bh->stage.alloc(sizeof(int) * std::max(1, Tr));
-      HIPCHK(hipMemset(bh->stage.p, 0, bh->stage.bytes));
+      HIPCHK(hipMemsetAsync(bh->stage.p, 0, bh->stage.bytes, st_));
       bh->sflag.alloc(sizeof(int) * std::max(1, T));
-      HIPCHK(hipMemset(bh->sflag.p, 0, bh->sflag.bytes));
+      HIPCHK(hipMemsetAsync(bh->sflag.p, 0, bh->sflag.bytes, st_));
       BigFront& B = bh->B;
       B.F = Fd.F;
       B.ld = Fd.ld;
@@ -790,7 +793,8 @@ void Solver::usolve(int kv) {
 void Solver::check_abort() {
   if (big_.empty()) return;
   std::vector<int> t(2 * big_.size());
-  HIPCHK(hipMemcpy(t.data(), d_bigtick_.p, t.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(t.data(), d_bigtick_.p, t.size() * sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
   for (size_t b = 0; b < big_.size(); ++b)
     if (t[2 * b + 1] != 0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
 }
@@ -861,7 +865,7 @@ void Solver::set_trace(bool on) {
   for (auto& bh : big_) {
     if (on && bh->trace.p == nullptr) {
       bh->trace.alloc(sizeof(unsigned long long) * 4 * std::max(1, bh->B.ntask));
-      HIPCHK(hipMemset(bh->trace.p, 0, bh->trace.bytes));
+      HIPCHK(hipMemsetAsync(bh->trace.p, 0, bh->trace.bytes, st_));
     }
     bh->B.trace = on ? bh->trace.as<unsigned long long>() : nullptr;
   }
@@ -873,7 +877,8 @@ int Solver::get_trace(int b, long long* out, int len) {
   int n = 4 * bh.B.ntask;
   if (bh.trace.p == nullptr || out == nullptr) return n;
   HIPCHK(hipStreamSynchronize(st_));
-  HIPCHK(hipMemcpy(out, bh.trace.p, sizeof(long long) * std::min(n, len), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(out, bh.trace.p, sizeof(long long) * std::min(n, len), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
   return n;
 }
 
@@ -1636,7 +1641,10 @@ void Solver::dense_factor(double* out) {
     std::vector<int> rows;
     for (int i = 0; i < Fd.r; ++i) rows.push_back(Fd.res0 + i);
     std::vector<int> sep(Fd.s);
-    if (Fd.s) HIPCHK(hipMemcpy(sep.data(), Fd.sepcols, Fd.s * 4, hipMemcpyDeviceToHost));
+    if (Fd.s) {
+      HIPCHK(hipMemcpyAsync(sep.data(), Fd.sepcols, Fd.s * 4, hipMemcpyDeviceToHost, st_));
+      HIPCHK(hipStreamSynchronize(st_));
+    }
     rows.insert(rows.end(), sep.begin(), sep.end());
     for (int j = 0; j < Fd.r; ++j)
       for (int i = j; i < Fd.ld; ++i) out[(size_t)rows[i] + (size_t)rows[j] * n_] = F[ofs + i + (size_t)j * Fd.ld];

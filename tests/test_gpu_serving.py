@@ -81,3 +81,31 @@ def test_a_failing_request_reports_its_error():
     finally:
         server.close()
     assert res.status == mh.IPMStatus.OPTIMAL
+
+
+def test_mixed_precision_requests_alongside_fp64_ones():
+    """FP32-factor requests (precision 1) and FP64 ones in one stream: each result is the same
+    as the same request solved alone, and the mixed results agree with FP64 to 1e-8."""
+    data = _mix()[:4]
+    probs = [hip_problem(d) for d in data]
+    mixed = mh.IPMSettings(precision=1)
+    solo64 = [mh.solve(p) for p in probs]
+    solo32, fp32_calls = [], 0
+    for p in probs:
+        s = mh.init(p, mixed)
+        solo32.append(s.solve())
+        fp32_calls += s.stats()["factor_calls_fp32"]
+        s.close()
+    assert fp32_calls > 0  # the mix has fronts large enough for the FP32 path
+    server = mh.IPMServer(nslots=4)
+    try:
+        tickets = [server.submit(p, s) for p in probs for s in (mixed, None)]
+        served = [server.wait(t)[0] for t in tickets]
+        st = server.stats()
+    finally:
+        server.close()
+    assert st["completed"] == 2 * len(probs)
+    for k, (r64, r32) in enumerate(zip(solo64, solo32)):
+        _close(served[2 * k], r32)
+        _close(served[2 * k + 1], r64)
+        assert rel(r32.pobj, r64.pobj) < 1e-8 and rel(r32.dobj, r64.dobj) < 1e-8
