@@ -205,6 +205,11 @@ int mb_server_stats(mb_server* s, double* out);
 double mb_clock(void);
 /* out[3] = compute units, free and total device bytes of the current device. */
 int mb_device_info(double* out);
+/* Device memory of destroyed solvers stays cached for later solvers (freeing it would wait for
+ * every kernel on the device). mb_cached_bytes: bytes cached now; mb_release_cached_memory: hand
+ * them back to the HIP runtime (waits for the device) and return the bytes released. */
+int64_t mb_cached_bytes(void);
+int64_t mb_release_cached_memory(void);
 
 const char* mb_last_error(void);
 

@@ -462,6 +462,10 @@ int mb_server_stats(mb_server* s, double* out) {
 
 double mb_clock(void) { return mb::Server::now(); }
 
+int64_t mb_cached_bytes(void) { return (int64_t)mb::cached_memory(); }
+
+int64_t mb_release_cached_memory(void) { return (int64_t)mb::release_cached_memory(); }
+
 int mb_device_info(double* out) {
   return guard([&]() {
     int dev = 0, ncu = 0;

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 
 namespace mb {
@@ -25,16 +26,109 @@ static const double FORCING_FRAC = 0.1;  // solver/solver.jl:3
 // device bytes allocated through DevBuf by the solver being built on this thread
 static thread_local size_t* g_dev_counter = nullptr;
 
+// Device and pinned-host memory cache. hipFree (and hipHostFree) waits for every kernel on the
+// device, whichever stream it runs on: a hipFree issued while a 50 ms kernel ran on another
+// stream returned when that kernel ended. A solver that freed its buffers while other solvers
+// ran would stall all of them, so freed blocks are kept in size classes and handed to later
+// allocations of the same class. Classes: powers of two up to 1 MiB, then eight per octave (at
+// most 12.5 % over the request). A block returns to the cache only when its solver's stream is
+// idle (the solver synchronizes its stream before its buffers are released). Out of memory, the
+// cache is emptied into the runtime and the allocation retried.
+namespace {
+struct MemCache {
+  std::mutex mu;
+  std::map<size_t, std::vector<void*>> dev;  // class bytes → free blocks
+  std::vector<double*> scal;                 // pinned scalar blocks (NSCALAR doubles)
+  size_t dev_cached = 0;
+};
+MemCache& memcache() {
+  static MemCache* c = new MemCache();  // outlives every solver, including static ones
+  return *c;
+}
+size_t size_class(size_t b) {
+  size_t c = 256;
+  while (c < b) c <<= 1;
+  if (c <= (size_t(1) << 20)) return c;
+  const size_t step = c / 16;  // c/2 < b ≤ c: eight classes over the octave
+  return (b + step - 1) / step * step;
+}
+void* dev_alloc(size_t b) {
+  const size_t c = size_class(b);
+  MemCache& mc = memcache();
+  {
+    std::lock_guard<std::mutex> lk(mc.mu);
+    auto it = mc.dev.find(c);
+    if (it != mc.dev.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      mc.dev_cached -= c;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    mb::release_cached_memory();
+    e = hipMalloc(&p, c);
+  }
+  if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " at hipMalloc");
+  return p;
+}
+void dev_free(void* p, size_t b) {
+  MemCache& mc = memcache();
+  std::lock_guard<std::mutex> lk(mc.mu);
+  mc.dev[size_class(b)].push_back(p);
+  mc.dev_cached += size_class(b);
+}
+double* scal_alloc() {
+  MemCache& mc = memcache();
+  {
+    std::lock_guard<std::mutex> lk(mc.mu);
+    if (!mc.scal.empty()) {
+      double* p = mc.scal.back();
+      mc.scal.pop_back();
+      return p;
+    }
+  }
+  double* p = nullptr;
+  HIPCHK(hipHostMalloc((void**)&p, NSCALAR * sizeof(double), hipHostMallocDefault));
+  return p;
+}
+void scal_free(double* p) {
+  MemCache& mc = memcache();
+  std::lock_guard<std::mutex> lk(mc.mu);
+  mc.scal.push_back(p);
+}
+}  // namespace
+
+size_t release_cached_memory() {
+  MemCache& mc = memcache();
+  std::lock_guard<std::mutex> lk(mc.mu);
+  size_t n = mc.dev_cached;
+  for (auto& kv : mc.dev)
+    for (void* p : kv.second) (void)hipFree(p);
+  mc.dev.clear();
+  mc.dev_cached = 0;
+  return n;
+}
+
+size_t cached_memory() {
+  MemCache& mc = memcache();
+  std::lock_guard<std::mutex> lk(mc.mu);
+  return mc.dev_cached;
+}
+
 DevBuf::~DevBuf() {
-  if (p) (void)hipFree(p);
+  if (p) dev_free(p, bytes);
 }
 void DevBuf::alloc(size_t b) {
   if (p) {
-    HIPCHK(hipFree(p));
+    dev_free(p, bytes);
     p = nullptr;
   }
   bytes = b;
-  if (b) HIPCHK(hipMalloc(&p, b));
+  if (b) p = dev_alloc(b);
   if (g_dev_counter) *g_dev_counter += b;
 }
 // Every copy and fill of a solver's buffers goes on the solver's own stream. The solver's stream
@@ -237,7 +331,7 @@ void Solver::device_setup(const double* p0, const double* d0, const double* y0) 
   g_dev_counter = &dev_bytes_;
   try {
     HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    HIPCHK(hipHostMalloc((void**)&hsc_, NSCALAR * sizeof(double), hipHostMallocDefault));
+    hsc_ = scal_alloc();
     HIPCHK(hipEventCreate(&ev0_));
     HIPCHK(hipEventCreate(&ev1_));
     build_device();
@@ -252,7 +346,7 @@ void Solver::device_setup(const double* p0, const double* d0, const double* y0) 
 
 Solver::~Solver() {
   if (st_) (void)hipStreamSynchronize(st_);
-  if (hsc_) (void)hipHostFree(hsc_);
+  if (hsc_) scal_free(hsc_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);

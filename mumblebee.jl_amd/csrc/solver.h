@@ -55,6 +55,11 @@ enum ProfKind {
 };
 constexpr int PROF_TRACE_BIT = 1 << 16;  // profile mask bit: record the large-front task timeline
 
+// Freed device buffers stay cached for later allocations (hipFree waits for the whole device);
+// release_cached_memory hands them back to the runtime and returns the bytes released.
+size_t release_cached_memory();
+size_t cached_memory();
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

@@ -147,6 +147,8 @@ EXPORTS = {
     "mb_server_stats": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_clock": (ctypes.c_double, []),
     "mb_device_info": (ctypes.c_int, [_DP]),
+    "mb_cached_bytes": (ctypes.c_int64, []),
+    "mb_release_cached_memory": (ctypes.c_int64, []),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
@@ -606,6 +608,16 @@ def device_info():
     if lib().mb_device_info(_p(out, ctypes.c_double)) != 0:
         raise _err("device_info")
     return dict(cus=int(out[0]), free_bytes=out[1], total_bytes=out[2])
+
+
+def cached_bytes():
+    """Device bytes held in the library's cache of freed solver buffers."""
+    return int(lib().mb_cached_bytes())
+
+
+def release_cached_memory():
+    """Return the cached device memory to the HIP runtime; returns the bytes released."""
+    return int(lib().mb_release_cached_memory())
 
 
 def probe_mfma_f64():
