@@ -299,9 +299,12 @@ __device__ __forceinline__ float rsqrt_nr(float d) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Cholesky factor and inverse of a 64×64 diagonal tile, recursively: 16×16 diagonal blocks in
-// wave-0 registers, combined as [A11 ·; A21 A22] → L11, X11 = L11⁻¹; L21 = A21 X11ᵀ;
-// A22 −= L21 L21ᵀ; L22, X22; X21 = −X22 (L21 X11), the products on MFMA 16×16 tiles.
+// Cholesky factor and inverse of a 64×64 diagonal tile. The factor is right-looking over four
+// 16-column panels: wave 0 factors a panel with one row per lane (all 64 rows at once), the
+// other waves wait, then all four waves subtract the panel's outer product from the trailing
+// tiles on MFMA. The inverse follows from the factor: wave b inverts diagonal block b (one column
+// per lane), then forms X_{b+s,b} = −X_{b+s,b+s} Σ_k L_{b+s,k} X_{k,b} for s = 1, 2, 3 in its
+// own block column, the products on MFMA 16×16 tiles.
 // ------------------------------------------------------------------------------------------
 
 // 16×16 MFMA tile: acc ± Σ_{k<K} A[ra+i][ca+k] · op(B)[k][j] from LDS (stride LDT), with
@@ -335,100 +338,63 @@ __device__ __forceinline__ void tile16_store(T* C, int rc, int cc, typename Mfma
   for (int t = 0; t < 4; ++t) C[(rc + Mfma<T>::row(lane, t)) + (cc + (lane & 15)) * LDT] = v[t];
 }
 
-// 16×16 diagonal block at (o, o) of S, by wave 0: lane i (i < 16) holds row o+i in registers
-// (lanes 16..63 repeat rows 0..15 and write nothing). Step j takes the pivot from lane j,
-// scales column j by 1/√d and updates the trailing columns with entries broadcast by
-// v_readlane. Then lane c forms column c of L⁻¹ right-looking (x_i /= L_ii, x_m −= L_mi x_i),
-// with L's entries again broadcast from the row registers. L → S and L⁻¹ → X (entries above
-// the diagonal are left for the caller to clear). Returns 1 on a non-positive pivot (taken as 1).
+// LDS writes of one wave become visible to the wave's later LDS reads in other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Panel c0..c0+15 of S, by wave 0: lane r holds row r (columns c0..c0+15) in registers. Step j
+// takes the pivot from lane c0+j, scales column j by 1/√d, and updates the trailing columns:
+// the next column's entry comes by v_readlane (it feeds the next pivot), the others through a
+// broadcast read of the scaled column from LDS. Rows ≥ c0 are written back; rv[c0+j] ← 1/L_jj.
+// Returns 1 on a non-positive pivot (taken as 1).
 template <class T>
-__device__ __forceinline__ int chol_inv16(T* S, T* X, int o) {
-  const int lane = threadIdx.x, i = lane & 15;
-  T row[16], rinv[16];
+__device__ __forceinline__ int panel16(T* S, T* rv, T* colj, int c0) {
+  const int lane = threadIdx.x & 63;
+  T row[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) row[k] = S[(o + i) + (o + k) * LDT];
+  for (int k = 0; k < 16; ++k) row[k] = S[lane + (c0 + k) * LDT];
   int bad = 0;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    T d = readlane(row[j], j);
+    T d = readlane(row[j], c0 + j);
     if (!(d > T(0)) || !isfinite(d)) {
       bad = 1;  // the factor is discarded: the caller retries with a diagonal shift
       d = T(1);
     }
     const T r = rsqrt_nr(d);
-    rinv[j] = r;
+    if (lane == 0) rv[c0 + j] = r;
     const T lij = row[j] * r;
     row[j] = lij;
+    if (j + 1 < 16) {
+      T* cb = colj + (j & 1) * 64;  // two buffers: step j+1 writes while step j's reads drain
+      cb[lane] = lij;
+      row[j + 1] -= lij * readlane(lij, c0 + j + 1);
+      wave_lds_sync();
 #pragma unroll
-    for (int k = j + 1; k < 16; ++k) row[k] -= lij * readlane(lij, k);
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  T x[16];
-#pragma unroll
-  for (int m = 0; m < 16; ++m) x[m] = (m == i) ? T(1) : T(0);
-#pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    x[c] *= rinv[c];
-#pragma unroll
-    for (int m = c + 1; m < 16; ++m) x[m] -= readlane(row[c], m) * x[c];
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  if (lane < 16) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      S[(o + i) + (o + k) * LDT] = row[k];
-      X[(o + k) + (o + i) * LDT] = x[k];
+      for (int k = j + 2; k < 16; ++k) row[k] -= lij * cb[c0 + k];
     }
   }
-  return bad;
-}
-
-// With L11, X11 final for the b×b block at (o, o): L21 = A21 X11ᵀ into S, then
-// A22 −= L21 L21ᵀ. One 16×16 output tile per wave.
-template <int b, class T>
-__device__ __forceinline__ void comb_lower(T* S, const T* X, int o) {
-  constexpr int nt = (b / 16) * (b / 16);
-  const int w = threadIdx.x >> 6, ti = 16 * (w % (b / 16)), tj = 16 * (w / (b / 16));
-  typename Mfma<T>::v4 acc = {0, 0, 0, 0};
-  if (w < nt) acc = mfma16<b, true, false>(acc, S, o + b + ti, o, X, o + tj, o);
-  __syncthreads();
-  if (w < nt) tile16_store(S, o + b + ti, o + tj, acc);
-  __syncthreads();
-  if (w < nt) {
-    acc = tile16_load(S, o + b + ti, o + b + tj);
-    acc = mfma16<b, true, true>(acc, S, o + b + ti, o, S, o + b + tj, o);
-    tile16_store(S, o + b + ti, o + b + tj, acc);
+  if (lane >= c0) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) S[lane + (c0 + k) * LDT] = row[k];
   }
-  __syncthreads();
-}
-
-// With X11, L21 and X22 final: X21 = −X22 (L21 X11), through X21's place as the temporary.
-template <int b, class T>
-__device__ __forceinline__ void comb_inverse(const T* S, T* X, int o) {
-  constexpr int nt = (b / 16) * (b / 16);
-  const int w = threadIdx.x >> 6, ti = 16 * (w % (b / 16)), tj = 16 * (w / (b / 16));
-  typename Mfma<T>::v4 acc = {0, 0, 0, 0};
-  if (w < nt) acc = mfma16<b, false, false>(acc, S, o + b + ti, o, X, o, o + tj);
-  __syncthreads();
-  if (w < nt) tile16_store(X, o + b + ti, o + tj, acc);
-  __syncthreads();
-  acc = typename Mfma<T>::v4{0, 0, 0, 0};
-  if (w < nt) acc = mfma16<b, false, true>(acc, X, o + b + ti, o + b, X, o + b, o + tj);
-  __syncthreads();
-  if (w < nt) tile16_store(X, o + b + ti, o + tj, acc);
-  __syncthreads();
-}
-
-template <class T>
-__device__ __forceinline__ int chol_inv32(T* S, T* X, int o) {
-  int bad = 0;
-  if (threadIdx.x < 64) bad = chol_inv16(S, X, o);
-  __syncthreads();
-  comb_lower<16>(S, X, o);
-  if (threadIdx.x < 64) bad |= chol_inv16(S, X, o + 16);
-  __syncthreads();
-  comb_inverse<16>(S, X, o);
   return bad;
+}
+
+// X_{b+SL,b} = −X_{b+SL,b+SL} Σ_{k} L_{b+SL,k} X_{k,b} (block indices, k = b..b+SL−1), by one
+// wave; the partial sum goes through X_{b+SL,b}'s place in LDS
+template <int SL, class T>
+__device__ __forceinline__ void inv_level(const T* S, T* X, int b) {
+  typedef typename Mfma<T>::v4 v4;
+  const v4 t = mfma16<16 * SL, false, false>(v4{0, 0, 0, 0}, S, 16 * (b + SL), 16 * b, X, 16 * b, 16 * b);
+  tile16_store(X, 16 * (b + SL), 16 * b, t);
+  wave_lds_sync();
+  const v4 a = mfma16<16, false, true>(v4{0, 0, 0, 0}, X, 16 * (b + SL), 16 * (b + SL), X, 16 * (b + SL), 16 * b);
+  wave_lds_sync();
+  tile16_store(X, 16 * (b + SL), 16 * b, a);
 }
 
 // Cholesky factor L (in S: column-major, zero upper part, identity padded past h) and its
@@ -437,14 +403,57 @@ __device__ __forceinline__ int chol_inv32(T* S, T* X, int o) {
 template <class T>
 __device__ __forceinline__ bool chol_inv64(T* S, T* X, int h) {
   __shared__ int okflag;
+  __shared__ T rv[NB];
+  __shared__ T colj[2 * 64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x >= h && threadIdx.x < NB) S[threadIdx.x * (LDT + 1)] = T(1);
-  for (int t = threadIdx.x; t < NB * NB; t += NT) X[(t % NB) + (t / NB) * LDT] = T(0);  // the products read X's upper blocks
   __syncthreads();
-  int bad = chol_inv32(S, X, 0);
-  comb_lower<32>(S, X, 0);
-  bad |= chol_inv32(S, X, 32);
-  comb_inverse<32>(S, X, 0);
+  int bad = 0;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int c0 = 16 * p;
+    if (w == 0) bad |= panel16(S, rv, colj, c0);
+    __syncthreads();
+    if (p < 3) {
+      // the lower 16×16 tiles (ti ≥ tk) of the trailing blocks p+1..3, spread over the waves
+      const int nb = 3 - p, ntile = nb * (nb + 1) / 2;
+      for (int t = w; t < ntile; t += NT / 64) {
+        int ti = 0, tk = t;
+        while (tk > ti) {
+          tk -= ti + 1;
+          ++ti;
+        }
+        const int ri = c0 + 16 + 16 * ti, rk = c0 + 16 + 16 * tk;
+        typename Mfma<T>::v4 acc = tile16_load(S, ri, rk);
+        acc = mfma16<16, true, true>(acc, S, ri, c0, S, rk, c0);
+        tile16_store(S, ri, rk, acc);
+      }
+      __syncthreads();
+    }
+  }
+  {
+    // wave w inverts diagonal block w: lane c < 16 forms column c by forward substitution
+    const int o = 16 * w, c = lane & 15;
+    T x[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) x[m] = (m == c) ? T(1) : T(0);
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+      x[cc] *= rv[o + cc];
+#pragma unroll
+      for (int m = cc + 1; m < 16; ++m) x[m] -= S[(o + m) + (o + cc) * LDT] * x[cc];
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) X[(o + m) + (o + c) * LDT] = x[m];
+    }
+  }
+  __syncthreads();
+  if (w <= 2) inv_level<1>(S, X, w);
+  if (w <= 1) inv_level<2>(S, X, w);
+  if (w == 0) inv_level<3>(S, X, w);
   if (threadIdx.x == 0) okflag = !bad;
+  __syncthreads();
   for (int t = threadIdx.x; t < NB * NB; t += NT) {
     const int r = t % NB, c = t / NB;
     if (r < c) {
