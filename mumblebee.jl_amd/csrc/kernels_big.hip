@@ -347,41 +347,45 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Panel c0..c0+15 of S, by wave 0: lane r holds row r (columns c0..c0+15) in registers. Step j
 // takes the pivot from lane c0+j, scales column j by 1/√d, and updates the trailing columns:
-// the next column's entry comes by v_readlane (it feeds the next pivot), the others through a
-// broadcast read of the scaled column from LDS. Rows ≥ c0 are written back; rv[c0+j] ← 1/L_jj.
-// Returns 1 on a non-positive pivot (taken as 1).
+// the next two columns' entries come by v_readlane (they feed the next pivots), the others
+// through a broadcast read of the scaled column from LDS. Rows ≥ c0 are written back, and
+// rv[c0+j] ← 1/L_jj.
+// Returns 1 on a non-positive or non-finite pivot: the factor then holds NaNs and is discarded
+// by the caller, which retries with a diagonal shift (the check stays off the step's chain).
 template <class T>
 __device__ __forceinline__ int panel16(T* S, T* rv, T* colj, int c0) {
   const int lane = threadIdx.x & 63;
-  T row[16];
+  T row[16], rr[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) row[k] = S[lane + (c0 + k) * LDT];
-  int bad = 0;
+  bool bad = false;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    T d = readlane(row[j], c0 + j);
-    if (!(d > T(0)) || !isfinite(d)) {
-      bad = 1;  // the factor is discarded: the caller retries with a diagonal shift
-      d = T(1);
-    }
+    const T d = readlane(row[j], c0 + j);
+    bad |= !(d > T(0)) || !isfinite(d);
     const T r = rsqrt_nr(d);
-    if (lane == 0) rv[c0 + j] = r;
+    rr[j] = r;
     const T lij = row[j] * r;
     row[j] = lij;
     if (j + 1 < 16) {
       T* cb = colj + (j & 1) * 64;  // two buffers: step j+1 writes while step j's reads drain
       cb[lane] = lij;
       row[j + 1] -= lij * readlane(lij, c0 + j + 1);
+      if (j + 2 < 16) row[j + 2] -= lij * readlane(lij, c0 + j + 2);  // slack for the LDS reads
       wave_lds_sync();
 #pragma unroll
-      for (int k = j + 2; k < 16; ++k) row[k] -= lij * cb[c0 + k];
+      for (int k = j + 3; k < 16; ++k) row[k] -= lij * cb[c0 + k];
     }
   }
+  T mine = rr[0];  // lane j < 16 stores 1/L_jj
+#pragma unroll
+  for (int j = 1; j < 16; ++j) mine = lane == j ? rr[j] : mine;
+  if (lane < 16) rv[c0 + lane] = mine;
   if (lane >= c0) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) S[lane + (c0 + k) * LDT] = row[k];
   }
-  return bad;
+  return bad ? 1 : 0;
 }
 
 // X_{b+SL,b} = −X_{b+SL,b+SL} Σ_{k} L_{b+SL,k} X_{k,b} (block indices, k = b..b+SL−1), by one
@@ -398,13 +402,14 @@ __device__ __forceinline__ void inv_level(const T* S, T* X, int b) {
 }
 
 // Cholesky factor L (in S: column-major, zero upper part, identity padded past h) and its
-// inverse X of the diagonal tile of order h held in S (zero padded past h). Returns false on a
-// non-positive pivot.
+// inverse X of the diagonal tile of order h held in S (zero padded past h). W: 192 elements of
+// LDS scratch, best below 64 KiB of LDS so that the panel's broadcast reads take immediate
+// offsets. Returns false on a non-positive pivot.
 template <class T>
-__device__ __forceinline__ bool chol_inv64(T* S, T* X, int h) {
+__device__ __forceinline__ bool chol_inv64(T* S, T* X, T* W, int h) {
   __shared__ int okflag;
-  __shared__ T rv[NB];
-  __shared__ T colj[2 * 64];
+  T* rv = W;
+  T* colj = W + NB;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x >= h && threadIdx.x < NB) S[threadIdx.x * (LDT + 1)] = T(1);
   __syncthreads();
@@ -577,7 +582,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
       // factor and invert; L_jj⁻¹ → Dinv and publish, then L_jj → F (read only by later launches)
       acc_to_lds(C, SA);
       __syncthreads();
-      bool ok = chol_inv64(SA, SC, hj);
+      bool ok = chol_inv64(SA, SC, SB, hj);  // SB is free here and lies below 64 KiB
       if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
       store_tile(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
       publish(&B.state[tri_index(j, j)], epoch);
