@@ -13,6 +13,7 @@
  *   mb_solver_result         result(solver, status) → IPMResult           src/IPM/src/ipm/solver/ipm.jl:30-74
  *   mb_solver_history        IPMHistory rows                              src/IPM/src/ipm/history/ipm.jl:1-24
  *   mb_solver_kkt_factor     initkkt!(kkt, H; δ) → (ok, ρ)                src/IPM/src/kkt/uzawa.jl:73-115
+ *   mb_solver_kkt_solve      solvekkt!(fstop, gstop, kkt, x, y, H, B, f, g; …)  src/IPM/src/kkt/uzawa.jl:265-344
  *   mb_cone_scale            scale!(sched, K, H, Q, caches, p, d, …)      src/IPM/src/cone/cone.jl:146-247
  *   mb_cone_corr0            initpredictor! / corr0!                      src/IPM/src/cone/cone.jl:361-444
  *   mb_cone_corr             initcorrector! / corr!                       src/IPM/src/cone/cone.jl:261-352
@@ -155,12 +156,21 @@ int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);
  * mb_cone_corr: r = the Mehrotra corrector for directions (dp, dd). mb_cone_maxsteps: step ←
  * the largest τ ≤ 1 keeping both iterates interior, × step_frac except on free cones. */
 int64_t mb_solver_ncone(const mb_solver* s);
-/* The internal layout: P2[j] = the caller's column of internal column j (n entries); cone v owns
- * internal columns xcol[v]..xcol[v+1]−1 and has kind[v], param[v] (ncone entries, xcol ncone+1);
- * colscale[j] = the equilibration factor (internal p = p_caller[P2[j]] / colscale[j], internal d =
- * d_caller[P2[j]] · colscale[j]). */
-int mb_solver_layout(const mb_solver* s, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,
-                     double* colscale);
+/* The internal layout: P1[i] = the caller's row of internal row i and rowscale[i] its
+ * equilibration factor (m entries; internal y = y_caller[P1[i]] / rowscale[i]); P2[j] = the
+ * caller's column of internal column j and colscale[j] its factor (n entries; internal p =
+ * p_caller[P2[j]] / colscale[j], internal d = d_caller[P2[j]] · colscale[j]); cone v owns internal
+ * columns xcol[v]..xcol[v+1]−1 and has kind[v], param[v] (ncone entries, xcol ncone+1). */
+int mb_solver_layout(const mb_solver* s, int64_t* P1, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,
+                     double* rowscale, double* colscale);
+/* solvekkt! on the solver's current H (from the last scale or step) and factor (from the last
+ * mb_solver_kkt_factor or step): H x − Bᵀ y = f, B x = g by Uzawa refinement with the factor of
+ * δH + BᵀB, then CG on the dual (uzawa.jl:319-507). x (n), y (m): in/out, read when warm ≠ 0;
+ * f (n), g (m). ftol, gtol: absolute residual tolerances; irmin: minimum refinement passes;
+ * use_stop ≠ 0 adds the scaled feasibility stops of the IPM's closures (solver/ipm.jl:301-307).
+ * counts[3] = ncg, nir, KKTStatus; window[2] = δmin, δmax (kktwindow, uzawa.jl:199-227). */
+int mb_solver_kkt_solve(mb_solver* s, double* x, double* y, const double* f, const double* g, int warm, double ftol,
+                        double gtol, int irmin, int use_stop, int32_t* counts, double* window);
 int mb_cone_scale(mb_solver* s, const double* p, const double* d, double* spsd);
 int64_t mb_cone_blocks_size(const mb_solver* s);
 int mb_cone_blocks(mb_solver* s, double* out);

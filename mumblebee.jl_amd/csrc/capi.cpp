@@ -319,10 +319,13 @@ int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho) {
 
 int64_t mb_solver_ncone(const mb_solver* s) { return (int64_t)s->S->internal().K.size(); }
 
-int mb_solver_layout(const mb_solver* s, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,
-                     double* colscale) {
+int mb_solver_layout(const mb_solver* s, int64_t* P1, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,
+                     double* rowscale, double* colscale) {
   return guard([&]() {
     const mb::Problem& P = s->S->internal();
+    for (size_t i = 0; i < P.P1.size(); ++i) P1[i] = P.P1[i];
+    const std::vector<double>& rs = s->S->row_scale();
+    for (size_t i = 0; i < rs.size(); ++i) rowscale[i] = rs[i];
     for (size_t j = 0; j < P.P2.size(); ++j) P2[j] = P.P2[j];
     for (size_t v = 0; v < P.K.size(); ++v) {
       xcol[v] = P.B.xcol[v];
@@ -332,6 +335,19 @@ int mb_solver_layout(const mb_solver* s, int64_t* P2, int64_t* xcol, int32_t* ki
     xcol[P.K.size()] = P.B.xcol[P.K.size()];
     const std::vector<double>& sc = s->S->col_scale();
     for (size_t j = 0; j < sc.size(); ++j) colscale[j] = sc[j];
+    return 0;
+  });
+}
+
+int mb_solver_kkt_solve(mb_solver* s, double* x, double* y, const double* f, const double* g, int warm, double ftol,
+                        double gtol, int irmin, int use_stop, int32_t* counts, double* window) {
+  return guard([&]() {
+    mb::Solver::KKTOut o = s->S->kkt_solve(x, y, f, g, warm != 0, ftol, gtol, irmin, use_stop != 0);
+    counts[0] = o.ncg;
+    counts[1] = o.nir;
+    counts[2] = o.status;
+    window[0] = o.dmin;
+    window[1] = o.dmax;
     return 0;
   });
 }

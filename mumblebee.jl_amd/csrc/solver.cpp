@@ -1042,6 +1042,7 @@ bool Solver::factor(double delta, double& rho) {
 
 int Solver::kkt_factor(double delta, double* rho) {
   double r = 0.0;
+  set_scalar(S_ALPHA, 1.0 / delta);  // the Uzawa step α = 1/δ of a following kkt_solve
   bool ok = factor(delta, r);
   if (rho) *rho = r;
   return ok ? 1 : 0;
@@ -1084,6 +1085,21 @@ void Solver::cone_corr(const double* dp, const double* dd, double smu, double* r
                   NV_DDA);
   HIPCHK(hipMemcpyAsync(r, idev_.nv + (size_t)NV_FR * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
+}
+
+Solver::KKTOut Solver::kkt_solve(double* x, double* y, const double* f, const double* g, bool warm, double ftol,
+                                 double gtol, int irmin, bool stops) {
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_FR * n_, f, n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_DG * m_, g, m_ * 8, hipMemcpyHostToDevice, st_));
+  if (warm) {
+    HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_DP * n_, x, n_ * 8, hipMemcpyHostToDevice, st_));
+    HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_DY * m_, y, m_ * 8, hipMemcpyHostToDevice, st_));
+  }
+  KKTOut o = solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, warm, ftol, gtol, irmin, stops);
+  HIPCHK(hipMemcpyAsync(x, idev_.nv + (size_t)NV_DP * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(y, idev_.mv + (size_t)MV_DY * m_, m_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return o;
 }
 
 double Solver::cone_maxsteps(const double* dp, const double* dd, double step_frac) {

@@ -120,6 +120,7 @@ class Solver {
   size_t cone_blocks_size() const { return d_D_.bytes / sizeof(double); }
   const Problem& internal() const { return P_; }  // the permuted problem (before equilibration)
   const std::vector<double>& col_scale() const { return pscl_; }
+  const std::vector<double>& row_scale() const { return yscl_; }
   void cone_corr0(double smu, double* r);
   void cone_corr(const double* dp, const double* dd, double smu, double* r);
   double cone_maxsteps(const double* dp, const double* dd, double step_frac);
@@ -159,11 +160,21 @@ class Solver {
   double maxsteps(int kdp, int kdd, double step_frac);
   void reinit(const double* p0, const double* d0, const double* y0);
   // KKT
+ public:
   struct KKTOut {
     int ncg, nir, status;
     double dmin, dmax;
   };
+
+ private:
   KKTOut solvekkt(int kx, int ky, int kf, int kg, bool warm, double ftol, double gtol, int irmin, bool stops);
+
+ public:
+  // solvekkt! (uzawa.jl:265-344) on the current H and factor, host vectors in internal coordinates
+  KKTOut kkt_solve(double* x, double* y, const double* f, const double* g, bool warm, double ftol, double gtol,
+                   int irmin, bool stops);
+
+ private:
   int cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& status);
   double getaug();
   bool isstalled() const;

@@ -154,8 +154,11 @@ EXPORTS = {
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
     "mb_solver_ncone": (ctypes.c_int64, [ctypes.c_void_p]),
     "mb_solver_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
-                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), _DP,
-                                         _DP]),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int32), _DP, _DP, _DP]),
+    "mb_solver_kkt_solve": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP, _DP, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int32), _DP]),
     "mb_cone_scale": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP]),
     "mb_cone_blocks_size": (ctypes.c_int64, [ctypes.c_void_p]),
     "mb_cone_blocks": (ctypes.c_int, [ctypes.c_void_p, _DP]),
@@ -444,16 +447,35 @@ class IPMSolver:
     # -- cone fan-outs (cone.jl:146-553) on vectors in the solver's internal coordinates --
 
     def layout(self):
-        """Internal column order and cones: P2 (internal → caller column), xcol, kind, param,
-        colscale (the equilibration factors)."""
-        n, N = self.prob.n, int(lib().mb_solver_ncone(self._h))
-        P2, xcol = np.zeros(n, np.int64), np.zeros(N + 1, np.int64)
-        kind, param, cs = np.zeros(N, np.int32), np.zeros(N), np.zeros(n)
+        """Internal row and column order and cones: P1 (internal → caller row), P2 (internal →
+        caller column), xcol, kind, param, rowscale and colscale (the equilibration factors)."""
+        n, m, N = self.prob.n, self.prob.m, int(lib().mb_solver_ncone(self._h))
+        P1, P2, xcol = np.zeros(m, np.int64), np.zeros(n, np.int64), np.zeros(N + 1, np.int64)
+        kind, param, rs, cs = np.zeros(N, np.int32), np.zeros(N), np.zeros(m), np.zeros(n)
         i = ctypes.c_int64
-        if lib().mb_solver_layout(self._h, _p(P2, i), _p(xcol, i), _p(kind, ctypes.c_int32),
-                                  _p(param, ctypes.c_double), _p(cs, ctypes.c_double)) != 0:
+        if lib().mb_solver_layout(self._h, _p(P1, i), _p(P2, i), _p(xcol, i), _p(kind, ctypes.c_int32),
+                                  _p(param, ctypes.c_double), _p(rs, ctypes.c_double),
+                                  _p(cs, ctypes.c_double)) != 0:
             raise _err("layout")
-        return dict(P2=P2, xcol=xcol, kind=kind, param=param, colscale=cs)
+        return dict(P1=P1, P2=P2, xcol=xcol, kind=kind, param=param, rowscale=rs, colscale=cs)
+
+    def kkt_solve(self, f, g, x0=None, y0=None, ftol=1e-10, gtol=1e-10, irmin=0, use_stop=False):
+        """solvekkt! (uzawa.jl:265-344) on the current H and factor: H x − Bᵀ y = f, B x = g in
+        internal coordinates. Returns (x, y, ncg, nir, KKTStatus, δmin, δmax)."""
+        f = self._vec(f)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        if g.shape != (self.prob.m,):
+            raise ValueError("kkt_solve: g must have length m")
+        warm = x0 is not None or y0 is not None
+        x = np.array(x0, dtype=np.float64) if x0 is not None else np.zeros(self.prob.n)
+        y = np.array(y0, dtype=np.float64) if y0 is not None else np.zeros(self.prob.m)
+        counts, window = np.zeros(3, np.int32), np.zeros(2)
+        if lib().mb_solver_kkt_solve(self._h, _p(x, ctypes.c_double), _p(y, ctypes.c_double),
+                                     _p(f, ctypes.c_double), _p(g, ctypes.c_double), int(warm), float(ftol),
+                                     float(gtol), int(irmin), int(bool(use_stop)), _p(counts, ctypes.c_int32),
+                                     _p(window, ctypes.c_double)) != 0:
+            raise _err("solvekkt!")
+        return x, y, int(counts[0]), int(counts[1]), KKTStatus(int(counts[2])), window[0], window[1]
 
     def _vec(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -231,3 +231,61 @@ def test_fanout_vectors_are_checked():
             f.s.cone_scale(f.p[:-1], f.d)
     finally:
         f.close()
+
+
+def test_kkt_solve_matches_a_dense_saddle_point_solve():
+    """initkkt! + solvekkt! through the C ABI (uzawa.jl:73-115, 265-344) on the H that scale!
+    formed at an interior point: the solution of H x − Bᵀ y = f, B x = g matches a dense solve of
+    the same saddle-point system (equilibration off, so the internal coordinates are the caller's
+    permuted by P1 and P2)."""
+    _print_case(test="kkt solve")
+    K = [mh.PositiveCone(), mh.CofreeCone(), mh.SecondOrderCone(), mh.SemidefiniteCone(), mh.ExponentialCone(),
+         mh.PowerCone(0.4)]
+    sizes = [6, 3, 5, 10, 3, 3]
+    n, m = sum(sizes), 7
+    rng = np.random.default_rng(77)
+    A = rng.normal(size=(n, 3))
+    Q = sp.csc_matrix(0.1 * A @ A.T)
+    B = sp.csc_matrix(rng.normal(size=(m, n)))
+    prob = mh.IPMProblem(Q, B, rng.normal(size=n), rng.normal(size=m), 0.0, K, sizes)
+    s = mh.init(prob, mh.IPMSettings(scale_max_iter=0))
+    try:
+        lay = s.layout()
+        assert np.all(lay["rowscale"] == 1.0) and np.all(lay["colscale"] == 1.0)
+        P1, P2, xcol = lay["P1"], lay["P2"], lay["xcol"]
+        cones = [_cone_of(k, a) for k, a in zip(lay["kind"], lay["param"])]
+        p, d = np.zeros(n), np.zeros(n)
+        for v, cone in enumerate(cones):
+            a, b = xcol[v], xcol[v + 1]
+            p[a:b] = _primal_point(rng, cone, b - a)
+            d[a:b] = _dual_point(rng, cone, b - a)
+        ok, _ = s.cone_scale(p, d)
+        assert ok
+        D = s.cone_blocks()
+        H = Q.toarray()[np.ix_(P2, P2)]
+        o = 0
+        for v, cone in enumerate(cones):
+            a, b = xcol[v], xcol[v + 1]
+            nv = b - a
+            if cone[0] == "pos":
+                H[np.arange(a, b), np.arange(a, b)] += D[o:o + nv]
+                o += nv
+            elif cone[0] != "free":
+                H[a:b, a:b] += D[o:o + nv * nv].reshape(nv, nv, order="F")
+                o += nv * nv
+        Bi = B.toarray()[np.ix_(P1, P2)]
+        okf, rho = s.kkt_factor(1.0)
+        assert okf and rho == 0.0
+        f, g = rng.normal(size=n), rng.normal(size=m)
+        x, y, ncg, nir, status, dmin, dmax = s.kkt_solve(f, g, ftol=1e-12, gtol=1e-12)
+        assert status in (mh.KKTStatus.KKT_SOLVED, mh.KKTStatus.KKT_STAGNATED) and nir >= 1
+        Kd = np.block([[H, -Bi.T], [Bi, np.zeros((m, m))]])
+        ref = np.linalg.solve(Kd, np.concatenate([f, g]))
+        assert np.linalg.norm(x - ref[:n]) <= 1e-8 * (1 + np.linalg.norm(ref[:n]))
+        assert np.linalg.norm(y - ref[n:]) <= 1e-8 * (1 + np.linalg.norm(ref[n:]))
+        # a warm start from the solution stops at once
+        x2, y2, _, nir2, status2, _, _ = s.kkt_solve(f, g, x0=x, y0=y, ftol=1e-9, gtol=1e-9)
+        assert status2 == mh.KKTStatus.KKT_SOLVED and nir2 == 0
+        assert np.linalg.norm(x2 - x) <= 1e-9 * (1 + np.linalg.norm(x))
+    finally:
+        s.close()
