@@ -677,18 +677,27 @@ void Solver::build_schedule() {
       for (int c = Fd.r; c < Fd.ld; c += NB) tb.push_back(c);
       tb.push_back(Fd.ld);
       int T = (int)tb.size() - 1, Tr = (Fd.r + NB - 1) / NB;
-      // one task per lower tile, column by column, diagonal first (the ticket order is a
-      // topological order of the left-looking dependencies). A diagonal task forms the
-      // sub-diagonal factor tile of the previous column itself, so its own update loop stops
-      // one column early and that tile's task only accumulates and stages it.
+      // one task per lower tile. A diagonal task forms the sub-diagonal factor tile of the
+      // previous column itself, so its own update loop stops one column early and that tile's
+      // task only accumulates and stages it. The ticket order is a topological order of the
+      // left-looking dependencies with the diagonal chain one column ahead of the other factor
+      // tiles: diag j, then the factor tiles of column j−1 (rows ≥ j+1), then the staged tile
+      // (j+1, j). Diag j needs only tickets before it (Dinv_{j−1}, the staged tile (j, j−1),
+      // factor tiles of columns ≤ j−2); so does every other task. A grid of few workgroups then
+      // still claims each diagonal task early enough to finish its updates before the chain
+      // reaches it.
       std::vector<int4> tasks;
-      for (int j = 0; j < T; ++j)
-        for (int i = j; i < T; ++i) {
-          if (j >= Tr) tasks.push_back(int4{2, i, j, Tr});
-          else if (i == j) tasks.push_back(int4{0, i, j, std::max(j - 1, 0)});
-          else if (i == j + 1 && i < Tr) tasks.push_back(int4{3, i, j, j});
-          else tasks.push_back(int4{1, i, j, j});
-        }
+      for (int j = 0; j < Tr; ++j) {
+        tasks.push_back(int4{0, j, j, std::max(j - 1, 0)});
+        if (j > 0)
+          for (int i = j + 1; i < T; ++i) tasks.push_back(int4{1, i, j - 1, j - 1});
+        if (j + 1 < Tr) tasks.push_back(int4{3, j + 1, j, j});
+      }
+      if (Tr > 0)
+        for (int i = Tr; i < T; ++i) tasks.push_back(int4{1, i, Tr - 1, Tr - 1});
+      for (int j = Tr; j < T; ++j)
+        for (int i = j; i < T; ++i) tasks.push_back(int4{2, i, j, Tr});
+      if ((int)tasks.size() != T * (T + 1) / 2) throw std::logic_error("tile task list incomplete");
       bh->tb.upload(tb, st_);
       bh->tasks.upload(tasks, st_);
       bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
