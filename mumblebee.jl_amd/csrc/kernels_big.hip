@@ -684,15 +684,63 @@ __device__ __forceinline__ int take_published(const int* flags, int j, int step,
   return r;
 }
 
-// forward: tile b (ticket order) accumulates L[b, j] x_j for published j < min(b, Tr); a
-// factor tile then forms x_b = L_bb⁻¹ (v_b − Σ) and publishes it, a separator tile adds Σ into
-// the front's update vector. TF: element type of the stored factor; the arithmetic is FP64.
+// 64-row solution blocks solved in order by one solve workgroup. Two blocks per workgroup halve
+// the hand-offs between workgroups but double the factor tiles each workgroup streams, which
+// measured slower (2000×2000 front: backward solve 96 → 152 µs); one it is.
+constexpr int SG = 1;
+static_assert(SG == 1 || SG == 2, "the solve kernels index their per-block accumulators statically");
+
+// A 64×64 factor tile held for a matrix-vector product, 16 elements per thread. Row layout
+// (forward solve): thread (r, q) = (t & 63, t >> 6) holds element (r, q + 4i). Column layout
+// (backward solve): thread (c, q) holds element (q + 4i, c). Entries past the tile are zero.
+struct GemvRegs {
+  double v[16];
+};
+
+template <class TF>
+__device__ __forceinline__ void gemv_rows(GemvRegs& R, const TF* F, int LD, int r0, int c0, int h, int w) {
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int c = q + 4 * i;
+    R.v[i] = (r < h && c < w) ? (double)F[(size_t)(r0 + r) + (size_t)(c0 + c) * LD] : 0.0;
+  }
+}
+
+// column layout through an LDS tile S (the global reads stay coalesced); S is free again on return
+template <class TF>
+__device__ __forceinline__ void gemv_cols(GemvRegs& R, double* S, const TF* F, int LD, int r0, int c0, int h, int w) {
+  const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
+  load_tile<false>(S, F, LD, r0, c0, h, w);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; ++i) R.v[i] = S[(q + 4 * i) + c * LDT];
+  __syncthreads();
+}
+
+// Σ over this thread's 16 elements of R times x[q + 4i]
+__device__ __forceinline__ double gemv_dot(const GemvRegs& R, const double* x) {
+  const int q = threadIdx.x >> 6;
+  double a = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a += R.v[i] * x[q + 4 * i];
+  return a;
+}
+
+// forward. Ticket t < ⌈Tr/SG⌉: the factor blocks b0 = SG·t … b0+SG−1 in order. The workgroup
+// accumulates L[b, j] x_j for the published blocks j < b0 while the earlier groups solve; the
+// tiles against block b0−1 (the last to arrive) and within the group are staged in registers and
+// the inverted diagonal blocks in LDS before it waits, so each hand-off between workgroups is
+// followed only by products from on-chip operands. Then x_b = L_bb⁻¹ (v_b − Σ) for each block of
+// the group, and its blocks are published together. Later tickets: one separator tile each, whose
+// Σ goes into the front's update vector. TF: element type of the stored factor; the arithmetic
+// is FP64.
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
-  __shared__ double Sinv[NB * LDT];
-  __shared__ double Sc[NB * LDT];
+  __shared__ double Sinv[SG][NB * LDT];
   __shared__ double xs[KB * NB];
+  __shared__ double xo[SG][NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -701,68 +749,117 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
   __syncthreads();
   if (I.sc[S_ACTIVE] == 0.0) return;
-  const int b = (int)tk;
-  if (b < 0 || b >= B.T) return;
+  const int nd = (B.Tr + SG - 1) / SG;
+  const int t = (int)tk;
+  if (t < 0 || t >= nd + (B.T - B.Tr)) return;
   double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
   const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
-  const int r0 = B.tb[b], h = B.tb[b + 1] - r0;
   const int row = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int jmax = b < B.Tr ? b : B.Tr;
-  const bool diag = b < B.Tr;
-  // stage the last step's operands before waiting
-  if (diag) load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
-  if (jmax > 0) load_tile<false>(Sc, F, LD, r0, B.tb[jmax - 1], h, B.tb[jmax] - B.tb[jmax - 1]);
-  double acc = 0.0;
+  const bool diag = t < nd;
+  const int b0 = diag ? t * SG : B.Tr + (t - nd);
+  const int ng = diag ? min(SG, B.Tr - b0) : 1;
+  const int jmax = diag ? b0 : B.Tr;
+  int rg[SG], hg[SG];
+#pragma unroll
+  for (int g = 0; g < SG; ++g) {
+    rg[g] = g < ng ? B.tb[b0 + g] : 0;
+    hg[g] = g < ng ? B.tb[b0 + g + 1] - rg[g] : 0;
+  }
+  // stage the operands of the last steps before waiting
+  GemvRegs Rin[SG], Rg;
+  if (diag)
+    for (int g = 0; g < ng; ++g) load_tile<false>(Sinv[g], dinv_of<TF>(B) + (size_t)(b0 + g) * NB * NB, NB, 0, 0, NB, NB);
+  const int cl = jmax > 0 ? B.tb[jmax - 1] : 0, wl = jmax > 0 ? B.tb[jmax] - cl : 0;
+#pragma unroll
+  for (int g = 0; g < SG; ++g)
+    if (g < ng && jmax > 0) gemv_rows(Rin[g], F, LD, rg[g], cl, hg[g], wl);
+  if (ng > 1) gemv_rows(Rg, F, LD, rg[1], rg[0], hg[1], hg[0]);
+  double acc[SG];
+#pragma unroll
+  for (int g = 0; g < SG; ++g) acc[g] = 0.0;
+  // the tiles of blocks j < jmax−1 stream through registers one block ahead of the solution,
+  // so each tile's loads overlap the wait for its block; every block's contribution is summed
+  // in block order, whatever the batches (reproducible results)
+  GemvRegs Rn[SG];
+  if (jmax > 1)
+#pragma unroll
+    for (int g = 0; g < SG; ++g)
+      if (g < ng) gemv_rows(Rn[g], F, LD, rg[g], B.tb[0], hg[g], B.tb[1] - B.tb[0]);
   for (int j = 0; j < jmax;) {
     const int n = take_published(B.sflag, j, 1, min(KB, jmax - j), epoch, B.abortf);
     if (n == 0) return;
     const int c0 = B.tb[j], wtot = B.tb[j + n] - c0;
-    for (int t = threadIdx.x; t < KB * NB; t += NT) xs[t] = t < wtot ? cload(&v[c0 + t]) : 0.0;
+    for (int u = threadIdx.x; u < KB * NB; u += NT) xs[u] = u < wtot ? cload(&v[c0 + u]) : 0.0;
     __syncthreads();
-    // the batch's columns straight from the factor, except block jmax−1 (staged in LDS)
-    const bool last = j + n == jmax;
-    const int wg = last ? B.tb[jmax - 1] - c0 : wtot;
-    if (row < h) {
-      const TF* col = F + (size_t)(r0 + row) + (size_t)c0 * LD;
-#pragma unroll 4
-      for (int c = q; c < wg; c += 4) acc += (double)col[(size_t)c * LD] * xs[c];
-    }
-    if (last) {
-      const int o = B.tb[jmax - 1] - c0;
+    for (int k = j; k < j + n; ++k) {
+      const double* x = xs + (B.tb[k] - c0);
+      if (k == jmax - 1) {
 #pragma unroll
-      for (int c = q; c < NB; c += 4) acc += Sc[row + c * LDT] * xs[o + c];
+        for (int g = 0; g < SG; ++g)
+          if (g < ng) acc[g] += gemv_dot(Rin[g], x);
+      } else {
+        GemvRegs Rc[SG];
+#pragma unroll
+        for (int g = 0; g < SG; ++g) Rc[g] = Rn[g];
+        if (k + 1 < jmax - 1)
+#pragma unroll
+          for (int g = 0; g < SG; ++g)
+            if (g < ng) gemv_rows(Rn[g], F, LD, rg[g], B.tb[k + 1], hg[g], B.tb[k + 2] - B.tb[k + 1]);
+#pragma unroll
+        for (int g = 0; g < SG; ++g)
+          if (g < ng) acc[g] += gemv_dot(Rc[g], x);
+      }
     }
     __syncthreads();
     j += n;
   }
-  part[q][row] = acc;
-  __syncthreads();
-  const double sum = part[0][row] + part[1][row] + part[2][row] + part[3][row];
-  if (diag) {
-    if (threadIdx.x < NB) xs[row] = row < h ? v[r0 + row] - sum : 0.0;
+  if (!diag) {
+    part[q][row] = acc[0];
+    __syncthreads();
+    if (threadIdx.x < hg[0])
+      Fd.uvec[rg[0] - Fd.r + row] += part[0][row] + part[1][row] + part[2][row] + part[3][row];
+    return;
+  }
+#pragma unroll
+  for (int g = 0; g < SG; ++g) {
+    if (g >= ng) break;
+    if (g == 1) acc[1] += gemv_dot(Rg, xo[0]);
+    part[q][row] = acc[g];
+    __syncthreads();
+    const double sum = part[0][row] + part[1][row] + part[2][row] + part[3][row];
+    if (threadIdx.x < NB) xs[row] = row < hg[g] ? v[rg[g] + row] - sum : 0.0;
     __syncthreads();
     double y = 0.0;
 #pragma unroll
-    for (int c = q; c < NB; c += 4) y += Sinv[row + c * LDT] * xs[c];
+    for (int c = q; c < NB; c += 4) y += Sinv[g][row + c * LDT] * xs[c];
     part[q][row] = y;
     __syncthreads();
-    if (threadIdx.x < NB && row < h) cstore(&v[r0 + row], part[0][row] + part[1][row] + part[2][row] + part[3][row]);
-    publish(&B.sflag[b], epoch);
-  } else if (threadIdx.x < h) {
-    Fd.uvec[r0 - Fd.r + row] += sum;
+    if (threadIdx.x < NB) {
+      const double x = part[0][row] + part[1][row] + part[2][row] + part[3][row];
+      xo[g][row] = row < hg[g] ? x : 0.0;
+      if (row < hg[g]) cstore(&v[rg[g] + row], x);
+    }
+    __syncthreads();
   }
+  // every thread's stores have completed, then the group's blocks are published
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x < ng) st_flag(&B.sflag[b0 + threadIdx.x], epoch);
 }
 
-// backward: column tile b (ticket order descending) accumulates L21[:, b]ᵀ x_sep and
-// L[j, b]ᵀ x_j for j > b, then forms x_b = L_bb⁻ᵀ (v_b − Σ)
+// backward. Ticket t: the factor blocks bt = Tr−1−SG·t, bt−1, … (descending), each a column
+// block whose x_b = L_bbᵀ⁻¹ (v_b − L[sep, b]ᵀ x_sep − Σ_{j>b} L[j, b]ᵀ x_j). The separator part
+// and the published blocks j > bt+1 stream through an LDS double buffer; the tiles against block
+// bt+1 (the last to arrive) and within the group are staged in registers and the inverted
+// diagonal blocks in LDS before the workgroup waits.
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
   __shared__ double S[2][NB * LDT];  // double buffer: the next tile loads while this one is used
-  __shared__ double Sinv[NB * LDT];
-  __shared__ double Sc[NB * LDT];
+  __shared__ double Sinv[SG][NB * LDT];
   __shared__ double xs[KB * NB];
+  __shared__ double xo[SG][NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -770,78 +867,119 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
   __syncthreads();
   if (I.sc[S_ACTIVE] == 0.0) return;
-  const int tkt = (int)tk;
-  if (tkt < 0 || tkt >= B.Tr) return;
-  const int b = B.Tr - 1 - tkt;
+  const int nd = (B.Tr + SG - 1) / SG;
+  const int t = (int)tk;
+  if (t < 0 || t >= nd) return;
+  const int bt = B.Tr - 1 - t * SG;
+  const int ng = min(SG, bt + 1);
   double* vall = I.nv + (size_t)kv * I.n;
   double* v = vall + Fd.res0;
   const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
-  const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
-  load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
-  if (b + 1 < B.Tr) load_tile<false>(Sc, F, LD, B.tb[b + 1], c0, B.tb[b + 2] - B.tb[b + 1], w);
-  double acc = 0.0;
+  int cg[SG], wg[SG];
+#pragma unroll
+  for (int g = 0; g < SG; ++g) {
+    cg[g] = g < ng ? B.tb[bt - g] : 0;
+    wg[g] = g < ng ? B.tb[bt - g + 1] - cg[g] : 0;
+  }
+  for (int g = 0; g < ng; ++g) load_tile<false>(Sinv[g], dinv_of<TF>(B) + (size_t)(bt - g) * NB * NB, NB, 0, 0, NB, NB);
+  GemvRegs Rin[SG], Rg;
+  const bool has_in = bt + 1 < B.Tr;
+  const int rin = has_in ? B.tb[bt + 1] : 0, hin = has_in ? B.tb[bt + 2] - rin : 0;
+#pragma unroll
+  for (int g = 0; g < SG; ++g)
+    if (g < ng && has_in) gemv_cols(Rin[g], S[0], F, LD, rin, cg[g], hin, wg[g]);
+  if (ng > 1) gemv_cols(Rg, S[0], F, LD, cg[0], cg[1], wg[0], wg[1]);
+  double acc[SG];
+#pragma unroll
+  for (int g = 0; g < SG; ++g) acc[g] = 0.0;
   int cur = 0;
   TileRegs<TF> R;
   // separator part (rows ≥ r; x_sep is final): 64-row chunks through LDS so that the global reads
   // run down the columns, the next chunk in flight while this one is multiplied
-  if (Fd.r < LD) fetch_tile<false>(R, F, LD, Fd.r, c0, min(NB, LD - Fd.r), w);
-  for (int rs0 = Fd.r; rs0 < LD; rs0 += NB) {
-    const int h = min(NB, LD - rs0);
-    put_tile(S[cur], R);
-    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
-    __syncthreads();
-    if (rs0 + NB < LD) fetch_tile<false>(R, F, LD, rs0 + NB, c0, min(NB, LD - rs0 - NB), w);
+  for (int g = 0; g < ng; ++g) {
+    if (Fd.r < LD) fetch_tile<false>(R, F, LD, Fd.r, cg[g], min(NB, LD - Fd.r), wg[g]);
+    double a = 0.0;
+    for (int rs0 = Fd.r; rs0 < LD; rs0 += NB) {
+      const int h = min(NB, LD - rs0);
+      put_tile(S[cur], R);
+      if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
+      __syncthreads();
+      if (rs0 + NB < LD) fetch_tile<false>(R, F, LD, rs0 + NB, cg[g], min(NB, LD - rs0 - NB), wg[g]);
 #pragma unroll
-    for (int rr = q; rr < NB; rr += 4) acc += S[cur][rr + col * LDT] * xs[rr];
-    __syncthreads();
-    cur ^= 1;
+      for (int rr = q; rr < NB; rr += 4) a += S[cur][rr + col * LDT] * xs[rr];
+      __syncthreads();
+      cur ^= 1;
+    }
+    acc[g] += a;
   }
-  for (int j = B.Tr - 1; j > b;) {
-    // blocks j, j−1, …, j−n+1 (all published), their rows [rlo, rhi); tile b+1 is staged (Sc),
-    // the others stream through the double buffer
-    const int n = take_published(B.sflag, j, -1, min(KB, j - b), epoch, B.abortf);
+  // published blocks j > bt, in batches, every block in descending order whatever the batches
+  // (so the sums, and the results, are reproducible): the tiles (k, bt−g) for k ≠ bt+1 stream
+  // through the double buffer, the next one in flight while this one is multiplied; block bt+1
+  // (staged) comes last
+  for (int j = B.Tr - 1; j > bt;) {
+    const int n = take_published(B.sflag, j, -1, min(KB, j - bt), epoch, B.abortf);
     if (n == 0) return;
     const int klo = j - n + 1, rlo = B.tb[klo], rhi = B.tb[j + 1];
-    for (int t = threadIdx.x; t < KB * NB; t += NT) xs[t] = t < rhi - rlo ? cload(&v[rlo + t]) : 0.0;
-    int kn = klo == b + 1 ? klo + 1 : klo;  // next tile to fetch
-    if (kn <= j) fetch_tile<false>(R, F, LD, B.tb[kn], c0, B.tb[kn + 1] - B.tb[kn], w);
+    for (int u = threadIdx.x; u < KB * NB; u += NT) xs[u] = u < rhi - rlo ? cload(&v[rlo + u]) : 0.0;
+    // streamed tiles: index s ↦ (k, g) with k = j − s / ng down to kstop (bt+1 excluded), g = s % ng
+    const int kstop = klo == bt + 1 ? klo + 1 : klo;
+    const int nstream = (j - kstop + 1) * ng;
+    if (nstream > 0) fetch_tile<false>(R, F, LD, B.tb[j], cg[0], B.tb[j + 1] - B.tb[j], wg[0]);
     __syncthreads();
-    for (int k = klo; k <= j; ++k) {
-      const double* T = Sc;
-      if (k != b + 1) {
-        put_tile(S[cur], R);
-        __syncthreads();
-        T = S[cur];
-        cur ^= 1;
-        kn = k + 1;
-        if (kn <= j) fetch_tile<false>(R, F, LD, B.tb[kn], c0, B.tb[kn + 1] - B.tb[kn], w);
+    for (int sidx = 0; sidx < nstream; ++sidx) {
+      const int k = j - sidx / ng, g = sidx % ng;
+      put_tile(S[cur], R);
+      __syncthreads();
+      const double* T = S[cur];
+      cur ^= 1;
+      if (sidx + 1 < nstream) {
+        const int k2 = j - (sidx + 1) / ng, g2 = (sidx + 1) % ng;
+        fetch_tile<false>(R, F, LD, B.tb[k2], cg[g2], B.tb[k2 + 1] - B.tb[k2], wg[g2]);
       }
       const double* x = xs + (B.tb[k] - rlo);
+      double a = 0.0;
 #pragma unroll
-      for (int rr = q; rr < NB; rr += 4) acc += T[rr + col * LDT] * x[rr];
+      for (int rr = q; rr < NB; rr += 4) a += T[rr + col * LDT] * x[rr];
+      if (g == 0) acc[0] += a;
+      else acc[SG - 1] += a;
+    }
+    if (klo == bt + 1) {
+#pragma unroll
+      for (int g = 0; g < SG; ++g)
+        if (g < ng) acc[g] += gemv_dot(Rin[g], xs);
     }
     __syncthreads();
     j -= n;
   }
-  part[q][col] = acc;
-  __syncthreads();
-  if (threadIdx.x < NB)
-    xs[col] = col < w ? v[c0 + col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : 0.0;
-  __syncthreads();
-  // x_b[c] = Σ_r (L⁻¹)[r][c] t[r]
-  double y = 0.0;
 #pragma unroll
-  for (int rr = q; rr < NB; rr += 4) y += Sinv[rr + col * LDT] * xs[rr];
-  part[q][col] = y;
+  for (int g = 0; g < SG; ++g) {
+    if (g >= ng) break;
+    if (g == 1) acc[1] += gemv_dot(Rg, xo[0]);
+    part[q][col] = acc[g];
+    __syncthreads();
+    if (threadIdx.x < NB)
+      xs[col] = col < wg[g] ? v[cg[g] + col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : 0.0;
+    __syncthreads();
+    // x_b[c] = Σ_r (L⁻¹)[r][c] t[r]
+    double y = 0.0;
+#pragma unroll
+    for (int rr = q; rr < NB; rr += 4) y += Sinv[g][rr + col * LDT] * xs[rr];
+    part[q][col] = y;
+    __syncthreads();
+    if (threadIdx.x < NB) {
+      const double x = part[0][col] + part[1][col] + part[2][col] + part[3][col];
+      xo[g][col] = col < wg[g] ? x : 0.0;
+      if (col < wg[g]) cstore(&v[cg[g] + col], x);
+    }
+    __syncthreads();
+  }
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (threadIdx.x < NB && col < w) cstore(&v[c0 + col], part[0][col] + part[1][col] + part[2][col] + part[3][col]);
-  publish(&B.sflag[b], epoch);
+  if (threadIdx.x < ng) st_flag(&B.sflag[bt - threadIdx.x], epoch);
 }
 
-// ------------------------------------------------------------------------------------------
-// launch wrappers
 // ------------------------------------------------------------------------------------------
 
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
@@ -850,17 +988,21 @@ void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int 
   else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
 }
 
-void launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
-                       const InstDev* I, int kv, int epoch, long long base, bool fp32) {
+int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
+                      const InstDev* I, int kv, int epoch, long long base, bool fp32) {
+  const int grid = (B.Tr + SG - 1) / SG + (B.T - B.Tr);
   k_big_gather<<<1, 256, 0, st>>>(F, f, childlist, I, kv);
-  if (fp32) k_big_fsolve<float><<<B.T, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
-  else k_big_fsolve<double><<<B.T, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  if (fp32) k_big_fsolve<float><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  else k_big_fsolve<double><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  return grid;
 }
 
-void launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
-                       int epoch, long long base, bool fp32) {
-  if (fp32) k_big_bsolve<float><<<B.Tr, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
-  else k_big_bsolve<double><<<B.Tr, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+int launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
+                      int epoch, long long base, bool fp32) {
+  const int grid = (B.Tr + SG - 1) / SG;
+  if (fp32) k_big_bsolve<float><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  else k_big_bsolve<double><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
+  return grid;
 }
 
 }  // namespace mb

@@ -867,9 +867,8 @@ void Solver::lsolve(int kv) {
     for (int b : L.big) {
       BigHost& bh = *big_[b];
       ++epoch_;
-      launch_big_fsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_childlist_.as<int>(), d_inst_.as<InstDev>(),
-                        kv, epoch_, bh.sbase, fac_fp32_);
-      bh.sbase += bh.B.T;
+      bh.sbase += launch_big_fsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_childlist_.as<int>(),
+                                    d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
     }
     prof_end(ps, PROF_FSOLVE, 0.0, L.solve_bytes);
   }
@@ -884,9 +883,8 @@ void Solver::usolve(int kv) {
     for (int b : L.big) {
       BigHost& bh = *big_[b];
       ++epoch_;
-      launch_big_bsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_inst_.as<InstDev>(), kv, epoch_, bh.sbase,
-                        fac_fp32_);
-      bh.sbase += bh.B.Tr;
+      bh.sbase += launch_big_bsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_inst_.as<InstDev>(), kv, epoch_,
+                                    bh.sbase, fac_fp32_);
     }
     prof_end(ps, PROF_BSOLVE, 0.0, L.solve_bytes);
   }

@@ -1,0 +1,39 @@
+"""Single-front solves are reproducible bit for bit: the persistent factorization subtracts each
+tile's updates in a fixed column order and the pipelined triangular solves sum their
+contributions in a fixed block order, however the published blocks were batched. (Fronts with
+several children add the children's update blocks into the parent with atomics, so multi-front
+problems agree only to rounding between runs.)"""
+import sys
+
+import numpy as np
+import pytest
+
+import mumblebee_hip as mh
+from mumblebee_hip import generators as G
+from helpers import hip_problem
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "entropy200": lambda: G.exppow(nexp=200, npow=0, m=0, eps=0.0, seed=0),
+    "qp1500": lambda: G.qp(n=1500, m=700, seed=3),
+    "socp900": lambda: G.socp(n=900, m=450, ncone=40, seed=14, adens=0.01),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("precision", [0, 1])
+def test_single_front_solves_repeat_exactly(name, precision):
+    print("case", name, "precision", precision, flush=True)
+    sys.stdout.flush()
+    prob = hip_problem(CASES[name]())
+    st = mh.IPMSettings(precision=precision)
+    runs = []
+    for _ in range(3):
+        s = mh.init(prob, st)
+        assert s.stats()["nfront"] == 1
+        runs.append(s.solve())
+        s.close()
+    for r in runs[1:]:
+        assert r.status == runs[0].status and r.niter == runs[0].niter
+        assert np.array_equal(r.p, runs[0].p) and np.array_equal(r.d, runs[0].d) and np.array_equal(r.y, runs[0].y)
