@@ -144,7 +144,8 @@ def _resident_run(args, mh, precision):
     K timed ones `--slots` at a time; returns the measurements and the timed results."""
     K, W, S = args.steps, args.warmup, max(1, args.slots)
     ncu = mh.device_info()["cus"]
-    settings = mh.IPMSettings(dag_grid=max(16, ncu // S) if S > 1 else 0, precision=precision)
+    grid = args.dag_grid if args.dag_grid > 0 else (max(16, ncu // S) if S > 1 else 0)
+    settings = mh.IPMSettings(dag_grid=grid, precision=precision)
     kinds = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
     # exponential cones can end NEAR_OPTIMAL: the reference's line search stalls within a few
     # noise bands of a cone boundary (a documented property of the path, not of this build)
@@ -344,6 +345,8 @@ def main():
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--dag-grid", type=int, default=0,
+                    help="workgroups of each large-front factorization (0: the CUs shared among the slots)")
     ap.add_argument("--n", type=int, default=2000)
     ap.add_argument("--m", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")

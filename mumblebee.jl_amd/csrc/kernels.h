@@ -51,8 +51,9 @@ void launch_axpby(hipStream_t, const InstDev*, const Tile*, int nt, int space, i
 void launch_axpbypcz(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kw, int kz,
                      Coef a, Coef b, Coef c);
 void launch_vmul(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kz, int mode);
+// counter: ninst ints, zero between launches (the kernel leaves them at zero)
 void launch_dot(hipStream_t, const InstDev*, const Tile*, int nt, const int* tile_ofs, int ninst, int space,
-                int kx, int ky, int mode, int kw0, int kw1, double* partial, int slot, int op);
+                int kx, int ky, int mode, int kw0, int kw1, double* partial, int* counter, int slot, int op);
 void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
 void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky, Coef a);
 // psd / dense: lists of the PSD cones and of the cones with dense scaling blocks (SOC, PSD)

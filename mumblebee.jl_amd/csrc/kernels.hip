@@ -130,6 +130,53 @@ __global__ void k_dot_partial(const InstDev* insts, const Tile* tiles, int space
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// The same partial sums, and in the last block of an instance to finish, sc[slot] = op(Σ partial)
+// over the instance's tiles in k_reduce_slot's order (identical result, one launch instead of
+// two). counter[inst] counts the instance's finished blocks; the last block puts it back to 0.
+__global__ void k_dot(const InstDev* insts, const Tile* tiles, const int* tile_ofs, int space, int kx, int ky,
+                      int mode, int kw0, int kw1, double* partial, int* counter, int slot, int op) {
+  __shared__ double red[TILE / 64];
+  __shared__ int last;
+  const Tile t = tiles[blockIdx.x];
+  const InstDev& I = insts[t.inst];
+  const int i = t.start + threadIdx.x;
+  double v = 0.0;
+  if (I.sc[S_ACTIVE] != 0.0 && i < spacelen(I, space)) {
+    const double* x = vecof(I, space, kx);
+    if (mode == 0) {
+      v = x[i] * vecof(I, space, ky)[i];
+    } else if (mode == 1) {
+      const double q = x[i] / vecof(I, space, ky)[i];
+      v = q * q;
+    } else if (mode == 2) {
+      v = x[i];
+    } else {
+      const double tau = I.sc[S_TAU];
+      const double a = x[i] + tau * vecof(I, space, kw0)[i];
+      const double b = vecof(I, space, ky)[i] + tau * vecof(I, space, kw1)[i];
+      v = a * b;
+    }
+  }
+  const double s = block_sum(v, red);
+  const int a = tile_ofs[t.inst], b = tile_ofs[t.inst + 1];
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s;
+    __threadfence();  // the partial is visible device-wide before the count
+    last = atomicAdd(&counter[t.inst], 1) == b - a - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double w = 0.0;
+  for (int k = a + (int)threadIdx.x; k < b; k += blockDim.x)
+    w += __hip_atomic_load(&partial[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double r = block_sum(w, red);
+  if (threadIdx.x == 0) {
+    if (I.sc[S_ACTIVE] != 0.0) I.sc[slot] = op == 1 ? sqrt(r) : r;
+    __hip_atomic_store(&counter[t.inst], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // sc[slot] = op(Σ partial over the instance's tiles), one block per instance, fixed order
 __global__ void k_reduce_slot(const InstDev* insts, const int* tile_ofs, const double* partial, int slot,
                               int op) {
@@ -1012,7 +1059,12 @@ void launch_vmul(hipStream_t st, const InstDev* I, const Tile* T, int nt, int sp
   if (nt) k_vmul<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, kz, mode);
 }
 void launch_dot(hipStream_t st, const InstDev* I, const Tile* T, int nt, const int* tile_ofs, int ninst,
-                int space, int kx, int ky, int mode, int kw0, int kw1, double* partial, int slot, int op) {
+                int space, int kx, int ky, int mode, int kw0, int kw1, double* partial, int* counter, int slot,
+                int op) {
+  if (nt && ninst == 1) {
+    k_dot<<<nt, TILE, 0, st>>>(I, T, tile_ofs, space, kx, ky, mode, kw0, kw1, partial, counter, slot, op);
+    return;
+  }
   if (nt) k_dot_partial<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, mode, kw0, kw1, partial);
   k_reduce_slot<<<ninst, TILE, 0, st>>>(I, tile_ofs, partial, slot, op);
 }

@@ -610,6 +610,8 @@ void Solver::build_device() {
   d_ntofs_.upload(std::vector<int>{0, ntile_n_}, st_);
   d_mtofs_.upload(std::vector<int>{0, ntile_m_}, st_);
   d_partial_.alloc(sizeof(double) * (std::max<i64>({(i64)ntile_n_, (i64)ntile_m_, NF, 1}) + 16));
+  d_dotcnt_.alloc(sizeof(int) * 2);
+  HIPCHK(hipMemsetAsync(d_dotcnt_.p, 0, d_dotcnt_.bytes, st_));
   d_coneout_.alloc(sizeof(double) * 3 * std::max<i64>(N, 1));
   // instance table
   idev_.nv = d_nv_.as<double>();
@@ -840,10 +842,10 @@ void Solver::dot(int space, int kx, int ky, int mode, int slot, int op, int kw0,
   const InstDev* I = d_inst_.as<InstDev>();
   if (space == 0)
     launch_dot(st_, I, d_ntiles_.as<Tile>(), ntile_n_, d_ntofs_.as<int>(), 1, 0, kx, ky, mode, kw0, kw1,
-               d_partial_.as<double>(), slot, op);
+               d_partial_.as<double>(), d_dotcnt_.as<int>(), slot, op);
   else
     launch_dot(st_, I, d_mtiles_.as<Tile>(), ntile_m_, d_mtofs_.as<int>(), 1, 1, kx, ky, mode, kw0, kw1,
-               d_partial_.as<double>(), slot, op);
+               d_partial_.as<double>(), d_dotcnt_.as<int>(), slot, op);
 }
 
 void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {

@@ -212,7 +212,7 @@ class Solver {
   DevBuf d_Bp_, d_Bc_, d_Bv_, d_Btp_, d_Btc_, d_Btv_, d_Qp_, d_Qc_, d_Qv_;
   DevBuf d_upos_, d_ul_, d_uq_, d_D_, d_cache_, d_uvec_, d_sepcols_, d_relidx_, d_childlist_;
   DevBuf d_steps_, d_ea_items_, d_ea_tile0_, d_solve_items_, d_ntiles_, d_mtiles_, d_ntofs_, d_mtofs_;
-  DevBuf d_partial_, d_coneout_, d_qdiag_;
+  DevBuf d_partial_, d_coneout_, d_qdiag_, d_dotcnt_;
   int ntile_n_ = 0, ntile_m_ = 0;
   DevBuf d_sptiles_[3];  // row tiles of B, Bᵀ, Q for the group-per-row spmv
   DevBuf d_psdlist_, d_denselist_;  // PSD cones; cones with dense scaling blocks (SOC, PSD)
