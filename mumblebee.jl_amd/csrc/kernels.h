@@ -59,8 +59,9 @@ void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, in
 // psd / dense: lists of the PSD cones and of the cones with dense scaling blocks (SOC, PSD)
 void launch_cone_scale(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
                        double* out, const int* psd, int npsd);
+// dense_nmax: the largest order among the dense-block cones; dpart: DENSE_SPLIT doubles per dense cone
 void launch_cone_hnorm2(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                        double* out, int slot, const int* dense, int ndense);
+                        double* out, int slot, const int* dense, int ndense, int dense_nmax, double* dpart);
 void launch_cone_corr0(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr);
 void launch_cone_corr(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr, int kdp, int kdd);
 void launch_cone_maxstep(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
@@ -73,17 +74,22 @@ void launch_psd_corr0(hipStream_t, const InstDev*, const ConeDesc*, const int* l
 void launch_psd_corr(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kr, int kdp, int kdd);
 void launch_psd_maxstep(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kdp, int kdd,
                         double step_frac, double* out);
-void launch_cone_matvec_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kx, int ky,
-                              Coef a);
-void launch_cone_hnorm2_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, double* out);
+// the dense-block cones split over many workgroups: 64-row slices for D·x, DENSE_SPLIT chunks per
+// cone for ‖Q_vv + D_v‖² (partials in part, summed in a fixed order); nmax: the largest order
+constexpr int DENSE_SPLIT = 32;
+void launch_cone_matvec_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int nmax, int kx,
+                              int ky, Coef a);
+void launch_cone_hnorm2_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, double* out,
+                              double* part);
 void launch_zerofree(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx);
 // the shadow-solve seed of every listed exponential / power cone (initcache!)
 void launch_td_initcache(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n);
 void launch_fill(hipStream_t, double* x, size_t n, double v);
 void launch_assemble_union(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent);
+// nmax: the largest cone order (sets how many workgroups share one cone's columns)
 void launch_assemble_cones(hipStream_t, const InstDev*, const ConeDesc*, const int* cone_front, const FrontDesc*,
-                           int nc);
+                           int nc, int nmax);
 void launch_add_diag(hipStream_t, const FrontDesc*, int nf, const InstDev*);
 void launch_front_norm2(hipStream_t, const FrontDesc*, int nf, double* partial);
 void launch_extend_add(hipStream_t, const FrontDesc*, const int* items, const int* item_tile0, int nitem,

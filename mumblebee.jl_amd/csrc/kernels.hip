@@ -9,6 +9,8 @@
 //  * per-front forward / backward triangular solves.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdint>
 
@@ -371,7 +373,7 @@ __global__ void k_cone_hnorm2(const InstDev* insts, const ConeDesc* cones, int n
   double s = 0.0;
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) {
-      double h = C.Qd[i + (size_t)j * n];
+      double h = C.Qd ? C.Qd[i + (size_t)j * n] : 0.0;
       if (C.kind == CONE_POS && i == j) h += C.D[i];
       if (C.kind == CONE_EXP || C.kind == CONE_POW) h += C.D[i + (size_t)j * n];
       s += h * h;
@@ -633,23 +635,27 @@ __global__ void k_assemble_union(const InstDev* insts, int inst, const int64_t* 
 
 // cone blocks: one thread per (cone, column); lower triangle of the block lands in the
 // front holding the cone's columns
+// grid (cone, column group): a dense block's columns j ≡ blockIdx.y (mod gridDim.y), the threads
+// along each column (coalesced); a nonnegative cone's diagonal spread over the whole group
 __global__ void k_assemble_cones(const InstDev* insts, const ConeDesc* cones, const int* cone_front,
                                  const FrontDesc* fronts, int ncone) {
-  int c = blockIdx.x;
+  const int c = blockIdx.x;
   if (c >= ncone) return;
-  ConeDesc C = cones[c];
+  const ConeDesc C = cones[c];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0 || C.kind == CONE_FREE) return;
   const FrontDesc& Fd = fronts[cone_front[c]];
-  double delta = I.sc[S_DELTA];
-  int off = C.col0 - Fd.res0;
-  for (int j = threadIdx.x; j < C.n; j += blockDim.x) {
+  const double delta = I.sc[S_DELTA];
+  const int off = C.col0 - Fd.res0;
+  if (C.kind == CONE_POS) {
+    for (int j = blockIdx.y * blockDim.x + threadIdx.x; j < C.n; j += gridDim.y * blockDim.x)
+      Fd.F[(size_t)(off + j) * Fd.ld + off + j] += delta * C.D[j];
+    return;
+  }
+  for (int j = blockIdx.y; j < C.n; j += gridDim.y) {
     double* col = Fd.F + (size_t)(off + j) * Fd.ld + off;
-    if (C.kind == CONE_POS) {
-      col[j] += delta * C.D[j];
-    } else {
-      for (int i = j; i < C.n; ++i) col[i] += delta * C.D[i + (size_t)j * C.n];
-    }
+    const double* Dj = C.D + (size_t)j * C.n;
+    for (int i = j + threadIdx.x; i < C.n; i += blockDim.x) col[i] += delta * Dj[i];
   }
 }
 
@@ -1082,9 +1088,11 @@ void launch_cone_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, int 
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 0);
 }
 void launch_cone_hnorm2(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                        int ninst, double* out, int slot, const int* dense, int ndense) {
+                        int ninst, double* out, int slot, const int* dense, int ndense, int dense_nmax,
+                        double* dpart) {
+  (void)dense_nmax;
   if (nc) k_cone_hnorm2<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
-  launch_cone_hnorm2_dense(st, I, C, dense, ndense, out);
+  launch_cone_hnorm2_dense(st, I, C, dense, ndense, out, dpart);
   k_cone_sum<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, slot);
 }
 void launch_cone_corr0(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kr) {
@@ -1110,8 +1118,9 @@ void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int
   if (nent) k_assemble_union<<<cdiv(nent, TILE), TILE, 0, st>>>(I, inst, pos, lval, qval, nent);
 }
 void launch_assemble_cones(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* cone_front,
-                           const FrontDesc* F, int nc) {
-  if (nc) k_assemble_cones<<<nc, 64, 0, st>>>(I, C, cone_front, F, nc);
+                           const FrontDesc* F, int nc, int nmax) {
+  const int groups = std::max(1, std::min(32, nmax / 64));
+  if (nc) k_assemble_cones<<<dim3(nc, groups), 256, 0, st>>>(I, C, cone_front, F, nc);
 }
 void launch_add_diag(hipStream_t st, const FrontDesc* F, int nf, const InstDev* I) {
   if (nf) k_add_diag<<<nf, TILE, 0, st>>>(F, nf, I);

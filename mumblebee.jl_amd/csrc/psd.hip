@@ -473,39 +473,61 @@ __global__ void __launch_bounds__(PT) k_psd_maxstep(const InstDev* insts, const 
 }
 
 // y += a D x on the dense scaling blocks (second-order and PSD cones), one workgroup per cone
+// y += a·D x over the dense-block cones: grid (cone, 64-row slice); each wave sums a quarter of
+// the columns (j ≡ wave mod 4) for the slice's rows (coalesced along the rows), and the four
+// partial sums are added in a fixed order
 __global__ void __launch_bounds__(PT) k_cone_matvec_dense(const InstDev* insts, const ConeDesc* cones,
                                                           const int* list, int kx, int ky, Coef a) {
+  __shared__ double red[PT / 64][64];
   const ConeDesc C = cones[list[blockIdx.x]];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
+  const int i0 = blockIdx.y * 64;
+  if (i0 >= C.n) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = i0 + lane;
   const double* x = cvec(I, kx) + C.col0;
-  double* y = cvec(I, ky) + C.col0;
-  const double av = a.slot < 0 ? a.c : (a.inv ? a.c / I.sc[a.slot] : a.c * I.sc[a.slot]);
-  for (int i = threadIdx.x; i < C.n; i += PT) {
-    double acc = 0.0;
-    for (int j = 0; j < C.n; ++j) acc = fma(C.D[i + (size_t)j * C.n], x[j], acc);
-    y[i] += av * acc;
+  double acc = 0.0;
+  if (i < C.n)
+    for (int j = w; j < C.n; j += PT / 64) acc = fma(C.D[i + (size_t)j * C.n], x[j], acc);
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && i < C.n) {
+    const double av = a.slot < 0 ? a.c : (a.inv ? a.c / I.sc[a.slot] : a.c * I.sc[a.slot]);
+    cvec(I, ky)[C.col0 + i] += av * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
   }
 }
 
-// ‖Q_vv + D_v‖² of the dense-block cones into out[c]
+// ‖Q_vv + D_v‖² of the dense-block cones: grid (cone, chunk) → part[DENSE_SPLIT·listed + chunk]
 __global__ void __launch_bounds__(PT) k_cone_hnorm2_dense(const InstDev* insts, const ConeDesc* cones,
-                                                          const int* list, double* out) {
+                                                          const int* list, double* part) {
   __shared__ double red[PT / 64];
-  const int c = list[blockIdx.x];
-  const ConeDesc C = cones[c];
+  const ConeDesc C = cones[list[blockIdx.x]];
   const InstDev& I = insts[C.inst];
-  if (I.sc[S_ACTIVE] == 0.0) return;
   double s = 0.0;
-  const size_t nn = (size_t)C.n * C.n;
-  for (size_t t = threadIdx.x; t < nn; t += PT) {
-    double h = C.Qd[t] + C.D[t];
-    s = fma(h, h, s);
+  if (I.sc[S_ACTIVE] != 0.0) {
+    const size_t nn = (size_t)C.n * C.n;
+    const size_t chunk = (nn + DENSE_SPLIT - 1) / DENSE_SPLIT;
+    const size_t lo = chunk * blockIdx.y, hi = lo + chunk < nn ? lo + chunk : nn;
+    for (size_t t = lo + threadIdx.x; t < hi; t += PT) {
+      const double h = (C.Qd ? C.Qd[t] : 0.0) + C.D[t];
+      s = fma(h, h, s);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[c] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) part[(size_t)blockIdx.x * DENSE_SPLIT + blockIdx.y] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out[cone] = Σ of its DENSE_SPLIT partials, in order
+__global__ void k_cone_dense_sum(const InstDev* insts, const ConeDesc* cones, const int* list, const double* part,
+                                 double* out) {
+  const int c = list[blockIdx.x];
+  const InstDev& I = insts[cones[c].inst];
+  if (threadIdx.x != 0 || I.sc[S_ACTIVE] == 0.0) return;
+  double s = 0.0;
+  for (int k = 0; k < DENSE_SPLIT; ++k) s += part[(size_t)blockIdx.x * DENSE_SPLIT + k];
+  out[c] = s;
 }
 
 void launch_psd_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* list, int n, double* out) {
@@ -522,13 +544,15 @@ void launch_psd_maxstep(hipStream_t st, const InstDev* I, const ConeDesc* C, con
                         double step_frac, double* out) {
   if (n) k_psd_maxstep<<<n, PT, 0, st>>>(I, C, list, kdp, kdd, step_frac, out);
 }
-void launch_cone_matvec_dense(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* list, int n, int kx,
-                              int ky, Coef a) {
-  if (n) k_cone_matvec_dense<<<n, PT, 0, st>>>(I, C, list, kx, ky, a);
+void launch_cone_matvec_dense(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* list, int n, int nmax,
+                              int kx, int ky, Coef a) {
+  if (n) k_cone_matvec_dense<<<dim3(n, (nmax + 63) / 64), PT, 0, st>>>(I, C, list, kx, ky, a);
 }
 void launch_cone_hnorm2_dense(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* list, int n,
-                              double* out) {
-  if (n) k_cone_hnorm2_dense<<<n, PT, 0, st>>>(I, C, list, out);
+                              double* out, double* part) {
+  if (!n) return;
+  k_cone_hnorm2_dense<<<dim3(n, DENSE_SPLIT), PT, 0, st>>>(I, C, list, part);
+  k_cone_dense_sum<<<n, 64, 0, st>>>(I, C, list, part, out);
 }
 
 }  // namespace mb

@@ -500,13 +500,21 @@ void Solver::build_device() {
     const bool td = k == CONE_EXP || k == CONE_POW;
     dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : (k == CONE_FREE ? 0 : nc * nc));
     cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : (k == CONE_PSD ? 2 * d * d + d : (td ? tdc::CACHE : 0)));
-    qofs[v + 1] = qofs[v] + nc * nc;
+  }
+  // the diagonal blocks of Q, for the cones that have one (‖H‖ reads them; the others read zero)
+  std::vector<i64> qarc(N, -1);
+  for (i64 v = 0; v < N; ++v)
+    for (i64 e = P_.Q.xsrc[v]; e < P_.Q.xsrc[v + 1]; ++e)
+      if (P_.Q.tgt[e] == v) qarc[v] = e;
+  for (i64 v = 0; v < N; ++v) {
+    const i64 nc = P_.B.ncols(v);
+    qofs[v + 1] = qofs[v] + (qarc[v] >= 0 ? nc * nc : 0);
   }
   std::vector<double> qdiag(qofs[N], 0.0);
   for (i64 v = 0; v < N; ++v)
-    for (i64 e = P_.Q.xsrc[v]; e < P_.Q.xsrc[v + 1]; ++e)
-      if (P_.Q.tgt[e] == v) std::copy(P_.Q.val.begin() + P_.Q.xblk[e], P_.Q.val.begin() + P_.Q.xblk[e + 1],
-                                      qdiag.begin() + qofs[v]);
+    if (qarc[v] >= 0)
+      std::copy(P_.Q.val.begin() + P_.Q.xblk[qarc[v]], P_.Q.val.begin() + P_.Q.xblk[qarc[v] + 1],
+                qdiag.begin() + qofs[v]);
   cone_front_.assign(N, 0);
   for (i64 f = 0; f < NF; ++f)
     for (i64 v = sym_.front_vtx[f]; v < sym_.front_vtx[f + 1]; ++v) cone_front_[v] = (int)f;
@@ -546,7 +554,8 @@ void Solver::build_device() {
     C.param = P_.K[v].param;
     C.D = d_D_.as<double>() + dofs[v];
     C.cache = d_cache_.as<double>() + cofs[v];
-    C.Qd = d_qdiag_.as<double>() + qofs[v];
+    C.Qd = qarc[v] >= 0 ? d_qdiag_.as<double>() + qofs[v] : nullptr;
+    if (C.kind != CONE_FREE) cone_nmax_ = std::max(cone_nmax_, C.n);
   }
   d_cones_.upload(cones_, st_);
   {
@@ -559,6 +568,8 @@ void Solver::build_device() {
     }
     npsd_ = (int)psd.size();
     ndense_ = (int)dense.size();
+    for (int v : dense) dense_nmax_ = std::max(dense_nmax_, cones_[v].n);
+    d_dpart_.alloc(sizeof(double) * DENSE_SPLIT * std::max(ndense_, 1));
     ntd_ = (int)td.size();
     if (npsd_) d_psdlist_.upload(psd, st_);
     if (ndense_) d_denselist_.upload(dense, st_);
@@ -857,8 +868,8 @@ void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
 void Solver::hmul(int kx, int ky) {
   spmv(2, kx, ky, K(1.0), K(0.0));
   launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, ky, K(1.0));
-  launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_, kx,
-                           ky, K(1.0));
+  launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
+                           dense_nmax_, kx, ky, K(1.0));
 }
 
 void Solver::lsolve(int kv) {
@@ -1001,7 +1012,7 @@ bool Solver::factor(double delta, double& rho) {
     launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
     launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_);
     launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(),
-                          ncone_);
+                          ncone_, cone_nmax_);
   };
   set_scalar(S_DELTA, delta);
   set_scalar(S_FAIL, 0.0);
@@ -1529,7 +1540,8 @@ int Solver::step() {
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                     d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_);
+                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
+                     d_dpart_.as<double>());
   spmv(2, NV_P, NV_QP, K(1.0), K(0.0));                                  // Qp
   spmv(1, MV_Y, NV_T0, K(1.0), K(0.0));                                  // Bᵀy
   axpbypcz(0, NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0));        // Δf = d + f − Qp

@@ -212,12 +212,13 @@ class Solver {
   DevBuf d_Bp_, d_Bc_, d_Bv_, d_Btp_, d_Btc_, d_Btv_, d_Qp_, d_Qc_, d_Qv_;
   DevBuf d_upos_, d_ul_, d_uq_, d_D_, d_cache_, d_uvec_, d_sepcols_, d_relidx_, d_childlist_;
   DevBuf d_steps_, d_ea_items_, d_ea_tile0_, d_solve_items_, d_ntiles_, d_mtiles_, d_ntofs_, d_mtofs_;
-  DevBuf d_partial_, d_coneout_, d_qdiag_, d_dotcnt_;
+  DevBuf d_partial_, d_coneout_, d_qdiag_, d_dotcnt_, d_dpart_;
   int ntile_n_ = 0, ntile_m_ = 0;
   DevBuf d_sptiles_[3];  // row tiles of B, Bᵀ, Q for the group-per-row spmv
   DevBuf d_psdlist_, d_denselist_;  // PSD cones; cones with dense scaling blocks (SOC, PSD)
   DevBuf d_tdlist_;                 // exponential and power cones
   int npsd_ = 0, ndense_ = 0, ntd_ = 0;
+  int cone_nmax_ = 0, dense_nmax_ = 0;  // largest cone order (non-free; dense-block cones)
   int nsptile_[3] = {0, 0, 0};
   int64_t nunion_ = 0;
   double fac_ms_ = 0.0, fac_ms_tot_ = 0.0;
