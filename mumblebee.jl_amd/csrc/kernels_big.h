@@ -27,6 +27,7 @@ struct BigFront {
   int* stage;           // Tr flags: sub-diagonal tile (j+1, j) accumulated (epoch-valued)
   unsigned long long* trace;  // optional: 4 timestamps per task (100 MHz wall clock), or null
   int* sflag;           // T solve flags (epoch-valued)
+  double* spart;        // backward solve: L[sep, b]ᵀ x_sep by 64-row separator chunk, ⌈s/64⌉·Tr·64 values
   long long* sticket;   // 64-bit monotone ticket of the solve kernels
   int inst;
 };

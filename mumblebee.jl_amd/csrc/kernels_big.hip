@@ -848,6 +848,34 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   if (threadIdx.x < ng) st_flag(&B.sflag[b0 + threadIdx.x], epoch);
 }
 
+// L[sep, b]ᵀ x_sep for one factor column block b and one 64-row chunk of the separator rows
+// (grid: Tr × ⌈s/64⌉): the tile goes through LDS so that the global reads run down its columns.
+// Separator solution values are final before the backward solve of a front starts.
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
+                                                    int kv) {
+  __shared__ double S[NB * LDT];
+  __shared__ double xsep[NB];
+  __shared__ double part[4][NB];
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int b = blockIdx.x, rs0 = Fd.r + (int)blockIdx.y * NB;
+  const int h = min(NB, B.ld - rs0), c0 = B.tb[b], w = B.tb[b + 1] - c0;
+  const double* vall = I.nv + (size_t)kv * I.n;
+  load_tile<false>(S, factor_of<TF>(B), B.ld, rs0, c0, h, w);
+  if (threadIdx.x < NB) xsep[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
+  __syncthreads();
+  const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
+  double a = 0.0;
+#pragma unroll
+  for (int rr = q; rr < NB; rr += 4) a += S[rr + col * LDT] * xsep[rr];
+  part[q][col] = a;
+  __syncthreads();
+  if (threadIdx.x < NB)
+    B.spart[((size_t)blockIdx.y * B.Tr + b) * NB + col] = (part[0][col] + part[1][col]) + (part[2][col] + part[3][col]);
+}
+
 // backward. Ticket t: the factor blocks bt = Tr−1−SG·t, bt−1, … (descending), each a column
 // block whose x_b = L_bbᵀ⁻¹ (v_b − L[sep, b]ᵀ x_sep − Σ_{j>b} L[j, b]ᵀ x_j). The separator part
 // and the published blocks j > bt+1 stream through an LDS double buffer; the tiles against block
@@ -896,23 +924,16 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   for (int g = 0; g < SG; ++g) acc[g] = 0.0;
   int cur = 0;
   TileRegs<TF> R;
-  // separator part (rows ≥ r; x_sep is final): 64-row chunks through LDS so that the global reads
-  // run down the columns, the next chunk in flight while this one is multiplied
-  for (int g = 0; g < ng; ++g) {
-    if (Fd.r < LD) fetch_tile<false>(R, F, LD, Fd.r, cg[g], min(NB, LD - Fd.r), wg[g]);
-    double a = 0.0;
-    for (int rs0 = Fd.r; rs0 < LD; rs0 += NB) {
-      const int h = min(NB, LD - rs0);
-      put_tile(S[cur], R);
-      if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
-      __syncthreads();
-      if (rs0 + NB < LD) fetch_tile<false>(R, F, LD, rs0 + NB, cg[g], min(NB, LD - rs0 - NB), wg[g]);
+  // separator part (rows ≥ r; x_sep is final): the chunk partials of k_big_sepgemv, in chunk order
+  if (q == 0) {
+    const int nch = (LD - Fd.r + NB - 1) / NB;
 #pragma unroll
-      for (int rr = q; rr < NB; rr += 4) a += S[cur][rr + col * LDT] * xs[rr];
-      __syncthreads();
-      cur ^= 1;
+    for (int g = 0; g < SG; ++g) {
+      if (g >= ng) break;
+      double a = 0.0;
+      for (int c = 0; c < nch; ++c) a += B.spart[((size_t)c * B.Tr + (bt - g)) * NB + col];
+      acc[g] += a;
     }
-    acc[g] += a;
   }
   // published blocks j > bt, in batches, every block in descending order whatever the batches
   // (so the sums, and the results, are reproducible): the tiles (k, bt−g) for k ≠ bt+1 stream
@@ -1000,6 +1021,11 @@ int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int
 int launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
                       int epoch, long long base, bool fp32) {
   const int grid = (B.Tr + SG - 1) / SG;
+  if (B.s > 0 && B.Tr > 0) {
+    const dim3 sg(B.Tr, (B.s + NB - 1) / NB);
+    if (fp32) k_big_sepgemv<float><<<sg, NT, 0, st>>>(B, F, f, I, kv);
+    else k_big_sepgemv<double><<<sg, NT, 0, st>>>(B, F, f, I, kv);
+  }
   if (fp32) k_big_bsolve<float><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
   else k_big_bsolve<double><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
   return grid;

@@ -723,6 +723,7 @@ void Solver::build_schedule() {
       bh->stage.alloc(sizeof(int) * std::max(1, Tr));
       HIPCHK(hipMemsetAsync(bh->stage.p, 0, bh->stage.bytes, st_));
       bh->sflag.alloc(sizeof(int) * std::max(1, T));
+      bh->spart.alloc(sizeof(double) * NB * std::max(1, Tr) * std::max(1, (Fd.s + NB - 1) / NB));
       HIPCHK(hipMemsetAsync(bh->sflag.p, 0, bh->sflag.bytes, st_));
       BigFront& B = bh->B;
       B.F = Fd.F;
@@ -744,6 +745,7 @@ void Solver::build_schedule() {
       B.stage = bh->stage.as<int>();
       B.trace = nullptr;
       B.sflag = bh->sflag.as<int>();
+      B.spart = bh->spart.as<double>();
       B.sticket = d_bigstick_.as<long long>() + big_.size();
       B.inst = Fd.inst;
       bh->front = f;
