@@ -14,6 +14,8 @@ namespace mb {
 
 static const double EPS = std::numeric_limits<double>::epsilon();
 static const double NaN = std::numeric_limits<double>::quiet_NaN();
+// mixed precision: a step below this with an FP32 factor sends the rest of the solve to FP64
+static const double MIXED_MIN_STEP = 0.05;
 static const double FORCING_FRAC = 0.1;  // solver/solver.jl:3
 
 #define HIPCHK(x)                                                                          \
@@ -1682,14 +1684,17 @@ int Solver::step() {
             row.rho = rho;
             if (ok) continue;
           }
-          // once the FP32 factor needs a second refinement pass, FP64 for the rest of the solve
-          if (fac_fp32_ && po.nir >= 2) mixed_off_ = true;
+          // once the FP32 factor needs a second refinement pass (predictor or corrector), FP64
+          // for the rest of the solve
+          if (fac_fp32_ && (po.nir >= 2 || co.nir >= 2)) mixed_off_ = true;
           if (po.status != KKT_SOLVED || co.status != KKT_SOLVED) {
             compute_stats();
             status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
           } else {
             double stp = maxsteps(NV_DP, NV_DD, set_.step_frac);
             row.step = stp;
+            // a collapsing step with an FP32 factor: its direction is not accurate enough here
+            if (fac_fp32_ && stp < MIXED_MIN_STEP) mixed_off_ = true;
             set_scalar(S_STEP, stp);
             axpby(0, NV_DP, NV_P, NV_P, KS(1.0, S_STEP), K(1.0));
             axpby(0, NV_DD, NV_D, NV_D, KS(1.0, S_STEP), K(1.0));
