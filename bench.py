@@ -309,6 +309,9 @@ def bench_serve(args, mh):
     st = server.stats()
     server.close()
     ok = sum(r.status == mh.IPMStatus.OPTIMAL for r, _ in out)
+    statuses = {}
+    for r, _ in out:
+        statuses[r.status.name] = statuses.get(r.status.name, 0) + 1
     fin = np.array([t["finished"] for _, t in out])
     lat_ms = (fin - np.array([t["submitted"] for _, t in out])) * 1e3
     elapsed = float(fin.max() - t0)
@@ -320,7 +323,7 @@ def bench_serve(args, mh):
                                f"log-uniform in [50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, "
                                f"{args.slots} slots",
                    "precision": "mixed" if args.precision else "fp64",
-                   "optimal": ok, "p50_ms": float(np.percentile(lat_ms, 50)),
+                   "optimal": ok, "statuses": statuses, "p50_ms": float(np.percentile(lat_ms, 50)),
                    "p99_ms": float(np.percentile(lat_ms, 99)), "peak_reserved_gb": st["peak_reserved_bytes"] / 2 ** 30},
     }
 
