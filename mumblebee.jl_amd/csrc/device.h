@@ -43,6 +43,7 @@ enum Scalar : int {
   S_R0,          // S_R0 .. S_R15: reduction outputs read back in one transfer
   S_R1, S_R2, S_R3, S_R4, S_R5, S_R6, S_R7,
   S_R8, S_R9, S_R10, S_R11, S_R12, S_R13, S_R14, S_R15,
+  S_ABORT,       // the persistent kernels' abort flag: an int in this slot's low word (nonzero: raised)
   NSCALAR
 };
 

@@ -15,8 +15,8 @@ struct BigFront {
   int T, Tr;            // tiles (res tiles first, then separator tiles from column r)
   const int* tb;        // T+1 tile boundaries
   int* state;           // T(T+1)/2 tile flags: the factorization epoch once the tile is final
-  int* ticket;          // [0] task ticket, [1] abort flag
-  int* abortf;          // = ticket + 1
+  int* ticket;          // task ticket
+  int* abortf;          // abort flag (an int in the instance's S_ABORT scalar)
   const int4* tasks;    // (type, i, j, kmax) in column order: 0 diagonal, 1 factor tile,
                         // 2 update-matrix tile, 3 sub-diagonal tile (staged for the diagonal)
   int ntask;

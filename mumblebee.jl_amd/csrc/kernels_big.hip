@@ -1004,7 +1004,7 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
 // ------------------------------------------------------------------------------------------
 
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
-  k_fill_int<<<1, 64, 0, st>>>(B.ticket, 2, 0);  // ticket and abort flag
+  k_fill_int<<<1, 64, 0, st>>>(B.ticket, 1, 0);  // the ticket (the abort flag stays raised once set)
   if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
   else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
 }

@@ -738,7 +738,8 @@ void Solver::build_schedule() {
       B.state = bh->state.as<int>();
       B.nstate = T * (T + 1) / 2;
       B.ticket = d_bigtick_.as<int>() + 2 * big_.size();
-      B.abortf = B.ticket + 1;
+      // the abort flag sits in the scalar block, so every scalar read-back also brings it home
+      B.abortf = reinterpret_cast<int*>(d_sc_.as<double>() + S_ABORT);
       B.tasks = bh->tasks.as<int4>();
       B.ntask = (int)tasks.size();
       B.Dinv = bh->dinv.as<double>();
@@ -907,14 +908,11 @@ void Solver::usolve(int kv) {
   }
 }
 
-// a persistent kernel that hit its wait limit raised its abort flag: fail loudly
+// a persistent kernel that hit its wait limit raised its abort flag: fail loudly. The flag is
+// part of the scalar block, so this reads the copy of the last sync_scalars, which every caller
+// has made after the kernels it checks.
 void Solver::check_abort() {
-  if (big_.empty()) return;
-  std::vector<int> t(2 * big_.size());
-  HIPCHK(hipMemcpyAsync(t.data(), d_bigtick_.p, t.size() * sizeof(int), hipMemcpyDeviceToHost, st_));
-  HIPCHK(hipStreamSynchronize(st_));
-  for (size_t b = 0; b < big_.size(); ++b)
-    if (t[2 * b + 1] != 0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
+  if (!big_.empty() && hsc_[S_ABORT] != 0.0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
 }
 
 void Solver::run_factor(bool fp32) {
