@@ -1301,11 +1301,9 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
 // ------------------------------------------------------------------------------------------
 
 int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& status) {
-  // cg.jl:69-167 with p = MV_CP, w = NV_CW
+  // cg.jl:69-167 with p = MV_CP, w = NV_CW. On entry hsc_[S_R8] holds ‖u‖² and hsc_[S_R9] its
+  // scaled stop norm (read back with the last Uzawa residual)
   int niter = 0;
-  dot(1, ku, ku, 0, S_R8, 0);
-  if (stops) dot(1, ku, MV_YSCL, 1, S_R9, 1);
-  sync_scalars();
   double nu2 = hsc_[S_R8], nu = std::sqrt(nu2), np2 = nu2;
   double chist0 = nu;
   std::vector<double>& ch = cghist_;
@@ -1366,35 +1364,44 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   const int irmax = set_.refine_max_iter, cgmax = set_.newton_max_iter;
   int nir = 0;
   std::vector<double> hist;
-  auto residual = [&]() {
+  // the norms of a residual: ‖KF‖ (and its scaled stop), and ‖KG‖² (and its scaled stop) for the
+  // dual CG that may follow, all in one read-back
+  auto norms = [&]() {
+    dot(0, NV_KF, NV_KF, 0, S_R10, 1);
+    if (stops) dot(0, NV_KF, NV_PSCL, 1, S_R11, 1);
+    dot(1, MV_KG, MV_KG, 0, S_R8, 0);
+    if (stops) dot(1, MV_KG, MV_YSCL, 1, S_R9, 1);
+    sync_scalars();
+  };
+  auto residual = [&](bool read) {
     // KF = f − H x + Bᵀ y ; KG = g − B x
     hmul(kx, NV_T0);
     spmv(1, ky, NV_T1, K(1.0), K(0.0));
     axpbypcz(0, kf, NV_T0, NV_T1, NV_KF, K(1.0), K(-1.0), K(1.0));
     axpby(1, kg, kg, MV_KG, K(1.0), K(0.0));
     spmv(0, kx, MV_KG, K(-1.0), K(1.0));
-    dot(0, NV_KF, NV_KF, 0, S_R10, 1);
-    if (stops) dot(0, NV_KF, NV_PSCL, 1, S_R11, 1);
-    sync_scalars();
+    if (read) norms();
   };
+  // with irmin ≥ 1 and irmax ≥ 1 the starting residual decides nothing (and its norm enters no
+  // later test), so it is not read back
+  const bool read0 = irmin == 0 || irmax == 0;
   if (warm) {
-    residual();
+    residual(read0);
   } else {
     axpby(0, kf, kf, NV_KF, K(1.0), K(0.0));
     axpby(1, kg, kg, MV_KG, K(1.0), K(0.0));
     axpby(0, kx, kx, kx, K(0.0), K(0.0));
     axpby(1, ky, ky, ky, K(0.0), K(0.0));
-    dot(0, NV_KF, NV_KF, 0, S_R10, 1);
-    if (stops) dot(0, NV_KF, NV_PSCL, 1, S_R11, 1);
-    sync_scalars();
+    if (read0) norms();
   }
   auto fstop = [&]() { return stops && hsc_[S_R11] / (1.0 + sf_) <= set_.feas_tol; };
-  double fres = hsc_[S_R10], fprv = fres;
+  double fres = read0 ? hsc_[S_R10] : NaN, fprv = fres;
   hist.push_back(fres);
-  int irstat;
-  if (nir >= irmin && (fres <= ftol || fstop())) irstat = KKT_SOLVED;
-  else if (nir >= irmax) irstat = KKT_IRMAX;
-  else irstat = KKT_CONTINUE;
+  int irstat = KKT_CONTINUE;
+  if (read0) {
+    if (nir >= irmin && (fres <= ftol || fstop())) irstat = KKT_SOLVED;
+    else if (nir >= irmax) irstat = KKT_IRMAX;
+  }
   while (irstat == KKT_CONTINUE) {
     nir += 1;
     spmv(1, MV_KG, NV_T0, K(1.0), K(0.0));                          // T0 = Bᵀ δg
@@ -1404,7 +1411,7 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
     spmv(0, NV_KX, MV_KG, K(-1.0), K(1.0));                         // δg −= B δx
     axpby(0, NV_KX, kx, kx, K(1.0), K(1.0));                        // x += δx
     axpby(1, MV_KG, ky, ky, KS(1.0, S_ALPHA), K(1.0));              // y += α δg
-    residual();
+    residual(true);
     fres = hsc_[S_R10];
     hist.push_back(fres);
     if (nir >= irmin && (fres <= ftol || fstop())) irstat = KKT_SOLVED;
