@@ -1130,9 +1130,14 @@ double Solver::cone_maxsteps(const double* dp, const double* dd, double step_fra
   return maxsteps(NV_DP, NV_DD, step_frac);
 }
 
-double Solver::maxsteps(int kdp, int kdd, double step_frac) {
+// the step into S_TAU on the device (no read-back)
+void Solver::launch_maxsteps(int kdp, int kdd, double step_frac) {
   launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
                       kdd, step_frac, d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
+}
+
+double Solver::maxsteps(int kdp, int kdd, double step_frac) {
+  launch_maxsteps(kdp, kdd, step_frac);
   sync_scalars();
   return hsc_[S_TAU];
 }
@@ -1646,7 +1651,8 @@ int Solver::step() {
           axpbypcz(0, NV_T0, NV_DF, NV_DDA, NV_DDA, K(-1.0), K(-1.0), K(1.0));
           launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DDA);
           // corrector (solver/ipm.jl:181-272)
-          double taua = maxsteps(NV_DPA, NV_DDA, 1.0);
+          // τa stays on the device: the dot reads it there, and one read-back brings μa home
+          launch_maxsteps(NV_DPA, NV_DDA, 1.0);
           dot(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA);
           sync_scalars();
           double smu = hsc_[S_TMP2];
@@ -1662,7 +1668,6 @@ int Solver::step() {
           } else {
             smu = std::min(std::max(mua * (mua / mu) * (mua / mu), 0.0), mu);
           }
-          (void)taua;
           set_scalar(S_SMU, smu);
           launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
           launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,

@@ -158,6 +158,7 @@ class Solver {
   bool factor(double delta, double& rho);
   void run_factor(bool fp32);
   double maxsteps(int kdp, int kdd, double step_frac);
+  void launch_maxsteps(int kdp, int kdd, double step_frac);
   void reinit(const double* p0, const double* d0, const double* y0);
   // KKT
  public:
