@@ -28,6 +28,8 @@ struct BigFront {
   unsigned long long* trace;  // optional: 4 timestamps per task (100 MHz wall clock), or null
   int* sflag;           // T solve flags (epoch-valued)
   double* spart;        // backward solve: L[sep, b]ᵀ x_sep by 64-row separator chunk, ⌈s/64⌉·Tr·64 values
+  double* Mf;           // Tr FP64 64×64 blocks M_b = L_bb⁻¹ L_{b,b−1} (forward-solve chain step)
+  double* Nb;           // Tr FP64 64×64 blocks N_b = L_{b+1,b} L_bb⁻¹ (backward-solve chain step)
   long long* sticket;   // 64-bit monotone ticket of the solve kernels
   int inst;
 };

@@ -684,12 +684,6 @@ __device__ __forceinline__ int take_published(const int* flags, int j, int step,
   return r;
 }
 
-// 64-row solution blocks solved in order by one solve workgroup. Two blocks per workgroup halve
-// the hand-offs between workgroups but double the factor tiles each workgroup streams, which
-// measured slower (2000×2000 front: backward solve 96 → 152 µs); one it is.
-constexpr int SG = 1;
-static_assert(SG == 1 || SG == 2, "the solve kernels index their per-block accumulators statically");
-
 // A 64×64 factor tile held for a matrix-vector product, 16 elements per thread. Row layout
 // (forward solve): thread (r, q) = (t & 63, t >> 6) holds element (r, q + 4i). Column layout
 // (backward solve): thread (c, q) holds element (q + 4i, c). Entries past the tile are zero.
@@ -697,8 +691,8 @@ struct GemvRegs {
   double v[16];
 };
 
-template <class TF>
-__device__ __forceinline__ void gemv_rows(GemvRegs& R, const TF* F, int LD, int r0, int c0, int h, int w) {
+template <class TS>
+__device__ __forceinline__ void gemv_rows(GemvRegs& R, const TS* F, int LD, int r0, int c0, int h, int w) {
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
@@ -708,8 +702,8 @@ __device__ __forceinline__ void gemv_rows(GemvRegs& R, const TF* F, int LD, int 
 }
 
 // column layout through an LDS tile S (the global reads stay coalesced); S is free again on return
-template <class TF>
-__device__ __forceinline__ void gemv_cols(GemvRegs& R, double* S, const TF* F, int LD, int r0, int c0, int h, int w) {
+template <class TS>
+__device__ __forceinline__ void gemv_cols(GemvRegs& R, double* S, const TS* F, int LD, int r0, int c0, int h, int w) {
   const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
   load_tile<false>(S, F, LD, r0, c0, h, w);
   __syncthreads();
@@ -727,20 +721,79 @@ __device__ __forceinline__ double gemv_dot(const GemvRegs& R, const double* x) {
   return a;
 }
 
-// forward. Ticket t < ⌈Tr/SG⌉: the factor blocks b0 = SG·t … b0+SG−1 in order. The workgroup
-// accumulates L[b, j] x_j for the published blocks j < b0 while the earlier groups solve; the
-// tiles against block b0−1 (the last to arrive) and within the group are staged in registers and
-// the inverted diagonal blocks in LDS before it waits, so each hand-off between workgroups is
-// followed only by products from on-chip operands. Then x_b = L_bb⁻¹ (v_b − Σ) for each block of
-// the group, and its blocks are published together. Later tickets: one separator tile each, whose
-// Σ goes into the front's update vector. TF: element type of the stored factor; the arithmetic
-// is FP64.
+// the 64×64 tile at (r0, c0) of F (h × w, zero padded) into LDS as FP64, transposed if TR
+template <bool TR, class TS>
+__device__ __forceinline__ void lds_tile64(double* S, const TS* F, int LD, int r0, int c0, int h, int w) {
+  for (int t = threadIdx.x; t < NB * NB; t += NT) {
+    const int i = t % NB, k = t / NB;
+    const double x = (i < h && k < w) ? (double)F[(size_t)(r0 + i) + (size_t)(c0 + k) * LD] : 0.0;
+    if (TR) S[k + i * LDT] = x;
+    else S[i + k * LDT] = x;
+  }
+}
+
+// After a factorization: for each factor block b, M_b = L_bb⁻¹ L_{b,b−1} (b ≥ 1) and
+// N_b = L_{b+1,b} L_bb⁻¹ (b+1 < Tr), FP64, 64×64 column-major. With them each step of the
+// solves' chains is one 64×64 product after the previous block arrives:
+// x_b = w_b − M_b x_{b−1} (forward), x_b = w_b − N_bᵀ x_{b+1} (backward), where w_b is formed
+// from everything else while the workgroup waits. One workgroup per block.
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* insts) {
+  __shared__ __align__(16) double SX[NB * LDT];
+  __shared__ __align__(16) double SY[NB * LDT];
+  const InstDev& I = insts[B.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int b = blockIdx.x;
+  if (b >= B.Tr) return;
+  const TF* F = factor_of<TF>(B);
+  const TF* Di = dinv_of<TF>(B) + (size_t)b * NB * NB;
+  const int LD = B.ld, c0 = B.tb[b], w = B.tb[b + 1] - c0;
+  if (b >= 1) {
+    // M_b = X_b L with L = L_{b,b−1}: mfma_acc forms A += X Yᵀ, so Y = Lᵀ
+    const int cp = B.tb[b - 1], wp = c0 - cp;
+    lds_tile64<false>(SX, Di, NB, 0, 0, NB, NB);
+    lds_tile64<true>(SY, F, LD, c0, cp, w, wp);
+    __syncthreads();
+    Acc<double> A;
+    acc_zero(A);
+    mfma_acc<false>(A, SX, SY, NB);
+    __syncthreads();
+    acc_to_lds(A, SX);
+    __syncthreads();
+    double* M = B.Mf + (size_t)b * NB * NB;
+    for (int t = threadIdx.x; t < NB * NB; t += NT) M[t] = SX[(t % NB) + (t / NB) * LDT];
+    __syncthreads();
+  }
+  if (b + 1 < B.Tr) {
+    // N_b = L' X_b with L' = L_{b+1,b}: Y = X_bᵀ
+    const int rn = B.tb[b + 1], hn = B.tb[b + 2] - rn;
+    lds_tile64<false>(SX, F, LD, rn, c0, hn, w);
+    lds_tile64<true>(SY, Di, NB, 0, 0, NB, NB);
+    __syncthreads();
+    Acc<double> A;
+    acc_zero(A);
+    mfma_acc<false>(A, SX, SY, NB);
+    __syncthreads();
+    acc_to_lds(A, SX);
+    __syncthreads();
+    double* N = B.Nb + (size_t)b * NB * NB;
+    for (int t = threadIdx.x; t < NB * NB; t += NT) N[t] = SX[(t % NB) + (t / NB) * LDT];
+  }
+}
+
+// forward: tile b (ticket order) accumulates L[b, j] x_j for the published j < b−1 while the
+// earlier tiles solve; the factor tiles stream through registers one block ahead of the solution,
+// and every block's contribution is summed in block order whatever the batches (reproducible
+// results). A factor tile then forms w_b = L_bb⁻¹ (v_b − Σ) before block b−1 arrives, and after
+// it x_b = w_b − M_b x_{b−1} (M_b staged in registers), and publishes. A separator tile adds its
+// Σ over all factor blocks into the front's update vector. TF: element type of the stored factor;
+// the arithmetic is FP64.
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
-  __shared__ double Sinv[SG][NB * LDT];
+  __shared__ double Sinv[NB * LDT];
   __shared__ double xs[KB * NB];
-  __shared__ double xo[SG][NB];
+  __shared__ double wv[NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -749,103 +802,81 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
   __syncthreads();
   if (I.sc[S_ACTIVE] == 0.0) return;
-  const int nd = (B.Tr + SG - 1) / SG;
-  const int t = (int)tk;
-  if (t < 0 || t >= nd + (B.T - B.Tr)) return;
+  const int b = (int)tk;
+  if (b < 0 || b >= B.T) return;
   double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
   const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
   const int row = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const bool diag = t < nd;
-  const int b0 = diag ? t * SG : B.Tr + (t - nd);
-  const int ng = diag ? min(SG, B.Tr - b0) : 1;
-  const int jmax = diag ? b0 : B.Tr;
-  int rg[SG], hg[SG];
-#pragma unroll
-  for (int g = 0; g < SG; ++g) {
-    rg[g] = g < ng ? B.tb[b0 + g] : 0;
-    hg[g] = g < ng ? B.tb[b0 + g + 1] - rg[g] : 0;
+  const bool diag = b < B.Tr;
+  const int jmax = diag ? b : B.Tr, jlast = jmax - 1;
+  const int r0 = B.tb[b], h = B.tb[b + 1] - r0;
+  // stage the operands of the last step before waiting: L_bb⁻¹ (LDS) and the tile against the
+  // last block to arrive (registers): M_b for a factor tile, L[b, Tr−1] for a separator tile
+  GemvRegs Rl;
+  if (diag) load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  if (jlast >= 0) {
+    if (diag) gemv_rows(Rl, B.Mf + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+    else gemv_rows(Rl, F, LD, r0, B.tb[jlast], h, B.tb[jmax] - B.tb[jlast]);
   }
-  // stage the operands of the last steps before waiting
-  GemvRegs Rin[SG], Rg;
-  if (diag)
-    for (int g = 0; g < ng; ++g) load_tile<false>(Sinv[g], dinv_of<TF>(B) + (size_t)(b0 + g) * NB * NB, NB, 0, 0, NB, NB);
-  const int cl = jmax > 0 ? B.tb[jmax - 1] : 0, wl = jmax > 0 ? B.tb[jmax] - cl : 0;
-#pragma unroll
-  for (int g = 0; g < SG; ++g)
-    if (g < ng && jmax > 0) gemv_rows(Rin[g], F, LD, rg[g], cl, hg[g], wl);
-  if (ng > 1) gemv_rows(Rg, F, LD, rg[1], rg[0], hg[1], hg[0]);
-  double acc[SG];
-#pragma unroll
-  for (int g = 0; g < SG; ++g) acc[g] = 0.0;
-  // the tiles of blocks j < jmax−1 stream through registers one block ahead of the solution,
-  // so each tile's loads overlap the wait for its block; every block's contribution is summed
-  // in block order, whatever the batches (reproducible results)
-  GemvRegs Rn[SG];
-  if (jmax > 1)
-#pragma unroll
-    for (int g = 0; g < SG; ++g)
-      if (g < ng) gemv_rows(Rn[g], F, LD, rg[g], B.tb[0], hg[g], B.tb[1] - B.tb[0]);
-  for (int j = 0; j < jmax;) {
-    const int n = take_published(B.sflag, j, 1, min(KB, jmax - j), epoch, B.abortf);
+  double acc = 0.0;
+  GemvRegs Rn;
+  if (jlast > 0) gemv_rows(Rn, F, LD, r0, B.tb[0], h, B.tb[1] - B.tb[0]);
+  for (int j = 0; j < jlast;) {
+    const int n = take_published(B.sflag, j, 1, min(KB, jlast - j), epoch, B.abortf);
     if (n == 0) return;
     const int c0 = B.tb[j], wtot = B.tb[j + n] - c0;
     for (int u = threadIdx.x; u < KB * NB; u += NT) xs[u] = u < wtot ? cload(&v[c0 + u]) : 0.0;
     __syncthreads();
     for (int k = j; k < j + n; ++k) {
-      const double* x = xs + (B.tb[k] - c0);
-      if (k == jmax - 1) {
-#pragma unroll
-        for (int g = 0; g < SG; ++g)
-          if (g < ng) acc[g] += gemv_dot(Rin[g], x);
-      } else {
-        GemvRegs Rc[SG];
-#pragma unroll
-        for (int g = 0; g < SG; ++g) Rc[g] = Rn[g];
-        if (k + 1 < jmax - 1)
-#pragma unroll
-          for (int g = 0; g < SG; ++g)
-            if (g < ng) gemv_rows(Rn[g], F, LD, rg[g], B.tb[k + 1], hg[g], B.tb[k + 2] - B.tb[k + 1]);
-#pragma unroll
-        for (int g = 0; g < SG; ++g)
-          if (g < ng) acc[g] += gemv_dot(Rc[g], x);
-      }
+      const GemvRegs Rc = Rn;
+      if (k + 1 < jlast) gemv_rows(Rn, F, LD, r0, B.tb[k + 1], h, B.tb[k + 2] - B.tb[k + 1]);
+      acc += gemv_dot(Rc, xs + (B.tb[k] - c0));
     }
     __syncthreads();
     j += n;
   }
-  if (!diag) {
-    part[q][row] = acc[0];
+  // the last block's x into xs[0, 64) (waits for its flag)
+  auto fetch_last = [&]() -> bool {
+    if (take_published(B.sflag, jlast, 1, 1, epoch, B.abortf) == 0) return false;
+    const int cl = B.tb[jlast], wl = B.tb[jmax] - cl;
+    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < wl ? cload(&v[cl + threadIdx.x]) : 0.0;
     __syncthreads();
-    if (threadIdx.x < hg[0])
-      Fd.uvec[rg[0] - Fd.r + row] += part[0][row] + part[1][row] + part[2][row] + part[3][row];
+    return true;
+  };
+  if (!diag) {
+    if (jlast >= 0) {
+      if (!fetch_last()) return;
+      acc += gemv_dot(Rl, xs);
+    }
+    part[q][row] = acc;
+    __syncthreads();
+    if (threadIdx.x < h) Fd.uvec[r0 - Fd.r + row] += part[0][row] + part[1][row] + part[2][row] + part[3][row];
     return;
   }
+  // w_b = L_bb⁻¹ (v_b − Σ_{j<b−1} L_bj x_j), before block b−1 arrives
+  part[q][row] = acc;
+  __syncthreads();
+  if (threadIdx.x < NB)
+    xs[row] = row < h ? v[r0 + row] - (part[0][row] + part[1][row] + part[2][row] + part[3][row]) : 0.0;
+  __syncthreads();
+  double y = 0.0;
 #pragma unroll
-  for (int g = 0; g < SG; ++g) {
-    if (g >= ng) break;
-    if (g == 1) acc[1] += gemv_dot(Rg, xo[0]);
-    part[q][row] = acc[g];
-    __syncthreads();
-    const double sum = part[0][row] + part[1][row] + part[2][row] + part[3][row];
-    if (threadIdx.x < NB) xs[row] = row < hg[g] ? v[rg[g] + row] - sum : 0.0;
-    __syncthreads();
-    double y = 0.0;
-#pragma unroll
-    for (int c = q; c < NB; c += 4) y += Sinv[g][row + c * LDT] * xs[c];
-    part[q][row] = y;
-    __syncthreads();
-    if (threadIdx.x < NB) {
-      const double x = part[0][row] + part[1][row] + part[2][row] + part[3][row];
-      xo[g][row] = row < hg[g] ? x : 0.0;
-      if (row < hg[g]) cstore(&v[rg[g] + row], x);
-    }
+  for (int c = q; c < NB; c += 4) y += Sinv[row + c * LDT] * xs[c];
+  part[q][row] = y;
+  __syncthreads();
+  if (threadIdx.x < NB) wv[row] = part[0][row] + part[1][row] + part[2][row] + part[3][row];
+  __syncthreads();
+  if (jlast >= 0) {
+    if (!fetch_last()) return;
+    part[q][row] = gemv_dot(Rl, xs);
     __syncthreads();
   }
-  // every thread's stores have completed, then the group's blocks are published
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x < ng) st_flag(&B.sflag[b0 + threadIdx.x], epoch);
+  if (threadIdx.x < NB && row < h) {
+    const double x = jlast >= 0 ? wv[row] - (part[0][row] + part[1][row] + part[2][row] + part[3][row]) : wv[row];
+    cstore(&v[r0 + row], x);
+  }
+  publish(&B.sflag[b], epoch);
 }
 
 // L[sep, b]ᵀ x_sep for one factor column block b and one 64-row chunk of the separator rows
@@ -876,18 +907,17 @@ __global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc*
     B.spart[((size_t)blockIdx.y * B.Tr + b) * NB + col] = (part[0][col] + part[1][col]) + (part[2][col] + part[3][col]);
 }
 
-// backward. Ticket t: the factor blocks bt = Tr−1−SG·t, bt−1, … (descending), each a column
-// block whose x_b = L_bbᵀ⁻¹ (v_b − L[sep, b]ᵀ x_sep − Σ_{j>b} L[j, b]ᵀ x_j). The separator part
-// and the published blocks j > bt+1 stream through an LDS double buffer; the tiles against block
-// bt+1 (the last to arrive) and within the group are staged in registers and the inverted
-// diagonal blocks in LDS before the workgroup waits.
+// backward: column block b = Tr−1−ticket accumulates L[sep, b]ᵀ x_sep (the chunk partials of
+// k_big_sepgemv, in chunk order) and L[j, b]ᵀ x_j for the published j > b+1 (in descending order
+// whatever the batches, through an LDS double buffer), forms w_b = L_bbᵀ⁻¹ (v_b − Σ) before block
+// b+1 arrives, and after it x_b = w_b − N_bᵀ x_{b+1} (N_b staged in registers).
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
   __shared__ double S[2][NB * LDT];  // double buffer: the next tile loads while this one is used
-  __shared__ double Sinv[SG][NB * LDT];
+  __shared__ double Sinv[NB * LDT];
   __shared__ double xs[KB * NB];
-  __shared__ double xo[SG][NB];
+  __shared__ double wv[NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -895,110 +925,73 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
   __syncthreads();
   if (I.sc[S_ACTIVE] == 0.0) return;
-  const int nd = (B.Tr + SG - 1) / SG;
-  const int t = (int)tk;
-  if (t < 0 || t >= nd) return;
-  const int bt = B.Tr - 1 - t * SG;
-  const int ng = min(SG, bt + 1);
-  double* vall = I.nv + (size_t)kv * I.n;
-  double* v = vall + Fd.res0;
+  const int tkt = (int)tk;
+  if (tkt < 0 || tkt >= B.Tr) return;
+  const int b = B.Tr - 1 - tkt;
+  double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
   const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
+  const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
-  int cg[SG], wg[SG];
-#pragma unroll
-  for (int g = 0; g < SG; ++g) {
-    cg[g] = g < ng ? B.tb[bt - g] : 0;
-    wg[g] = g < ng ? B.tb[bt - g + 1] - cg[g] : 0;
-  }
-  for (int g = 0; g < ng; ++g) load_tile<false>(Sinv[g], dinv_of<TF>(B) + (size_t)(bt - g) * NB * NB, NB, 0, 0, NB, NB);
-  GemvRegs Rin[SG], Rg;
-  const bool has_in = bt + 1 < B.Tr;
-  const int rin = has_in ? B.tb[bt + 1] : 0, hin = has_in ? B.tb[bt + 2] - rin : 0;
-#pragma unroll
-  for (int g = 0; g < SG; ++g)
-    if (g < ng && has_in) gemv_cols(Rin[g], S[0], F, LD, rin, cg[g], hin, wg[g]);
-  if (ng > 1) gemv_cols(Rg, S[0], F, LD, cg[0], cg[1], wg[0], wg[1]);
-  double acc[SG];
-#pragma unroll
-  for (int g = 0; g < SG; ++g) acc[g] = 0.0;
-  int cur = 0;
-  TileRegs<TF> R;
-  // separator part (rows ≥ r; x_sep is final): the chunk partials of k_big_sepgemv, in chunk order
+  const bool has_next = b + 1 < B.Tr;
+  load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  GemvRegs Rl;
+  if (has_next) gemv_cols(Rl, S[0], B.Nb + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  double acc = 0.0;
   if (q == 0) {
     const int nch = (LD - Fd.r + NB - 1) / NB;
-#pragma unroll
-    for (int g = 0; g < SG; ++g) {
-      if (g >= ng) break;
-      double a = 0.0;
-      for (int c = 0; c < nch; ++c) a += B.spart[((size_t)c * B.Tr + (bt - g)) * NB + col];
-      acc[g] += a;
-    }
+    for (int c = 0; c < nch; ++c) acc += B.spart[((size_t)c * B.Tr + b) * NB + col];
   }
-  // published blocks j > bt, in batches, every block in descending order whatever the batches
-  // (so the sums, and the results, are reproducible): the tiles (k, bt−g) for k ≠ bt+1 stream
-  // through the double buffer, the next one in flight while this one is multiplied; block bt+1
-  // (staged) comes last
-  for (int j = B.Tr - 1; j > bt;) {
-    const int n = take_published(B.sflag, j, -1, min(KB, j - bt), epoch, B.abortf);
+  int cur = 0;
+  TileRegs<TF> R;
+  for (int j = B.Tr - 1; j > b + 1;) {
+    const int n = take_published(B.sflag, j, -1, min(KB, j - b - 1), epoch, B.abortf);
     if (n == 0) return;
     const int klo = j - n + 1, rlo = B.tb[klo], rhi = B.tb[j + 1];
     for (int u = threadIdx.x; u < KB * NB; u += NT) xs[u] = u < rhi - rlo ? cload(&v[rlo + u]) : 0.0;
-    // streamed tiles: index s ↦ (k, g) with k = j − s / ng down to kstop (bt+1 excluded), g = s % ng
-    const int kstop = klo == bt + 1 ? klo + 1 : klo;
-    const int nstream = (j - kstop + 1) * ng;
-    if (nstream > 0) fetch_tile<false>(R, F, LD, B.tb[j], cg[0], B.tb[j + 1] - B.tb[j], wg[0]);
+    fetch_tile<false>(R, F, LD, B.tb[j], c0, B.tb[j + 1] - B.tb[j], w);
     __syncthreads();
-    for (int sidx = 0; sidx < nstream; ++sidx) {
-      const int k = j - sidx / ng, g = sidx % ng;
+    for (int k = j; k >= klo; --k) {
       put_tile(S[cur], R);
       __syncthreads();
       const double* T = S[cur];
       cur ^= 1;
-      if (sidx + 1 < nstream) {
-        const int k2 = j - (sidx + 1) / ng, g2 = (sidx + 1) % ng;
-        fetch_tile<false>(R, F, LD, B.tb[k2], cg[g2], B.tb[k2 + 1] - B.tb[k2], wg[g2]);
-      }
+      if (k > klo) fetch_tile<false>(R, F, LD, B.tb[k - 1], c0, B.tb[k] - B.tb[k - 1], w);
       const double* x = xs + (B.tb[k] - rlo);
       double a = 0.0;
 #pragma unroll
       for (int rr = q; rr < NB; rr += 4) a += T[rr + col * LDT] * x[rr];
-      if (g == 0) acc[0] += a;
-      else acc[SG - 1] += a;
-    }
-    if (klo == bt + 1) {
-#pragma unroll
-      for (int g = 0; g < SG; ++g)
-        if (g < ng) acc[g] += gemv_dot(Rin[g], xs);
+      acc += a;
     }
     __syncthreads();
     j -= n;
   }
+  // w_b = L_bbᵀ⁻¹ (v_b − Σ), before block b+1 arrives
+  part[q][col] = acc;
+  __syncthreads();
+  if (threadIdx.x < NB)
+    xs[col] = col < w ? v[c0 + col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : 0.0;
+  __syncthreads();
+  double y = 0.0;
 #pragma unroll
-  for (int g = 0; g < SG; ++g) {
-    if (g >= ng) break;
-    if (g == 1) acc[1] += gemv_dot(Rg, xo[0]);
-    part[q][col] = acc[g];
+  for (int rr = q; rr < NB; rr += 4) y += Sinv[rr + col * LDT] * xs[rr];
+  part[q][col] = y;
+  __syncthreads();
+  if (threadIdx.x < NB) wv[col] = part[0][col] + part[1][col] + part[2][col] + part[3][col];
+  __syncthreads();
+  if (has_next) {
+    if (take_published(B.sflag, b + 1, -1, 1, epoch, B.abortf) == 0) return;
+    const int rn = B.tb[b + 1], hn = B.tb[b + 2] - rn;
+    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < hn ? cload(&v[rn + threadIdx.x]) : 0.0;
     __syncthreads();
-    if (threadIdx.x < NB)
-      xs[col] = col < wg[g] ? v[cg[g] + col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : 0.0;
-    __syncthreads();
-    // x_b[c] = Σ_r (L⁻¹)[r][c] t[r]
-    double y = 0.0;
-#pragma unroll
-    for (int rr = q; rr < NB; rr += 4) y += Sinv[g][rr + col * LDT] * xs[rr];
-    part[q][col] = y;
-    __syncthreads();
-    if (threadIdx.x < NB) {
-      const double x = part[0][col] + part[1][col] + part[2][col] + part[3][col];
-      xo[g][col] = col < wg[g] ? x : 0.0;
-      if (col < wg[g]) cstore(&v[cg[g] + col], x);
-    }
+    part[q][col] = gemv_dot(Rl, xs);
     __syncthreads();
   }
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x < ng) st_flag(&B.sflag[bt - threadIdx.x], epoch);
+  if (threadIdx.x < NB && col < w) {
+    const double x = has_next ? wv[col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : wv[col];
+    cstore(&v[c0 + col], x);
+  }
+  publish(&B.sflag[b], epoch);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1007,11 +1000,15 @@ void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int 
   k_fill_int<<<1, 64, 0, st>>>(B.ticket, 1, 0);  // the ticket (the abort flag stays raised once set)
   if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
   else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
+  if (B.Tr > 1) {
+    if (fp32) k_big_mtiles<float><<<B.Tr, NT, 0, st>>>(B, I);
+    else k_big_mtiles<double><<<B.Tr, NT, 0, st>>>(B, I);
+  }
 }
 
 int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,
                       const InstDev* I, int kv, int epoch, long long base, bool fp32) {
-  const int grid = (B.Tr + SG - 1) / SG + (B.T - B.Tr);
+  const int grid = B.T;
   k_big_gather<<<1, 256, 0, st>>>(F, f, childlist, I, kv);
   if (fp32) k_big_fsolve<float><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
   else k_big_fsolve<double><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
@@ -1020,7 +1017,7 @@ int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int
 
 int launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
                       int epoch, long long base, bool fp32) {
-  const int grid = (B.Tr + SG - 1) / SG;
+  const int grid = B.Tr;
   if (B.s > 0 && B.Tr > 0) {
     const dim3 sg(B.Tr, (B.s + NB - 1) / NB);
     if (fp32) k_big_sepgemv<float><<<sg, NT, 0, st>>>(B, F, f, I, kv);

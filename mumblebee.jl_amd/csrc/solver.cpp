@@ -726,6 +726,8 @@ void Solver::build_schedule() {
       HIPCHK(hipMemsetAsync(bh->stage.p, 0, bh->stage.bytes, st_));
       bh->sflag.alloc(sizeof(int) * std::max(1, T));
       bh->spart.alloc(sizeof(double) * NB * std::max(1, Tr) * std::max(1, (Fd.s + NB - 1) / NB));
+      bh->mtile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
+      bh->ntile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       HIPCHK(hipMemsetAsync(bh->sflag.p, 0, bh->sflag.bytes, st_));
       BigFront& B = bh->B;
       B.F = Fd.F;
@@ -749,6 +751,8 @@ void Solver::build_schedule() {
       B.trace = nullptr;
       B.sflag = bh->sflag.as<int>();
       B.spart = bh->spart.as<double>();
+      B.Mf = bh->mtile.as<double>();
+      B.Nb = bh->ntile.as<double>();
       B.sticket = d_bigstick_.as<long long>() + big_.size();
       B.inst = Fd.inst;
       bh->front = f;

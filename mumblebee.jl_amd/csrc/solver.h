@@ -244,7 +244,7 @@ class Solver {
     BigFront B;
     int front;
     long long sbase = 0;
-    DevBuf tb, state, tasks, sflag, spart, dinv, stage, trace, f32, dinv32;
+    DevBuf tb, state, tasks, sflag, spart, mtile, ntile, dinv, stage, trace, f32, dinv32;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
   DevBuf d_bigtick_, d_bigstick_;
