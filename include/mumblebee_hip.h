@@ -135,9 +135,11 @@ int mb_solver_reinit(mb_solver* s, const double* p0, const double* d0, const dou
 int mb_solver_set_profile(mb_solver* s, int32_t mask);
 int mb_solver_profile(mb_solver* s, double* out, int32_t len);
 
-/* Task timeline of the last factorization of large front `front` (in schedule order): four
- * 100 MHz wall-clock stamps per task (claimed, updates done, last input ready, published).
- * Copies at most len values; returns 4 × the task count (0 if there is no such front).
+/* Task timeline of the last factorization of large front `front` (in schedule order): eight
+ * 100 MHz wall-clock stamps per task (claimed, updates done, last input ready, done; a diagonal
+ * task also: factor panels done, diagonal-block inverses done, inverse complete, inverse
+ * published; 0 where a task has no such stamp). Copies at most len values; returns 8 × the task
+ * count (0 if there is no such front).
  * Diagnostic only; no counterpart in the reference. */
 int mb_solver_dag_trace(mb_solver* s, int32_t front, int64_t* out, int32_t len);
 

@@ -54,6 +54,30 @@ void launch_vmul(hipStream_t, const InstDev*, const Tile*, int nt, int space, in
 // counter: ninst ints, zero between launches (the kernel leaves them at zero)
 void launch_dot(hipStream_t, const InstDev*, const Tile*, int nt, const int* tile_ofs, int ninst, int space,
                 int kx, int ky, int mode, int kw0, int kw1, double* partial, int* counter, int slot, int op);
+// A batch of dot products of one instance in one launch, one workgroup per dot over the whole
+// vector (for vectors of at most DOTS_LEN_MAX entries): sc[slot] = op(Σ term), the term as in
+// launch_dot's mode, op 1 = square root.
+struct DotSpec {
+  int space, kx, ky, mode, kw0, kw1, slot, op;
+};
+constexpr int DOTS_MAX = 8;
+constexpr int DOTS_LEN_MAX = 1 << 14;
+struct DotBatch {
+  DotSpec s[DOTS_MAX];
+  int n = 0;
+  DotBatch& add(int space, int kx, int ky, int mode, int slot, int op, int kw0 = 0, int kw1 = 0) {
+    s[n++] = DotSpec{space, kx, ky, mode, kw0, kw1, slot, op};
+    return *this;
+  }
+};
+void launch_dots(hipStream_t, const InstDev*, int inst, const DotBatch&);
+// z = a x + b y + c w on the n-space and another on the m-space, in one launch
+struct Lin {
+  int kx, ky, kw, kz;
+  Coef a, b, c;
+};
+void launch_lin2(hipStream_t, const InstDev*, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt, Lin on,
+                 Lin om);
 void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
 void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky, Coef a);
 // psd / dense: lists of the PSD cones and of the cones with dense scaling blocks (SOC, PSD)

@@ -179,6 +179,61 @@ __global__ void k_dot(const InstDev* insts, const Tile* tiles, const int* tile_o
   }
 }
 
+// The dot products of a batch, one workgroup each over the whole vector, summed in a fixed order
+// (thread-strided, then the wave tree, then the waves in order).
+constexpr int DOTS_THREADS = 1024;
+__global__ void __launch_bounds__(DOTS_THREADS) k_dots(const InstDev* insts, int inst, DotBatch B) {
+  __shared__ double red[DOTS_THREADS / 64];
+  const InstDev& I = insts[inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const DotSpec d = B.s[blockIdx.x];
+  const int len = spacelen(I, d.space);
+  const double* x = vecof(I, d.space, d.kx);
+  const double* y = vecof(I, d.space, d.ky);
+  double v = 0.0;
+  if (d.mode == 0) {
+    for (int i = threadIdx.x; i < len; i += DOTS_THREADS) v += x[i] * y[i];
+  } else if (d.mode == 1) {
+    for (int i = threadIdx.x; i < len; i += DOTS_THREADS) {
+      const double q = x[i] / y[i];
+      v += q * q;
+    }
+  } else if (d.mode == 2) {
+    for (int i = threadIdx.x; i < len; i += DOTS_THREADS) v += x[i];
+  } else {
+    const double tau = I.sc[S_TAU];
+    const double* dx = vecof(I, d.space, d.kw0);
+    const double* dy = vecof(I, d.space, d.kw1);
+    for (int i = threadIdx.x; i < len; i += DOTS_THREADS) v += (x[i] + tau * dx[i]) * (y[i] + tau * dy[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < DOTS_THREADS / 64; ++k) s += red[k];
+    I.sc[d.slot] = d.op == 1 ? sqrt(s) : s;
+  }
+}
+
+// z = a x + b y + c w, blocks [0, nnt) on the n-space tiles with `on`, the rest on the m-space
+// tiles with `om`
+__global__ void k_lin2(const InstDev* insts, const Tile* ntiles, int nnt, const Tile* mtiles, Lin on, Lin om) {
+  const bool isn = (int)blockIdx.x < nnt;
+  const Tile t = isn ? ntiles[blockIdx.x] : mtiles[blockIdx.x - nnt];
+  const InstDev& I = insts[t.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int space = isn ? 0 : 1;
+  const int i = t.start + threadIdx.x;
+  if (i >= spacelen(I, space)) return;
+  const Lin L = isn ? on : om;
+  const double av = coefval(L.a, I.sc), bv = coefval(L.b, I.sc), cv = coefval(L.c, I.sc);
+  double acc = av * vecof(I, space, L.kx)[i];
+  if (bv != 0.0) acc += bv * vecof(I, space, L.ky)[i];
+  if (cv != 0.0) acc += cv * vecof(I, space, L.kw)[i];
+  vecof(I, space, L.kz)[i] = acc;
+}
+
 // sc[slot] = op(Σ partial over the instance's tiles), one block per instance, fixed order
 __global__ void k_reduce_slot(const InstDev* insts, const int* tile_ofs, const double* partial, int slot,
                               int op) {
@@ -619,9 +674,13 @@ __global__ void k_zerofree(const InstDev* insts, const ConeDesc* cones, int ncon
 // assembly: F ← 0; F[pos] = L + δ Q on the union pattern; F[diag blocks] += δ D
 // ------------------------------------------------------------------------------------------
 
+// x[0, n) = v with 16-byte stores, grid-strided (x 16-byte aligned)
 __global__ void k_fill(double* x, size_t n, double v) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = v;
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const size_t np = n / 2, stride = (size_t)gridDim.x * blockDim.x;
+  d2* x2 = reinterpret_cast<d2*>(x);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += stride) x2[i] = d2{v, v};
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) x[n - 1] = v;
 }
 
 __global__ void k_assemble_union(const InstDev* insts, int inst, const int64_t* pos, const double* lval,
@@ -1074,6 +1133,13 @@ void launch_dot(hipStream_t st, const InstDev* I, const Tile* T, int nt, const i
   if (nt) k_dot_partial<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, mode, kw0, kw1, partial);
   k_reduce_slot<<<ninst, TILE, 0, st>>>(I, tile_ofs, partial, slot, op);
 }
+void launch_dots(hipStream_t st, const InstDev* I, int inst, const DotBatch& B) {
+  if (B.n > 0) k_dots<<<B.n, DOTS_THREADS, 0, st>>>(I, inst, B);
+}
+void launch_lin2(hipStream_t st, const InstDev* I, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt, Lin on,
+                 Lin om) {
+  if (nnt + nmt) k_lin2<<<nnt + nmt, TILE, 0, st>>>(I, ntiles, nnt, mtiles, on, om);
+}
 void launch_spmv(hipStream_t st, const InstDev* I, const Tile* T, int nt, int which, int kx, int ky, Coef a,
                  Coef b) {
   if (nt) k_spmv<<<nt, TILE, 0, st>>>(I, T, which, kx, ky, a, b);
@@ -1111,7 +1177,7 @@ void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc
   if (nc) k_zerofree<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx);
 }
 void launch_fill(hipStream_t st, double* x, size_t n, double v) {
-  if (n) k_fill<<<cdiv((long)n, TILE), TILE, 0, st>>>(x, n, v);
+  if (n) k_fill<<<std::min(cdiv((long)(n / 2 + 1), TILE), 2048), TILE, 0, st>>>(x, n, v);
 }
 void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent) {

@@ -25,11 +25,11 @@ struct BigFront {
   float* F32;           // mixed precision: the FP32 factor (ld × r, same indexing as F), or null
   float* Dinv32;        // mixed precision: FP32 inverted diagonal blocks, or null
   int* stage;           // Tr flags: sub-diagonal tile (j+1, j) accumulated (epoch-valued)
-  unsigned long long* trace;  // optional: 4 timestamps per task (100 MHz wall clock), or null
-  int* sflag;           // T solve flags (epoch-valued)
+  unsigned long long* trace;  // optional: 8 timestamps per task (100 MHz wall clock), or null
+  unsigned long long* sll;  // solve hand-off: T blocks × 64 entries × 2 (value half, epoch) words
   double* spart;        // backward solve: L[sep, b]ᵀ x_sep by 64-row separator chunk, ⌈s/64⌉·Tr·64 values
   double* Mf;           // Tr FP64 64×64 blocks M_b = L_bb⁻¹ L_{b,b−1} (forward-solve chain step)
-  double* Nb;           // Tr FP64 64×64 blocks N_b = L_{b+1,b} L_bb⁻¹ (backward-solve chain step)
+  double* Nt;           // Tr FP64 64×64 blocks N_bᵀ, N_b = L_{b+1,b} L_bb⁻¹ (backward-solve chain step)
   long long* sticket;   // 64-bit monotone ticket of the solve kernels
   int inst;
 };

@@ -86,9 +86,13 @@ __device__ __forceinline__ void publish(int* p, int v) {
 
 __device__ __forceinline__ int tri_index(int i, int j) { return i * (i + 1) / 2 + j; }
 
-// task timeline (optional): claim, updates done, inputs of the last step ready, published
+// task timeline (optional), TRACE_SLOTS stamps per task: 0 claimed, 1 updates done, 2 inputs of
+// the last step ready, 3 done; a diagonal task also stamps 4 factor panels done, 5 diagonal-block
+// inverses done, 6 inverse complete, 7 inverse published
+constexpr int TRACE_SLOTS = 8;
 __device__ __forceinline__ void stamp(unsigned long long* trace, int t, int slot) {
-  if (trace != nullptr && threadIdx.x == 0) trace[4 * (size_t)t + slot] = (unsigned long long)wall_clock64();
+  if (trace != nullptr && threadIdx.x == 0)
+    trace[TRACE_SLOTS * (size_t)t + slot] = (unsigned long long)wall_clock64();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -281,16 +285,15 @@ __device__ __forceinline__ float readlane(float v, int lane) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
-// 1/√d to full precision: the hardware estimate refined by Newton steps (two for FP64, one for
-// FP32)
+// 1/√d to full precision from the hardware estimate y (relative error about 2^-23). FP64: with
+// e = 1 − d y², 1/√d = y (1 − e)^(−1/2) = y (1 + e/2 + 3e²/8 + O(e³)), one correction whose
+// truncation error (about 2^-66) is below the rounding, four dependent operations instead of the
+// six of two Newton steps (the factorization's pivot chain runs through it). FP32: one Newton step.
 __device__ __forceinline__ double rsqrt_nr(double d) {
-  double y = __builtin_amdgcn_rsq(d);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    double e = __builtin_fma(-d * y, y, 1.0);  // 1 − d y²
-    y = __builtin_fma(0.5 * y, e, y);
-  }
-  return y;
+  const double y = __builtin_amdgcn_rsq(d);
+  const double e = __builtin_fma(-d * y, y, 1.0);  // 1 − d y²
+  const double p = __builtin_fma(0.375, e, 0.5);   // 1/2 + 3e/8
+  return __builtin_fma(y * e, p, y);
 }
 __device__ __forceinline__ float rsqrt_nr(float d) {
   float y = __builtin_amdgcn_rsqf(d);
@@ -406,7 +409,7 @@ __device__ __forceinline__ void inv_level(const T* S, T* X, int b) {
 // LDS scratch, best below 64 KiB of LDS so that the panel's broadcast reads take immediate
 // offsets. Returns false on a non-positive pivot.
 template <class T>
-__device__ __forceinline__ bool chol_inv64(T* S, T* X, T* W, int h) {
+__device__ __forceinline__ bool chol_inv64(T* S, T* X, T* W, int h, unsigned long long* trace, int task) {
   __shared__ int okflag;
   T* rv = W;
   T* colj = W + NB;
@@ -436,6 +439,7 @@ __device__ __forceinline__ bool chol_inv64(T* S, T* X, T* W, int h) {
       __syncthreads();
     }
   }
+  stamp(trace, task, 4);
   {
     // wave w inverts diagonal block w: lane c < 16 forms column c by forward substitution
     const int o = 16 * w, c = lane & 15;
@@ -454,6 +458,7 @@ __device__ __forceinline__ bool chol_inv64(T* S, T* X, T* W, int h) {
     }
   }
   __syncthreads();
+  stamp(trace, task, 5);
   if (w <= 2) inv_level<1>(S, X, w);
   if (w <= 1) inv_level<2>(S, X, w);
   if (w == 0) inv_level<3>(S, X, w);
@@ -582,10 +587,12 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
       // factor and invert; L_jj⁻¹ → Dinv and publish, then L_jj → F (read only by later launches)
       acc_to_lds(C, SA);
       __syncthreads();
-      bool ok = chol_inv64(SA, SC, SB, hj);  // SB is free here and lies below 64 KiB
+      bool ok = chol_inv64(SA, SC, SB, hj, B.trace, t);  // SB is free here and lies below 64 KiB
+      stamp(B.trace, t, 6);
       if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
       store_tile(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
       publish(&B.state[tri_index(j, j)], epoch);
+      stamp(B.trace, t, 7);
       store_tile(Lf, LD, r0j, r0j, hj, hj, SA, true);
     } else if (type == 3) {
       // sub-diagonal tile: hand the accumulated tile to the next diagonal task
@@ -651,37 +658,83 @@ __global__ void k_big_gather(const FrontDesc* fronts, int f, const int* childlis
   }
 }
 
-constexpr int KB = 8;  // solution blocks a solve workgroup takes at once
 
-// The solve chains: thread 0 waits (bounded) until block j is published; lanes 0..n−1 of wave 0
-// read the flags of blocks j, j+step, …, and the count of consecutive published blocks from j
-// is returned to every thread (0 on abort). Taking every block that is already there at once
-// spreads a workgroup's per-block overhead (flag and x round trips, barriers) over the batch, so
-// a workgroup that lags the chain catches up instead of pacing it.
-__device__ __forceinline__ int take_published(const int* flags, int j, int step, int n, int epoch, int* abortf) {
-  __shared__ int cnt;
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    bool ok = lane < n && ld_flag(&flags[j + step * lane]) >= epoch;
-    if (lane == 0 && !ok) {
-      long spins = 0;
-      while (!ok) {
-        if (ld_flag(abortf) != 0) break;
-        if (++spins > SPIN_LIMIT) {
-          st_flag(abortf, 1);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-        ok = ld_flag(&flags[j]) >= epoch;
-      }
+// Solution blocks pass between the solve workgroups as (value, epoch) words: entry r of block b
+// is two 64-bit words, each one 32-bit half of x_b[r] beside the launch's epoch, stored and loaded
+// whole (single-copy atomic at device scope). A reader that sees the epoch in both words of an
+// entry holds its value: no flag, and no wait for the producer's stores to complete. Entries past
+// a block's order carry zeros. Every launch has a new epoch, so older contents never match.
+__device__ __forceinline__ void ll_put(unsigned long long* L, int blk, int r, double x, int epoch) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned long long e = (unsigned long long)(unsigned)epoch << 32;
+  unsigned long long* p = L + ((size_t)blk * NB + r) * 2;
+  __hip_atomic_store(p, (u & 0xffffffffull) | e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, (u >> 32) | e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one wave reads block blk (lane r: entry r); true (wave-uniform) when every entry is complete
+__device__ __forceinline__ bool ll_get(const unsigned long long* L, int blk, int epoch, double& x) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long* p = const_cast<unsigned long long*>(L) + ((size_t)blk * NB + lane) * 2;
+  const unsigned long long lo = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long hi = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const bool ok = (unsigned)(lo >> 32) == (unsigned)epoch && (unsigned)(hi >> 32) == (unsigned)epoch;
+  x = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+  return __ballot(ok) == ~0ull;
+}
+
+// one wave waits for block blk; false on abort or at the spin limit (which raises the abort flag)
+__device__ __forceinline__ bool ll_wait(const unsigned long long* L, int blk, int epoch, int* abortf, double& x) {
+  long spins = 0;
+  for (;;) {
+    if (ll_get(L, blk, epoch, x)) return true;
+    if (ld_flag(abortf) != 0) return false;
+    if (++spins > SPIN_LIMIT) {
+      if ((threadIdx.x & 63) == 0) st_flag(abortf, 1);
+      return false;
     }
-    const unsigned long long m = __ballot(ok);
-    if (lane == 0) cnt = (m & 1ull) ? __builtin_ctzll(~m) : 0;  // leading published blocks
+    __builtin_amdgcn_s_sleep(1);
   }
+}
+
+// The workgroup takes blocks j, j+step, … (at most n ≤ 4): wave w reads block j + w·step into
+// xs[w·64, w·64+64), wave 0 waiting for its block. Returns how many leading blocks are complete
+// (≥ 1), or 0 on abort.
+__device__ __forceinline__ int ll_take(const unsigned long long* L, int j, int step, int n, int epoch, int* abortf,
+                                       double* xs) {
+  __shared__ int okw[NT / 64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  bool ok = false;
+  double x = 0.0;
+  if (w == 0) ok = ll_wait(L, j, epoch, abortf, x);
+  else if (w < n) ok = ll_get(L, j + step * w, epoch, x);
+  if (ok) xs[w * NB + lane] = x;
+  if (lane == 0) okw[w] = ok;
   __syncthreads();
-  const int r = cnt;
-  __syncthreads();  // nobody may overwrite cnt before every thread has read it
-  return r;
+  int c = 0;
+  while (c < n && okw[c]) ++c;
+  __syncthreads();  // okw is rewritten by the next call
+  return c;
+}
+
+// The last step of a chain, by wave 0 alone: lane r holds w[r] and row r of the 64×64 step tile
+// S (column-major FP64); it waits for block blk, then x[r] = w[r] − Σ_c S[r][c] x_blk[c] (four
+// partial sums in a fixed order). xb: 64 doubles of LDS for the broadcast.
+__device__ __forceinline__ bool chain_step(const double* S, const unsigned long long* L, int blk, int epoch,
+                                           int* abortf, double* xb, double& x) {
+  const int lane = threadIdx.x & 63;
+  double srow[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) srow[c] = S[lane + c * NB];
+  double xp;
+  if (!ll_wait(L, blk, epoch, abortf, xp)) return false;
+  xb[lane] = xp;
+  wave_lds_sync();
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < NB; ++c) a[c & 3] += srow[c] * xb[c];
+  x -= (a[0] + a[1]) + (a[2] + a[3]);
+  return true;
 }
 
 // A 64×64 factor tile held for a matrix-vector product, 16 elements per thread. Row layout
@@ -724,17 +777,23 @@ __device__ __forceinline__ double gemv_dot(const GemvRegs& R, const double* x) {
 // the 64×64 tile at (r0, c0) of F (h × w, zero padded) into LDS as FP64, transposed if TR
 template <bool TR, class TS>
 __device__ __forceinline__ void lds_tile64(double* S, const TS* F, int LD, int r0, int c0, int h, int w) {
-  for (int t = threadIdx.x; t < NB * NB; t += NT) {
-    const int i = t % NB, k = t / NB;
-    const double x = (i < h && k < w) ? (double)F[(size_t)(r0 + i) + (size_t)(c0 + k) * LD] : 0.0;
-    if (TR) S[k + i * LDT] = x;
-    else S[i + k * LDT] = x;
+  double x[NB * NB / NT];  // all loads in flight before the first LDS write
+#pragma unroll
+  for (int u = 0; u < NB * NB / NT; ++u) {
+    const int t = threadIdx.x + u * NT, i = t % NB, k = t / NB;
+    x[u] = (i < h && k < w) ? (double)F[(size_t)(r0 + i) + (size_t)(c0 + k) * LD] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < NB * NB / NT; ++u) {
+    const int t = threadIdx.x + u * NT, i = t % NB, k = t / NB;
+    if (TR) S[k + i * LDT] = x[u];
+    else S[i + k * LDT] = x[u];
   }
 }
 
 // After a factorization: for each factor block b, M_b = L_bb⁻¹ L_{b,b−1} (b ≥ 1) and
-// N_b = L_{b+1,b} L_bb⁻¹ (b+1 < Tr), FP64, 64×64 column-major. With them each step of the
-// solves' chains is one 64×64 product after the previous block arrives:
+// N_bᵀ with N_b = L_{b+1,b} L_bb⁻¹ (b+1 < Tr), FP64, 64×64 column-major. With them each step of
+// the solves' chains is one 64×64 product after the previous block arrives:
 // x_b = w_b − M_b x_{b−1} (forward), x_b = w_b − N_bᵀ x_{b+1} (backward), where w_b is formed
 // from everything else while the workgroup waits. One workgroup per block.
 template <class TF>
@@ -743,12 +802,14 @@ __global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* in
   __shared__ __align__(16) double SY[NB * LDT];
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
-  const int b = blockIdx.x;
+  // workgroups [0, Tr) form M_b, workgroups [Tr, 2 Tr) form N_b
+  const bool mpart = (int)blockIdx.x < B.Tr;
+  const int b = mpart ? (int)blockIdx.x : (int)blockIdx.x - B.Tr;
   if (b >= B.Tr) return;
   const TF* F = factor_of<TF>(B);
   const TF* Di = dinv_of<TF>(B) + (size_t)b * NB * NB;
   const int LD = B.ld, c0 = B.tb[b], w = B.tb[b + 1] - c0;
-  if (b >= 1) {
+  if (mpart && b >= 1) {
     // M_b = X_b L with L = L_{b,b−1}: mfma_acc forms A += X Yᵀ, so Y = Lᵀ
     const int cp = B.tb[b - 1], wp = c0 - cp;
     lds_tile64<false>(SX, Di, NB, 0, 0, NB, NB);
@@ -762,9 +823,8 @@ __global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* in
     __syncthreads();
     double* M = B.Mf + (size_t)b * NB * NB;
     for (int t = threadIdx.x; t < NB * NB; t += NT) M[t] = SX[(t % NB) + (t / NB) * LDT];
-    __syncthreads();
   }
-  if (b + 1 < B.Tr) {
+  if (!mpart && b + 1 < B.Tr) {
     // N_b = L' X_b with L' = L_{b+1,b}: Y = X_bᵀ
     const int rn = B.tb[b + 1], hn = B.tb[b + 2] - rn;
     lds_tile64<false>(SX, F, LD, rn, c0, hn, w);
@@ -776,24 +836,22 @@ __global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* in
     __syncthreads();
     acc_to_lds(A, SX);
     __syncthreads();
-    double* N = B.Nb + (size_t)b * NB * NB;
-    for (int t = threadIdx.x; t < NB * NB; t += NT) N[t] = SX[(t % NB) + (t / NB) * LDT];
+    double* Nt = B.Nt + (size_t)b * NB * NB;
+    for (int t = threadIdx.x; t < NB * NB; t += NT) Nt[t] = SX[(t / NB) + (t % NB) * LDT];
   }
 }
 
-// forward: tile b (ticket order) accumulates L[b, j] x_j for the published j < b−1 while the
+// forward: tile b (ticket order) accumulates L[b, j] x_j for the complete j < b−1 while the
 // earlier tiles solve; the factor tiles stream through registers one block ahead of the solution,
 // and every block's contribution is summed in block order whatever the batches (reproducible
-// results). A factor tile then forms w_b = L_bb⁻¹ (v_b − Σ) before block b−1 arrives, and after
-// it x_b = w_b − M_b x_{b−1} (M_b staged in registers), and publishes. A separator tile adds its
-// Σ over all factor blocks into the front's update vector. TF: element type of the stored factor;
-// the arithmetic is FP64.
+// results). A factor tile then forms w_b = L_bb⁻¹ (v_b − Σ) before block b−1 arrives, and wave 0
+// finishes x_b = w_b − M_b x_{b−1} after it. A separator tile adds its Σ over all factor blocks
+// into the front's update vector. TF: element type of the stored factor; the arithmetic is FP64.
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
   __shared__ double Sinv[NB * LDT];
-  __shared__ double xs[KB * NB];
-  __shared__ double wv[NB];
+  __shared__ double xs[4 * NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -809,54 +867,30 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   const int LD = B.ld;
   const int row = threadIdx.x & 63, q = threadIdx.x >> 6;
   const bool diag = b < B.Tr;
-  const int jmax = diag ? b : B.Tr, jlast = jmax - 1;
+  const int jacc = diag ? max(b - 1, 0) : B.Tr;  // blocks summed before the last step
   const int r0 = B.tb[b], h = B.tb[b + 1] - r0;
-  // stage the operands of the last step before waiting: L_bb⁻¹ (LDS) and the tile against the
-  // last block to arrive (registers): M_b for a factor tile, L[b, Tr−1] for a separator tile
-  GemvRegs Rl;
   if (diag) load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
-  if (jlast >= 0) {
-    if (diag) gemv_rows(Rl, B.Mf + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
-    else gemv_rows(Rl, F, LD, r0, B.tb[jlast], h, B.tb[jmax] - B.tb[jlast]);
-  }
   double acc = 0.0;
   GemvRegs Rn;
-  if (jlast > 0) gemv_rows(Rn, F, LD, r0, B.tb[0], h, B.tb[1] - B.tb[0]);
-  for (int j = 0; j < jlast;) {
-    const int n = take_published(B.sflag, j, 1, min(KB, jlast - j), epoch, B.abortf);
+  if (jacc > 0) gemv_rows(Rn, F, LD, r0, B.tb[0], h, B.tb[1] - B.tb[0]);
+  for (int j = 0; j < jacc;) {
+    const int n = ll_take(B.sll, j, 1, min(4, jacc - j), epoch, B.abortf, xs);
     if (n == 0) return;
-    const int c0 = B.tb[j], wtot = B.tb[j + n] - c0;
-    for (int u = threadIdx.x; u < KB * NB; u += NT) xs[u] = u < wtot ? cload(&v[c0 + u]) : 0.0;
-    __syncthreads();
     for (int k = j; k < j + n; ++k) {
       const GemvRegs Rc = Rn;
-      if (k + 1 < jlast) gemv_rows(Rn, F, LD, r0, B.tb[k + 1], h, B.tb[k + 2] - B.tb[k + 1]);
-      acc += gemv_dot(Rc, xs + (B.tb[k] - c0));
+      if (k + 1 < jacc) gemv_rows(Rn, F, LD, r0, B.tb[k + 1], h, B.tb[k + 2] - B.tb[k + 1]);
+      acc += gemv_dot(Rc, xs + (k - j) * NB);
     }
     __syncthreads();
     j += n;
   }
-  // the last block's x into xs[0, 64) (waits for its flag)
-  auto fetch_last = [&]() -> bool {
-    if (take_published(B.sflag, jlast, 1, 1, epoch, B.abortf) == 0) return false;
-    const int cl = B.tb[jlast], wl = B.tb[jmax] - cl;
-    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < wl ? cload(&v[cl + threadIdx.x]) : 0.0;
-    __syncthreads();
-    return true;
-  };
+  part[q][row] = acc;
+  __syncthreads();
   if (!diag) {
-    if (jlast >= 0) {
-      if (!fetch_last()) return;
-      acc += gemv_dot(Rl, xs);
-    }
-    part[q][row] = acc;
-    __syncthreads();
     if (threadIdx.x < h) Fd.uvec[r0 - Fd.r + row] += part[0][row] + part[1][row] + part[2][row] + part[3][row];
     return;
   }
   // w_b = L_bb⁻¹ (v_b − Σ_{j<b−1} L_bj x_j), before block b−1 arrives
-  part[q][row] = acc;
-  __syncthreads();
   if (threadIdx.x < NB)
     xs[row] = row < h ? v[r0 + row] - (part[0][row] + part[1][row] + part[2][row] + part[3][row]) : 0.0;
   __syncthreads();
@@ -865,18 +899,12 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   for (int c = q; c < NB; c += 4) y += Sinv[row + c * LDT] * xs[c];
   part[q][row] = y;
   __syncthreads();
-  if (threadIdx.x < NB) wv[row] = part[0][row] + part[1][row] + part[2][row] + part[3][row];
-  __syncthreads();
-  if (jlast >= 0) {
-    if (!fetch_last()) return;
-    part[q][row] = gemv_dot(Rl, xs);
-    __syncthreads();
-  }
-  if (threadIdx.x < NB && row < h) {
-    const double x = jlast >= 0 ? wv[row] - (part[0][row] + part[1][row] + part[2][row] + part[3][row]) : wv[row];
-    cstore(&v[r0 + row], x);
-  }
-  publish(&B.sflag[b], epoch);
+  if (threadIdx.x >= NB) return;  // wave 0 finishes
+  double x = part[0][row] + part[1][row] + part[2][row] + part[3][row];
+  if (b > 0 && !chain_step(B.Mf + (size_t)b * NB * NB, B.sll, b - 1, epoch, B.abortf, xs, x)) return;
+  if (row >= h) x = 0.0;
+  ll_put(B.sll, b, row, x, epoch);
+  if (row < h) v[r0 + row] = x;
 }
 
 // L[sep, b]ᵀ x_sep for one factor column block b and one 64-row chunk of the separator rows
@@ -907,17 +935,17 @@ __global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc*
     B.spart[((size_t)blockIdx.y * B.Tr + b) * NB + col] = (part[0][col] + part[1][col]) + (part[2][col] + part[3][col]);
 }
 
-// backward: column block b = Tr−1−ticket accumulates L[sep, b]ᵀ x_sep (the chunk partials of
-// k_big_sepgemv, in chunk order) and L[j, b]ᵀ x_j for the published j > b+1 (in descending order
-// whatever the batches, through an LDS double buffer), forms w_b = L_bbᵀ⁻¹ (v_b − Σ) before block
-// b+1 arrives, and after it x_b = w_b − N_bᵀ x_{b+1} (N_b staged in registers).
+// backward: column block b = Tr−1−ticket accumulates L[j, b]ᵀ x_j for the complete j > b+1:
+// thread (r, q) sums L[j, b][r][q + 4i] x_j[r] over the blocks (in descending order whatever the
+// batches), one transpose through LDS then sums over r. With L[sep, b]ᵀ x_sep (the chunk partials
+// of k_big_sepgemv, in chunk order) it forms w_b = L_bbᵀ⁻¹ (v_b − Σ) before block b+1 arrives,
+// and wave 0 finishes x_b = w_b − N_bᵀ x_{b+1} after it.
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
-  __shared__ double S[2][NB * LDT];  // double buffer: the next tile loads while this one is used
+  __shared__ double S[NB * LDT];
   __shared__ double Sinv[NB * LDT];
-  __shared__ double xs[KB * NB];
-  __shared__ double wv[NB];
+  __shared__ double xs[4 * NB];
   __shared__ double part[4][NB];
   __shared__ long long tk;
   const FrontDesc& Fd = fronts[f];
@@ -932,66 +960,55 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
   const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
-  const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const bool has_next = b + 1 < B.Tr;
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
-  GemvRegs Rl;
-  if (has_next) gemv_cols(Rl, S[0], B.Nb + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
-  double acc = 0.0;
-  if (q == 0) {
-    const int nch = (LD - Fd.r + NB - 1) / NB;
-    for (int c = 0; c < nch; ++c) acc += B.spart[((size_t)c * B.Tr + b) * NB + col];
-  }
-  int cur = 0;
-  TileRegs<TF> R;
-  for (int j = B.Tr - 1; j > b + 1;) {
-    const int n = take_published(B.sflag, j, -1, min(KB, j - b - 1), epoch, B.abortf);
-    if (n == 0) return;
-    const int klo = j - n + 1, rlo = B.tb[klo], rhi = B.tb[j + 1];
-    for (int u = threadIdx.x; u < KB * NB; u += NT) xs[u] = u < rhi - rlo ? cload(&v[rlo + u]) : 0.0;
-    fetch_tile<false>(R, F, LD, B.tb[j], c0, B.tb[j + 1] - B.tb[j], w);
-    __syncthreads();
-    for (int k = j; k >= klo; --k) {
-      put_tile(S[cur], R);
-      __syncthreads();
-      const double* T = S[cur];
-      cur ^= 1;
-      if (k > klo) fetch_tile<false>(R, F, LD, B.tb[k - 1], c0, B.tb[k] - B.tb[k - 1], w);
-      const double* x = xs + (B.tb[k] - rlo);
-      double a = 0.0;
+  const int jlo = b + 2;  // blocks jlo..Tr−1 are summed before the last step
+  double av[16];
 #pragma unroll
-      for (int rr = q; rr < NB; rr += 4) a += T[rr + col * LDT] * x[rr];
-      acc += a;
+  for (int i = 0; i < 16; ++i) av[i] = 0.0;
+  GemvRegs Rn;
+  if (B.Tr - 1 >= jlo) gemv_rows(Rn, F, LD, B.tb[B.Tr - 1], c0, B.tb[B.Tr] - B.tb[B.Tr - 1], w);
+  for (int j = B.Tr - 1; j >= jlo;) {
+    const int n = ll_take(B.sll, j, -1, min(4, j - jlo + 1), epoch, B.abortf, xs);
+    if (n == 0) return;
+    for (int k = j; k > j - n; --k) {
+      const GemvRegs Rc = Rn;
+      if (k - 1 >= jlo) gemv_rows(Rn, F, LD, B.tb[k - 1], c0, B.tb[k] - B.tb[k - 1], w);
+      const double xr = xs[(j - k) * NB + lane];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) av[i] += Rc.v[i] * xr;
     }
     __syncthreads();
     j -= n;
   }
+  // column sums: S[r][c] = thread (r, ·)'s partial for column c, then thread (c, q) sums rows q + 4m
+#pragma unroll
+  for (int i = 0; i < 16; ++i) S[lane + (q + 4 * i) * LDT] = av[i];
+  __syncthreads();
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) acc += S[(q + 4 * m) + lane * LDT];
+  if (q == 0) {
+    const int nch = (LD - Fd.r + NB - 1) / NB;
+    for (int c = 0; c < nch; ++c) acc += B.spart[((size_t)c * B.Tr + b) * NB + lane];
+  }
   // w_b = L_bbᵀ⁻¹ (v_b − Σ), before block b+1 arrives
-  part[q][col] = acc;
+  part[q][lane] = acc;
   __syncthreads();
   if (threadIdx.x < NB)
-    xs[col] = col < w ? v[c0 + col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : 0.0;
+    xs[lane] = lane < w ? v[c0 + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]) : 0.0;
   __syncthreads();
   double y = 0.0;
 #pragma unroll
-  for (int rr = q; rr < NB; rr += 4) y += Sinv[rr + col * LDT] * xs[rr];
-  part[q][col] = y;
+  for (int rr = q; rr < NB; rr += 4) y += Sinv[rr + lane * LDT] * xs[rr];
+  part[q][lane] = y;
   __syncthreads();
-  if (threadIdx.x < NB) wv[col] = part[0][col] + part[1][col] + part[2][col] + part[3][col];
-  __syncthreads();
-  if (has_next) {
-    if (take_published(B.sflag, b + 1, -1, 1, epoch, B.abortf) == 0) return;
-    const int rn = B.tb[b + 1], hn = B.tb[b + 2] - rn;
-    if (threadIdx.x < NB) xs[threadIdx.x] = threadIdx.x < hn ? cload(&v[rn + threadIdx.x]) : 0.0;
-    __syncthreads();
-    part[q][col] = gemv_dot(Rl, xs);
-    __syncthreads();
-  }
-  if (threadIdx.x < NB && col < w) {
-    const double x = has_next ? wv[col] - (part[0][col] + part[1][col] + part[2][col] + part[3][col]) : wv[col];
-    cstore(&v[c0 + col], x);
-  }
-  publish(&B.sflag[b], epoch);
+  if (threadIdx.x >= NB) return;  // wave 0 finishes
+  double x = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+  if (b + 1 < B.Tr && !chain_step(B.Nt + (size_t)b * NB * NB, B.sll, b + 1, epoch, B.abortf, xs, x)) return;
+  if (lane >= w) x = 0.0;
+  ll_put(B.sll, b, lane, x, epoch);
+  if (lane < w) v[c0 + lane] = x;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1001,8 +1018,8 @@ void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int 
   if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
   else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
   if (B.Tr > 1) {
-    if (fp32) k_big_mtiles<float><<<B.Tr, NT, 0, st>>>(B, I);
-    else k_big_mtiles<double><<<B.Tr, NT, 0, st>>>(B, I);
+    if (fp32) k_big_mtiles<float><<<2 * B.Tr, NT, 0, st>>>(B, I);
+    else k_big_mtiles<double><<<2 * B.Tr, NT, 0, st>>>(B, I);
   }
 }
 

@@ -724,11 +724,11 @@ void Solver::build_schedule() {
       }
       bh->stage.alloc(sizeof(int) * std::max(1, Tr));
       HIPCHK(hipMemsetAsync(bh->stage.p, 0, bh->stage.bytes, st_));
-      bh->sflag.alloc(sizeof(int) * std::max(1, T));
+      bh->sll.alloc(sizeof(unsigned long long) * 2 * NB * std::max(1, T));
       bh->spart.alloc(sizeof(double) * NB * std::max(1, Tr) * std::max(1, (Fd.s + NB - 1) / NB));
       bh->mtile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       bh->ntile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
-      HIPCHK(hipMemsetAsync(bh->sflag.p, 0, bh->sflag.bytes, st_));
+      HIPCHK(hipMemsetAsync(bh->sll.p, 0, bh->sll.bytes, st_));
       BigFront& B = bh->B;
       B.F = Fd.F;
       B.ld = Fd.ld;
@@ -749,10 +749,10 @@ void Solver::build_schedule() {
       B.Dinv32 = bh->dinv32.as<float>();
       B.stage = bh->stage.as<int>();
       B.trace = nullptr;
-      B.sflag = bh->sflag.as<int>();
+      B.sll = bh->sll.as<unsigned long long>();
       B.spart = bh->spart.as<double>();
       B.Mf = bh->mtile.as<double>();
-      B.Nb = bh->ntile.as<double>();
+      B.Nt = bh->ntile.as<double>();
       B.sticket = d_bigstick_.as<long long>() + big_.size();
       B.inst = Fd.inst;
       bh->front = f;
@@ -866,6 +866,22 @@ void Solver::dot(int space, int kx, int ky, int mode, int slot, int op, int kw0,
   else
     launch_dot(st_, I, d_mtiles_.as<Tile>(), ntile_m_, d_mtofs_.as<int>(), 1, 1, kx, ky, mode, kw0, kw1,
                d_partial_.as<double>(), d_dotcnt_.as<int>(), slot, op);
+}
+
+// several dot products in one launch while the vectors are short enough for one workgroup each
+void Solver::dots(const DotBatch& b) {
+  if (std::max(n_, m_) <= DOTS_LEN_MAX) {
+    launch_dots(st_, d_inst_.as<InstDev>(), 0, b);
+    return;
+  }
+  for (int k = 0; k < b.n; ++k) {
+    const DotSpec& d = b.s[k];
+    dot(d.space, d.kx, d.ky, d.mode, d.slot, d.op, d.kw0, d.kw1);
+  }
+}
+
+void Solver::lin2(const Lin& on, const Lin& om) {
+  launch_lin2(st_, d_inst_.as<InstDev>(), d_ntiles_.as<Tile>(), ntile_n_, d_mtiles_.as<Tile>(), ntile_m_, on, om);
 }
 
 void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
@@ -984,7 +1000,7 @@ void Solver::prof_flush() {
 void Solver::set_trace(bool on) {
   for (auto& bh : big_) {
     if (on && bh->trace.p == nullptr) {
-      bh->trace.alloc(sizeof(unsigned long long) * 4 * std::max(1, bh->B.ntask));
+      bh->trace.alloc(sizeof(unsigned long long) * 8 * std::max(1, bh->B.ntask));
       HIPCHK(hipMemsetAsync(bh->trace.p, 0, bh->trace.bytes, st_));
     }
     bh->B.trace = on ? bh->trace.as<unsigned long long>() : nullptr;
@@ -994,7 +1010,7 @@ void Solver::set_trace(bool on) {
 int Solver::get_trace(int b, long long* out, int len) {
   if (b < 0 || b >= (int)big_.size()) return 0;
   BigHost& bh = *big_[b];
-  int n = 4 * bh.B.ntask;
+  int n = 8 * bh.B.ntask;
   if (bh.trace.p == nullptr || out == nullptr) return n;
   HIPCHK(hipStreamSynchronize(st_));
   HIPCHK(hipMemcpyAsync(out, bh.trace.p, sizeof(long long) * std::min(n, len), hipMemcpyDeviceToHost, st_));
@@ -1331,7 +1347,7 @@ int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& 
         niter += 1;
         spmv(1, MV_CP, NV_CW, K(1.0), K(0.0));  // w = Bᵀ p
         lsolve(NV_CW);
-        dot(0, NV_CW, NV_CW, 0, S_CG_EP2, 0);
+        dots(DotBatch().add(0, NV_CW, NV_CW, 0, S_CG_EP2, 0));
         sync_scalars();
         double ep2 = hsc_[S_CG_EP2];
         if (ep2 <= EPS * np2) {
@@ -1339,11 +1355,14 @@ int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& 
         } else {
           usolve(NV_CW);
           launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, 0);  // α = ν²/ε²
-          axpby(1, MV_CP, ky, ky, KS(1.0, S_CG_ALPHA), K(1.0));  // y += α p
-          axpby(0, NV_CW, kx, kx, KS(1.0, S_CG_ALPHA), K(1.0));  // x += α w
-          spmv(0, NV_CW, ku, KS(-1.0, S_CG_ALPHA), K(1.0));       // u −= α B w
-          dot(1, ku, ku, 0, S_NORM0, 0);
-          if (stops) dot(1, ku, MV_YSCL, 1, S_R9, 1);
+          // x += α w, y += α p
+          lin2(Lin{NV_CW, kx, kx, kx, KS(1.0, S_CG_ALPHA), K(1.0), K(0.0)},
+               Lin{MV_CP, ky, ky, ky, KS(1.0, S_CG_ALPHA), K(1.0), K(0.0)});
+          spmv(0, NV_CW, ku, KS(-1.0, S_CG_ALPHA), K(1.0));  // u −= α B w
+          DotBatch nb;
+          nb.add(1, ku, ku, 0, S_NORM0, 0);
+          if (stops) nb.add(1, ku, MV_YSCL, 1, S_R9, 1);
+          dots(nb);
           sync_scalars();
           double pnu2 = nu2;
           nu2 = hsc_[S_NORM0];
@@ -1376,18 +1395,19 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   // the norms of a residual: ‖KF‖ (and its scaled stop), and ‖KG‖² (and its scaled stop) for the
   // dual CG that may follow, all in one read-back
   auto norms = [&]() {
-    dot(0, NV_KF, NV_KF, 0, S_R10, 1);
-    if (stops) dot(0, NV_KF, NV_PSCL, 1, S_R11, 1);
-    dot(1, MV_KG, MV_KG, 0, S_R8, 0);
-    if (stops) dot(1, MV_KG, MV_YSCL, 1, S_R9, 1);
+    DotBatch b;
+    b.add(0, NV_KF, NV_KF, 0, S_R10, 1);
+    if (stops) b.add(0, NV_KF, NV_PSCL, 1, S_R11, 1);
+    b.add(1, MV_KG, MV_KG, 0, S_R8, 0);
+    if (stops) b.add(1, MV_KG, MV_YSCL, 1, S_R9, 1);
+    dots(b);
     sync_scalars();
   };
   auto residual = [&](bool read) {
     // KF = f − H x + Bᵀ y ; KG = g − B x
     hmul(kx, NV_T0);
     spmv(1, ky, NV_T1, K(1.0), K(0.0));
-    axpbypcz(0, kf, NV_T0, NV_T1, NV_KF, K(1.0), K(-1.0), K(1.0));
-    axpby(1, kg, kg, MV_KG, K(1.0), K(0.0));
+    lin2(Lin{kf, NV_T0, NV_T1, NV_KF, K(1.0), K(-1.0), K(1.0)}, Lin{kg, kg, kg, MV_KG, K(1.0), K(0.0), K(0.0)});
     spmv(0, kx, MV_KG, K(-1.0), K(1.0));
     if (read) norms();
   };
@@ -1397,10 +1417,8 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   if (warm) {
     residual(read0);
   } else {
-    axpby(0, kf, kf, NV_KF, K(1.0), K(0.0));
-    axpby(1, kg, kg, MV_KG, K(1.0), K(0.0));
-    axpby(0, kx, kx, kx, K(0.0), K(0.0));
-    axpby(1, ky, ky, ky, K(0.0), K(0.0));
+    lin2(Lin{kf, kf, kf, NV_KF, K(1.0), K(0.0), K(0.0)}, Lin{kg, kg, kg, MV_KG, K(1.0), K(0.0), K(0.0)});
+    lin2(Lin{kx, kx, kx, kx, K(0.0), K(0.0), K(0.0)}, Lin{ky, ky, ky, ky, K(0.0), K(0.0), K(0.0)});
     if (read0) norms();
   }
   auto fstop = [&]() { return stops && hsc_[S_R11] / (1.0 + sf_) <= set_.feas_tol; };
@@ -1418,8 +1436,8 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
     lsolve(NV_KX);
     usolve(NV_KX);
     spmv(0, NV_KX, MV_KG, K(-1.0), K(1.0));                         // δg −= B δx
-    axpby(0, NV_KX, kx, kx, K(1.0), K(1.0));                        // x += δx
-    axpby(1, MV_KG, ky, ky, KS(1.0, S_ALPHA), K(1.0));              // y += α δg
+    lin2(Lin{NV_KX, kx, kx, kx, K(1.0), K(1.0), K(0.0)},            // x += δx
+         Lin{MV_KG, ky, ky, ky, KS(1.0, S_ALPHA), K(1.0), K(0.0)});  // y += α δg
     residual(true);
     fres = hsc_[S_R10];
     hist.push_back(fres);
@@ -1507,17 +1525,14 @@ bool Solver::isopt(double mu, double mus, double pobj, double dobj, double pres,
 // statistics of the current iterate for the termination and certificate tests
 void Solver::compute_stats() {
   // R0 gy, R1 ‖y‖, R2 ‖Qp‖, R3 ‖d‖, R4 ‖Qp − d − Bᵀy‖, R5 fp, R6 ‖p‖, R7 ‖Bp‖
-  dot(1, MV_G, MV_Y, 0, S_R0, 0);
-  dot(1, MV_Y, MV_Y, 0, S_R1, 1);
-  dot(0, NV_QP, NV_QP, 0, S_R2, 1);
-  dot(0, NV_D, NV_D, 0, S_R3, 1);
   spmv(1, MV_Y, NV_T1, K(1.0), K(0.0));
   axpbypcz(0, NV_QP, NV_D, NV_T1, NV_T2, K(1.0), K(-1.0), K(-1.0));
-  dot(0, NV_T2, NV_T2, 0, S_R4, 1);
-  dot(0, NV_F, NV_P, 0, S_R5, 0);
-  dot(0, NV_P, NV_P, 0, S_R6, 1);
   spmv(0, NV_P, MV_T0, K(1.0), K(0.0));
-  dot(1, MV_T0, MV_T0, 0, S_R7, 1);
+  DotBatch b;
+  b.add(1, MV_G, MV_Y, 0, S_R0, 0).add(1, MV_Y, MV_Y, 0, S_R1, 1).add(0, NV_QP, NV_QP, 0, S_R2, 1);
+  b.add(0, NV_D, NV_D, 0, S_R3, 1).add(0, NV_T2, NV_T2, 0, S_R4, 1).add(0, NV_F, NV_P, 0, S_R5, 0);
+  b.add(0, NV_P, NV_P, 0, S_R6, 1).add(1, MV_T0, MV_T0, 0, S_R7, 1);
+  dots(b);
   sync_scalars();
 }
 
@@ -1554,7 +1569,6 @@ int Solver::step() {
   set_scalar(S_DELTA, delta);
   set_scalar(S_ALPHA, 1.0 / delta);
   // pᵀd, scale!, residuals!
-  dot(0, NV_P, NV_D, 0, S_TMP0, 0);
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                     d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
@@ -1562,15 +1576,16 @@ int Solver::step() {
                      d_dpart_.as<double>());
   spmv(2, NV_P, NV_QP, K(1.0), K(0.0));                                  // Qp
   spmv(1, MV_Y, NV_T0, K(1.0), K(0.0));                                  // Bᵀy
-  axpbypcz(0, NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0));        // Δf = d + f − Qp
+  lin2(Lin{NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0)},          // Δf = d + f − Qp
+       Lin{MV_G, MV_G, MV_G, MV_DG, K(1.0), K(0.0), K(0.0)});           // Δg = g
   axpby(0, NV_T0, NV_DF, NV_DF, K(1.0), K(1.0));                         //      + Bᵀy
-  axpby(1, MV_G, MV_G, MV_DG, K(1.0), K(0.0));                           // Δg = g
   spmv(0, NV_P, MV_DG, K(-1.0), K(1.0));                                 //      − Bp
-  dot(1, MV_DG, MV_YSCL, 1, S_NORM0, 1);
-  dot(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
-  dot(0, NV_P, NV_QP, 0, S_NORM2, 0);
-  dot(0, NV_F, NV_P, 0, S_NORM3, 0);
-  dot(1, MV_G, MV_Y, 0, S_NORM4, 0);
+  {
+    DotBatch b;
+    b.add(0, NV_P, NV_D, 0, S_TMP0, 0).add(1, MV_DG, MV_YSCL, 1, S_NORM0, 1).add(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
+    b.add(0, NV_P, NV_QP, 0, S_NORM2, 0).add(0, NV_F, NV_P, 0, S_NORM3, 0).add(1, MV_G, MV_Y, 0, S_NORM4, 0);
+    dots(b);
+  }
   sync_scalars();
   double pd = hsc_[S_TMP0];
   bool flag = hsc_[S_FLAGOK] != 0.0;
@@ -1657,7 +1672,7 @@ int Solver::step() {
           // corrector (solver/ipm.jl:181-272)
           // τa stays on the device: the dot reads it there, and one read-back brings μa home
           launch_maxsteps(NV_DPA, NV_DDA, 1.0);
-          dot(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA);
+          dots(DotBatch().add(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA));
           sync_scalars();
           double smu = hsc_[S_TMP2];
           double mua = smu / nu_;
@@ -1748,12 +1763,10 @@ void Solver::result(double* p, double* d, double* y, double* info) {
     axpby(0, NV_T0, NV_DF, NV_DF, K(1.0), K(1.0));
     axpby(1, MV_G, MV_G, MV_DG, K(1.0), K(0.0));
     spmv(0, NV_P, MV_DG, K(-1.0), K(1.0));
-    dot(1, MV_DG, MV_YSCL, 1, S_NORM0, 1);
-    dot(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
-    dot(0, NV_P, NV_QP, 0, S_NORM2, 0);
-    dot(0, NV_F, NV_P, 0, S_NORM3, 0);
-    dot(1, MV_G, MV_Y, 0, S_NORM4, 0);
-    dot(0, NV_P, NV_D, 0, S_TMP0, 0);
+    DotBatch b;
+    b.add(1, MV_DG, MV_YSCL, 1, S_NORM0, 1).add(0, NV_DF, NV_PSCL, 1, S_NORM1, 1).add(0, NV_P, NV_QP, 0, S_NORM2, 0);
+    b.add(0, NV_F, NV_P, 0, S_NORM3, 0).add(1, MV_G, MV_Y, 0, S_NORM4, 0).add(0, NV_P, NV_D, 0, S_TMP0, 0);
+    dots(b);
     sync_scalars();
     double pQp = hsc_[S_NORM2];
     pobj = pQp / 2 - hsc_[S_NORM3];

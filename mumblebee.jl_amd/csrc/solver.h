@@ -151,6 +151,8 @@ class Solver {
   void axpby(int space, int kx, int ky, int kz, Coef a, Coef b);
   void axpbypcz(int space, int kx, int ky, int kw, int kz, Coef a, Coef b, Coef c);
   void dot(int space, int kx, int ky, int mode, int slot, int op, int kw0 = 0, int kw1 = 0);
+  void dots(const DotBatch& b);
+  void lin2(const Lin& on, const Lin& om);
   void spmv(int which, int kx, int ky, Coef a, Coef b);
   void hmul(int kx, int ky);  // ky = H kx
   void lsolve(int kv);
@@ -244,7 +246,7 @@ class Solver {
     BigFront B;
     int front;
     long long sbase = 0;
-    DevBuf tb, state, tasks, sflag, spart, mtile, ntile, dinv, stage, trace, f32, dinv32;
+    DevBuf tb, state, tasks, sll, spart, mtile, ntile, dinv, stage, trace, f32, dinv32;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
   DevBuf d_bigtick_, d_bigstick_;

@@ -420,15 +420,16 @@ class IPMSolver:
 
     def dag_trace(self, front=0):
         """Task timeline of the last persistent factorization of large front `front`: an
-        (ntask, 4) array of 100 MHz wall-clock stamps (claimed, updates done, last input ready,
-        published), in schedule order."""
+        (ntask, 8) array of 100 MHz wall-clock stamps in schedule order (claimed, updates done,
+        last input ready, done; a diagonal task also: factor panels done, diagonal-block
+        inverses done, inverse complete, inverse published; 0 where a task has none)."""
         n = lib().mb_solver_dag_trace(self._h, int(front), None, 0)
         if n < 0:
             raise _err("trace")
         out = np.zeros(max(n, 0), dtype=np.int64)
         if n > 0 and lib().mb_solver_dag_trace(self._h, int(front), _p(out, ctypes.c_int64), n) < 0:
             raise _err("trace")
-        return out.reshape(-1, 4)
+        return out.reshape(-1, 8)
 
     def profile(self):
         out = np.zeros(4 * len(self.PROF_KINDS))

@@ -27,6 +27,14 @@ def _print_case(**kw):
     sys.stdout.flush()
 
 
+def _objective_rel(res):
+    """How close the stopping rule puts pobj to a closed-form optimum. OPTIMAL: the gap and the
+    residuals are within gap_tol = feas_tol = 1e-8 (relative), which together move pobj off the
+    optimum by up to about twice that. NEAR_OPTIMAL: the same rule with tolerances
+    near_factor = 1000 times looser."""
+    return 2e-8 if res.status == mh.IPMStatus.OPTIMAL else 2e-5
+
+
 def _oracle(data, **kw):
     from oracle import ipm as O
     return O.Solver(oracle_problem(data), **kw).solve()
@@ -38,7 +46,7 @@ def test_entropy_on_the_simplex(k):
     data = G.exppow(nexp=k, npow=0, m=0, eps=0.0, seed=0)
     res = mh.solve(hip_problem(data))
     assert res.status in GOOD
-    assert res.pobj == pytest.approx(-math.log(k), rel=1e-8, abs=1e-8)
+    assert res.pobj == pytest.approx(-math.log(k), rel=_objective_rel(res), abs=1e-8), res.status
 
 
 def test_softmax_entropy_closed_form():
@@ -48,7 +56,7 @@ def test_softmax_entropy_closed_form():
     eps = data["f"][1:60:3]
     res = mh.solve(hip_problem(data))
     assert res.status in GOOD
-    assert res.pobj == pytest.approx(-math.log(np.exp(eps).sum()), rel=1e-8)
+    assert res.pobj == pytest.approx(-math.log(np.exp(eps).sum()), rel=_objective_rel(res)), res.status
 
 
 def test_weighted_geometric_means_under_budgets():
