@@ -79,6 +79,19 @@ struct Lin {
 void launch_lin2(hipStream_t, const InstDev*, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt, Lin on,
                  Lin om);
 void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
+// Sparse products over the n rows and over the m rows in one launch. n rows (the row tiles of
+// matrix ntm: 1 = Bᵀ, 2 = Q): y = aq Q xq + at Bᵀ xt + c z + d w; m rows (the row tiles of B):
+// y = a B x + c z, then y2 += e y. A term whose kind is < 0 is absent.
+struct SpmvN {
+  int kxq, kxt, kz, kw, ky;
+  Coef aq, at, c, d;
+};
+struct SpmvM {
+  int kx, kz, ky, ky2;
+  Coef a, c, e;
+};
+void launch_spmv_nm(hipStream_t, const InstDev*, int ntm, const Tile* ntiles, int nn, const Tile* mtiles, int nm,
+                    const SpmvN& N, const SpmvM& M);
 void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky, Coef a);
 // psd / dense: lists of the PSD cones and of the cones with dense scaling blocks (SOC, PSD)
 void launch_cone_scale(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,

@@ -279,6 +279,56 @@ __global__ void __launch_bounds__(TILE) k_spmv(const InstDev* insts, const Tile*
   }
 }
 
+// row i of A against x by a group of G lanes: a lane-strided slice each, then the fixed xor tree
+// (every lane of the group reaches the shuffles)
+__device__ __forceinline__ double csr_row(const CSRDev& A, int i, bool ok, int lane, int G, const double* x) {
+  double acc = 0.0;
+  const int k0 = ok ? A.ptr[i] : 0, k1 = ok ? A.ptr[i + 1] : 0;
+  for (int k = k0 + lane; k < k1; k += G) acc += A.val[k] * x[A.col[k]];
+  for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  return acc;
+}
+
+__device__ __forceinline__ double coefterm(const Coef& c, const double* sc, const double* v, int i) {
+  const double cv = coefval(c, sc);
+  return cv == 0.0 ? 0.0 : cv * v[i];
+}
+
+// blocks [0, nn): n rows in the tiles of matrix ntm; the rest: m rows in the tiles of B
+__global__ void __launch_bounds__(TILE) k_spmv_nm(const InstDev* insts, int ntm, const Tile* ntiles, int nn,
+                                                  const Tile* mtiles, SpmvN N, SpmvM M) {
+  const bool isn = (int)blockIdx.x < nn;
+  const Tile t = isn ? ntiles[blockIdx.x] : mtiles[blockIdx.x - nn];
+  const InstDev& I = insts[t.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  if (isn) {
+    const int G = (ntm == 2 ? I.Q : I.Bt).lanes;
+    const int lane = threadIdx.x & (G - 1), i = t.start + (int)threadIdx.x / G;
+    const bool ok = i < I.Q.nrow;
+    const double sq = N.kxq >= 0 ? csr_row(I.Q, i, ok, lane, G, nvec(I, N.kxq)) : 0.0;
+    const double st = N.kxt >= 0 ? csr_row(I.Bt, i, ok, lane, G, mvec(I, N.kxt)) : 0.0;
+    if (ok && lane == 0) {
+      double y = 0.0;
+      if (N.kxq >= 0) y += coefval(N.aq, I.sc) * sq;
+      if (N.kxt >= 0) y += coefval(N.at, I.sc) * st;
+      if (N.kz >= 0) y += coefterm(N.c, I.sc, nvec(I, N.kz), i);
+      if (N.kw >= 0) y += coefterm(N.d, I.sc, nvec(I, N.kw), i);
+      nvec(I, N.ky)[i] = y;
+    }
+  } else {
+    const int G = I.B.lanes;
+    const int lane = threadIdx.x & (G - 1), i = t.start + (int)threadIdx.x / G;
+    const bool ok = i < I.B.nrow;
+    const double s = csr_row(I.B, i, ok, lane, G, nvec(I, M.kx));
+    if (ok && lane == 0) {
+      double y = coefval(M.a, I.sc) * s;
+      if (M.kz >= 0) y += coefterm(M.c, I.sc, mvec(I, M.kz), i);
+      mvec(I, M.ky)[i] = y;
+      if (M.ky2 >= 0) mvec(I, M.ky2)[i] += coefval(M.e, I.sc) * y;
+    }
+  }
+}
+
 // y[cols] += a D_v x[cols] over all cones (the cone-block part of H x)
 __global__ void k_cone_matvec(const InstDev* insts, const ConeDesc* cones, int ncone, int kx, int ky, Coef a) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1143,6 +1193,10 @@ void launch_lin2(hipStream_t st, const InstDev* I, const Tile* ntiles, int nnt, 
 void launch_spmv(hipStream_t st, const InstDev* I, const Tile* T, int nt, int which, int kx, int ky, Coef a,
                  Coef b) {
   if (nt) k_spmv<<<nt, TILE, 0, st>>>(I, T, which, kx, ky, a, b);
+}
+void launch_spmv_nm(hipStream_t st, const InstDev* I, int ntm, const Tile* ntiles, int nn, const Tile* mtiles, int nm,
+                    const SpmvN& N, const SpmvM& M) {
+  if (nn + nm) k_spmv_nm<<<nn + nm, TILE, 0, st>>>(I, ntm, ntiles, nn, mtiles, N, M);
 }
 void launch_cone_matvec(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx, int ky, Coef a) {
   if (nc) k_cone_matvec<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx, ky, a);

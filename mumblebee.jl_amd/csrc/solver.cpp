@@ -884,6 +884,14 @@ void Solver::lin2(const Lin& on, const Lin& om) {
   launch_lin2(st_, d_inst_.as<InstDev>(), d_ntiles_.as<Tile>(), ntile_n_, d_mtiles_.as<Tile>(), ntile_m_, on, om);
 }
 
+// n-row and m-row sparse products in one launch (either may be absent)
+void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
+  static const SpmvN n0{-1, -1, -1, -1, -1, K(0.0), K(0.0), K(0.0), K(0.0)};
+  static const SpmvM m0{-1, -1, -1, -1, K(0.0), K(0.0), K(0.0)};
+  launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), N ? nsptile_[ntm] : 0,
+                 d_sptiles_[0].as<Tile>(), M ? nsptile_[0] : 0, N ? *N : n0, M ? *M : m0);
+}
+
 void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
   const InstDev* I = d_inst_.as<InstDev>();
   launch_spmv(st_, I, d_sptiles_[which].as<Tile>(), nsptile_[which], which, kx, ky, a, b);
@@ -1404,11 +1412,13 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
     sync_scalars();
   };
   auto residual = [&](bool read) {
-    // KF = f − H x + Bᵀ y ; KG = g − B x
-    hmul(kx, NV_T0);
-    spmv(1, ky, NV_T1, K(1.0), K(0.0));
-    lin2(Lin{kf, NV_T0, NV_T1, NV_KF, K(1.0), K(-1.0), K(1.0)}, Lin{kg, kg, kg, MV_KG, K(1.0), K(0.0), K(0.0)});
-    spmv(0, kx, MV_KG, K(-1.0), K(1.0));
+    // KF = f − Q x + Bᵀ y (one pass over the rows), then − (cone blocks) x ; KG = g − B x
+    const SpmvN rn{kx, ky, kf, -1, NV_KF, K(-1.0), K(1.0), K(1.0), K(0.0)};
+    const SpmvM rm{kx, kg, MV_KG, -1, K(-1.0), K(1.0), K(0.0)};
+    spmv_nm(2, &rn, &rm);
+    launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, NV_KF, K(-1.0));
+    launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
+                             dense_nmax_, kx, NV_KF, K(-1.0));
     if (read) norms();
   };
   // with irmin ≥ 1 and irmax ≥ 1 the starting residual decides nothing (and its norm enters no
@@ -1431,13 +1441,14 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   }
   while (irstat == KKT_CONTINUE) {
     nir += 1;
-    spmv(1, MV_KG, NV_T0, K(1.0), K(0.0));                          // T0 = Bᵀ δg
-    axpby(0, NV_KF, NV_T0, NV_KX, KS(1.0, S_DELTA), K(1.0));        // δx = δ δf + Bᵀ δg
+    const SpmvN dx{-1, MV_KG, NV_KF, -1, NV_KX, K(0.0), K(1.0), KS(1.0, S_DELTA), K(0.0)};
+    spmv_nm(1, &dx, nullptr);  // δx = δ δf + Bᵀ δg
     lsolve(NV_KX);
     usolve(NV_KX);
-    spmv(0, NV_KX, MV_KG, K(-1.0), K(1.0));                         // δg −= B δx
-    lin2(Lin{NV_KX, kx, kx, kx, K(1.0), K(1.0), K(0.0)},            // x += δx
-         Lin{MV_KG, ky, ky, ky, KS(1.0, S_ALPHA), K(1.0), K(0.0)});  // y += α δg
+    // δg −= B δx, then y += α δg ; x += δx
+    const SpmvN xu{-1, -1, NV_KX, kx, kx, K(0.0), K(0.0), K(1.0), K(1.0)};
+    const SpmvM gu{NV_KX, MV_KG, MV_KG, ky, K(-1.0), K(1.0), KS(1.0, S_ALPHA)};
+    spmv_nm(1, &xu, &gu);
     residual(true);
     fres = hsc_[S_R10];
     hist.push_back(fres);

@@ -76,8 +76,12 @@ def test_mixed_cones_match_oracle(nexp, npow, m, seed):
     ref = _oracle(data)
     res = mh.solve(hip_problem(data))
     assert int(ref.status) in GOOD and res.status in GOOD
-    assert rel(res.pobj, ref.pobj) < 1e-8
-    assert rel(res.dobj, ref.dobj) < 1e-8
+    # both OPTIMAL: within the stopping rule's 1e-8; a NEAR_OPTIMAL end on either side meets the
+    # rule only with near_factor = 1000 times looser tolerances
+    both = res.status == mh.IPMStatus.OPTIMAL and int(ref.status) == int(mh.IPMStatus.OPTIMAL)
+    tol = 1e-8 if both else 1e-5
+    assert rel(res.pobj, ref.pobj) < tol, (res.status, ref.status)
+    assert rel(res.dobj, ref.dobj) < tol, (res.status, ref.status)
     assert res.pres < 1e-8 and res.dres < 1e-8
 
 
