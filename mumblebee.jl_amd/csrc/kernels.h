@@ -71,24 +71,35 @@ struct DotBatch {
   }
 };
 void launch_dots(hipStream_t, const InstDev*, int inst, const DotBatch&);
-// z = a x + b y + c w on the n-space and another on the m-space, in one launch
+// z = a x + b y + c w; up to LIN_MAX of them on the n-space and on the m-space in one launch,
+// applied in order at each index (a later one sees the earlier ones' results)
 struct Lin {
   int kx, ky, kw, kz;
   Coef a, b, c;
 };
-void launch_lin2(hipStream_t, const InstDev*, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt, Lin on,
-                 Lin om);
+constexpr int LIN_MAX = 3;
+struct LinSet {
+  Lin op[LIN_MAX];
+  int n = 0;
+  LinSet& add(const Lin& l) {
+    op[n++] = l;
+    return *this;
+  }
+};
+void launch_lin(hipStream_t, const InstDev*, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt,
+                const LinSet& N, const LinSet& M);
 void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
 // Sparse products over the n rows and over the m rows in one launch. n rows (the row tiles of
-// matrix ntm: 1 = Bᵀ, 2 = Q): y = aq Q xq + at Bᵀ xt + c z + d w; m rows (the row tiles of B):
-// y = a B x + c z, then y2 += e y. A term whose kind is < 0 is absent.
+// matrix ntm: 1 = Bᵀ, 2 = Q): y = aq Q xq + at Bᵀ xt + c z + d w + e v; m rows (the row tiles
+// of B): y = a B x + c z, then y2 += e y (y2 = e y if y2set). A term whose kind is < 0 is absent.
 struct SpmvN {
-  int kxq, kxt, kz, kw, ky;
-  Coef aq, at, c, d;
+  int kxq, kxt, kz, kw, kv, ky;
+  Coef aq, at, c, d, e;
 };
 struct SpmvM {
   int kx, kz, ky, ky2;
   Coef a, c, e;
+  int y2set;
 };
 void launch_spmv_nm(hipStream_t, const InstDev*, int ntm, const Tile* ntiles, int nn, const Tile* mtiles, int nm,
                     const SpmvN& N, const SpmvM& M);
@@ -139,5 +150,6 @@ void launch_bsolve(hipStream_t, const FrontDesc*, const int* items, int nitem, c
 void launch_cg_scalars(hipStream_t, const InstDev*, int ninst, int op);
 void launch_probe_mfma(hipStream_t, double* out, bool fp32);
 void launch_set_scalar(hipStream_t, double* sc, int slot, double v);
+void launch_set_scalar2(hipStream_t, double* sc, int s1, double v1, int s2, double v2);
 
 }  // namespace mb

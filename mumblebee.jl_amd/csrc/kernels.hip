@@ -216,9 +216,9 @@ __global__ void __launch_bounds__(DOTS_THREADS) k_dots(const InstDev* insts, int
   }
 }
 
-// z = a x + b y + c w, blocks [0, nnt) on the n-space tiles with `on`, the rest on the m-space
-// tiles with `om`
-__global__ void k_lin2(const InstDev* insts, const Tile* ntiles, int nnt, const Tile* mtiles, Lin on, Lin om) {
+// z = a x + b y + c w for each op of the set, in order: blocks [0, nnt) on the n-space tiles with
+// N, the rest on the m-space tiles with M
+__global__ void k_lin(const InstDev* insts, const Tile* ntiles, int nnt, const Tile* mtiles, LinSet N, LinSet M) {
   const bool isn = (int)blockIdx.x < nnt;
   const Tile t = isn ? ntiles[blockIdx.x] : mtiles[blockIdx.x - nnt];
   const InstDev& I = insts[t.inst];
@@ -226,12 +226,15 @@ __global__ void k_lin2(const InstDev* insts, const Tile* ntiles, int nnt, const 
   const int space = isn ? 0 : 1;
   const int i = t.start + threadIdx.x;
   if (i >= spacelen(I, space)) return;
-  const Lin L = isn ? on : om;
-  const double av = coefval(L.a, I.sc), bv = coefval(L.b, I.sc), cv = coefval(L.c, I.sc);
-  double acc = av * vecof(I, space, L.kx)[i];
-  if (bv != 0.0) acc += bv * vecof(I, space, L.ky)[i];
-  if (cv != 0.0) acc += cv * vecof(I, space, L.kw)[i];
-  vecof(I, space, L.kz)[i] = acc;
+  const int nop = isn ? N.n : M.n;
+  for (int k = 0; k < nop; ++k) {
+    const Lin L = isn ? N.op[k] : M.op[k];
+    const double av = coefval(L.a, I.sc), bv = coefval(L.b, I.sc), cv = coefval(L.c, I.sc);
+    double acc = av * vecof(I, space, L.kx)[i];
+    if (bv != 0.0) acc += bv * vecof(I, space, L.ky)[i];
+    if (cv != 0.0) acc += cv * vecof(I, space, L.kw)[i];
+    vecof(I, space, L.kz)[i] = acc;
+  }
 }
 
 // sc[slot] = op(Σ partial over the instance's tiles), one block per instance, fixed order
@@ -313,6 +316,7 @@ __global__ void __launch_bounds__(TILE) k_spmv_nm(const InstDev* insts, int ntm,
       if (N.kxt >= 0) y += coefval(N.at, I.sc) * st;
       if (N.kz >= 0) y += coefterm(N.c, I.sc, nvec(I, N.kz), i);
       if (N.kw >= 0) y += coefterm(N.d, I.sc, nvec(I, N.kw), i);
+      if (N.kv >= 0) y += coefterm(N.e, I.sc, nvec(I, N.kv), i);
       nvec(I, N.ky)[i] = y;
     }
   } else {
@@ -324,7 +328,11 @@ __global__ void __launch_bounds__(TILE) k_spmv_nm(const InstDev* insts, int ntm,
       double y = coefval(M.a, I.sc) * s;
       if (M.kz >= 0) y += coefterm(M.c, I.sc, mvec(I, M.kz), i);
       mvec(I, M.ky)[i] = y;
-      if (M.ky2 >= 0) mvec(I, M.ky2)[i] += coefval(M.e, I.sc) * y;
+      if (M.ky2 >= 0) {
+        double* y2 = mvec(I, M.ky2) + i;
+        const double ey = coefval(M.e, I.sc) * y;
+        *y2 = M.y2set ? ey : *y2 + ey;
+      }
     }
   }
 }
@@ -1186,9 +1194,10 @@ void launch_dot(hipStream_t st, const InstDev* I, const Tile* T, int nt, const i
 void launch_dots(hipStream_t st, const InstDev* I, int inst, const DotBatch& B) {
   if (B.n > 0) k_dots<<<B.n, DOTS_THREADS, 0, st>>>(I, inst, B);
 }
-void launch_lin2(hipStream_t st, const InstDev* I, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt, Lin on,
-                 Lin om) {
-  if (nnt + nmt) k_lin2<<<nnt + nmt, TILE, 0, st>>>(I, ntiles, nnt, mtiles, on, om);
+void launch_lin(hipStream_t st, const InstDev* I, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt,
+                const LinSet& N, const LinSet& M) {
+  const int nn = N.n ? nnt : 0, nm = M.n ? nmt : 0;
+  if (nn + nm) k_lin<<<nn + nm, TILE, 0, st>>>(I, ntiles, nn, mtiles, N, M);
 }
 void launch_spmv(hipStream_t st, const InstDev* I, const Tile* T, int nt, int which, int kx, int ky, Coef a,
                  Coef b) {
@@ -1274,6 +1283,13 @@ void launch_probe_mfma(hipStream_t st, double* out, bool fp32) {
 }
 
 __global__ void k_set_scalar(double* sc, int slot, double v) { sc[slot] = v; }
+__global__ void k_set_scalar2(double* sc, int s1, double v1, int s2, double v2) {
+  sc[s1] = v1;
+  sc[s2] = v2;
+}
+void launch_set_scalar2(hipStream_t st, double* sc, int s1, double v1, int s2, double v2) {
+  k_set_scalar2<<<1, 1, 0, st>>>(sc, s1, v1, s2, v2);
+}
 void launch_set_scalar(hipStream_t st, double* sc, int slot, double v) { k_set_scalar<<<1, 1, 0, st>>>(sc, slot, v); }
 
 }  // namespace mb

@@ -880,14 +880,16 @@ void Solver::dots(const DotBatch& b) {
   }
 }
 
-void Solver::lin2(const Lin& on, const Lin& om) {
-  launch_lin2(st_, d_inst_.as<InstDev>(), d_ntiles_.as<Tile>(), ntile_n_, d_mtiles_.as<Tile>(), ntile_m_, on, om);
+void Solver::lin(const LinSet& N, const LinSet& M) {
+  launch_lin(st_, d_inst_.as<InstDev>(), d_ntiles_.as<Tile>(), ntile_n_, d_mtiles_.as<Tile>(), ntile_m_, N, M);
 }
+
+void Solver::lin2(const Lin& on, const Lin& om) { lin(LinSet().add(on), LinSet().add(om)); }
 
 // n-row and m-row sparse products in one launch (either may be absent)
 void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
-  static const SpmvN n0{-1, -1, -1, -1, -1, K(0.0), K(0.0), K(0.0), K(0.0)};
-  static const SpmvM m0{-1, -1, -1, -1, K(0.0), K(0.0), K(0.0)};
+  static const SpmvN n0{-1, -1, -1, -1, -1, -1, K(0.0), K(0.0), K(0.0), K(0.0), K(0.0)};
+  static const SpmvM m0{-1, -1, -1, -1, K(0.0), K(0.0), K(0.0), 0};
   launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), N ? nsptile_[ntm] : 0,
                  d_sptiles_[0].as<Tile>(), M ? nsptile_[0] : 0, N ? *N : n0, M ? *M : m0);
 }
@@ -1413,8 +1415,8 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   };
   auto residual = [&](bool read) {
     // KF = f − Q x + Bᵀ y (one pass over the rows), then − (cone blocks) x ; KG = g − B x
-    const SpmvN rn{kx, ky, kf, -1, NV_KF, K(-1.0), K(1.0), K(1.0), K(0.0)};
-    const SpmvM rm{kx, kg, MV_KG, -1, K(-1.0), K(1.0), K(0.0)};
+    const SpmvN rn{kx, ky, kf, -1, -1, NV_KF, K(-1.0), K(1.0), K(1.0), K(0.0), K(0.0)};
+    const SpmvM rm{kx, kg, MV_KG, -1, K(-1.0), K(1.0), K(0.0), 0};
     spmv_nm(2, &rn, &rm);
     launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, NV_KF, K(-1.0));
     launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
@@ -1424,9 +1426,12 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   // with irmin ≥ 1 and irmax ≥ 1 the starting residual decides nothing (and its norm enters no
   // later test), so it is not read back
   const bool read0 = irmin == 0 || irmax == 0;
+  // a cold start that refines at least once starts from x = y = 0, whose residual is (f, g): the
+  // first refinement step then reads f and g directly and sets x and y instead of adding to them
+  const bool cold1 = !warm && !read0;
   if (warm) {
     residual(read0);
-  } else {
+  } else if (!cold1) {
     lin2(Lin{kf, kf, kf, NV_KF, K(1.0), K(0.0), K(0.0)}, Lin{kg, kg, kg, MV_KG, K(1.0), K(0.0), K(0.0)});
     lin2(Lin{kx, kx, kx, kx, K(0.0), K(0.0), K(0.0)}, Lin{ky, ky, ky, ky, K(0.0), K(0.0), K(0.0)});
     if (read0) norms();
@@ -1441,13 +1446,15 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
   }
   while (irstat == KKT_CONTINUE) {
     nir += 1;
-    const SpmvN dx{-1, MV_KG, NV_KF, -1, NV_KX, K(0.0), K(1.0), KS(1.0, S_DELTA), K(0.0)};
+    const bool first = cold1 && nir == 1;  // δf = f, δg = g, x = y = 0
+    const SpmvN dx{-1, first ? kg : MV_KG, first ? kf : NV_KF, -1, -1, NV_KX, K(0.0), K(1.0), KS(1.0, S_DELTA),
+                   K(0.0), K(0.0)};
     spmv_nm(1, &dx, nullptr);  // δx = δ δf + Bᵀ δg
     lsolve(NV_KX);
     usolve(NV_KX);
     // δg −= B δx, then y += α δg ; x += δx
-    const SpmvN xu{-1, -1, NV_KX, kx, kx, K(0.0), K(0.0), K(1.0), K(1.0)};
-    const SpmvM gu{NV_KX, MV_KG, MV_KG, ky, K(-1.0), K(1.0), KS(1.0, S_ALPHA)};
+    const SpmvN xu{-1, -1, NV_KX, kx, -1, kx, K(0.0), K(0.0), K(1.0), first ? K(0.0) : K(1.0), K(0.0)};
+    const SpmvM gu{NV_KX, first ? kg : MV_KG, MV_KG, ky, K(-1.0), K(1.0), KS(1.0, S_ALPHA), first ? 1 : 0};
     spmv_nm(1, &xu, &gu);
     residual(true);
     fres = hsc_[S_R10];
@@ -1536,9 +1543,10 @@ bool Solver::isopt(double mu, double mus, double pobj, double dobj, double pres,
 // statistics of the current iterate for the termination and certificate tests
 void Solver::compute_stats() {
   // R0 gy, R1 ‖y‖, R2 ‖Qp‖, R3 ‖d‖, R4 ‖Qp − d − Bᵀy‖, R5 fp, R6 ‖p‖, R7 ‖Bp‖
-  spmv(1, MV_Y, NV_T1, K(1.0), K(0.0));
-  axpbypcz(0, NV_QP, NV_D, NV_T1, NV_T2, K(1.0), K(-1.0), K(-1.0));
-  spmv(0, NV_P, MV_T0, K(1.0), K(0.0));
+  // T2 = Qp − d − Bᵀy ; T0 = B p
+  const SpmvN sn{-1, MV_Y, NV_QP, NV_D, -1, NV_T2, K(0.0), K(-1.0), K(1.0), K(-1.0), K(0.0)};
+  const SpmvM sm{NV_P, -1, MV_T0, -1, K(1.0), K(0.0), K(0.0), 0};
+  spmv_nm(1, &sn, &sm);
   DotBatch b;
   b.add(1, MV_G, MV_Y, 0, S_R0, 0).add(1, MV_Y, MV_Y, 0, S_R1, 1).add(0, NV_QP, NV_QP, 0, S_R2, 1);
   b.add(0, NV_D, NV_D, 0, S_R3, 1).add(0, NV_T2, NV_T2, 0, S_R4, 1).add(0, NV_F, NV_P, 0, S_R5, 0);
@@ -1577,20 +1585,20 @@ int Solver::step() {
   const double mut = mu_t_;
   double delta = getaug();
   row.delta = delta;
-  set_scalar(S_DELTA, delta);
-  set_scalar(S_ALPHA, 1.0 / delta);
+  launch_set_scalar2(st_, d_sc_.as<double>(), S_DELTA, delta, S_ALPHA, 1.0 / delta);
   // pᵀd, scale!, residuals!
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                     d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                      d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
                      d_dpart_.as<double>());
-  spmv(2, NV_P, NV_QP, K(1.0), K(0.0));                                  // Qp
-  spmv(1, MV_Y, NV_T0, K(1.0), K(0.0));                                  // Bᵀy
-  lin2(Lin{NV_D, NV_F, NV_QP, NV_DF, K(1.0), K(1.0), K(-1.0)},          // Δf = d + f − Qp
-       Lin{MV_G, MV_G, MV_G, MV_DG, K(1.0), K(0.0), K(0.0)});           // Δg = g
-  axpby(0, NV_T0, NV_DF, NV_DF, K(1.0), K(1.0));                         //      + Bᵀy
-  spmv(0, NV_P, MV_DG, K(-1.0), K(1.0));                                 //      − Bp
+  spmv(2, NV_P, NV_QP, K(1.0), K(0.0));  // Qp
+  {
+    // Δf = Bᵀy + d + f − Qp ; Δg = g − Bp
+    const SpmvN fn{-1, MV_Y, NV_D, NV_F, NV_QP, NV_DF, K(0.0), K(1.0), K(1.0), K(1.0), K(-1.0)};
+    const SpmvM gm{NV_P, MV_G, MV_DG, -1, K(-1.0), K(1.0), K(0.0), 0};
+    spmv_nm(1, &fn, &gm);
+  }
   {
     DotBatch b;
     b.add(0, NV_P, NV_D, 0, S_TMP0, 0).add(1, MV_DG, MV_YSCL, 1, S_NORM0, 1).add(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
@@ -1670,15 +1678,14 @@ int Solver::step() {
             set_scalar(S_SMU, mut);
             launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
             launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
+            axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
           } else {
-            axpby(0, NV_D, NV_D, NV_FR, K(-1.0), K(0.0));
+            lin(LinSet().add(Lin{NV_DF, NV_D, NV_D, NV_FR, K(1.0), K(-1.0), K(0.0)}), LinSet());  // Δf − d
           }
-          axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
           KKTOut po = solvekkt(NV_DPA, MV_DYA, NV_FR, MV_DG, false, ftol, gtol, 1, false);
           // Δda = Q Δpa − Bᵀ Δya − Δf
-          spmv(1, MV_DYA, NV_T0, K(1.0), K(0.0));
-          spmv(2, NV_DPA, NV_DDA, K(1.0), K(0.0));
-          axpbypcz(0, NV_T0, NV_DF, NV_DDA, NV_DDA, K(-1.0), K(-1.0), K(1.0));
+          const SpmvN da{NV_DPA, MV_DYA, NV_DF, -1, -1, NV_DDA, K(1.0), K(-1.0), K(-1.0), K(0.0), K(0.0)};
+          spmv_nm(2, &da, nullptr);
           launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DDA);
           // corrector (solver/ipm.jl:181-272)
           // τa stays on the device: the dot reads it there, and one read-back brings μa home
@@ -1702,13 +1709,14 @@ int Solver::step() {
           launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
           launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
                           NV_DDA);
-          axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
-          axpby(0, NV_DPA, NV_DPA, NV_DP, K(1.0), K(0.0));
-          axpby(1, MV_DYA, MV_DYA, MV_DY, K(1.0), K(0.0));
+          // corrector right-hand side Δf + fr, warm start (Δpa, Δya)
+          lin(LinSet()
+                  .add(Lin{NV_DF, NV_FR, NV_FR, NV_FR, K(1.0), K(1.0), K(0.0)})
+                  .add(Lin{NV_DPA, NV_DPA, NV_DPA, NV_DP, K(1.0), K(0.0), K(0.0)}),
+              LinSet().add(Lin{MV_DYA, MV_DYA, MV_DYA, MV_DY, K(1.0), K(0.0), K(0.0)}));
           KKTOut co = solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, true, ftol, gtol, 1, false);
-          spmv(1, MV_DY, NV_T0, K(1.0), K(0.0));
-          spmv(2, NV_DP, NV_DD, K(1.0), K(0.0));
-          axpbypcz(0, NV_T0, NV_DF, NV_DD, NV_DD, K(-1.0), K(-1.0), K(1.0));
+          const SpmvN dd{NV_DP, MV_DY, NV_DF, -1, -1, NV_DD, K(1.0), K(-1.0), K(-1.0), K(0.0), K(0.0)};
+          spmv_nm(2, &dd, nullptr);
           launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DD);
           row.piter = po.ncg;
           row.ppass = po.nir;
@@ -1735,10 +1743,10 @@ int Solver::step() {
             row.step = stp;
             // a collapsing step with an FP32 factor: its direction is not accurate enough here
             if (fac_fp32_ && stp < MIXED_MIN_STEP) mixed_off_ = true;
-            set_scalar(S_STEP, stp);
-            axpby(0, NV_DP, NV_P, NV_P, KS(1.0, S_STEP), K(1.0));
-            axpby(0, NV_DD, NV_D, NV_D, KS(1.0, S_STEP), K(1.0));
-            axpby(1, MV_DY, MV_Y, MV_Y, KS(1.0, S_STEP), K(1.0));
+            lin(LinSet()
+                    .add(Lin{NV_DP, NV_P, NV_P, NV_P, K(stp), K(1.0), K(0.0)})
+                    .add(Lin{NV_DD, NV_D, NV_D, NV_D, K(stp), K(1.0), K(0.0)}),
+                LinSet().add(Lin{MV_DY, MV_Y, MV_Y, MV_Y, K(stp), K(1.0), K(0.0)}));
           }
           break;
         }

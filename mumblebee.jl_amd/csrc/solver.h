@@ -153,6 +153,7 @@ class Solver {
   void dot(int space, int kx, int ky, int mode, int slot, int op, int kw0 = 0, int kw1 = 0);
   void dots(const DotBatch& b);
   void lin2(const Lin& on, const Lin& om);
+  void lin(const LinSet& N, const LinSet& M);
   void spmv_nm(int ntm, const SpmvN* N, const SpmvM* M);
   void spmv(int which, int kx, int ky, Coef a, Coef b);
   void hmul(int kx, int ky);  // ky = H kx
