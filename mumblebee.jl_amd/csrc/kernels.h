@@ -90,11 +90,13 @@ void launch_lin(hipStream_t, const InstDev*, const Tile* ntiles, int nnt, const 
                 const LinSet& N, const LinSet& M);
 void launch_spmv(hipStream_t, const InstDev*, const Tile*, int nt, int which, int kx, int ky, Coef a, Coef b);
 // Sparse products over the n rows and over the m rows in one launch. n rows (the row tiles of
-// matrix ntm: 1 = Bᵀ, 2 = Q): y = aq Q xq + at Bᵀ xt + c z + d w + e v; m rows (the row tiles
-// of B): y = a B x + c z, then y2 += e y (y2 = e y if y2set). A term whose kind is < 0 is absent.
+// matrix ntm: 1 = Bᵀ, 2 = Q): y = aq Q xq + at Bᵀ xt + c z + d w + e v, and Q xq alone into kyq;
+// m rows (the row tiles of B): y = a B x + c z, then y2 += e y (y2 = e y if y2set). A term or
+// output whose kind is < 0 is absent.
 struct SpmvN {
   int kxq, kxt, kz, kw, kv, ky;
   Coef aq, at, c, d, e;
+  int kyq = -1;
 };
 struct SpmvM {
   int kx, kz, ky, ky2;

@@ -318,6 +318,7 @@ __global__ void __launch_bounds__(TILE) k_spmv_nm(const InstDev* insts, int ntm,
       if (N.kw >= 0) y += coefterm(N.d, I.sc, nvec(I, N.kw), i);
       if (N.kv >= 0) y += coefterm(N.e, I.sc, nvec(I, N.kv), i);
       nvec(I, N.ky)[i] = y;
+      if (N.kyq >= 0) nvec(I, N.kyq)[i] = sq;
     }
   } else {
     const int G = I.B.lanes;

@@ -800,6 +800,8 @@ template <class TF>
 __global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* insts) {
   __shared__ __align__(16) double SX[NB * LDT];
   __shared__ __align__(16) double SY[NB * LDT];
+  // the factorization that just ended took its last ticket: the next one starts from 0
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
   // workgroups [0, Tr) form M_b, workgroups [Tr, 2 Tr) form N_b
@@ -1013,14 +1015,14 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
 
 // ------------------------------------------------------------------------------------------
 
+// The ticket starts at 0 (zeroed with the front's buffers, then put back by k_big_mtiles after
+// each factorization); the abort flag stays raised once set.
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
-  k_fill_int<<<1, 64, 0, st>>>(B.ticket, 1, 0);  // the ticket (the abort flag stays raised once set)
   if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
   else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
-  if (B.Tr > 1) {
-    if (fp32) k_big_mtiles<float><<<2 * B.Tr, NT, 0, st>>>(B, I);
-    else k_big_mtiles<double><<<2 * B.Tr, NT, 0, st>>>(B, I);
-  }
+  const int mgrid = std::max(1, 2 * B.Tr);
+  if (fp32) k_big_mtiles<float><<<mgrid, NT, 0, st>>>(B, I);
+  else k_big_mtiles<double><<<mgrid, NT, 0, st>>>(B, I);
 }
 
 int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,

@@ -1046,8 +1046,7 @@ bool Solver::factor(double delta, double& rho) {
     launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(),
                           ncone_, cone_nmax_);
   };
-  set_scalar(S_DELTA, delta);
-  set_scalar(S_FAIL, 0.0);
+  launch_set_scalar2(st_, d_sc_.as<double>(), S_DELTA, delta, S_FAIL, 0.0);
   assemble();
   // mixed precision: the large fronts in FP32 while μ is above the switch point; an FP32
   // breakdown (non-positive pivot) is redone in FP64 before any diagonal shift
@@ -1335,6 +1334,12 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
 // KKT: Uzawa refinement + dual CG (uzawa.jl:319-507, cg.jl:69-167)
 // ------------------------------------------------------------------------------------------
 
+// whether cg below takes a step, from the same read-back values
+bool Solver::cg_runs(double atol, int itmax, bool stops) const {
+  const bool stop = stops && hsc_[S_R9] / (1.0 + sg_) <= set_.feas_tol;
+  return itmax > 0 && std::sqrt(hsc_[S_R8]) > atol && !stop;
+}
+
 int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& status) {
   // cg.jl:69-167 with p = MV_CP, w = NV_CW. On entry hsc_[S_R8] holds ‖u‖² and hsc_[S_R9] its
   // scaled stop norm (read back with the last Uzawa residual)
@@ -1343,16 +1348,17 @@ int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& 
   double chist0 = nu;
   std::vector<double>& ch = cghist_;
   ch.assign(1, chist0);
-  axpby(1, ku, ku, ky, K(0.0), K(0.0));  // y ← 0
-  status = 1;                             // CG_SOLVED
+  status = 1;  // CG_SOLVED
   auto stopf = [&](double scaled) { return stops && scaled / (1.0 + sg_) <= set_.feas_tol; };
   if (nu > atol && !stopf(hsc_[S_R9])) {
     status = 3;  // CG_ITMAX
     if (itmax > 0) {
-      axpby(1, ku, ku, MV_CP, K(1.0), K(0.0));  // p ← u
+      // y ← 0, p ← u (y is read only after a step, so it is cleared only when one is taken)
+      lin(LinSet(), LinSet()
+                        .add(Lin{ku, ku, ku, MV_CP, K(1.0), K(0.0), K(0.0)})
+                        .add(Lin{ku, ku, ku, ky, K(0.0), K(0.0), K(0.0)}));
       status = 0;
-      set_scalar(S_CG_NU2, nu2);
-      set_scalar(S_CG_NP2, np2);
+      launch_set_scalar2(st_, d_sc_.as<double>(), S_CG_NU2, nu2, S_CG_NP2, np2);
       while (status == 0) {
         niter += 1;
         spmv(1, MV_CP, NV_CW, K(1.0), K(0.0));  // w = Bᵀ p
@@ -1465,11 +1471,14 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
     else irstat = KKT_CONTINUE;
     fprv = fres;
   }
-  axpby(1, MV_KG, ky, ky, KS(-1.0, S_ALPHA), K(1.0));  // y −= α δg
+  // the dual CG on the last residual: y −= α δg, CG, y += α δy + α δg. When CG takes no step
+  // (δy = 0, δg unchanged) that is y itself, and nothing is launched.
+  const bool cgruns = cg_runs(gtol, cgmax, stops);
+  if (cgruns) axpby(1, MV_KG, ky, ky, KS(-1.0, S_ALPHA), K(1.0));
   int cgstat = 1;
   int ncg = cg(kx, MV_KY, MV_KG, gtol, cgmax, stops, cgstat);
   check_abort();
-  axpbypcz(1, MV_KY, MV_KG, ky, ky, KS(1.0, S_ALPHA), KS(1.0, S_ALPHA), K(1.0));  // y += α δy + α δg
+  if (cgruns) axpbypcz(1, MV_KY, MV_KG, ky, ky, KS(1.0, S_ALPHA), KS(1.0, S_ALPHA), K(1.0));
   KKTOut o;
   o.ncg = ncg;
   o.nir = nir;
@@ -1592,12 +1601,12 @@ int Solver::step() {
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                      d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
                      d_dpart_.as<double>());
-  spmv(2, NV_P, NV_QP, K(1.0), K(0.0));  // Qp
   {
-    // Δf = Bᵀy + d + f − Qp ; Δg = g − Bp
-    const SpmvN fn{-1, MV_Y, NV_D, NV_F, NV_QP, NV_DF, K(0.0), K(1.0), K(1.0), K(1.0), K(-1.0)};
+    // Qp ; Δf = d + f − Qp + Bᵀy ; Δg = g − Bp
+    SpmvN fn{NV_P, MV_Y, NV_D, NV_F, -1, NV_DF, K(-1.0), K(1.0), K(1.0), K(1.0), K(0.0)};
+    fn.kyq = NV_QP;
     const SpmvM gm{NV_P, MV_G, MV_DG, -1, K(-1.0), K(1.0), K(0.0), 0};
-    spmv_nm(1, &fn, &gm);
+    spmv_nm(2, &fn, &gm);
   }
   {
     DotBatch b;

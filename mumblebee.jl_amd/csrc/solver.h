@@ -155,6 +155,7 @@ class Solver {
   void lin2(const Lin& on, const Lin& om);
   void lin(const LinSet& N, const LinSet& M);
   void spmv_nm(int ntm, const SpmvN* N, const SpmvM* M);
+  bool cg_runs(double atol, int itmax, bool stops) const;
   void spmv(int which, int kx, int ky, Coef a, Coef b);
   void hmul(int kx, int ky);  // ky = H kx
   void lsolve(int kv);
