@@ -60,7 +60,7 @@ void launch_dot(hipStream_t, const InstDev*, const Tile*, int nt, const int* til
 struct DotSpec {
   int space, kx, ky, mode, kw0, kw1, slot, op;
 };
-constexpr int DOTS_MAX = 8;
+constexpr int DOTS_MAX = 16;
 constexpr int DOTS_LEN_MAX = 1 << 14;
 struct DotBatch {
   DotSpec s[DOTS_MAX];
@@ -153,5 +153,7 @@ void launch_cg_scalars(hipStream_t, const InstDev*, int ninst, int op);
 void launch_probe_mfma(hipStream_t, double* out, bool fp32);
 void launch_set_scalar(hipStream_t, double* sc, int slot, double v);
 void launch_set_scalar2(hipStream_t, double* sc, int s1, double v1, int s2, double v2);
+// the corrector's σμ (solver/ipm.jl:181-200) from μa = sc[S_TMP2]/ν, into sc[S_SMU]
+void launch_smu(hipStream_t, double* sc, double nu, double mu, double mut);
 
 }  // namespace mb

@@ -104,7 +104,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // partial[tile] = Σ x⊙y (mode 0), Σ (x/s)² with s = kind ky (mode 1), Σ x (mode 2),
-// Σ (x + τ dx)(y + τ dy) with τ = S_TAU (mode 3: x=P, y=D, extra kinds in kw0/kw1)
+// Σ (x + τ dx)(y + τ dy) with τ = S_TAU (mode 3: x=P, y=D, extra kinds in kw0/kw1), Σ (x − y)²
+// (mode 4)
 __global__ void k_dot_partial(const InstDev* insts, const Tile* tiles, int space, int kx, int ky,
                               int mode, int kw0, int kw1, double* partial) {
   __shared__ double red[TILE / 64];
@@ -121,6 +122,9 @@ __global__ void k_dot_partial(const InstDev* insts, const Tile* tiles, int space
       v = q * q;
     } else if (mode == 2) {
       v = x[i];
+    } else if (mode == 4) {
+      const double q = x[i] - vecof(I, space, ky)[i];
+      v = q * q;
     } else {
       double tau = I.sc[S_TAU];
       double a = x[i] + tau * vecof(I, space, kw0)[i];
@@ -152,6 +156,9 @@ __global__ void k_dot(const InstDev* insts, const Tile* tiles, const int* tile_o
       v = q * q;
     } else if (mode == 2) {
       v = x[i];
+    } else if (mode == 4) {
+      const double q = x[i] - vecof(I, space, ky)[i];
+      v = q * q;
     } else {
       const double tau = I.sc[S_TAU];
       const double a = x[i] + tau * vecof(I, space, kw0)[i];
@@ -200,6 +207,11 @@ __global__ void __launch_bounds__(DOTS_THREADS) k_dots(const InstDev* insts, int
     }
   } else if (d.mode == 2) {
     for (int i = threadIdx.x; i < len; i += DOTS_THREADS) v += x[i];
+  } else if (d.mode == 4) {
+    for (int i = threadIdx.x; i < len; i += DOTS_THREADS) {
+      const double q = x[i] - y[i];
+      v += q * q;
+    }
   } else {
     const double tau = I.sc[S_TAU];
     const double* dx = vecof(I, d.space, d.kw0);
@@ -1290,6 +1302,28 @@ __global__ void k_set_scalar2(double* sc, int s1, double v1, int s2, double v2) 
 }
 void launch_set_scalar2(hipStream_t st, double* sc, int s1, double v1, int s2, double v2) {
   k_set_scalar2<<<1, 1, 0, st>>>(sc, s1, v1, s2, v2);
+}
+
+// solver/ipm.jl:181-200: σμ from the affine centrality μa, on the device so that the corrector's
+// right-hand side needs no read-back
+__global__ void k_smu(double* sc, double nu, double mu, double mut) {
+  double smu = sc[S_TMP2];
+  const double mua = smu / nu;
+  if (mut > 0) {
+    const double dmu = mu - mut;
+    if (dmu > 0) {
+      const double dmua = fmin(fmax(mua - mut, 0.0), dmu);
+      smu = mut + fmin(fmax(dmua * (dmua / dmu) * (dmua / dmu), 0.0), dmu);
+    } else {
+      smu = mut;
+    }
+  } else {
+    smu = fmin(fmax(mua * (mua / mu) * (mua / mu), 0.0), mu);
+  }
+  sc[S_SMU] = smu;
+}
+void launch_smu(hipStream_t st, double* sc, double nu, double mu, double mut) {
+  k_smu<<<1, 1, 0, st>>>(sc, nu, mu, mut);
 }
 void launch_set_scalar(hipStream_t st, double* sc, int slot, double v) { k_set_scalar<<<1, 1, 0, st>>>(sc, slot, v); }
 

@@ -1325,6 +1325,7 @@ void Solver::reinit(const double* p0, const double* d0, const double* y0) {
   HIPCHK(hipStreamSynchronize(st_));
   hist_.clear();
   hist_fp32_.clear();
+  step_pending_ = false;
   nH_ = norm2(P_.Q.val);
   status_ = CONTINUE;
   mixed_off_ = false;
@@ -1588,6 +1589,24 @@ int Solver::nearstatus(int status, double mu, double mus, double pobj, double do
   return status;
 }
 
+// The last update's step length, from a read-back made after it: into its history row, and a
+// collapsing step with an FP32 factor (its direction was not accurate enough) ends mixed precision
+void Solver::settle_step() {
+  if (!step_pending_) return;
+  step_pending_ = false;
+  const double stp = hsc_[S_TAU];
+  if (!hist_.empty()) hist_.back().step = stp;
+  if (step_fp32_ && stp < MIXED_MIN_STEP) mixed_off_ = true;
+}
+
+const std::vector<HistRow>& Solver::history() {
+  if (step_pending_) {
+    sync_scalars();
+    settle_step();
+  }
+  return hist_;
+}
+
 int Solver::step() {
   int status = CONTINUE;
   HistRow row{NaN, NaN, NaN, NaN, 0.0, 0, 0, KKT_SOLVED, 0, 0, KKT_SOLVED, NaN, NaN, NaN, 0.0, NaN, NaN};
@@ -1609,12 +1628,18 @@ int Solver::step() {
     spmv_nm(2, &fn, &gm);
   }
   {
+    // the residual norms and objectives, and the statistics of the termination and certificate
+    // tests (compute_stats) of the same iterate, in one launch and one read-back
     DotBatch b;
     b.add(0, NV_P, NV_D, 0, S_TMP0, 0).add(1, MV_DG, MV_YSCL, 1, S_NORM0, 1).add(0, NV_DF, NV_PSCL, 1, S_NORM1, 1);
     b.add(0, NV_P, NV_QP, 0, S_NORM2, 0).add(0, NV_F, NV_P, 0, S_NORM3, 0).add(1, MV_G, MV_Y, 0, S_NORM4, 0);
+    b.add(1, MV_G, MV_Y, 0, S_R0, 0).add(1, MV_Y, MV_Y, 0, S_R1, 1).add(0, NV_QP, NV_QP, 0, S_R2, 1);
+    b.add(0, NV_D, NV_D, 0, S_R3, 1).add(0, NV_F, NV_DF, 4, S_R4, 1).add(0, NV_F, NV_P, 0, S_R5, 0);
+    b.add(0, NV_P, NV_P, 0, S_R6, 1).add(1, MV_G, MV_DG, 4, S_R7, 1);
     dots(b);
   }
   sync_scalars();
+  settle_step();
   double pd = hsc_[S_TMP0];
   bool flag = hsc_[S_FLAGOK] != 0.0;
   double spsd = hsc_[S_SPSD];
@@ -1635,7 +1660,7 @@ int Solver::step() {
     row.dres = dres;
     row.pobj = pobj;
     row.dobj = dobj;
-    compute_stats();
+    // S_R0..S_R7 came with the read-back above: ‖f − Δf‖ = ‖Qp − d − Bᵀy‖, ‖g − Δg‖ = ‖Bp‖
     if (isopt(mu, mus, pobj, dobj, pres, dres, 1.0)) {
       status = OPTIMAL;
     } else if (isprimalinfeasible(set_.infeas_rel, set_.infeas_abs)) {
@@ -1700,21 +1725,7 @@ int Solver::step() {
           // τa stays on the device: the dot reads it there, and one read-back brings μa home
           launch_maxsteps(NV_DPA, NV_DDA, 1.0);
           dots(DotBatch().add(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA));
-          sync_scalars();
-          double smu = hsc_[S_TMP2];
-          double mua = smu / nu_;
-          if (mut > 0) {
-            double dmu = mu - mut;
-            if (dmu > 0) {
-              double dmua = std::min(std::max(mua - mut, 0.0), dmu);
-              smu = mut + std::min(std::max(dmua * (dmua / dmu) * (dmua / dmu), 0.0), dmu);
-            } else {
-              smu = mut;
-            }
-          } else {
-            smu = std::min(std::max(mua * (mua / mu) * (mua / mu), 0.0), mu);
-          }
-          set_scalar(S_SMU, smu);
+          launch_smu(st_, d_sc_.as<double>(), (double)nu_, mu, mut);  // σμ into S_SMU
           launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
           launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
                           NV_DDA);
@@ -1748,14 +1759,15 @@ int Solver::step() {
             compute_stats();
             status = nearstatus(NUMERICAL_FAILURE, mu, mus, pobj, dobj, pres, dres);
           } else {
-            double stp = maxsteps(NV_DP, NV_DD, set_.step_frac);
-            row.step = stp;
-            // a collapsing step with an FP32 factor: its direction is not accurate enough here
-            if (fac_fp32_ && stp < MIXED_MIN_STEP) mixed_off_ = true;
+            // the step length stays on the device (S_TAU) for the update; the host reads it with
+            // the next read-back (settle_step)
+            launch_maxsteps(NV_DP, NV_DD, set_.step_frac);
             lin(LinSet()
-                    .add(Lin{NV_DP, NV_P, NV_P, NV_P, K(stp), K(1.0), K(0.0)})
-                    .add(Lin{NV_DD, NV_D, NV_D, NV_D, K(stp), K(1.0), K(0.0)}),
-                LinSet().add(Lin{MV_DY, MV_Y, MV_Y, MV_Y, K(stp), K(1.0), K(0.0)}));
+                    .add(Lin{NV_DP, NV_P, NV_P, NV_P, KS(1.0, S_TAU), K(1.0), K(0.0)})
+                    .add(Lin{NV_DD, NV_D, NV_D, NV_D, KS(1.0, S_TAU), K(1.0), K(0.0)}),
+                LinSet().add(Lin{MV_DY, MV_Y, MV_Y, MV_Y, KS(1.0, S_TAU), K(1.0), K(0.0)}));
+            step_pending_ = true;
+            step_fp32_ = fac_fp32_;
           }
           break;
         }

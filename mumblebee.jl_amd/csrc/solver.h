@@ -101,7 +101,7 @@ class Solver {
   int step();
   int solve();
   void result(double* p, double* d, double* y, double* info);  // info: mu pres dres pobj dobj niter nsolve status
-  const std::vector<HistRow>& history() const { return hist_; }
+  const std::vector<HistRow>& history();  // settles a pending step length first
   // diagnostics
   double exact_flops() const { return sym_.exact_flops; }
   double front_flops() const;
@@ -199,6 +199,9 @@ class Solver {
   std::vector<double> pscl_, yscl_;
   double nB_ = 0, nf_ = 0, ng_ = 0, sf_ = 0, sg_ = 0, nH_ = 0, qoff2_ = 0;
   std::vector<HistRow> hist_;
+  bool step_pending_ = false;  // the last update's step length is on the device only (S_TAU)
+  bool step_fp32_ = false;     // that update's direction came from an FP32 factor
+  void settle_step();
   std::vector<double> cghist_;
   int status_ = CONTINUE;
 
