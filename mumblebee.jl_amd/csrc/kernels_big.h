@@ -37,6 +37,8 @@ struct BigFront {
 // fp32: the factor in FP32 on FP32 MFMA (B.F32 / B.Dinv32 must be allocated); the solves read
 // the factor of the same precision and compute in FP64
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32);
+// after each launch_dag_factor: the solves' chain-step tiles, and the factorization ticket back to 0
+void launch_big_mtiles(hipStream_t st, const BigFront& B, const InstDev* I, bool fp32);
 // the solve launchers return their grid size: every workgroup consumes one ticket of B.sticket,
 // so the caller advances its ticket base by that many
 int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const int* childlist,

@@ -1016,10 +1016,13 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
 // ------------------------------------------------------------------------------------------
 
 // The ticket starts at 0 (zeroed with the front's buffers, then put back by k_big_mtiles after
-// each factorization); the abort flag stays raised once set.
+// each factorization, which launch_big_mtiles must follow); the abort flag stays raised once set.
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
   if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
   else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
+}
+
+void launch_big_mtiles(hipStream_t st, const BigFront& B, const InstDev* I, bool fp32) {
   const int mgrid = std::max(1, 2 * B.Tr);
   if (fp32) k_big_mtiles<float><<<mgrid, NT, 0, st>>>(B, I);
   else k_big_mtiles<double><<<mgrid, NT, 0, st>>>(B, I);

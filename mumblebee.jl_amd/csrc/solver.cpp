@@ -969,6 +969,7 @@ void Solver::run_factor(bool fp32) {
       int pd = prof_begin(PROF_DAG);
       for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_, fac_epoch_, fp32);
       prof_end(pd, PROF_DAG, L.big_flops, 0.0);
+      for (int b : L.big) launch_big_mtiles(st_, big_[b]->B, I, fp32);
     }
   }
   prof_end(pf, PROF_FACTOR, sym_.exact_flops, 0.0);
