@@ -38,7 +38,7 @@ PEAK_HBM = 8000.0      # GB/s
 METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP/s"
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # scaled by a 256 MiB device copy (profiles/k_dag_factor_qp2000_fp64.md); keyed by (n, m)
-TRAFFIC_BYTES = {(2000, 1000): 142.7e6}
+TRAFFIC_BYTES = {(2000, 1000): 141.5e6}
 
 
 WORKLOADS = {

@@ -1253,7 +1253,7 @@ void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc
   if (nc) k_zerofree<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx);
 }
 void launch_fill(hipStream_t st, double* x, size_t n, double v) {
-  if (n) k_fill<<<std::min(cdiv((long)(n / 2 + 1), TILE), 2048), TILE, 0, st>>>(x, n, v);
+  if (n) k_fill<<<std::min(cdiv((long)(n / 2 + 1), TILE), 16384), TILE, 0, st>>>(x, n, v);
 }
 void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent) {
