@@ -745,7 +745,8 @@ __global__ void k_zerofree(const InstDev* insts, const ConeDesc* cones, int ncon
 // assembly: F ← 0; F[pos] = L + δ Q on the union pattern; F[diag blocks] += δ D
 // ------------------------------------------------------------------------------------------
 
-// x[0, n) = v with 16-byte stores, grid-strided (x 16-byte aligned)
+// x[0, n) = v with 16-byte stores, one per thread (grid-strided if the grid is smaller; x 16-byte
+// aligned)
 __global__ void k_fill(double* x, size_t n, double v) {
   typedef double d2 __attribute__((ext_vector_type(2)));
   const size_t np = n / 2, stride = (size_t)gridDim.x * blockDim.x;
@@ -1253,7 +1254,7 @@ void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc
   if (nc) k_zerofree<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx);
 }
 void launch_fill(hipStream_t st, double* x, size_t n, double v) {
-  if (n) k_fill<<<std::min(cdiv((long)(n / 2 + 1), TILE), 16384), TILE, 0, st>>>(x, n, v);
+  if (n) k_fill<<<std::max(cdiv((long)(n / 2), TILE), 1), TILE, 0, st>>>(x, n, v);
 }
 void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent) {

@@ -940,8 +940,8 @@ __global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc*
 // backward: column block b = Tr−1−ticket accumulates L[j, b]ᵀ x_j for the complete j > b+1:
 // thread (r, q) sums L[j, b][r][q + 4i] x_j[r] over the blocks (in descending order whatever the
 // batches), one transpose through LDS then sums over r. With L[sep, b]ᵀ x_sep (the chunk partials
-// of k_big_sepgemv, in chunk order) it forms w_b = L_bbᵀ⁻¹ (v_b − Σ) before block b+1 arrives,
-// and wave 0 finishes x_b = w_b − N_bᵀ x_{b+1} after it.
+// of k_big_sepgemv) it forms w_b = L_bbᵀ⁻¹ (v_b − Σ) before block b+1 arrives, and wave 0
+// finishes x_b = w_b − N_bᵀ x_{b+1} after it.
 template <class TF>
 __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
                                                    int kv, int epoch, long long base) {
@@ -964,6 +964,14 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  // L[sep, b]ᵀ x_sep: the chunk partials of k_big_sepgemv, chunks c ≡ q (mod 4) by wave q in
+  // chunk order, read while the later blocks solve
+  double sep = 0.0;
+  {
+    const int nch = (LD - Fd.r + NB - 1) / NB;
+#pragma unroll 4
+    for (int c = q; c < nch; c += 4) sep += B.spart[((size_t)c * B.Tr + b) * NB + lane];
+  }
   const int jlo = b + 2;  // blocks jlo..Tr−1 are summed before the last step
   double av[16];
 #pragma unroll
@@ -990,10 +998,7 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   double acc = 0.0;
 #pragma unroll
   for (int m = 0; m < 16; ++m) acc += S[(q + 4 * m) + lane * LDT];
-  if (q == 0) {
-    const int nch = (LD - Fd.r + NB - 1) / NB;
-    for (int c = 0; c < nch; ++c) acc += B.spart[((size_t)c * B.Tr + b) * NB + lane];
-  }
+  acc += sep;
   // w_b = L_bbᵀ⁻¹ (v_b − Σ), before block b+1 arrives
   part[q][lane] = acc;
   __syncthreads();
