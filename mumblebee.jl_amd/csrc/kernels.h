@@ -35,6 +35,10 @@ struct Tile {
   int start;
 };
 constexpr int TILE_THREADS = 256;  // threads of a tile-list workgroup
+// A sparse-product tile list holds tiles of 256/lanes consecutive rows (start ≥ 0) and, after them,
+// one tile per row longer than SPMV_LONG × lanes (start = −1 − row), which a whole workgroup sums;
+// the row tiles leave such rows to it.
+constexpr int SPMV_LONG = 64;
 
 // coefficient value: c (slot < 0), c·sc[slot], or c/sc[slot] (inv)
 struct Coef {

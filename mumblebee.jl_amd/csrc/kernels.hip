@@ -270,8 +270,22 @@ __global__ void k_reduce_slot(const InstDev* insts, const int* tile_ofs, const d
 // summing a lane-strided slice of it, and a fixed xor tree combines the G partial sums (the
 // result does not depend on timing). Every lane runs the same trip count of the row loop so
 // the shuffles see whole groups.
+// row i of A against x by the whole workgroup (thread-strided, then the fixed block tree); the
+// sum is in thread 0
+__device__ __forceinline__ double csr_row_block(const CSRDev& A, int i, const double* x, double* red) {
+  double acc = 0.0;
+  for (int k = A.ptr[i] + (int)threadIdx.x; k < A.ptr[i + 1]; k += (int)blockDim.x) acc += A.val[k] * x[A.col[k]];
+  return block_sum(acc, red);
+}
+
+// whether row i of A is left to a workgroup of its own (see SPMV_LONG)
+__device__ __forceinline__ bool csr_long(const CSRDev& A, int i) {
+  return A.ptr[i + 1] - A.ptr[i] > SPMV_LONG * A.lanes;
+}
+
 __global__ void __launch_bounds__(TILE) k_spmv(const InstDev* insts, const Tile* tiles, int which, int kx, int ky,
                                                Coef a, Coef b) {
+  __shared__ double red[TILE / 64];
   const Tile t = tiles[blockIdx.x];
   const InstDev& I = insts[t.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
@@ -279,11 +293,20 @@ __global__ void __launch_bounds__(TILE) k_spmv(const InstDev* insts, const Tile*
   const int xspace = which == 0 ? 0 : (which == 1 ? 1 : 0);
   const int yspace = which == 0 ? 1 : 0;
   const int G = A.lanes;
-  const int lane = threadIdx.x & (G - 1);
-  const int i = t.start + (int)threadIdx.x / G;
-  const bool ok = i < A.nrow;
   const double* x = vecof(I, xspace, kx);
   double* y = vecof(I, yspace, ky);
+  if (t.start < 0) {  // one long row
+    const int i = -1 - t.start;
+    const double acc = csr_row_block(A, i, x, red);
+    if (threadIdx.x == 0) {
+      const double av = coefval(a, I.sc), bv = coefval(b, I.sc);
+      y[i] = av * acc + (bv == 0.0 ? 0.0 : bv * y[i]);
+    }
+    return;
+  }
+  const int lane = threadIdx.x & (G - 1);
+  const int i = t.start + (int)threadIdx.x / G;
+  const bool ok = i < A.nrow && !csr_long(A, i);
   double acc = 0.0;
   const int k0 = ok ? A.ptr[i] : 0, k1 = ok ? A.ptr[i + 1] : 0;
   for (int k = k0 + lane; k < k1; k += G) acc += A.val[k] * x[A.col[k]];
@@ -309,19 +332,35 @@ __device__ __forceinline__ double coefterm(const Coef& c, const double* sc, cons
   return cv == 0.0 ? 0.0 : cv * v[i];
 }
 
-// blocks [0, nn): n rows in the tiles of matrix ntm; the rest: m rows in the tiles of B
+// blocks [0, nn): n rows in the tiles of matrix ntm; the rest: m rows in the tiles of B. A long-row
+// tile (start < 0) sums its row with the whole workgroup; thread 0 then writes.
 __global__ void __launch_bounds__(TILE) k_spmv_nm(const InstDev* insts, int ntm, const Tile* ntiles, int nn,
                                                   const Tile* mtiles, SpmvN N, SpmvM M) {
+  __shared__ double red[TILE / 64];
   const bool isn = (int)blockIdx.x < nn;
   const Tile t = isn ? ntiles[blockIdx.x] : mtiles[blockIdx.x - nn];
   const InstDev& I = insts[t.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
+  const bool lrow = t.start < 0;
   if (isn) {
-    const int G = (ntm == 2 ? I.Q : I.Bt).lanes;
-    const int lane = threadIdx.x & (G - 1), i = t.start + (int)threadIdx.x / G;
-    const bool ok = i < I.Q.nrow;
-    const double sq = N.kxq >= 0 ? csr_row(I.Q, i, ok, lane, G, nvec(I, N.kxq)) : 0.0;
-    const double st = N.kxt >= 0 ? csr_row(I.Bt, i, ok, lane, G, mvec(I, N.kxt)) : 0.0;
+    const CSRDev& At = ntm == 2 ? I.Q : I.Bt;
+    const int G = At.lanes;
+    int lane, i;
+    bool ok;
+    double sq = 0.0, st = 0.0;
+    if (lrow) {
+      lane = (int)threadIdx.x;
+      i = -1 - t.start;
+      ok = true;
+      if (N.kxq >= 0) sq = csr_row_block(I.Q, i, nvec(I, N.kxq), red);
+      if (N.kxt >= 0) st = csr_row_block(I.Bt, i, mvec(I, N.kxt), red);
+    } else {
+      lane = threadIdx.x & (G - 1);
+      i = t.start + (int)threadIdx.x / G;
+      ok = i < At.nrow && !csr_long(At, i);
+      if (N.kxq >= 0) sq = csr_row(I.Q, i, ok, lane, G, nvec(I, N.kxq));
+      if (N.kxt >= 0) st = csr_row(I.Bt, i, ok, lane, G, mvec(I, N.kxt));
+    }
     if (ok && lane == 0) {
       double y = 0.0;
       if (N.kxq >= 0) y += coefval(N.aq, I.sc) * sq;
@@ -334,9 +373,20 @@ __global__ void __launch_bounds__(TILE) k_spmv_nm(const InstDev* insts, int ntm,
     }
   } else {
     const int G = I.B.lanes;
-    const int lane = threadIdx.x & (G - 1), i = t.start + (int)threadIdx.x / G;
-    const bool ok = i < I.B.nrow;
-    const double s = csr_row(I.B, i, ok, lane, G, nvec(I, M.kx));
+    int lane, i;
+    bool ok;
+    double s;
+    if (lrow) {
+      lane = (int)threadIdx.x;
+      i = -1 - t.start;
+      ok = true;
+      s = csr_row_block(I.B, i, nvec(I, M.kx), red);
+    } else {
+      lane = threadIdx.x & (G - 1);
+      i = t.start + (int)threadIdx.x / G;
+      ok = i < I.B.nrow && !csr_long(I.B, i);
+      s = csr_row(I.B, i, ok, lane, G, nvec(I, M.kx));
+    }
     if (ok && lane == 0) {
       double y = coefval(M.a, I.sc) * s;
       if (M.kz >= 0) y += coefterm(M.c, I.sc, mvec(I, M.kz), i);

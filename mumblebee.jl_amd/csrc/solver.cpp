@@ -591,6 +591,8 @@ void Solver::build_device() {
     lanes[which] = G;
     std::vector<Tile> t;
     for (i64 s = 0; s < nrow; s += TILE_THREADS / G) t.push_back(Tile{0, (int)s});
+    for (i64 r = 0; r < nrow; ++r)  // rows the row tiles leave to a workgroup of their own
+      if (p[r + 1] - p[r] > SPMV_LONG * G) t.push_back(Tile{0, (int)(-1 - r)});
     nsptile_[which] = (int)t.size();
     if (!t.empty()) d_sptiles_[which].upload(t, st_);
   };
@@ -886,10 +888,14 @@ void Solver::lin(const LinSet& N, const LinSet& M) {
 
 void Solver::lin2(const Lin& on, const Lin& om) { lin(LinSet().add(on), LinSet().add(om)); }
 
-// n-row and m-row sparse products in one launch (either may be absent)
+// n-row and m-row sparse products in one launch (either may be absent). The n rows take the row
+// tiling (lanes per row) of Q or Bᵀ, whichever of the products present has more lanes per row.
 void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
   static const SpmvN n0{-1, -1, -1, -1, -1, -1, K(0.0), K(0.0), K(0.0), K(0.0), K(0.0)};
   static const SpmvM m0{-1, -1, -1, -1, K(0.0), K(0.0), K(0.0), 0};
+  if (N && N->kxq >= 0 && N->kxt >= 0) ntm = idev_.Bt.lanes > idev_.Q.lanes ? 1 : 2;
+  else if (N && N->kxq >= 0) ntm = 2;
+  else if (N && N->kxt >= 0) ntm = 1;
   launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), N ? nsptile_[ntm] : 0,
                  d_sptiles_[0].as<Tile>(), M ? nsptile_[0] : 0, N ? *N : n0, M ? *M : m0);
 }

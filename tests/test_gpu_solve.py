@@ -109,6 +109,37 @@ def test_factor_large_front():
     _check_factor(np.tril(solver.dense_factor()), M)
 
 
+def test_long_sparse_rows_match_oracle():
+    """A dense row of B and a variable in every constraint, among short rows: each exceeds
+    SPMV_LONG × its matrix's lanes per row, so the sparse products hand it to a workgroup of its
+    own (B row 0 in the m-row products, Bᵀ row 1 in the Bᵀ-tiled n-row products). The solve
+    matches the oracle."""
+    from mumblebee_hip import generators as G
+    from oracle import ipm as O
+    n, m = 600, 150
+    rng = np.random.default_rng(41)
+    d = G.qp(n=n, m=m, seed=41, adens=4.0 / n)
+    B = d["B"].tolil()
+    B[0, :] = rng.standard_normal((1, n))
+    B[:, 1] = rng.standard_normal((m, 1))
+    B = B.tocsc()
+    B.sort_indices()
+    def lanes(nnz, rows):  # lanes per row of the tiling: largest power of two ≤ the mean, ≤ 64
+        g = 1
+        while g < 64 and 2 * g <= nnz / rows:
+            g *= 2
+        return g
+
+    Bc = B.tocsr()
+    rowlen, collen = np.diff(Bc.indptr), np.diff(B.indptr)
+    assert rowlen[0] > 64 * lanes(B.nnz, m) and collen[1] > 64 * lanes(B.nnz, n)
+    d = dict(d, B=B, g=B @ rng.uniform(0.1, 1.0, n), name="qp_long_rows")
+    res = mh.solve(hip_problem(d))
+    ores = O.solve(oracle_problem(d))
+    assert res.status == ores.status == mh.IPMStatus.OPTIMAL
+    assert rel(res.pobj, ores.pobj) < 1e-8 and rel(res.dobj, ores.dobj) < 1e-8
+
+
 @pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_solve_persistent_kernels_match_oracle(case):
     from oracle import ipm as O
