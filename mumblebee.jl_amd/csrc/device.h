@@ -15,7 +15,6 @@ constexpr int NB = 64;  // frontal tile / panel width
 enum Scalar : int {
   S_DELTA = 0,   // augmentation δ
   S_ALPHA,       // 1/δ
-  S_STEP,        // step length for the iterate update
   S_SMU,         // σμ for the correctors
   S_RHO,         // diagonal shift of the retried factorization
   S_NORM0,       // reduction outputs
@@ -30,7 +29,7 @@ enum Scalar : int {
   S_CG_BETA,
   S_CG_NP2,
   S_SPSD,        // Σ ⟨p*, d*⟩
-  S_TAU,         // min max-step
+  S_TAU,         // min max-step (the corrector's is the iterate update's step length)
   S_FAIL,        // factorization failure flag (> 0 failed)
   S_FLAGOK,      // scaling flag (1 ok, 0 failed)
   S_ACTIVE,      // 1 if this instance takes part in the current launch sequence

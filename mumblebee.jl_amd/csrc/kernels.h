@@ -54,7 +54,6 @@ void launch_axpby(hipStream_t, const InstDev*, const Tile*, int nt, int space, i
                   Coef b);
 void launch_axpbypcz(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kw, int kz,
                      Coef a, Coef b, Coef c);
-void launch_vmul(hipStream_t, const InstDev*, const Tile*, int nt, int space, int kx, int ky, int kz, int mode);
 // counter: ninst ints, zero between launches (the kernel leaves them at zero)
 void launch_dot(hipStream_t, const InstDev*, const Tile*, int nt, const int* tile_ofs, int ninst, int space,
                 int kx, int ky, int mode, int kw0, int kw1, double* partial, int* counter, int slot, int op);

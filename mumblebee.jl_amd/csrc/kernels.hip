@@ -78,18 +78,6 @@ __global__ void k_axpbypcz(const InstDev* insts, const Tile* tiles, int space, i
   z[i] = acc;
 }
 
-// z = x ⊙ y or x ⊘ y (mode 0 multiply, 1 divide)
-__global__ void k_vmul(const InstDev* insts, const Tile* tiles, int space, int kx, int ky, int kz, int mode) {
-  Tile t = tiles[blockIdx.x];
-  const InstDev& I = insts[t.inst];
-  if (I.sc[S_ACTIVE] == 0.0) return;
-  int i = t.start + threadIdx.x;
-  if (i >= spacelen(I, space)) return;
-  const double* x = vecof(I, space, kx);
-  const double* y = vecof(I, space, ky);
-  double* z = vecof(I, space, kz);
-  z[i] = mode == 0 ? x[i] * y[i] : x[i] / y[i];
-}
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1240,10 +1228,6 @@ void launch_axpby(hipStream_t st, const InstDev* I, const Tile* T, int nt, int s
 void launch_axpbypcz(hipStream_t st, const InstDev* I, const Tile* T, int nt, int space, int kx, int ky, int kw,
                      int kz, Coef a, Coef b, Coef c) {
   if (nt) k_axpbypcz<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, kw, kz, a, b, c);
-}
-void launch_vmul(hipStream_t st, const InstDev* I, const Tile* T, int nt, int space, int kx, int ky, int kz,
-                 int mode) {
-  if (nt) k_vmul<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, kz, mode);
 }
 void launch_dot(hipStream_t st, const InstDev* I, const Tile* T, int nt, const int* tile_ofs, int ninst,
                 int space, int kx, int ky, int mode, int kw0, int kw1, double* partial, int* counter, int slot,

@@ -628,10 +628,6 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
   }
 }
 
-__global__ void k_fill_int(int* x, int n, int v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = v;
-}
 
 // ------------------------------------------------------------------------------------------
 // pipelined triangular solves of one big front

@@ -905,13 +905,6 @@ void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
   launch_spmv(st_, I, d_sptiles_[which].as<Tile>(), nsptile_[which], which, kx, ky, a, b);
 }
 
-// ky = H kx = Q kx + D kx
-void Solver::hmul(int kx, int ky) {
-  spmv(2, kx, ky, K(1.0), K(0.0));
-  launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, ky, K(1.0));
-  launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
-                           dense_nmax_, kx, ky, K(1.0));
-}
 
 void Solver::lsolve(int kv) {
   for (const Level& L : levels_) {

@@ -157,7 +157,6 @@ class Solver {
   void spmv_nm(int ntm, const SpmvN* N, const SpmvM* M);
   bool cg_runs(double atol, int itmax, bool stops) const;
   void spmv(int which, int kx, int ky, Coef a, Coef b);
-  void hmul(int kx, int ky);  // ky = H kx
   void lsolve(int kv);
   void usolve(int kv);
   bool factor(double delta, double& rho);
