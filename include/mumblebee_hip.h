@@ -180,6 +180,18 @@ int mb_cone_corr0(mb_solver* s, double sigma_mu, double* r);
 int mb_cone_corr(mb_solver* s, const double* dp, const double* dd, double sigma_mu, double* r);
 int mb_cone_maxsteps(mb_solver* s, const double* dp, const double* dd, double step_frac, double* step);
 
+/* The frontal POTRF of the supernodal factorization on one dense front the caller holds in HBM:
+ * the dense kernel CliqueTrees' cholesky! runs per front (called at src/IPM/src/kkt/uzawa.jl:93
+ * and :111), here the persistent FP64-MFMA tile kernel of the large fronts. A: device pointer to
+ * an n×n column-major matrix with leading dimension lda ≥ max(n, 1); its lower triangle is
+ * replaced by L with A = L Lᵀ, the strict upper triangle is not touched. grid: workgroups of the
+ * persistent kernel (≤ 0: one per CU). stream: a hipStream_t (NULL: the null stream); the call
+ * returns after the factorization has finished on it. Returns 0 on success, 1 if a pivot was not
+ * positive (A is not positive definite: the reference's issuccess(F) is false and the retry with
+ * ρ follows), negative on error. The workspace (≈ 48·T² + 32 KiB·T bytes for T = ⌈n/64⌉ tiles) is
+ * allocated per call from the library's cache of device memory. */
+int mb_dense_cholesky(double* A, int64_t n, int64_t lda, int32_t grid, void* stream);
+
 /* Debug: the current factor as a dense n×n column-major lower triangle (small n only). */
 int mb_solver_dense_factor(mb_solver* s, double* L);
 
@@ -205,8 +217,9 @@ void mb_server_destroy(mb_server* s);
  * Returns a ticket ≥ 0, or −1. */
 int64_t mb_server_submit(mb_server* s, const mb_problem* p, const mb_settings* st);
 /* Wait up to timeout_s (< 0: no limit) for a ticket. Returns its IPMStatus, −2 on timeout, −1 on
- * error. p, d (length n), y (length m), info[8] as in mb_solver_result; times[4] = submitted,
- * admitted and finished (mb_clock seconds), device bytes used. Any pointer may be NULL. */
+ * error. p, d (length n), y (length m), info[8] as in mb_solver_result; times[5] = submitted,
+ * admitted and finished (mb_clock seconds), device bytes allocated, device bytes reserved at
+ * admission (the request's upper bound). Any pointer may be NULL. */
 int mb_server_wait(mb_server* s, int64_t ticket, double timeout_s, double* p, double* d, double* y, double* info,
                    double* times);
 /* Forget a collected ticket. */

@@ -25,6 +25,10 @@ int guard(F&& fn) {
 }
 }  // namespace
 
+namespace mb {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace mb
+
 struct mb_problem {
   mb::Problem P;
   int64_t n = 0, m = 0;
@@ -457,6 +461,7 @@ int mb_server_wait(mb_server* s, int64_t ticket, double timeout_s, double* p, do
       times[1] = job->t_admit;
       times[2] = job->t_end;
       times[3] = job->bytes;
+      times[4] = job->reserve;
     }
     return job->status;
   });

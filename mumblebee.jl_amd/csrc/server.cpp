@@ -13,7 +13,7 @@ double Server::now() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-Server::Server(int nslots, double mem_budget_bytes, int dag_grid)
+Server::Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis)
     : nslots_(std::max(1, nslots)), dag_grid_(dag_grid), budget_(mem_budget_bytes) {
   if (budget_ <= 0.0) {
     size_t free_b = 0, total_b = 0;
@@ -27,6 +27,9 @@ Server::Server(int nslots, double mem_budget_bytes, int dag_grid)
       ncu = 256;
     dag_grid_ = std::max(16, ncu / nslots_);
   }
+  if (nanalysis <= 0) nanalysis = std::min(8, std::max(2, (int)std::thread::hardware_concurrency() / 2));
+  lookahead_ = (size_t)(2 * nslots_ + nanalysis);
+  for (int k = 0; k < nanalysis; ++k) workers_.emplace_back(&Server::analysis_main, this);
   for (int s = 0; s < nslots_; ++s) slots_.emplace_back(&Server::slot_main, this, s);
 }
 
@@ -36,7 +39,9 @@ Server::~Server() {
     stop_ = true;
   }
   cv_work_.notify_all();
-  cv_mem_.notify_all();
+  cv_ready_.notify_all();
+  cv_room_.notify_all();
+  for (auto& t : workers_) t.join();
   for (auto& t : slots_) t.join();
 }
 
@@ -77,11 +82,56 @@ void Server::release(int64_t id) {
 
 void Server::stats(double* out) {
   std::lock_guard<std::mutex> lk(mu_);
-  out[0] = (double)queue_.size();
+  out[0] = (double)(queue_.size() + ready_.size() + (size_t)analysing_);
   out[1] = (double)running_;
   out[2] = (double)completed_;
   out[3] = reserved_;
   out[4] = peak_;
+}
+
+// Host analysis ahead of admission, in arrival order, at most lookahead_ requests ahead of the
+// slots (an analysed request holds its permuted problem and symbolic factor in host memory).
+void Server::analysis_main() {
+  for (;;) {
+    std::shared_ptr<ServeJob> job;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_work_.wait(lk, [&] {
+        return (stop_ && queue_.empty()) || (!queue_.empty() && ready_.size() + (size_t)analysing_ < lookahead_);
+      });
+      if (queue_.empty()) return;  // stop_ with nothing left
+      job = queue_.front();
+      queue_.pop_front();
+      ++analysing_;
+    }
+    try {
+      job->solver = std::make_unique<Solver>(*job->prob, job->set, Solver::Deferred{});
+      job->reserve = job->solver->estimate_device_bytes();
+    } catch (const std::exception& e) {
+      job->solver.reset();
+      job->error = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      --analysing_;
+      ready_[job->id] = job;
+    }
+    cv_ready_.notify_all();
+  }
+}
+
+void Server::finish(const std::shared_ptr<ServeJob>& job, double reserve) {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    reserved_ -= reserve;
+    if (reserved_ < 1.0) reserved_ = 0.0;
+    --running_;
+    ++completed_;
+    job->t_end = now();
+    job->done = true;
+  }
+  cv_room_.notify_all();
+  cv_done_.notify_all();
 }
 
 void Server::slot_main(int slot) {
@@ -90,27 +140,38 @@ void Server::slot_main(int slot) {
     std::shared_ptr<ServeJob> job;
     {
       std::unique_lock<std::mutex> lk(mu_);
-      cv_work_.wait(lk, [&] { return stop_ || !queue_.empty(); });
-      if (queue_.empty()) return;  // stop_ with nothing left
-      job = queue_.front();
-      queue_.pop_front();
+      cv_ready_.wait(lk, [&] {
+        return !ready_.empty() || (stop_ && queue_.empty() && analysing_ == 0);
+      });
+      if (ready_.empty()) return;  // stop_ with nothing left
+      job = ready_.begin()->second;
+      ready_.erase(ready_.begin());
       ++running_;
     }
-    double reserve = 0.0;
-    try {
-      Solver S(*job->prob, job->set, Solver::Deferred{});
-      reserve = S.estimate_device_bytes();
-      {
-        // admission: the bound must fit what the running solves leave of the budget; a request
-        // larger than the whole budget runs alone
-        std::unique_lock<std::mutex> lk(mu_);
-        if (reserve > budget_ && running_ > 1)
-          cv_mem_.wait(lk, [&] { return reserved_ == 0.0; });
-        else
-          cv_mem_.wait(lk, [&] { return reserved_ + reserve <= budget_; });
-        reserved_ += reserve;
-        peak_ = std::max(peak_, reserved_);
+    cv_work_.notify_all();  // room for the analysis workers to run ahead again
+    if (!job->solver) {     // the host analysis failed
+      job->status = -1;
+      finish(job, 0.0);
+      continue;
+    }
+    const double reserve = job->reserve;
+    {
+      // Admission: the bound must fit what the running solves leave of the budget. A request
+      // larger than the whole budget runs alone: it waits for an empty device, and no other
+      // request is admitted while it waits (so it cannot be starved by smaller ones).
+      std::unique_lock<std::mutex> lk(mu_);
+      if (reserve > budget_) {
+        ++oversized_waiting_;
+        cv_room_.wait(lk, [&] { return reserved_ == 0.0; });
+        --oversized_waiting_;
+      } else {
+        cv_room_.wait(lk, [&] { return oversized_waiting_ == 0 && reserved_ + reserve <= budget_; });
       }
+      reserved_ += reserve;
+      peak_ = std::max(peak_, reserved_);
+    }
+    try {
+      Solver& S = *job->solver;
       job->t_admit = now();
       S.device_setup(nullptr, nullptr, nullptr);
       job->bytes = (double)S.device_bytes();
@@ -123,17 +184,8 @@ void Server::slot_main(int slot) {
       job->error = e.what();
       job->status = -1;
     }
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      reserved_ -= reserve;
-      if (reserved_ < 1.0) reserved_ = 0.0;
-      --running_;
-      ++completed_;
-      job->t_end = now();
-      job->done = true;
-    }
-    cv_mem_.notify_all();
-    cv_done_.notify_all();
+    job->solver.reset();  // buffers back to the cache before the reservation is released
+    finish(job, reserve);
   }
 }
 

@@ -131,7 +131,7 @@ void DevBuf::alloc(size_t b) {
   }
   bytes = b;
   if (b) p = dev_alloc(b);
-  if (g_dev_counter) *g_dev_counter += b;
+  if (g_dev_counter && b) *g_dev_counter += size_class(b);  // what the allocator hands out
 }
 // Every copy and fill of a solver's buffers goes on the solver's own stream. The solver's stream
 // is non-blocking, so work on the null stream (hipMemcpy, hipMemset: both may return before the
@@ -266,6 +266,11 @@ void Solver::host_setup(const Problem& prob) {
   }
   n_ = prob.B.ncol();
   m_ = prob.B.nrow();
+  // the device copies index rows, columns and nonzeros with 32-bit ints
+  const i64 imax = std::numeric_limits<int32_t>::max();
+  if (n_ > imax || m_ > imax || prob.B.nbnz() > imax || prob.Q.nbnz() > imax)
+    throw std::invalid_argument("problem too large for 32-bit device indices (n, m and the stored entries of B "
+                                "and Q must each be below 2^31)");
   nu_ = 0;
   for (i64 v = 0; v < prob.B.nvtx; ++v) nu_ += cone_degree(prob.K[v], prob.B.ncols(v));
   mu_t_ = prob.mu;
@@ -302,31 +307,75 @@ void Solver::host_setup(const Problem& prob) {
   ng_ = norm2(P_.g);
   sg_ = scalenorm(P_.g, yscl_);
   sf_ = scalenorm(P_.f, pscl_);
+  L_ = gram(P_.B);
 }
 
-// An upper bound of what device_setup allocates, from the symbolic structure alone: frontal
-// storage, inverted diagonal blocks, vector arenas, CSR copies of B, Bᵀ, Q, cone blocks, and the
-// assembly map (at most one entry per frontal position).
+// An upper bound of what device_setup allocates: every buffer of build_device and
+// build_schedule, from the symbolic structure alone, each rounded up to the size class the
+// allocator hands out (the assembly map is bounded by the stored entries of BᵀB and Q).
 double Solver::estimate_device_bytes() const {
-  double fr = 0.0, dinv = 0.0, mixed = 0.0;
-  for (i64 f = 0; f < sym_.nfront(); ++f) {
-    double r = (double)(sym_.xcol[sym_.front_vtx[f + 1]] - sym_.xcol[sym_.front_vtx[f]]);
+  double tot = 0.0;
+  auto buf = [&](double bytes) { tot += (double)size_class((size_t)std::max(bytes, 8.0)); };
+  const double n = (double)std::max<i64>(n_, 1), m = (double)std::max<i64>(m_, 1);
+  const i64 NF = sym_.nfront(), N = P_.B.nvtx;
+  double fsize = 0.0, ssum = 0.0, nsteps = 0.0;
+  int nbig = 0;
+  std::vector<double> bigs;  // bytes of each large front's own buffers
+  for (i64 f = 0; f < NF; ++f) {
+    const double r = (double)(sym_.xcol[sym_.front_vtx[f + 1]] - sym_.xcol[sym_.front_vtx[f]]);
     double sc = 0.0;
     for (i64 u : sym_.front_sep[f]) sc += (double)(sym_.xcol[u + 1] - sym_.xcol[u]);
-    fr += (r + sc) * (r + sc);
-    dinv += std::ceil(r / NB) * NB * NB;
-    if (set_.precision == 1 && r + sc >= set_.big_front_ld)  // FP32 factor and inverses, in doubles
-      mixed += 0.5 * ((r + sc) * r + std::ceil(r / NB) * NB * NB);
+    const double ld = r + sc;
+    fsize += ld * ld;
+    ssum += sc;
+    if (ld < set_.big_front_ld) {
+      nsteps += 2.0 * std::ceil(r / NB);
+      continue;
+    }
+    ++nbig;
+    const double T = std::ceil(r / NB) + std::ceil(sc / NB), Tr = std::ceil(r / NB), nt = T * (T + 1) / 2;
+    bigs.push_back(4.0 * (T + 1));                           // tile bounds
+    bigs.push_back(16.0 * nt);                               // tasks
+    bigs.push_back(4.0 * nt);                                // tile flags
+    bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // inverted diagonal blocks
+    if (set_.precision == 1) {
+      bigs.push_back(4.0 * ld * std::max(1.0, r));           // FP32 factor columns
+      bigs.push_back(4.0 * NB * NB * std::max(1.0, Tr));     // FP32 inverted diagonal blocks
+    }
+    bigs.push_back(4.0 * std::max(1.0, Tr));                 // stage flags
+    bigs.push_back(16.0 * NB * std::max(1.0, T));            // solve hand-off words
+    bigs.push_back(8.0 * NB * std::max(1.0, Tr) * std::max(1.0, std::ceil(sc / NB)));  // solve partials
+    bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // M tiles
+    bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // N tiles
   }
-  double cone = 0.0;
-  for (i64 v = 0; v < P_.B.nvtx; ++v) {
-    double c = (double)P_.B.ncols(v);
-    cone += c * c + (double)tdc::CACHE;
+  for (double b : bigs) buf(b);
+  double dsz = 0.0, csz = 0.0, qsz = 0.0;
+  for (i64 v = 0; v < N; ++v) {
+    const double c = (double)P_.B.ncols(v);
+    const int k = P_.K[v].kind;
+    const double d = k == CONE_PSD ? (double)cone_degree(P_.K[v], P_.B.ncols(v)) : 0.0;
+    dsz += k == CONE_POS ? c : (k == CONE_FREE ? 0.0 : c * c);
+    csz += k == CONE_SOC ? 5 + c : (k == CONE_PSD ? 2 * d * d + d : ((k == CONE_EXP || k == CONE_POW) ? tdc::CACHE : 0));
+    qsz += c * c;  // a Q diagonal block, where there is one
   }
-  double nnz = 2.0 * (double)P_.B.val.size() + (double)P_.Q.val.size();
-  double b = 8.0 * fr * 1.5 + 8.0 * dinv + 8.0 * mixed + 8.0 * (NV_COUNT * (double)n_ + MV_COUNT * (double)m_) + 12.0 * nnz +
-             8.0 * 3.0 * cone + 24.0 * (double)(n_ + m_);
-  return 1.1 * b + (4 << 20);
+  const double nnzB = (double)P_.B.val.size(), nnzQ = (double)P_.Q.val.size();
+  const double nunion = (double)L_.val.size() + nnzQ;
+  buf(8.0 * NV_COUNT * n); buf(8.0 * MV_COUNT * m); buf(8.0 * NSCALAR); buf(24.0 * n);
+  buf(8.0 * fsize); buf(8.0 * dsz); buf(8.0 * csz); buf(8.0 * qsz);
+  buf(8.0 * ssum); buf(4.0 * ssum); buf(4.0 * ssum); buf(4.0 * NF);
+  buf((double)sizeof(FrontDesc) * NF); buf((double)sizeof(ConeDesc) * N);
+  buf(8.0 * DENSE_SPLIT * std::max<i64>(N, 1)); buf(4.0 * N); buf(4.0 * N); buf(4.0 * N); buf(4.0 * N); buf(8.0);
+  buf(4.0 * (m + 1)); buf(4.0 * nnzB); buf(8.0 * nnzB);  // B
+  buf(4.0 * (n + 1)); buf(4.0 * nnzB); buf(8.0 * nnzB);  // Bᵀ
+  buf(4.0 * (n + 1)); buf(4.0 * nnzQ); buf(8.0 * nnzQ);  // Q
+  buf(8.0 * (m + 1)); buf(8.0 * (n + 1)); buf(8.0 * (n + 1));  // row tiles
+  buf(8.0 * nunion); buf(8.0 * nunion); buf(8.0 * nunion);     // assembly map
+  buf(8.0 * (n / 256 + 1)); buf(8.0 * (m / 256 + 1)); buf(8.0); buf(8.0);
+  buf(8.0 * (std::max({n / 256 + 1, m / 256 + 1, (double)NF, 1.0}) + 16)); buf(8.0);
+  buf(24.0 * std::max<i64>(N, 1)); buf((double)sizeof(InstDev));
+  buf(8.0 * std::max(nbig, 1)); buf(8.0 * std::max(nbig, 1));
+  buf((double)sizeof(StepDesc) * nsteps); buf(4.0 * NF); buf(4.0 * NF); buf(4.0 * NF);
+  return tot + (1 << 20);
 }
 
 void Solver::device_setup(const double* p0, const double* d0, const double* y0) {
@@ -456,7 +505,7 @@ void Solver::build_device() {
     uofs[f + 1] = uofs[f] + fr_[f].s;
   }
   // union pattern of L = BᵀB and Q (lower triangle)
-  BSM L = gram(P_.B);
+  const BSM& L = L_;
   std::vector<int64_t> upos;
   std::vector<double> ul, uq;
   for (i64 v = 0; v < N; ++v) {
@@ -940,8 +989,14 @@ void Solver::usolve(int kv) {
 // a persistent kernel that hit its wait limit raised its abort flag: fail loudly. The flag is
 // part of the scalar block, so this reads the copy of the last sync_scalars, which every caller
 // has made after the kernels it checks.
+// The device raises the flag as an int in the slot's low word, which read as a double is a
+// subnormal: the test reads the word's bits, so a caller running with denormals-are-zero still
+// sees it.
 void Solver::check_abort() {
-  if (!big_.empty() && hsc_[S_ABORT] != 0.0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
+  if (big_.empty()) return;
+  int32_t raised = 0;
+  std::memcpy(&raised, &hsc_[S_ABORT], sizeof(raised));
+  if (raised != 0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
 }
 
 void Solver::run_factor(bool fp32) {

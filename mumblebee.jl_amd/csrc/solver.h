@@ -192,6 +192,7 @@ class Solver {
   Settings set_;
   Problem P_;          // permuted + scaled
   Symbolic sym_;
+  BSM L_;              // BᵀB of the permuted, scaled B (both triangles)
   int64_t n_ = 0, m_ = 0;
   int nu_ = 0;
   double mu_t_ = 0.0;
@@ -266,6 +267,9 @@ class Solver {
   size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
   void check_abort();
 };
+
+// the message mb_last_error() returns on the calling thread
+void set_last_error(const std::string& msg);
 
 // restatement of equilibrate! (src/IPM/src/scaling/scaling.jl:122-184) on host BSMs
 void equilibrate(BSM& B, BSM& Q, std::vector<double>& f, std::vector<double>& g, int itmax, double tol,

@@ -166,6 +166,8 @@ EXPORTS = {
     "mb_cone_corr": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, ctypes.c_double, _DP]),
     "mb_cone_maxsteps": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, ctypes.c_double, _DP]),
     "mb_probe_mfma_f32": (ctypes.c_int, [_DP]),
+    "mb_dense_cholesky": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                          ctypes.c_void_p]),
     "mb_last_error": (ctypes.c_char_p, []),
 }
 
@@ -583,11 +585,11 @@ class IPMServer:
 
     def wait(self, ticket, timeout=None):
         """Block until the ticket's solve ends (or `timeout` seconds pass: returns None). Returns
-        (IPMResult without history, dict of submitted/admitted/finished clock times and device
-        bytes)."""
+        (IPMResult without history, dict of submitted/admitted/finished clock times, device
+        bytes allocated and device bytes reserved at admission)."""
         prob = self._probs[ticket]
         n, m = prob.n, prob.m
-        p, d, y, info, tm = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(8), np.zeros(4)
+        p, d, y, info, tm = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(8), np.zeros(5)
         r = lib().mb_server_wait(self._h, int(ticket), -1.0 if timeout is None else float(timeout),
                                  _p(p, ctypes.c_double), _p(d, ctypes.c_double), _p(y, ctypes.c_double),
                                  _p(info, ctypes.c_double), _p(tm, ctypes.c_double))
@@ -600,7 +602,7 @@ class IPMServer:
         self.release(ticket)
         res = IPMResult(p, d, y, IPMStatus(int(info[7])), int(info[5]), int(info[6]), [],
                         info[0], info[1], info[2], info[3], info[4])
-        return res, dict(submitted=tm[0], admitted=tm[1], finished=tm[2], device_bytes=tm[3])
+        return res, dict(submitted=tm[0], admitted=tm[1], finished=tm[2], device_bytes=tm[3], reserved_bytes=tm[4])
 
     def release(self, ticket):
         lib().mb_server_release(self._h, int(ticket))
@@ -641,6 +643,17 @@ def cached_bytes():
 def release_cached_memory():
     """Return the cached device memory to the HIP runtime; returns the bytes released."""
     return int(lib().mb_release_cached_memory())
+
+
+def dense_cholesky(A_ptr, n, lda, grid=0, stream=None):
+    """Factor the dense n×n front at device address A_ptr (column-major, leading dimension lda)
+    in place: its lower triangle becomes L with A = L Lᵀ. Returns True on success, False when a
+    pivot was not positive. The frontal POTRF inside CliqueTrees' cholesky! (uzawa.jl:93, 111)."""
+    r = lib().mb_dense_cholesky(ctypes.c_void_p(int(A_ptr)), int(n), int(lda), int(grid),
+                                ctypes.c_void_p(int(stream) if stream else 0))
+    if r < 0:
+        raise _err("dense_cholesky")
+    return r == 0
 
 
 def probe_mfma_f64():

@@ -45,7 +45,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     for (res, times), ref in zip(served, solo):
         _close(res, ref)
         assert times["submitted"] <= times["admitted"] <= times["finished"]
-        assert times["device_bytes"] > 0
+        assert 0 < times["device_bytes"] <= times["reserved_bytes"]  # the admission bound holds
 
 
 def test_admission_keeps_reservations_within_the_budget():
@@ -67,15 +67,15 @@ def test_admission_keeps_reservations_within_the_budget():
 
 
 def test_a_failing_request_reports_its_error():
-    # one 50×50 PSD block in svec form (50·51/2 = 1275), above the device path's 48×48 limit
-    bad = G.qp(n=1275, m=40, seed=300)
-    bad_prob = mh.IPMProblem(bad["Q"], bad["B"], bad["f"], bad["g"], 0.0, [mh.SemidefiniteCone()], [1275])
+    # a power cone whose exponent lies outside (0, 1): the host analysis rejects it
+    bad = G.qp(n=3, m=1, seed=300, pdens=0.5, adens=1.0)
+    bad_prob = mh.IPMProblem(bad["Q"], bad["B"], bad["f"], bad["g"], 0.0, [mh.PowerCone(1.5)], [3])
     good = hip_problem(G.qp(n=200, m=80, seed=301))
     server = mh.IPMServer(nslots=2)
     try:
         tb = server.submit(bad_prob)
         tg = server.submit(good)
-        with pytest.raises(RuntimeError, match="block limit"):
+        with pytest.raises(RuntimeError, match="exponent"):
             server.wait(tb)
         res, _ = server.wait(tg)
     finally:
@@ -100,12 +100,44 @@ def test_mixed_precision_requests_alongside_fp64_ones():
     server = mh.IPMServer(nslots=4)
     try:
         tickets = [server.submit(p, s) for p in probs for s in (mixed, None)]
-        served = [server.wait(t)[0] for t in tickets]
+        out = [server.wait(t) for t in tickets]
         st = server.stats()
     finally:
         server.close()
+    served = [r for r, _ in out]
+    assert all(0 < t["device_bytes"] <= t["reserved_bytes"] for _, t in out)
     assert st["completed"] == 2 * len(probs)
     for k, (r64, r32) in enumerate(zip(solo64, solo32)):
         _close(served[2 * k], r32)
         _close(served[2 * k + 1], r64)
         assert rel(r32.pobj, r64.pobj) < 1e-8 and rel(r32.dobj, r64.dobj) < 1e-8
+
+
+def test_a_request_larger_than_the_budget_runs_alone():
+    """A request whose device bound exceeds the whole budget is admitted once the device is
+    empty, and the requests behind it wait for it instead of starving it."""
+    big = hip_problem(G.qp(n=900, m=400, seed=400))
+    small = [hip_problem(G.qp(n=200, m=80, seed=401 + k)) for k in range(3)]
+    server = mh.IPMServer(nslots=3, mem_budget_gb=1e-4)  # 100 KiB: every request is oversized
+    try:
+        tickets = [server.submit(p) for p in [small[0], big, small[1], small[2]]]
+        out = [server.wait(t, timeout=120) for t in tickets]
+        st = server.stats()
+    finally:
+        server.close()
+    assert all(o is not None for o in out)
+    assert all(r.status == mh.IPMStatus.OPTIMAL for r, _ in out)
+    assert st["completed"] == 4
+    spans = sorted((t["admitted"], t["finished"]) for _, t in out)
+    for (a0, f0), (a1, _) in zip(spans, spans[1:]):
+        assert f0 <= a1  # one at a time
+
+
+def test_a_single_oversized_request_completes():
+    server = mh.IPMServer(nslots=2, mem_budget_gb=1e-4)
+    try:
+        t = server.submit(hip_problem(G.qp(n=300, m=120, seed=410)))
+        out = server.wait(t, timeout=60)
+    finally:
+        server.close()
+    assert out is not None and out[0].status == mh.IPMStatus.OPTIMAL
