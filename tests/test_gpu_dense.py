@@ -1,11 +1,12 @@
 """GPU tests of mb_dense_cholesky: the frontal POTRF on a dense front held in HBM (caller-owned
-device memory from torch), checked against LAPACK's Cholesky (numpy) and by the backward error
+device memory from the HIP runtime), checked against LAPACK's Cholesky (numpy) and by the backward error
 ‖LLᵀ − A‖/‖A‖; a matrix that is not positive definite reports failure as the reference's
 issuccess(F) = false (uzawa.jl:93-97)."""
 import numpy as np
 import pytest
 
 import mumblebee_hip as mh
+from devmem import DeviceArray
 
 pytestmark = pytest.mark.gpu
 
@@ -16,23 +17,17 @@ def spd(n, seed):
     return M @ M.T / n + np.eye(n)
 
 
-def to_device(A, lda):
-    """Column-major A with leading dimension lda in a torch buffer (row j of the (n, lda) buffer
-    holds column j); the padding rows hold a sentinel."""
-    import torch
+def factor_on_device(A, lda, grid=0):
+    """Column-major A with leading dimension lda in device memory (row j of the (n, lda) host
+    image holds column j; the padding rows hold a sentinel), factored in place."""
     n = A.shape[0]
     buf = np.full((n, lda), 7.25)
     buf[:, :n] = A.T
-    return torch.from_numpy(buf).to("cuda:0")
-
-
-def factor_on_device(A, lda, grid=0):
-    import torch
-    t = to_device(A, lda)
-    torch.cuda.synchronize()
-    ok = mh.dense_cholesky(t.data_ptr(), A.shape[0], lda, grid)
-    out = t.cpu().numpy()
-    return ok, out[:, :A.shape[0]].T.copy(), out[:, A.shape[0]:]
+    t = DeviceArray(buf)
+    ok = mh.dense_cholesky(t.ptr, n, lda, grid)
+    out = t.numpy()
+    t.free()
+    return ok, out[:, :n].T.copy(), out[:, n:]
 
 
 @pytest.mark.parametrize("n,lda,grid", [(1, 1, 0), (63, 63, 0), (64, 70, 0), (65, 65, 0), (300, 301, 0),
