@@ -192,6 +192,16 @@ int mb_cone_maxsteps(mb_solver* s, const double* dp, const double* dd, double st
  * allocated per call from the library's cache of device memory. */
 int mb_dense_cholesky(double* A, int64_t n, int64_t lda, int32_t grid, void* stream);
 
+/* mb_dense_cholesky with no device allocation: no counterpart in the reference. It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc). Same arguments, results and return codes as mb_dense_cholesky, and the
+ * same kernels; the workspace is one file-scope device array per GPU (4.4 MB, kept until the
+ * process ends), so the order is limited to n ≤ 8192 (larger n: an error, nothing launched). The
+ * call allocates nothing: no hipMalloc, hipFree or allocating library routine. Every call on one
+ * GPU shares that GPU's array and each call resets what it reads: calls that overlap on one GPU
+ * (several host threads or streams at once) race and must be serialised by the caller. */
+int mb_dense_cholesky_noalloc(double* A, int64_t n, int64_t lda, int32_t grid, void* stream);
+
 /* Debug: the current factor as a dense n×n column-major lower triangle (small n only). */
 int mb_solver_dense_factor(mb_solver* s, double* L);
 

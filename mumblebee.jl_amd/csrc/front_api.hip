@@ -34,12 +34,12 @@ namespace {
   } while (0)
 
 constexpr size_t ALIGN = 256;
-size_t up(size_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
+constexpr size_t up(size_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
 
 // byte offsets of the workspace parts for T tiles
 struct WsLayout {
-  size_t inst, sc, ticket, tb, tasks, state, stage, dinv, total;
-  explicit WsLayout(int T) {
+  size_t inst = 0, sc = 0, ticket = 0, tb = 0, tasks = 0, state = 0, stage = 0, dinv = 0, total = 0;
+  constexpr explicit WsLayout(int T) {
     const size_t nt = (size_t)T * (T + 1) / 2;
     size_t o = 0;
     inst = o; o += up(sizeof(InstDev));
@@ -150,10 +150,46 @@ int guard_front(F&& fn) {
   }
 }
 
+// The no-allocation form: one workspace per GPU in a file-scope device array, sized for fronts
+// of order up to NOALLOC_NMAX (4.4 MB; held by each GPU the library runs on until the process
+// ends). In host code `scratch` only names the symbol: its device address on the current GPU comes
+// from hipGetSymbolAddress, taken once per device at the entry point's first call there.
+constexpr int64_t NOALLOC_NMAX = 8192;
+constexpr size_t NOALLOC_BYTES = WsLayout((int)((NOALLOC_NMAX + NB - 1) / NB)).total;
+__device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[MAX_DEVICES] = {};
+
+char* scratch_on_current_device() {
+  int dev = 0;
+  HIPCHK_F(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) throw std::runtime_error("dense cholesky: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK_F(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("dense cholesky: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace mb
 
 extern "C" {
+
+int mb_dense_cholesky_noalloc(double* A, int64_t n, int64_t lda, int32_t grid, void* stream) {
+  return mb::guard_front([&]() {
+    if (n > mb::NOALLOC_NMAX)
+      throw std::invalid_argument("dense cholesky (no allocation): order " + std::to_string(n) +
+                                  " exceeds the workspace limit " + std::to_string(mb::NOALLOC_NMAX));
+    mb::check_args(A, n, lda);
+    return mb::dense_factor(A, n, lda, grid, mb::scratch_on_current_device(), (hipStream_t)stream);
+  });
+}
 
 int mb_dense_cholesky(double* A, int64_t n, int64_t lda, int32_t grid, void* stream) {
   return mb::guard_front([&]() {

@@ -168,6 +168,8 @@ EXPORTS = {
     "mb_probe_mfma_f32": (ctypes.c_int, [_DP]),
     "mb_dense_cholesky": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                                           ctypes.c_void_p]),
+    "mb_dense_cholesky_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                                  ctypes.c_void_p]),
     "mb_last_error": (ctypes.c_char_p, []),
 }
 
@@ -653,6 +655,16 @@ def dense_cholesky(A_ptr, n, lda, grid=0, stream=None):
                                 ctypes.c_void_p(int(stream) if stream else 0))
     if r < 0:
         raise _err("dense_cholesky")
+    return r == 0
+
+
+def dense_cholesky_noalloc(A_ptr, n, lda, grid=0, stream=None):
+    """dense_cholesky without any device allocation (workspace: a per-GPU device array, n ≤ 8192;
+    calls overlapping on one GPU race)."""
+    r = lib().mb_dense_cholesky_noalloc(ctypes.c_void_p(int(A_ptr)), int(n), int(lda), int(grid),
+                                        ctypes.c_void_p(int(stream) if stream else 0))
+    if r < 0:
+        raise _err("dense_cholesky_noalloc")
     return r == 0
 
 
