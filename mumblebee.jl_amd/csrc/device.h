@@ -79,6 +79,7 @@ struct ConeDesc {
   double* D;       // n×n scaling block (column-major); n entries (its diagonal) for POS
   double* cache;   // per-cone cache slots
   const double* Qd;  // the n×n diagonal block of Q (for ‖H‖)
+  double* ws;      // PSD cones above the LDS kernels' order: the d×d workspace (else null)
 };
 
 struct SegDesc {    // a contiguous range of one instance's vector

@@ -110,15 +110,23 @@ void launch_spmv_nm(hipStream_t, const InstDev*, int ntm, const Tile* ntiles, in
                     const SpmvN& N, const SpmvM& M);
 void launch_cone_matvec(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky, Coef a);
 // psd / dense: lists of the PSD cones and of the cones with dense scaling blocks (SOC, PSD)
+// PSD cones of order above PSD_MAX (psd_big.hip): the list, its length and the largest order
+struct BigPsd {
+  const int* list = nullptr;
+  int n = 0;
+  int dmax = 0;
+};
 void launch_cone_scale(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                       double* out, const int* psd, int npsd);
+                       double* out, const int* psd, int npsd, const BigPsd& big);
 // dense_nmax: the largest order among the dense-block cones; dpart: DENSE_SPLIT doubles per dense cone
 void launch_cone_hnorm2(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                        double* out, int slot, const int* dense, int ndense, int dense_nmax, double* dpart);
+                        double* out, int slot, const int* dense, int ndense, int dense_nmax, double* dpart,
+                        const BigPsd& big, double* bpart);
 void launch_cone_corr0(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr);
 void launch_cone_corr(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kr, int kdp, int kdd);
 void launch_cone_maxstep(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                         int kdp, int kdd, double step_frac, double* out, const int* psd, int npsd);
+                         int kdp, int kdd, double step_frac, double* out, const int* psd, int npsd,
+                         const BigPsd& big);
 
 // PSD cones (order d ≤ PSD_MAX) and dense-block cones: one workgroup per listed cone (psd.hip)
 constexpr int PSD_MAX = 48;
@@ -127,6 +135,27 @@ void launch_psd_corr0(hipStream_t, const InstDev*, const ConeDesc*, const int* l
 void launch_psd_corr(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kr, int kdp, int kdd);
 void launch_psd_maxstep(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kdp, int kdd,
                         double step_frac, double* out);
+// PSD cones above PSD_MAX (psd_big.hip), order ≤ PSD_BIG_MAX. Per cone: cache R, S (d² each),
+// s (d), W (d²) — psd_big_cache(d) doubles; workspace psd_big_ws(d) doubles. Their scaling block
+// W ⊗ₛ W is not stored (ConeDesc::D is null): the products, the KKT assembly and ‖H‖ form it
+// from W. Matrix slots of the launches: workspace X0..X2, cache R, S, W.
+constexpr int PSD_BIG_MAX = 4096;
+enum BigPsdSlot : int { BP_X0 = 0, BP_X1, BP_X2, BP_R, BP_S, BP_W };
+inline long long psd_big_cache(long long d) { return 3 * d * d + d; }
+inline long long psd_big_ws(long long d) { return 3 * d * d + ((d + NB - 1) / NB) * NB * NB + 16; }
+void launch_bpsd_scale(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, double* out);
+void launch_bpsd_corr0(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, int kr);
+void launch_bpsd_corr(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, int kr, int kdp, int kdd);
+void launch_bpsd_maxstep(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, int kdp, int kdd,
+                         double step_frac, double* out);
+// y += a·svec(W smat(x) W) over the listed cones
+void launch_bpsd_matvec(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, int kx, int ky, Coef a);
+// out[c] = ‖Q_vv + W ⊗ₛ W‖²; part: 2·DENSE_SPLIT doubles per listed cone
+void launch_bpsd_hnorm2(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, double* out, double* part);
+void launch_bpsd_assemble(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, const int* cone_front,
+                          const FrontDesc*);
+// the n×n block W ⊗ₛ W of the cone at one[0] into D (column-major)
+void launch_bpsd_block(hipStream_t, const InstDev*, const ConeDesc*, const int* one, int n, double* D);
 // the dense-block cones split over many workgroups: 64-row slices for D·x, DENSE_SPLIT chunks per
 // cone for ‖Q_vv + D_v‖² (partials in part, summed in a fixed order); nmax: the largest order
 constexpr int DENSE_SPLIT = 32;

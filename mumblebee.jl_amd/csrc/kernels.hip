@@ -812,7 +812,8 @@ __global__ void k_assemble_cones(const InstDev* insts, const ConeDesc* cones, co
   if (c >= ncone) return;
   const ConeDesc C = cones[c];
   const InstDev& I = insts[C.inst];
-  if (I.sc[S_ACTIVE] == 0.0 || C.kind == CONE_FREE) return;
+  // free cones add nothing; large PSD cones (no stored block) are assembled by k_bp_assemble
+  if (I.sc[S_ACTIVE] == 0.0 || C.kind == CONE_FREE || C.D == nullptr) return;
   const FrontDesc& Fd = fronts[cone_front[c]];
   const double delta = I.sc[S_DELTA];
   const int off = C.col0 - Fd.res0;
@@ -1259,17 +1260,19 @@ void launch_cone_matvec(hipStream_t st, const InstDev* I, const ConeDesc* C, int
   if (nc) k_cone_matvec<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx, ky, a);
 }
 void launch_cone_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                       int ninst, double* out, const int* psd, int npsd) {
+                       int ninst, double* out, const int* psd, int npsd, const BigPsd& big) {
   if (nc) k_cone_scale<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
   launch_psd_scale(st, I, C, psd, npsd, out);
+  launch_bpsd_scale(st, I, C, big, out);
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 0);
 }
 void launch_cone_hnorm2(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
                         int ninst, double* out, int slot, const int* dense, int ndense, int dense_nmax,
-                        double* dpart) {
+                        double* dpart, const BigPsd& big, double* bpart) {
   (void)dense_nmax;
   if (nc) k_cone_hnorm2<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
   launch_cone_hnorm2_dense(st, I, C, dense, ndense, out, dpart);
+  launch_bpsd_hnorm2(st, I, C, big, out, bpart);
   k_cone_sum<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, slot);
 }
 void launch_cone_corr0(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kr) {
@@ -1279,9 +1282,11 @@ void launch_cone_corr(hipStream_t st, const InstDev* I, const ConeDesc* C, int n
   if (nc) k_cone_corr<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kr, kdp, kdd);
 }
 void launch_cone_maxstep(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                         int ninst, int kdp, int kdd, double step_frac, double* out, const int* psd, int npsd) {
+                         int ninst, int kdp, int kdd, double step_frac, double* out, const int* psd, int npsd,
+                         const BigPsd& big) {
   if (nc) k_cone_maxstep<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kdp, kdd, step_frac, out);
   launch_psd_maxstep(st, I, C, psd, npsd, kdp, kdd, step_frac, out);
+  launch_bpsd_maxstep(st, I, C, big, kdp, kdd, step_frac, out);
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 1);
 }
 void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx) {

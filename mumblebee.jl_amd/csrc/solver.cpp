@@ -261,7 +261,7 @@ void Solver::host_setup(const Problem& prob) {
     if (k == CONE_PSD) {
       i64 nc = prob.B.ncols(v), d = cone_degree(prob.K[v], nc);
       if (d * (d + 1) / 2 != nc) throw std::invalid_argument("PSD cone size is not d(d+1)/2");
-      if (d > PSD_MAX) throw std::invalid_argument("PSD cone larger than the device path's block limit");
+      if (d > PSD_BIG_MAX) throw std::invalid_argument("PSD cone larger than the device path's order limit (4096)");
     }
   }
   n_ = prob.B.ncol();
@@ -349,15 +349,25 @@ double Solver::estimate_device_bytes() const {
     bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // N tiles
   }
   for (double b : bigs) buf(b);
-  double dsz = 0.0, csz = 0.0, qsz = 0.0;
+  double dsz = 0.0, csz = 0.0, qsz = 0.0, wsz = 0.0;
+  int nbig_psd = 0;
   for (i64 v = 0; v < N; ++v) {
     const double c = (double)P_.B.ncols(v);
     const int k = P_.K[v].kind;
     const double d = k == CONE_PSD ? (double)cone_degree(P_.K[v], P_.B.ncols(v)) : 0.0;
-    dsz += k == CONE_POS ? c : (k == CONE_FREE ? 0.0 : c * c);
-    csz += k == CONE_SOC ? 5 + c : (k == CONE_PSD ? 2 * d * d + d : ((k == CONE_EXP || k == CONE_POW) ? tdc::CACHE : 0));
+    const bool big = k == CONE_PSD && d > PSD_MAX;
+    dsz += k == CONE_POS ? c : (k == CONE_FREE || big ? 0.0 : c * c);
+    csz += k == CONE_SOC ? 5 + c
+                         : (k == CONE_PSD ? (big ? (double)psd_big_cache((long long)d) : 2 * d * d + d)
+                                          : ((k == CONE_EXP || k == CONE_POW) ? tdc::CACHE : 0));
+    if (big) {
+      wsz += (double)psd_big_ws((long long)d);
+      ++nbig_psd;
+    }
     qsz += c * c;  // a Q diagonal block, where there is one
   }
+  buf(8.0 * wsz);
+  buf(16.0 * DENSE_SPLIT * std::max(nbig_psd, 1));
   const double nnzB = (double)P_.B.val.size(), nnzQ = (double)P_.Q.val.size();
   const double nunion = (double)L_.val.size() + nnzQ;
   buf(8.0 * NV_COUNT * n); buf(8.0 * MV_COUNT * m); buf(8.0 * NSCALAR); buf(24.0 * n);
@@ -543,14 +553,18 @@ void Solver::build_device() {
         for (i64 b = P_.Q.xblk[e]; b < P_.Q.xblk[e + 1]; ++b) qoff2_ += P_.Q.val[b] * P_.Q.val[b];
   // cones
   ncone_ = (int)N;
-  std::vector<i64> dofs(N + 1, 0), cofs(N + 1, 0), qofs(N + 1, 0);
+  std::vector<i64> dofs(N + 1, 0), cofs(N + 1, 0), qofs(N + 1, 0), wofs(N + 1, 0);
   for (i64 v = 0; v < N; ++v) {
     i64 nc = P_.B.ncols(v);
     int k = P_.K[v].kind;
     const i64 d = k == CONE_PSD ? cone_degree(P_.K[v], nc) : 0;
     const bool td = k == CONE_EXP || k == CONE_POW;
-    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : (k == CONE_FREE ? 0 : nc * nc));
-    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc : (k == CONE_PSD ? 2 * d * d + d : (td ? tdc::CACHE : 0)));
+    const bool big = k == CONE_PSD && d > PSD_MAX;  // W ⊗ₛ W not stored; cache R, S, s, W
+    dofs[v + 1] = dofs[v] + (k == CONE_POS ? nc : (k == CONE_FREE || big ? 0 : nc * nc));
+    cofs[v + 1] = cofs[v] + (k == CONE_SOC ? 5 + nc
+                                            : (k == CONE_PSD ? (big ? psd_big_cache(d) : 2 * d * d + d)
+                                                             : (td ? tdc::CACHE : 0)));
+    wofs[v + 1] = wofs[v] + (big ? psd_big_ws(d) : 0);
   }
   // the diagonal blocks of Q, for the cones that have one (‖H‖ reads them; the others read zero)
   std::vector<i64> qarc(N, -1);
@@ -577,12 +591,14 @@ void Solver::build_device() {
   d_F_.alloc(sizeof(double) * std::max<i64>(fsize_, 1));
   d_D_.alloc(sizeof(double) * std::max<i64>(dofs[N], 1));
   d_cache_.alloc(sizeof(double) * std::max<i64>(cofs[N], 1));
+  d_bws_.alloc(sizeof(double) * std::max<i64>(wofs[N], 1));
   d_qdiag_.upload(qdiag, st_);
   d_uvec_.alloc(sizeof(double) * std::max<i64>(uofs[NF], 1));
   HIPCHK(hipMemsetAsync(d_nv_.p, 0, d_nv_.bytes, st_));
   HIPCHK(hipMemsetAsync(d_mv_.p, 0, d_mv_.bytes, st_));
   HIPCHK(hipMemsetAsync(d_sc_.p, 0, d_sc_.bytes, st_));
   HIPCHK(hipMemsetAsync(d_cache_.p, 0, d_cache_.bytes, st_));
+  HIPCHK(hipMemsetAsync(d_bws_.p, 0, d_bws_.bytes, st_));
   HIPCHK(hipMemsetAsync(d_D_.p, 0, d_D_.bytes, st_));
   d_sepcols_.upload(allsep, st_);
   d_relidx_.upload(relidx, st_);
@@ -603,21 +619,31 @@ void Solver::build_device() {
     C.col0 = (int)P_.B.xcol[v];
     C.n = (int)P_.B.ncols(v);
     C.param = P_.K[v].param;
-    C.D = d_D_.as<double>() + dofs[v];
+    const bool big = C.kind == CONE_PSD && cone_degree(P_.K[v], C.n) > PSD_MAX;
+    C.D = big ? nullptr : d_D_.as<double>() + dofs[v];
     C.cache = d_cache_.as<double>() + cofs[v];
+    C.ws = big ? d_bws_.as<double>() + wofs[v] : nullptr;
     C.Qd = qarc[v] >= 0 ? d_qdiag_.as<double>() + qofs[v] : nullptr;
     if (C.kind != CONE_FREE) cone_nmax_ = std::max(cone_nmax_, C.n);
   }
   d_cones_.upload(cones_, st_);
   {
-    std::vector<int> psd, dense, td;
+    std::vector<int> psd, dense, td, bpsd;
     for (i64 v = 0; v < N; ++v) {
       const int k = cones_[v].kind;
-      if (k == CONE_PSD) psd.push_back((int)v);
-      if (k == CONE_PSD || k == CONE_SOC) dense.push_back((int)v);
+      const bool big = k == CONE_PSD && cones_[v].D == nullptr;
+      if (k == CONE_PSD && !big) psd.push_back((int)v);
+      if (big) {
+        bpsd.push_back((int)v);
+        bpsd_dmax_ = std::max(bpsd_dmax_, cone_degree(P_.K[v], cones_[v].n));
+      }
+      if ((k == CONE_PSD && !big) || k == CONE_SOC) dense.push_back((int)v);
       if (k == CONE_EXP || k == CONE_POW) td.push_back((int)v);
     }
     npsd_ = (int)psd.size();
+    nbpsd_ = (int)bpsd.size();
+    if (nbpsd_) d_bpsdlist_.upload(bpsd, st_);
+    d_bpart_.alloc(sizeof(double) * 2 * DENSE_SPLIT * std::max(nbpsd_, 1));
     ndense_ = (int)dense.size();
     for (int v : dense) dense_nmax_ = std::max(dense_nmax_, cones_[v].n);
     d_dpart_.alloc(sizeof(double) * DENSE_SPLIT * std::max(ndense_, 1));
@@ -1098,6 +1124,7 @@ bool Solver::factor(double delta, double& rho) {
   auto assemble = [&]() {
     launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
     launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_);
+    launch_bpsd_assemble(st_, I, d_cones_.as<ConeDesc>(), bpsd(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>());
     launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(),
                           ncone_, cone_nmax_);
   };
@@ -1163,21 +1190,54 @@ bool Solver::cone_scale(const double* p, const double* d, double* spsd) {
   HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_P * n_, p, n_ * 8, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_D * n_, d, n_ * 8, hipMemcpyHostToDevice, st_));
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd());
   sync_scalars();
   if (spsd) *spsd = hsc_[S_SPSD];
   return hsc_[S_FLAGOK] != 0.0;
 }
 
+// the blocks in cone order; a large PSD cone's block W ⊗ₛ W is formed on the device for the copy
 void Solver::cone_blocks(double* out) {
-  HIPCHK(hipMemcpyAsync(out, d_D_.p, d_D_.bytes, hipMemcpyDeviceToHost, st_));
+  std::vector<double> D(d_D_.bytes / sizeof(double));
+  HIPCHK(hipMemcpyAsync(D.data(), d_D_.p, d_D_.bytes, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
+  size_t o = 0, od = 0;
+  for (int v = 0; v < ncone_; ++v) {
+    const ConeDesc& C = cones_[v];
+    const size_t nc = (size_t)C.n;
+    if (C.kind == CONE_FREE) continue;
+    if (C.kind == CONE_POS) {
+      std::copy(D.begin() + od, D.begin() + od + nc, out + o);
+      o += nc;
+      od += nc;
+    } else if (C.D) {
+      std::copy(D.begin() + od, D.begin() + od + nc * nc, out + o);
+      o += nc * nc;
+      od += nc * nc;
+    } else {
+      DevBuf blk, one;
+      blk.alloc(sizeof(double) * nc * nc);
+      one.upload(std::vector<int>{v}, st_);
+      launch_bpsd_block(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), one.as<int>(), C.n, blk.as<double>());
+      HIPCHK(hipMemcpyAsync(out + o, blk.p, sizeof(double) * nc * nc, hipMemcpyDeviceToHost, st_));
+      HIPCHK(hipStreamSynchronize(st_));
+      o += nc * nc;
+    }
+  }
+}
+
+size_t Solver::cone_blocks_size() const {
+  size_t t = 0;
+  for (const ConeDesc& C : cones_)
+    t += C.kind == CONE_POS ? (size_t)C.n : (C.kind == CONE_FREE ? 0 : (size_t)C.n * C.n);
+  return t;
 }
 
 void Solver::cone_corr0(double smu, double* r) {
   set_scalar(S_SMU, smu);
   launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
   launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
+  launch_bpsd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), NV_FR);
   HIPCHK(hipMemcpyAsync(r, idev_.nv + (size_t)NV_FR * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
 }
@@ -1189,6 +1249,7 @@ void Solver::cone_corr(const double* dp, const double* dd, double smu, double* r
   launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
   launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
                   NV_DDA);
+  launch_bpsd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), NV_FR, NV_DPA, NV_DDA);
   HIPCHK(hipMemcpyAsync(r, idev_.nv + (size_t)NV_FR * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
 }
@@ -1217,7 +1278,7 @@ double Solver::cone_maxsteps(const double* dp, const double* dd, double step_fra
 // the step into S_TAU on the device (no read-back)
 void Solver::launch_maxsteps(int kdp, int kdd, double step_frac) {
   launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
-                      kdd, step_frac, d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
+                      kdd, step_frac, d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd());
 }
 
 double Solver::maxsteps(int kdp, int kdd, double step_frac) {
@@ -1483,6 +1544,7 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
     launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, NV_KF, K(-1.0));
     launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
                              dense_nmax_, kx, NV_KF, K(-1.0));
+    launch_bpsd_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), kx, NV_KF, K(-1.0));
     if (read) norms();
   };
   // with irmin ≥ 1 and irmax ≥ 1 the starting residual decides nothing (and its norm enters no
@@ -1671,10 +1733,10 @@ int Solver::step() {
   launch_set_scalar2(st_, d_sc_.as<double>(), S_DELTA, delta, S_ALPHA, 1.0 / delta);
   // pᵀd, scale!, residuals!
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_);
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd());
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                      d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
-                     d_dpart_.as<double>());
+                     d_dpart_.as<double>(), bpsd(), d_bpart_.as<double>());
   {
     // Qp ; Δf = d + f − Qp + Bᵀy ; Δg = g − Bp
     SpmvN fn{NV_P, MV_Y, NV_D, NV_F, -1, NV_DF, K(-1.0), K(1.0), K(1.0), K(1.0), K(0.0)};
@@ -1767,6 +1829,7 @@ int Solver::step() {
             set_scalar(S_SMU, mut);
             launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
             launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
+            launch_bpsd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), NV_FR);
             axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
           } else {
             lin(LinSet().add(Lin{NV_DF, NV_D, NV_D, NV_FR, K(1.0), K(-1.0), K(0.0)}), LinSet());  // Δf − d
@@ -1784,6 +1847,7 @@ int Solver::step() {
           launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
           launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
                           NV_DDA);
+          launch_bpsd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), NV_FR, NV_DPA, NV_DDA);
           // corrector right-hand side Δf + fr, warm start (Δpa, Δya)
           lin(LinSet()
                   .add(Lin{NV_DF, NV_FR, NV_FR, NV_FR, K(1.0), K(1.0), K(0.0)})

@@ -117,7 +117,7 @@ class Solver {
   // (cone.jl:146-553); scale replaces the iterate by (p, d)
   bool cone_scale(const double* p, const double* d, double* spsd);
   void cone_blocks(double* out);  // D_v per cone: n_v entries (nonneg), none (free), n_v² (others)
-  size_t cone_blocks_size() const { return d_D_.bytes / sizeof(double); }
+  size_t cone_blocks_size() const;
   const Problem& internal() const { return P_; }  // the permuted problem (before equilibration)
   const std::vector<double>& col_scale() const { return pscl_; }
   const std::vector<double>& row_scale() const { return yscl_; }
@@ -225,6 +225,9 @@ class Solver {
   int ntile_n_ = 0, ntile_m_ = 0;
   DevBuf d_sptiles_[3];  // row tiles of B, Bᵀ, Q for the group-per-row spmv
   DevBuf d_psdlist_, d_denselist_;  // PSD cones; cones with dense scaling blocks (SOC, PSD)
+  DevBuf d_bpsdlist_, d_bws_, d_bpart_;  // PSD cones above PSD_MAX, their workspaces, ‖H‖ partials
+  int nbpsd_ = 0, bpsd_dmax_ = 0;
+  BigPsd bpsd() const { return BigPsd{d_bpsdlist_.as<int>(), nbpsd_, bpsd_dmax_}; }
   DevBuf d_tdlist_;                 // exponential and power cones
   int npsd_ = 0, ndense_ = 0, ntd_ = 0;
   int cone_nmax_ = 0, dense_nmax_ = 0;  // largest cone order (non-free; dense-block cones)

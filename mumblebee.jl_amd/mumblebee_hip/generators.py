@@ -180,6 +180,168 @@ def maxcut(nv=2000, band=20, clique=48, extra=0.001, seed=0):
                 name=f"maxcut_nv{nv}_w{band}_c{clique}_s{seed}")
 
 
+def chordal_cliques(nv, edges):
+    """Maximal cliques of a chordal extension of the graph (nv vertices, edge set of (j, i) pairs)
+    and a clique tree over them: minimum-degree elimination (ties to the smallest index) gives a
+    perfect elimination ordering with higher neighbourhoods H(v); the vertex cliques {v} ∪ H(v)
+    form a tree through the elimination-tree parent p(v) = the first-eliminated vertex of H(v),
+    with H(v) ⊆ {p(v)} ∪ H(p(v)) (running intersection); each non-maximal vertex clique merges into
+    a child clique that contains it. Returns (cliques as sorted vertex lists, parent clique index
+    or −1)."""
+    adj = [set() for _ in range(nv)]
+    for j, i in edges:
+        adj[i].add(j)
+        adj[j].add(i)
+    alive = set(range(nv))
+    order, higher = [], {}
+    # bucketed minimum degree: degree → set of vertices
+    deg = [len(a) for a in adj]
+    buckets = {}
+    for v in range(nv):
+        buckets.setdefault(deg[v], set()).add(v)
+    while alive:
+        dmin = min(k for k, b in buckets.items() if b)
+        v = min(buckets[dmin])
+        buckets[dmin].discard(v)
+        nb = set(adj[v])
+        order.append(v)
+        higher[v] = nb
+        alive.discard(v)
+        for a in nb:
+            adj[a].discard(v)
+            adj[a] |= nb - {a}
+            old = deg[a]
+            deg[a] = len(adj[a])
+            if old != deg[a]:
+                buckets[old].discard(a)
+                buckets.setdefault(deg[a], set()).add(a)
+    pos = {v: k for k, v in enumerate(order)}
+    parent = {v: (min(higher[v], key=pos.get) if higher[v] else None) for v in order}
+    kids = {v: [] for v in order}
+    for v in order:
+        if parent[v] is not None:
+            kids[parent[v]].append(v)
+    rep = {}
+    for v in order:  # children come first in elimination order
+        r = v
+        for w in kids[v]:
+            if len(higher[w]) == len(higher[v]) + 1:  # {v} ∪ H(v) ⊂ {w} ∪ H(w)
+                r = rep[w]
+                break
+        rep[v] = r
+    reps = [v for v in order if rep[v] == v]
+    index = {v: k for k, v in enumerate(reps)}
+    cliques = [sorted({v} | higher[v]) for v in reps]
+    cparent = [-1] * len(reps)
+    for v in order:
+        pv = parent[v]
+        if pv is not None and rep[v] != rep[pv]:
+            cparent[index[rep[v]]] = index[rep[pv]]
+    return cliques, cparent
+
+
+def merge_cliques(cliques, cparent, cap):
+    """Clique-tree merging: children first (index order is a topological order), a clique joins
+    its parent when their union has at most `cap` vertices; the merged tree keeps the running
+    intersection property. Returns the merged (cliques, parent)."""
+    sets = [set(c) for c in cliques]
+    par = list(cparent)
+    alive = [True] * len(sets)
+    for k in range(len(sets)):
+        q = par[k]
+        if q < 0 or len(sets[k] | sets[q]) > cap:
+            continue
+        sets[q] |= sets[k]
+        alive[k] = False
+        for c in range(len(sets)):
+            if par[c] == k:
+                par[c] = q
+    keep = [k for k in range(len(sets)) if alive[k]]
+    index = {k: i for i, k in enumerate(keep)}
+    return [sorted(sets[k]) for k in keep], [index[par[k]] if par[k] >= 0 else -1 for k in keep]
+
+
+def maxcut_chordal(nv=2000, band=20, extra=1e-4, seed=0, merge=48):
+    """Max-cut SDP relaxation min ⟨C, X⟩ s.t. diag(X) = 1, X ⪰ 0 on a graph of nv vertices: the
+    band of half-width `band` plus extra edges drawn uniformly among all other vertex pairs, each
+    with probability `extra` (a binomial count, placed uniformly); weights U(0, 1); C = −L/4. X is
+    decomposed over the maximal cliques of a chordal extension (chordal_cliques), one PSD cone per
+    clique in svec form, with the clique-tree consensus rows X_child[i, j] = X_parent[i, j] on
+    every entry of each child's intersection with its parent, diag(X) = 1 rows on each vertex's
+    first clique, and each C entry in the first clique that holds it (the chordal conversion of
+    the reference's networked H₂ example, examples/sdp/networked_h2.jl:172-296). Cliques are merged
+    into their parents while the union stays within `merge` vertices (0: no merging). Its optimum
+    is the dense max-cut SDP's."""
+    rng = np.random.default_rng(seed)
+    W = {}
+    for i in range(nv):
+        for j in range(i + 1, min(nv, i + band + 1)):
+            W[(j, i)] = rng.uniform(0.0, 1.0)
+    npairs = nv * (nv - 1) // 2 - len(W)
+    k = int(rng.binomial(npairs, extra)) if npairs > 0 else 0
+    while k > 0:
+        i, j = (int(x) for x in rng.integers(0, nv, 2))
+        if i == j:
+            continue
+        a, b = max(i, j), min(i, j)
+        if (a, b) in W:
+            continue
+        W[(a, b)] = rng.uniform(0.0, 1.0)
+        k -= 1
+    cliques, cparent = chordal_cliques(nv, W.keys())
+    if merge:
+        cliques, cparent = merge_cliques(cliques, cparent, merge)
+    deg = np.zeros(nv)
+    for (j, i), w in W.items():
+        deg[i] += w
+        deg[j] += w
+    Cent = {(i, i): -deg[i] / 4.0 for i in range(nv)}
+    for (j, i), w in W.items():
+        Cent[(j, i)] = w / 4.0
+    sizes = [len(c) * (len(c) + 1) // 2 for c in cliques]
+    col0 = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(col0[-1])
+    where = [{v: a for a, v in enumerate(c)} for c in cliques]
+
+    def pos(k, i, j):  # global column of X_k[i, j] (vertices i, j of clique k)
+        d = len(cliques[k])
+        a, b = where[k][i], where[k][j]
+        if a < b:
+            a, b = b, a
+        return int(col0[k] + b * d - b * (b - 1) // 2 + (a - b))
+
+    r2 = math.sqrt(2.0)
+    f = np.zeros(n)
+    owner = set()
+    for k, c in enumerate(cliques):
+        for b_, j in enumerate(c):
+            for i in c[b_:]:
+                key = (max(i, j), min(i, j))
+                if key in Cent and key not in owner:
+                    owner.add(key)
+                    f[pos(k, i, j)] = -(Cent[key] if i == j else r2 * Cent[key])
+    rows, cols, vals, g = [], [], [], []
+    seen = set()
+    for k, c in enumerate(cliques):
+        for i in c:
+            if i not in seen:
+                seen.add(i)
+                rows.append(len(g)); cols.append(pos(k, i, i)); vals.append(1.0); g.append(1.0)
+    for k, c in enumerate(cliques):
+        q = cparent[k]
+        if q < 0:
+            continue
+        sep = sorted(set(c) & set(cliques[q]))
+        for b_, j in enumerate(sep):
+            for i in sep[b_:]:
+                r = len(g)
+                rows += [r, r]; cols += [pos(k, i, j), pos(q, i, j)]; vals += [1.0, -1.0]; g.append(0.0)
+    B = sp.csc_matrix((vals, (rows, cols)), shape=(len(g), n))
+    B.sort_indices()
+    return dict(Q=sp.csc_matrix((n, n)), B=B, f=f, g=np.array(g), K=[("psd",)] * len(cliques), s=sizes,
+                cliques=cliques, name=f"maxcut_chordal_nv{nv}_w{band}_x{extra}_s{seed}")
+
+
 def exppow(nexp=2000, npow=500, m=499, seed=0, dens=0.005, eps=0.1):
     rng = np.random.default_rng(seed)
     n = 3 * (nexp + npow)
