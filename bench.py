@@ -1,17 +1,20 @@
 """Benchmark of the MI355X conic IPM solve path.
 
-A step is one full IPM solve to OPTIMAL at gap_tol = feas_tol = 1e-8 of one seeded instance of
-the 2000-variable sparse QP workload (n=2000, m=1000, nonnegative orthant, Q = PᵀP + 1e-3 I with
-P at 0.5 % density, B at 1 % density). Every timed instance is distinct (its own seed) and
-resident in HBM — problem data, symbolic factor and buffers set up before the timed region — and
-is solved from the cold starting point. `--slots S` instances are solved at a time, each from its
-own host thread on its own HIP stream, so their kernels overlap on the GPU; the persistent
-factorization of each takes 1/S of the CUs. value = solves/s over the K timed solves (whole job);
-p50/p99 latency over the same solves; solo_ms = one instance alone on the whole GPU.
+A step is one solve(prob, settings) — the reference's own unit of measurement
+(examples/utils.jl:72-85 times `solve(prob, settings)`): init (symbolic analysis, equilibration,
+BᵀB, device buffers and uploads, starting point), the IPM loop to OPTIMAL at gap_tol = feas_tol =
+1e-8, and the result, for one seeded instance of the 2000-variable sparse QP workload (n=2000,
+m=1000, nonnegative orthant, Q = PᵀP + 1e-3 I with P at 0.5 % density, B at 1 % density). Every
+timed instance is distinct (its own seed); the problems are constructed (IPMProblem) before the
+timed region, as the reference's prob is. `--threads T` host threads run the solves, so one
+instance's host init overlaps the others' device work; each solver has its own HIP stream and the
+persistent factorization of each takes 1/`--slots` of the CUs. value = solves/s over the K timed
+solves (whole job); p50/p99 latency over the same solves. The IPM loop alone on instances
+already resident in HBM is reported beside it (config.resident_loop).
 
 roofline: the dominant kernel, the persistent FP64 MFMA Cholesky of the 2000×2000 front,
 timed live with HIP events on each solver's stream over the timed region; achieved = its
-algorithmic flops r³/3 per launch ÷ its measured average launch time (under the S-way sharing).
+algorithmic flops r³/3 per launch ÷ its measured average launch time (under the sharing).
 
 cpu_baseline: the oracle (numpy + LAPACK restatement of the reference path) on the host cores,
 one instance.
@@ -192,10 +195,78 @@ def _resident_run(args, mh, precision):
                 solo_ms=solo_ms, t_setup=t_setup)
 
 
+def _e2e_run(args, mh, precision):
+    """The reference's unit of work, solve(prob, settings) (examples/utils.jl:72-85 times exactly
+    this): init (symbolic analysis, equilibration, BᵀB, device buffers and uploads, starting
+    point) plus the IPM loop plus the result, from `--threads` host threads. The problems are
+    constructed (IPMProblem) before the timed region, as the reference's prob is."""
+    K, W, T, S = args.steps, args.warmup, max(1, args.threads), max(1, args.slots)
+    ncu = mh.device_info()["cus"]
+    grid = args.dag_grid if args.dag_grid > 0 else (max(16, ncu // S) if S > 1 else 0)
+    settings = mh.IPMSettings(dag_grid=grid, precision=precision)
+    kinds = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
+    good = (mh.IPMStatus.OPTIMAL, mh.IPMStatus.NEAR_OPTIMAL) if args.workload == "exppow" else (mh.IPMStatus.OPTIMAL,)
+    probs = [hip_problem(mh, workload(args.workload, 1000 + k, args.n, args.m)) for k in range(W + K)]
+    note(f"{args.workload} precision {precision}: {W + K} problems constructed")
+    lat = [0.0] * (W + K)
+    res = [None] * (W + K)
+    stats = [None] * (W + K)
+    prof = {}
+    lock = threading.Lock()
+
+    def one(i, profile):
+        a = time.perf_counter()
+        s = mh.init(probs[i], settings)
+        if profile:
+            s.set_profile(kinds)
+        r = s.solve()
+        lat[i] = time.perf_counter() - a
+        res[i] = r
+        stats[i] = s.stats()
+        if profile:
+            with lock:
+                for k, v in s.profile().items():
+                    acc = prof.setdefault(k, dict(ms=0.0, launches=0, flops=0.0, bytes=0.0))
+                    for key in acc:
+                        acc[key] += v[key]
+        s.close()
+
+    def run(lo, hi, profile):
+        nxt = [lo]
+
+        def worker():
+            while True:
+                with lock:
+                    i = nxt[0]
+                    nxt[0] += 1
+                if i >= hi:
+                    return
+                one(i, profile)
+
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=worker) for _ in range(T)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        return time.perf_counter() - t0
+
+    if W:
+        run(0, W, False)
+    elapsed = run(W, W + K, True)
+    note(f"{args.workload} precision {precision}: {K} timed solve(prob, settings) in {elapsed:.2f} s "
+         f"on {T} host threads")
+    r = res[W:]
+    assert all(x.status in good for x in r), [x.status for x in r]
+    return dict(settings=settings, elapsed=elapsed, lat_ms=np.array(lat[W:]) * 1e3, res=r, prof=prof,
+                stats=stats[W:])
+
+
 def bench_resident(args, mh):
     K, W, S = args.steps, args.warmup, max(1, args.slots)
-    plain = _resident_run(args, mh, 0)
-    mixed = _resident_run(args, mh, 1) if not args.no_mixed else None
+    plain = _e2e_run(args, mh, 0)
+    mixed = _e2e_run(args, mh, 1) if not args.no_mixed else None
+    loop = _resident_run(args, mh, 0) if not args.no_loop else None
     settings, elapsed, lat_ms, res, prof = (plain[k] for k in ("settings", "elapsed", "lat_ms", "res", "prof"))
     st0 = plain["stats"][0]
     fac = prof["factor"]
@@ -217,23 +288,24 @@ def bench_resident(args, mh):
         "data": "synthetic",
         "config": {
             "workload": WORKLOADS[args.workload].format(n=args.n, m=args.m) +
-                        f"; {K} distinct seeded instances, resident, cold start, gap_tol=feas_tol=1e-8, "
-                        f"{S} solved at a time",
+                        f"; {K} distinct seeded instances, each a whole solve(prob, settings) (init + IPM "
+                        f"loop + result) from a cold start to gap_tol=feas_tol=1e-8, on {args.threads} host "
+                        f"threads",
             "statuses": {mh.IPMStatus(int(x.status)).name: int(sum(int(y.status) == int(x.status) for y in res))
                          for x in res},
             "precision": "fp64 (plain); the mixed form is reported under mixed",
-            "slots": S,
+            "threads": args.threads,
             "dag_grid": settings.dag_grid,
             "p50_ms": float(np.percentile(lat_ms, 50)),
             "p99_ms": float(np.percentile(lat_ms, 99)),
-            "solo_ms": plain["solo_ms"],
+            "init_host_ms_mean": float(np.mean([x["init_host_ms"] for x in plain["stats"]])),
+            "init_device_ms_mean": float(np.mean([x["init_device_ms"] for x in plain["stats"]])),
             "iterations_mean": float(np.mean([x.niter for x in res])),
             "factor_tflops_aggregate": fac["flops"] / elapsed / 1e12,
             "factor_ms_per_call": fac["ms"] / max(1, fac["launches"]),
             "factor_exact_flops": st0["exact_flops"],
             "factor_front_flops": st0["front_flops"],
             "nfront": st0["nfront"],
-            "setup_s_per_instance": plain["t_setup"],
             "kernel_ms": {k: v["ms"] for k, v in prof.items()},
             "kernel_launches": {k: v["launches"] for k, v in prof.items()},
         },
@@ -251,6 +323,15 @@ def bench_resident(args, mh):
             "flops_per_launch": dom["flops"] / max(1, dom["launches"]),
         },
     }
+    if loop is not None:
+        line["config"]["resident_loop"] = {
+            "what": f"the IPM loop alone (solve! after init) on the same kind of instances, resident in HBM, "
+                    f"{S} at a time",
+            "solves_per_s": K / loop["elapsed"],
+            "p50_ms": float(np.percentile(loop["lat_ms"], 50)),
+            "p99_ms": float(np.percentile(loop["lat_ms"], 99)),
+            "solo_ms": loop["solo_ms"],
+        }
     if mixed is not None:
         # the same seeded instances: per-result error of the mixed form against the FP64 results
         err = [max(abs(a.pobj - b.pobj) / max(1.0, abs(b.pobj)), abs(a.dobj - b.dobj) / max(1.0, abs(b.dobj)))
@@ -262,7 +343,6 @@ def bench_resident(args, mh):
             "vs_plain": (K / mixed["elapsed"]) / (K / elapsed),
             "p50_ms": float(np.percentile(mixed["lat_ms"], 50)),
             "p99_ms": float(np.percentile(mixed["lat_ms"], 99)),
-            "solo_ms": mixed["solo_ms"],
             "iterations_mean": float(np.mean([x.niter for x in mixed["res"]])),
             "fp32_factor_share": float(np.sum([s["factor_calls_fp32"] for s in mixed["stats"]]) /
                                        max(1, np.sum([s["factor_calls"] for s in mixed["stats"]]))),
@@ -347,7 +427,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=4,
+                    help="the CUs of a large-front factorization are 1/slots of the chip; the resident-loop "
+                         "measurement solves this many at a time")
+    ap.add_argument("--threads", type=int, default=12, help="host threads running solve(prob, settings)")
+    ap.add_argument("--no-loop", action="store_true", help="skip the resident-loop measurement")
     ap.add_argument("--dag-grid", type=int, default=0,
                     help="workgroups of each large-front factorization (0: the CUs shared among the slots)")
     ap.add_argument("--n", type=int, default=2000)
@@ -370,7 +454,7 @@ def main():
         sys.exit(2)
     # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); slots beyond that
     # would share a queue and run their kernels one after another, so give each slot its own
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, args.slots))))
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, args.slots, args.threads))))
     import mumblebee_hip as mh
     line = bench_serve(args, mh) if args.serve else bench_resident(args, mh)
     print(json.dumps(line))

@@ -117,9 +117,10 @@ int64_t mb_solver_niter(const mb_solver* s);
  * delta mu chi (history/ipm.jl:1-24) */
 int mb_solver_history(const mb_solver* s, double* rows);
 
-/* stats[0..7] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
+/* stats[0..9] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
  * total factorization ms, factorization calls, last factorization ms, factorizations whose
- * large fronts were FP32 (precision 1) */
+ * large fronts were FP32 (precision 1), init ms on the host (symbolic analysis, equilibration,
+ * BᵀB), init ms of the device setup (buffers, uploads, starting point) */
 int mb_solver_stats(const mb_solver* s, double* stats);
 
 /* reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429): new starting point, empty history;

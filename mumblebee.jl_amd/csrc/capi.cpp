@@ -60,10 +60,7 @@ static mb::CSC make_csc(int64_t m, int64_t n, const int64_t* colptr, const int64
 
 static void ensure_symbolic(mb_problem* p, int amalg) {
   if (p->sym_amalg == amalg) return;
-  std::vector<int64_t> w;
-  std::vector<std::vector<int64_t>> adj;
-  mb::vertex_graph(p->P.B, p->P.Q, w, adj);
-  p->sym = mb::analyse(w, adj, amalg != 0);
+  p->sym = mb::analyse_kkt(p->P.B, p->P.Q, amalg != 0);
   p->sym_amalg = amalg;
 }
 
@@ -285,6 +282,8 @@ int mb_solver_stats(const mb_solver* s, double* st) {
     st[5] = s->S->factor_calls();
     st[6] = s->S->last_factor_ms();
     st[7] = s->S->factor_calls_fp32();
+    st[8] = s->S->host_setup_ms();
+    st[9] = s->S->device_setup_ms();
     return 0;
   });
 }

@@ -81,6 +81,9 @@ struct Symbolic {
 // Ordering, etree, postorder, fundamental supernodes, relaxed amalgamation.
 Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj,
                  bool amalgamate);
+// analyse() of vertex_graph(B, Q), with the graph set straight into the ordering's bitsets when
+// the vertex count allows; the same result
+Symbolic analyse_kkt(const BSM& B, const BSM& Q, bool amalgamate);
 void vertex_graph(const BSM& B, const BSM& Q, std::vector<i64>& weights,
                   std::vector<std::vector<i64>>& adj);
 

@@ -519,52 +519,63 @@ BSM transpose_bsm(const BSM& A) {
   return R;
 }
 
-// uzawa.jl:85 (`B' * B`), pattern as matprod_dest (block_sparse_matrix.jl:1171-1252)
+// uzawa.jl:85 (`B' * B`), pattern as matprod_dest (block_sparse_matrix.jl:1171-1252). For each
+// vertex v: its neighbours w (vertices sharing an out vertex with v) in increasing order, each
+// block Σ_rows B_wᵀ B_v accumulated in arc order straight into the result.
 BSM gram(const BSM& B) {
-  std::vector<std::vector<std::pair<i64, i64>>> touch(B.nout);
-  for (i64 v = 0; v < B.nvtx; ++v)
-    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) touch[B.tgt[e]].emplace_back(v, e);
+  const i64 N = B.nvtx, U = B.nout, NA = B.narc();
+  std::vector<i64> tp(U + 1, 0), tv(NA), te(NA);  // out vertex → (vertex, arc), by vertex
+  for (i64 e = 0; e < NA; ++e) tp[B.tgt[e] + 1]++;
+  for (i64 u = 0; u < U; ++u) tp[u + 1] += tp[u];
+  {
+    std::vector<i64> nx(tp.begin(), tp.end() - 1);
+    for (i64 v = 0; v < N; ++v)
+      for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
+        const i64 k = nx[B.tgt[e]]++;
+        tv[k] = v;
+        te[k] = e;
+      }
+  }
   BSM R;
-  R.nout = R.nvtx = B.nvtx;
+  R.nout = R.nvtx = N;
   R.xcol = B.xcol;
   R.xrow = B.xcol;
+  R.xsrc.reserve(N + 1);
   R.xsrc.push_back(0);
   R.xblk.push_back(0);
-  std::vector<i64> mark(B.nvtx, -1), nbr;
-  std::vector<std::vector<std::pair<i64, i64>>> pairs(B.nvtx);
-  for (i64 v = 0; v < B.nvtx; ++v) {
+  std::vector<i64> mark(N, -1), slot(N, 0), nbr;
+  for (i64 v = 0; v < N; ++v) {
     nbr.clear();
     for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e)
-      for (auto& wf : touch[B.tgt[e]]) {
-        i64 w = wf.first;
+      for (i64 k = tp[B.tgt[e]]; k < tp[B.tgt[e] + 1]; ++k) {
+        const i64 w = tv[k];
         if (mark[w] != v) {
           mark[w] = v;
           nbr.push_back(w);
-          pairs[w].clear();
         }
-        pairs[w].emplace_back(e, wf.second);
       }
     std::sort(nbr.begin(), nbr.end());
-    i64 nv = B.ncols(v);
+    const i64 nv = B.ncols(v);
     for (i64 w : nbr) {
-      i64 nw = B.ncols(w);
-      size_t base = R.val.size();
-      R.val.resize(base + nw * nv, 0.0);
-      double* blk = R.val.data() + base;
-      for (auto& ef : pairs[w]) {
-        i64 e = ef.first, f = ef.second;
-        i64 nr = B.nrows(B.tgt[e]);
-        const double* Be = B.val.data() + B.xblk[e];
-        const double* Bf = B.val.data() + B.xblk[f];
-        for (i64 j = 0; j < nv; ++j)
-          for (i64 i = 0; i < nw; ++i) {
-            double acc = 0.0;
-            for (i64 k = 0; k < nr; ++k) acc += Bf[i * nr + k] * Be[j * nr + k];
-            blk[j * nw + i] += acc;
-          }
-      }
+      slot[w] = (i64)R.val.size();
+      R.val.resize(R.val.size() + B.ncols(w) * nv, 0.0);
       R.tgt.push_back(w);
       R.xblk.push_back((i64)R.val.size());
+    }
+    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
+      const i64 nr = B.nrows(B.tgt[e]);
+      const double* Be = B.val.data() + B.xblk[e];
+      for (i64 k = tp[B.tgt[e]]; k < tp[B.tgt[e] + 1]; ++k) {
+        const i64 w = tv[k], nw = B.ncols(w);
+        const double* Bf = B.val.data() + B.xblk[te[k]];
+        double* blk = R.val.data() + slot[w];
+        for (i64 jj = 0; jj < nv; ++jj)
+          for (i64 ii = 0; ii < nw; ++ii) {
+            double acc = 0.0;
+            for (i64 q = 0; q < nr; ++q) acc += Bf[ii * nr + q] * Be[jj * nr + q];
+            blk[jj * nw + ii] += acc;
+          }
+      }
     }
     R.xsrc.push_back((i64)R.tgt.size());
   }

@@ -8,6 +8,7 @@
 // supernodes, then relaxed amalgamation so that fronts are dense enough for MFMA tiles.
 #include <algorithm>
 #include <cstdint>
+#include <queue>
 #include <set>
 #include <stdexcept>
 
@@ -17,30 +18,45 @@ namespace mb {
 
 void vertex_graph(const BSM& B, const BSM& Q, std::vector<i64>& weights,
                   std::vector<std::vector<i64>>& adj) {
-  const i64 N = B.nvtx;
+  const i64 N = B.nvtx, U = B.nout;
   weights.resize(N);
   for (i64 v = 0; v < N; ++v) weights[v] = B.ncols(v);
-  std::vector<std::vector<i64>> rows(B.nout);
-  for (i64 v = 0; v < N; ++v)
-    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) rows[B.tgt[e]].push_back(v);
-  std::vector<std::vector<i64>> a(N);
-  for (i64 v = 0; v < Q.nvtx; ++v)
-    for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) {
-      i64 u = Q.tgt[e];
-      if (u != v) {
-        a[u].push_back(v);
-        a[v].push_back(u);
-      }
-    }
-  for (auto& lst : rows)
-    for (i64 x : lst)
-      for (i64 y : lst)
-        if (x != y) a[x].push_back(y);
-  for (auto& l : a) {
-    std::sort(l.begin(), l.end());
-    l.erase(std::unique(l.begin(), l.end()), l.end());
+  // the vertices of each out vertex (row block) of B, and the transposed pattern of Q (so that
+  // the graph is the symmetric closure whatever Q stores), as compressed lists
+  std::vector<i64> rp(U + 1, 0), rv(B.narc());
+  for (i64 e = 0; e < B.narc(); ++e) rp[B.tgt[e] + 1]++;
+  for (i64 u = 0; u < U; ++u) rp[u + 1] += rp[u];
+  {
+    std::vector<i64> nx(rp.begin(), rp.end() - 1);
+    for (i64 v = 0; v < N; ++v)
+      for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) rv[nx[B.tgt[e]]++] = v;
   }
-  adj.swap(a);
+  std::vector<i64> qp(N + 1, 0), qv(Q.narc());
+  for (i64 e = 0; e < Q.narc(); ++e) qp[Q.tgt[e] + 1]++;
+  for (i64 u = 0; u < N; ++u) qp[u + 1] += qp[u];
+  {
+    std::vector<i64> nx(qp.begin(), qp.end() - 1);
+    for (i64 v = 0; v < Q.nvtx; ++v)
+      for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) qv[nx[Q.tgt[e]]++] = v;
+  }
+  std::vector<i64> mark(N, -1);
+  adj.assign(N, {});
+  for (i64 v = 0; v < N; ++v) {
+    std::vector<i64>& a = adj[v];
+    mark[v] = v;  // no self loop
+    auto add = [&](i64 u) {
+      if (mark[u] != v) {
+        mark[u] = v;
+        a.push_back(u);
+      }
+    };
+    if (v < Q.nvtx)
+      for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) add(Q.tgt[e]);
+    for (i64 k = qp[v]; k < qp[v + 1]; ++k) add(qv[k]);
+    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e)
+      for (i64 k = rp[B.tgt[e]]; k < rp[B.tgt[e] + 1]; ++k) add(rv[k]);
+    std::sort(a.begin(), a.end());
+  }
 }
 
 namespace {
@@ -159,6 +175,8 @@ struct MinDegreeBits {
   std::vector<uint64_t> R;  // N rows of nw words
   std::vector<i64> deg;
 
+  i64 w0 = 0;  // the common weight when every vertex has it (else 0): degrees by popcount
+
   MinDegreeBits(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj)
       : N((i64)weights.size()), nw(((i64)weights.size() + 63) / 64), w(weights),
         R((size_t)((i64)weights.size() * (((i64)weights.size() + 63) / 64)), 0), deg(weights.size(), 0) {
@@ -167,6 +185,47 @@ struct MinDegreeBits {
         R[(size_t)(i * nw + j / 64)] |= 1ull << (j % 64);
         deg[i] += w[j];
       }
+    set_uniform();
+  }
+
+  // the vertex graph of vertex_graph (pattern(Q) ∪ pattern(Qᵀ) ∪ linegraph(B), no self loops)
+  // set straight into the reach bitsets
+  MinDegreeBits(const std::vector<i64>& weights, const BSM& B, const BSM& Q)
+      : N((i64)weights.size()), nw(((i64)weights.size() + 63) / 64), w(weights),
+        R((size_t)((i64)weights.size() * (((i64)weights.size() + 63) / 64)), 0), deg(weights.size(), 0) {
+    auto set = [&](i64 i, i64 j) { R[(size_t)(i * nw + j / 64)] |= 1ull << (j % 64); };
+    for (i64 v = 0; v < Q.nvtx; ++v)
+      for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) {
+        set(v, Q.tgt[e]);
+        set(Q.tgt[e], v);
+      }
+    std::vector<i64> rp(B.nout + 1, 0), rv(B.narc());
+    for (i64 e = 0; e < B.narc(); ++e) rp[B.tgt[e] + 1]++;
+    for (i64 u = 0; u < B.nout; ++u) rp[u + 1] += rp[u];
+    std::vector<i64> nx(rp.begin(), rp.end() - 1);
+    for (i64 v = 0; v < B.nvtx; ++v)
+      for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) rv[nx[B.tgt[e]]++] = v;
+    for (i64 u = 0; u < B.nout; ++u)
+      for (i64 a = rp[u]; a < rp[u + 1]; ++a)
+        for (i64 b = rp[u]; b < rp[u + 1]; ++b) set(rv[a], rv[b]);
+    for (i64 i = 0; i < N; ++i) R[(size_t)(i * nw + i / 64)] &= ~(1ull << (i % 64));
+    set_uniform();
+    for (i64 i = 0; i < N; ++i) {
+      const uint64_t* r = &R[(size_t)(i * nw)];
+      i64 d = 0;
+      for (i64 k = 0; k < nw; ++k) {
+        if (w0) d += w0 * __builtin_popcountll(r[k]);
+        else
+          for (uint64_t b = r[k]; b; b &= b - 1) d += w[k * 64 + __builtin_ctzll(b)];
+      }
+      deg[i] = d;
+    }
+  }
+
+  void set_uniform() {
+    w0 = N > 0 ? w[0] : 0;
+    for (i64 x : w)
+      if (x != w0) w0 = 0;
   }
 
   void members(i64 i, std::vector<i64>& out) const {
@@ -181,13 +240,21 @@ struct MinDegreeBits {
     strct.assign(N, {});
     i64 W = 0;
     for (i64 x : w) W += x;
-    std::set<std::pair<i64, i64>> pq;
-    for (i64 i = 0; i < N; ++i) pq.insert({deg[i], i});
+    // lazy min-heap of (degree, vertex): an entry is current while its degree is the vertex's
+    // and the vertex is not eliminated, so the first current entry is the set minimum that an
+    // ordered set would give (ties to the smallest vertex)
+    typedef std::pair<i64, i64> Item;
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+    std::vector<char> gone(N, 0);
+    for (i64 i = 0; i < N; ++i) pq.push({deg[i], i});
     std::vector<i64> Lp;
     std::vector<uint64_t> lp(nw);
-    while (!pq.empty()) {
-      i64 p = pq.begin()->second;
-      pq.erase(pq.begin());
+    for (;;) {
+      while (!pq.empty() && (gone[pq.top().second] || pq.top().first != deg[pq.top().second])) pq.pop();
+      if (pq.empty()) break;
+      i64 p = pq.top().second;
+      pq.pop();
+      gone[p] = 1;
       members(p, Lp);  // sorted
       if (deg[p] + w[p] == W) {
         // p reaches every remaining variable: after it the rest is a clique, where minimum
@@ -199,11 +266,12 @@ struct MinDegreeBits {
           if (w[a] != w[b]) return w[a] > w[b];
           return a < b;
         });
+        const bool sorted = std::is_sorted(rest.begin(), rest.end());
         for (size_t k = 0; k < rest.size(); ++k) {
           order.push_back(rest[k]);
           std::vector<i64> s(rest.begin() + k + 1, rest.end());
-          std::sort(s.begin(), s.end());
-          strct[rest[k]] = s;
+          if (!sorted) std::sort(s.begin(), s.end());
+          strct[rest[k]] = std::move(s);
         }
         break;
       }
@@ -217,19 +285,25 @@ struct MinDegreeBits {
         for (i64 k = 0; k < nw; ++k) {
           uint64_t nb = lp[k] & ~r[k];
           if (k == i / 64) nb &= ~(1ull << (i % 64));
-          for (uint64_t b = nb; b; b &= b - 1) d += w[k * 64 + __builtin_ctzll(b)];
+          if (w0) d += w0 * __builtin_popcountll(nb);
+          else
+            for (uint64_t b = nb; b; b &= b - 1) d += w[k * 64 + __builtin_ctzll(b)];
           r[k] |= nb;
         }
         r[p / 64] &= ~(1ull << (p % 64));
-        pq.erase({deg[i], i});
-        deg[i] = d;
-        pq.insert({deg[i], i});
+        if (d != deg[i]) {
+          deg[i] = d;
+          pq.push({d, i});
+        }
       }
     }
   }
 };
 
 }  // namespace
+
+static Symbolic finish(const std::vector<i64>& weights, std::vector<i64>& order,
+                       std::vector<std::vector<i64>>& strct, bool amalgamate);
 
 Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj,
                  bool amalgamate) {
@@ -243,6 +317,31 @@ Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<
     MinDegree md(weights, adj);
     md.run(order, strct);
   }
+  return finish(weights, order, strct, amalgamate);
+}
+
+Symbolic analyse_kkt(const BSM& B, const BSM& Q, bool amalgamate) {
+  const i64 N = B.nvtx;
+  if (N > 16384) {
+    std::vector<i64> w;
+    std::vector<std::vector<i64>> adj;
+    vertex_graph(B, Q, w, adj);
+    return analyse(w, adj, amalgamate);
+  }
+  std::vector<i64> weights(N);
+  for (i64 v = 0; v < N; ++v) weights[v] = B.ncols(v);
+  std::vector<i64> order;
+  std::vector<std::vector<i64>> strct;
+  MinDegreeBits md(weights, B, Q);
+  md.run(order, strct);
+  return finish(weights, order, strct, amalgamate);
+}
+
+// etree, postorder, fundamental supernodes and relaxed amalgamation of an elimination order with
+// its structures (strct[v]: the later neighbours of v, sorted)
+static Symbolic finish(const std::vector<i64>& weights, std::vector<i64>& order,
+                       std::vector<std::vector<i64>>& strct, bool amalgamate) {
+  const i64 N = (i64)weights.size();
   std::vector<i64> pos(N);
   for (i64 k = 0; k < N; ++k) pos[order[k]] = k;
   std::vector<i64> parent(N, -1);
@@ -282,8 +381,9 @@ Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<
   for (i64 k = 0; k < N; ++k) {
     i64 v = post[k];
     npar[k] = parent[v] >= 0 ? newpos[parent[v]] : -1;
+    nstr[k].reserve(strct[v].size());
     for (i64 u : strct[v]) nstr[k].push_back(newpos[u]);
-    std::sort(nstr[k].begin(), nstr[k].end());
+    if (!std::is_sorted(nstr[k].begin(), nstr[k].end())) std::sort(nstr[k].begin(), nstr[k].end());
   }
   for (i64 k = 0; k < N; ++k)
     if (npar[k] >= 0) nchild[npar[k]]++;

@@ -3,6 +3,7 @@
 #include "tdc.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -248,7 +249,12 @@ Solver::Solver(const Problem& prob, const Settings& st, const double* p0, const 
 
 Solver::Solver(const Problem& prob, const Settings& st, Deferred) : set_(st) { host_setup(prob); }
 
+static double wall_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void Solver::host_setup(const Problem& prob) {
+  const double t0 = wall_ms();
   for (const Cone& c : prob.K)
     if (c.kind < CONE_POS || c.kind > CONE_POW)
       throw std::invalid_argument("cone kind not supported by this build's device path");
@@ -275,10 +281,7 @@ void Solver::host_setup(const Problem& prob) {
   for (i64 v = 0; v < prob.B.nvtx; ++v) nu_ += cone_degree(prob.K[v], prob.B.ncols(v));
   mu_t_ = prob.mu;
   // symbkkt (kkt.jl:23-53)
-  std::vector<i64> w;
-  std::vector<std::vector<i64>> adj;
-  vertex_graph(prob.B, prob.Q, w, adj);
-  sym_ = analyse(w, adj, set_.amalgamate != 0);
+  sym_ = analyse_kkt(prob.B, prob.Q, set_.amalgamate != 0);
   const std::vector<i64>& vp = sym_.vperm;
   P_.Q = symselectvtxs(prob.Q, vp);
   P_.B = selectvtxs(prob.B, vp);
@@ -308,6 +311,7 @@ void Solver::host_setup(const Problem& prob) {
   sg_ = scalenorm(P_.g, yscl_);
   sf_ = scalenorm(P_.f, pscl_);
   L_ = gram(P_.B);
+  host_ms_ = wall_ms() - t0;
 }
 
 // An upper bound of what device_setup allocates: every buffer of build_device and
@@ -389,6 +393,7 @@ double Solver::estimate_device_bytes() const {
 }
 
 void Solver::device_setup(const double* p0, const double* d0, const double* y0) {
+  const double t0 = wall_ms();
   g_dev_counter = &dev_bytes_;
   try {
     HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -403,6 +408,7 @@ void Solver::device_setup(const double* p0, const double* d0, const double* y0) 
   }
   g_dev_counter = nullptr;
   reinit(p0, d0, y0);
+  device_ms_ = wall_ms() - t0;
 }
 
 Solver::~Solver() {

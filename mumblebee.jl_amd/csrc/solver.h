@@ -111,6 +111,8 @@ class Solver {
   double total_factor_ms() const { return fac_ms_tot_; }
   int factor_calls() const { return fac_calls_; }
   int factor_calls_fp32() const { return fac_fp32_calls_; }
+  double host_setup_ms() const { return host_ms_; }      // init on the host: analysis, scaling, BᵀB
+  double device_setup_ms() const { return device_ms_; }  // init on the device: buffers, uploads, start
   void dense_factor(double* out);  // n×n lower factor (debug; small n)
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
   // cone fan-outs on the iterate, in the solver's internal (permuted, scaled) coordinates
@@ -268,6 +270,7 @@ class Solver {
   double cur_mu_ = 0.0;   // μ of the iteration being factored (selects the factor precision)
   int fac_fp32_calls_ = 0;
   size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
+  double host_ms_ = 0.0, device_ms_ = 0.0;
   void check_abort();
 };
 
