@@ -39,6 +39,7 @@ import numpy as np  # noqa: E402
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
 METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP/s"
+ARGS = None
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # scaled by a 256 MiB device copy (profiles/k_dag_factor_qp2000_fp64.md); keyed by (n, m)
 TRAFFIC_BYTES = {(2000, 1000): 141.5e6}
@@ -48,10 +49,13 @@ WORKLOADS = {
     "qp": "sparse convex QP n={n} m={m} on the nonnegative orthant, Q=PᵀP+1e-3I (P 0.5% dense), B 1% dense, N(0,1)",
     "socp": "SOCP n=10000 m=5000: 500 second-order cones (sizes U{{3..50}} scaled to 9000 columns) + nonneg, "
             "B 0.2% dense, N(0,1), strictly feasible g, dual-feasible f",
-    "maxcut": "max-cut SDP relaxation, 2000 vertices, band 20 + 0.1% extra edges inside windows, C = -L/4, "
-              "chordal windows of 48 vertices (71 PSD cones of order 48, diag and consensus rows)",
-    "exppow": "2000 exponential cones (entropy on the simplex) + 500 power cones (alpha U(0.2,0.8)) with budgets, "
-              "499 sparse equalities at 0.5% density, strictly feasible g",
+    "maxcut": "max-cut SDP relaxation, 2000 vertices, band 20 + uniformly random extra edges with probability "
+              "{extra} per vertex pair, weights U(0,1), C = -L/4, decomposed over the maximal cliques of a chordal "
+              "extension (minimum-degree elimination, clique tree, cliques merged up to 48 vertices), one PSD cone "
+              "per clique, consensus rows on clique-tree intersections, diag(X) = 1",
+    "exppow": "2000 exponential cones (entropy on the simplex) + 500 power cones (alpha U(0.2,0.8)) with budgets "
+              "+ one {psd}x{psd} PSD block (dense symmetrised N(0,1) C, tr X = 1), 3000 sparse equalities at 0.5% "
+              "density over all columns, strictly feasible g",
 }
 
 
@@ -62,9 +66,9 @@ def workload(kind, seed, n, m):
     if kind == "socp":
         return G.socp(seed=seed)
     if kind == "maxcut":
-        return G.maxcut(nv=2000, band=20, clique=48, seed=seed)
+        return G.maxcut_chordal(nv=2000, band=20, extra=ARGS.maxcut_extra, seed=seed, merge=48)
     if kind == "exppow":
-        return G.exppow(seed=seed)
+        return G.exppow(nexp=2000, npow=500, m=3000, psd=ARGS.psd, rows="all", seed=seed)
     raise ValueError(kind)
 
 
@@ -287,7 +291,7 @@ def bench_resident(args, mh):
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOADS[args.workload].format(n=args.n, m=args.m) +
+            "workload": WORKLOADS[args.workload].format(n=args.n, m=args.m, extra=args.maxcut_extra, psd=args.psd) +
                         f"; {K} distinct seeded instances, each a whole solve(prob, settings) (init + IPM "
                         f"loop + result) from a cold start to gap_tol=feas_tol=1e-8, on {args.threads} host "
                         f"threads",
@@ -364,15 +368,7 @@ def bench_serve(args, mh):
     K = args.steps
     reqs = []
     for k in range(K):
-        n = int(G.log_uniform_n(rng, 50, args.serve_nmax))
-        u = rng.random()
-        if u < 0.6:
-            d = G.qp(n=n, m=max(1, n // 2), seed=5000 + k)
-        elif u < 0.9:
-            d = G.socp(n=n, m=max(1, n // 2), ncone=max(1, n // 40), seed=5000 + k, adens=min(0.2, 20.0 / n))
-        else:  # a small max-cut SDP: windows of 10 vertices (PSD cones of order 10, 55 columns each)
-            d = G.maxcut(nv=max(12, n // 25), band=4, clique=10, seed=5000 + k)
-        reqs.append(hip_problem(mh, d))
+        reqs.append(hip_problem(mh, G.served_request(rng, k, args.serve_nmax)))
     gaps = rng.exponential(1.0 / args.rate, K)
     server = mh.IPMServer(nslots=args.slots)
     settings = mh.IPMSettings(precision=args.precision)
@@ -399,7 +395,7 @@ def bench_serve(args, mh):
         "metric": METRIC, "value": K / elapsed, "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": 0,
         "ms_per_step": elapsed / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"served stream: {K} requests, 60% QP / 30% SOCP / 10% small max-cut SDP, n "
+        "config": {"workload": f"served stream: {K} requests, 60% QP / 30% SOCP / 10% chordal max-cut SDP, n "
                                f"log-uniform in [50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, "
                                f"{args.slots} slots",
                    "precision": "mixed" if args.precision else "fp64",
@@ -444,7 +440,12 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--serve-nmax", type=int, default=4000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    ap.add_argument("--maxcut-extra", type=float, default=1e-4,
+                    help="max-cut: probability of an extra edge per vertex pair")
+    ap.add_argument("--psd", type=int, default=128, help="exp/pow: order of the PSD block")
     args = ap.parse_args()
+    global ARGS
+    ARGS = args
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

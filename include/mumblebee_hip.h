@@ -5,6 +5,8 @@
  * (paths relative to the Mumblebee.jl repository root):
  *
  *   mb_problem_create        IPMProblem(Q, B, f, g, μ, K, s; compress)   src/IPM/src/ipm/problem.jl:64-127
+ *   mb_problem_create_bsm    IPMProblem(Q::BlockSparseMatrix, B::BlockSparseMatrix, f, g, μ, K)
+ *                                                                          src/IPM/src/ipm/problem.jl:49-53
  *   mb_problem_symbolic*     symbkkt → CliqueTrees.symbolic               src/IPM/src/kkt/kkt.jl:23-53
  *   mb_settings_default      IPMSettings{Float64}() defaults              src/IPM/src/ipm/settings/ipm.jl:1-18
  *   mb_solver_create         init(prob, settings; p0, d0, y0)             src/IPM/src/ipm/solver/ipm.jl:431-491
@@ -13,6 +15,7 @@
  *   mb_solver_result         result(solver, status) → IPMResult           src/IPM/src/ipm/solver/ipm.jl:30-74
  *   mb_solver_history        IPMHistory rows                              src/IPM/src/ipm/history/ipm.jl:1-24
  *   mb_solver_kkt_factor     initkkt!(kkt, H; δ) → (ok, ρ)                src/IPM/src/kkt/uzawa.jl:73-115
+ *   mb_solver_kkt_factor_h   initkkt!(kkt, H; δ) with the caller's H       src/IPM/src/kkt/uzawa.jl:73-76
  *   mb_solver_kkt_solve      solvekkt!(fstop, gstop, kkt, x, y, H, B, f, g; …)  src/IPM/src/kkt/uzawa.jl:265-344
  *   mb_cone_scale            scale!(sched, K, H, Q, caches, p, d, …)      src/IPM/src/cone/cone.jl:146-247
  *   mb_cone_corr0            initpredictor! / corr0!                      src/IPM/src/cone/cone.jl:361-444
@@ -78,6 +81,23 @@ mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, con
                               const double* B_nzval, const double* f, const double* g, double mu,
                               int64_t ncone, const int32_t* cone_kind, const double* cone_param,
                               const int64_t* cone_size, double compress);
+/* IPMProblem(Q::BlockSparseMatrix, B::BlockSparseMatrix, f, g, μ, K) — problem.jl:49-53, the form
+ * every reference example builds with blocksparse(...): the caller's block-sparse Q and B as they
+ * are (no column partition or compression; identity P1, P2). The arrays are those of
+ * block_sparse_matrix.jl:10-109: vertex v owns columns xcol[v] .. xcol[v+1]−1, out vertex u rows
+ * xrow[u] .. xrow[u+1]−1; arc e of vertex v (xsrc[v] .. xsrc[v+1]−1) targets out vertex tgt[e] with
+ * the dense column-major nrows(tgt[e]) × ncols(v) block val[xblk[e] .. xblk[e+1]−1]; targets
+ * strictly increasing within a vertex. Q is nvtx × nvtx over the same partition xcol (its out
+ * vertices are the vertices). index_base = 1 takes Julia's 1-based arrays unchanged (every offset
+ * and index is read minus one); 0 takes 0-based arrays. cone_kind[v] / cone_param[v]: the cone
+ * of vertex v. f (length n = xcol[nvtx] − base) and g (m = xrow[nout] − base) in the same column
+ * and row order. Returns NULL on an inconsistent layout, with the reason in mb_last_error(). */
+mb_problem* mb_problem_create_bsm(int64_t nvtx, int64_t nout, const int64_t* xcol, const int64_t* xrow,
+                                  const int64_t* B_xsrc, const int64_t* B_tgt, const int64_t* B_xblk,
+                                  const double* B_val, const int64_t* Q_xsrc, const int64_t* Q_tgt,
+                                  const int64_t* Q_xblk, const double* Q_val, const double* f, const double* g,
+                                  double mu, const int32_t* cone_kind, const double* cone_param,
+                                  int32_t index_base);
 void mb_problem_destroy(mb_problem* p);
 
 /* dims[0..6] = nvtx, nout, B narc, B nbnz, Q narc, Q nbnz, n */
@@ -147,6 +167,17 @@ int mb_solver_dag_trace(mb_solver* s, int32_t front, int64_t* out, int32_t len);
 /* initkkt!(kkt, H; δ) on the solver's current H (uzawa.jl:73-115): returns 1 on success,
  * 0 on a failed factorization, negative on error; *rho receives the shift used. */
 int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);
+
+/* initkkt!(kkt, H; δ) with the caller's H (uzawa.jl:73-76): H block-sparse over the solver's
+ * internal vertex partition (mb_solver_layout's xcol, for rows and columns alike; arcs H_xsrc /
+ * H_tgt / H_xblk / H_val as in mb_problem_create_bsm, both triangles, index_base 0 or 1) in its
+ * internal coordinates. Factors δH + BᵀB (retry with ρ as mb_solver_kkt_factor); the following
+ * mb_solver_kkt_solve calls use this H in their residuals, until the next mb_cone_scale,
+ * mb_solver_step or mb_solver_kkt_factor brings back the solver's own H (Q and the cone blocks).
+ * H's lower entries must lie in the factor's structure (the pattern of Q, BᵀB and the diagonal
+ * blocks): else an error. Returns 1 on success, 0 on a failed factorization, negative on error. */
+int mb_solver_kkt_factor_h(mb_solver* s, const int64_t* H_xsrc, const int64_t* H_tgt, const int64_t* H_xblk,
+                           const double* H_val, int32_t index_base, double delta, double* rho);
 
 /* Cone fan-outs on the solver's iterate in its internal coordinates (columns permuted by the
  * problem construction and scaled by the equilibration, as the reference's IPMSolver holds p and

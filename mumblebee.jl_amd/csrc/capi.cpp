@@ -144,6 +144,86 @@ mb_problem* mb_problem_create(int64_t n, int64_t m, const int64_t* Q_colptr, con
   return out;
 }
 
+// One block-sparse matrix from the caller's arrays in the layout of block_sparse_matrix.jl:10-109
+// (index_base 1: Julia's 1-based arrays as they are; 0: 0-based), checked: monotone offsets,
+// targets in range and strictly increasing within each vertex (block_sparse_matrix.jl:703-711),
+// block lengths nrows(tgt) × ncols(v).
+static mb::BSM make_bsm(int64_t nout, int64_t nvtx, const int64_t* xcol, const int64_t* xrow, const int64_t* xsrc,
+                        const int64_t* tgt, const int64_t* xblk, const double* val, int64_t base, const char* name) {
+  auto bad = [&](const std::string& what) { throw std::invalid_argument(std::string(name) + ": " + what); };
+  if (nout < 0 || nvtx < 0) bad("negative size");
+  if (!xcol || !xrow || !xsrc) bad("null offset array");
+  mb::BSM A;
+  A.nout = nout;
+  A.nvtx = nvtx;
+  A.xcol.resize(nvtx + 1);
+  A.xrow.resize(nout + 1);
+  A.xsrc.resize(nvtx + 1);
+  for (int64_t v = 0; v <= nvtx; ++v) {
+    A.xcol[v] = xcol[v] - base;
+    A.xsrc[v] = xsrc[v] - base;
+    if (v > 0 && (A.xcol[v] < A.xcol[v - 1] || A.xsrc[v] < A.xsrc[v - 1])) bad("xcol / xsrc not monotone");
+  }
+  for (int64_t u = 0; u <= nout; ++u) {
+    A.xrow[u] = xrow[u] - base;
+    if (u > 0 && A.xrow[u] < A.xrow[u - 1]) bad("xrow not monotone");
+  }
+  if (A.xcol[0] != 0 || A.xrow[0] != 0 || A.xsrc[0] != 0) bad("offsets must start at the index base");
+  const int64_t narc = A.xsrc[nvtx];
+  if (narc > 0 && (!tgt || !xblk)) bad("null arc array");
+  A.tgt.resize(narc);
+  A.xblk.resize(narc + 1);
+  A.xblk[0] = narc >= 0 && xblk ? xblk[0] - base : 0;
+  if (A.xblk[0] != 0) bad("xblk must start at the index base");
+  for (int64_t v = 0; v < nvtx; ++v)
+    for (int64_t e = A.xsrc[v]; e < A.xsrc[v + 1]; ++e) {
+      A.tgt[e] = tgt[e] - base;
+      A.xblk[e + 1] = xblk[e + 1] - base;
+      if (A.tgt[e] < 0 || A.tgt[e] >= nout) bad("arc target out of range");
+      if (e > A.xsrc[v] && A.tgt[e] <= A.tgt[e - 1]) bad("arc targets not strictly increasing within a vertex");
+      if (A.xblk[e + 1] - A.xblk[e] != A.nrows(A.tgt[e]) * A.ncols(v)) bad("block length is not nrows × ncols");
+    }
+  const int64_t nbnz = A.xblk[narc];
+  if (nbnz > 0 && !val) bad("null values");
+  A.val.assign(val, val + nbnz);
+  return A;
+}
+
+mb_problem* mb_problem_create_bsm(int64_t nvtx, int64_t nout, const int64_t* xcol, const int64_t* xrow,
+                                  const int64_t* B_xsrc, const int64_t* B_tgt, const int64_t* B_xblk,
+                                  const double* B_val, const int64_t* Q_xsrc, const int64_t* Q_tgt,
+                                  const int64_t* Q_xblk, const double* Q_val, const double* f, const double* g,
+                                  double mu, const int32_t* cone_kind, const double* cone_param,
+                                  int32_t index_base) {
+  mb_problem* out = nullptr;
+  guard([&]() {
+    if (index_base != 0 && index_base != 1) throw std::invalid_argument("index_base must be 0 or 1");
+    auto p = std::make_unique<mb_problem>();
+    mb::Problem& P = p->P;
+    P.B = make_bsm(nout, nvtx, xcol, xrow, B_xsrc, B_tgt, B_xblk, B_val, index_base, "B");
+    P.Q = make_bsm(nvtx, nvtx, xcol, xcol, Q_xsrc, Q_tgt, Q_xblk, Q_val, index_base, "Q");
+    const int64_t n = P.B.ncol(), m = P.B.nrow();
+    P.f.assign(f, f + n);
+    P.g.assign(g, g + m);
+    P.mu = mu;
+    P.K.resize(nvtx);
+    for (int64_t v = 0; v < nvtx; ++v) {
+      if (cone_kind[v] < 0 || cone_kind[v] > 5) throw std::invalid_argument("unknown cone kind");
+      P.K[v].kind = cone_kind[v];
+      P.K[v].param = cone_param ? cone_param[v] : 0.0;
+    }
+    P.P1.resize(m);
+    P.P2.resize(n);
+    for (int64_t i = 0; i < m; ++i) P.P1[i] = i;
+    for (int64_t j = 0; j < n; ++j) P.P2[j] = j;
+    p->n = n;
+    p->m = m;
+    out = p.release();
+    return 0;
+  });
+  return out;
+}
+
 void mb_problem_destroy(mb_problem* p) { delete p; }
 
 int mb_problem_dims(const mb_problem* p, int64_t* dims) {
@@ -318,6 +398,18 @@ int mb_solver_dag_trace(mb_solver* s, int32_t front, int64_t* out, int32_t len) 
 
 int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho) {
   return guard([&]() { return s->S->kkt_factor(delta, rho); });
+}
+
+int mb_solver_kkt_factor_h(mb_solver* s, const int64_t* H_xsrc, const int64_t* H_tgt, const int64_t* H_xblk,
+                           const double* H_val, int32_t index_base, double delta, double* rho) {
+  return guard([&]() {
+    if (index_base != 0 && index_base != 1) throw std::invalid_argument("index_base must be 0 or 1");
+    const mb::BSM& B = s->S->internal().B;
+    std::vector<int64_t> xc(B.xcol.begin(), B.xcol.end());
+    for (auto& x : xc) x += index_base;  // the solver's partition in the caller's index base
+    mb::BSM H = make_bsm(B.nvtx, B.nvtx, xc.data(), xc.data(), H_xsrc, H_tgt, H_xblk, H_val, index_base, "H");
+    return s->S->kkt_factor_h(H, delta, rho);
+  });
 }
 
 int64_t mb_solver_ncone(const mb_solver* s) { return (int64_t)s->S->internal().K.size(); }

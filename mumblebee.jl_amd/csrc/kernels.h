@@ -116,8 +116,9 @@ struct BigPsd {
   int n = 0;
   int dmax = 0;
 };
+// soc: the second-order cones (one wavefront each)
 void launch_cone_scale(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
-                       double* out, const int* psd, int npsd, const BigPsd& big);
+                       double* out, const int* psd, int npsd, const BigPsd& big, const int* soc, int nsoc);
 // dense_nmax: the largest order among the dense-block cones; dpart: DENSE_SPLIT doubles per dense cone
 void launch_cone_hnorm2(hipStream_t, const InstDev*, const ConeDesc*, int nc, const int* cone_ofs, int ninst,
                         double* out, int slot, const int* dense, int ndense, int dense_nmax, double* dpart,
@@ -167,8 +168,11 @@ void launch_zerofree(hipStream_t, const InstDev*, const ConeDesc*, int nc, int k
 // the shadow-solve seed of every listed exponential / power cone (initcache!)
 void launch_td_initcache(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n);
 void launch_fill(hipStream_t, double* x, size_t n, double v);
+// F[pos] = lval + δ qval (withq) or lval
 void launch_assemble_union(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* lval,
-                           const double* qval, int64_t nent);
+                           const double* qval, int64_t nent, int withq);
+// F[pos] += δ val
+void launch_scatter_add(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* val, int64_t nent);
 // nmax: the largest cone order (sets how many workgroups share one cone's columns)
 void launch_assemble_cones(hipStream_t, const InstDev*, const ConeDesc*, const int* cone_front, const FrontDesc*,
                            int nc, int nmax);

@@ -469,7 +469,7 @@ __global__ void k_cone_scale(const InstDev* insts, const ConeDesc* cones, int nc
   ConeDesc C = cones[c];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
-  if (C.kind == CONE_PSD) return;  // k_psd_scale
+  if (C.kind == CONE_PSD || C.kind == CONE_SOC) return;  // k_psd_scale, k_soc_scale
   const double* p = nvec(I, NV_P) + C.col0;
   const double* d = nvec(I, NV_D) + C.col0;
   int n = C.n;
@@ -480,10 +480,34 @@ __global__ void k_cone_scale(const InstDev* insts, const ConeDesc* cones, int nc
       C.D[i] = d[i] / p[i];
       spsd += 1.0 / (p[i] * d[i]);
     }
-  } else if (C.kind == CONE_SOC) {
-    // soc.jl:192-263
-    double* cache = C.cache;
-    double* w = cache + 5;
+  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
+    // tdscale! (tdc.jl:185-270)
+    flag = tdc::scale(C.kind, C.param, p, d, C.cache, C.D, spsd) ? 1.0 : 0.0;
+  }
+  out[2 * c] = flag;
+  out[2 * c + 1] = spsd;
+}
+
+// socscale! (soc.jl:192-263), one wavefront per second-order cone: lane 0 forms the compensated
+// determinants, ⟨p, d⟩, β and κ in the reference's order (soc.jl:48-70), the lanes then form w
+// and write the n×n block D = η(2 w wᵀ − J) column by column (coalesced along each column).
+// SOC caches: β, pdet, ddet, pdetlo, ddetlo, w[n]. out[2c] = flag, out[2c+1] = ⟨p*, d*⟩.
+constexpr int SOC_LDS = 4096;  // w staged in LDS up to this order
+__global__ void __launch_bounds__(64) k_soc_scale(const InstDev* insts, const ConeDesc* cones, const int* list,
+                                                  double* out) {
+  __shared__ double ws[SOC_LDS];
+  __shared__ double sh[3];
+  __shared__ int okf;
+  const int c = list[blockIdx.x];
+  const ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int lane = threadIdx.x, n = C.n;
+  const double* p = nvec(I, NV_P) + C.col0;
+  const double* d = nvec(I, NV_D) + C.col0;
+  double* cache = C.cache;
+  double* w = cache + 5;
+  if (lane == 0) {
     double pdet, pdetlo, ddet, ddetlo;
     socdot(p, p, n, pdet, pdetlo);
     socdot(d, d, n, ddet, ddetlo);
@@ -491,38 +515,46 @@ __global__ void k_cone_scale(const InstDev* insts, const ConeDesc* cones, int nc
     cache[2] = ddet;
     cache[3] = pdetlo;
     cache[4] = ddetlo;
-    bool ok = pdet > 0 && p[0] > 0 && ddet > 0 && d[0] > 0;
-    double beta = 0.0;
+    const bool ok = pdet > 0 && p[0] > 0 && ddet > 0 && d[0] > 0;
+    double beta = 0.0, spsd = 0.0;
     if (ok) {
-      double pdot = cdot(p, d, n);
-      double spdet = sqrt(pdet), sddet = sqrt(ddet);
-      double kappa = sqrt(2.0 * fma(spdet, sddet, pdot));
+      const double pdot = cdot(p, d, n);
+      const double spdet = sqrt(pdet), sddet = sqrt(ddet);
+      sh[1] = sqrt(2.0 * fma(spdet, sddet, pdot));  // κ
       beta = sqrt(spdet / sddet);
-      w[0] = fma(d[0], beta, p[0] / beta) / kappa;
-      for (int i = 1; i < n; ++i) w[i] = fma(-d[i], beta, p[i] / beta) / kappa;
-      double eta = 1.0 / (beta * beta), eta2 = 2.0 * eta;
-      double* D = C.D;
-      double w1 = w[0];
-      D[0] = fma(eta2, w1 * w1, -eta);
-      double nw1 = -eta2 * w1;
-      for (int i = 1; i < n; ++i) D[i] = D[(size_t)i * n] = nw1 * w[i];
-      for (int j = 1; j < n; ++j) {
-        double wj = w[j];
-        D[j + (size_t)j * n] = fma(eta2, wj * wj, eta);
-        double ew = eta2 * wj;
-        for (int i = j + 1; i < n; ++i) D[i + (size_t)j * n] = D[j + (size_t)i * n] = ew * w[i];
-      }
       spsd = 4.0 * pdot / (pdet * ddet);
-    } else {
-      flag = 0.0;
     }
     cache[0] = beta;
-  } else if (C.kind == CONE_EXP || C.kind == CONE_POW) {
-    // tdscale! (tdc.jl:185-270)
-    flag = tdc::scale(C.kind, C.param, p, d, C.cache, C.D, spsd) ? 1.0 : 0.0;
+    sh[0] = beta;
+    okf = ok ? 1 : 0;
+    out[2 * c] = ok ? 1.0 : 0.0;
+    out[2 * c + 1] = spsd;
   }
-  out[2 * c] = flag;
-  out[2 * c + 1] = spsd;
+  __syncthreads();
+  if (!okf) return;
+  const double beta = sh[0], kappa = sh[1];
+  for (int i = lane; i < n; i += 64) {
+    const double wi = i == 0 ? fma(d[0], beta, p[0] / beta) / kappa : fma(-d[i], beta, p[i] / beta) / kappa;
+    w[i] = wi;
+    if (i < SOC_LDS) ws[i] = wi;
+  }
+  __syncthreads();
+  const double eta = 1.0 / (beta * beta), eta2 = 2.0 * eta;
+  const double w1 = ws[0], nw1 = -eta2 * w1;
+  for (int j = 0; j < n; ++j) {
+    const double wj = j < SOC_LDS ? ws[j] : w[j];
+    double* Dj = C.D + (size_t)j * n;
+    for (int i = lane; i < n; i += 64) {
+      const double wi = i < SOC_LDS ? ws[i] : w[i];
+      double v;
+      if (i == 0 && j == 0) v = fma(eta2, w1 * w1, -eta);
+      else if (i == 0) v = nw1 * wj;
+      else if (j == 0) v = nw1 * wi;
+      else if (i == j) v = fma(eta2, wj * wj, eta);
+      else v = i > j ? (eta2 * wj) * wi : (eta2 * wi) * wj;
+      Dj[i] = v;
+    }
+  }
 }
 
 // ‖Q_vv + D_v‖² per cone into out[c] (the diagonal-block part of ‖H‖)
@@ -794,12 +826,19 @@ __global__ void k_fill(double* x, size_t n, double v) {
 }
 
 __global__ void k_assemble_union(const InstDev* insts, int inst, const int64_t* pos, const double* lval,
-                                 const double* qval, int64_t nent) {
+                                 const double* qval, int64_t nent, int withq) {
   const InstDev& I = insts[inst];
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nent) return;
   double delta = I.sc[S_DELTA];
-  I.F[pos[k]] = lval[k] + delta * qval[k];
+  I.F[pos[k]] = withq ? lval[k] + delta * qval[k] : lval[k];
+}
+
+__global__ void k_scatter_add(const InstDev* insts, int inst, const int64_t* pos, const double* val, int64_t nent) {
+  const InstDev& I = insts[inst];
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nent) return;
+  I.F[pos[k]] += I.sc[S_DELTA] * val[k];
 }
 
 // cone blocks: one thread per (cone, column); lower triangle of the block lands in the
@@ -1260,8 +1299,10 @@ void launch_cone_matvec(hipStream_t st, const InstDev* I, const ConeDesc* C, int
   if (nc) k_cone_matvec<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx, ky, a);
 }
 void launch_cone_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, const int* cone_ofs,
-                       int ninst, double* out, const int* psd, int npsd, const BigPsd& big) {
+                       int ninst, double* out, const int* psd, int npsd, const BigPsd& big, const int* soc,
+                       int nsoc) {
   if (nc) k_cone_scale<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, out);
+  if (nsoc) k_soc_scale<<<nsoc, 64, 0, st>>>(I, C, soc, out);
   launch_psd_scale(st, I, C, psd, npsd, out);
   launch_bpsd_scale(st, I, C, big, out);
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 0);
@@ -1296,8 +1337,12 @@ void launch_fill(hipStream_t st, double* x, size_t n, double v) {
   if (n) k_fill<<<std::max(cdiv((long)(n / 2), TILE), 1), TILE, 0, st>>>(x, n, v);
 }
 void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* lval,
-                           const double* qval, int64_t nent) {
-  if (nent) k_assemble_union<<<cdiv(nent, TILE), TILE, 0, st>>>(I, inst, pos, lval, qval, nent);
+                           const double* qval, int64_t nent, int withq) {
+  if (nent) k_assemble_union<<<cdiv(nent, TILE), TILE, 0, st>>>(I, inst, pos, lval, qval, nent, withq);
+}
+void launch_scatter_add(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* val,
+                        int64_t nent) {
+  if (nent) k_scatter_add<<<cdiv(nent, TILE), TILE, 0, st>>>(I, inst, pos, val, nent);
 }
 void launch_assemble_cones(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* cone_front,
                            const FrontDesc* F, int nc, int nmax) {

@@ -457,9 +457,12 @@ void Solver::build_device() {
   const i64 NF = sym_.nfront();
   // fronts
   fr_.assign(NF, FrontDesc{});
-  std::vector<i64> fofs(NF + 1, 0);
-  std::vector<std::vector<int>> sepcols(NF);
-  std::vector<int> colfront(n_);
+  std::vector<i64>& fofs = fofs_;
+  std::vector<std::vector<int>>& sepcols = sepcols_;
+  std::vector<int>& colfront = colfront_;
+  fofs.assign(NF + 1, 0);
+  sepcols.assign(NF, {});
+  colfront.assign(n_, 0);
   for (i64 f = 0; f < NF; ++f) {
     i64 res0 = sym_.xcol[sym_.front_vtx[f]];
     i64 r = sym_.xcol[sym_.front_vtx[f + 1]] - res0;
@@ -477,20 +480,10 @@ void Solver::build_device() {
     for (i64 j = res0; j < res0 + r; ++j) colfront[j] = (int)f;
   }
   fsize_ = fofs[NF];
-  // position of (i, j), i ≥ j, in the frontal storage
   auto fpos = [&](i64 i, i64 j) -> i64 {
-    int f = colfront[j];
-    const FrontDesc& F = fr_[f];
-    i64 jl = j - F.res0, il;
-    if (i < F.res0 + F.r) {
-      il = i - F.res0;
-    } else {
-      auto& sc = sepcols[f];
-      auto it = std::lower_bound(sc.begin(), sc.end(), (int)i);
-      if (it == sc.end() || *it != i) throw std::logic_error("symbolic structure misses an entry of M");
-      il = F.r + (it - sc.begin());
-    }
-    return fofs[f] + il + jl * F.ld;
+    const i64 q = front_pos(i, j);
+    if (q < 0) throw std::logic_error("symbolic structure misses an entry of M");
+    return q;
   };
   // children lists, relidx, sepcols
   std::vector<std::vector<int>> children(NF);
@@ -634,7 +627,7 @@ void Solver::build_device() {
   }
   d_cones_.upload(cones_, st_);
   {
-    std::vector<int> psd, dense, td, bpsd;
+    std::vector<int> psd, dense, td, bpsd, soc;
     for (i64 v = 0; v < N; ++v) {
       const int k = cones_[v].kind;
       const bool big = k == CONE_PSD && cones_[v].D == nullptr;
@@ -645,8 +638,11 @@ void Solver::build_device() {
       }
       if ((k == CONE_PSD && !big) || k == CONE_SOC) dense.push_back((int)v);
       if (k == CONE_EXP || k == CONE_POW) td.push_back((int)v);
+      if (k == CONE_SOC) soc.push_back((int)v);
     }
     npsd_ = (int)psd.size();
+    nsoc_ = (int)soc.size();
+    if (nsoc_) d_soclist_.upload(soc, st_);
     nbpsd_ = (int)bpsd.size();
     if (nbpsd_) d_bpsdlist_.upload(bpsd, st_);
     d_bpart_.alloc(sizeof(double) * 2 * DENSE_SPLIT * std::max(nbpsd_, 1));
@@ -904,6 +900,90 @@ void Solver::build_schedule() {
   HIPCHK(hipStreamSynchronize(st_));
 }
 
+// position of (i, j), i ≥ j, in the frontal storage; −1 outside the factor's structure
+i64 Solver::front_pos(i64 i, i64 j) const {
+  const int f = colfront_[j];
+  const FrontDesc& F = fr_[f];
+  const i64 jl = j - F.res0;
+  i64 il;
+  if (i < F.res0 + F.r) {
+    if (i < j) return -1;
+    il = i - F.res0;
+  } else {
+    const auto& sc = sepcols_[f];
+    auto it = std::lower_bound(sc.begin(), sc.end(), (int)i);
+    if (it == sc.end() || *it != i) return -1;
+    il = F.r + (it - sc.begin());
+  }
+  return fofs_[f] + il + jl * F.ld;
+}
+
+// The caller's H in place of the solver's (Q plus the cone blocks) for the KKT plugin methods:
+// its CSR rows take the place of Q's in the products, and the assembly adds δH instead of δQ and
+// the cone blocks. use_user_h(false) brings back the solver's own H.
+void Solver::use_user_h(bool on) {
+  if (on == user_h_) return;
+  std::swap(idev_.Q, hsave_);
+  std::swap(d_sptiles_[2].p, d_htiles_.p);
+  std::swap(d_sptiles_[2].bytes, d_htiles_.bytes);
+  std::swap(nsptile_[2], nhtiles_);
+  HIPCHK(hipMemcpyAsync(d_inst_.p, &idev_, sizeof(InstDev), hipMemcpyHostToDevice, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  user_h_ = on;
+}
+
+int Solver::kkt_factor_h(const BSM& H, double delta, double* rho) {
+  if (H.nvtx != P_.B.nvtx || H.nout != P_.B.nvtx || H.xcol != P_.B.xcol || H.xrow != P_.B.xcol)
+    throw std::invalid_argument("H must be block-sparse over the solver's vertex partition (mb_solver_layout)");
+  use_user_h(false);
+  std::vector<int> p, c;
+  std::vector<double> x;
+  bsm_to_csr(H, false, p, c, x);
+  // rows of H as Q's are tiled (lanes per row from the mean row length; long rows on their own)
+  double avg = n_ > 0 ? (double)c.size() / (double)n_ : 0.0;
+  int G = 1;
+  while (G < 64 && 2.0 * G <= avg) G *= 2;
+  std::vector<Tile> t;
+  for (i64 r = 0; r < n_; r += TILE_THREADS / G) t.push_back(Tile{0, (int)r});
+  for (i64 r = 0; r < n_; ++r)
+    if (p[r + 1] - p[r] > SPMV_LONG * G) t.push_back(Tile{0, (int)(-1 - r)});
+  d_Hp_.upload(p, st_);
+  d_Hc_.upload(c, st_);
+  d_Hv_.upload(x, st_);
+  d_htiles_.upload(t, st_);
+  nhtiles_ = (int)t.size();
+  hsave_ = CSRDev{d_Hp_.as<int>(), d_Hc_.as<int>(), d_Hv_.as<double>(), (int)n_, G};
+  // assembly positions of the lower triangle
+  std::vector<int64_t> pos;
+  std::vector<double> val;
+  for (i64 v = 0; v < H.nvtx; ++v) {
+    const i64 nc = H.ncols(v), c0 = H.xcol[v];
+    for (i64 e = H.xsrc[v]; e < H.xsrc[v + 1]; ++e) {
+      const i64 u = H.tgt[e], nr = H.nrows(u), r0 = H.xrow[u];
+      for (i64 jl = 0; jl < nc; ++jl)
+        for (i64 il = 0; il < nr; ++il) {
+          const i64 i = r0 + il, j = c0 + jl;
+          const double h = H.val[H.xblk[e] + jl * nr + il];
+          if (i < j || h == 0.0) continue;
+          const i64 q = front_pos(i, j);
+          if (q < 0) throw std::invalid_argument("H has an entry outside the factor's structure (the pattern of Q, BᵀB "
+                                                 "and the diagonal blocks)");
+          pos.push_back(q);
+          val.push_back(h);
+        }
+    }
+  }
+  d_hpos_.upload(pos, st_);
+  d_hval_.upload(val, st_);
+  nhent_ = (i64)pos.size();
+  use_user_h(true);
+  double r = 0.0;
+  set_scalar(S_ALPHA, 1.0 / delta);
+  const bool ok = factor(delta, r);
+  if (rho) *rho = r;
+  return ok ? 1 : 0;
+}
+
 double Solver::front_flops() const {
   double t = 0.0;
   for (const FrontDesc& F : fr_) {
@@ -1129,7 +1209,12 @@ bool Solver::factor(double delta, double& rho) {
   const InstDev* I = d_inst_.as<InstDev>();
   auto assemble = [&]() {
     launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
-    launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_);
+    if (user_h_) {  // BᵀB + δ (the caller's H)
+      launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 0);
+      launch_scatter_add(st_, I, 0, d_hpos_.as<int64_t>(), d_hval_.as<double>(), nhent_);
+      return;
+    }
+    launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 1);
     launch_bpsd_assemble(st_, I, d_cones_.as<ConeDesc>(), bpsd(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>());
     launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(),
                           ncone_, cone_nmax_);
@@ -1180,6 +1265,7 @@ bool Solver::factor(double delta, double& rho) {
 }
 
 int Solver::kkt_factor(double delta, double* rho) {
+  use_user_h(false);
   double r = 0.0;
   set_scalar(S_ALPHA, 1.0 / delta);  // the Uzawa step α = 1/δ of a following kkt_solve
   bool ok = factor(delta, r);
@@ -1193,10 +1279,11 @@ int Solver::kkt_factor(double delta, double* rho) {
 // ------------------------------------------------------------------------------------------
 
 bool Solver::cone_scale(const double* p, const double* d, double* spsd) {
+  use_user_h(false);
   HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_P * n_, p, n_ * 8, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_D * n_, d, n_ * 8, hipMemcpyHostToDevice, st_));
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd());
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd(), d_soclist_.as<int>(), nsoc_);
   sync_scalars();
   if (spsd) *spsd = hsc_[S_SPSD];
   return hsc_[S_FLAGOK] != 0.0;
@@ -1547,6 +1634,11 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
     const SpmvN rn{kx, ky, kf, -1, -1, NV_KF, K(-1.0), K(1.0), K(1.0), K(0.0), K(0.0)};
     const SpmvM rm{kx, kg, MV_KG, -1, K(-1.0), K(1.0), K(0.0), 0};
     spmv_nm(2, &rn, &rm);
+    if (read && user_h_) {  // the caller's H holds the cone blocks too
+      norms();
+      return;
+    }
+    if (user_h_) return;
     launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, NV_KF, K(-1.0));
     launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
                              dense_nmax_, kx, NV_KF, K(-1.0));
@@ -1731,6 +1823,7 @@ const std::vector<HistRow>& Solver::history() {
 }
 
 int Solver::step() {
+  use_user_h(false);
   int status = CONTINUE;
   HistRow row{NaN, NaN, NaN, NaN, 0.0, 0, 0, KKT_SOLVED, 0, 0, KKT_SOLVED, NaN, NaN, NaN, 0.0, NaN, NaN};
   const double mut = mu_t_;
@@ -1739,7 +1832,7 @@ int Solver::step() {
   launch_set_scalar2(st_, d_sc_.as<double>(), S_DELTA, delta, S_ALPHA, 1.0 / delta);
   // pᵀd, scale!, residuals!
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd());
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd(), d_soclist_.as<int>(), nsoc_);
   launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
                      d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
                      d_dpart_.as<double>(), bpsd(), d_bpart_.as<double>());
@@ -1913,6 +2006,7 @@ int Solver::solve() {
 
 // result (solver/ipm.jl:30-74)
 void Solver::result(double* p, double* d, double* y, double* info) {
+  use_user_h(false);
   std::vector<double> hp(n_), hd(n_), hy(m_);
   HIPCHK(hipMemcpyAsync(hp.data(), idev_.nv + (size_t)NV_P * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipMemcpyAsync(hd.data(), idev_.nv + (size_t)NV_D * n_, n_ * 8, hipMemcpyDeviceToHost, st_));

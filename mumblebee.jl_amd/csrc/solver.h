@@ -115,6 +115,9 @@ class Solver {
   double device_setup_ms() const { return device_ms_; }  // init on the device: buffers, uploads, start
   void dense_factor(double* out);  // n×n lower factor (debug; small n)
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
+  // initkkt!(kkt, H; δ) with the caller's H (internal vertex partition, both triangles); the KKT
+  // solves then use it until the next scale / step / kkt_factor
+  int kkt_factor_h(const BSM& H, double delta, double* rho);
   // cone fan-outs on the iterate, in the solver's internal (permuted, scaled) coordinates
   // (cone.jl:146-553); scale replaces the iterate by (p, d)
   bool cone_scale(const double* p, const double* d, double* spsd);
@@ -227,6 +230,8 @@ class Solver {
   int ntile_n_ = 0, ntile_m_ = 0;
   DevBuf d_sptiles_[3];  // row tiles of B, Bᵀ, Q for the group-per-row spmv
   DevBuf d_psdlist_, d_denselist_;  // PSD cones; cones with dense scaling blocks (SOC, PSD)
+  DevBuf d_soclist_;                 // second-order cones
+  int nsoc_ = 0;
   DevBuf d_bpsdlist_, d_bws_, d_bpart_;  // PSD cones above PSD_MAX, their workspaces, ‖H‖ partials
   int nbpsd_ = 0, bpsd_dmax_ = 0;
   BigPsd bpsd() const { return BigPsd{d_bpsdlist_.as<int>(), nbpsd_, bpsd_dmax_}; }
@@ -271,6 +276,18 @@ class Solver {
   int fac_fp32_calls_ = 0;
   size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
   double host_ms_ = 0.0, device_ms_ = 0.0;
+  // front layout on the host (assembly positions)
+  std::vector<i64> fofs_;
+  std::vector<std::vector<int>> sepcols_;
+  std::vector<int> colfront_;
+  i64 front_pos(i64 i, i64 j) const;
+  // a caller's H (kkt_factor_h): CSR rows, row tiles, assembly positions and values
+  bool user_h_ = false;
+  CSRDev hsave_{};  // the CSR not in use (Q's while the caller's H is)
+  DevBuf d_Hp_, d_Hc_, d_Hv_, d_htiles_, d_hpos_, d_hval_;
+  int nhtiles_ = 0;
+  i64 nhent_ = 0;
+  void use_user_h(bool on);
   void check_abort();
 };
 

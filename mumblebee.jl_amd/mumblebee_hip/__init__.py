@@ -121,6 +121,9 @@ EXPORTS = {
                                             _DP, _DP, ctypes.c_double, ctypes.c_int64, _I32P, _DP, _I64P,
                                             ctypes.c_double]),
     "mb_problem_destroy": (None, [ctypes.c_void_p]),
+    "mb_problem_create_bsm": (ctypes.c_void_p, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P, _I64P, _I64P,
+                                                _DP, _I64P, _I64P, _I64P, _DP, _DP, _DP, ctypes.c_double, _I32P,
+                                                _DP, ctypes.c_int32]),
     "mb_problem_dims": (ctypes.c_int, [ctypes.c_void_p, _I64P]),
     "mb_problem_export": (ctypes.c_int, [ctypes.c_void_p, _I64P, _I64P, _I32P, _I64P, _I64P, _I64P, _I64P, _I64P,
                                          _DP, _I64P, _I64P, _I64P, _DP, _DP, _DP]),
@@ -150,6 +153,8 @@ EXPORTS = {
     "mb_cached_bytes": (ctypes.c_int64, []),
     "mb_release_cached_memory": (ctypes.c_int64, []),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
+    "mb_solver_kkt_factor_h": (ctypes.c_int, [ctypes.c_void_p, _I64P, _I64P, _I64P, _DP, ctypes.c_int32,
+                                               ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_probe_mfma_f64": (ctypes.c_int, [_DP]),
     "mb_solver_ncone": (ctypes.c_int64, [ctypes.c_void_p]),
@@ -239,6 +244,80 @@ class IPMSettings:
         return s
 
 
+class BlockSparse:
+    """BlockSparseMatrix{Float64, Int} (block_sparse_matrix.jl:10-109) as 0-based arrays: vertex v
+    owns columns xcol[v]:xcol[v+1], out vertex u rows xrow[u]:xrow[u+1], arc e of vertex v
+    (xsrc[v]:xsrc[v+1]) targets out vertex tgt[e] with the column-major block
+    val[xblk[e]:xblk[e+1]] of shape (rows of tgt[e], columns of v)."""
+
+    def __init__(self, nout, nvtx, xsrc, xcol, xrow, xblk, tgt, val):
+        self.nout, self.nvtx = int(nout), int(nvtx)
+        self.xsrc = np.ascontiguousarray(xsrc, dtype=np.int64)
+        self.xcol = np.ascontiguousarray(xcol, dtype=np.int64)
+        self.xrow = np.ascontiguousarray(xrow, dtype=np.int64)
+        self.xblk = np.ascontiguousarray(xblk, dtype=np.int64)
+        self.tgt = np.ascontiguousarray(tgt, dtype=np.int64)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+
+    @property
+    def shape(self):
+        return int(self.xrow[-1]), int(self.xcol[-1])
+
+    def toscipy(self):
+        """SparseArrays.sparse(A) (block_sparse_matrix.jl:361-395): every block entry stored."""
+        import scipy.sparse as sp
+        rows, cols, vals = [], [], []
+        for v in range(self.nvtx):
+            c0, nc = int(self.xcol[v]), int(self.xcol[v + 1] - self.xcol[v])
+            for e in range(int(self.xsrc[v]), int(self.xsrc[v + 1])):
+                u = int(self.tgt[e])
+                r0, nr = int(self.xrow[u]), int(self.xrow[u + 1] - self.xrow[u])
+                blk = self.val[self.xblk[e]:self.xblk[e + 1]].reshape((nr, nc), order="F")
+                for jl in range(nc):
+                    for il in range(nr):
+                        rows.append(r0 + il)
+                        cols.append(c0 + jl)
+                        vals.append(blk[il, jl])
+        return sp.csc_matrix((vals, (rows, cols)), shape=self.shape)
+
+
+def blocksparse(I, J, V, R=None, C=None):
+    """blocksparse(I, J, V[, R, C]) (block_sparse_matrix.jl:330-359): block (I[k], J[k]) (0-based
+    out-vertex and vertex ids) is the dense matrix V[k]; blocks at the same position are summed,
+    as sparse(I, J, V) sums them. R / C: rows per out vertex and columns per vertex (default: from
+    the blocks; a vertex or out vertex without a block then has none)."""
+    I = [int(x) for x in I]
+    J = [int(x) for x in J]
+    V = [np.atleast_2d(np.asarray(x, dtype=np.float64)) for x in V]
+    nout = len(R) if R is not None else (max(I) + 1 if I else 0)
+    nvtx = len(C) if C is not None else (max(J) + 1 if J else 0)
+    rsz = list(R) if R is not None else [0] * nout
+    csz = list(C) if C is not None else [0] * nvtx
+    if R is None or C is None:
+        for i, j, b in zip(I, J, V):
+            if R is None:
+                rsz[i] = b.shape[0]
+            if C is None:
+                csz[j] = b.shape[1]
+    blocks = {}
+    for i, j, b in zip(I, J, V):
+        if b.shape != (rsz[i], csz[j]):
+            raise ValueError(f"blocksparse: block ({i}, {j}) has shape {b.shape}, expected {(rsz[i], csz[j])}")
+        blocks[(j, i)] = blocks[(j, i)] + b if (j, i) in blocks else b.copy()
+    xcol = np.concatenate([[0], np.cumsum(csz)]).astype(np.int64)
+    xrow = np.concatenate([[0], np.cumsum(rsz)]).astype(np.int64)
+    xsrc, tgt, xblk, val = [0], [], [0], []
+    for v in range(nvtx):
+        for u in sorted(i for (j, i) in blocks if j == v):
+            b = blocks[(v, u)]
+            tgt.append(u)
+            val.append(b.ravel(order="F"))
+            xblk.append(xblk[-1] + b.size)
+        xsrc.append(len(tgt))
+    return BlockSparse(nout, nvtx, xsrc, xcol, xrow, xblk, tgt,
+                       np.concatenate(val) if val else np.zeros(0))
+
+
 class IPMProblem:
     """IPMProblem(Q, B, f, g, μ, K, s; compress=1.0) (problem.jl:64-127).
 
@@ -268,6 +347,38 @@ class IPMProblem:
             raise _err("IPMProblem")
         self._h = ctypes.c_void_p(h)
         _live_problems.add(self)
+
+    @classmethod
+    def from_blocks(cls, Q, B, f, g, mu, K):
+        """IPMProblem(Q::BlockSparseMatrix, B::BlockSparseMatrix, f, g, μ, K) (problem.jl:49-53): the
+        block-sparse Q and B (BlockSparse, e.g. from blocksparse) as they are, one cone per vertex,
+        identity row and column permutations."""
+        self = cls.__new__(cls)
+        m, n = B.shape
+        self.n, self.m = int(n), int(m)
+        if Q.nvtx != B.nvtx or not np.array_equal(Q.xcol, B.xcol) or not np.array_equal(Q.xrow, B.xcol):
+            raise ValueError("IPMProblem: Q must be square over B's vertex partition")
+        if len(K) != B.nvtx:
+            raise ValueError("IPMProblem: one cone per vertex")
+        self._f = np.ascontiguousarray(f, dtype=np.float64)
+        self._g = np.ascontiguousarray(g, dtype=np.float64)
+        if self._f.shape != (n,) or self._g.shape != (m,):
+            raise ValueError("IPMProblem: f and g must have lengths n and m")
+        self._blocks = (Q, B)
+        self.K = list(K)
+        kinds = np.array([k.kind for k in K], dtype=np.int32)
+        params = np.array([k.param for k in K], dtype=np.float64)
+        i64, dbl = ctypes.c_int64, ctypes.c_double
+        h = lib().mb_problem_create_bsm(B.nvtx, B.nout, _p(B.xcol, i64), _p(B.xrow, i64), _p(B.xsrc, i64),
+                                        _p(B.tgt, i64), _p(B.xblk, i64), _p(B.val, dbl), _p(Q.xsrc, i64),
+                                        _p(Q.tgt, i64), _p(Q.xblk, i64), _p(Q.val, dbl), _p(self._f, dbl),
+                                        _p(self._g, dbl), float(mu), _p(kinds, ctypes.c_int32),
+                                        _p(params, dbl), 0)
+        if not h:
+            raise _err("IPMProblem")
+        self._h = ctypes.c_void_p(h)
+        _live_problems.add(self)
+        return self
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -445,6 +556,17 @@ class IPMSolver:
     def kkt_factor(self, delta):
         rho = ctypes.c_double(0.0)
         r = lib().mb_solver_kkt_factor(self._h, float(delta), ctypes.byref(rho))
+        if r < 0:
+            raise _err("initkkt!")
+        return bool(r), rho.value
+
+    def kkt_factor_h(self, H, delta):
+        """initkkt!(kkt, H; δ) (uzawa.jl:73-76) with the caller's H: a BlockSparse over the internal
+        vertex partition (layout()["xcol"]) in internal coordinates; kkt_solve then uses it."""
+        rho = ctypes.c_double(0.0)
+        i64 = ctypes.c_int64
+        r = lib().mb_solver_kkt_factor_h(self._h, _p(H.xsrc, i64), _p(H.tgt, i64), _p(H.xblk, i64),
+                                         _p(H.val, ctypes.c_double), 0, float(delta), ctypes.byref(rho))
         if r < 0:
             raise _err("initkkt!")
         return bool(r), rho.value

@@ -342,39 +342,85 @@ def maxcut_chordal(nv=2000, band=20, extra=1e-4, seed=0, merge=48):
                 cliques=cliques, name=f"maxcut_chordal_nv{nv}_w{band}_x{extra}_s{seed}")
 
 
-def exppow(nexp=2000, npow=500, m=499, seed=0, dens=0.005, eps=0.1):
+def exppow(nexp=2000, npow=500, m=499, seed=0, dens=0.005, eps=0.1, psd=0, rows="xab"):
+    """Exponential and power cones (see the module docstring), optionally with one PSD block of
+    order `psd` (svec columns after the power cones): objective ⟨C, X⟩ with C dense N(0,1)
+    symmetrised, bounded by tr(X) = 1. rows = "xab": the m random equalities run over the x, a, b
+    coordinates; rows = "all": over every column (X included) at density `dens`, with at least two
+    entries per row. g = B x0 at the strictly interior x0 (u = 1, x = 1/nexp, t = 0, a = b = 1,
+    c = 0, X = I/psd)."""
     rng = np.random.default_rng(seed)
-    n = 3 * (nexp + npow)
+    ntd = 3 * (nexp + npow)
+    npsd = psd * (psd + 1) // 2
+    n = ntd + npsd
     alphas = rng.uniform(0.2, 0.8, npow)
-    rows, cols, vals = [], [], []
+    rows_, cols, vals = [], [], []
     r = 0
     for i in range(nexp):  # u_i = 1
-        rows.append(r), cols.append(3 * i), vals.append(1.0)
+        rows_.append(r), cols.append(3 * i), vals.append(1.0)
         r += 1
     for i in range(nexp):  # Σ x_i = 1
-        rows.append(r), cols.append(3 * i + 1), vals.append(1.0)
+        rows_.append(r), cols.append(3 * i + 1), vals.append(1.0)
     r += 1
     for j in range(npow):  # a_j + b_j = 2
         c0 = 3 * (nexp + j)
-        rows += [r, r]
+        rows_ += [r, r]
         cols += [c0, c0 + 1]
         vals += [1.0, 1.0]
         r += 1
-    xab = np.array([3 * i + 1 for i in range(nexp)] + [3 * (nexp + j) + k for j in range(npow) for k in (0, 1)])
-    for _ in range(m):  # sparse equalities over x, a, b; at least two entries per row
-        k = max(2, rng.binomial(len(xab), dens))
-        for c in rng.choice(xab, size=k, replace=False):
-            rows.append(r), cols.append(int(c)), vals.append(float(rng.standard_normal()))
+    if psd:  # tr(X) = 1
+        k = ntd
+        for jj in range(psd):
+            rows_.append(r), cols.append(k), vals.append(1.0)
+            k += psd - jj
         r += 1
-    B = sp.csc_matrix((vals, (rows, cols)), shape=(r, n))
+    xab = np.array([3 * i + 1 for i in range(nexp)] + [3 * (nexp + j) + k for j in range(npow) for k in (0, 1)])
+    pool = xab if rows == "xab" else np.arange(n)
+    for _ in range(m):  # sparse equalities; at least two entries per row
+        k = max(2, rng.binomial(len(pool), dens))
+        for c in rng.choice(pool, size=k, replace=False):
+            rows_.append(r), cols.append(int(c)), vals.append(float(rng.standard_normal()))
+        r += 1
+    B = sp.csc_matrix((vals, (rows_, cols)), shape=(r, n))
     x0 = np.zeros(n)
     x0[0:3 * nexp:3] = 1.0
     x0[1:3 * nexp:3] = 1.0 / max(nexp, 1)
-    x0[3 * nexp::3] = 1.0
-    x0[3 * nexp + 1::3] = 1.0
+    x0[3 * nexp:ntd:3] = 1.0
+    x0[3 * nexp + 1:ntd:3] = 1.0
+    if psd:
+        k = ntd
+        for jj in range(psd):
+            x0[k] = 1.0 / psd
+            k += psd - jj
     g = B @ x0
     f = np.zeros(n)
-    f[2::3] = 1.0  # t_i and c_j (objective −fᵀp)
+    f[2:ntd:3] = 1.0  # t_i and c_j (objective −fᵀp)
     f[xab] = eps * rng.standard_normal(len(xab))
     K = [("exp",)] * nexp + [("pow", float(a)) for a in alphas]
-    return dict(Q=sp.csc_matrix((n, n)), B=B, f=f, g=g, K=K, s=[3] * (nexp + npow), alphas=alphas)
+    s_ = [3] * (nexp + npow)
+    if psd:
+        M = rng.standard_normal((psd, psd))
+        Cm = (M + M.T) / 2.0
+        r2 = math.sqrt(2.0)
+        k = ntd
+        for jj in range(psd):
+            f[k] = -Cm[jj, jj]
+            f[k + 1:k + psd - jj] = -r2 * Cm[jj + 1:, jj]
+            k += psd - jj
+        K.append(("psd",))
+        s_.append(npsd)
+    return dict(Q=sp.csc_matrix((n, n)), B=B, f=f, g=g, K=K, s=s_, alphas=alphas,
+                name=f"exppow_e{nexp}_p{npow}_m{m}_psd{psd}_{rows}_s{seed}")
+
+
+def served_request(rng, k, nmax=4000, seed0=5000):
+    """Request k of the served stream: 60 % QP, 30 % SOCP, 10 % max-cut SDP (the chordal clique
+    decomposition of maxcut_chordal on nv = n/25 vertices), n log-uniform in [50, nmax], the
+    densities of the full-size workloads (SOCP rows scaled up for small n)."""
+    n = int(log_uniform_n(rng, 50, nmax))
+    u = rng.random()
+    if u < 0.6:
+        return qp(n=n, m=max(1, n // 2), seed=seed0 + k)
+    if u < 0.9:
+        return socp(n=n, m=max(1, n // 2), ncone=max(1, n // 40), seed=seed0 + k, adens=min(0.2, 20.0 / n))
+    return maxcut_chordal(nv=max(12, n // 25), band=4, extra=0.01, seed=seed0 + k, merge=16)

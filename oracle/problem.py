@@ -438,3 +438,11 @@ def make_problem(Q, B, f, g, mu, K, s, compress=1.0):
     fp = np.asarray(f, dtype=np.float64)[colperm]
     gp = np.asarray(g, dtype=np.float64)[rowperm]
     return Problem(Qb, Bb, fp, gp, float(mu), cones, np.asarray(rowperm), np.asarray(colperm))
+
+
+def problem_from_blocks(Q, B, f, g, mu, K):
+    """IPMProblem(Q::BlockSparseMatrix, B::BlockSparseMatrix, f, g, μ, K) (problem.jl:49-53): the
+    block-sparse matrices as given, identity permutations (P1 = rows(B), P2 = cols(B))."""
+    m, n = int(B.xrow[B.nout]), int(B.xcol[B.nvtx])
+    return Problem(Q, B, np.asarray(f, dtype=np.float64), np.asarray(g, dtype=np.float64), float(mu), list(K),
+                   np.arange(m), np.arange(n))

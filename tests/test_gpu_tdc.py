@@ -6,7 +6,9 @@ refuses steps that stay within a few noise bands (65536·eps) of a cone boundary
 iterations that tail takes, and whether the gap test is met before the stall detector ends the
 solve (OPTIMAL vs NEAR_OPTIMAL), then depends on the last bits of the iterates. Those bits
 differ between the device's and the oracle's log/exp and double-double log, so statuses are
-compared as the set {OPTIMAL, NEAR_OPTIMAL}, and iteration counts are not compared."""
+compared as the set {OPTIMAL, NEAR_OPTIMAL}, and iteration counts are not compared. Objectives are
+held to 1e-8 when both sides end OPTIMAL, and otherwise to the duality gaps the two solves
+actually reached (each at most 1e-6), never to a blanket looser tolerance."""
 import math
 import sys
 
@@ -27,12 +29,14 @@ def _print_case(**kw):
     sys.stdout.flush()
 
 
-def _objective_rel(res):
-    """How close the stopping rule puts pobj to a closed-form optimum. OPTIMAL: the gap and the
-    residuals are within gap_tol = feas_tol = 1e-8 (relative), which together move pobj off the
-    optimum by up to about twice that. NEAR_OPTIMAL: the same rule with tolerances
-    near_factor = 1000 times looser."""
-    return 2e-8 if res.status == mh.IPMStatus.OPTIMAL else 2e-5
+def _within_gap(res, opt):
+    """A closed-form optimum lies between the primal and dual objectives up to the residuals
+    (pres, dres ≤ 1e-8): |pobj − opt| ≤ |pobj − dobj| + 1e-8. For OPTIMAL that gap is itself
+    below gap_tol = 1e-8; a NEAR_OPTIMAL end (the line-search stall of the module docstring) is
+    held to the gap it actually reached, which the test also bounds (≤ 1e-6)."""
+    gap = abs(res.pobj - res.dobj)
+    assert gap <= 1e-6 * max(1.0, abs(opt)), (res.status, gap)
+    return abs(res.pobj - opt) <= gap + 1e-8 * max(1.0, abs(opt))
 
 
 def _oracle(data, **kw):
@@ -46,7 +50,7 @@ def test_entropy_on_the_simplex(k):
     data = G.exppow(nexp=k, npow=0, m=0, eps=0.0, seed=0)
     res = mh.solve(hip_problem(data))
     assert res.status in GOOD
-    assert res.pobj == pytest.approx(-math.log(k), rel=_objective_rel(res), abs=1e-8), res.status
+    assert _within_gap(res, -math.log(k)), res.status
 
 
 def test_softmax_entropy_closed_form():
@@ -56,7 +60,7 @@ def test_softmax_entropy_closed_form():
     eps = data["f"][1:60:3]
     res = mh.solve(hip_problem(data))
     assert res.status in GOOD
-    assert res.pobj == pytest.approx(-math.log(np.exp(eps).sum()), rel=_objective_rel(res)), res.status
+    assert _within_gap(res, -math.log(np.exp(eps).sum())), res.status
 
 
 def test_weighted_geometric_means_under_budgets():
@@ -76,12 +80,17 @@ def test_mixed_cones_match_oracle(nexp, npow, m, seed):
     ref = _oracle(data)
     res = mh.solve(hip_problem(data))
     assert int(ref.status) in GOOD and res.status in GOOD
-    # both OPTIMAL: within the stopping rule's 1e-8; a NEAR_OPTIMAL end on either side meets the
-    # rule only with near_factor = 1000 times looser tolerances
+    # both OPTIMAL: within the stopping rule's 1e-8. A NEAR_OPTIMAL end on either side (the
+    # line-search stall of the module docstring) stops at a gap of its own: each objective is then
+    # within its gap of the common optimum, so they agree to the sum of the two gaps (checked to be
+    # ≤ 1e-6), not to a blanket looser tolerance
     both = res.status == mh.IPMStatus.OPTIMAL and int(ref.status) == int(mh.IPMStatus.OPTIMAL)
-    tol = 1e-8 if both else 1e-5
-    assert rel(res.pobj, ref.pobj) < tol, (res.status, ref.status)
-    assert rel(res.dobj, ref.dobj) < tol, (res.status, ref.status)
+    scale = max(1.0, abs(ref.pobj))
+    gaps = (abs(res.pobj - res.dobj) + abs(ref.pobj - ref.dobj)) / scale
+    tol = 1e-8 if both else max(1e-8, gaps + 1e-8)
+    assert tol <= 1e-6, (res.status, ref.status, gaps)
+    assert rel(res.pobj, ref.pobj) < tol, (res.status, ref.status, gaps)
+    assert rel(res.dobj, ref.dobj) < tol, (res.status, ref.status, gaps)
     assert res.pres < 1e-8 and res.dres < 1e-8
 
 

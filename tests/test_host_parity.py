@@ -119,3 +119,26 @@ def test_symbolic_structure_invariants(gen, kw, amalg):
         f = vfront[k]
         cover = set(range(k + 1, fv[f + 1])) | set(int(u) for u in S.front_sep[f])
         assert struct[k] <= cover
+
+
+def test_block_sparse_input_is_taken_as_given():
+    """mb_problem_create_bsm keeps the caller's block-sparse Q and B (identity permutations) and
+    rejects inconsistent layouts with a message."""
+    rng = np.random.default_rng(3)
+    widths, heights = [2, 3, 1, 4], [2, 1, 3]
+    B = mh.blocksparse([0, 1, 2, 0, 2], [0, 1, 1, 3, 2],
+                       [rng.standard_normal((2, 2)), rng.standard_normal((1, 3)), rng.standard_normal((3, 3)),
+                        rng.standard_normal((2, 4)), rng.standard_normal((3, 1))], heights, widths)
+    Q = mh.blocksparse([v for v in range(4)], [v for v in range(4)], [np.eye(w) for w in widths], widths, widths)
+    K = [mh.PositiveCone(), mh.SecondOrderCone(), mh.CofreeCone(), mh.PositiveCone()]
+    P = mh.IPMProblem.from_blocks(Q, B, np.arange(10.0), np.arange(6.0), 0.0, K)
+    e = P.export()
+    for key in ("xsrc", "xcol", "xrow", "xblk", "tgt", "val"):
+        assert np.array_equal(e["B_" + key], getattr(B, key)), key
+    assert np.array_equal(e["Q_val"], Q.val)
+    assert np.array_equal(e["P1"], np.arange(6)) and np.array_equal(e["P2"], np.arange(10))
+    assert np.array_equal(e["f"], np.arange(10.0))
+    assert np.allclose(B.toscipy().toarray()[:2, :2], B.val[:4].reshape(2, 2, order="F"))
+    bad = mh.BlockSparse(B.nout, B.nvtx, B.xsrc, B.xcol, B.xrow, B.xblk, B.tgt[::-1].copy(), B.val)
+    with pytest.raises(RuntimeError, match="not strictly increasing|nrows"):
+        mh.IPMProblem.from_blocks(Q, bad, np.zeros(10), np.zeros(6), 0.0, K)
