@@ -17,6 +17,8 @@
  *   mb_solver_kkt_factor     initkkt!(kkt, H; δ) → (ok, ρ)                src/IPM/src/kkt/uzawa.jl:73-115
  *   mb_solver_kkt_factor_h   initkkt!(kkt, H; δ) with the caller's H       src/IPM/src/kkt/uzawa.jl:73-76
  *   mb_solver_kkt_solve      solvekkt!(fstop, gstop, kkt, x, y, H, B, f, g; …)  src/IPM/src/kkt/uzawa.jl:265-344
+ *   mb_solver_frule          frule!(Δp, Δy, Δd, s, Δf, Δg, Δμ)            src/IPM/src/ipm/differential.jl:5-75
+ *   mb_solver_rrule          rrule!(Δf, Δg, s, Δp, Δy, Δd)                src/IPM/src/ipm/differential.jl:252-334
  *   mb_cone_scale            scale!(sched, K, H, Q, caches, p, d, …)      src/IPM/src/cone/cone.jl:146-247
  *   mb_cone_corr0            initpredictor! / corr0!                      src/IPM/src/cone/cone.jl:361-444
  *   mb_cone_corr             initcorrector! / corr!                       src/IPM/src/cone/cone.jl:261-352
@@ -178,6 +180,19 @@ int mb_solver_kkt_factor(mb_solver* s, double delta, double* rho);
  * blocks): else an error. Returns 1 on success, 0 on a failed factorization, negative on error. */
 int mb_solver_kkt_factor_h(mb_solver* s, const int64_t* H_xsrc, const int64_t* H_tgt, const int64_t* H_xblk,
                            const double* H_val, int32_t index_base, double delta, double* rho);
+
+/* Differentiation of the solution map at the solver's last iterate (after mb_solver_solve or
+ * steps), in the caller's space (the same as mb_solver_result): a fresh factor of δH + BᵀB at
+ * getaug's δ, one KKT solve each, the KKT tolerances of differential.jl:5-16.
+ * mb_solver_frule: frule!(Δp, Δy, Δd, s, Δf, Δg, Δμ) (differential.jl:5-75): the tangent of
+ * (p, y, d) for perturbations Δf (n), Δg (m) of f and g and Δμ of the barrier target.
+ * mb_solver_rrule: rrule!(Δf, Δg, s, Δp, Δy, Δd) (differential.jl:252-334): the cotangent; *dmu
+ * receives the μ component. Return 0, or negative on error (a solver with no iterations, a
+ * failed factorization). */
+int mb_solver_frule(mb_solver* s, const double* df, const double* dg, double dmu, double* dp, double* dy,
+                    double* dd);
+int mb_solver_rrule(mb_solver* s, const double* dp, const double* dy, const double* dd, double* df, double* dg,
+                    double* dmu);
 
 /* Cone fan-outs on the solver's iterate in its internal coordinates (columns permuted by the
  * problem construction and scaled by the equilibration, as the reference's IPMSolver holds p and

@@ -412,6 +412,23 @@ int mb_solver_kkt_factor_h(mb_solver* s, const int64_t* H_xsrc, const int64_t* H
   });
 }
 
+int mb_solver_frule(mb_solver* s, const double* df, const double* dg, double dmu, double* dp, double* dy,
+                    double* dd) {
+  return guard([&]() {
+    s->S->frule(df, dg, dmu, dp, dy, dd);
+    return 0;
+  });
+}
+
+int mb_solver_rrule(mb_solver* s, const double* dp, const double* dy, const double* dd, double* df, double* dg,
+                    double* dmu) {
+  return guard([&]() {
+    const double v = s->S->rrule(dp, dy, dd, df, dg);
+    if (dmu) *dmu = v;
+    return 0;
+  });
+}
+
 int64_t mb_solver_ncone(const mb_solver* s) { return (int64_t)s->S->internal().K.size(); }
 
 int mb_solver_layout(const mb_solver* s, int64_t* P1, int64_t* P2, int64_t* xcol, int32_t* kind, double* param,

@@ -165,6 +165,11 @@ void launch_cone_matvec_dense(hipStream_t, const InstDev*, const ConeDesc*, cons
 void launch_cone_hnorm2_dense(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, double* out,
                               double* part);
 void launch_zerofree(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx);
+// dualshadow! y = −f′(x) (cone.jl:556-566) of every cone but the PSD ones (launch_psd_dshadow,
+// launch_bpsd_dshadow)
+void launch_cone_dshadow(hipStream_t, const InstDev*, const ConeDesc*, int nc, int kx, int ky);
+void launch_psd_dshadow(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n, int kx, int ky);
+void launch_bpsd_dshadow(hipStream_t, const InstDev*, const ConeDesc*, const BigPsd&, int kx, int ky);
 // the shadow-solve seed of every listed exponential / power cone (initcache!)
 void launch_td_initcache(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n);
 void launch_fill(hipStream_t, double* x, size_t n, double v);

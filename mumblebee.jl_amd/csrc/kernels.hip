@@ -557,6 +557,38 @@ __global__ void __launch_bounds__(64) k_soc_scale(const InstDev* insts, const Co
   }
 }
 
+// dualshadow! (cone.jl:556-566): y = −f′(x) per cone, one thread per cone — nonnegative 1/x
+// (pos.jl), free 0 (noc.jl), second-order 2 J x / det(x) with the compensated determinant
+// (soc.jl:488-504), exponential / power by the Halley shadow solve seeded from the cache (exp.jl,
+// pow.jl); PSD cones: k_psd_dshadow and the large-cone pipeline. y[cols] stays untouched where
+// the cone's point is outside the cone (the reference's shadow is then not defined).
+__global__ void k_cone_dshadow(const InstDev* insts, const ConeDesc* cones, int ncone, int kx, int ky) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncone) return;
+  ConeDesc C = cones[c];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0 || C.kind == CONE_PSD) return;
+  const double* x = nvec(I, kx) + C.col0;
+  double* y = nvec(I, ky) + C.col0;
+  const int n = C.n;
+  if (C.kind == CONE_POS) {
+    for (int i = 0; i < n; ++i) y[i] = 1.0 / x[i];
+  } else if (C.kind == CONE_FREE) {
+    for (int i = 0; i < n; ++i) y[i] = 0.0;
+  } else if (C.kind == CONE_SOC) {
+    double det, lo;
+    socdot(x, x, n, det, lo);
+    det += lo;
+    if (det > 0.0 && x[0] > 0.0) {
+      const double gm = 2.0 / det;
+      y[0] = gm * x[0];
+      for (int i = 1; i < n; ++i) y[i] = -gm * x[i];
+    }
+  } else {
+    tdc::dual_shadow(C.kind, C.param, y, x, C.cache);
+  }
+}
+
 // ‖Q_vv + D_v‖² per cone into out[c] (the diagonal-block part of ‖H‖)
 __global__ void k_cone_hnorm2(const InstDev* insts, const ConeDesc* cones, int ncone, double* out) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1329,6 +1361,9 @@ void launch_cone_maxstep(hipStream_t st, const InstDev* I, const ConeDesc* C, in
   launch_psd_maxstep(st, I, C, psd, npsd, kdp, kdd, step_frac, out);
   launch_bpsd_maxstep(st, I, C, big, kdp, kdd, step_frac, out);
   k_cone_reduce<<<ninst, TILE, 0, st>>>(I, cone_ofs, out, 1);
+}
+void launch_cone_dshadow(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx, int ky) {
+  if (nc) k_cone_dshadow<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx, ky);
 }
 void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc, int kx) {
   if (nc) k_zerofree<<<cdiv(nc, 64), 64, 0, st>>>(I, C, nc, kx);

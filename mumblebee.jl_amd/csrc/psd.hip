@@ -472,6 +472,39 @@ __global__ void __launch_bounds__(PT) k_psd_maxstep(const InstDev* insts, const 
   if (threadIdx.x == 0) out[c] = step_frac * fmin(tau[0], tau[1]);
 }
 
+// sdpshadow! (sdp.jl:666-678): y = svec(X⁻¹) with X = smat(x) = L Lᵀ; X⁻¹ = L⁻ᵀ L⁻¹. y is left
+// untouched when X is not positive definite.
+__global__ void __launch_bounds__(PT) k_psd_dshadow(const InstDev* insts, const ConeDesc* cones, const int* list,
+                                                    int kx, int ky) {
+  __shared__ double L0[PM], L1[PM], L2[PM];
+  const ConeDesc C = cones[list[blockIdx.x]];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int n = C.n, d = psd_order(n);
+  smat_lds(L0, cvec(I, kx) + C.col0, d);
+  __syncthreads();
+  if (!chol_lds(L0, nullptr, d)) return;
+  for (int j = threadIdx.x; j < d; j += PT)  // L1 = L⁻¹ column by column (forward substitution)
+    for (int i = 0; i < d; ++i) {
+      double a = i == j ? 1.0 : 0.0;
+      for (int k = j; k < i; ++k) a -= L0[i + k * PLD] * L1[k + j * PLD];
+      L1[i + j * PLD] = i < j ? 0.0 : a / L0[i + i * PLD];
+    }
+  __syncthreads();
+  gemm_lds<true, false>(L2, L1, L1, d);  // L⁻ᵀ L⁻¹
+  __syncthreads();
+  double* y = cvec(I, ky) + C.col0;
+  for (int j = 0; j < d; ++j)
+    for (int i = j + (int)threadIdx.x; i < d; i += PT) {
+      const double x = 0.5 * (L2[i + j * PLD] + L2[j + i * PLD]);
+      y[svpos(i, j, d)] = i == j ? x : R2 * x;
+    }
+}
+
+void launch_psd_dshadow(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* list, int n, int kx, int ky) {
+  if (n) k_psd_dshadow<<<n, PT, 0, st>>>(I, C, list, kx, ky);
+}
+
 // y += a D x on the dense scaling blocks (second-order and PSD cones), one workgroup per cone
 // y += a·D x over the dense-block cones: grid (cone, 64-row slice); each wave sums a quarter of
 // the columns (j ≡ wave mod 4) for the slice's rows (coalesced along the rows), and the four

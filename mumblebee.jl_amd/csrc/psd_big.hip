@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(NT) k_bp_smat(const InstDev* insts, const Cone
   }
 }
 
-// op 0: X1 = R·diag(√s) (the factor Y of W = Y Yᵀ); op 1: X0 = R·diag(σμ − s²) (corr0);
+// op 0: X1 = R·diag(√s) (the factor Y of W = Y Yᵀ); op 1: X0 = R·diag(σμ − s²) (corr0); op 4: X1 = I;
 // op 2: X0 = Ŵ from X2 = A B (corr: Ŵ_ij = −(s_i X_ij + s_j X_ji)/(s_i + s_j), Ŵ_jj = σμ − s_j² −
 // X_jj); op 3: the max step's M (X2 symmetrised; side 1: ÷ s_i s_j) into X0 and I + M into X1
 __global__ void __launch_bounds__(NT) k_bp_ew(const InstDev* insts, const ConeDesc* cones, const int* list, int op,
@@ -168,11 +168,13 @@ __global__ void __launch_bounds__(NT) k_bp_ew(const InstDev* insts, const ConeDe
       if (i == j) w = smu - v.s[j] * v.s[j] - v.X2[t];
       else w = -(v.s[i] * v.X2[t] + v.s[j] * v.X2[j + (size_t)i * d]) / (v.s[i] + v.s[j]);
       v.X0[t] = w;
-    } else {
+    } else if (op == 3) {
       double m = 0.5 * (v.X2[t] + v.X2[j + (size_t)i * d]);
       if (side == 1) m /= v.s[i] * v.s[j];
       v.X0[t] = m;
       v.X1[t] = m + (i == j ? 1.0 : 0.0);
+    } else if (op == 4) {
+      v.X1[t] = i == j ? 1.0 : 0.0;
     }
   }
 }
@@ -324,14 +326,14 @@ __global__ void k_bp_flag_and(const InstDev* insts, const ConeDesc* cones, const
 // over tile rows; one workgroup per cone
 // ------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_bp_trsm_lt(const InstDev* insts, const ConeDesc* cones, const int* list, int sl,
-                                                   int sbm, int so) {
+                                                   int sbm, int so, int flag) {
   __shared__ __align__(16) double SA[NB * LDT];
   __shared__ __align__(16) double SB[NB * LDT];
   const ConeDesc C = cones[list[blockIdx.x]];
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
   const BV v = bview(C);
-  if (v.aux[AX_OK] == 0.0) return;
+  if (v.aux[flag] == 0.0) return;  // the Cholesky factor L is not there
   const double* L = slot_ptr(v, sl);
   const double* Bm = slot_ptr(v, sbm);
   double* O = slot_ptr(v, so);
@@ -729,7 +731,7 @@ void launch_bpsd_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, cons
   gemm(st, I, C, B, BP_X0, 1, BP_X1, 0, BP_X2);                       // LPᵀ LD
   k_bp_jacobi<<<B.n, NT, 0, st>>>(I, C, B.list);                      // X2 = U, s = Σ
   gemm(st, I, C, B, BP_X0, 0, BP_X2, 0, BP_S);                        // S = LP U
-  k_bp_trsm_lt<<<B.n, NT, 0, st>>>(I, C, B.list, BP_X0, BP_X2, BP_R); // R = LP⁻ᵀ U
+  k_bp_trsm_lt<<<B.n, NT, 0, st>>>(I, C, B.list, BP_X0, BP_X2, BP_R, AX_OK);  // R = LP⁻ᵀ U
   ew(st, I, C, B, 0, 0);                                              // X1 = R Σ^½
   gemm(st, I, C, B, BP_X1, 0, BP_X1, 1, BP_W);                        // W = X1 X1ᵀ
   k_bp_scale_out<<<B.n, 64, 0, st>>>(I, C, B.list, out);
@@ -794,6 +796,34 @@ void launch_bpsd_assemble(hipStream_t st, const InstDev* I, const ConeDesc* C, c
   if (!B.n) return;
   const int groups = std::max(1, std::min(1024, B.dmax * (B.dmax + 1) / 2 / 4));
   k_bp_assemble<<<dim3(B.n, groups), NT, 0, st>>>(I, C, B.list, cone_front, F, nullptr);
+}
+
+// sdpshadow! on the large cones: X = smat(x) = L Lᵀ, Y = L⁻ᵀ (the solve on I), X⁻¹ = Y Yᵀ → y;
+// y stays untouched where X is not positive definite
+__global__ void k_bp_svec_if(const InstDev* insts, const ConeDesc* cones, const int* list, int slot, int kr, int flag) {
+  const ConeDesc C = cones[list[blockIdx.x]];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const BV v = bview(C);
+  if (v.aux[flag] == 0.0) return;
+  const double* M = slot_ptr(v, slot);
+  double* r = cvec(I, kr) + C.col0;
+  for (int a = blockIdx.y * NT * 8 + threadIdx.x; a < v.n && a < (int)(blockIdx.y + 1) * NT * 8; a += NT) {
+    int i, j;
+    svec_ij(a, v.d, i, j);
+    const double x = 0.5 * (M[i + (size_t)j * v.d] + M[j + (size_t)i * v.d]);
+    r[a] = i == j ? x : R2 * x;
+  }
+}
+
+void launch_bpsd_dshadow(hipStream_t st, const InstDev* I, const ConeDesc* C, const BigPsd& B, int kx, int ky) {
+  if (!B.n) return;
+  smat(st, I, C, B, kx, BP_X0);
+  k_bp_chol<<<B.n, NT, 0, st>>>(I, C, B.list, BP_X0, AX_CHOL1, 0);
+  ew(st, I, C, B, 4, 0);  // X1 = I
+  k_bp_trsm_lt<<<B.n, NT, 0, st>>>(I, C, B.list, BP_X0, BP_X1, BP_X2, AX_CHOL1);
+  gemm(st, I, C, B, BP_X2, 0, BP_X2, 1, BP_X1);  // L⁻ᵀ L⁻¹
+  k_bp_svec_if<<<dim3(B.n, sv_chunks(B.dmax)), NT, 0, st>>>(I, C, B.list, BP_X1, ky, AX_CHOL1);
 }
 
 void launch_bpsd_block(hipStream_t st, const InstDev* I, const ConeDesc* C, const int* one, int n, double* D) {

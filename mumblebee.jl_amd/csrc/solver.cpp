@@ -1723,6 +1723,98 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
 }
 
 // ------------------------------------------------------------------------------------------
+// differentiation (src/IPM/src/ipm/differential.jl): frule! and rrule! on the solver's last
+// iterate, its scaling H and a fresh factor, every vector on the device
+// ------------------------------------------------------------------------------------------
+
+// dualshadow! (cone.jl:556-566) of vector kind kx into ky, every cone kind
+void Solver::dshadow(int kx, int ky) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  launch_cone_dshadow(st_, I, d_cones_.as<ConeDesc>(), ncone_, kx, ky);
+  launch_psd_dshadow(st_, I, d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, kx, ky);
+  launch_bpsd_dshadow(st_, I, d_cones_.as<ConeDesc>(), bpsd(), kx, ky);
+}
+
+// y += a (H − Q) x: the cone blocks of H
+void Solver::cone_matvec_all(int kx, int ky, Coef a) {
+  const InstDev* I = d_inst_.as<InstDev>();
+  launch_cone_matvec(st_, I, d_cones_.as<ConeDesc>(), ncone_, kx, ky, a);
+  launch_cone_matvec_dense(st_, I, d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_, dense_nmax_, kx, ky, a);
+  launch_bpsd_matvec(st_, I, d_cones_.as<ConeDesc>(), bpsd(), kx, ky, a);
+}
+
+// the factor of δH + BᵀB at getaug's δ and the forcing tolerances of differential.jl:5-16
+void Solver::diff_setup(double& ftol, double& gtol) {
+  if (hist_.empty()) throw std::logic_error("differentiation needs a solved (stepped) solver");
+  use_user_h(false);
+  const double delta = getaug();
+  set_scalar(S_ALPHA, 1.0 / delta);
+  double rho = 0.0;
+  if (!factor(delta, rho)) throw std::runtime_error("initkkt! failed");
+  const double mu0 = hist_.front().mu;
+  const double tol = (mu0 == 0.0 || !std::isfinite(mu0)) ? FORCING_FRAC * set_.gap_tol
+                                                          : FORCING_FRAC * hist_.back().mu / mu0;
+  ftol = tol * (1 + nf_);
+  gtol = tol * (1 + ng_);
+}
+
+// frule! (differential.jl:5-75): the tangent (Δp, Δy, Δd) of the solution for seeds (Δf, Δg, Δμ)
+void Solver::frule(const double* df, const double* dg, double dmu, double* dp, double* dy, double* dd) {
+  double ftol, gtol;
+  diff_setup(ftol, gtol);
+  std::vector<double> fx(n_), gx(m_), px(n_), yx(m_), dx(n_);
+  for (i64 j = 0; j < n_; ++j) fx[j] = df[P_.P2[j]] * pscl_[j];
+  for (i64 i = 0; i < m_; ++i) gx[i] = dg[P_.P1[i]] * yscl_[i];
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_FR * n_, fx.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_DG * m_, gx.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  dshadow(NV_P, NV_T1);  // d*(p)
+  // Δd = Δμ d* (into T2), f-side right-hand side Δf + Δμ d*
+  lin(LinSet()
+          .add(Lin{NV_T1, NV_T1, NV_T1, NV_T2, K(dmu), K(0.0), K(0.0)})
+          .add(Lin{NV_T2, NV_FR, NV_FR, NV_FR, K(1.0), K(1.0), K(0.0)}),
+      LinSet());
+  solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, false, ftol, gtol, 1, false);
+  cone_matvec_all(NV_DP, NV_T2, K(-1.0));  // Δd = Δμ d* − (H − Q) Δp
+  HIPCHK(hipMemcpyAsync(px.data(), idev_.nv + (size_t)NV_DP * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(yx.data(), idev_.mv + (size_t)MV_DY * m_, m_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(dx.data(), idev_.nv + (size_t)NV_T2 * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  for (i64 j = 0; j < n_; ++j) {
+    dp[P_.P2[j]] = px[j] * pscl_[j];
+    dd[P_.P2[j]] = dx[j] / pscl_[j];
+  }
+  for (i64 i = 0; i < m_; ++i) dy[P_.P1[i]] = yx[i] * yscl_[i];
+}
+
+// rrule! (differential.jl:252-334): the cotangent (Δf, Δg, dμ) of the seeds (Δp, Δy, Δd)
+double Solver::rrule(const double* dp, const double* dy, const double* dd, double* df, double* dg) {
+  double ftol, gtol;
+  diff_setup(ftol, gtol);
+  std::vector<double> px(n_), dx(n_), yx(m_), fx(n_), gx(m_);
+  for (i64 j = 0; j < n_; ++j) {
+    px[j] = dp[P_.P2[j]] * pscl_[j];
+    dx[j] = dd[P_.P2[j]] / pscl_[j];
+  }
+  for (i64 i = 0; i < m_; ++i) yx[i] = -dy[P_.P1[i]] * yscl_[i];
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_FR * n_, px.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_T2 * n_, dx.data(), n_ * 8, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_DG * m_, yx.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  cone_matvec_all(NV_T2, NV_FR, K(-1.0));  // Δp −= (H − Q) Δd
+  solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, false, ftol, gtol, 1, false);
+  lin(LinSet().add(Lin{NV_DP, NV_T2, NV_T2, NV_T2, K(1.0), K(1.0), K(0.0)}), LinSet());  // λd = Δd + λp
+  dshadow(NV_P, NV_T1);
+  dots(DotBatch().add(0, NV_T2, NV_T1, 0, S_TMP3, 0));
+  sync_scalars();
+  const double dmu = hsc_[S_TMP3];
+  HIPCHK(hipMemcpyAsync(fx.data(), idev_.nv + (size_t)NV_DP * n_, n_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(gx.data(), idev_.mv + (size_t)MV_DY * m_, m_ * 8, hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  for (i64 j = 0; j < n_; ++j) df[P_.P2[j]] = fx[j] * pscl_[j];
+  for (i64 i = 0; i < m_; ++i) dg[P_.P1[i]] = -gx[i] * yscl_[i];
+  return dmu;
+}
+
+// ------------------------------------------------------------------------------------------
 // step! (solver/ipm.jl:577-774)
 // ------------------------------------------------------------------------------------------
 

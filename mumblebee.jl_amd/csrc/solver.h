@@ -118,6 +118,9 @@ class Solver {
   // initkkt!(kkt, H; δ) with the caller's H (internal vertex partition, both triangles); the KKT
   // solves then use it until the next scale / step / kkt_factor
   int kkt_factor_h(const BSM& H, double delta, double* rho);
+  // frule! / rrule! (differential.jl:5-75, 252-334) at the last iterate, in the caller's space
+  void frule(const double* df, const double* dg, double dmu, double* dp, double* dy, double* dd);
+  double rrule(const double* dp, const double* dy, const double* dd, double* df, double* dg);
   // cone fan-outs on the iterate, in the solver's internal (permuted, scaled) coordinates
   // (cone.jl:146-553); scale replaces the iterate by (p, d)
   bool cone_scale(const double* p, const double* d, double* spsd);
@@ -288,6 +291,9 @@ class Solver {
   int nhtiles_ = 0;
   i64 nhent_ = 0;
   void use_user_h(bool on);
+  void dshadow(int kx, int ky);
+  void cone_matvec_all(int kx, int ky, Coef a);
+  void diff_setup(double& ftol, double& gtol);
   void check_abort();
 };
 

@@ -153,6 +153,8 @@ EXPORTS = {
     "mb_cached_bytes": (ctypes.c_int64, []),
     "mb_release_cached_memory": (ctypes.c_int64, []),
     "mb_solver_kkt_factor": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _DP]),
+    "mb_solver_frule": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, ctypes.c_double, _DP, _DP, _DP]),
+    "mb_solver_rrule": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP, _DP, _DP, _DP]),
     "mb_solver_kkt_factor_h": (ctypes.c_int, [ctypes.c_void_p, _I64P, _I64P, _I64P, _DP, ctypes.c_int32,
                                                ctypes.c_double, _DP]),
     "mb_solver_dense_factor": (ctypes.c_int, [ctypes.c_void_p, _DP]),
@@ -559,6 +561,32 @@ class IPMSolver:
         if r < 0:
             raise _err("initkkt!")
         return bool(r), rho.value
+
+    def frule(self, df, dg, dmu=0.0):
+        """frule!(Δp, Δy, Δd, s, Δf, Δg, Δμ) (differential.jl:5-75) at the last iterate, in the
+        problem's space: returns (Δp, Δy, Δd)."""
+        n, m = self.prob.n, self.prob.m
+        df = np.ascontiguousarray(df, dtype=np.float64)
+        dg = np.ascontiguousarray(dg, dtype=np.float64)
+        if df.shape != (n,) or dg.shape != (m,):
+            raise ValueError("frule: Δf and Δg must have lengths n and m")
+        dp, dy, dd = np.zeros(n), np.zeros(m), np.zeros(n)
+        if lib().mb_solver_frule(self._h, _p(df, ctypes.c_double), _p(dg, ctypes.c_double), float(dmu),
+                                 _p(dp, ctypes.c_double), _p(dy, ctypes.c_double), _p(dd, ctypes.c_double)) != 0:
+            raise _err("frule!")
+        return dp, dy, dd
+
+    def rrule(self, dp, dy, dd):
+        """rrule!(Δf, Δg, s, Δp, Δy, Δd) (differential.jl:252-334): returns (Δf, Δg, dμ)."""
+        n, m = self.prob.n, self.prob.m
+        dp, dy, dd = (np.ascontiguousarray(a, dtype=np.float64) for a in (dp, dy, dd))
+        if dp.shape != (n,) or dd.shape != (n,) or dy.shape != (m,):
+            raise ValueError("rrule: Δp, Δd must have length n and Δy length m")
+        df, dg, dmu = np.zeros(n), np.zeros(m), ctypes.c_double(0.0)
+        if lib().mb_solver_rrule(self._h, _p(dp, ctypes.c_double), _p(dy, ctypes.c_double), _p(dd, ctypes.c_double),
+                                 _p(df, ctypes.c_double), _p(dg, ctypes.c_double), ctypes.byref(dmu)) != 0:
+            raise _err("rrule!")
+        return df, dg, dmu.value
 
     def kkt_factor_h(self, H, delta):
         """initkkt!(kkt, H; δ) (uzawa.jl:73-76) with the caller's H: a BlockSparse over the internal

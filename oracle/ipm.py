@@ -448,5 +448,67 @@ class Result:
     dobj: float
 
 
+def _dualshadow(solver):
+    """dualshadow!(d*, B, p, K, …) (cone.jl:556-566) at the solver's iterate."""
+    ds = np.zeros_like(solver.p)
+    for v, a, b in solver.vranges():
+        C.shadow(solver.K[v], ds[a:b], solver.p[a:b])
+    return ds
+
+
+def _diff_setup(solver):
+    """differential.jl:5-16: initkkt! at getaug's δ, the forcing tolerances from μ's history."""
+    delta = solver.getaug()
+    ok, _ = solver.kkt.init(solver.Hd, delta)
+    if not ok:
+        raise RuntimeError("initkkt! failed")
+    mu0 = solver.hist[0]["mu"]
+    tol = FORCING_FRAC * solver.settings.gap_tol if (mu0 == 0 or not math.isfinite(mu0)) else \
+        FORCING_FRAC * solver.hist[-1]["mu"] / mu0
+    return tol * (1 + solver.nf), tol * (1 + solver.ng)
+
+
+def _kkt(solver, f, g, ftol, gtol):
+    st = solver.settings
+    x, y = np.zeros_like(f), np.zeros_like(g)
+    solvekkt(solver.kkt, solver.Hd, solver.Bd, x, y, f, g, False, ftol, gtol, st.refine_stall_tol,
+             st.refine_max_iter, st.newton_max_iter, 1)
+    return x, y
+
+
+def frule(solver, df, dg, dmu):
+    """frule!(Δp, Δy, Δd, s, Δf, Δg, Δμ) (differential.jl:5-75) at the solver's last iterate, in
+    the problem's space. Pinned by tests/test_oracle_ipm.py: the equality-QP closed form, the
+    adjoint identity with rrule and central finite differences of whole solves."""
+    ftol, gtol = _diff_setup(solver)
+    ps, ys = solver.pscl, solver.yscl
+    fx = np.asarray(df, dtype=np.float64)[solver.P2] * ps
+    gx = np.asarray(dg, dtype=np.float64)[solver.P1] * ys
+    dx = dmu * _dualshadow(solver)
+    px, yx = _kkt(solver, fx + dx, gx, ftol, gtol)
+    dx = dx - (solver.Hd - solver.Qd) @ px
+    dp, dy, dd = (np.zeros(len(px)), np.zeros(len(yx)), np.zeros(len(px)))
+    dp[solver.P2] = px * ps
+    dy[solver.P1] = yx * ys
+    dd[solver.P2] = dx / ps
+    return dp, dy, dd
+
+
+def rrule(solver, dp, dy, dd):
+    """rrule!(Δf, Δg, s, Δp, Δy, Δd) (differential.jl:252-334): (Δf, Δg, dμ)."""
+    ftol, gtol = _diff_setup(solver)
+    ps, ys = solver.pscl, solver.yscl
+    px = np.asarray(dp, dtype=np.float64)[solver.P2] * ps
+    dx = np.asarray(dd, dtype=np.float64)[solver.P2] / ps
+    yx = -np.asarray(dy, dtype=np.float64)[solver.P1] * ys
+    px = px - (solver.Hd - solver.Qd) @ dx
+    fx, gx = _kkt(solver, px, yx, ftol, gtol)
+    dx = dx + fx
+    df, dg = np.zeros(len(fx)), np.zeros(len(gx))
+    df[solver.P2] = fx * ps
+    dg[solver.P1] = -gx * ys
+    return df, dg, float(dx @ _dualshadow(solver))
+
+
 def solve(prob, settings=None, order="mindegree", **kw):
     return Solver(prob, settings, order=order, **kw).solve()

@@ -95,3 +95,83 @@ def test_dual_infeasible():
     r = O.solve(oracle_problem(d))
     assert r.status in (O.DUAL_INFEASIBLE, O.NEAR_DUAL_INFEASIBLE)
 
+
+
+def test_frule_of_an_equality_qp_is_its_kkt_solve():
+    """ν = 0 (free cones): the solution is linear in (f, g), so frule! is the KKT solve of the
+    seeds (differential.jl:5-75)."""
+    rng = np.random.default_rng(51)
+    n, m = 20, 6
+    M = rng.standard_normal((n, n))
+    Q = M @ M.T / n + np.eye(n)
+    A = rng.standard_normal((m, n))
+    d = dict(Q=sp.csc_matrix(Q), B=sp.csc_matrix(A), f=rng.standard_normal(n), g=rng.standard_normal(m),
+             K=[("free",)], s=[n])
+    s = O.Solver(oracle_problem(d))
+    assert s.solve().status == O.OPTIMAL
+    df, dg = rng.standard_normal(n), rng.standard_normal(m)
+    dp, dy, dd = O.frule(s, df, dg, 0.0)
+    ref = np.linalg.solve(np.block([[Q, -A.T], [A, np.zeros((m, m))]]), np.concatenate([df, dg]))
+    np.testing.assert_allclose(dp, ref[:n], rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(dy, ref[n:], rtol=1e-8, atol=1e-10)
+    assert np.abs(dd).max() < 1e-12
+
+
+def _adjoint_gap(s, rng, n, m, dmu):
+    df, dg = rng.standard_normal(n), rng.standard_normal(m)
+    dp, dy, dd = O.frule(s, df, dg, dmu)
+    pb, yb, db = rng.standard_normal(n), rng.standard_normal(m), rng.standard_normal(n)
+    fb, gb, mub = O.rrule(s, pb, yb, db)
+    lhs = pb @ dp + yb @ dy + db @ dd
+    rhs = fb @ df + gb @ dg + mub * dmu
+    scale = np.linalg.norm(np.concatenate([pb, yb, db])) * np.linalg.norm(np.concatenate([dp, dy, dd]))
+    return abs(lhs - rhs) / scale
+
+
+DIFF_CASES = [(G.qp, dict(n=60, m=20, seed=52, pdens=0.05, adens=0.1)),
+              (G.socp, dict(n=90, m=40, ncone=5, seed=53, adens=0.06)),
+              (G.maxcut_chordal, dict(nv=30, band=4, extra=0.0, seed=54))]
+
+
+@pytest.mark.parametrize("gen,kw", DIFF_CASES, ids=["qp", "socp", "psd"])
+def test_frule_is_the_derivative_of_the_central_path_and_rrule_its_adjoint(gen, kw):
+    """On the central path (target barrier μt = 1e-2, barrier gap 1e-12) the NT scaling is
+    μt ∇²F(p), so frule!(Δf, Δg, Δμ) is the derivative of (f, g, μt) ↦ (p, y, d): central
+    finite differences of whole solves pin it for every cone kind, Δμ included. rrule! is pinned
+    by the adjoint identity ⟨p̄, Δp⟩ + ⟨ȳ, Δy⟩ + ⟨d̄, Δd⟩ = ⟨f̄, Δf⟩ + ⟨ḡ, Δg⟩ + μ̄ Δμ, at the
+    optimum (Δμ = 0) and on the central path."""
+    d = gen(**kw)
+    rng = np.random.default_rng(60)
+    n, m = d["B"].shape[1], d["B"].shape[0]
+    st = O.Settings(gap_tol=1e-12)
+    s = O.Solver(oracle_problem(d))
+    assert s.solve().status == O.OPTIMAL
+    assert _adjoint_gap(s, rng, n, m, 0.0) <= 1e-5  # forcing tolerance × KKT condition ~1/μ
+    mut = 1e-2
+    s = O.Solver(oracle_problem(d, mu=mut), st)
+    assert s.solve().status == O.OPTIMAL
+    assert _adjoint_gap(s, rng, n, m, 0.3 * mut) <= 1e-8
+    df, dg, dmu = rng.standard_normal(n) * 0.1, rng.standard_normal(m) * 0.1, 0.3 * mut
+    dx = O.frule(s, df, dg, dmu)
+    eps = 1e-4  # the O(ε²) term is ~1e-4 relative at ε = 1e-3 on the chordal max-cut
+    r = [O.solve(oracle_problem(dict(d, f=d["f"] + sg * eps * df, g=d["g"] + sg * eps * dg), mu=mut + sg * eps * dmu),
+                 st) for sg in (1, -1)]
+    assert all(x.status == O.OPTIMAL for x in r)
+    # d carries the centring error of the last iterate (pᵢdᵢ = μt to the barrier gap) most visibly
+    for k, dk, tol in zip("pyd", dx, (1e-4, 1e-4, 1e-3)):
+        fd = (getattr(r[0], k) - getattr(r[1], k)) / (2 * eps)
+        assert np.linalg.norm(fd - dk) <= tol * np.linalg.norm(dk), k
+
+
+def test_a_dropped_dual_shadow_term_fails_the_central_path_check():
+    """The check above is sharp: frule! without the Δμ d* term misses the μ-derivative."""
+    d = G.qp(n=60, m=20, seed=52, pdens=0.05, adens=0.1)
+    mut, eps = 1e-2, 1e-3
+    st = O.Settings(gap_tol=1e-12)
+    s = O.Solver(oracle_problem(d, mu=mut), st)
+    assert s.solve().status == O.OPTIMAL
+    n, m = d["B"].shape[1], d["B"].shape[0]
+    dp, _, _ = O.frule(s, np.zeros(n), np.zeros(m), 0.0)  # the planted mistake: Δμ not seeded
+    r = [O.solve(oracle_problem(d, mu=mut + sg * eps * mut), st) for sg in (1, -1)]
+    fd = (r[0].p - r[1].p) / (2 * eps)
+    assert np.linalg.norm(fd - dp) > 0.1 * np.linalg.norm(fd)
