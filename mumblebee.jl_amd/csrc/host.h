@@ -65,7 +65,12 @@ CSC dropoffdz(const CSC& Q);
 BSM selectvtxs(const BSM& A, const std::vector<i64>& sel);
 BSM symselectvtxs(const BSM& A, const std::vector<i64>& sel);
 BSM transpose_bsm(const BSM& A);
-BSM gram(const BSM& B);  // BᵀB, both triangles, arcs sorted
+// The lower triangle (i ≥ j) of BᵀB as scalar CSC (colp: n + 1 offsets; rows sorted within each
+// column), from the stored nonzeros of B given by rows (rp, rc, rv) and by columns (cp, cr, cv).
+// Each entry sums its products in the order of B's rows.
+void gram_lower(i64 n, const std::vector<int>& rp, const std::vector<int>& rc, const std::vector<double>& rv,
+                const std::vector<int>& cp, const std::vector<int>& cr, const std::vector<double>& cv,
+                std::vector<i64>& colp, std::vector<int>& rows, std::vector<double>& vals);
 
 // Symbolic factor structure in the final vertex order.
 struct Symbolic {

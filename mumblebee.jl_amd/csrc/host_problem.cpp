@@ -522,64 +522,52 @@ BSM transpose_bsm(const BSM& A) {
 // uzawa.jl:85 (`B' * B`), pattern as matprod_dest (block_sparse_matrix.jl:1171-1252). For each
 // vertex v: its neighbours w (vertices sharing an out vertex with v) in increasing order, each
 // block Σ_rows B_wᵀ B_v accumulated in arc order straight into the result.
-BSM gram(const BSM& B) {
-  const i64 N = B.nvtx, U = B.nout, NA = B.narc();
-  std::vector<i64> tp(U + 1, 0), tv(NA), te(NA);  // out vertex → (vertex, arc), by vertex
-  for (i64 e = 0; e < NA; ++e) tp[B.tgt[e] + 1]++;
-  for (i64 u = 0; u < U; ++u) tp[u + 1] += tp[u];
-  {
-    std::vector<i64> nx(tp.begin(), tp.end() - 1);
-    for (i64 v = 0; v < N; ++v)
-      for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
-        const i64 k = nx[B.tgt[e]]++;
-        tv[k] = v;
-        te[k] = e;
-      }
-  }
-  BSM R;
-  R.nout = R.nvtx = N;
-  R.xcol = B.xcol;
-  R.xrow = B.xcol;
-  R.xsrc.reserve(N + 1);
-  R.xsrc.push_back(0);
-  R.xblk.push_back(0);
-  std::vector<i64> mark(N, -1), slot(N, 0), nbr;
-  for (i64 v = 0; v < N; ++v) {
-    nbr.clear();
-    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e)
-      for (i64 k = tp[B.tgt[e]]; k < tp[B.tgt[e] + 1]; ++k) {
-        const i64 w = tv[k];
-        if (mark[w] != v) {
-          mark[w] = v;
-          nbr.push_back(w);
+void gram_lower(i64 n, const std::vector<int>& rp, const std::vector<int>& rc, const std::vector<double>& rv,
+                const std::vector<int>& cp, const std::vector<int>& cr, const std::vector<double>& cv,
+                std::vector<i64>& colp, std::vector<int>& rows, std::vector<double>& vals) {
+  colp.assign(n + 1, 0);
+  rows.clear();
+  vals.clear();
+  std::vector<i64> mark(n, -1);
+  std::vector<double> acc(n, 0.0);
+  std::vector<int> pat;
+  // the pattern of a column comes out sorted by a bitset scan where n is small, else by a sort
+  const bool bits = n <= (1 << 14);
+  std::vector<uint64_t> row(bits ? (n + 63) / 64 : 0, 0);
+  for (i64 j = 0; j < n; ++j) {
+    pat.clear();
+    for (int t = cp[j]; t < cp[j + 1]; ++t) {
+      const int r = cr[t];
+      const double bj = cv[t];
+      // the row's columns are sorted: start at the first one ≥ j
+      const int* c0 = rc.data() + rp[r];
+      const int* c1 = rc.data() + rp[r + 1];
+      for (const int* q = std::lower_bound(c0, c1, (int)j); q < c1; ++q) {
+        const int i = *q;
+        if (mark[i] != j) {
+          mark[i] = j;
+          acc[i] = 0.0;
+          pat.push_back(i);
         }
-      }
-    std::sort(nbr.begin(), nbr.end());
-    const i64 nv = B.ncols(v);
-    for (i64 w : nbr) {
-      slot[w] = (i64)R.val.size();
-      R.val.resize(R.val.size() + B.ncols(w) * nv, 0.0);
-      R.tgt.push_back(w);
-      R.xblk.push_back((i64)R.val.size());
-    }
-    for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
-      const i64 nr = B.nrows(B.tgt[e]);
-      const double* Be = B.val.data() + B.xblk[e];
-      for (i64 k = tp[B.tgt[e]]; k < tp[B.tgt[e] + 1]; ++k) {
-        const i64 w = tv[k], nw = B.ncols(w);
-        const double* Bf = B.val.data() + B.xblk[te[k]];
-        double* blk = R.val.data() + slot[w];
-        for (i64 jj = 0; jj < nv; ++jj)
-          for (i64 ii = 0; ii < nw; ++ii) {
-            double acc = 0.0;
-            for (i64 q = 0; q < nr; ++q) acc += Bf[ii * nr + q] * Be[jj * nr + q];
-            blk[jj * nw + ii] += acc;
-          }
+        acc[i] += rv[q - rc.data()] * bj;
       }
     }
-    R.xsrc.push_back((i64)R.tgt.size());
+    if (bits) {
+      for (int i : pat) row[i >> 6] |= 1ull << (i & 63);
+      const size_t np = pat.size();
+      pat.clear();
+      for (size_t w = (size_t)(j >> 6); pat.size() < np; ++w)
+        for (uint64_t x = row[w]; x; x &= x - 1) pat.push_back((int)(w * 64 + __builtin_ctzll(x)));
+      for (int i : pat) row[i >> 6] = 0;
+    } else {
+      std::sort(pat.begin(), pat.end());
+    }
+    for (int i : pat) {
+      rows.push_back(i);
+      vals.push_back(acc[i]);
+    }
+    colp[j + 1] = (i64)rows.size();
   }
-  return R;
 }
 
 }  // namespace mb

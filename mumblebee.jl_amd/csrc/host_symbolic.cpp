@@ -169,6 +169,26 @@ struct MinDegree {
 // reach is unchanged (a vertex outside L_p shares no element or edge with p). So each exact
 // external degree is updated by the weights of the few new members instead of recomputing a
 // union over the quotient graph, and the elimination decisions are identical.
+// r |= lp over nw words; returns the number of bits lp adds to r. The popcount instruction is
+// used where the host CPU has it (checked once at run time), the portable sequence elsewhere.
+template <int>
+static inline __attribute__((always_inline)) i64 merge_count_body(uint64_t* r, const uint64_t* lp, i64 nw) {
+  i64 c = 0;
+  for (i64 k = 0; k < nw; ++k) {
+    c += __builtin_popcountll(lp[k] & ~r[k]);
+    r[k] |= lp[k];
+  }
+  return c;
+}
+__attribute__((target("popcnt"))) static i64 merge_count_hw(uint64_t* r, const uint64_t* lp, i64 nw) {
+  return merge_count_body<1>(r, lp, nw);
+}
+static i64 merge_count_sw(uint64_t* r, const uint64_t* lp, i64 nw) { return merge_count_body<0>(r, lp, nw); }
+static i64 merge_count(uint64_t* r, const uint64_t* lp, i64 nw) {
+  static const bool hw = __builtin_cpu_supports("popcnt");
+  return hw ? merge_count_hw(r, lp, nw) : merge_count_sw(r, lp, nw);
+}
+
 struct MinDegreeBits {
   i64 N, nw;
   const std::vector<i64>& w;
@@ -235,9 +255,13 @@ struct MinDegreeBits {
       for (uint64_t b = r[k]; b; b &= b - 1) out.push_back(k * 64 + __builtin_ctzll(b));
   }
 
-  void run(std::vector<i64>& order, std::vector<std::vector<i64>>& strct) {
+  // tail: when not null, the vertices from the first one that reaches every remaining vertex
+  // onwards form a clique eliminated in order; their structures (each one's later vertices) are
+  // left empty and their count is returned here instead
+  void run(std::vector<i64>& order, std::vector<std::vector<i64>>& strct, i64* tail = nullptr) {
     order.clear();
     strct.assign(N, {});
+    if (tail) *tail = 0;
     i64 W = 0;
     for (i64 x : w) W += x;
     // lazy min-heap of (degree, vertex): an entry is current while its degree is the vertex's
@@ -260,15 +284,19 @@ struct MinDegreeBits {
         // p reaches every remaining variable: after it the rest is a clique, where minimum
         // degree picks by (-weight, index)
         order.push_back(p);
-        strct[p] = Lp;
         std::vector<i64> rest = Lp;
         std::stable_sort(rest.begin(), rest.end(), [&](i64 a, i64 b) {
           if (w[a] != w[b]) return w[a] > w[b];
           return a < b;
         });
+        for (i64 v : rest) order.push_back(v);
+        if (tail) {
+          *tail = (i64)rest.size() + 1;
+          break;
+        }
+        strct[p] = Lp;
         const bool sorted = std::is_sorted(rest.begin(), rest.end());
         for (size_t k = 0; k < rest.size(); ++k) {
-          order.push_back(rest[k]);
           std::vector<i64> s(rest.begin() + k + 1, rest.end());
           if (!sorted) std::sort(s.begin(), s.end());
           strct[rest[k]] = std::move(s);
@@ -281,15 +309,19 @@ struct MinDegreeBits {
       std::copy(&R[(size_t)(p * nw)], &R[(size_t)(p * nw)] + nw, lp.begin());
       for (i64 i : Lp) {
         uint64_t* r = &R[(size_t)(i * nw)];
+        // i joins p's reach set: every vertex of lp that i did not reach yet adds its weight. i
+        // itself is in lp and not in r (no self loops), so it is counted once and taken off
         i64 d = deg[i] - w[p];
-        for (i64 k = 0; k < nw; ++k) {
-          uint64_t nb = lp[k] & ~r[k];
-          if (k == i / 64) nb &= ~(1ull << (i % 64));
-          if (w0) d += w0 * __builtin_popcountll(nb);
-          else
-            for (uint64_t b = nb; b; b &= b - 1) d += w[k * 64 + __builtin_ctzll(b)];
-          r[k] |= nb;
+        if (w0) {
+          d += w0 * (merge_count(r, lp.data(), nw) - 1);
+        } else {
+          for (i64 k = 0; k < nw; ++k) {
+            for (uint64_t b = lp[k] & ~r[k]; b; b &= b - 1) d += w[k * 64 + __builtin_ctzll(b)];
+            r[k] |= lp[k];
+          }
+          d -= w[i];
         }
+        r[i / 64] &= ~(1ull << (i % 64));
         r[p / 64] &= ~(1ull << (p % 64));
         if (d != deg[i]) {
           deg[i] = d;
@@ -303,7 +335,7 @@ struct MinDegreeBits {
 }  // namespace
 
 static Symbolic finish(const std::vector<i64>& weights, std::vector<i64>& order,
-                       std::vector<std::vector<i64>>& strct, bool amalgamate);
+                       std::vector<std::vector<i64>>& strct, bool amalgamate, i64 tail = 0);
 
 Symbolic analyse(const std::vector<i64>& weights, const std::vector<std::vector<i64>>& adj,
                  bool amalgamate) {
@@ -333,19 +365,28 @@ Symbolic analyse_kkt(const BSM& B, const BSM& Q, bool amalgamate) {
   std::vector<i64> order;
   std::vector<std::vector<i64>> strct;
   MinDegreeBits md(weights, B, Q);
-  md.run(order, strct);
-  return finish(weights, order, strct, amalgamate);
+  i64 tail = 0;
+  md.run(order, strct, &tail);
+  return finish(weights, order, strct, amalgamate, tail);
 }
 
 // etree, postorder, fundamental supernodes and relaxed amalgamation of an elimination order with
-// its structures (strct[v]: the later neighbours of v, sorted)
+// its structures (strct[v]: the later neighbours of v, sorted). The last `tail` vertices of the
+// order form a clique eliminated in that order: their structures are implicit (each one's later
+// vertices) and strct holds nothing for them.
 static Symbolic finish(const std::vector<i64>& weights, std::vector<i64>& order,
-                       std::vector<std::vector<i64>>& strct, bool amalgamate) {
+                       std::vector<std::vector<i64>>& strct, bool amalgamate, i64 tail) {
   const i64 N = (i64)weights.size();
+  const i64 T0 = N - tail;  // first position of the clique tail
   std::vector<i64> pos(N);
   for (i64 k = 0; k < N; ++k) pos[order[k]] = k;
   std::vector<i64> parent(N, -1);
-  for (i64 v = 0; v < N; ++v) {
+  for (i64 k = 0; k < N; ++k) {
+    const i64 v = order[k];
+    if (k >= T0) {
+      parent[v] = k + 1 < N ? order[k + 1] : -1;
+      continue;
+    }
     i64 best = -1;
     for (i64 u : strct[v])
       if (best < 0 || pos[u] < pos[best]) best = u;
@@ -376,21 +417,27 @@ static Symbolic finish(const std::vector<i64>& weights, std::vector<i64>& order,
   }
   std::vector<i64> newpos(N);
   for (i64 k = 0; k < N; ++k) newpos[post[k]] = k;
+  // the clique tail is a chain whose every vertex is its parent's last child, so the postorder
+  // ends with it, in elimination order: its structures are the position ranges (k, N)
+  for (i64 k = T0; k < N; ++k)
+    if (newpos[order[k]] != k) throw std::logic_error("symbolic: clique tail out of place in the postorder");
   std::vector<i64> npar(N), nchild(N, 0);
   std::vector<std::vector<i64>> nstr(N);
   for (i64 k = 0; k < N; ++k) {
     i64 v = post[k];
     npar[k] = parent[v] >= 0 ? newpos[parent[v]] : -1;
+    if (k >= T0) continue;
     nstr[k].reserve(strct[v].size());
     for (i64 u : strct[v]) nstr[k].push_back(newpos[u]);
     if (!std::is_sorted(nstr[k].begin(), nstr[k].end())) std::sort(nstr[k].begin(), nstr[k].end());
   }
+  auto nsize = [&](i64 k) { return k >= T0 ? N - 1 - k : (i64)nstr[k].size(); };
   for (i64 k = 0; k < N; ++k)
     if (npar[k] >= 0) nchild[npar[k]]++;
   // fundamental supernodes
   std::vector<i64> fv{0};
   for (i64 k = 0; k + 1 < N; ++k) {
-    bool merge = npar[k] == k + 1 && nchild[k + 1] == 1 && nstr[k].size() == nstr[k + 1].size() + 1;
+    bool merge = npar[k] == k + 1 && nchild[k + 1] == 1 && nsize(k) == nsize(k + 1) + 1;
     if (!merge) fv.push_back(k + 1);
   }
   if (N > 0) fv.push_back(N);
@@ -407,7 +454,13 @@ static Symbolic finish(const std::vector<i64>& weights, std::vector<i64>& order,
   std::vector<std::vector<i64>> fsep(F);
   double flops = 0.0;
   for (i64 f = 0; f < F; ++f) {
-    fsep[f] = nstr[fv[f + 1] - 1];
+    const i64 last = fv[f + 1] - 1;
+    if (last >= T0) {
+      fsep[f].resize(N - 1 - last);
+      for (i64 u = last + 1; u < N; ++u) fsep[f][u - last - 1] = u;
+    } else {
+      fsep[f] = std::move(nstr[last]);
+    }
     fpar[f] = fsep[f].empty() ? -1 : vfront[fsep[f][0]];
     r[f] = xc[fv[f + 1]] - xc[fv[f]];
     s[f] = 0;

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     const int r0i = B.tb[i], hi = B.tb[i + 1] - r0i;
     const int r0j = B.tb[j], hj = B.tb[j + 1] - r0j;
     // the assembled tile (written by earlier launches; only this task writes it here)
-    load_tile<false>(SA, F, LD, r0i, r0j, hi, hj);
+    load_tile<Coh::None>(SA, F, LD, r0i, r0j, hi, hj);
     __syncthreads();
     Acc<T> C;
     acc_from_lds(C, SA);
@@ -149,8 +149,8 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
           (i != j && !wait_geq(&B.state[tri_index(j, 0)], epoch, B.abortf)))
         break;
       const int c00 = B.tb[0], w0 = B.tb[1] - c00;
-      fetch_tile<true>(RA, Lf, LD, r0i, c00, hi, w0);
-      if (i != j) fetch_tile<true>(RB, Lf, LD, r0j, c00, hj, w0);
+      fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c00, hi, w0);
+      if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c00, hj, w0);
     }
     for (int k = 0; k < kmax; ++k) {
       T* XA = (k & 1) ? SC : SA;
@@ -165,8 +165,8 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
           break;
         }
         const int c0n = B.tb[k + 1], wn = B.tb[k + 2] - c0n;
-        fetch_tile<true>(RA, Lf, LD, r0i, c0n, hi, wn);
-        if (i != j) fetch_tile<true>(RB, Lf, LD, r0j, c0n, hj, wn);
+        fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c0n, hi, wn);
+        if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c0n, hj, wn);
       }
       mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
     }
@@ -179,9 +179,9 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
         // ready), then C_jj −= L_{j,j−1} L_{j,j−1}ᵀ
         const int c0p = B.tb[j - 1], wp = B.tb[j] - c0p;
         if (!wait_geq(&B.stage[j - 1], epoch, B.abortf)) break;
-        load_tile<true>(SA, Lf, LD, r0j, c0p, hj, wp);
+        load_tile<Coh::Dev>(SA, Lf, LD, r0j, c0p, hj, wp);
         if (!wait_geq(&B.state[tri_index(j - 1, j - 1)], epoch, B.abortf)) break;
-        load_tile<true>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
+        load_tile<Coh::Dev>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
         __syncthreads();
         Acc<T> X;
         acc_zero(X);
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
         __syncthreads();
         acc_to_lds(X, SA);
         __syncthreads();
-        store_tile(Lf, LD, r0j, c0p, hj, wp, SA, false);
+        store_tile<Coh::Dev>(Lf, LD, r0j, c0p, hj, wp, SA, false);
         mfma_acc<true>(C, SA, SA, wp);
         publish(&B.state[tri_index(j, j - 1)], epoch);  // the stores drained during the update
       }
@@ -200,23 +200,23 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
       bool ok = chol_inv64(SA, SC, SB, hj, B.trace, t);  // SB is free here and lies below 64 KiB
       stamp(B.trace, t, 6);
       if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-      store_tile(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
+      store_tile<Coh::Dev>(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
       publish(&B.state[tri_index(j, j)], epoch);
       stamp(B.trace, t, 7);
-      store_tile(Lf, LD, r0j, r0j, hj, hj, SA, true);
+      store_tile<Coh::Dev>(Lf, LD, r0j, r0j, hj, hj, SA, true);
     } else if (type == 3) {
       // sub-diagonal tile: hand the accumulated tile to the next diagonal task
       acc_to_lds(C, SA);
       __syncthreads();
       stamp(B.trace, t, 2);
-      store_tile(Lf, LD, r0i, r0j, hi, hj, SA, false);
+      store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
       publish(&B.stage[j], epoch);
     } else if (type == 1) {
       // L_ij = C_ij L_jj⁻ᵀ
       acc_to_lds(C, SA);
       if (!wait_geq(&B.state[tri_index(j, j)], epoch, B.abortf)) break;
       stamp(B.trace, t, 2);
-      load_tile<true>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
+      load_tile<Coh::Dev>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
       __syncthreads();
       Acc<T> X;
       acc_zero(X);
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
       __syncthreads();
       acc_to_lds(X, SA);
       __syncthreads();
-      store_tile(Lf, LD, r0i, r0j, hi, hj, SA, false);
+      store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
       publish(&B.state[tri_index(i, j)], epoch);
     } else {
       // update-matrix tile: final once every factor column has been subtracted (FP64 for the
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
       acc_to_lds(C, SA);
       __syncthreads();
       stamp(B.trace, t, 2);
-      store_tile(F, LD, r0i, r0j, hi, hj, SA, i == j);
+      store_tile<Coh::Dev>(F, LD, r0i, r0j, hi, hj, SA, i == j);
     }
     stamp(B.trace, t, 3);
   }
@@ -364,7 +364,7 @@ __device__ __forceinline__ void gemv_rows(GemvRegs& R, const TS* F, int LD, int 
 template <class TS>
 __device__ __forceinline__ void gemv_cols(GemvRegs& R, double* S, const TS* F, int LD, int r0, int c0, int h, int w) {
   const int c = threadIdx.x & 63, q = threadIdx.x >> 6;
-  load_tile<false>(S, F, LD, r0, c0, h, w);
+  load_tile<Coh::None>(S, F, LD, r0, c0, h, w);
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < 16; ++i) R.v[i] = S[(q + 4 * i) + c * LDT];
@@ -477,7 +477,7 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   const bool diag = b < B.Tr;
   const int jacc = diag ? max(b - 1, 0) : B.Tr;  // blocks summed before the last step
   const int r0 = B.tb[b], h = B.tb[b + 1] - r0;
-  if (diag) load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  if (diag) load_tile<Coh::None>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
   double acc = 0.0;
   GemvRegs Rn;
   if (jacc > 0) gemv_rows(Rn, F, LD, r0, B.tb[0], h, B.tb[1] - B.tb[0]);
@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc*
   const int b = blockIdx.x, rs0 = Fd.r + (int)blockIdx.y * NB;
   const int h = min(NB, B.ld - rs0), c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const double* vall = I.nv + (size_t)kv * I.n;
-  load_tile<false>(S, factor_of<TF>(B), B.ld, rs0, c0, h, w);
+  load_tile<Coh::None>(S, factor_of<TF>(B), B.ld, rs0, c0, h, w);
   if (threadIdx.x < NB) xsep[threadIdx.x] = threadIdx.x < h ? vall[Fd.sepcols[rs0 - Fd.r + threadIdx.x]] : 0.0;
   __syncthreads();
   const int col = threadIdx.x & 63, q = threadIdx.x >> 6;
@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   const int LD = B.ld;
   const int c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  load_tile<false>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
+  load_tile<Coh::None>(Sinv, dinv_of<TF>(B) + (size_t)b * NB * NB, NB, 0, 0, NB, NB);
   // L[sep, b]ᵀ x_sep: the chunk partials of k_big_sepgemv, chunks c ≡ q (mod 4) by wave q in
   // chunk order, read while the later blocks solve
   double sep = 0.0;

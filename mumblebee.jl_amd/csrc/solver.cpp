@@ -253,6 +253,9 @@ static double wall_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+static void bsm_to_csr(const BSM& A, bool transpose, std::vector<int>& ptr, std::vector<int>& col,
+                       std::vector<double>& val);
+
 void Solver::host_setup(const Problem& prob) {
   const double t0 = wall_ms();
   for (const Cone& c : prob.K)
@@ -310,7 +313,9 @@ void Solver::host_setup(const Problem& prob) {
   ng_ = norm2(P_.g);
   sg_ = scalenorm(P_.g, yscl_);
   sf_ = scalenorm(P_.f, pscl_);
-  L_ = gram(P_.B);
+  bsm_to_csr(P_.B, false, brp_, brc_, brv_);
+  bsm_to_csr(P_.B, true, bcp_, bcr_, bcv_);
+  gram_lower(n_, brp_, brc_, brv_, bcp_, bcr_, bcv_, Lp_, Li_, Lx_);
   host_ms_ = wall_ms() - t0;
 }
 
@@ -373,7 +378,7 @@ double Solver::estimate_device_bytes() const {
   buf(8.0 * wsz);
   buf(16.0 * DENSE_SPLIT * std::max(nbig_psd, 1));
   const double nnzB = (double)P_.B.val.size(), nnzQ = (double)P_.Q.val.size();
-  const double nunion = (double)L_.val.size() + nnzQ;
+  const double nunion = (double)Lx_.size() + nnzQ;
   buf(8.0 * NV_COUNT * n); buf(8.0 * MV_COUNT * m); buf(8.0 * NSCALAR); buf(24.0 * n);
   buf(8.0 * fsize); buf(8.0 * dsz); buf(8.0 * csz); buf(8.0 * qsz);
   buf(8.0 * ssum); buf(4.0 * ssum); buf(4.0 * ssum); buf(4.0 * NF);
@@ -513,34 +518,28 @@ void Solver::build_device() {
     }
     uofs[f + 1] = uofs[f] + fr_[f].s;
   }
-  // union pattern of L = BᵀB and Q (lower triangle)
-  const BSM& L = L_;
+  // union pattern of BᵀB and Q (lower triangle), column by column
+  std::vector<int> qcp, qcr;
+  std::vector<double> qcv;
+  bsm_to_csr(P_.Q, true, qcp, qcr, qcv);  // Q by columns, rows sorted
   std::vector<int64_t> upos;
   std::vector<double> ul, uq;
-  for (i64 v = 0; v < N; ++v) {
-    i64 nc = P_.B.ncols(v), c0 = P_.B.xcol[v];
-    i64 a = L.xsrc[v], ae = L.xsrc[v + 1], b = P_.Q.xsrc[v], be = P_.Q.xsrc[v + 1];
+  upos.reserve(Lx_.size() + qcv.size() / 2 + 1);
+  ul.reserve(upos.capacity());
+  uq.reserve(upos.capacity());
+  for (i64 j = 0; j < n_; ++j) {
+    i64 a = Lp_[j], ae = Lp_[j + 1];
+    i64 b = std::lower_bound(qcr.begin() + qcp[j], qcr.begin() + qcp[j + 1], (int)j) - qcr.begin();
+    const i64 be = qcp[j + 1];
     while (a < ae || b < be) {
-      i64 ua = a < ae ? L.tgt[a] : std::numeric_limits<i64>::max();
-      i64 ub = b < be ? P_.Q.tgt[b] : std::numeric_limits<i64>::max();
-      i64 u = std::min(ua, ub);
-      const double* lb = ua == u ? L.val.data() + L.xblk[a] : nullptr;
-      const double* qb = ub == u ? P_.Q.val.data() + P_.Q.xblk[b] : nullptr;
-      i64 nr = P_.B.ncols(u), r0 = P_.B.xcol[u];
-      if (r0 + nr - 1 >= c0) {
-        for (i64 jl = 0; jl < nc; ++jl)
-          for (i64 il = 0; il < nr; ++il) {
-            i64 i = r0 + il, j = c0 + jl;
-            if (i < j) continue;
-            double lv = lb ? lb[jl * nr + il] : 0.0, qv = qb ? qb[jl * nr + il] : 0.0;
-            if (lv == 0.0 && qv == 0.0) continue;
-            upos.push_back(fpos(i, j));
-            ul.push_back(lv);
-            uq.push_back(qv);
-          }
-      }
-      if (ua == u) ++a;
-      if (ub == u) ++b;
+      const i64 ia = a < ae ? Li_[a] : std::numeric_limits<i64>::max();
+      const i64 ib = b < be ? qcr[b] : std::numeric_limits<i64>::max();
+      const i64 i = std::min(ia, ib);
+      const double lv = ia == i ? Lx_[a++] : 0.0, qv = ib == i ? qcv[b++] : 0.0;
+      if (lv == 0.0 && qv == 0.0) continue;
+      upos.push_back(fpos(i, j));
+      ul.push_back(lv);
+      uq.push_back(qv);
     }
   }
   nunion_ = (i64)upos.size();
@@ -661,8 +660,9 @@ void Solver::build_device() {
   std::vector<double> x;
   int lanes[3];
   // lanes per row: the largest power of two ≤ the mean row length (1..64); tiles of 256/lanes rows
-  auto plan = [&](int which, i64 nrow) {
-    double avg = nrow > 0 ? (double)c.size() / (double)nrow : 0.0;
+  auto plan = [&](int which, i64 nrow, const std::vector<int>& ptr) {
+    const std::vector<int>& p = ptr;
+    double avg = nrow > 0 ? (double)p[nrow] / (double)nrow : 0.0;
     int G = 1;
     while (G < 64 && 2.0 * G <= avg) G *= 2;
     lanes[which] = G;
@@ -673,21 +673,19 @@ void Solver::build_device() {
     nsptile_[which] = (int)t.size();
     if (!t.empty()) d_sptiles_[which].upload(t, st_);
   };
-  bsm_to_csr(P_.B, false, p, c, x);
-  d_Bp_.upload(p, st_);
-  d_Bc_.upload(c, st_);
-  d_Bv_.upload(x, st_);
-  plan(0, m_);
-  bsm_to_csr(P_.B, true, p, c, x);
-  d_Btp_.upload(p, st_);
-  d_Btc_.upload(c, st_);
-  d_Btv_.upload(x, st_);
-  plan(1, n_);
+  d_Bp_.upload(brp_, st_);
+  d_Bc_.upload(brc_, st_);
+  d_Bv_.upload(brv_, st_);
+  plan(0, m_, brp_);
+  d_Btp_.upload(bcp_, st_);
+  d_Btc_.upload(bcr_, st_);
+  d_Btv_.upload(bcv_, st_);
+  plan(1, n_, bcp_);
   bsm_to_csr(P_.Q, false, p, c, x);
   d_Qp_.upload(p, st_);
   d_Qc_.upload(c, st_);
   d_Qv_.upload(x, st_);
-  plan(2, n_);
+  plan(2, n_, p);
   d_upos_.upload(upos, st_);
   d_ul_.upload(ul, st_);
   d_uq_.upload(uq, st_);

@@ -200,7 +200,11 @@ class Solver {
   Settings set_;
   Problem P_;          // permuted + scaled
   Symbolic sym_;
-  BSM L_;              // BᵀB of the permuted, scaled B (both triangles)
+  // B of the permuted, scaled problem as stored nonzeros by rows and by columns, and the lower
+  // triangle of BᵀB as scalar CSC
+  std::vector<int> brp_, brc_, bcp_, bcr_, Li_;
+  std::vector<double> brv_, bcv_, Lx_;
+  std::vector<i64> Lp_;
   int64_t n_ = 0, m_ = 0;
   int nu_ = 0;
   double mu_t_ = 0.0;

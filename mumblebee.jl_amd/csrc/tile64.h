@@ -16,12 +16,25 @@ constexpr int NT = 256;        // threads per workgroup of the tile kernels
 
 // Tiles move through raw buffer instructions over a resource spanning exactly the tile's
 // extent in F: a lane outside the h×w tile gets an offset past that extent, which the hardware
-// range check turns into a zero load or a dropped store. With the device-coherent cache policy
-// (sc1, the same policy as the agent-scope atomics above) the loads are ordinary loads to the
-// compiler, so a thread's 16 loads are in flight together instead of one after another.
+// range check turns into a zero load or a dropped store. The cache policy says who wrote the
+// data (Coh):
+//   None — earlier launches: ordinary cached loads;
+//   Xcd  — another workgroup of the running kernel on the same XCD: loads with sc0 miss the CU's
+//          L1 and read the XCD's L2, where every store of the XCD lands (the L1 writes through);
+//   Dev  — a workgroup of the running kernel anywhere on the device: loads and stores with sc1
+//          (device scope, the policy of the agent-scope atomics) go past the XCD's L2.
+// Either way the loads are ordinary loads to the compiler, so a thread's 16 loads are in flight
+// together instead of one after another.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-constexpr int CPOL_SC1 = 16;           // buffer cache policy bit: coherent at device scope
+constexpr int CPOL_SC0 = 1;            // buffer cache policy bits
+constexpr int CPOL_SC1 = 16;
 constexpr unsigned OFF_OUT = 0x80000000u;  // an offset past every tile extent
+
+enum class Coh { None, Xcd, Dev };
+template <Coh C>
+constexpr int load_pol() { return C == Coh::Dev ? CPOL_SC1 : (C == Coh::Xcd ? CPOL_SC0 : 0); }
+template <Coh C>
+constexpr int store_pol() { return C == Coh::Dev ? CPOL_SC1 : 0; }
 
 template <class T>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const T* F, int LD, int r0, int c0, int h, int w) {
@@ -30,19 +43,21 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const T* F, int LD, 
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), 0, (int)bytes, 0x00020000);
 }
 
-template <bool COH>
+template <Coh C>
 __device__ __forceinline__ void buf_load(double& v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
-  v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+  v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, load_pol<C>()));
 }
-template <bool COH>
+template <Coh C>
 __device__ __forceinline__ void buf_load(float& v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
-  v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, COH ? CPOL_SC1 : 0));
+  v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, load_pol<C>()));
 }
+template <Coh C>
 __device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, (int)off, 0, CPOL_SC1);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, (int)off, 0, store_pol<C>());
 }
+template <Coh C>
 __device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t rs, unsigned off) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)off, 0, CPOL_SC1);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)off, 0, store_pol<C>());
 }
 
 // A thread's share of a tile in flight: element t = threadIdx.x + u·NT is (t % NB, t / NB).
@@ -51,9 +66,9 @@ struct TileRegs {
   T v[NB * NB / NT];
 };
 
-// issue the loads of the h×w tile at (r0, c0) of F (leading dimension LD), zero padded.
-// COH: coherent loads (the tile was written by another workgroup of the running kernel).
-template <bool COH, class T>
+// issue the loads of the h×w tile at (r0, c0) of F (leading dimension LD), zero padded, with the
+// cache policy of the tile's writer
+template <Coh COH, class T>
 __device__ __forceinline__ void fetch_tile(TileRegs<T>& R, const T* F, int LD, int r0, int c0, int h, int w) {
   const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
 #pragma unroll
@@ -75,16 +90,16 @@ __device__ __forceinline__ void put_tile(TD* S, const TileRegs<TS>& R) {
   }
 }
 
-template <bool COH, class TD, class TS>
+template <Coh COH, class TD, class TS>
 __device__ __forceinline__ void load_tile(TD* S, const TS* F, int LD, int r0, int c0, int h, int w) {
   TileRegs<TS> R;
   fetch_tile<COH>(R, F, LD, r0, c0, h, w);
   put_tile(S, R);
 }
 
-// LDS column-major tile → h×w tile at (r0, c0) of F with coherent stores (lower part only if
-// lower), converted to F's element type
-template <class TD, class TS>
+// LDS column-major tile → h×w tile at (r0, c0) of F (lower part only if lower), converted to F's
+// element type, with the cache policy its readers in the running kernel need
+template <Coh COH, class TD, class TS>
 __device__ __forceinline__ void store_tile(TD* F, int LD, int r0, int c0, int h, int w, const TS* S, bool lower) {
   const __amdgpu_buffer_rsrc_t rs = tile_rsrc(F, LD, r0, c0, h, w);
   TD v[NB * NB / NT];
@@ -98,7 +113,7 @@ __device__ __forceinline__ void store_tile(TD* F, int LD, int r0, int c0, int h,
     int t = threadIdx.x + u * NT;
     int i = t % NB, j = t / NB;
     unsigned off = i < h && j < w && (!lower || i >= j) ? (unsigned)((i + (size_t)j * LD) * sizeof(TD)) : OFF_OUT;
-    buf_store(v[u], rs, off);
+    buf_store<COH>(v[u], rs, off);
   }
 }
 

@@ -63,6 +63,10 @@ def test_served_stream_matches_solo_and_oracle():
         assert times["submitted"] <= times["admitted"] <= times["finished"]
         assert 0 < times["device_bytes"] <= times["reserved_bytes"]
         solo = mh.solve(probs[k])
+        if res.status not in GOOD or solo.status not in GOOD:
+            print(f"request {k} {d['name']} B {d['B'].shape}: served {res.status.name} in {res.niter} iterations "
+                  f"(mu {res.mu:.3g}, pres {res.pres:.3g}, dres {res.dres:.3g}), alone {solo.status.name} in "
+                  f"{solo.niter}", flush=True)
         assert _agree(res, solo), (k, d["name"], res.status, solo.status, res.pobj, solo.pobj)
         if d["B"].shape[1] <= 300:
             ores = O.solve(oracle_problem(d))
