@@ -319,6 +319,10 @@ def bench_resident(args, mh):
             "p99_ms": float(np.percentile(lat_ms, 99)),
             "init_host_ms_mean": float(np.mean([x["init_host_ms"] for x in plain["stats"]])),
             "init_device_ms_mean": float(np.mean([x["init_device_ms"] for x in plain["stats"]])),
+            "init_device_parts_ms_mean": {k: float(np.mean([x["init_device_parts_ms"][k] for x in plain["stats"]]))
+                                          for k in ("stream", "buffers", "schedule", "start")},
+            "init_buffer_marks_ms_mean": {k: float(np.mean([x["build_marks_ms"][k] for x in plain["stats"]]))
+                                          for k in ("fronts", "union", "cones", "allocs", "sparse")},
             "iterations_mean": float(np.mean([x.niter for x in res])),
             "factor_tflops_aggregate": fac["flops"] / elapsed / 1e12,
             "factor_ms_per_call": fac["ms"] / max(1, fac["launches"]),

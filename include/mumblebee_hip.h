@@ -139,10 +139,13 @@ int64_t mb_solver_niter(const mb_solver* s);
  * delta mu chi (history/ipm.jl:1-24) */
 int mb_solver_history(const mb_solver* s, double* rows);
 
-/* stats[0..9] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
+/* stats[0..18] = exact factor flops, front flops (after amalgamation), nfront, nlevel,
  * total factorization ms, factorization calls, last factorization ms, factorizations whose
  * large fronts were FP32 (precision 1), init ms on the host (symbolic analysis, equilibration,
- * BᵀB), init ms of the device setup (buffers, uploads, starting point) */
+ * BᵀB), init ms of the device setup (buffers, uploads, starting point), and that device setup's
+ * parts: stream and events, buffers and uploads, launch schedule, starting point, and ms from the
+ * start of the buffers part to the end of its phases: front layout, assembly map, cone tables,
+ * allocations, sparse matrices */
 int mb_solver_stats(const mb_solver* s, double* stats);
 
 /* reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429): new starting point, empty history;

@@ -364,6 +364,8 @@ int mb_solver_stats(const mb_solver* s, double* st) {
     st[7] = s->S->factor_calls_fp32();
     st[8] = s->S->host_setup_ms();
     st[9] = s->S->device_setup_ms();
+    for (int k = 0; k < 4; ++k) st[10 + k] = s->S->device_setup_part(k);
+    for (int k = 0; k < 5; ++k) st[14 + k] = s->S->build_device_mark(k);
     return 0;
   });
 }

@@ -451,15 +451,23 @@ BSM selectvtxs(const BSM& A, const std::vector<i64>& sel) {
   R.xcol.push_back(0);
   R.xsrc.push_back(0);
   R.xblk.push_back(0);
+  R.xcol.reserve(sel.size() + 1);
+  R.xsrc.reserve(sel.size() + 1);
+  R.tgt.reserve(A.tgt.size());
+  R.xblk.reserve(A.xblk.size());
+  R.val.resize(A.val.size());
+  i64 nv = 0;
   for (i64 av : sel) {
     R.xcol.push_back(R.xcol.back() + A.ncols(av));
     for (i64 e = A.xsrc[av]; e < A.xsrc[av + 1]; ++e) {
       R.tgt.push_back(A.tgt[e]);
-      R.val.insert(R.val.end(), A.val.begin() + A.xblk[e], A.val.begin() + A.xblk[e + 1]);
-      R.xblk.push_back((i64)R.val.size());
+      std::copy(A.val.begin() + A.xblk[e], A.val.begin() + A.xblk[e + 1], R.val.begin() + nv);
+      nv += A.xblk[e + 1] - A.xblk[e];
+      R.xblk.push_back(nv);
     }
     R.xsrc.push_back((i64)R.tgt.size());
   }
+  R.val.resize(nv);
   return R;
 }
 
@@ -475,6 +483,11 @@ BSM symselectvtxs(const BSM& A, const std::vector<i64>& sel) {
   R.xrow = R.xcol;
   R.xsrc.push_back(0);
   R.xblk.push_back(0);
+  R.xsrc.reserve(N + 1);
+  R.tgt.reserve(A.tgt.size());
+  R.xblk.reserve(A.xblk.size());
+  R.val.resize(A.val.size());
+  i64 nv = 0;
   std::vector<std::pair<i64, i64>> items;
   for (i64 av : sel) {
     items.clear();
@@ -483,11 +496,13 @@ BSM symselectvtxs(const BSM& A, const std::vector<i64>& sel) {
     for (auto& it : items) {
       i64 e = it.second;
       R.tgt.push_back(it.first);
-      R.val.insert(R.val.end(), A.val.begin() + A.xblk[e], A.val.begin() + A.xblk[e + 1]);
-      R.xblk.push_back((i64)R.val.size());
+      std::copy(A.val.begin() + A.xblk[e], A.val.begin() + A.xblk[e + 1], R.val.begin() + nv);
+      nv += A.xblk[e + 1] - A.xblk[e];
+      R.xblk.push_back(nv);
     }
     R.xsrc.push_back((i64)R.tgt.size());
   }
+  R.val.resize(nv);
   return R;
 }
 

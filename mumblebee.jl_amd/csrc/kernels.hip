@@ -963,8 +963,8 @@ __global__ void k_extend_add(const FrontDesc* fronts, const int* items, const in
     int j = tj * NB + jj;
     if (j > i || j >= s) continue;
     int pj = C.relidx[j];
-    double v = U[i + (size_t)j * C.ld];
-    atomicAdd(&P.F[pi + (size_t)pj * P.ld], v);
+    // one child per parent in a launch: the entry is this thread's alone
+    P.F[pi + (size_t)pj * P.ld] += U[i + (size_t)j * C.ld];
   }
 }
 

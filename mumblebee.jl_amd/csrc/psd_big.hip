@@ -383,6 +383,7 @@ __global__ void __launch_bounds__(NT) k_bp_trsm_lt(const InstDev* insts, const C
 // clears aux[AX_OK]. One workgroup per cone; slot: LDS round-robin table of d+1 entries.
 // ------------------------------------------------------------------------------------------
 constexpr int BP_DMAX_JACOBI = 4096;
+constexpr int BP_JACOBI_LDS = 128;  // orders up to this run the LDS kernel
 
 __global__ void __launch_bounds__(NT) k_bp_jacobi(const InstDev* insts, const ConeDesc* cones, const int* list) {
   __shared__ int slot[BP_DMAX_JACOBI + 2];
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(NT) k_bp_jacobi(const InstDev* insts, const Co
   const InstDev& I = insts[C.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
   const BV v = bview(C);
-  if (v.aux[AX_OK] == 0.0) return;
+  if (v.d <= BP_JACOBI_LDS || v.aux[AX_OK] == 0.0) return;  // k_bp_jacobi_lds
   const int d = v.d, dd = d + (d & 1), npair = dd / 2;
   double* A = v.X2;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -455,6 +456,87 @@ __global__ void __launch_bounds__(NT) k_bp_jacobi(const InstDev* insts, const Co
   for (int t = threadIdx.x; t < d * d; t += NT) {
     const double sj = v.s[t / d];
     if (sj > 0.0) A[t] /= sj;
+  }
+  if (threadIdx.x == 0 && !(conv && zero == 0)) v.aux[AX_OK] = 0.0;
+}
+
+// The same one-sided Jacobi for cones of order ≤ BP_JACOBI_LDS with the matrix in LDS (128 KiB at
+// order 128) and sixteen waves: the column pairs, their rotations and the norms are the HBM
+// kernel's, in the same order, so the results are the same bit for bit.
+constexpr int NTJ = 1024;
+__global__ void __launch_bounds__(NTJ) k_bp_jacobi_lds(const InstDev* insts, const ConeDesc* cones, const int* list) {
+  __shared__ double S[BP_JACOBI_LDS * BP_JACOBI_LDS];
+  __shared__ int slot[BP_JACOBI_LDS + 2];
+  __shared__ int rotated, zero;
+  const ConeDesc C = cones[list[blockIdx.x]];
+  const InstDev& I = insts[C.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const BV v = bview(C);
+  if (v.d > BP_JACOBI_LDS || v.aux[AX_OK] == 0.0) return;
+  const int d = v.d, dd = d + (d & 1), npair = dd / 2;
+  double* A = v.X2;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t = threadIdx.x; t < d * d; t += NTJ) S[t] = A[t];
+  for (int t = threadIdx.x; t < dd; t += NTJ) slot[t] = t;  // index d (odd d) is a blank
+  __syncthreads();
+  bool conv = false;
+  for (int sweep = 0; sweep < 30 && !conv; ++sweep) {
+    if (threadIdx.x == 0) rotated = 0;
+    __syncthreads();
+    for (int round = 0; round < dd - 1; ++round) {
+      for (int q = w; q < npair; q += NTJ / 64) {
+        const int a = slot[q], b = slot[dd - 1 - q];
+        if (a >= d || b >= d) continue;
+        double* xa = S + a * d;
+        double* xb = S + b * d;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = lane; i < d; i += 64) {
+          const double x = xa[i], y = xb[i];
+          al = fma(x, x, al);
+          be = fma(y, y, be);
+          ga = fma(x, y, ga);
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        ga = wave_sum(ga);
+        if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double tn = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + tn * tn), sn = c * tn;
+          for (int i = lane; i < d; i += 64) {
+            const double x = xa[i], y = xb[i];
+            xa[i] = c * x - sn * y;
+            xb[i] = sn * x + c * y;
+          }
+          if (lane == 0) rotated = 1;
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {  // rotate every slot but the first
+        const int last = slot[dd - 1];
+        for (int k = dd - 1; k > 1; --k) slot[k] = slot[k - 1];
+        slot[1] = last;
+      }
+      __syncthreads();
+    }
+    conv = rotated == 0;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) zero = 0;
+  __syncthreads();
+  for (int j = w; j < d; j += NTJ / 64) {  // s_j = ‖column j‖, one wave per column
+    double nr = 0.0;
+    for (int i = lane; i < d; i += 64) nr = fma(S[i + j * d], S[i + j * d], nr);
+    nr = sqrt(wave_sum(nr));
+    if (lane == 0) {
+      v.s[j] = nr;
+      if (!(nr > 0.0)) zero = 1;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < d * d; t += NTJ) {
+    const double sj = v.s[t / d];
+    A[t] = sj > 0.0 ? S[t] / sj : S[t];
   }
   if (threadIdx.x == 0 && !(conv && zero == 0)) v.aux[AX_OK] = 0.0;
 }
@@ -729,7 +811,8 @@ void launch_bpsd_scale(hipStream_t st, const InstDev* I, const ConeDesc* C, cons
   k_bp_flag_and<<<B.n, 64, 0, st>>>(I, C, B.list, AX_CHOL1);
   k_bp_chol<<<B.n, NT, 0, st>>>(I, C, B.list, BP_X0, AX_CHOL0, 1);  // LP, inverse blocks for the solve
   gemm(st, I, C, B, BP_X0, 1, BP_X1, 0, BP_X2);                       // LPᵀ LD
-  k_bp_jacobi<<<B.n, NT, 0, st>>>(I, C, B.list);                      // X2 = U, s = Σ
+  k_bp_jacobi_lds<<<B.n, NTJ, 0, st>>>(I, C, B.list);                // X2 = U, s = Σ (d ≤ 128)
+  if (B.dmax > BP_JACOBI_LDS) k_bp_jacobi<<<B.n, NT, 0, st>>>(I, C, B.list);  // the larger cones
   gemm(st, I, C, B, BP_X0, 0, BP_X2, 0, BP_S);                        // S = LP U
   k_bp_trsm_lt<<<B.n, NT, 0, st>>>(I, C, B.list, BP_X0, BP_X2, BP_R, AX_OK);  // R = LP⁻ᵀ U
   ew(st, I, C, B, 0, 0);                                              // X1 = R Σ^½

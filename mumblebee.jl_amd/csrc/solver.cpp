@@ -29,6 +29,49 @@ static const double FORCING_FRAC = 0.1;  // solver/solver.jl:3
 // device bytes allocated through DevBuf by the solver being built on this thread
 static thread_local size_t* g_dev_counter = nullptr;
 
+// Uploads of the solver being set up on this thread: each is copied once into a pinned host chunk
+// and its asynchronous copy queued at once; the setup waits for them all at its end (an upload
+// from pageable memory waits for its staging copy, and a wait for the stream per buffer costs a
+// host-device round trip: ~50 buffers per solver, with other solvers' kernels on the device).
+namespace {
+void* pinned_alloc(size_t c);
+void pinned_free(void* p, size_t c);
+size_t size_class(size_t b);
+struct Stager {
+  static constexpr size_t CHUNK = size_t(4) << 20;
+  hipStream_t st = nullptr;
+  std::vector<std::pair<char*, size_t>> chunks;  // pinned block, its size class
+  size_t used = 0;                                // bytes used in the last chunk
+  void add(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    if (chunks.empty() || used + bytes > chunks.back().second) {
+      const size_t c = size_class(std::max(CHUNK, bytes));
+      chunks.emplace_back(static_cast<char*>(pinned_alloc(c)), c);
+      used = 0;
+    }
+    char* h = chunks.back().first + used;
+    std::memcpy(h, src, bytes);
+    used += (bytes + 255) / 256 * 256;
+    HIPCHK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, st));
+  }
+  // wait for the copies, then the chunks go back to the cache
+  void finish() {
+    hipError_t e = chunks.empty() ? hipSuccess : hipStreamSynchronize(st);
+    for (auto& c : chunks) pinned_free(c.first, c.second);
+    chunks.clear();
+    used = 0;
+    HIPCHK(e);
+  }
+  ~Stager() {
+    if (!chunks.empty()) {
+      (void)hipStreamSynchronize(st);
+      for (auto& c : chunks) pinned_free(c.first, c.second);
+    }
+  }
+};
+thread_local Stager* g_stager = nullptr;
+}  // namespace
+
 // Device and pinned-host memory cache. hipFree (and hipHostFree) waits for every kernel on the
 // device, whichever stream it runs on: a hipFree issued while a 50 ms kernel ran on another
 // stream returned when that kernel ended. A solver that freed its buffers while other solvers
@@ -139,11 +182,41 @@ void DevBuf::alloc(size_t b) {
 // device has finished them) is not ordered before the solver's kernels.
 template <class T>
 void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
+  if (g_stager) {  // during device setup: the copy is queued, the setup waits for all at its end
+    if (p) HIPCHK(hipStreamSynchronize(st));  // a copy into the old buffer may still be queued
+    alloc(std::max<size_t>(v.size() * sizeof(T), sizeof(T)));
+    g_stager->add(p, v.data(), v.size() * sizeof(T));
+    return;
+  }
   HIPCHK(hipStreamSynchronize(st));  // the old buffer is no longer in use
   alloc(std::max<size_t>(v.size() * sizeof(T), sizeof(T)));
   if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));  // synchronous: callers pass temporaries
 }
+
+// pinned host blocks for the setup uploads, kept in size classes like the device blocks
+namespace {
+std::mutex g_pin_mu;
+std::map<size_t, std::vector<void*>> g_pinned;
+void* pinned_alloc(size_t c) {
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    auto it = g_pinned.find(c);
+    if (it != g_pinned.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      return p;
+    }
+  }
+  void* p = nullptr;
+  HIPCHK(hipHostMalloc(&p, c, hipHostMallocDefault));
+  return p;
+}
+void pinned_free(void* p, size_t c) {
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  g_pinned[c].push_back(p);
+}
+}  // namespace
 
 // ------------------------------------------------------------------------------------------
 // Ruiz equilibration (scaling.jl:52-184)
@@ -158,8 +231,8 @@ void equilibrate(BSM& B, BSM& Q, std::vector<double>& f, std::vector<double>& g,
     std::fill(rnrm.begin(), rnrm.end(), 0.0);
     for (i64 v = 0; v < nvtx; ++v) {
       double nv = 0.0;
-      for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e)
-        for (i64 b = Q.xblk[e]; b < Q.xblk[e + 1]; ++b) nv = std::max(nv, std::fabs(Q.val[b]));
+      // the blocks of a vertex's arcs lie one after another in val
+      for (i64 b = Q.xblk[Q.xsrc[v]], be = Q.xblk[Q.xsrc[v + 1]]; b < be; ++b) nv = std::max(nv, std::fabs(Q.val[b]));
       i64 nc = B.ncols(v);
       for (i64 e = B.xsrc[v]; e < B.xsrc[v + 1]; ++e) {
         i64 u = B.tgt[e], nr = B.nrows(u), r0 = B.xrow[u];
@@ -198,8 +271,13 @@ void equilibrate(BSM& B, BSM& Q, std::vector<double>& f, std::vector<double>& g,
           for (i64 il = 0; il < nr; ++il) B.val[B.xblk[e] + jl * nr + il] *= rswp[r0 + il] * sv;
       }
       for (i64 e = Q.xsrc[v]; e < Q.xsrc[v + 1]; ++e) {
-        double s = vswp[Q.tgt[e]] * sv;
-        for (i64 b = Q.xblk[e]; b < Q.xblk[e + 1]; ++b) Q.val[b] *= s;
+        const double s = vswp[Q.tgt[e]] * sv;
+        const i64 b0 = Q.xblk[e], b1 = Q.xblk[e + 1];
+        if (b1 == b0 + 1) {
+          Q.val[b0] *= s;
+          continue;
+        }
+        for (i64 b = b0; b < b1; ++b) Q.val[b] *= s;
       }
     }
     if (d < tol) break;
@@ -319,6 +397,19 @@ void Solver::host_setup(const Problem& prob) {
   host_ms_ = wall_ms() - t0;
 }
 
+// The arc of Q's diagonal block of vertex v, or −1 where there is none or it holds only zeros (the
+// block a PSD vertex always has in Q's layout, which for a large cone would be an n×n block of
+// zeros on the device): the cone kernels read no block as zero.
+i64 Solver::qdiag_arc(i64 v) const {
+  for (i64 e = P_.Q.xsrc[v]; e < P_.Q.xsrc[v + 1]; ++e)
+    if (P_.Q.tgt[e] == v) {
+      for (i64 t = P_.Q.xblk[e]; t < P_.Q.xblk[e + 1]; ++t)
+        if (P_.Q.val[t] != 0.0) return e;
+      return -1;
+    }
+  return -1;
+}
+
 // An upper bound of what device_setup allocates: every buffer of build_device and
 // build_schedule, from the symbolic structure alone, each rounded up to the size class the
 // allocator hands out (the assembly map is bounded by the stored entries of BᵀB and Q).
@@ -373,7 +464,7 @@ double Solver::estimate_device_bytes() const {
       wsz += (double)psd_big_ws((long long)d);
       ++nbig_psd;
     }
-    qsz += c * c;  // a Q diagonal block, where there is one
+    if (qdiag_arc(v) >= 0) qsz += c * c;  // a Q diagonal block, where there is one
   }
   buf(8.0 * wsz);
   buf(16.0 * DENSE_SPLIT * std::max(nbig_psd, 1));
@@ -400,24 +491,37 @@ double Solver::estimate_device_bytes() const {
 void Solver::device_setup(const double* p0, const double* d0, const double* y0) {
   const double t0 = wall_ms();
   g_dev_counter = &dev_bytes_;
+  Stager stage;
   try {
     HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     hsc_ = scal_alloc();
     HIPCHK(hipEventCreate(&ev0_));
     HIPCHK(hipEventCreate(&ev1_));
+    dev_parts_[0] = wall_ms() - t0;
+    stage.st = st_;
+    g_stager = &stage;
     build_device();
+    dev_parts_[1] = wall_ms() - t0 - dev_parts_[0];
     build_schedule();
+    g_stager = nullptr;
+    stage.finish();
+    dev_parts_[2] = wall_ms() - t0 - dev_parts_[0] - dev_parts_[1];
   } catch (...) {
+    g_stager = nullptr;
     g_dev_counter = nullptr;
     throw;
   }
   g_dev_counter = nullptr;
   reinit(p0, d0, y0);
   device_ms_ = wall_ms() - t0;
+  dev_parts_[3] = device_ms_ - dev_parts_[0] - dev_parts_[1] - dev_parts_[2];
 }
 
 Solver::~Solver() {
   if (st_) (void)hipStreamSynchronize(st_);
+  for (hipStream_t a : aux_) (void)hipStreamSynchronize(a);
+  for (hipEvent_t e : auxev_) (void)hipEventDestroy(e);
+  for (hipStream_t a : aux_) (void)hipStreamDestroy(a);
   if (hsc_) scal_free(hsc_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -427,37 +531,57 @@ Solver::~Solver() {
 
 static void bsm_to_csr(const BSM& A, bool transpose, std::vector<int>& ptr, std::vector<int>& col,
                        std::vector<double>& val) {
-  // entries (i, j, x) of A; CSR over rows i (or over columns j if transpose)
-  i64 nr = transpose ? A.ncol() : A.nrow();
-  std::vector<i64> cnt(nr + 1, 0);
+  // entries (i, j, x) of A; CSR over rows i (or over columns j if transpose). Stored zeros are
+  // dropped. A is visited column by column (each column's arcs in target order, rows in order),
+  // which is CSR order over columns, and leaves the columns of each CSR row sorted.
+  const i64 nr = transpose ? A.ncol() : A.nrow();
+  std::vector<int> cnt(nr + 1, 0);
   auto visit = [&](auto&& fn) {
     for (i64 v = 0; v < A.nvtx; ++v) {
-      i64 nc = A.ncols(v);
-      for (i64 e = A.xsrc[v]; e < A.xsrc[v + 1]; ++e) {
-        i64 u = A.tgt[e], nrw = A.nrows(u);
-        for (i64 jl = 0; jl < nc; ++jl)
-          for (i64 il = 0; il < nrw; ++il) {
-            double x = A.val[A.xblk[e] + jl * nrw + il];
-            if (x != 0.0) fn(A.xrow[u] + il, A.xcol[v] + jl, x);
-          }
+      const i64 nc = A.ncols(v), c0 = A.xcol[v];
+      if (nc == 1) {  // one column: scalar blocks of nonnegative and free vertices, among others
+        for (i64 e = A.xsrc[v]; e < A.xsrc[v + 1]; ++e) {
+          const i64 u = A.tgt[e], nrw = A.nrows(u), r0 = A.xrow[u];
+          const double* blk = A.val.data() + A.xblk[e];
+          for (i64 il = 0; il < nrw; ++il)
+            if (blk[il] != 0.0) fn(r0 + il, c0, blk[il]);
+        }
+        continue;
       }
+      for (i64 jl = 0; jl < nc; ++jl)
+        for (i64 e = A.xsrc[v]; e < A.xsrc[v + 1]; ++e) {
+          const i64 u = A.tgt[e], nrw = A.nrows(u), r0 = A.xrow[u];
+          const double* blk = A.val.data() + A.xblk[e] + jl * nrw;
+          for (i64 il = 0; il < nrw; ++il)
+            if (blk[il] != 0.0) fn(r0 + il, c0 + jl, blk[il]);
+        }
     }
   };
-  visit([&](i64 i, i64 j, double) { cnt[(transpose ? j : i) + 1]++; });
+  if (transpose) visit([&](i64, i64 j, double) { cnt[j + 1]++; });
+  else visit([&](i64 i, i64, double) { cnt[i + 1]++; });
   for (i64 r = 0; r < nr; ++r) cnt[r + 1] += cnt[r];
   ptr.assign(cnt.begin(), cnt.end());
   col.resize(cnt[nr]);
   val.resize(cnt[nr]);
-  std::vector<i64> nxt(cnt.begin(), cnt.end() - 1);
-  visit([&](i64 i, i64 j, double x) {
-    i64 r = transpose ? j : i, c = transpose ? i : j;
-    i64 q = nxt[r]++;
-    col[q] = (int)c;
-    val[q] = x;
-  });
+  if (transpose) {
+    // column order is CSR order: one pass, no scatter
+    i64 q = 0;
+    visit([&](i64 i, i64, double x) {
+      col[q] = (int)i;
+      val[q] = x;
+      ++q;
+    });
+  } else {
+    visit([&](i64 i, i64 j, double x) {
+      const int q = cnt[i]++;
+      col[q] = (int)j;
+      val[q] = x;
+    });
+  }
 }
 
 void Solver::build_device() {
+  const double tb0 = wall_ms();
   const i64 N = P_.B.nvtx;
   const i64 NF = sym_.nfront();
   // fronts
@@ -518,6 +642,7 @@ void Solver::build_device() {
     }
     uofs[f + 1] = uofs[f] + fr_[f].s;
   }
+  bd_parts_[0] = wall_ms() - tb0;
   // union pattern of BᵀB and Q (lower triangle), column by column
   std::vector<int> qcp, qcr;
   std::vector<double> qcv;
@@ -543,6 +668,7 @@ void Solver::build_device() {
     }
   }
   nunion_ = (i64)upos.size();
+  bd_parts_[1] = wall_ms() - tb0;
   // ‖Q‖² off the diagonal blocks
   qoff2_ = 0.0;
   for (i64 v = 0; v < N; ++v)
@@ -566,9 +692,7 @@ void Solver::build_device() {
   }
   // the diagonal blocks of Q, for the cones that have one (‖H‖ reads them; the others read zero)
   std::vector<i64> qarc(N, -1);
-  for (i64 v = 0; v < N; ++v)
-    for (i64 e = P_.Q.xsrc[v]; e < P_.Q.xsrc[v + 1]; ++e)
-      if (P_.Q.tgt[e] == v) qarc[v] = e;
+  for (i64 v = 0; v < N; ++v) qarc[v] = qdiag_arc(v);
   for (i64 v = 0; v < N; ++v) {
     const i64 nc = P_.B.ncols(v);
     qofs[v + 1] = qofs[v] + (qarc[v] >= 0 ? nc * nc : 0);
@@ -581,6 +705,7 @@ void Solver::build_device() {
   cone_front_.assign(N, 0);
   for (i64 f = 0; f < NF; ++f)
     for (i64 v = sym_.front_vtx[f]; v < sym_.front_vtx[f + 1]; ++v) cone_front_[v] = (int)f;
+  bd_parts_[2] = wall_ms() - tb0;
   // device allocations
   d_nv_.alloc(sizeof(double) * NV_COUNT * std::max<i64>(n_, 1));
   d_mv_.alloc(sizeof(double) * MV_COUNT * std::max<i64>(m_, 1));
@@ -655,6 +780,7 @@ void Solver::build_device() {
   }
   d_cone_front_.upload(cone_front_, st_);
   d_cone_ofs_.upload(std::vector<int>{0, (int)N}, st_);
+  bd_parts_[3] = wall_ms() - tb0;
   // sparse matrices
   std::vector<int> p, c;
   std::vector<double> x;
@@ -689,6 +815,7 @@ void Solver::build_device() {
   d_upos_.upload(upos, st_);
   d_ul_.upload(ul, st_);
   d_uq_.upload(uq, st_);
+  bd_parts_[4] = wall_ms() - tb0;
   // tiles
   std::vector<Tile> tn, tm;
   for (i64 s = 0; s < n_; s += 256) tn.push_back(Tile{0, (int)s});
@@ -716,12 +843,11 @@ void Solver::build_device() {
   idev_.m = (int)m_;
   d_inst_.upload(std::vector<InstDev>{idev_}, st_);
   // constant vectors: f, g, pscl, yscl
-  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_F * n_, P_.f.data(), n_ * 8, hipMemcpyHostToDevice, st_));
-  HIPCHK(hipMemcpyAsync(idev_.nv + (size_t)NV_PSCL * n_, pscl_.data(), n_ * 8, hipMemcpyHostToDevice, st_));
-  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_G * m_, P_.g.data(), m_ * 8, hipMemcpyHostToDevice, st_));
-  HIPCHK(hipMemcpyAsync(idev_.mv + (size_t)MV_YSCL * m_, yscl_.data(), m_ * 8, hipMemcpyHostToDevice, st_));
+  g_stager->add(idev_.nv + (size_t)NV_F * n_, P_.f.data(), n_ * 8);
+  g_stager->add(idev_.nv + (size_t)NV_PSCL * n_, pscl_.data(), n_ * 8);
+  g_stager->add(idev_.mv + (size_t)MV_G * m_, P_.g.data(), m_ * 8);
+  g_stager->add(idev_.mv + (size_t)MV_YSCL * m_, yscl_.data(), m_ * 8);
   set_scalar(S_ACTIVE, 1.0);
-  HIPCHK(hipStreamSynchronize(st_));
 }
 
 void Solver::build_schedule() {
@@ -841,19 +967,28 @@ void Solver::build_schedule() {
     L.items_off = (int)solve_items.size();
     L.nsmall = (int)small.size();
     solve_items.insert(solve_items.end(), small.begin(), small.end());
-    // extend-add items: children of this level's fronts
-    L.ea_off = (int)ea_items.size();
-    int tiles = 0;
-    for (int f : L.fronts)
-      for (int c : kids[f])
-        if (fr_[c].s > 0) {
+    // extend-add items: children of this level's fronts, in rounds: round k adds the k-th child
+    // (with an update matrix) of every front, so no parent entry takes two additions in one launch
+    // and each sums its children in a fixed order
+    for (int k = 0;; ++k) {
+      const int off = (int)ea_items.size();
+      int tiles = 0;
+      for (int f : L.fronts) {
+        int rank = 0;
+        for (int c : kids[f]) {
+          if (fr_[c].s <= 0) continue;
+          if (rank++ != k) continue;
           int T = (fr_[c].s + NB - 1) / NB;
           ea_items.push_back(c);
           ea_tile0.push_back(tiles);
           tiles += T * (T + 1) / 2;
         }
-    L.ea_n = (int)ea_items.size() - L.ea_off;
-    L.ea_tiles = tiles;
+      }
+      if (tiles == 0) break;
+      L.ea_off.push_back(off);
+      L.ea_n.push_back((int)ea_items.size() - off);
+      L.ea_tiles.push_back(tiles);
+    }
     for (int f : L.fronts) {
       double r = fr_[f].r, s = fr_[f].s;
       L.solve_bytes += (r * (r + 1) / 2 + s * r) * 8.0;
@@ -896,6 +1031,35 @@ void Solver::build_schedule() {
   d_ea_tile0_.upload(ea_tile0, st_);
   d_solve_items_.upload(solve_items, st_);
   HIPCHK(hipStreamSynchronize(st_));
+  // the large fronts of one level are independent: up to NAUX at a time, each on a stream of its
+  // own with a share of the CUs (a chordal clique tree has levels of tens of them)
+  size_t maxbig = 0;
+  for (const Level& L : levels_) maxbig = std::max(maxbig, L.big.size());
+  naux_ = maxbig > 1 ? (int)std::min<size_t>(NAUX, maxbig) : 0;
+  for (int k = 0; k < naux_; ++k) {
+    hipStream_t a = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+    aux_.push_back(a);
+  }
+  for (int k = 0; k <= naux_ && naux_ > 0; ++k) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    auxev_.push_back(e);
+  }
+}
+
+// streams 0..k−1 of the auxiliary set start after the work queued on the solver's stream so far
+void Solver::fork_aux(int k) {
+  HIPCHK(hipEventRecord(auxev_[0], st_));
+  for (int i = 0; i < k; ++i) HIPCHK(hipStreamWaitEvent(aux_[i], auxev_[0], 0));
+}
+
+// the solver's stream continues after everything queued on streams 0..k−1
+void Solver::join_aux(int k) {
+  for (int i = 0; i < k; ++i) {
+    HIPCHK(hipEventRecord(auxev_[1 + i], aux_[i]));
+    HIPCHK(hipStreamWaitEvent(st_, auxev_[1 + i], 0));
+  }
 }
 
 // position of (i, j), i ≥ j, in the frontal storage; −1 outside the factor's structure
@@ -1070,12 +1234,15 @@ void Solver::lsolve(int kv) {
     int ps = prof_begin(PROF_FSOLVE);
     launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, L.nsmall,
                   d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv);
-    for (int b : L.big) {
-      BigHost& bh = *big_[b];
+    const int k = L.big.size() > 1 ? std::min<int>(naux_, (int)L.big.size()) : 0;
+    if (k) fork_aux(k);
+    for (size_t i = 0; i < L.big.size(); ++i) {
+      BigHost& bh = *big_[L.big[i]];
       ++epoch_;
-      bh.sbase += launch_big_fsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_childlist_.as<int>(),
-                                    d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
+      bh.sbase += launch_big_fsolve(k ? aux_[i % k] : st_, bh.B, d_fr_.as<FrontDesc>(), bh.front,
+                                    d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
     }
+    if (k) join_aux(k);
     prof_end(ps, PROF_FSOLVE, 0.0, L.solve_bytes);
   }
 }
@@ -1086,12 +1253,15 @@ void Solver::usolve(int kv) {
     int ps = prof_begin(PROF_BSOLVE);
     launch_bsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, L.nsmall,
                   d_inst_.as<InstDev>(), kv);
-    for (int b : L.big) {
-      BigHost& bh = *big_[b];
+    const int k = L.big.size() > 1 ? std::min<int>(naux_, (int)L.big.size()) : 0;
+    if (k) fork_aux(k);
+    for (size_t i = 0; i < L.big.size(); ++i) {
+      BigHost& bh = *big_[L.big[i]];
       ++epoch_;
-      bh.sbase += launch_big_bsolve(st_, bh.B, d_fr_.as<FrontDesc>(), bh.front, d_inst_.as<InstDev>(), kv, epoch_,
-                                    bh.sbase, fac_fp32_);
+      bh.sbase += launch_big_bsolve(k ? aux_[i % k] : st_, bh.B, d_fr_.as<FrontDesc>(), bh.front,
+                                    d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
     }
+    if (k) join_aux(k);
     prof_end(ps, PROF_BSOLVE, 0.0, L.solve_bytes);
   }
 }
@@ -1117,8 +1287,9 @@ void Solver::run_factor(bool fp32) {
   int pf = prof_begin(PROF_FACTOR);
   for (const Level& L : levels_) {
     int pe = prof_begin(PROF_EXTEND);
-    launch_extend_add(st_, F, d_ea_items_.as<int>() + L.ea_off, d_ea_tile0_.as<int>() + L.ea_off, L.ea_n,
-                      L.ea_tiles);
+    for (size_t k = 0; k < L.ea_off.size(); ++k)
+      launch_extend_add(st_, F, d_ea_items_.as<int>() + L.ea_off[k], d_ea_tile0_.as<int>() + L.ea_off[k], L.ea_n[k],
+                        L.ea_tiles[k]);
     prof_end(pe, PROF_EXTEND, 0.0, 0.0);
     for (size_t k = 0; k < L.pan_off.size(); ++k) {
       int pp = prof_begin(PROF_PANEL);
@@ -1129,11 +1300,23 @@ void Solver::run_factor(bool fp32) {
       launch_update(st_, F, S + L.upd_off[k], L.upd_n[k], L.upd_blk[k], I);
       prof_end(pu, PROF_UPDATE, L.upd_fl[k], 0.0);
     }
-    if (!L.big.empty()) {
+    if (L.big.size() == 1) {
       int pd = prof_begin(PROF_DAG);
-      for (int b : L.big) launch_dag_factor(st_, big_[b]->B, I, dag_grid_, fac_epoch_, fp32);
+      launch_dag_factor(st_, big_[L.big[0]]->B, I, dag_grid_, fac_epoch_, fp32);
       prof_end(pd, PROF_DAG, L.big_flops, 0.0);
-      for (int b : L.big) launch_big_mtiles(st_, big_[b]->B, I, fp32);
+      launch_big_mtiles(st_, big_[L.big[0]]->B, I, fp32);
+    } else if (!L.big.empty()) {
+      const int k = std::min<int>(naux_, (int)L.big.size());
+      const int grid = std::max(16, dag_grid_ / k);
+      int pd = prof_begin(PROF_DAG);
+      fork_aux(k);
+      for (size_t i = 0; i < L.big.size(); ++i) {
+        const BigFront& B = big_[L.big[i]]->B;
+        launch_dag_factor(aux_[i % k], B, I, grid, fac_epoch_, fp32);
+        launch_big_mtiles(aux_[i % k], B, I, fp32);
+      }
+      join_aux(k);
+      prof_end(pd, PROF_DAG, L.big_flops, 0.0);
     }
   }
   prof_end(pf, PROF_FACTOR, sym_.exact_flops, 0.0);

@@ -76,8 +76,9 @@ struct DevBuf {
 
 struct Level {
   std::vector<int> fronts;   // fronts at this level
-  // extend-add items (children of this level's fronts) and their tile prefix
-  int ea_off = 0, ea_n = 0, ea_tiles = 0;
+  // extend-add items (children of this level's fronts) and their tile prefix, per round (round k:
+  // the k-th child of each front)
+  std::vector<int> ea_off, ea_n, ea_tiles;
   // panel / update steps: (offset into step table, count, blocks) per panel step k
   std::vector<int> pan_off, pan_n, pan_blk, upd_off, upd_n, upd_blk;
   std::vector<double> pan_fl, upd_fl;  // algorithmic flops of each launch
@@ -113,6 +114,8 @@ class Solver {
   int factor_calls_fp32() const { return fac_fp32_calls_; }
   double host_setup_ms() const { return host_ms_; }      // init on the host: analysis, scaling, BᵀB
   double device_setup_ms() const { return device_ms_; }  // init on the device: buffers, uploads, start
+  double device_setup_part(int k) const { return dev_parts_[k]; }
+  double build_device_mark(int k) const { return bd_parts_[k]; }
   void dense_factor(double* out);  // n×n lower factor (debug; small n)
   int kkt_factor(double delta, double* rho);  // initkkt! on the current H
   // initkkt!(kkt, H; δ) with the caller's H (internal vertex partition, both triangles); the KKT
@@ -272,6 +275,14 @@ class Solver {
     DevBuf tb, state, tasks, sll, spart, mtile, ntile, dinv, stage, trace, f32, dinv32;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
+  // auxiliary streams for the independent large fronts of one level, with a fork event and one
+  // join event per stream
+  static constexpr int NAUX = 4;
+  std::vector<hipStream_t> aux_;
+  std::vector<hipEvent_t> auxev_;
+  int naux_ = 0;
+  void fork_aux(int k);
+  void join_aux(int k);
   DevBuf d_bigtick_, d_bigstick_;
   int epoch_ = 0;
   int dag_grid_ = 512;
@@ -283,11 +294,14 @@ class Solver {
   int fac_fp32_calls_ = 0;
   size_t dev_bytes_ = 0;  // device bytes allocated by device_setup
   double host_ms_ = 0.0, device_ms_ = 0.0;
+  double dev_parts_[4] = {0.0, 0.0, 0.0, 0.0};  // device setup: stream/events, buffers+uploads, schedule, start
+  double bd_parts_[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // build_device: ms from its start at its phases
   // front layout on the host (assembly positions)
   std::vector<i64> fofs_;
   std::vector<std::vector<int>> sepcols_;
   std::vector<int> colfront_;
   i64 front_pos(i64 i, i64 j) const;
+  i64 qdiag_arc(i64 v) const;
   // a caller's H (kkt_factor_h): CSR rows, row tiles, assembly positions and values
   bool user_h_ = false;
   CSRDev hsave_{};  // the CSR not in use (Q's while the caller's H is)

@@ -17,22 +17,19 @@ constexpr int NT = 256;        // threads per workgroup of the tile kernels
 // Tiles move through raw buffer instructions over a resource spanning exactly the tile's
 // extent in F: a lane outside the h×w tile gets an offset past that extent, which the hardware
 // range check turns into a zero load or a dropped store. The cache policy says who wrote the
-// data (Coh):
-//   None — earlier launches: ordinary cached loads;
-//   Xcd  — another workgroup of the running kernel on the same XCD: loads with sc0 miss the CU's
-//          L1 and read the XCD's L2, where every store of the XCD lands (the L1 writes through);
-//   Dev  — a workgroup of the running kernel anywhere on the device: loads and stores with sc1
-//          (device scope, the policy of the agent-scope atomics) go past the XCD's L2.
-// Either way the loads are ordinary loads to the compiler, so a thread's 16 loads are in flight
-// together instead of one after another.
+// data (Coh): None — earlier launches, ordinary cached loads; Dev — another workgroup of the
+// running kernel: loads and stores with sc1 (device scope, the policy of the agent-scope
+// atomics). A CU's L1 keeps a line it read earlier even after another CU of the same XCD stored
+// to it, and no cheaper load or cache invalidate reaches that store, so device scope is needed
+// within one XCD too. Either way the loads are ordinary loads to the compiler, so a thread's 16
+// loads are in flight together instead of one after another.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-constexpr int CPOL_SC0 = 1;            // buffer cache policy bits
-constexpr int CPOL_SC1 = 16;
+constexpr int CPOL_SC1 = 16;               // buffer cache policy bit: device scope
 constexpr unsigned OFF_OUT = 0x80000000u;  // an offset past every tile extent
 
-enum class Coh { None, Xcd, Dev };
+enum class Coh { None, Dev };
 template <Coh C>
-constexpr int load_pol() { return C == Coh::Dev ? CPOL_SC1 : (C == Coh::Xcd ? CPOL_SC0 : 0); }
+constexpr int load_pol() { return C == Coh::Dev ? CPOL_SC1 : 0; }
 template <Coh C>
 constexpr int store_pol() { return C == Coh::Dev ? CPOL_SC1 : 0; }
 

@@ -511,12 +511,14 @@ class IPMSolver:
         return out
 
     def stats(self):
-        st = np.zeros(10)
+        st = np.zeros(19)
         if lib().mb_solver_stats(self._h, _p(st, ctypes.c_double)) != 0:
             raise _err("stats")
         return dict(exact_flops=st[0], front_flops=st[1], nfront=int(st[2]), nlevel=int(st[3]),
                     factor_ms_total=st[4], factor_calls=int(st[5]), factor_ms_last=st[6],
-                    factor_calls_fp32=int(st[7]), init_host_ms=st[8], init_device_ms=st[9])
+                    factor_calls_fp32=int(st[7]), init_host_ms=st[8], init_device_ms=st[9],
+                    init_device_parts_ms=dict(stream=st[10], buffers=st[11], schedule=st[12], start=st[13]),
+                    build_marks_ms=dict(fronts=st[14], union=st[15], cones=st[16], allocs=st[17], sparse=st[18]))
 
     def reinit(self, p0=None, d0=None, y0=None):
         """reinit!(solver; p0, d0, y0) (solver/ipm.jl:371-429)."""

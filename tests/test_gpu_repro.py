@@ -1,8 +1,8 @@
-"""Single-front solves are reproducible bit for bit: the persistent factorization subtracts each
-tile's updates in a fixed column order and the pipelined triangular solves sum their
-contributions in a fixed block order, however the published blocks were batched. (Fronts with
-several children add the children's update blocks into the parent with atomics, so multi-front
-problems agree only to rounding between runs.)"""
+"""Solves are reproducible bit for bit: the persistent factorization subtracts each tile's updates
+in a fixed column order, the pipelined triangular solves sum their contributions in a fixed block
+order however the published blocks were batched, and the extend-add adds a front's children in a
+fixed order (one child per parent in each launch). So results do not depend on the run, on the
+number of workgroups the persistent kernels get, or on the concurrency of independent fronts."""
 import sys
 
 import numpy as np
@@ -34,6 +34,29 @@ def test_single_front_solves_repeat_exactly(name, precision):
         assert s.stats()["nfront"] == 1
         runs.append(s.solve())
         s.close()
+    for r in runs[1:]:
+        assert r.status == runs[0].status and r.niter == runs[0].niter
+        assert np.array_equal(r.p, runs[0].p) and np.array_equal(r.d, runs[0].d) and np.array_equal(r.y, runs[0].y)
+
+
+MULTI = {
+    "maxcut_chordal": lambda: G.maxcut_chordal(nv=160, band=6, extra=2e-3, seed=5, merge=24),
+    "qp_multifront": lambda: G.qp(n=900, m=400, seed=8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MULTI))
+def test_multi_front_solves_repeat_exactly(name):
+    print("case", name, flush=True)
+    sys.stdout.flush()
+    prob = hip_problem(MULTI[name]())
+    runs = []
+    for grid in (0, 0, 24):
+        s = mh.init(prob, mh.IPMSettings(big_front_ld=64, dag_grid=grid))
+        assert s.stats()["nfront"] > 1
+        runs.append(s.solve())
+        s.close()
+    assert runs[0].status == mh.IPMStatus.OPTIMAL
     for r in runs[1:]:
         assert r.status == runs[0].status and r.niter == runs[0].niter
         assert np.array_equal(r.p, runs[0].p) and np.array_equal(r.d, runs[0].d) and np.array_equal(r.y, runs[0].y)
