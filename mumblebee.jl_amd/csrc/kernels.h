@@ -185,7 +185,9 @@ void launch_add_diag(hipStream_t, const FrontDesc*, int nf, const InstDev*);
 void launch_front_norm2(hipStream_t, const FrontDesc*, int nf, double* partial);
 void launch_extend_add(hipStream_t, const FrontDesc*, const int* items, const int* item_tile0, int nitem,
                        int ntile);
-void launch_panel(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*);
+// flags: one int per step item (epoch-valued, zeroed once); abortf: the instance's abort word
+void launch_panel(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*, int* flags,
+                  int epoch, int* abortf);
 void launch_update(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*);
 void launch_fsolve(hipStream_t, const FrontDesc*, const int* items, int nitem, const int* childlist,
                    const InstDev*, int kv);

@@ -1014,10 +1014,18 @@ __device__ bool lds_potrf(double* A, int w) {
 
 // Panel step: block 0 of each item factors the diagonal block and writes it; blocks 1..ntile
 // factor it redundantly and solve their 64-row tile below: X = A L⁻ᵀ.
+// One 64-column panel step of every small front of a level. Tile 0 of an item factors the
+// diagonal block in LDS, stores L_jj and publishes the factorization epoch in the item's flag
+// (release, device scope); the row tiles below wait for that flag (acquire) and read L_jj, which
+// they then apply to their rows. The diagonal workgroup precedes its row workgroups in the grid,
+// so it runs before (or beside) them and the wait ends; it is bounded all the same, and raises
+// the abort flag at its limit. (Row tiles that factored the block themselves from F raced with
+// tile 0's store of L_jj into the same place.)
 __global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const StepDesc* steps, int nstep,
-                                               const InstDev* insts) {
+                                               const InstDev* insts, int* flags, int epoch, int* abortf) {
   __shared__ double Ld[NB * (NB + 1)];
   __shared__ double X[NB * (NB + 1)];
+  __shared__ int okw;
   const int ld = NB + 1;
   int it = find_item(steps, nstep, blockIdx.x);
   StepDesc S = steps[it];
@@ -1027,20 +1035,43 @@ __global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const St
   if (I.sc[S_ACTIVE] == 0.0) return;
   double* F = Fd.F;
   int LD = Fd.ld, kb = S.kb, w = S.w;
-  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-    int i = t % NB, j = t / NB;
-    Ld[i + j * ld] = (i < w && j < w && i >= j) ? F[(size_t)(kb + i) + (size_t)(kb + j) * LD] : 0.0;
-  }
-  __syncthreads();
-  bool ok = lds_potrf(Ld, w);
   if (tile == 0) {
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+      int i = t % NB, j = t / NB;
+      Ld[i + j * ld] = (i < w && j < w && i >= j) ? F[(size_t)(kb + i) + (size_t)(kb + j) * LD] : 0.0;
+    }
+    __syncthreads();
+    bool ok = lds_potrf(Ld, w);
     if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
     for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
       int i = t % NB, j = t / NB;
       if (i < w && j < w && i >= j) F[(size_t)(kb + i) + (size_t)(kb + j) * LD] = Ld[i + j * ld];
     }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&flags[it], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
+  if (threadIdx.x == 0) {
+    long spins = 0;
+    int good = 1;
+    while (__hip_atomic_load(&flags[it], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+      if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1L << 24)) {
+        __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        good = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    okw = good;
+  }
+  __syncthreads();
+  if (!okw) return;
+  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+    int i = t % NB, j = t / NB;
+    Ld[i + j * ld] = (i < w && j < w && i >= j) ? F[(size_t)(kb + i) + (size_t)(kb + j) * LD] : 0.0;
+  }
+  __syncthreads();
   int r0 = kb + w + (tile - 1) * NB;
   int nr = min(NB, LD - r0);
   for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
@@ -1394,8 +1425,9 @@ void launch_extend_add(hipStream_t st, const FrontDesc* F, const int* items, con
                        int ntile) {
   if (ntile) k_extend_add<<<ntile, TILE, 0, st>>>(F, items, item_tile0, nitem);
 }
-void launch_panel(hipStream_t st, const FrontDesc* F, const StepDesc* S, int nstep, int nblk, const InstDev* I) {
-  if (nblk) k_panel<<<nblk, 256, 0, st>>>(F, S, nstep, I);
+void launch_panel(hipStream_t st, const FrontDesc* F, const StepDesc* S, int nstep, int nblk, const InstDev* I,
+                  int* flags, int epoch, int* abortf) {
+  if (nblk) k_panel<<<nblk, 256, 0, st>>>(F, S, nstep, I, flags, epoch, abortf);
 }
 void launch_update(hipStream_t st, const FrontDesc* F, const StepDesc* S, int nstep, int nblk, const InstDev* I) {
   if (nblk) k_update<<<nblk, 256, 0, st>>>(F, S, nstep, I);

@@ -1027,6 +1027,8 @@ void Solver::build_schedule() {
     }
   }
   d_steps_.upload(steps, st_);
+  d_panflag_.alloc(sizeof(int) * std::max<size_t>(steps.size(), 1));
+  HIPCHK(hipMemsetAsync(d_panflag_.p, 0, d_panflag_.bytes, st_));
   d_ea_items_.upload(ea_items, st_);
   d_ea_tile0_.upload(ea_tile0, st_);
   d_solve_items_.upload(solve_items, st_);
@@ -1273,10 +1275,9 @@ void Solver::usolve(int kv) {
 // subnormal: the test reads the word's bits, so a caller running with denormals-are-zero still
 // sees it.
 void Solver::check_abort() {
-  if (big_.empty()) return;
   int32_t raised = 0;
   std::memcpy(&raised, &hsc_[S_ABORT], sizeof(raised));
-  if (raised != 0) throw std::runtime_error("persistent front kernel exceeded its wait limit");
+  if (raised != 0) throw std::runtime_error("front kernel exceeded its wait limit");
 }
 
 void Solver::run_factor(bool fp32) {
@@ -1293,7 +1294,8 @@ void Solver::run_factor(bool fp32) {
     prof_end(pe, PROF_EXTEND, 0.0, 0.0);
     for (size_t k = 0; k < L.pan_off.size(); ++k) {
       int pp = prof_begin(PROF_PANEL);
-      launch_panel(st_, F, S + L.pan_off[k], L.pan_n[k], L.pan_blk[k], I);
+      launch_panel(st_, F, S + L.pan_off[k], L.pan_n[k], L.pan_blk[k], I, d_panflag_.as<int>() + L.pan_off[k],
+                   fac_epoch_, reinterpret_cast<int*>(d_sc_.as<double>() + S_ABORT));
       prof_end(pp, PROF_PANEL, L.pan_fl[k], 0.0);
       if (L.upd_blk[k] == 0) continue;
       int pu = prof_begin(PROF_UPDATE);

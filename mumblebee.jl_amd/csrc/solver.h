@@ -284,6 +284,7 @@ class Solver {
   void fork_aux(int k);
   void join_aux(int k);
   DevBuf d_bigtick_, d_bigstick_;
+  DevBuf d_panflag_;  // small-front panel steps: diagonal factor published (epoch-valued)
   int epoch_ = 0;
   int dag_grid_ = 512;
   int fac_epoch_ = 0;     // factorization counter: a large front's tile flags hold it once final

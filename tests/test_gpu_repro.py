@@ -60,3 +60,35 @@ def test_multi_front_solves_repeat_exactly(name):
     for r in runs[1:]:
         assert r.status == runs[0].status and r.niter == runs[0].niter
         assert np.array_equal(r.p, runs[0].p) and np.array_equal(r.d, runs[0].d) and np.array_equal(r.y, runs[0].y)
+
+
+def test_small_front_solves_repeat_exactly_under_concurrency():
+    """Small fronts (order < 512) take the level-batched panel kernels. Solved alone and as eight
+    copies at once from eight host threads (their kernels interleaving on the device, so the
+    workgroups of one panel launch start at different times), the results are the same bit for
+    bit: a row tile of a panel step reads the diagonal factor only after the diagonal workgroup
+    published it."""
+    import threading
+    print("case small fronts under concurrency", flush=True)
+    sys.stdout.flush()
+    data = [G.qp(n=200, m=100, seed=64 + k) for k in range(4)] + \
+           [G.socp(n=300, m=140, ncone=12, seed=70 + k, adens=0.03) for k in range(4)]
+    probs = [hip_problem(d) for d in data]
+    solo = [mh.solve(p) for p in probs]
+    for rep in range(3):
+        solvers = [mh.init(p) for p in probs]
+        out = [None] * len(solvers)
+
+        def run(k):
+            out[k] = solvers[k].solve()
+
+        threads = [threading.Thread(target=run, args=(k,)) for k in range(len(solvers))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for s in solvers:
+            s.close()
+        for k, (a, b) in enumerate(zip(out, solo)):
+            assert a.status == b.status and a.niter == b.niter, (rep, k)
+            assert np.array_equal(a.p, b.p) and np.array_equal(a.y, b.y), (rep, k)

@@ -3,11 +3,11 @@ chordal max-cut SDP, n log-uniform in [50, 4000]) arriving as a Poisson process 
 through the continuous-batching loop. Each served result is compared with the same request
 solved alone, and with the oracle where the oracle is quick (n ≤ 300).
 
-Served and solo solves run the same kernels, but the small-front extend-add sums children in
-whatever order their atomics land, so the two are equal to rounding, not bit for bit; a request
-may then end OPTIMAL on one side and NEAR_OPTIMAL on the other (small SOCPs whose last iterate
-sits just above the gap test). Objectives are compared to 1e-8, or to the two solves' own duality
-gaps where one ended NEAR_OPTIMAL."""
+Served and solo solves run the same kernels with the same summation orders (the persistent
+kernels' sums do not depend on how many workgroups they get, and the extend-add adds a front's
+children in a fixed order), so a served result equals the solo one bit for bit. Against the
+oracle, objectives are compared to 1e-8, or to the two solves' own duality gaps where one ended
+NEAR_OPTIMAL (small SOCPs whose last iterate sits just above the gap test)."""
 import time
 
 import numpy as np
@@ -67,6 +67,8 @@ def test_served_stream_matches_solo_and_oracle():
             print(f"request {k} {d['name']} B {d['B'].shape}: served {res.status.name} in {res.niter} iterations "
                   f"(mu {res.mu:.3g}, pres {res.pres:.3g}, dres {res.dres:.3g}), alone {solo.status.name} in "
                   f"{solo.niter}", flush=True)
+        assert res.status == solo.status and res.niter == solo.niter, (k, d["name"], res.status, solo.status)
+        assert np.array_equal(res.p, solo.p) and np.array_equal(res.y, solo.y), (k, d["name"])
         assert _agree(res, solo), (k, d["name"], res.status, solo.status, res.pobj, solo.pobj)
         if d["B"].shape[1] <= 300:
             ores = O.solve(oracle_problem(d))
