@@ -286,6 +286,38 @@ int mb_server_wait(mb_server* s, int64_t ticket, double timeout_s, double* p, do
 int mb_server_release(mb_server* s, int64_t ticket);
 /* out[5] = queued, running, completed, device bytes reserved now, peak bytes reserved. */
 int mb_server_stats(mb_server* s, double* out);
+/* The same loop with its slots as members of one gang (below): the solves of the admitted
+ * requests advance in lockstep and share every launch; a finished request's slot takes the next
+ * admitted one between two launches. */
+mb_server* mb_server_create_gang(int32_t nslots, double mem_budget_bytes);
+
+/* ------------------------------------------------------------------------------------------
+ * Lockstep batching ("gang"). No counterpart in the reference: several independent
+ * solve!(solver) calls (solver/solver.jl:119-150) advance together, each taking the reference's
+ * decisions for its own instance, while the device work of each phase of step! is one launch
+ * for all of them (vector kernels over concatenated tile lists, the cone kernels over one cone
+ * table, the small fronts of one elimination-tree level of all instances per launch, the large
+ * fronts of all instances in one persistent factorization and one pipelined solve launch, one
+ * scalar read-back per phase). Each result is bit for bit the result of the same solver run
+ * alone. capacity ≤ 32 members at a time; dag_grid ≤ 0: one workgroup per CU for the combined
+ * factorization; profile ≠ 0: time the combined factorizations with events.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct mb_gang mb_gang;
+mb_gang* mb_gang_create(int32_t capacity, int32_t dag_grid, int32_t profile);
+void mb_gang_destroy(mb_gang* g);
+/* solve!(solver) as a member of the gang: joins (waiting for a free slot), runs the IPM loop and
+ * leaves; returns the IPMStatus or −1. Call it from one host thread per solver; read the result
+ * with mb_solver_result afterwards. */
+int mb_solver_solve_in(mb_solver* s, mb_gang* g);
+/* n solvers through the gang from min(n, capacity) host threads of the library, a finished
+ * solver's slot taken by the next one; status[i] = IPMStatus of solver i (−1: failed). Returns 0,
+ * or −1 if any failed (mb_last_error: the first message). */
+int mb_solve_batch(mb_solver* const* s, int32_t n, mb_gang* g, int32_t* status);
+/* out[14] = ticks, launches, ops, ops in combined launches, read-backs, joins, combined
+ * factorizations, their large fronts, their algorithmic flops, their time in ms (profile ≠ 0),
+ * persistent factorization launches, flops in them, their time in ms, members now. */
+int mb_gang_stats(mb_gang* g, double* out);
+
 /* Seconds on the steady clock the server's timestamps use. */
 double mb_clock(void);
 /* out[3] = compute units, free and total device bytes of the current device. */

@@ -5,8 +5,14 @@
 #include <string>
 
 #include "../../include/mumblebee_hip.h"
+#include "gang.h"
 #include "solver.h"
 #include "server.h"
+
+#include <atomic>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 namespace {
 thread_local std::string g_err;
@@ -587,6 +593,83 @@ int mb_server_release(mb_server* s, int64_t ticket) {
 int mb_server_stats(mb_server* s, double* out) {
   return guard([&]() {
     s->S->stats(out);
+    return 0;
+  });
+}
+
+mb_server* mb_server_create_gang(int32_t nslots, double mem_budget_bytes) {
+  mb_server* out = nullptr;
+  guard([&]() {
+    auto h = std::make_unique<mb_server>();
+    h->S = std::make_unique<mb::Server>(nslots, mem_budget_bytes, 0, 0, true);
+    out = h.release();
+    return 0;
+  });
+  return out;
+}
+
+// ------------------------------------------------------------------------------------------
+// lockstep batching
+// ------------------------------------------------------------------------------------------
+
+struct mb_gang {
+  std::unique_ptr<mb::Gang> G;
+};
+
+mb_gang* mb_gang_create(int32_t capacity, int32_t dag_grid, int32_t profile) {
+  mb_gang* out = nullptr;
+  guard([&]() {
+    if (capacity < 1 || capacity > mb::GANG_MAX) throw std::invalid_argument("gang capacity must be 1..32");
+    auto h = std::make_unique<mb_gang>();
+    h->G = std::make_unique<mb::Gang>(capacity, dag_grid, profile != 0);
+    out = h.release();
+    return 0;
+  });
+  return out;
+}
+
+void mb_gang_destroy(mb_gang* g) { delete g; }
+
+int mb_solver_solve_in(mb_solver* s, mb_gang* g) {
+  return guard([&]() { return s->S->solve_in(*g->G); });
+}
+
+int mb_solve_batch(mb_solver* const* s, int32_t n, mb_gang* g, int32_t* status) {
+  return guard([&]() {
+    if (n <= 0) return 0;
+    std::atomic<int> next{0};
+    std::string first;
+    std::mutex emu;
+    const int nt = std::min<int>(n, g->G->capacity());
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&]() {
+        for (;;) {
+          const int i = next++;
+          if (i >= n) return;
+          try {
+            status[i] = s[i]->S->solve_in(*g->G);
+          } catch (const std::exception& e) {
+            status[i] = -1;
+            std::lock_guard<std::mutex> lk(emu);
+            if (first.empty()) first = e.what();
+          }
+        }
+      });
+    for (auto& t : th) t.join();
+    if (!first.empty()) throw std::runtime_error(first);
+    return 0;
+  });
+}
+
+int mb_gang_stats(mb_gang* g, double* out) {
+  return guard([&]() {
+    mb::GangStats st;
+    g->G->stats(st);
+    const double v[14] = {st.ticks, st.launches, st.ops, st.combined_ops, st.readbacks, st.joins,
+                          st.factor_launches, st.fronts_per_factor, st.factor_flops, st.factor_ms,
+                          st.dag_launches, st.dag_flops, st.dag_ms, (double)g->G->members()};
+    std::memcpy(out, v, sizeof(v));
     return 0;
   });
 }

@@ -73,7 +73,8 @@ struct DotBatch {
     return *this;
   }
 };
-void launch_dots(hipStream_t, const InstDev*, int inst, const DotBatch&);
+// ninst > 1: the same batch for instances inst .. inst+ninst−1 of the table in one launch
+void launch_dots(hipStream_t, const InstDev*, int inst, const DotBatch&, int ninst = 1);
 // z = a x + b y + c w; up to LIN_MAX of them on the n-space and on the m-space in one launch,
 // applied in order at each index (a later one sees the earlier ones' results)
 struct Lin {
@@ -184,8 +185,9 @@ void launch_assemble_cones(hipStream_t, const InstDev*, const ConeDesc*, const i
 void launch_add_diag(hipStream_t, const FrontDesc*, int nf, const InstDev*);
 void launch_front_norm2(hipStream_t, const FrontDesc*, int nf, double* partial);
 void launch_extend_add(hipStream_t, const FrontDesc*, const int* items, const int* item_tile0, int nitem,
-                       int ntile);
+                       int ntile, const InstDev*);
 // flags: one int per step item (epoch-valued, zeroed once); abortf: the instance's abort word
+// (nullptr: each front's own instance's)
 void launch_panel(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*, int* flags,
                   int epoch, int* abortf);
 void launch_update(hipStream_t, const FrontDesc*, const StepDesc*, int nstep, int nblk, const InstDev*);
@@ -198,5 +200,16 @@ void launch_set_scalar(hipStream_t, double* sc, int slot, double v);
 void launch_set_scalar2(hipStream_t, double* sc, int s1, double v1, int s2, double v2);
 // the corrector's σμ (solver/ipm.jl:181-200) from μa = sc[S_TMP2]/ν, into sc[S_SMU]
 void launch_smu(hipStream_t, double* sc, double nu, double mu, double mut);
+
+// gang kernels over a combined instance table of k ≤ GANG_SLOTS slots (csrc/gang.cpp)
+constexpr int GANG_SLOTS = 32;
+struct GangVals {
+  double a[GANG_SLOTS], b[GANG_SLOTS], c[GANG_SLOTS];
+};
+void launch_gang_set2(hipStream_t, const InstDev*, int k, int s1, int s2, const GangVals&);
+void launch_gang_smu(hipStream_t, const InstDev*, int k, const GangVals&);
+void launch_gang_active(hipStream_t, const InstDev*, int k, unsigned long long mask, unsigned long long occupied);
+void launch_gang_setinst(hipStream_t, InstDev*, int slot, const InstDev&);
+void launch_gang_gather(hipStream_t, const InstDev*, int k, unsigned long long mask, double* out);
 
 }  // namespace mb

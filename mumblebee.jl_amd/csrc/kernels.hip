@@ -179,7 +179,7 @@ __global__ void k_dot(const InstDev* insts, const Tile* tiles, const int* tile_o
 constexpr int DOTS_THREADS = 1024;
 __global__ void __launch_bounds__(DOTS_THREADS) k_dots(const InstDev* insts, int inst, DotBatch B) {
   __shared__ double red[DOTS_THREADS / 64];
-  const InstDev& I = insts[inst];
+  const InstDev& I = insts[inst + (int)blockIdx.y];
   if (I.sc[S_ACTIVE] == 0.0) return;
   const DotSpec d = B.s[blockIdx.x];
   const int len = spacelen(I, d.space);
@@ -933,7 +933,8 @@ __global__ void k_front_norm2(const FrontDesc* fronts, int nfront, double* parti
 // siblings of one parent run concurrently)
 // ------------------------------------------------------------------------------------------
 
-__global__ void k_extend_add(const FrontDesc* fronts, const int* items, const int* item_tile0, int nitem) {
+__global__ void k_extend_add(const FrontDesc* fronts, const int* items, const int* item_tile0, int nitem,
+                             const InstDev* insts) {
   // binary search the item owning this block
   int b = blockIdx.x;
   int lo = 0, hi = nitem - 1;
@@ -945,6 +946,7 @@ __global__ void k_extend_add(const FrontDesc* fronts, const int* items, const in
   int child = items[lo];
   int tile = b - item_tile0[lo];
   const FrontDesc& C = fronts[child];
+  if (insts[C.inst].sc[S_ACTIVE] == 0.0) return;
   const FrontDesc& P = fronts[C.parent];
   int s = C.s;
   int T = (s + NB - 1) / NB;
@@ -1033,6 +1035,7 @@ __global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const St
   const FrontDesc& Fd = fronts[S.front];
   const InstDev& I = insts[Fd.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
+  if (!abortf) abortf = reinterpret_cast<int*>(I.sc + S_ABORT);
   double* F = Fd.F;
   int LD = Fd.ld, kb = S.kb, w = S.w;
   if (tile == 0) {
@@ -1342,8 +1345,8 @@ void launch_dot(hipStream_t st, const InstDev* I, const Tile* T, int nt, const i
   if (nt) k_dot_partial<<<nt, TILE, 0, st>>>(I, T, space, kx, ky, mode, kw0, kw1, partial);
   k_reduce_slot<<<ninst, TILE, 0, st>>>(I, tile_ofs, partial, slot, op);
 }
-void launch_dots(hipStream_t st, const InstDev* I, int inst, const DotBatch& B) {
-  if (B.n > 0) k_dots<<<B.n, DOTS_THREADS, 0, st>>>(I, inst, B);
+void launch_dots(hipStream_t st, const InstDev* I, int inst, const DotBatch& B, int ninst) {
+  if (B.n > 0 && ninst > 0) k_dots<<<dim3(B.n, ninst), DOTS_THREADS, 0, st>>>(I, inst, B);
 }
 void launch_lin(hipStream_t st, const InstDev* I, const Tile* ntiles, int nnt, const Tile* mtiles, int nmt,
                 const LinSet& N, const LinSet& M) {
@@ -1422,8 +1425,8 @@ void launch_front_norm2(hipStream_t st, const FrontDesc* F, int nf, double* part
   if (nf) k_front_norm2<<<nf, TILE, 0, st>>>(F, nf, partial);
 }
 void launch_extend_add(hipStream_t st, const FrontDesc* F, const int* items, const int* item_tile0, int nitem,
-                       int ntile) {
-  if (ntile) k_extend_add<<<ntile, TILE, 0, st>>>(F, items, item_tile0, nitem);
+                       int ntile, const InstDev* I) {
+  if (ntile) k_extend_add<<<ntile, TILE, 0, st>>>(F, items, item_tile0, nitem, I);
 }
 void launch_panel(hipStream_t st, const FrontDesc* F, const StepDesc* S, int nstep, int nblk, const InstDev* I,
                   int* flags, int epoch, int* abortf) {
@@ -1458,7 +1461,7 @@ void launch_set_scalar2(hipStream_t st, double* sc, int s1, double v1, int s2, d
 
 // solver/ipm.jl:181-200: σμ from the affine centrality μa, on the device so that the corrector's
 // right-hand side needs no read-back
-__global__ void k_smu(double* sc, double nu, double mu, double mut) {
+__device__ __forceinline__ void smu_value(double* sc, double nu, double mu, double mut) {
   double smu = sc[S_TMP2];
   const double mua = smu / nu;
   if (mut > 0) {
@@ -1474,9 +1477,69 @@ __global__ void k_smu(double* sc, double nu, double mu, double mut) {
   }
   sc[S_SMU] = smu;
 }
+__global__ void k_smu(double* sc, double nu, double mu, double mut) { smu_value(sc, nu, mu, mut); }
 void launch_smu(hipStream_t st, double* sc, double nu, double mu, double mut) {
   k_smu<<<1, 1, 0, st>>>(sc, nu, mu, mut);
 }
 void launch_set_scalar(hipStream_t st, double* sc, int slot, double v) { k_set_scalar<<<1, 1, 0, st>>>(sc, slot, v); }
+
+// ------------------------------------------------------------------------------------------
+// gang (lockstep batch) kernels: one thread per member slot of the combined instance table
+// ------------------------------------------------------------------------------------------
+
+// sc[s1] = a[k], sc[s2] = b[k] (s2 < 0: only s1) for the members taking part (S_ACTIVE)
+__global__ void k_gang_set2(const InstDev* insts, int k, int s1, int s2, GangVals v) {
+  const int i = threadIdx.x;
+  if (i >= k) return;
+  double* sc = insts[i].sc;
+  if (sc[S_ACTIVE] == 0.0) return;
+  sc[s1] = v.a[i];
+  if (s2 >= 0) sc[s2] = v.b[i];
+}
+void launch_gang_set2(hipStream_t st, const InstDev* I, int k, int s1, int s2, const GangVals& v) {
+  k_gang_set2<<<1, GANG_SLOTS, 0, st>>>(I, k, s1, s2, v);
+}
+
+// σμ of each member taking part (k_smu with ν = a, μ = b, μt = c)
+__global__ void k_gang_smu(const InstDev* insts, int k, GangVals v) {
+  const int i = threadIdx.x;
+  if (i >= k) return;
+  double* sc = insts[i].sc;
+  if (sc[S_ACTIVE] == 0.0) return;
+  smu_value(sc, v.a[i], v.b[i], v.c[i]);
+}
+void launch_gang_smu(hipStream_t st, const InstDev* I, int k, const GangVals& v) {
+  k_gang_smu<<<1, GANG_SLOTS, 0, st>>>(I, k, v);
+}
+
+// S_ACTIVE = bit i of mask for the occupied slots
+__global__ void k_gang_active(const InstDev* insts, int k, unsigned long long mask, unsigned long long occupied) {
+  const int i = threadIdx.x;
+  if (i >= k || !((occupied >> i) & 1ull)) return;
+  insts[i].sc[S_ACTIVE] = ((mask >> i) & 1ull) ? 1.0 : 0.0;
+}
+void launch_gang_active(hipStream_t st, const InstDev* I, int k, unsigned long long mask,
+                        unsigned long long occupied) {
+  k_gang_active<<<1, GANG_SLOTS, 0, st>>>(I, k, mask, occupied);
+}
+
+// one entry of the instance table (stream-ordered with the launches that read it)
+__global__ void k_gang_setinst(InstDev* insts, int slot, InstDev v) {
+  if (threadIdx.x == 0) insts[slot] = v;
+}
+void launch_gang_setinst(hipStream_t st, InstDev* I, int slot, const InstDev& v) {
+  k_gang_setinst<<<1, 64, 0, st>>>(I, slot, v);
+}
+
+// the scalar blocks of the slots in mask, side by side (slot i at out + i·NSCALAR)
+__global__ void k_gang_gather(const InstDev* insts, unsigned long long mask, double* out) {
+  const int i = blockIdx.x;
+  if (!((mask >> i) & 1ull)) return;
+  const double* sc = insts[i].sc;
+  for (int t = threadIdx.x; t < NSCALAR; t += blockDim.x) out[(size_t)i * NSCALAR + t] = sc[t];
+}
+void launch_gang_gather(hipStream_t st, const InstDev* I, int k, unsigned long long mask, double* out) {
+  if (k > 0) k_gang_gather<<<k, 64, 0, st>>>(I, mask, out);
+}
 
 }  // namespace mb

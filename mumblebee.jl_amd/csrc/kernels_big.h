@@ -46,4 +46,19 @@ int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int
 int launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
                       int epoch, long long base, bool fp32);
 
+// The large fronts of a gang (several instances) in one launch each. Bs: the nf fronts (device
+// copies of their BigFront), bfront: their indices in the front table F; order: ntot (front,
+// task or block) pairs, each front's own items in their order. The factorization's ticket must be
+// 0 on entry and is put back to 0 (max2tr: 2·max Tr, the chain-step tile grid); the solves take
+// one ticket of the 64-bit counter per workgroup (the caller advances base by ntot).
+void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const int2* order, int ntot, int* ticket,
+                             const InstDev* I, int grid, int epoch, bool fp32, int max2tr);
+void launch_big_fsolve_multi(hipStream_t st, const BigFront* Bs, const int* bfront, int nf, const int2* order,
+                             int ntot, unsigned long long* ticket, const FrontDesc* F, const int* childlist,
+                             const InstDev* I, int kv, int epoch, long long base, bool fp32);
+// maxtr, maxch: the separator-product grid (largest Tr and number of 64-row separator chunks)
+void launch_big_bsolve_multi(hipStream_t st, const BigFront* Bs, const int* bfront, int nf, const int2* order,
+                             int ntot, unsigned long long* ticket, const FrontDesc* F, const InstDev* I, int kv,
+                             int epoch, long long base, bool fp32, int maxtr, int maxch);
+
 }  // namespace mb

@@ -19,6 +19,7 @@
 // it starts waiting, so each step of the dependency chain is two 64×64 products.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -106,6 +107,124 @@ __device__ __forceinline__ double* dinv_of<double>(const BigFront& B) { return B
 template <>
 __device__ __forceinline__ float* dinv_of<float>(const BigFront& B) { return B.Dinv32; }
 
+// One task of the tile DAG (tasks[t] of front B), by the whole workgroup. False when a wait
+// ended on the abort flag or at its limit.
+template <class T>
+__device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, int epoch, int t, T* SA, T* SB, T* SC,
+                                         T* SD) {
+  double* F = B.F;
+  T* Lf = factor_of<T>(B);
+  T* Di = dinv_of<T>(B);
+  const int LD = B.ld;
+  stamp(B.trace, t, 0);
+  const int4 task = B.tasks[t];
+  const int type = task.x, i = task.y, j = task.z, kmax = task.w;
+  const int r0i = B.tb[i], hi = B.tb[i + 1] - r0i;
+  const int r0j = B.tb[j], hj = B.tb[j + 1] - r0j;
+  // the assembled tile (written by earlier launches; only this task writes it here)
+  load_tile<Coh::None>(SA, F, LD, r0i, r0j, hi, hj);
+  __syncthreads();
+  Acc<T> C;
+  acc_from_lds(C, SA);
+  __syncthreads();
+  // C −= Σ_k L_ik L_jkᵀ, software pipelined: the tiles of column k+1 are in flight while the
+  // MFMAs of column k run from the other LDS buffer pair
+  bool alive = true;
+  TileRegs<T> RA, RB;
+  if (kmax > 0) {
+    if (!wait_geq(&B.state[tri_index(i, 0)], epoch, B.abortf) ||
+        (i != j && !wait_geq(&B.state[tri_index(j, 0)], epoch, B.abortf)))
+      return false;
+    const int c00 = B.tb[0], w0 = B.tb[1] - c00;
+    fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c00, hi, w0);
+    if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c00, hj, w0);
+  }
+  for (int k = 0; k < kmax; ++k) {
+    T* XA = (k & 1) ? SC : SA;
+    T* XB = (k & 1) ? SD : SB;
+    put_tile(XA, RA);
+    if (i != j) put_tile(XB, RB);
+    __syncthreads();  // also: the buffer pair written next was last read before this point
+    if (k + 1 < kmax) {
+      if (!wait_geq(&B.state[tri_index(i, k + 1)], epoch, B.abortf) ||
+          (i != j && !wait_geq(&B.state[tri_index(j, k + 1)], epoch, B.abortf))) {
+        alive = false;
+        break;
+      }
+      const int c0n = B.tb[k + 1], wn = B.tb[k + 2] - c0n;
+      fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c0n, hi, wn);
+      if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c0n, hj, wn);
+    }
+    mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
+  }
+  __syncthreads();
+  if (!alive) return false;
+  stamp(B.trace, t, 1);
+  if (type == 0) {
+    if (j > 0) {
+      // L_{j,j−1} = C_{j,j−1} L_{j−1,j−1}⁻ᵀ from the staged tile (fetched before the inverse is
+      // ready), then C_jj −= L_{j,j−1} L_{j,j−1}ᵀ
+      const int c0p = B.tb[j - 1], wp = B.tb[j] - c0p;
+      if (!wait_geq(&B.stage[j - 1], epoch, B.abortf)) return false;
+      load_tile<Coh::Dev>(SA, Lf, LD, r0j, c0p, hj, wp);
+      if (!wait_geq(&B.state[tri_index(j - 1, j - 1)], epoch, B.abortf)) return false;
+      load_tile<Coh::Dev>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
+      __syncthreads();
+      Acc<T> X;
+      acc_zero(X);
+      mfma_acc<false>(X, SA, SB, NB);
+      __syncthreads();
+      acc_to_lds(X, SA);
+      __syncthreads();
+      store_tile<Coh::Dev>(Lf, LD, r0j, c0p, hj, wp, SA, false);
+      mfma_acc<true>(C, SA, SA, wp);
+      publish(&B.state[tri_index(j, j - 1)], epoch);  // the stores drained during the update
+    }
+    stamp(B.trace, t, 2);
+    // factor and invert; L_jj⁻¹ → Dinv and publish, then L_jj → F (read only by later launches)
+    acc_to_lds(C, SA);
+    __syncthreads();
+    bool ok = chol_inv64(SA, SC, SB, hj, B.trace, t);  // SB is free here and lies below 64 KiB
+    stamp(B.trace, t, 6);
+    if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
+    store_tile<Coh::Dev>(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
+    publish(&B.state[tri_index(j, j)], epoch);
+    stamp(B.trace, t, 7);
+    store_tile<Coh::Dev>(Lf, LD, r0j, r0j, hj, hj, SA, true);
+  } else if (type == 3) {
+    // sub-diagonal tile: hand the accumulated tile to the next diagonal task
+    acc_to_lds(C, SA);
+    __syncthreads();
+    stamp(B.trace, t, 2);
+    store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
+    publish(&B.stage[j], epoch);
+  } else if (type == 1) {
+    // L_ij = C_ij L_jj⁻ᵀ
+    acc_to_lds(C, SA);
+    if (!wait_geq(&B.state[tri_index(j, j)], epoch, B.abortf)) return false;
+    stamp(B.trace, t, 2);
+    load_tile<Coh::Dev>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
+    __syncthreads();
+    Acc<T> X;
+    acc_zero(X);
+    mfma_acc<false>(X, SA, SB, NB);
+    __syncthreads();
+    acc_to_lds(X, SA);
+    __syncthreads();
+    store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
+    publish(&B.state[tri_index(i, j)], epoch);
+  } else {
+    // update-matrix tile: final once every factor column has been subtracted (FP64 for the
+    // parent's extend-add)
+    acc_to_lds(C, SA);
+    __syncthreads();
+    stamp(B.trace, t, 2);
+    store_tile<Coh::Dev>(F, LD, r0i, r0j, hi, hj, SA, i == j);
+  }
+  stamp(B.trace, t, 3);
+  return true;
+}
+
 template <class T>
 __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* insts, int epoch) {
   __shared__ __align__(16) T SA[NB * LROW];
@@ -115,10 +234,6 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
   __shared__ int task_s;
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
-  double* F = B.F;
-  T* Lf = factor_of<T>(B);
-  T* Di = dinv_of<T>(B);
-  const int LD = B.ld;
   for (;;) {
     if (threadIdx.x == 0) {
       int t = atomicAdd(B.ticket, 1);
@@ -129,115 +244,36 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     const int t = task_s;
     __syncthreads();
     if (t >= B.ntask) break;
-    stamp(B.trace, t, 0);
-    const int4 task = B.tasks[t];
-    const int type = task.x, i = task.y, j = task.z, kmax = task.w;
-    const int r0i = B.tb[i], hi = B.tb[i + 1] - r0i;
-    const int r0j = B.tb[j], hj = B.tb[j + 1] - r0j;
-    // the assembled tile (written by earlier launches; only this task writes it here)
-    load_tile<Coh::None>(SA, F, LD, r0i, r0j, hi, hj);
-    __syncthreads();
-    Acc<T> C;
-    acc_from_lds(C, SA);
-    __syncthreads();
-    // C −= Σ_k L_ik L_jkᵀ, software pipelined: the tiles of column k+1 are in flight while the
-    // MFMAs of column k run from the other LDS buffer pair
-    bool alive = true;
-    TileRegs<T> RA, RB;
-    if (kmax > 0) {
-      if (!wait_geq(&B.state[tri_index(i, 0)], epoch, B.abortf) ||
-          (i != j && !wait_geq(&B.state[tri_index(j, 0)], epoch, B.abortf)))
-        break;
-      const int c00 = B.tb[0], w0 = B.tb[1] - c00;
-      fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c00, hi, w0);
-      if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c00, hj, w0);
-    }
-    for (int k = 0; k < kmax; ++k) {
-      T* XA = (k & 1) ? SC : SA;
-      T* XB = (k & 1) ? SD : SB;
-      put_tile(XA, RA);
-      if (i != j) put_tile(XB, RB);
-      __syncthreads();  // also: the buffer pair written next was last read before this point
-      if (k + 1 < kmax) {
-        if (!wait_geq(&B.state[tri_index(i, k + 1)], epoch, B.abortf) ||
-            (i != j && !wait_geq(&B.state[tri_index(j, k + 1)], epoch, B.abortf))) {
-          alive = false;
-          break;
-        }
-        const int c0n = B.tb[k + 1], wn = B.tb[k + 2] - c0n;
-        fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c0n, hi, wn);
-        if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c0n, hj, wn);
-      }
-      mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
-    }
-    __syncthreads();
-    if (!alive) break;
-    stamp(B.trace, t, 1);
-    if (type == 0) {
-      if (j > 0) {
-        // L_{j,j−1} = C_{j,j−1} L_{j−1,j−1}⁻ᵀ from the staged tile (fetched before the inverse is
-        // ready), then C_jj −= L_{j,j−1} L_{j,j−1}ᵀ
-        const int c0p = B.tb[j - 1], wp = B.tb[j] - c0p;
-        if (!wait_geq(&B.stage[j - 1], epoch, B.abortf)) break;
-        load_tile<Coh::Dev>(SA, Lf, LD, r0j, c0p, hj, wp);
-        if (!wait_geq(&B.state[tri_index(j - 1, j - 1)], epoch, B.abortf)) break;
-        load_tile<Coh::Dev>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
-        __syncthreads();
-        Acc<T> X;
-        acc_zero(X);
-        mfma_acc<false>(X, SA, SB, NB);
-        __syncthreads();
-        acc_to_lds(X, SA);
-        __syncthreads();
-        store_tile<Coh::Dev>(Lf, LD, r0j, c0p, hj, wp, SA, false);
-        mfma_acc<true>(C, SA, SA, wp);
-        publish(&B.state[tri_index(j, j - 1)], epoch);  // the stores drained during the update
-      }
-      stamp(B.trace, t, 2);
-      // factor and invert; L_jj⁻¹ → Dinv and publish, then L_jj → F (read only by later launches)
-      acc_to_lds(C, SA);
-      __syncthreads();
-      bool ok = chol_inv64(SA, SC, SB, hj, B.trace, t);  // SB is free here and lies below 64 KiB
-      stamp(B.trace, t, 6);
-      if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-      store_tile<Coh::Dev>(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
-      publish(&B.state[tri_index(j, j)], epoch);
-      stamp(B.trace, t, 7);
-      store_tile<Coh::Dev>(Lf, LD, r0j, r0j, hj, hj, SA, true);
-    } else if (type == 3) {
-      // sub-diagonal tile: hand the accumulated tile to the next diagonal task
-      acc_to_lds(C, SA);
-      __syncthreads();
-      stamp(B.trace, t, 2);
-      store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
-      publish(&B.stage[j], epoch);
-    } else if (type == 1) {
-      // L_ij = C_ij L_jj⁻ᵀ
-      acc_to_lds(C, SA);
-      if (!wait_geq(&B.state[tri_index(j, j)], epoch, B.abortf)) break;
-      stamp(B.trace, t, 2);
-      load_tile<Coh::Dev>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
-      __syncthreads();
-      Acc<T> X;
-      acc_zero(X);
-      mfma_acc<false>(X, SA, SB, NB);
-      __syncthreads();
-      acc_to_lds(X, SA);
-      __syncthreads();
-      store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
-      publish(&B.state[tri_index(i, j)], epoch);
-    } else {
-      // update-matrix tile: final once every factor column has been subtracted (FP64 for the
-      // parent's extend-add)
-      acc_to_lds(C, SA);
-      __syncthreads();
-      stamp(B.trace, t, 2);
-      store_tile<Coh::Dev>(F, LD, r0i, r0j, hi, hj, SA, i == j);
-    }
-    stamp(B.trace, t, 3);
+    if (!dag_task<T>(B, I, epoch, t, SA, SB, SC, SD)) break;
   }
 }
 
+// The large fronts of several instances (a gang) in one persistent launch: ticket g is task
+// order[g] = (front, task) of the list; each front's tasks appear in their own topological
+// order, so every wait is on a smaller ticket, as in the single-front kernel. A front whose
+// instance is not taking part, or whose abort flag is raised, has its remaining tasks skipped;
+// the other fronts go on. The ticket goes back to 0 in k_big_mtiles_multi.
+template <class T>
+__global__ void __launch_bounds__(NT) k_dag_factor_multi(const BigFront* Bs, const int2* order, int ntot,
+                                                         int* ticket, const InstDev* insts, int epoch) {
+  __shared__ __align__(16) T SA[NB * LROW];
+  __shared__ __align__(16) T SB[NB * LROW];
+  __shared__ __align__(16) T SC[NB * LROW];
+  __shared__ __align__(16) T SD[NB * LROW];
+  __shared__ int task_s;
+  for (;;) {
+    if (threadIdx.x == 0) task_s = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int g = __builtin_amdgcn_readfirstlane(task_s);
+    __syncthreads();
+    if (g >= ntot) break;
+    const int2 o = order[g];
+    const BigFront& B = Bs[o.x];
+    const InstDev& I = insts[B.inst];
+    if (I.sc[S_ACTIVE] == 0.0 || ld_flag(B.abortf) != 0) continue;
+    (void)dag_task<T>(B, I, epoch, o.y, SA, SB, SC, SD);
+  }
+}
 
 // ------------------------------------------------------------------------------------------
 // pipelined triangular solves of one big front
@@ -264,6 +300,28 @@ __global__ void k_big_gather(const FrontDesc* fronts, int f, const int* childlis
   }
 }
 
+
+// the gathers of every front of a gang launch, one workgroup each
+__global__ void k_big_gather_multi(const int* bfront, const FrontDesc* fronts, const int* childlist,
+                                   const InstDev* insts, int kv) {
+  const FrontDesc& Fd = fronts[bfront[blockIdx.x]];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
+  double* u = Fd.uvec;
+  int r = Fd.r;
+  for (int k = threadIdx.x; k < Fd.s; k += blockDim.x) u[k] = 0.0;
+  __syncthreads();
+  for (int c = 0; c < Fd.nchild; ++c) {
+    const FrontDesc& C = fronts[childlist[Fd.child0 + c]];
+    for (int k = threadIdx.x; k < C.s; k += blockDim.x) {
+      int p = C.relidx[k];
+      if (p < r) v[p] -= C.uvec[k];
+      else u[p - r] += C.uvec[k];
+    }
+    __syncthreads();
+  }
+}
 
 // Solution blocks pass between the solve workgroups as (value, epoch) words: entry r of block b
 // is two 64-bit words, each one 32-bit half of x_b[r] beside the launch's epoch, stored and loaded
@@ -403,16 +461,10 @@ __device__ __forceinline__ void lds_tile64(double* S, const TS* F, int LD, int r
 // x_b = w_b − M_b x_{b−1} (forward), x_b = w_b − N_bᵀ x_{b+1} (backward), where w_b is formed
 // from everything else while the workgroup waits. One workgroup per block.
 template <class TF>
-__global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* insts) {
-  __shared__ __align__(16) double SX[NB * LDT];
-  __shared__ __align__(16) double SY[NB * LDT];
-  // the factorization that just ended took its last ticket: the next one starts from 0
-  if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
-  const InstDev& I = insts[B.inst];
-  if (I.sc[S_ACTIVE] == 0.0) return;
+__device__ __forceinline__ void mtiles_body(const BigFront& B, int blk, double* SX, double* SY) {
   // workgroups [0, Tr) form M_b, workgroups [Tr, 2 Tr) form N_b
-  const bool mpart = (int)blockIdx.x < B.Tr;
-  const int b = mpart ? (int)blockIdx.x : (int)blockIdx.x - B.Tr;
+  const bool mpart = blk < B.Tr;
+  const int b = mpart ? blk : blk - B.Tr;
   if (b >= B.Tr) return;
   const TF* F = factor_of<TF>(B);
   const TF* Di = dinv_of<TF>(B) + (size_t)b * NB * NB;
@@ -449,6 +501,31 @@ __global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* in
   }
 }
 
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* insts) {
+  __shared__ __align__(16) double SX[NB * LDT];
+  __shared__ __align__(16) double SY[NB * LDT];
+  // the factorization that just ended took its last ticket: the next one starts from 0
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.ticket = 0;
+  const InstDev& I = insts[B.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  mtiles_body<TF>(B, (int)blockIdx.x, SX, SY);
+}
+
+// the chain-step tiles of every front of a gang launch (grid: 2·max Tr × fronts), and the gang's
+// factorization ticket back to 0
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_mtiles_multi(const BigFront* Bs, int* ticket, const InstDev* insts) {
+  __shared__ __align__(16) double SX[NB * LDT];
+  __shared__ __align__(16) double SY[NB * LDT];
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *ticket = 0;
+  const BigFront& B = Bs[blockIdx.y];
+  if ((int)blockIdx.x >= 2 * B.Tr) return;
+  const InstDev& I = insts[B.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  mtiles_body<TF>(B, (int)blockIdx.x, SX, SY);
+}
+
 // forward: tile b (ticket order) accumulates L[b, j] x_j for the complete j < b−1 while the
 // earlier tiles solve; the factor tiles stream through registers one block ahead of the solution,
 // and every block's contribution is summed in block order whatever the batches (reproducible
@@ -456,20 +533,8 @@ __global__ void __launch_bounds__(NT) k_big_mtiles(BigFront B, const InstDev* in
 // finishes x_b = w_b − M_b x_{b−1} after it. A separator tile adds its Σ over all factor blocks
 // into the front's update vector. TF: element type of the stored factor; the arithmetic is FP64.
 template <class TF>
-__global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
-                                                   int kv, int epoch, long long base) {
-  __shared__ double Sinv[NB * LDT];
-  __shared__ double xs[4 * NB];
-  __shared__ double part[4][NB];
-  __shared__ long long tk;
-  const FrontDesc& Fd = fronts[f];
-  const InstDev& I = insts[Fd.inst];
-  // every workgroup consumes exactly one ticket (the host advances the base by the grid size)
-  if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
-  __syncthreads();
-  if (I.sc[S_ACTIVE] == 0.0) return;
-  const int b = (int)tk;
-  if (b < 0 || b >= B.T) return;
+__device__ __forceinline__ void fsolve_body(const BigFront& B, const FrontDesc& Fd, const InstDev& I, int kv,
+                                            int epoch, int b, double* Sinv, double* xs, double (*part)[NB]) {
   double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
   const TF* F = factor_of<TF>(B);
   const int LD = B.ld;
@@ -515,19 +580,54 @@ __global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* 
   if (row < h) v[r0 + row] = x;
 }
 
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_fsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
+                                                   int kv, int epoch, long long base) {
+  __shared__ double Sinv[NB * LDT];
+  __shared__ double xs[4 * NB];
+  __shared__ double part[4][NB];
+  __shared__ long long tk;
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  // every workgroup consumes exactly one ticket (the host advances the base by the grid size)
+  if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
+  __syncthreads();
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int b = (int)tk;
+  if (b < 0 || b >= B.T) return;
+  fsolve_body<TF>(B, Fd, I, kv, epoch, b, Sinv, xs, part);
+}
+
+// The forward solves of the large fronts of a gang launch: ticket g is block order[g].y of front
+// order[g].x (each front's blocks in increasing order, so a block's inputs are claimed before it);
+// bfront: each listed front's index in the front table. One ticket per workgroup.
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_fsolve_multi(const BigFront* Bs, const int* bfront, const int2* order,
+                                                         int ntot, unsigned long long* ticket, const FrontDesc* fronts,
+                                                         const InstDev* insts, int kv, int epoch, long long base) {
+  __shared__ double Sinv[NB * LDT];
+  __shared__ double xs[4 * NB];
+  __shared__ double part[4][NB];
+  __shared__ long long tk;
+  if (threadIdx.x == 0) tk = (long long)atomicAdd(ticket, 1ull) - base;
+  __syncthreads();
+  const int g = (int)tk;
+  if (g < 0 || g >= ntot) return;
+  const int2 o = order[g];
+  const BigFront& B = Bs[o.x];
+  const FrontDesc& Fd = fronts[bfront[o.x]];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  fsolve_body<TF>(B, Fd, I, kv, epoch, o.y, Sinv, xs, part);
+}
+
 // L[sep, b]ᵀ x_sep for one factor column block b and one 64-row chunk of the separator rows
 // (grid: Tr × ⌈s/64⌉): the tile goes through LDS so that the global reads run down its columns.
 // Separator solution values are final before the backward solve of a front starts.
 template <class TF>
-__global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
-                                                    int kv) {
-  __shared__ double S[NB * LDT];
-  __shared__ double xsep[NB];
-  __shared__ double part[4][NB];
-  const FrontDesc& Fd = fronts[f];
-  const InstDev& I = insts[Fd.inst];
-  if (I.sc[S_ACTIVE] == 0.0) return;
-  const int b = blockIdx.x, rs0 = Fd.r + (int)blockIdx.y * NB;
+__device__ __forceinline__ void sepgemv_body(const BigFront& B, const FrontDesc& Fd, const InstDev& I, int kv, int b,
+                                             int chunk, double* S, double* xsep, double (*part)[NB]) {
+  const int rs0 = Fd.r + chunk * NB;
   const int h = min(NB, B.ld - rs0), c0 = B.tb[b], w = B.tb[b + 1] - c0;
   const double* vall = I.nv + (size_t)kv * I.n;
   load_tile<Coh::None>(S, factor_of<TF>(B), B.ld, rs0, c0, h, w);
@@ -540,7 +640,34 @@ __global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc*
   part[q][col] = a;
   __syncthreads();
   if (threadIdx.x < NB)
-    B.spart[((size_t)blockIdx.y * B.Tr + b) * NB + col] = (part[0][col] + part[1][col]) + (part[2][col] + part[3][col]);
+    B.spart[((size_t)chunk * B.Tr + b) * NB + col] = (part[0][col] + part[1][col]) + (part[2][col] + part[3][col]);
+}
+
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
+                                                    int kv) {
+  __shared__ double S[NB * LDT];
+  __shared__ double xsep[NB];
+  __shared__ double part[4][NB];
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  sepgemv_body<TF>(B, Fd, I, kv, (int)blockIdx.x, (int)blockIdx.y, S, xsep, part);
+}
+
+// the separator products of every front of a gang launch (grid: max Tr × max chunks × fronts)
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_sepgemv_multi(const BigFront* Bs, const int* bfront, const FrontDesc* fronts,
+                                                          const InstDev* insts, int kv) {
+  __shared__ double S[NB * LDT];
+  __shared__ double xsep[NB];
+  __shared__ double part[4][NB];
+  const BigFront& B = Bs[blockIdx.z];
+  const FrontDesc& Fd = fronts[bfront[blockIdx.z]];
+  if ((int)blockIdx.x >= B.Tr || (int)blockIdx.y * NB >= B.s) return;
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  sepgemv_body<TF>(B, Fd, I, kv, (int)blockIdx.x, (int)blockIdx.y, S, xsep, part);
 }
 
 // backward: column block b = Tr−1−ticket accumulates L[j, b]ᵀ x_j for the complete j > b+1:
@@ -549,20 +676,9 @@ __global__ void __launch_bounds__(NT) k_big_sepgemv(BigFront B, const FrontDesc*
 // of k_big_sepgemv) it forms w_b = L_bbᵀ⁻¹ (v_b − Σ) before block b+1 arrives, and wave 0
 // finishes x_b = w_b − N_bᵀ x_{b+1} after it.
 template <class TF>
-__global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
-                                                   int kv, int epoch, long long base) {
-  __shared__ double S[NB * LDT];
-  __shared__ double Sinv[NB * LDT];
-  __shared__ double xs[4 * NB];
-  __shared__ double part[4][NB];
-  __shared__ long long tk;
-  const FrontDesc& Fd = fronts[f];
-  const InstDev& I = insts[Fd.inst];
-  if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
-  __syncthreads();
-  if (I.sc[S_ACTIVE] == 0.0) return;
-  const int tkt = (int)tk;
-  if (tkt < 0 || tkt >= B.Tr) return;
+__device__ __forceinline__ void bsolve_body(const BigFront& B, const FrontDesc& Fd, const InstDev& I, int kv,
+                                            int epoch, int tkt, double* S, double* Sinv, double* xs,
+                                            double (*part)[NB]) {
   const int b = B.Tr - 1 - tkt;
   double* v = (I.nv + (size_t)kv * I.n) + Fd.res0;
   const TF* F = factor_of<TF>(B);
@@ -624,6 +740,47 @@ __global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* 
   if (lane < w) v[c0 + lane] = x;
 }
 
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_bsolve(BigFront B, const FrontDesc* fronts, int f, const InstDev* insts,
+                                                   int kv, int epoch, long long base) {
+  __shared__ double S[NB * LDT];
+  __shared__ double Sinv[NB * LDT];
+  __shared__ double xs[4 * NB];
+  __shared__ double part[4][NB];
+  __shared__ long long tk;
+  const FrontDesc& Fd = fronts[f];
+  const InstDev& I = insts[Fd.inst];
+  if (threadIdx.x == 0) tk = (long long)atomicAdd((unsigned long long*)B.sticket, 1ull) - base;
+  __syncthreads();
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const int tkt = (int)tk;
+  if (tkt < 0 || tkt >= B.Tr) return;
+  bsolve_body<TF>(B, Fd, I, kv, epoch, tkt, S, Sinv, xs, part);
+}
+
+// The backward solves of the large fronts of a gang launch: ticket g is the order[g].y-th block
+// from the last of front order[g].x
+template <class TF>
+__global__ void __launch_bounds__(NT) k_big_bsolve_multi(const BigFront* Bs, const int* bfront, const int2* order,
+                                                         int ntot, unsigned long long* ticket, const FrontDesc* fronts,
+                                                         const InstDev* insts, int kv, int epoch, long long base) {
+  __shared__ double S[NB * LDT];
+  __shared__ double Sinv[NB * LDT];
+  __shared__ double xs[4 * NB];
+  __shared__ double part[4][NB];
+  __shared__ long long tk;
+  if (threadIdx.x == 0) tk = (long long)atomicAdd(ticket, 1ull) - base;
+  __syncthreads();
+  const int g = (int)tk;
+  if (g < 0 || g >= ntot) return;
+  const int2 o = order[g];
+  const BigFront& B = Bs[o.x];
+  const FrontDesc& Fd = fronts[bfront[o.x]];
+  const InstDev& I = insts[Fd.inst];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  bsolve_body<TF>(B, Fd, I, kv, epoch, o.y, S, Sinv, xs, part);
+}
+
 // ------------------------------------------------------------------------------------------
 
 // The ticket starts at 0 (zeroed with the front's buffers, then put back by k_big_mtiles after
@@ -646,6 +803,38 @@ int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int
   if (fp32) k_big_fsolve<float><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
   else k_big_fsolve<double><<<grid, NT, 0, st>>>(B, F, f, I, kv, epoch, base);
   return grid;
+}
+
+void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const int2* order, int ntot, int* ticket,
+                             const InstDev* I, int grid, int epoch, bool fp32, int max2tr) {
+  if (ntot <= 0) return;
+  if (fp32) k_dag_factor_multi<float><<<grid, NT, 0, st>>>(Bs, order, ntot, ticket, I, epoch);
+  else k_dag_factor_multi<double><<<grid, NT, 0, st>>>(Bs, order, ntot, ticket, I, epoch);
+  const dim3 mg(std::max(1, max2tr), nf);
+  if (fp32) k_big_mtiles_multi<float><<<mg, NT, 0, st>>>(Bs, ticket, I);
+  else k_big_mtiles_multi<double><<<mg, NT, 0, st>>>(Bs, ticket, I);
+}
+
+void launch_big_fsolve_multi(hipStream_t st, const BigFront* Bs, const int* bfront, int nf, const int2* order,
+                             int ntot, unsigned long long* ticket, const FrontDesc* F, const int* childlist,
+                             const InstDev* I, int kv, int epoch, long long base, bool fp32) {
+  if (ntot <= 0) return;
+  k_big_gather_multi<<<nf, 256, 0, st>>>(bfront, F, childlist, I, kv);
+  if (fp32) k_big_fsolve_multi<float><<<ntot, NT, 0, st>>>(Bs, bfront, order, ntot, ticket, F, I, kv, epoch, base);
+  else k_big_fsolve_multi<double><<<ntot, NT, 0, st>>>(Bs, bfront, order, ntot, ticket, F, I, kv, epoch, base);
+}
+
+void launch_big_bsolve_multi(hipStream_t st, const BigFront* Bs, const int* bfront, int nf, const int2* order,
+                             int ntot, unsigned long long* ticket, const FrontDesc* F, const InstDev* I, int kv,
+                             int epoch, long long base, bool fp32, int maxtr, int maxch) {
+  if (ntot <= 0) return;
+  if (maxch > 0) {
+    const dim3 sg(std::max(1, maxtr), maxch, nf);
+    if (fp32) k_big_sepgemv_multi<float><<<sg, NT, 0, st>>>(Bs, bfront, F, I, kv);
+    else k_big_sepgemv_multi<double><<<sg, NT, 0, st>>>(Bs, bfront, F, I, kv);
+  }
+  if (fp32) k_big_bsolve_multi<float><<<ntot, NT, 0, st>>>(Bs, bfront, order, ntot, ticket, F, I, kv, epoch, base);
+  else k_big_bsolve_multi<double><<<ntot, NT, 0, st>>>(Bs, bfront, order, ntot, ticket, F, I, kv, epoch, base);
 }
 
 int launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,

@@ -13,8 +13,12 @@ double Server::now() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-Server::Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis)
+Server::Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis, bool gang)
     : nslots_(std::max(1, nslots)), dag_grid_(dag_grid), budget_(mem_budget_bytes) {
+  if (gang) {
+    nslots_ = std::min(nslots_, GANG_MAX);
+    gang_ = std::make_unique<Gang>(nslots_, 0, false);
+  }
   if (budget_ <= 0.0) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
@@ -25,7 +29,8 @@ Server::Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis)
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       ncu = 256;
-    dag_grid_ = std::max(16, ncu / nslots_);
+    // a gang's members run one at a time on its stream: each may take every CU
+    dag_grid_ = gang_ ? ncu : std::max(16, ncu / nslots_);
   }
   if (nanalysis <= 0) nanalysis = std::min(8, std::max(2, (int)std::thread::hardware_concurrency() / 2));
   lookahead_ = (size_t)(2 * nslots_ + nanalysis);
@@ -43,6 +48,7 @@ Server::~Server() {
   cv_room_.notify_all();
   for (auto& t : workers_) t.join();
   for (auto& t : slots_) t.join();
+  gang_.reset();
 }
 
 int64_t Server::submit(const Problem* prob, const Settings& st) {
@@ -175,7 +181,7 @@ void Server::slot_main(int slot) {
       job->t_admit = now();
       S.device_setup(nullptr, nullptr, nullptr);
       job->bytes = (double)S.device_bytes();
-      job->status = S.solve();
+      job->status = gang_ ? S.solve_in(*gang_) : S.solve();
       job->p.resize(S.n());
       job->d.resize(S.n());
       job->y.resize(S.m());

@@ -21,6 +21,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "gang.h"
 #include "host.h"
 #include "solver.h"
 
@@ -44,8 +45,9 @@ struct ServeJob {
 
 class Server {
  public:
-  // nanalysis ≤ 0: one analysis worker per two hardware threads (at least two)
-  Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis = 0);
+  // nanalysis ≤ 0: one analysis worker per two hardware threads (at least two); gang: the slots
+  // are the members of one gang (their solves advance in lockstep and share every launch)
+  Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis = 0, bool gang = false);
   ~Server();  // finishes the queued and running requests, then joins the threads
   int64_t submit(const Problem* prob, const Settings& st);
   // waits up to timeout_s (< 0: no limit); returns the job once done, nullptr on timeout
@@ -75,6 +77,7 @@ class Server {
   std::map<int64_t, std::shared_ptr<ServeJob>> ready_;    // analysed, by ticket
   std::unordered_map<int64_t, std::shared_ptr<ServeJob>> jobs_;
   std::vector<std::thread> workers_, slots_;
+  std::unique_ptr<Gang> gang_;
 };
 
 }  // namespace mb

@@ -1,8 +1,10 @@
 // IPM driver over device-resident state (see solver.h).
 #include "solver.h"
+#include "gang.h"
 #include "tdc.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -12,6 +14,11 @@
 #include <stdexcept>
 
 namespace mb {
+
+int next_epoch() {
+  static std::atomic<int> e{0};
+  return ++e;
+}
 
 static const double EPS = std::numeric_limits<double>::epsilon();
 static const double NaN = std::numeric_limits<double>::quiet_NaN();
@@ -193,6 +200,16 @@ void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
   if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));  // synchronous: callers pass temporaries
 }
+
+// the element types other translation units upload (the gang's combined tables)
+template void DevBuf::upload<int>(const std::vector<int>&, hipStream_t);
+template void DevBuf::upload<Tile>(const std::vector<Tile>&, hipStream_t);
+template void DevBuf::upload<InstDev>(const std::vector<InstDev>&, hipStream_t);
+template void DevBuf::upload<ConeDesc>(const std::vector<ConeDesc>&, hipStream_t);
+template void DevBuf::upload<FrontDesc>(const std::vector<FrontDesc>&, hipStream_t);
+template void DevBuf::upload<StepDesc>(const std::vector<StepDesc>&, hipStream_t);
+template void DevBuf::upload<BigFront>(const std::vector<BigFront>&, hipStream_t);
+template void DevBuf::upload<int2>(const std::vector<int2>&, hipStream_t);
 
 // pinned host blocks for the setup uploads, kept in size classes like the device blocks
 namespace {
@@ -726,6 +743,7 @@ void Solver::build_device() {
   d_sepcols_.upload(allsep, st_);
   d_relidx_.upload(relidx, st_);
   d_childlist_.upload(childlist, st_);
+  hl_.childlist = childlist;
   for (i64 f = 0; f < NF; ++f) {
     fr_[f].F = d_F_.as<double>() + fofs[f];
     fr_[f].F32 = nullptr;
@@ -777,6 +795,10 @@ void Solver::build_device() {
     if (npsd_) d_psdlist_.upload(psd, st_);
     if (ndense_) d_denselist_.upload(dense, st_);
     if (ntd_) d_tdlist_.upload(td, st_);
+    hl_.psd = psd;
+    hl_.dense = dense;
+    hl_.soc = soc;
+    hl_.td = td;
   }
   d_cone_front_.upload(cone_front_, st_);
   d_cone_ofs_.upload(std::vector<int>{0, (int)N}, st_);
@@ -798,6 +820,7 @@ void Solver::build_device() {
       if (p[r + 1] - p[r] > SPMV_LONG * G) t.push_back(Tile{0, (int)(-1 - r)});
     nsptile_[which] = (int)t.size();
     if (!t.empty()) d_sptiles_[which].upload(t, st_);
+    hl_.sp[which] = std::move(t);
   };
   d_Bp_.upload(brp_, st_);
   d_Bc_.upload(brc_, st_);
@@ -824,6 +847,8 @@ void Solver::build_device() {
   ntile_m_ = (int)tm.size();
   d_ntiles_.upload(tn, st_);
   d_mtiles_.upload(tm, st_);
+  hl_.tn = tn;
+  hl_.tm = tm;
   d_ntofs_.upload(std::vector<int>{0, ntile_n_}, st_);
   d_mtofs_.upload(std::vector<int>{0, ntile_m_}, st_);
   d_partial_.alloc(sizeof(double) * (std::max<i64>({(i64)ntile_n_, (i64)ntile_m_, NF, 1}) + 16));
@@ -1032,6 +1057,10 @@ void Solver::build_schedule() {
   d_ea_items_.upload(ea_items, st_);
   d_ea_tile0_.upload(ea_tile0, st_);
   d_solve_items_.upload(solve_items, st_);
+  hl_.steps = steps;
+  hl_.ea_items = ea_items;
+  hl_.ea_tile0 = ea_tile0;
+  hl_.solve_items = solve_items;
   HIPCHK(hipStreamSynchronize(st_));
   // the large fronts of one level are independent: up to NAUX at a time, each on a stream of its
   // own with a share of the CUs (a chordal clique tree has levels of tens of them)
@@ -1161,7 +1190,20 @@ double Solver::front_flops() const {
 // device helpers
 // ------------------------------------------------------------------------------------------
 
+void Solver::gang_submit(GOp& op) { gang_->submit(this, op); }
+
 void Solver::sync_scalars() {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_READBACK;
+    op.solo = [this] { raw_sync_scalars(); };
+    gang_submit(op);
+    return;
+  }
+  raw_sync_scalars();
+}
+
+void Solver::raw_sync_scalars() {
   HIPCHK(hipMemcpyAsync(hsc_, d_sc_.p, NSCALAR * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   prof_flush();
@@ -1169,17 +1211,86 @@ void Solver::sync_scalars() {
 
 void Solver::set_scalar(int slot, double v) {
   hsc_[slot] = v;
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SET2;
+    op.a[0] = slot;
+    op.a[1] = -1;
+    op.v[0] = v;
+    op.solo = [this, slot, v] { launch_set_scalar(st_, d_sc_.as<double>(), slot, v); };
+    gang_submit(op);
+    return;
+  }
   // the value travels as a kernel argument: no host buffer outlives the call
   launch_set_scalar(st_, d_sc_.as<double>(), slot, v);
 }
 
+void Solver::set_scalar2(int s1, double v1, int s2, double v2) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SET2;
+    op.a[0] = s1;
+    op.a[1] = s2;
+    op.v[0] = v1;
+    op.v[1] = v2;
+    op.solo = [this, s1, v1, s2, v2] { launch_set_scalar2(st_, d_sc_.as<double>(), s1, v1, s2, v2); };
+    gang_submit(op);
+    return;
+  }
+  launch_set_scalar2(st_, d_sc_.as<double>(), s1, v1, s2, v2);
+}
+
+void Solver::smu(double nu, double mu, double mut) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SMU;
+    op.v[0] = nu;
+    op.v[1] = mu;
+    op.v[2] = mut;
+    op.solo = [this, nu, mu, mut] { launch_smu(st_, d_sc_.as<double>(), nu, mu, mut); };
+    gang_submit(op);
+    return;
+  }
+  launch_smu(st_, d_sc_.as<double>(), nu, mu, mut);
+}
+
+void Solver::cg_scalars(int which) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_CGSC;
+    op.a[0] = which;
+    op.solo = [this, which] { launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, which); };
+    gang_submit(op);
+    return;
+  }
+  launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, which);
+}
+
 void Solver::axpby(int space, int kx, int ky, int kz, Coef a, Coef b) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_AXPBY;
+    op.a[0] = space, op.a[1] = kx, op.a[2] = ky, op.a[3] = kz;
+    op.c[0] = a, op.c[1] = b;
+    op.solo = [=] { axpby(space, kx, ky, kz, a, b); };
+    gang_submit(op);
+    return;
+  }
   const InstDev* I = d_inst_.as<InstDev>();
   if (space == 0) launch_axpby(st_, I, d_ntiles_.as<Tile>(), ntile_n_, 0, kx, ky, kz, a, b);
   else launch_axpby(st_, I, d_mtiles_.as<Tile>(), ntile_m_, 1, kx, ky, kz, a, b);
 }
 
 void Solver::axpbypcz(int space, int kx, int ky, int kw, int kz, Coef a, Coef b, Coef c) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_AXPBYPCZ;
+    op.a[0] = space, op.a[1] = kx, op.a[2] = ky, op.a[3] = kw, op.a[4] = kz;
+    op.c[0] = a, op.c[1] = b, op.c[2] = c;
+    op.solo = [=] { axpbypcz(space, kx, ky, kw, kz, a, b, c); };
+    gang_submit(op);
+    return;
+  }
   const InstDev* I = d_inst_.as<InstDev>();
   if (space == 0) launch_axpbypcz(st_, I, d_ntiles_.as<Tile>(), ntile_n_, 0, kx, ky, kw, kz, a, b, c);
   else launch_axpbypcz(st_, I, d_mtiles_.as<Tile>(), ntile_m_, 1, kx, ky, kw, kz, a, b, c);
@@ -1197,6 +1308,15 @@ void Solver::dot(int space, int kx, int ky, int mode, int slot, int op, int kw0,
 
 // several dot products in one launch while the vectors are short enough for one workgroup each
 void Solver::dots(const DotBatch& b) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_DOTS;
+    op.db = b;
+    op.solo_only = std::max(n_, m_) > DOTS_LEN_MAX;
+    op.solo = [this, b] { dots(b); };
+    gang_submit(op);
+    return;
+  }
   if (std::max(n_, m_) <= DOTS_LEN_MAX) {
     launch_dots(st_, d_inst_.as<InstDev>(), 0, b);
     return;
@@ -1208,6 +1328,15 @@ void Solver::dots(const DotBatch& b) {
 }
 
 void Solver::lin(const LinSet& N, const LinSet& M) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_LIN;
+    op.N = N;
+    op.M = M;
+    op.solo = [this, N, M] { lin(N, M); };
+    gang_submit(op);
+    return;
+  }
   launch_lin(st_, d_inst_.as<InstDev>(), d_ntiles_.as<Tile>(), ntile_n_, d_mtiles_.as<Tile>(), ntile_m_, N, M);
 }
 
@@ -1221,17 +1350,130 @@ void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
   if (N && N->kxq >= 0 && N->kxt >= 0) ntm = idev_.Bt.lanes > idev_.Q.lanes ? 1 : 2;
   else if (N && N->kxq >= 0) ntm = 2;
   else if (N && N->kxt >= 0) ntm = 1;
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SPMVNM;
+    op.a[0] = ntm;
+    op.a[1] = N ? 1 : 0;
+    op.a[2] = M ? 1 : 0;
+    op.sn = N ? *N : n0;
+    op.sm = M ? *M : m0;
+    const SpmvN nn = op.sn;
+    const SpmvM mm = op.sm;
+    const bool hn = N != nullptr, hm = M != nullptr;
+    op.solo = [=] { launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), hn ? nsptile_[ntm] : 0,
+                                   d_sptiles_[0].as<Tile>(), hm ? nsptile_[0] : 0, nn, mm); };
+    gang_submit(op);
+    return;
+  }
   launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), N ? nsptile_[ntm] : 0,
                  d_sptiles_[0].as<Tile>(), M ? nsptile_[0] : 0, N ? *N : n0, M ? *M : m0);
 }
 
 void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SPMV;
+    op.a[0] = which, op.a[1] = kx, op.a[2] = ky;
+    op.c[0] = a, op.c[1] = b;
+    op.solo = [=] { spmv(which, kx, ky, a, b); };
+    gang_submit(op);
+    return;
+  }
   const InstDev* I = d_inst_.as<InstDev>();
   launch_spmv(st_, I, d_sptiles_[which].as<Tile>(), nsptile_[which], which, kx, ky, a, b);
 }
 
+// scale! (cone.jl:146-247) with the cone blocks' part of ‖H‖² (into S_TMP1)
+void Solver::cone_scale_norm() {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SCALE;
+    op.solo_only = nbpsd_ > 0;
+    op.solo = [this] { cone_scale_norm(); };
+    gang_submit(op);
+    return;
+  }
+  launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
+                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd(), d_soclist_.as<int>(), nsoc_);
+  launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
+                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
+                     d_dpart_.as<double>(), bpsd(), d_bpart_.as<double>());
+}
+
+// initpredictor! with μt > 0: the right-hand side r = −d + σμ(−f′(p)) per cone into kind kr
+void Solver::corr0_op(int kr) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_CORR0;
+    op.a[0] = kr;
+    op.solo_only = nbpsd_ > 0;
+    op.solo = [this, kr] { corr0_op(kr); };
+    gang_submit(op);
+    return;
+  }
+  launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kr);
+  launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, kr);
+  launch_bpsd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), kr);
+}
+
+// initcorrector! per cone into kind kr from the affine direction (kdp, kdd)
+void Solver::corr_op(int kr, int kdp, int kdd) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_CORR;
+    op.a[0] = kr, op.a[1] = kdp, op.a[2] = kdd;
+    op.solo_only = nbpsd_ > 0;
+    op.solo = [this, kr, kdp, kdd] { corr_op(kr, kdp, kdd); };
+    gang_submit(op);
+    return;
+  }
+  launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kr, kdp, kdd);
+  launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, kr, kdp, kdd);
+  launch_bpsd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), kr, kdp, kdd);
+}
+
+void Solver::zerofree(int kx) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_ZEROFREE;
+    op.a[0] = kx;
+    op.solo = [this, kx] { zerofree(kx); };
+    gang_submit(op);
+    return;
+  }
+  launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx);
+}
 
 void Solver::lsolve(int kv) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_LSOLVE;
+    op.park = true;
+    op.a[0] = kv;
+    op.a[1] = fac_fp32_ ? 1 : 0;
+    op.solo = [this, kv] { raw_lsolve(kv); };
+    gang_submit(op);
+    return;
+  }
+  raw_lsolve(kv);
+}
+
+void Solver::usolve(int kv) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_USOLVE;
+    op.park = true;
+    op.a[0] = kv;
+    op.a[1] = fac_fp32_ ? 1 : 0;
+    op.solo = [this, kv] { raw_usolve(kv); };
+    gang_submit(op);
+    return;
+  }
+  raw_usolve(kv);
+}
+
+void Solver::raw_lsolve(int kv) {
   for (const Level& L : levels_) {
     int ps = prof_begin(PROF_FSOLVE);
     launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, L.nsmall,
@@ -1240,7 +1482,7 @@ void Solver::lsolve(int kv) {
     if (k) fork_aux(k);
     for (size_t i = 0; i < L.big.size(); ++i) {
       BigHost& bh = *big_[L.big[i]];
-      ++epoch_;
+      epoch_ = next_epoch();
       bh.sbase += launch_big_fsolve(k ? aux_[i % k] : st_, bh.B, d_fr_.as<FrontDesc>(), bh.front,
                                     d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
     }
@@ -1249,7 +1491,7 @@ void Solver::lsolve(int kv) {
   }
 }
 
-void Solver::usolve(int kv) {
+void Solver::raw_usolve(int kv) {
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
     int ps = prof_begin(PROF_BSOLVE);
@@ -1259,7 +1501,7 @@ void Solver::usolve(int kv) {
     if (k) fork_aux(k);
     for (size_t i = 0; i < L.big.size(); ++i) {
       BigHost& bh = *big_[L.big[i]];
-      ++epoch_;
+      epoch_ = next_epoch();
       bh.sbase += launch_big_bsolve(k ? aux_[i % k] : st_, bh.B, d_fr_.as<FrontDesc>(), bh.front,
                                     d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
     }
@@ -1284,13 +1526,13 @@ void Solver::run_factor(bool fp32) {
   const FrontDesc* F = d_fr_.as<FrontDesc>();
   const StepDesc* S = d_steps_.as<StepDesc>();
   const InstDev* I = d_inst_.as<InstDev>();
-  ++fac_epoch_;
+  fac_epoch_ = next_epoch();
   int pf = prof_begin(PROF_FACTOR);
   for (const Level& L : levels_) {
     int pe = prof_begin(PROF_EXTEND);
     for (size_t k = 0; k < L.ea_off.size(); ++k)
       launch_extend_add(st_, F, d_ea_items_.as<int>() + L.ea_off[k], d_ea_tile0_.as<int>() + L.ea_off[k], L.ea_n[k],
-                        L.ea_tiles[k]);
+                        L.ea_tiles[k], I);
     prof_end(pe, PROF_EXTEND, 0.0, 0.0);
     for (size_t k = 0; k < L.pan_off.size(); ++k) {
       int pp = prof_begin(PROF_PANEL);
@@ -1387,61 +1629,104 @@ void Solver::prof_get(double* out) const {
   }
 }
 
+// F ← BᵀB + δH (the solver's H: Q and the cone blocks; or the caller's H)
+void Solver::raw_assemble() {
+  const InstDev* I = d_inst_.as<InstDev>();
+  launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
+  if (user_h_) {  // BᵀB + δ (the caller's H)
+    launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 0);
+    launch_scatter_add(st_, I, 0, d_hpos_.as<int64_t>(), d_hval_.as<double>(), nhent_);
+    return;
+  }
+  launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 1);
+  launch_bpsd_assemble(st_, I, d_cones_.as<ConeDesc>(), bpsd(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>());
+  launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(), ncone_,
+                        cone_nmax_);
+}
+
+// assembly and factorization, timed by events when the solver runs alone
+void Solver::raw_factor_run(bool fp32) {
+  raw_assemble();
+  if (!gang_) HIPCHK(hipEventRecord(ev0_, st_));
+  run_factor(fp32);
+  if (!gang_) HIPCHK(hipEventRecord(ev1_, st_));
+}
+
+void Solver::factor_run(bool fp32) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_FACTOR;
+    op.park = true;
+    op.a[0] = fp32 ? 1 : 0;
+    op.solo_only = user_h_ || nbpsd_ > 0;
+    op.solo = [this, fp32] { raw_factor_run(fp32); };
+    gang_submit(op);
+    return;
+  }
+  raw_factor_run(fp32);
+}
+
 // initkkt! (uzawa.jl:73-115): F Fᵀ = δH + BᵀB, on failure retry with ρ = eps ‖δH + BᵀB‖ I
 bool Solver::factor(double delta, double& rho) {
-  const InstDev* I = d_inst_.as<InstDev>();
-  auto assemble = [&]() {
-    launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
-    if (user_h_) {  // BᵀB + δ (the caller's H)
-      launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 0);
-      launch_scatter_add(st_, I, 0, d_hpos_.as<int64_t>(), d_hval_.as<double>(), nhent_);
-      return;
-    }
-    launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 1);
-    launch_bpsd_assemble(st_, I, d_cones_.as<ConeDesc>(), bpsd(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>());
-    launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(),
-                          ncone_, cone_nmax_);
-  };
-  launch_set_scalar2(st_, d_sc_.as<double>(), S_DELTA, delta, S_FAIL, 0.0);
-  assemble();
+  set_scalar2(S_DELTA, delta, S_FAIL, 0.0);
   // mixed precision: the large fronts in FP32 while μ is above the switch point; an FP32
   // breakdown (non-positive pivot) is redone in FP64 before any diagonal shift
   fac_fp32_ = set_.precision == 1 && !mixed_off_ && !big_.empty() && cur_mu_ > set_.mixed_mu;
-  HIPCHK(hipEventRecord(ev0_, st_));
-  run_factor(fac_fp32_);
-  HIPCHK(hipEventRecord(ev1_, st_));
+  factor_run(fac_fp32_);
   sync_scalars();
   check_abort();
   if (fac_fp32_ && hsc_[S_FAIL] != 0.0) {
     fac_fp32_ = false;
     set_scalar(S_FAIL, 0.0);
-    assemble();
-    HIPCHK(hipEventRecord(ev0_, st_));
-    run_factor(false);
-    HIPCHK(hipEventRecord(ev1_, st_));
+    factor_run(false);
     sync_scalars();
     check_abort();
   }
   if (fac_fp32_) ++fac_fp32_calls_;
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
-  fac_ms_ = ms;
-  fac_ms_tot_ += ms;
+  if (!gang_) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    fac_ms_ = ms;
+    fac_ms_tot_ += ms;
+  }
   fac_calls_ += 1;
   rho = 0.0;
   if (hsc_[S_FAIL] == 0.0) return true;
-  assemble();
-  launch_front_norm2(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), d_partial_.as<double>());
-  std::vector<double> part(fr_.size());
-  HIPCHK(hipMemcpyAsync(part.data(), d_partial_.p, part.size() * 8, hipMemcpyDeviceToHost, st_));
-  HIPCHK(hipStreamSynchronize(st_));
+  // the diagonal shift: ‖F‖ of the assembled matrix, then F + ρI factored again (the solver's own
+  // launches, also inside a gang)
   double s = 0.0;
-  for (double x : part) s += x;
+  auto fnorm = [this, &s] {
+    raw_assemble();
+    launch_front_norm2(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), d_partial_.as<double>());
+    std::vector<double> part(fr_.size());
+    HIPCHK(hipMemcpyAsync(part.data(), d_partial_.p, part.size() * 8, hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    for (double x : part) s += x;
+  };
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SOLO;
+    op.solo = fnorm;
+    gang_submit(op);
+  } else {
+    fnorm();
+  }
   rho = EPS * std::sqrt(s);
   set_scalar(S_RHO, rho);
   set_scalar(S_FAIL, 0.0);
-  launch_add_diag(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), I);
-  run_factor(false);
+  auto refactor = [this] {
+    launch_add_diag(st_, d_fr_.as<FrontDesc>(), (int)fr_.size(), d_inst_.as<InstDev>());
+    run_factor(false);
+  };
+  fac_fp32_ = false;
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_SOLO;
+    op.solo = refactor;
+    gang_submit(op);
+  } else {
+    refactor();
+  }
   sync_scalars();
   check_abort();
   return hsc_[S_FAIL] == 0.0;
@@ -1553,6 +1838,16 @@ double Solver::cone_maxsteps(const double* dp, const double* dd, double step_fra
 
 // the step into S_TAU on the device (no read-back)
 void Solver::launch_maxsteps(int kdp, int kdd, double step_frac) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_MAXSTEP;
+    op.a[0] = kdp, op.a[1] = kdd;
+    op.c[0] = K(step_frac);
+    op.solo_only = nbpsd_ > 0;
+    op.solo = [this, kdp, kdd, step_frac] { launch_maxsteps(kdp, kdd, step_frac); };
+    gang_submit(op);
+    return;
+  }
   launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
                       kdd, step_frac, d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd());
 }
@@ -1751,7 +2046,7 @@ int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& 
                         .add(Lin{ku, ku, ku, MV_CP, K(1.0), K(0.0), K(0.0)})
                         .add(Lin{ku, ku, ku, ky, K(0.0), K(0.0), K(0.0)}));
       status = 0;
-      launch_set_scalar2(st_, d_sc_.as<double>(), S_CG_NU2, nu2, S_CG_NP2, np2);
+      set_scalar2(S_CG_NU2, nu2, S_CG_NP2, np2);
       while (status == 0) {
         niter += 1;
         spmv(1, MV_CP, NV_CW, K(1.0), K(0.0));  // w = Bᵀ p
@@ -1763,7 +2058,7 @@ int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& 
           status = 2;  // CG_NUMERICAL_FAILURE
         } else {
           usolve(NV_CW);
-          launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, 0);  // α = ν²/ε²
+          cg_scalars(0);  // α = ν²/ε²
           // x += α w, y += α p
           lin2(Lin{NV_CW, kx, kx, kx, KS(1.0, S_CG_ALPHA), K(1.0), K(0.0)},
                Lin{MV_CP, ky, ky, ky, KS(1.0, S_CG_ALPHA), K(1.0), K(0.0)});
@@ -1784,7 +2079,7 @@ int Solver::cg(int kx, int ky, int ku, double atol, int itmax, bool stops, int& 
             status = 3;
           } else {
             double beta = nu2 / pnu2;
-            launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, 1);  // β, ν², np² on device
+            cg_scalars(1);  // β, ν², np² on device
             axpby(1, ku, MV_CP, MV_CP, K(1.0), KS(1.0, S_CG_BETA));
             np2 = nu2 + beta * beta * np2;
           }
@@ -1822,10 +2117,7 @@ Solver::KKTOut Solver::solvekkt(int kx, int ky, int kf, int kg, bool warm, doubl
       return;
     }
     if (user_h_) return;
-    launch_cone_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx, NV_KF, K(-1.0));
-    launch_cone_matvec_dense(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_,
-                             dense_nmax_, kx, NV_KF, K(-1.0));
-    launch_bpsd_matvec(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), kx, NV_KF, K(-1.0));
+    cone_matvec_all(kx, NV_KF, K(-1.0));
     if (read) norms();
   };
   // with irmin ≥ 1 and irmax ≥ 1 the starting residual decides nothing (and its norm enters no
@@ -1920,6 +2212,16 @@ void Solver::dshadow(int kx, int ky) {
 
 // y += a (H − Q) x: the cone blocks of H
 void Solver::cone_matvec_all(int kx, int ky, Coef a) {
+  if (ganged()) {
+    GOp op;
+    op.kind = GK_MATVEC;
+    op.a[0] = kx, op.a[1] = ky;
+    op.c[0] = a;
+    op.solo_only = nbpsd_ > 0;
+    op.solo = [this, kx, ky, a] { cone_matvec_all(kx, ky, a); };
+    gang_submit(op);
+    return;
+  }
   const InstDev* I = d_inst_.as<InstDev>();
   launch_cone_matvec(st_, I, d_cones_.as<ConeDesc>(), ncone_, kx, ky, a);
   launch_cone_matvec_dense(st_, I, d_cones_.as<ConeDesc>(), d_denselist_.as<int>(), ndense_, dense_nmax_, kx, ky, a);
@@ -2104,13 +2406,9 @@ int Solver::step() {
   const double mut = mu_t_;
   double delta = getaug();
   row.delta = delta;
-  launch_set_scalar2(st_, d_sc_.as<double>(), S_DELTA, delta, S_ALPHA, 1.0 / delta);
+  set_scalar2(S_DELTA, delta, S_ALPHA, 1.0 / delta);
   // pᵀd, scale!, residuals!
-  launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                    d_coneout_.as<double>(), d_psdlist_.as<int>(), npsd_, bpsd(), d_soclist_.as<int>(), nsoc_);
-  launch_cone_hnorm2(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
-                     d_coneout_.as<double>() + 2 * ncone_, S_TMP1, d_denselist_.as<int>(), ndense_, dense_nmax_,
-                     d_dpart_.as<double>(), bpsd(), d_bpart_.as<double>());
+  cone_scale_norm();
   {
     // Qp ; Δf = d + f − Qp + Bᵀy ; Δg = g − Bp
     SpmvN fn{NV_P, MV_Y, NV_D, NV_F, -1, NV_DF, K(-1.0), K(1.0), K(1.0), K(1.0), K(0.0)};
@@ -2201,9 +2499,7 @@ int Solver::step() {
           // predictor (solver/ipm.jl:111-163)
           if (mut > 0) {
             set_scalar(S_SMU, mut);
-            launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR);
-            launch_psd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR);
-            launch_bpsd_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), NV_FR);
+            corr0_op(NV_FR);
             axpby(0, NV_DF, NV_FR, NV_FR, K(1.0), K(1.0));
           } else {
             lin(LinSet().add(Lin{NV_DF, NV_D, NV_D, NV_FR, K(1.0), K(-1.0), K(0.0)}), LinSet());  // Δf − d
@@ -2212,16 +2508,13 @@ int Solver::step() {
           // Δda = Q Δpa − Bᵀ Δya − Δf
           const SpmvN da{NV_DPA, MV_DYA, NV_DF, -1, -1, NV_DDA, K(1.0), K(-1.0), K(-1.0), K(0.0), K(0.0)};
           spmv_nm(2, &da, nullptr);
-          launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DDA);
+          zerofree(NV_DDA);
           // corrector (solver/ipm.jl:181-272)
           // τa stays on the device: the dot reads it there, and one read-back brings μa home
           launch_maxsteps(NV_DPA, NV_DDA, 1.0);
           dots(DotBatch().add(0, NV_P, NV_D, 3, S_TMP2, 0, NV_DPA, NV_DDA));
-          launch_smu(st_, d_sc_.as<double>(), (double)nu_, mu, mut);  // σμ into S_SMU
-          launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_FR, NV_DPA, NV_DDA);
-          launch_psd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), d_psdlist_.as<int>(), npsd_, NV_FR, NV_DPA,
-                          NV_DDA);
-          launch_bpsd_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), bpsd(), NV_FR, NV_DPA, NV_DDA);
+          smu((double)nu_, mu, mut);  // σμ into S_SMU
+          corr_op(NV_FR, NV_DPA, NV_DDA);
           // corrector right-hand side Δf + fr, warm start (Δpa, Δya)
           lin(LinSet()
                   .add(Lin{NV_DF, NV_FR, NV_FR, NV_FR, K(1.0), K(1.0), K(0.0)})
@@ -2230,7 +2523,7 @@ int Solver::step() {
           KKTOut co = solvekkt(NV_DP, MV_DY, NV_FR, MV_DG, true, ftol, gtol, 1, false);
           const SpmvN dd{NV_DP, MV_DY, NV_DF, -1, -1, NV_DD, K(1.0), K(-1.0), K(-1.0), K(0.0), K(0.0)};
           spmv_nm(2, &dd, nullptr);
-          launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, NV_DD);
+          zerofree(NV_DD);
           row.piter = po.ncg;
           row.ppass = po.nir;
           row.pstat = po.status;
@@ -2276,6 +2569,19 @@ int Solver::step() {
 int Solver::solve() {
   int status = CONTINUE;
   while (status == CONTINUE) status = step();
+  return status;
+}
+
+int Solver::solve_in(Gang& g) {
+  g.join(this);
+  int status = CONTINUE;
+  try {
+    while (status == CONTINUE) status = step();
+  } catch (...) {
+    g.leave(this);
+    throw;
+  }
+  g.leave(this);
   return status;
 }
 

@@ -89,7 +89,19 @@ struct Level {
   double big_flops = 0.0;    // algorithmic flops of this level's large fronts
 };
 
+class Gang;
+struct GOp;
+
+// Host copies of the solver's device lists, kept for the gang's combined tables (csrc/gang.cpp)
+struct HostLists {
+  std::vector<Tile> tn, tm, sp[3];  // n-space / m-space tiles; row tiles of B, Bᵀ, Q
+  std::vector<int> psd, dense, soc, td, childlist, ea_items, ea_tile0, solve_items;
+  std::vector<StepDesc> steps;
+};
+
 class Solver {
+  friend class Gang;
+
  public:
   struct Deferred {};
   Solver(const Problem& prob, const Settings& st, const double* p0, const double* d0, const double* y0);
@@ -150,15 +162,32 @@ class Solver {
   int n() const { return (int)n_; }
   int m() const { return (int)m_; }
   int last_status() const { return status_; }
+  // lockstep batching: solve() inside a gang (join, IPM loop with every launch shared with the
+  // other members, leave); the result is then read with result() as after solve()
+  int solve_in(Gang& g);
 
  private:
   // setup
   void host_setup(const Problem& prob);
   void build_device();
   void build_schedule();
-  // device ops
+  // device ops. Inside a gang each of these is handed to the gang (one shared launch per phase
+  // for all members); the raw_ forms launch on the solver's own tables
+  bool ganged() const { return gang_ && !issuing_; }
+  void gang_submit(GOp& op);
   void sync_scalars();
+  void raw_sync_scalars();
   void set_scalar(int slot, double v);
+  void set_scalar2(int s1, double v1, int s2, double v2);
+  void smu(double nu, double mu, double mut);
+  void cg_scalars(int op);
+  void cone_scale_norm();
+  void corr0_op(int kr);
+  void corr_op(int kr, int kdp, int kdd);
+  void zerofree(int kx);
+  void factor_run(bool fp32);
+  void raw_factor_run(bool fp32);
+  void raw_assemble();
   void axpby(int space, int kx, int ky, int kz, Coef a, Coef b);
   void axpbypcz(int space, int kx, int ky, int kw, int kz, Coef a, Coef b, Coef c);
   void dot(int space, int kx, int ky, int mode, int slot, int op, int kw0 = 0, int kw1 = 0);
@@ -170,6 +199,8 @@ class Solver {
   void spmv(int which, int kx, int ky, Coef a, Coef b);
   void lsolve(int kv);
   void usolve(int kv);
+  void raw_lsolve(int kv);
+  void raw_usolve(int kv);
   bool factor(double delta, double& rho);
   void run_factor(bool fp32);
   double maxsteps(int kdp, int kdd, double step_frac);
@@ -314,7 +345,19 @@ class Solver {
   void cone_matvec_all(int kx, int ky, Coef a);
   void diff_setup(double& ftol, double& gtol);
   void check_abort();
+  // gang membership
+  HostLists hl_;
+  Gang* gang_ = nullptr;
+  int gslot_ = -1;
+  bool issuing_ = false;       // the gang's leader is running this solver's own launches
+  int gang_prof_ = 0;          // the profile mask to restore on leaving
+  hipStream_t own_st_ = nullptr;
 };
+
+// a process-wide monotone counter for the epochs of factorizations and solve launches: tile flags
+// and solve hand-off words hold the epoch of the launch that wrote them, so every launch on any
+// front (alone or in a gang) needs a value above all earlier ones
+int next_epoch();
 
 // the message mb_last_error() returns on the calling thread
 void set_last_error(const std::string& msg);

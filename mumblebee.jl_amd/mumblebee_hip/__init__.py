@@ -177,6 +177,12 @@ EXPORTS = {
                                           ctypes.c_void_p]),
     "mb_dense_cholesky_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                                                   ctypes.c_void_p]),
+    "mb_server_create_gang": (ctypes.c_void_p, [ctypes.c_int32, ctypes.c_double]),
+    "mb_gang_create": (ctypes.c_void_p, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "mb_gang_destroy": (None, [ctypes.c_void_p]),
+    "mb_solver_solve_in": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mb_solve_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.c_void_p, _I32P]),
+    "mb_gang_stats": (ctypes.c_int, [ctypes.c_void_p, _DP]),
     "mb_last_error": (ctypes.c_char_p, []),
 }
 
@@ -720,8 +726,11 @@ class IPMServer:
     (`mem_budget_gb`, default 90 % of the free memory) free up; their kernels overlap on the GPU.
     `dag_grid` workgroups per slot for the persistent factorization (0: the CUs split evenly)."""
 
-    def __init__(self, nslots=8, mem_budget_gb=0.0, dag_grid=0):
-        h = lib().mb_server_create(int(nslots), float(mem_budget_gb) * 2.0 ** 30, int(dag_grid))
+    def __init__(self, nslots=8, mem_budget_gb=0.0, dag_grid=0, gang=False):
+        if gang:  # the slots are the members of one gang (IPMGang): lockstep, shared launches
+            h = lib().mb_server_create_gang(int(nslots), float(mem_budget_gb) * 2.0 ** 30)
+        else:
+            h = lib().mb_server_create(int(nslots), float(mem_budget_gb) * 2.0 ** 30, int(dag_grid))
         if not h:
             raise _err("server")
         self._h = ctypes.c_void_p(h)
@@ -774,6 +783,51 @@ class IPMServer:
             lib().mb_server_destroy(self._h)
             self._h = None
             self._probs.clear()
+
+
+class IPMGang:
+    """Lockstep batching of independent solves (no counterpart in the reference; each member is a
+    solve!(solver), solver/solver.jl:119-150). Up to `capacity` solvers advance together, each
+    taking the reference's decisions for its own instance, while every phase of their IPM steps
+    is one launch for all of them; results equal the solvers' solo results bit for bit."""
+
+    def __init__(self, capacity=8, dag_grid=0, profile=False):
+        h = lib().mb_gang_create(int(capacity), int(dag_grid), 1 if profile else 0)
+        if not h:
+            raise _err("gang")
+        self._h = ctypes.c_void_p(h)
+        self.capacity = int(capacity)
+        _live_servers.add(self)
+
+    def solve_in(self, solver):
+        """solve!(solver) as a member (call from one thread per solver); returns the IPMStatus."""
+        r = lib().mb_solver_solve_in(solver._h, self._h)
+        if r < 0:
+            raise _err("solve! in gang")
+        return IPMStatus(r)
+
+    def solve(self, solvers):
+        """Solve the solvers through the gang (library threads, a finished solver's slot taken by
+        the next); returns their IPMResults."""
+        n = len(solvers)
+        hs = (ctypes.c_void_p * max(n, 1))(*[s._h.value for s in solvers])
+        st = np.zeros(max(n, 1), dtype=np.int32)
+        if lib().mb_solve_batch(hs, n, self._h, _p(st, ctypes.c_int32)) != 0:
+            raise _err("batch solve")
+        return [s.result() for s in solvers]
+
+    def stats(self):
+        out = np.zeros(14)
+        if lib().mb_gang_stats(self._h, _p(out, ctypes.c_double)) != 0:
+            raise _err("gang stats")
+        keys = ("ticks", "launches", "ops", "combined_ops", "readbacks", "joins", "factor_launches",
+                "factor_fronts", "factor_flops", "factor_ms", "dag_launches", "dag_flops", "dag_ms", "members")
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mb_gang_destroy(self._h)
+            self._h = None
 
 
 def clock():

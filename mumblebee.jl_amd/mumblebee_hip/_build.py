@@ -8,7 +8,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 OBJ = os.path.join(os.path.dirname(HERE), "_build")
 LIB = os.path.join(HERE, "libmumblebee_hip.so")
 
-SOURCES = ["kernels.hip", "kernels_big.hip", "psd.hip", "psd_big.hip", "front_api.hip", "host_problem.cpp", "host_symbolic.cpp", "solver.cpp", "server.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "kernels_big.hip", "psd.hip", "psd_big.hip", "front_api.hip", "host_problem.cpp", "host_symbolic.cpp", "solver.cpp", "server.cpp", "gang.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
