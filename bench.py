@@ -42,7 +42,7 @@ METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP
 ARGS = None
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # scaled by a 256 MiB device copy (profiles/k_dag_factor_qp2000_fp64.md); keyed by (n, m)
-TRAFFIC_BYTES = {(2000, 1000): 141.5e6}
+TRAFFIC_BYTES = {(2000, 1000, 0): 141.5e6}
 
 
 WORKLOADS = {
@@ -146,12 +146,27 @@ def run_concurrent(solvers, slots, tag=None):
     return time.perf_counter() - t0, lat, res
 
 
+def pct(x, q):
+    """Percentile of the latencies, None where they were not measured per solve."""
+    x = np.asarray(x, dtype=float)
+    return None if x.size == 0 or np.isnan(x).any() else float(np.percentile(x, q))
+
+
+def _gang_stats(g):
+    """The combined factorizations' rate from a gang's own events (profile=True)."""
+    st = g.stats()
+    st["dag_tflops"] = st["dag_flops"] / (st["dag_ms"] * 1e-3) / 1e12 if st["dag_ms"] > 0 else 0.0
+    st["dag_us_per_launch"] = st["dag_ms"] / max(1.0, st["dag_launches"]) * 1e3
+    return st
+
+
 def _resident_run(args, mh, precision):
     """Set up W + K resident instances at the given precision, solve the W untimed and then the
-    K timed ones `--slots` at a time; returns the measurements and the timed results."""
+    K timed ones (`--gang G`: through one gang of G members; else `--slots` at a time, each on its
+    own stream); returns the measurements and the timed results."""
     K, W, S = args.steps, args.warmup, max(1, args.slots)
     ncu = mh.device_info()["cus"]
-    grid = args.dag_grid if args.dag_grid > 0 else (max(16, ncu // S) if S > 1 else 0)
+    grid = args.dag_grid if args.dag_grid > 0 else (max(16, ncu // S) if S > 1 and not args.gang else 0)
     settings = mh.IPMSettings(dag_grid=grid, precision=precision)
     kinds = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
     # exponential cones can end NEAR_OPTIMAL: the reference's line search stalls within a few
@@ -176,6 +191,30 @@ def _resident_run(args, mh, precision):
     assert r.status in good, r.status
     solo.close()
     note(f"{args.workload} precision {precision}: solo solve {solo_ms:.1f} ms")
+    gst = None
+    if args.gang:
+        g = mh.IPMGang(capacity=args.gang, profile=True)
+        if W:
+            wres = g.solve(solvers[:W])
+            assert all(x.status in good for x in wres), [x.status for x in wres]
+            for s in solvers[:W]:
+                s.close()
+        g.close()
+        g = mh.IPMGang(capacity=args.gang, profile=True)
+        timed = solvers[W:]
+        t0 = time.perf_counter()
+        res = g.solve(timed)
+        elapsed = time.perf_counter() - t0
+        lat = [float("nan")] * len(timed)  # one batch: no per-solve latency
+        gst = _gang_stats(g)
+        g.close()
+        note(f"{args.workload} precision {precision}: {K} timed solves in {elapsed:.2f} s (gang of {args.gang})")
+        assert all(x.status in good for x in res), [x.status for x in res]
+        stats = [s.stats() for s in timed]
+        for s in timed:
+            s.close()
+        return dict(settings=settings, elapsed=elapsed, lat_ms=np.array(lat) * 1e3, res=res, prof={}, stats=stats,
+                    solo_ms=solo_ms, t_setup=t_setup, gang=gst)
     if W:
         _, _, wres = run_concurrent(solvers[:W], S, tag)
         assert all(x.status in good for x in wres), [x.status for x in wres]
@@ -198,7 +237,7 @@ def _resident_run(args, mh, precision):
         s.close()
     lat_ms = np.array(lat) * 1e3
     return dict(settings=settings, elapsed=elapsed, lat_ms=lat_ms, res=res, prof=prof, stats=stats,
-                solo_ms=solo_ms, t_setup=t_setup)
+                solo_ms=solo_ms, t_setup=t_setup, gang=None)
 
 
 def _e2e_run(args, mh, precision):
@@ -208,10 +247,11 @@ def _e2e_run(args, mh, precision):
     constructed (IPMProblem) before the timed region, as the reference's prob is."""
     K, W, T, S = args.steps, args.warmup, max(1, args.threads), max(1, args.slots)
     ncu = mh.device_info()["cus"]
-    grid = args.dag_grid if args.dag_grid > 0 else (max(16, ncu // S) if S > 1 else 0)
+    grid = args.dag_grid if args.dag_grid > 0 else (max(16, ncu // S) if S > 1 and not args.gang else 0)
     settings = mh.IPMSettings(dag_grid=grid, precision=precision)
     kinds = ["factor", "update", "panel", "fsolve", "bsolve", "extend_add", "dag"]
     good = (mh.IPMStatus.OPTIMAL, mh.IPMStatus.NEAR_OPTIMAL) if args.workload == "exppow" else (mh.IPMStatus.OPTIMAL,)
+    gang = [mh.IPMGang(capacity=args.gang, profile=True)] if args.gang else None
     probs = [hip_problem(mh, workload(args.workload, 1000 + k, args.n, args.m)) for k in range(W + K)]
     note(f"{args.workload} precision {precision}: {W + K} problems constructed")
     lat = [0.0] * (W + K)
@@ -223,9 +263,13 @@ def _e2e_run(args, mh, precision):
     def one(i, profile):
         a = time.perf_counter()
         s = mh.init(probs[i], settings)
-        if profile:
-            s.set_profile(kinds)
-        r = s.solve()
+        if gang:
+            gang[0].solve_in(s)
+            r = s.result()
+        else:
+            if profile:
+                s.set_profile(kinds)
+            r = s.solve()
         lat[i] = time.perf_counter() - a
         res[i] = r
         stats[i] = s.stats()
@@ -262,13 +306,20 @@ def _e2e_run(args, mh, precision):
 
     if W:
         run(0, W, False)
+    if gang:  # the timed solves' statistics only
+        gang[0].close()
+        gang[0] = mh.IPMGang(capacity=args.gang, profile=True)
     elapsed = run(W, W + K, True)
+    gst = None
+    if gang:
+        gst = _gang_stats(gang[0])
+        gang[0].close()
     note(f"{args.workload} precision {precision}: {K} timed solve(prob, settings) in {elapsed:.2f} s "
          f"on {T} host threads")
     r = res[W:]
     assert all(x.status in good for x in r), [x.status for x in r]
     return dict(settings=settings, elapsed=elapsed, lat_ms=np.array(lat[W:]) * 1e3, res=r, prof=prof,
-                stats=stats[W:])
+                stats=stats[W:], gang=gst)
 
 
 def bench_resident(args, mh):
@@ -285,8 +336,15 @@ def bench_resident(args, mh):
         loop = _resident_run(args, mh, 0) if not args.no_loop else None
     settings, elapsed, lat_ms, res, prof = (plain[k] for k in ("settings", "elapsed", "lat_ms", "res", "prof"))
     st0 = plain["stats"][0]
-    fac = prof["factor"]
-    dom_name = "dag" if prof["dag"]["ms"] >= prof["update"]["ms"] else "update"
+    gst = plain["gang"]
+    if gst is not None:
+        # the gang's combined factorizations, timed with events on its stream
+        fac = dict(ms=gst["dag_ms"], launches=gst["dag_launches"], flops=gst["dag_flops"], bytes=0.0)
+        prof = dict(factor=fac, dag=fac)
+        dom_name = "dag"
+    else:
+        fac = prof["factor"]
+        dom_name = "dag" if prof["dag"]["ms"] >= prof["update"]["ms"] else "update"
     dom = prof[dom_name]
     achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else 0.0
     line = {
@@ -315,8 +373,8 @@ def bench_resident(args, mh):
             "precision": "fp64 (plain); the mixed form is reported under mixed",
             "threads": args.threads,
             "dag_grid": settings.dag_grid,
-            "p50_ms": float(np.percentile(lat_ms, 50)),
-            "p99_ms": float(np.percentile(lat_ms, 99)),
+            "p50_ms": pct(lat_ms, 50),
+            "p99_ms": pct(lat_ms, 99),
             "init_host_ms_mean": float(np.mean([x["init_host_ms"] for x in plain["stats"]])),
             "init_device_ms_mean": float(np.mean([x["init_device_ms"] for x in plain["stats"]])),
             "init_device_parts_ms_mean": {k: float(np.mean([x["init_device_parts_ms"][k] for x in plain["stats"]]))
@@ -324,6 +382,7 @@ def bench_resident(args, mh):
             "init_buffer_marks_ms_mean": {k: float(np.mean([x["build_marks_ms"][k] for x in plain["stats"]]))
                                           for k in ("fronts", "union", "cones", "allocs", "sparse")},
             "iterations_mean": float(np.mean([x.niter for x in res])),
+            "gang_members": args.gang,
             "factor_tflops_aggregate": fac["flops"] / elapsed / 1e12,
             "factor_ms_per_call": fac["ms"] / max(1, fac["launches"]),
             "factor_exact_flops": st0["exact_flops"],
@@ -331,6 +390,7 @@ def bench_resident(args, mh):
             "nfront": st0["nfront"],
             "kernel_ms": {k: v["ms"] for k, v in prof.items()},
             "kernel_launches": {k: v["launches"] for k, v in prof.items()},
+            "gang": gst,
         },
         "roofline": {
             "bound": "mfma",
@@ -338,9 +398,11 @@ def bench_resident(args, mh):
             "peak": PEAK_FP64_MFMA,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_MFMA,
-            "traffic": (TRAFFIC_BYTES.get((args.n, args.m)) if dom_name == "dag" and args.workload == "qp"
+            "traffic": (TRAFFIC_BYTES.get((args.n, args.m, args.gang)) if dom_name == "dag" and args.workload == "qp"
                         else None),
-            "kernel": ("k_dag_factor (persistent left-looking tile FP64 MFMA Cholesky of the large fronts)"
+            "kernel": (("k_dag_factor_multi (one persistent left-looking tile FP64 MFMA Cholesky launch over "
+                        "the large fronts of every gang member)" if gst is not None else
+                        "k_dag_factor (persistent left-looking tile FP64 MFMA Cholesky of the large fronts)")
                        if dom_name == "dag" else "k_update (FP64 MFMA trailing update of the frontal Cholesky)"),
             "avg_launch_us": dom["ms"] / max(1, dom["launches"]) * 1e3,
             "flops_per_launch": dom["flops"] / max(1, dom["launches"]),
@@ -353,11 +415,12 @@ def bench_resident(args, mh):
     if loop is not None:
         line["config"]["resident_loop"] = {
             "what": f"the IPM loop alone (solve! after init) on the same kind of instances, resident in HBM, "
-                    f"{S} at a time",
+                    + (f"through one gang of {args.gang} members" if args.gang else f"{S} at a time"),
             "solves_per_s": K / loop["elapsed"],
-            "p50_ms": float(np.percentile(loop["lat_ms"], 50)),
-            "p99_ms": float(np.percentile(loop["lat_ms"], 99)),
+            "p50_ms": pct(loop["lat_ms"], 50),
+            "p99_ms": pct(loop["lat_ms"], 99),
             "solo_ms": loop["solo_ms"],
+            "gang": loop["gang"],
         }
     if mixed is not None:
         # the same seeded instances: per-result error of the mixed form against the FP64 results
@@ -365,11 +428,14 @@ def bench_resident(args, mh):
                for a, b in zip(mixed["res"], res)]
         perr = [float(np.linalg.norm(a.p - b.p) / max(1e-300, np.linalg.norm(b.p))) for a, b in zip(mixed["res"], res)]
         mp = mixed["prof"]
+        if mixed["gang"] is not None:
+            mg = mixed["gang"]
+            mp = {"factor": dict(ms=mg["dag_ms"], launches=mg["dag_launches"], flops=mg["dag_flops"], bytes=0.0)}
         line["config"]["mixed"] = {
             "solves_per_s": K / mixed["elapsed"],
             "vs_plain": (K / mixed["elapsed"]) / (K / elapsed),
-            "p50_ms": float(np.percentile(mixed["lat_ms"], 50)),
-            "p99_ms": float(np.percentile(mixed["lat_ms"], 99)),
+            "p50_ms": pct(mixed["lat_ms"], 50),
+            "p99_ms": pct(mixed["lat_ms"], 99),
             "iterations_mean": float(np.mean([x.niter for x in mixed["res"]])),
             "fp32_factor_share": float(np.sum([s["factor_calls_fp32"] for s in mixed["stats"]]) /
                                        max(1, np.sum([s["factor_calls"] for s in mixed["stats"]]))),
@@ -444,12 +510,14 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=128)
+    ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--slots", type=int, default=4,
                     help="the CUs of a large-front factorization are 1/slots of the chip; the resident-loop "
                          "measurement solves this many at a time")
-    ap.add_argument("--threads", type=int, default=12, help="host threads running solve(prob, settings)")
+    ap.add_argument("--threads", type=int, default=24, help="host threads running solve(prob, settings)")
+    ap.add_argument("--gang", type=int, default=16,
+                    help="members of the lockstep gang the solves run in (0: each solver on its own stream)")
     ap.add_argument("--no-loop", action="store_true", help="skip the resident-loop measurement")
     ap.add_argument("--loop-only", action="store_true",
                     help="time the IPM loop on instances set up before timing (init excluded and reported)")
@@ -480,7 +548,7 @@ def main():
         sys.exit(2)
     # HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); slots beyond that
     # would share a queue and run their kernels one after another, so give each slot its own
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, args.slots, args.threads))))
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, args.slots, 0 if args.gang else args.threads))))
     import mumblebee_hip as mh
     line = bench_serve(args, mh) if args.serve else bench_resident(args, mh)
     print(json.dumps(line))

@@ -7,7 +7,8 @@
 #include <stdexcept>
 #include <thread>
 
-#include <immintrin.h>
+#include <sys/mman.h>
+#include <ucontext.h>
 
 namespace mb {
 
@@ -22,10 +23,62 @@ namespace mb {
 // index = gang slot), front-table indices, and the interleaved ticket orders of the
 // factorization's tasks and of the two solves' blocks
 struct Gang::BigCombo {
-  DevBuf bs, bfront, odag, ofs, obs;
+  std::unique_ptr<DevBuf> buf;
+  BigFront* bs = nullptr;
+  int* bfront = nullptr;
+  int2 *odag = nullptr, *ofs = nullptr, *obs = nullptr;
   int nf = 0, ndag = 0, nfs = 0, nbs = 0, max2tr = 0, maxtr = 0, maxch = 0;
   double flops = 0.0;
 };
+
+// Several host arrays packed into one device block (parts 256-byte aligned), copied with one
+// asynchronous transfer: no stream synchronisation, so the device goes on with the running launches
+struct Gang::Pack {
+  std::vector<char> h;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    const size_t off = (h.size() + 255) & ~size_t(255);
+    h.resize(off + std::max<size_t>(v.size(), 1) * sizeof(T), 0);
+    if (!v.empty()) std::memcpy(h.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+  size_t zeros(size_t bytes) {  // a part the device writes (or needs zeroed)
+    const size_t off = (h.size() + 255) & ~size_t(255);
+    h.resize(off + std::max<size_t>(bytes, 8), 0);
+    return off;
+  }
+};
+
+// the packed arrays into a new device block, through a pinned staging block; both stay alive
+// until a stream synchronisation shows that every launch before has passed
+std::unique_ptr<DevBuf> Gang::upload(const Pack& p) {
+  auto d = std::make_unique<DevBuf>();
+  d->alloc(p.h.size());
+  void* hp = nullptr;
+  for (size_t i = 0; i < pin_free_.size(); ++i)
+    if (pin_free_[i].second >= p.h.size()) {
+      hp = pin_free_[i].first;
+      pin_busy_.push_back(pin_free_[i]);
+      pin_free_.erase(pin_free_.begin() + (long)i);
+      break;
+    }
+  if (!hp) {
+    const size_t c = std::max<size_t>(p.h.size(), size_t(1) << 20);
+    HIPCHK(hipHostMalloc(&hp, c, hipHostMallocDefault));
+    pin_busy_.push_back({hp, c});
+  }
+  std::memcpy(hp, p.h.data(), p.h.size());
+  HIPCHK(hipMemcpyAsync(d->p, hp, p.h.size(), hipMemcpyHostToDevice, st_));
+  return d;
+}
+
+// after a synchronisation of the gang's stream: what earlier launches used can go
+void Gang::passed() {
+  graveyard_.clear();
+  old_tabs_.clear();
+  for (auto& b : pin_busy_) pin_free_.push_back(b);
+  pin_busy_.clear();
+}
 
 namespace {
 
@@ -97,13 +150,6 @@ bool same_op(const GOp& x, const GOp& y) {
   }
 }
 
-void spin_until(const std::atomic<bool>& f) {
-  for (int i = 0; !f.load(std::memory_order_acquire); ++i) {
-    if (i < 4096) _mm_pause();
-    else std::this_thread::yield();
-  }
-}
-
 }  // namespace
 
 Gang::Gang(int kmax, int dag_grid, bool profile)
@@ -120,6 +166,9 @@ Gang::Gang(int kmax, int dag_grid, bool profile)
   }
   mem_.assign(kmax_, nullptr);
   op_.assign(kmax_, nullptr);
+  q_.assign(kmax_, {});
+  err_.assign(kmax_, {});
+  fib_.resize(kmax_);
   HIPCHK(hipHostMalloc((void**)&h_gather_, sizeof(double) * GANG_MAX * NSCALAR, hipHostMallocDefault));
   d_gather_.alloc(sizeof(double) * GANG_MAX * NSCALAR);
   d_dead_.alloc(sizeof(double) * NSCALAR);
@@ -131,12 +180,26 @@ Gang::Gang(int kmax, int dag_grid, bool profile)
   HIPCHK(hipStreamSynchronize(st_));
   dead_ = InstDev{};
   dead_.sc = d_dead_.as<double>();  // S_ACTIVE stays 0: every kernel skips a vacant slot
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  sched_ = std::thread([this, dev] {
+    (void)hipSetDevice(dev);
+    sched_main();
+  });
 }
 
 Gang::~Gang() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  if (sched_.joinable()) sched_.join();
   (void)hipStreamSynchronize(st_);
-  graveyard_.clear();
+  passed();
   combos_.clear();
+  tab_.reset();
+  for (auto& b : pin_free_) (void)hipHostFree(b.first);
   if (h_gather_) (void)hipHostFree(h_gather_);
   (void)hipEventDestroy(ev0_);
   (void)hipEventDestroy(ev1_);
@@ -146,158 +209,251 @@ Gang::~Gang() {
 
 int Gang::members() {
   std::lock_guard<std::mutex> lk(mu_);
-  return nrun_;
+  return (int)st_pub_.members;
 }
 
 void Gang::stats(GangStats& out) {
   std::lock_guard<std::mutex> lk(mu_);
-  out = st_acc_;
+  out = st_pub_;
 }
 
 // ------------------------------------------------------------------------------------------
-// membership and ticks
+// membership, fibers and ticks
 // ------------------------------------------------------------------------------------------
 
-void Gang::join(Solver* s) {
+// A member's IPM loop runs on a fiber (its own stack, switched with swapcontext) of the gang's
+// scheduler thread: handing in an op is a function call, and a blocking op switches back to the
+// scheduler, which runs the other members' fibers and the ticks. No OS thread waits on another
+// within a solve.
+struct Gang::Fiber {
+  ucontext_t ctx;
+  void* stack = nullptr;
+  size_t stack_bytes = 0;
+  Solver* s = nullptr;
+  Job* job = nullptr;
+  bool runnable = true;
+  bool finished = false;
+};
+
+struct Gang::Job {
+  Solver* s = nullptr;
+  int status = -1;
+  std::string err;
+  bool done = false;
+};
+
+int Gang::run(Solver* s) {
   if (s->gang_) throw std::logic_error("solver is already in a gang");
-  HIPCHK(hipStreamSynchronize(s->st_));  // its setup and starting point are on the device
-  std::unique_lock<std::mutex> lk(mu_);
-  joining_.push_back(s);
-  for (;;) {
-    if (s->gang_ == this) return;
-    if (!leading_ && nrun_ == 0) {
-      // no tick will come: admit the waiting solvers here
-      leading_ = true;
-      lk.unlock();
-      std::string err;
-      try {
-        admit();
-      } catch (const std::exception& e) {
-        err = e.what();
-      }
-      lk.lock();
-      leading_ = false;
-      cv_.notify_all();
-      if (!err.empty()) throw std::runtime_error(err);
-      continue;
-    }
-    cv_.wait_for(lk, std::chrono::microseconds(200));
-  }
-}
-
-// the waiting solvers take free slots (the leader, between ticks); the tables are rebuilt
-void Gang::admit() {
-  std::vector<Solver*> take;
+  Job job;
+  job.s = s;
   {
     std::lock_guard<std::mutex> lk(mu_);
-    std::vector<Solver*> rest;
-    for (Solver* s : joining_) {
-      int k = -1;
-      for (int j = 0; j < kmax_; ++j) {
-        bool used = mem_[j] != nullptr;
-        for (Solver* t : take)
-          if (t->gslot_ == j) used = true;
-        if (!used) {
-          k = j;
-          break;
-        }
-      }
-      if (k < 0) {
-        rest.push_back(s);
-        continue;
-      }
-      s->gslot_ = k;
-      take.push_back(s);
+    if (stop_) throw std::runtime_error("gang is shutting down");
+    joining_.push_back(&job);
+  }
+  cv_.notify_all();
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [&] { return job.done; });
+  if (!job.err.empty()) throw std::runtime_error(job.err);
+  return job.status;
+}
+
+void Gang::fiber_entry(unsigned lo, unsigned hi) {
+  Fiber* f = reinterpret_cast<Fiber*>((uintptr_t)lo | ((uintptr_t)hi << 32));
+  Gang* g = f->s->gang_;
+  Solver* s = f->s;
+  try {
+    int status = CONTINUE;
+    while (status == CONTINUE) status = s->step();
+    f->job->status = status;
+  } catch (const std::exception& e) {
+    f->job->err = e.what();
+  }
+  try {
+    GOp* op = new GOp;
+    op->kind = GK_LEAVE;
+    op->blocking = true;
+    g->submit(s, op);
+  } catch (const std::exception& e) {
+    if (f->job->err.empty()) f->job->err = e.what();
+  }
+  f->finished = true;
+  swapcontext(&f->ctx, &g->sched_ctx_);  // not resumed again
+}
+
+// the waiting solvers take free slots (between ticks, on the scheduler thread); the tables are
+// rebuilt and each new member gets a fiber
+void Gang::admit() {
+  // a slot is free once its last member's fiber has ended too
+  std::vector<int> slots;
+  for (int k = 0; k < kmax_; ++k)
+    if (!mem_[k] && !fib_[k]) slots.push_back(k);
+  std::vector<Job*> take;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    while (!joining_.empty() && take.size() < slots.size()) {
+      take.push_back(joining_.front());
+      joining_.pop_front();
     }
-    joining_ = rest;
   }
   if (take.empty()) return;
-  HIPCHK(hipStreamSynchronize(st_));
-  for (Solver* s : take) {
-    mem_[s->gslot_] = s;
+  for (size_t t = 0; t < take.size(); ++t) {
+    Job* j = take[t];
+    Solver* s = j->s;
+    const int k = slots[t];
+    HIPCHK(hipStreamSynchronize(s->st_));  // its setup and starting point are on the device
+    mem_[k] = s;
+    s->gslot_ = k;
     s->own_st_ = s->st_;
     s->st_ = st_;
     s->gang_prof_ = s->prof_mask_;
     s->prof_mask_ = 0;  // per-kernel events are the gang's (its combined launches serve all members)
-  }
-  dirty_ = true;
-  rebuild();
-  std::lock_guard<std::mutex> lk(mu_);
-  for (Solver* s : take) {
     s->gang_ = this;
+    auto f = std::make_unique<Fiber>();
+    f->s = s;
+    f->job = j;
+    f->stack_bytes = FIBER_STACK;
+    f->stack = mmap(nullptr, f->stack_bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE,
+                    -1, 0);
+    if (f->stack == MAP_FAILED) throw std::runtime_error("gang: no memory for a fiber stack");
+    getcontext(&f->ctx);
+    f->ctx.uc_stack.ss_sp = f->stack;
+    f->ctx.uc_stack.ss_size = f->stack_bytes;
+    f->ctx.uc_link = nullptr;
+    const uintptr_t pf = reinterpret_cast<uintptr_t>(f.get());
+    makecontext(&f->ctx, (void (*)())&Gang::fiber_entry, 2, (unsigned)(pf & 0xffffffffu), (unsigned)(pf >> 32));
+    fib_[k] = std::move(f);
     ++nrun_;
     st_acc_.joins += 1;
   }
+  rebuild();
 }
 
-void Gang::leave(Solver* s) {
-  if (s->gang_ != this) return;
-  GOp op;
-  op.kind = GK_LEAVE;
-  submit(s, op);
-}
-
-void Gang::submit(Solver* s, GOp& op) {
-  const int k = s->gslot_;
-  op.done.store(false, std::memory_order_relaxed);
-  bool lead = false;
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    op_[k] = &op;
-    ++narr_;
-    if (!leading_ && narr_ == nrun_) lead = leading_ = true;
-  }
-  while (lead) {
-    try {
-      tick();
-    } catch (const std::exception& e) {
-      // a failure outside any one op (table rebuild, read-back): every pending op fails, and the
-      // members that were leaving are gone
-      std::lock_guard<std::mutex> lk(mu_);
-      for (int j = 0; j < kmax_; ++j)
-        if (op_[j]) {
-          GOp* o = op_[j];
-          op_[j] = nullptr;
-          --narr_;
-          if (o->kind == GK_LEAVE) {
-            detach(j);
-            mem_[j] = nullptr;
-            --nrun_;
-          }
-          o->err = e.what();
-          o->done.store(true, std::memory_order_release);
-        }
+void Gang::sched_main() {
+  for (;;) {
+    bool fibers = false;
+    for (auto& f : fib_) fibers = fibers || f != nullptr;
+    if (!fibers) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || !joining_.empty(); });
+      if (joining_.empty()) return;  // stop_ with nothing left
     }
-    // every running member waits on a handed-in op: this thread leads the next tick too
-    std::lock_guard<std::mutex> lk(mu_);
-    lead = nrun_ > 0 && narr_ == nrun_;
-    leading_ = lead;
-    cv_.notify_all();
+    std::string fail;
+    try {
+      admit();
+    } catch (const std::exception& e) {
+      fail = e.what();
+    }
+    if (!fail.empty()) {
+      // the waiting jobs cannot be admitted: they fail
+      std::lock_guard<std::mutex> lk(mu_);
+      for (Job* j : joining_) {
+        j->err = fail;
+        j->done = true;
+      }
+      joining_.clear();
+      cv_done_.notify_all();
+      if (nrun_ == 0) continue;
+    }
+    // every runnable member runs until it waits on an op (or ends)
+    for (int k = 0; k < kmax_; ++k) {
+      Fiber* f = fib_[k].get();
+      if (!f || !f->runnable) continue;
+      f->runnable = false;
+      swapcontext(&sched_ctx_, &f->ctx);
+      if (f->finished) {
+        Job* j = f->job;
+        munmap(f->stack, f->stack_bytes);
+        fib_[k].reset();
+        std::lock_guard<std::mutex> lk(mu_);
+        j->done = true;
+        cv_done_.notify_all();
+      }
+    }
+    // ticks while every member has an op queued
+    while (ready()) {
+      try {
+        tick();
+      } catch (const std::exception& e) {
+        fail_fronts(e.what());
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      st_pub_ = st_acc_;
+      st_pub_.members = nrun_;
+    }
   }
-  spin_until(op.done);
-  if (!op.err.empty()) throw std::runtime_error(op.err);
+}
+
+// a failure outside any one op (table rebuild, read-back): the front ops fail, and the members
+// that were leaving are gone
+void Gang::fail_fronts(const std::string& what) {
+  for (int j = 0; j < kmax_; ++j) {
+    if (!mem_[j] || q_[j].empty()) continue;
+    GOp* o = q_[j].front();
+    q_[j].pop_front();
+    if (o->kind == GK_LEAVE) {
+      detach(j);
+      mem_[j] = nullptr;
+      --nrun_;
+    }
+    if (o->blocking) {
+      o->err = what;
+      o->done.store(true, std::memory_order_relaxed);
+      if (fib_[j]) fib_[j]->runnable = true;
+    } else {
+      if (err_[j].empty()) err_[j] = what;
+      delete o;
+    }
+  }
+}
+
+bool Gang::ready() const {
+  if (nrun_ == 0) return false;
+  for (int k = 0; k < kmax_; ++k)
+    if (mem_[k] && q_[k].empty()) return false;
+  return true;
+}
+
+// on a member's fiber: queue the op; a blocking op switches to the scheduler until it was issued
+void Gang::submit(Solver* s, GOp* op) {
+  const int k = s->gslot_;
+  op->done.store(false, std::memory_order_relaxed);
+  q_[k].push_back(op);
+  if (!op->blocking) return;
+  Fiber* f = fib_[k].get();
+  swapcontext(&f->ctx, &sched_ctx_);
+  std::string err = op->err;
+  delete op;
+  if (err.empty() && s->gang_ == this && !err_[k].empty()) std::swap(err, err_[k]);
+  if (!err.empty()) throw std::runtime_error(err);
 }
 
 void Gang::tick() {
-  {
-    bool wait;
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      wait = !joining_.empty();
-    }
-    if (wait) admit();
-  }
   std::vector<int> cheap, parked;
-  bool all;
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    for (int k = 0; k < kmax_; ++k)
-      if (op_[k]) (op_[k]->park ? parked : cheap).push_back(k);
-    all = narr_ == nrun_;
+  bool all = true;
+  for (int k = 0; k < kmax_; ++k) {
+    op_[k] = nullptr;
+    if (!mem_[k]) continue;
+    if (q_[k].empty()) {
+      all = false;
+      continue;
+    }
+    op_[k] = q_[k].front();
+    (op_[k]->park ? parked : cheap).push_back(k);
   }
+  // Cheap ops first. With every member parked: the triangular solves, and the factorizations only
+  // once every member waits on one, so that the factorizations of all members meet in one launch
+  // (a member that joined late, or refined more, catches up by the next factorization)
   std::vector<int> issue;
-  if (!cheap.empty()) issue = cheap;
-  else if (all) issue = parked;
+  if (!cheap.empty()) {
+    issue = cheap;
+  } else if (all) {
+    for (int k : parked)
+      if (op_[k]->kind != GK_FACTOR) issue.push_back(k);
+    if (issue.empty()) issue = parked;
+  }
   if (issue.empty()) return;
   st_acc_.ticks += 1;
   // groups of ops with one combined form, in slot order
@@ -343,7 +499,7 @@ void Gang::tick() {
     Solver* s = mem_[k];
     set_mask(dev_mask_ & ~(1ull << k));
     hinst_[k] = dead_;
-    launch_gang_setinst(st_, d_inst_.as<InstDev>(), k, dead_);
+    launch_gang_setinst(st_, T_.inst, k, dead_);
     launch_set_scalar(st_, s->d_sc_.as<double>(), S_ACTIVE, 1.0);
     HIPCHK(hipEventRecord(evleave_, st_));
     HIPCHK(hipStreamWaitEvent(s->own_st_, evleave_, 0));  // its own stream starts after its last op here
@@ -352,16 +508,25 @@ void Gang::tick() {
     for (auto& kv : combos_) graveyard_.push_back(std::move(kv.second));
     combos_.clear();
   }
-  std::lock_guard<std::mutex> lk(mu_);
   for (int k : issue) {
     GOp* o = op_[k];
     op_[k] = nullptr;
-    --narr_;
+    q_[k].pop_front();
     if (o->kind == GK_LEAVE) {
       mem_[k] = nullptr;
       --nrun_;
+      for (GOp* r : q_[k]) delete r;  // nothing follows a leave
+      q_[k].clear();
+      if (o->err.empty()) o->err.swap(err_[k]);  // an unreported failure of its last ops
+      err_[k].clear();
     }
-    o->done.store(true, std::memory_order_release);
+    if (o->blocking) {
+      o->done.store(true, std::memory_order_relaxed);
+      if (fib_[k]) fib_[k]->runnable = true;
+    } else {
+      if (!o->err.empty() && err_[k].empty()) err_[k] = o->err;
+      delete o;
+    }
   }
 }
 
@@ -392,7 +557,7 @@ void Gang::set_mask(uint64_t mask) {
   uint64_t occ = 0;
   for (int k = 0; k < kmax_; ++k)
     if (mem_[k]) occ |= 1ull << k;
-  launch_gang_active(st_, d_inst_.as<InstDev>(), kmax_, mask & occ, occ);
+  launch_gang_active(st_, T_.inst, kmax_, mask & occ, occ);
   dev_mask_ = mask & occ;
   st_acc_.launches += 1;
 }
@@ -401,22 +566,22 @@ void Gang::issue_solo(int k) {
   Solver* s = mem_[k];
   GOp& op = *op_[k];
   if (!((dev_mask_ >> k) & 1ull)) set_mask(dev_mask_ | (1ull << k));
-  s->issuing_ = true;
+  Solver::set_issuing(s);
   try {
     if (op.solo) op.solo();
   } catch (const std::exception& e) {
     op.err = e.what();
   }
-  s->issuing_ = false;
+  Solver::set_issuing(nullptr);
 }
 
 void Gang::readbacks(const std::vector<int>& slots) {
   uint64_t mask = 0;
   for (int k : slots) mask |= 1ull << k;
-  launch_gang_gather(st_, d_inst_.as<InstDev>(), kmax_, mask, d_gather_.as<double>());
+  launch_gang_gather(st_, T_.inst, kmax_, mask, d_gather_.as<double>());
   HIPCHK(hipMemcpyAsync(h_gather_, d_gather_.p, sizeof(double) * NSCALAR * kmax_, hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
-  graveyard_.clear();  // the stream has passed every launch that could read them
+  passed();  // the stream has passed every launch that could read them
   for (int k : slots) std::memcpy(mem_[k]->hsc_, h_gather_ + (size_t)k * NSCALAR, sizeof(double) * NSCALAR);
   st_acc_.readbacks += 1;
   st_acc_.launches += 1;
@@ -435,8 +600,8 @@ void Gang::readbacks(const std::vector<int>& slots) {
 // ------------------------------------------------------------------------------------------
 
 void Gang::rebuild() {
-  HIPCHK(hipStreamSynchronize(st_));
-  graveyard_.clear();
+  // cached launch data names the old front table's indices
+  for (auto& kv : combos_) graveyard_.push_back(std::move(kv.second));
   combos_.clear();
   const int K = kmax_;
   hinst_.assign(K, dead_);
@@ -549,35 +714,65 @@ void Gang::rebuild() {
       }
     }
   }
-  auto up_tiles = [&](DevBuf& d, const std::vector<Tile>& v) { d.upload(v.empty() ? std::vector<Tile>{Tile{0, 0}} : v, st_); };
-  auto up_int = [&](DevBuf& d, const std::vector<int>& v) { d.upload(v.empty() ? std::vector<int>{0} : v, st_); };
-  d_inst_.upload(hinst_, st_);
-  up_tiles(d_tn_, tn);
-  up_tiles(d_tm_, tm);
-  for (int w = 0; w < 3; ++w) up_tiles(d_sp_[w], sp[w]);
-  d_cones_.upload(cones.empty() ? std::vector<ConeDesc>(1) : cones, st_);
-  up_int(d_cone_ofs_, cone_ofs);
-  d_coneout_.alloc(sizeof(double) * 3 * std::max(ncone_, 1));
-  up_int(d_psd_, psd);
-  up_int(d_dense_, dense);
-  up_int(d_soc_, soc);
-  d_dpart_.alloc(sizeof(double) * DENSE_SPLIT * std::max(ndense_, 1));
-  up_int(d_cone_front_, cone_front);
-  d_fr_.upload(fr.empty() ? std::vector<FrontDesc>(1) : fr, st_);
-  up_int(d_childlist_, childlist);
-  up_int(d_items_, items);
-  d_steps_.upload(steps.empty() ? std::vector<StepDesc>(1) : steps, st_);
-  up_int(d_ea_, ea);
-  up_int(d_ea0_, ea0);
-  d_flags_.alloc(sizeof(int) * std::max<size_t>(steps.size(), 1));
-  HIPCHK(hipMemsetAsync(d_flags_.p, 0, d_flags_.bytes, st_));
+  // the members' assembly arrays (vacant slots: empty)
+  std::vector<GangAsm> asmv(K);
+  long long fb = 0, ub = 0;
+  for (int k = 0; k < K; ++k) {
+    GangAsm& A = asmv[k];
+    A = GangAsm{nullptr, 0, nullptr, nullptr, nullptr, 0, fb, ub};
+    Solver* s = mem_[k];
+    if (!s) continue;
+    A.F = s->d_F_.as<double>();
+    A.nF = s->fsize_;
+    A.pos = s->d_upos_.as<int64_t>();
+    A.l = s->d_ul_.as<double>();
+    A.q = s->d_uq_.as<double>();
+    A.nent = s->nunion_;
+    fb += (A.nF / 2 + TILE_THREADS - 1) / TILE_THREADS + (A.nF & 1 ? 1 : 0);
+    ub += (A.nent + TILE_THREADS - 1) / TILE_THREADS;
+  }
+  fill_blocks_ = fb;
+  union_blocks_ = ub;
+  Pack P;
+  const size_t o_asm = P.add(asmv);
+  const size_t o_inst = P.add(hinst_), o_tn = P.add(tn), o_tm = P.add(tm), o_sp0 = P.add(sp[0]),
+               o_sp1 = P.add(sp[1]), o_sp2 = P.add(sp[2]), o_cones = P.add(cones), o_cofs = P.add(cone_ofs),
+               o_cout = P.zeros(sizeof(double) * 3 * std::max(ncone_, 1)), o_psd = P.add(psd), o_dense = P.add(dense),
+               o_soc = P.add(soc), o_dpart = P.zeros(sizeof(double) * DENSE_SPLIT * std::max(ndense_, 1)),
+               o_cf = P.add(cone_front), o_fr = P.add(fr), o_cl = P.add(childlist), o_items = P.add(items),
+               o_steps = P.add(steps), o_ea = P.add(ea), o_ea0 = P.add(ea0),
+               o_flags = P.zeros(sizeof(int) * std::max<size_t>(steps.size(), 1));
+  if (tab_) old_tabs_.push_back(std::move(tab_));  // launches queued before may still read it
+  tab_ = upload(P);
+  char* b = static_cast<char*>(tab_->p);
+  T_.inst = reinterpret_cast<InstDev*>(b + o_inst);
+  T_.tn = reinterpret_cast<Tile*>(b + o_tn);
+  T_.tm = reinterpret_cast<Tile*>(b + o_tm);
+  T_.sp[0] = reinterpret_cast<Tile*>(b + o_sp0);
+  T_.sp[1] = reinterpret_cast<Tile*>(b + o_sp1);
+  T_.sp[2] = reinterpret_cast<Tile*>(b + o_sp2);
+  T_.cones = reinterpret_cast<ConeDesc*>(b + o_cones);
+  T_.cone_ofs = reinterpret_cast<int*>(b + o_cofs);
+  T_.coneout = reinterpret_cast<double*>(b + o_cout);
+  T_.psd = reinterpret_cast<int*>(b + o_psd);
+  T_.dense = reinterpret_cast<int*>(b + o_dense);
+  T_.soc = reinterpret_cast<int*>(b + o_soc);
+  T_.dpart = reinterpret_cast<double*>(b + o_dpart);
+  T_.cone_front = reinterpret_cast<int*>(b + o_cf);
+  T_.fr = reinterpret_cast<FrontDesc*>(b + o_fr);
+  T_.childlist = reinterpret_cast<int*>(b + o_cl);
+  T_.items = reinterpret_cast<int*>(b + o_items);
+  T_.steps = reinterpret_cast<StepDesc*>(b + o_steps);
+  T_.ea = reinterpret_cast<int*>(b + o_ea);
+  T_.ea0 = reinterpret_cast<int*>(b + o_ea0);
+  T_.flags = reinterpret_cast<int*>(b + o_flags);
+  T_.asmv = reinterpret_cast<GangAsm*>(b + o_asm);
   // every member takes part until a launch says otherwise
   uint64_t occ = 0;
   for (int k = 0; k < K; ++k)
     if (mem_[k]) occ |= 1ull << k;
-  launch_gang_active(st_, d_inst_.as<InstDev>(), K, occ, occ);
+  launch_gang_active(st_, T_.inst, K, occ, occ);
   dev_mask_ = occ;
-  HIPCHK(hipStreamSynchronize(st_));
   dirty_ = false;
 }
 
@@ -613,11 +808,15 @@ Gang::BigCombo& Gang::combo(const std::vector<std::pair<int, int>>& fronts) {
   c->ndag = (int)odag.size();
   c->nfs = (int)ofs.size();
   c->nbs = (int)obs.size();
-  c->bs.upload(bs, st_);
-  c->bfront.upload(bfront, st_);
-  c->odag.upload(odag, st_);
-  c->ofs.upload(ofs, st_);
-  c->obs.upload(obs, st_);
+  Pack P;
+  const size_t o_bs = P.add(bs), o_bf = P.add(bfront), o_dag = P.add(odag), o_fs = P.add(ofs), o_bsv = P.add(obs);
+  c->buf = upload(P);
+  char* b = static_cast<char*>(c->buf->p);
+  c->bs = reinterpret_cast<BigFront*>(b + o_bs);
+  c->bfront = reinterpret_cast<int*>(b + o_bf);
+  c->odag = reinterpret_cast<int2*>(b + o_dag);
+  c->ofs = reinterpret_cast<int2*>(b + o_fs);
+  c->obs = reinterpret_cast<int2*>(b + o_bsv);
   BigCombo& ref = *c;
   combos_[key] = std::move(c);
   return ref;
@@ -631,17 +830,13 @@ void Gang::combined_factor(const std::vector<int>& slots, bool fp32) {
   uint64_t mask = 0;
   for (int k : slots) mask |= 1ull << k;
   set_mask(mask);
-  const InstDev* I = d_inst_.as<InstDev>();
-  const FrontDesc* F = d_fr_.as<FrontDesc>();
-  // assembly: each member's fill and union map, then the cone blocks of all in one launch
-  for (int k : slots) {
-    Solver* s = mem_[k];
-    launch_fill(st_, s->d_F_.as<double>(), (size_t)s->fsize_, 0.0);
-    launch_assemble_union(st_, s->d_inst_.as<InstDev>(), 0, s->d_upos_.as<int64_t>(), s->d_ul_.as<double>(),
-                          s->d_uq_.as<double>(), s->nunion_, 1);
-  }
-  launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), F, ncone_, cone_nmax_);
-  st_acc_.launches += 2.0 * slots.size() + 1;
+  const InstDev* I = T_.inst;
+  const FrontDesc* F = T_.fr;
+  // assembly: the members' fronts cleared, BᵀB + δQ scattered and the cone blocks added, each
+  // step one launch for all
+  launch_gang_assemble(st_, I, T_.asmv, kmax_, fill_blocks_, union_blocks_);
+  launch_assemble_cones(st_, I, T_.cones, T_.cone_front, F, ncone_, cone_nmax_);
+  st_acc_.launches += 3;
   const int epoch = next_epoch();
   if (profile_) HIPCHK(hipEventRecord(ev0_, st_));
   double flops = 0.0;
@@ -649,19 +844,19 @@ void Gang::combined_factor(const std::vector<int>& slots, bool fp32) {
   for (size_t l = 0; l < lv_.size(); ++l) {
     const GLevel& G = lv_[l];
     for (size_t r = 0; r < G.ea_off.size(); ++r)
-      launch_extend_add(st_, F, d_ea_.as<int>() + G.ea_off[r], d_ea0_.as<int>() + G.ea_off[r], G.ea_n[r],
+      launch_extend_add(st_, F, T_.ea + G.ea_off[r], T_.ea0 + G.ea_off[r], G.ea_n[r],
                         G.ea_tiles[r], I);
     for (size_t q = 0; q < G.pan_off.size(); ++q) {
-      launch_panel(st_, F, d_steps_.as<StepDesc>() + G.pan_off[q], G.pan_n[q], G.pan_blk[q], I,
-                   d_flags_.as<int>() + G.pan_off[q], epoch, nullptr);
-      if (G.upd_blk[q] > 0) launch_update(st_, F, d_steps_.as<StepDesc>() + G.upd_off[q], G.upd_n[q], G.upd_blk[q], I);
+      launch_panel(st_, F, T_.steps + G.pan_off[q], G.pan_n[q], G.pan_blk[q], I,
+                   T_.flags + G.pan_off[q], epoch, nullptr);
+      if (G.upd_blk[q] > 0) launch_update(st_, F, T_.steps + G.upd_off[q], G.upd_n[q], G.upd_blk[q], I);
     }
     std::vector<std::pair<int, int>> big;
     for (auto& p : G.big)
       if ((mask >> p.first) & 1ull) big.push_back(p);
     if (!big.empty()) {
       BigCombo& c = combo(big);
-      launch_dag_factor_multi(st_, c.bs.as<BigFront>(), c.nf, c.odag.as<int2>(), c.ndag, d_tick_.as<int>(), I,
+      launch_dag_factor_multi(st_, c.bs, c.nf, c.odag, c.ndag, d_tick_.as<int>(), I,
                               dag_grid_, epoch, fp32, c.max2tr);
       flops += c.flops;
       nf += c.nf;
@@ -680,34 +875,35 @@ void Gang::combined_factor(const std::vector<int>& slots, bool fp32) {
   st_acc_.fronts_per_factor += nf;
 }
 
-void Gang::combined_solve(const std::vector<int>& slots, int kv, bool forward) {
+void Gang::combined_solve(const std::vector<int>& slots, int kv, bool forward, bool fp32) {
   uint64_t mask = 0;
   for (int k : slots) mask |= 1ull << k;
   set_mask(mask);
-  const InstDev* I = d_inst_.as<InstDev>();
-  const FrontDesc* F = d_fr_.as<FrontDesc>();
+  const InstDev* I = T_.inst;
+  const FrontDesc* F = T_.fr;
   const int H = (int)lv_.size();
   for (int i = 0; i < H; ++i) {
     const int l = forward ? i : H - 1 - i;
     const GLevel& G = lv_[l];
-    if (forward) launch_fsolve(st_, F, d_items_.as<int>() + G.items_off, G.nsmall, d_childlist_.as<int>(), I, kv);
-    else launch_bsolve(st_, F, d_items_.as<int>() + G.items_off, G.nsmall, I, kv);
+    if (forward) launch_fsolve(st_, F, T_.items + G.items_off, G.nsmall, T_.childlist, I, kv);
+    else launch_bsolve(st_, F, T_.items + G.items_off, G.nsmall, I, kv);
     st_acc_.launches += G.nsmall > 0;
-    // the large fronts, by the precision of their member's current factor
-    for (int prec = 0; prec < 2; ++prec) {
+    // the large fronts (the group's factors are all of one precision)
+    {
+      const int prec = fp32 ? 1 : 0;
       std::vector<std::pair<int, int>> big;
       for (auto& p : G.big)
-        if (((mask >> p.first) & 1ull) && (mem_[p.first]->fac_fp32_ ? 1 : 0) == prec) big.push_back(p);
+        if ((mask >> p.first) & 1ull) big.push_back(p);
       if (big.empty()) continue;
       BigCombo& c = combo(big);
       const int epoch = next_epoch();
       if (forward) {
-        launch_big_fsolve_multi(st_, c.bs.as<BigFront>(), c.bfront.as<int>(), c.nf, c.ofs.as<int2>(), c.nfs,
-                                d_stick_.as<unsigned long long>(), F, d_childlist_.as<int>(), I, kv, epoch, sbase_,
+        launch_big_fsolve_multi(st_, c.bs, c.bfront, c.nf, c.ofs, c.nfs,
+                                d_stick_.as<unsigned long long>(), F, T_.childlist, I, kv, epoch, sbase_,
                                 prec == 1);
         sbase_ += c.nfs;
       } else {
-        launch_big_bsolve_multi(st_, c.bs.as<BigFront>(), c.bfront.as<int>(), c.nf, c.obs.as<int2>(), c.nbs,
+        launch_big_bsolve_multi(st_, c.bs, c.bfront, c.nf, c.obs, c.nbs,
                                 d_stick_.as<unsigned long long>(), F, I, kv, epoch, sbase_, prec == 1, c.maxtr,
                                 c.maxch);
         sbase_ += c.nbs;
@@ -721,14 +917,14 @@ void Gang::issue_group(const std::vector<int>& g) {
   const GOp& op = *op_[g[0]];
   uint64_t mask = 0;
   for (int k : g) mask |= 1ull << k;
-  const InstDev* I = d_inst_.as<InstDev>();
-  const ConeDesc* C = d_cones_.as<ConeDesc>();
+  const InstDev* I = T_.inst;
+  const ConeDesc* C = T_.cones;
   if (op.kind == GK_FACTOR) {
     combined_factor(g, op.a[0] != 0);
     return;
   }
   if (op.kind == GK_LSOLVE || op.kind == GK_USOLVE) {
-    combined_solve(g, op.a[0], op.kind == GK_LSOLVE);
+    combined_solve(g, op.a[0], op.kind == GK_LSOLVE, op.a[1] != 0);
     return;
   }
   set_mask(mask);
@@ -750,26 +946,26 @@ void Gang::issue_group(const std::vector<int>& g) {
       launch_cg_scalars(st_, I, kmax_, op.a[0]);
       break;
     case GK_LIN:
-      launch_lin(st_, I, d_tn_.as<Tile>(), ntn_, d_tm_.as<Tile>(), ntm_, op.N, op.M);
+      launch_lin(st_, I, T_.tn, ntn_, T_.tm, ntm_, op.N, op.M);
       break;
     case GK_AXPBY:
-      if (op.a[0] == 0) launch_axpby(st_, I, d_tn_.as<Tile>(), ntn_, 0, op.a[1], op.a[2], op.a[3], op.c[0], op.c[1]);
-      else launch_axpby(st_, I, d_tm_.as<Tile>(), ntm_, 1, op.a[1], op.a[2], op.a[3], op.c[0], op.c[1]);
+      if (op.a[0] == 0) launch_axpby(st_, I, T_.tn, ntn_, 0, op.a[1], op.a[2], op.a[3], op.c[0], op.c[1]);
+      else launch_axpby(st_, I, T_.tm, ntm_, 1, op.a[1], op.a[2], op.a[3], op.c[0], op.c[1]);
       break;
     case GK_AXPBYPCZ:
       if (op.a[0] == 0)
-        launch_axpbypcz(st_, I, d_tn_.as<Tile>(), ntn_, 0, op.a[1], op.a[2], op.a[3], op.a[4], op.c[0], op.c[1],
+        launch_axpbypcz(st_, I, T_.tn, ntn_, 0, op.a[1], op.a[2], op.a[3], op.a[4], op.c[0], op.c[1],
                         op.c[2]);
       else
-        launch_axpbypcz(st_, I, d_tm_.as<Tile>(), ntm_, 1, op.a[1], op.a[2], op.a[3], op.a[4], op.c[0], op.c[1],
+        launch_axpbypcz(st_, I, T_.tm, ntm_, 1, op.a[1], op.a[2], op.a[3], op.a[4], op.c[0], op.c[1],
                         op.c[2]);
       break;
     case GK_SPMV:
-      launch_spmv(st_, I, d_sp_[op.a[0]].as<Tile>(), nsp_[op.a[0]], op.a[0], op.a[1], op.a[2], op.c[0], op.c[1]);
+      launch_spmv(st_, I, T_.sp[op.a[0]], nsp_[op.a[0]], op.a[0], op.a[1], op.a[2], op.c[0], op.c[1]);
       break;
     case GK_SPMVNM: {
       const int ntm = op.a[0];
-      launch_spmv_nm(st_, I, ntm, d_sp_[ntm].as<Tile>(), op.a[1] ? nsp_[ntm] : 0, d_sp_[0].as<Tile>(),
+      launch_spmv_nm(st_, I, ntm, T_.sp[ntm], op.a[1] ? nsp_[ntm] : 0, T_.sp[0],
                      op.a[2] ? nsp_[0] : 0, op.sn, op.sm);
       break;
     }
@@ -777,30 +973,30 @@ void Gang::issue_group(const std::vector<int>& g) {
       launch_dots(st_, I, 0, op.db, kmax_);
       break;
     case GK_SCALE:
-      launch_cone_scale(st_, I, C, ncone_, d_cone_ofs_.as<int>(), kmax_, d_coneout_.as<double>(), d_psd_.as<int>(),
-                        npsd_, BigPsd{}, d_soc_.as<int>(), nsoc_);
-      launch_cone_hnorm2(st_, I, C, ncone_, d_cone_ofs_.as<int>(), kmax_, d_coneout_.as<double>() + 2 * ncone_, S_TMP1,
-                         d_dense_.as<int>(), ndense_, dense_nmax_, d_dpart_.as<double>(), BigPsd{}, nullptr);
+      launch_cone_scale(st_, I, C, ncone_, T_.cone_ofs, kmax_, T_.coneout, T_.psd,
+                        npsd_, BigPsd{}, T_.soc, nsoc_);
+      launch_cone_hnorm2(st_, I, C, ncone_, T_.cone_ofs, kmax_, T_.coneout + 2 * ncone_, S_TMP1,
+                         T_.dense, ndense_, dense_nmax_, T_.dpart, BigPsd{}, nullptr);
       st_acc_.launches += 3;
       break;
     case GK_MATVEC:
       launch_cone_matvec(st_, I, C, ncone_, op.a[0], op.a[1], op.c[0]);
-      launch_cone_matvec_dense(st_, I, C, d_dense_.as<int>(), ndense_, dense_nmax_, op.a[0], op.a[1], op.c[0]);
+      launch_cone_matvec_dense(st_, I, C, T_.dense, ndense_, dense_nmax_, op.a[0], op.a[1], op.c[0]);
       st_acc_.launches += 1;
       break;
     case GK_CORR0:
       launch_cone_corr0(st_, I, C, ncone_, op.a[0]);
-      launch_psd_corr0(st_, I, C, d_psd_.as<int>(), npsd_, op.a[0]);
+      launch_psd_corr0(st_, I, C, T_.psd, npsd_, op.a[0]);
       st_acc_.launches += 1;
       break;
     case GK_CORR:
       launch_cone_corr(st_, I, C, ncone_, op.a[0], op.a[1], op.a[2]);
-      launch_psd_corr(st_, I, C, d_psd_.as<int>(), npsd_, op.a[0], op.a[1], op.a[2]);
+      launch_psd_corr(st_, I, C, T_.psd, npsd_, op.a[0], op.a[1], op.a[2]);
       st_acc_.launches += 1;
       break;
     case GK_MAXSTEP:
-      launch_cone_maxstep(st_, I, C, ncone_, d_cone_ofs_.as<int>(), kmax_, op.a[0], op.a[1], op.c[0].c,
-                          d_coneout_.as<double>(), d_psd_.as<int>(), npsd_, BigPsd{});
+      launch_cone_maxstep(st_, I, C, ncone_, T_.cone_ofs, kmax_, op.a[0], op.a[1], op.c[0].c,
+                          T_.coneout, T_.psd, npsd_, BigPsd{});
       st_acc_.launches += 2;
       break;
     case GK_ZEROFREE:

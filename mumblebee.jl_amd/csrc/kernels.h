@@ -211,5 +211,20 @@ void launch_gang_smu(hipStream_t, const InstDev*, int k, const GangVals&);
 void launch_gang_active(hipStream_t, const InstDev*, int k, unsigned long long mask, unsigned long long occupied);
 void launch_gang_setinst(hipStream_t, InstDev*, int slot, const InstDev&);
 void launch_gang_gather(hipStream_t, const InstDev*, int k, unsigned long long mask, double* out);
+// one member's assembly arrays: its front storage (nF doubles) and the BᵀB + δQ scatter map; fb0 /
+// ub0: its first block of the fill and of the scatter launch (TILE threads a block, two doubles a
+// thread for the fill)
+struct GangAsm {
+  double* F;
+  long long nF;
+  const int64_t* pos;
+  const double* l;
+  const double* q;
+  long long nent;
+  long long fb0, ub0;
+};
+// the fill and the scatter of all slots of the table A (k entries) in one launch each
+void launch_gang_assemble(hipStream_t, const InstDev*, const GangAsm* A, int k, long long fblocks,
+                          long long ublocks);
 
 }  // namespace mb

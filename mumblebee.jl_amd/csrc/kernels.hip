@@ -1512,6 +1512,40 @@ void launch_gang_smu(hipStream_t st, const InstDev* I, int k, const GangVals& v)
   k_gang_smu<<<1, GANG_SLOTS, 0, st>>>(I, k, v);
 }
 
+// the fronts of every member taking part set to zero, then BᵀB + δQ scattered into them (the
+// members' own k_fill and k_assemble_union in one launch each); block b belongs to the member i
+// with A[i].fb0 ≤ b < A[i+1].fb0 (ub0 likewise), A holding k+1 entries
+__device__ __forceinline__ int gang_member(const GangAsm* A, int k, long long b, bool fill) {
+  int i = 0;
+  while (i + 1 < k && (fill ? A[i + 1].fb0 : A[i + 1].ub0) <= b) ++i;
+  return i;
+}
+__global__ void k_gang_fill(const InstDev* insts, const GangAsm* A, int k) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const long long b = blockIdx.x;
+  const int i = gang_member(A, k, b, true);
+  if (insts[i].sc[S_ACTIVE] == 0.0) return;
+  const GangAsm& M = A[i];
+  const long long t = (b - M.fb0) * blockDim.x + threadIdx.x;
+  if (t < M.nF / 2) reinterpret_cast<d2*>(M.F)[t] = d2{0.0, 0.0};
+  if ((M.nF & 1) && t == 0) M.F[M.nF - 1] = 0.0;
+}
+__global__ void k_gang_union(const InstDev* insts, const GangAsm* A, int k) {
+  const long long b = blockIdx.x;
+  const int i = gang_member(A, k, b, false);
+  const InstDev& I = insts[i];
+  if (I.sc[S_ACTIVE] == 0.0) return;
+  const GangAsm& M = A[i];
+  const long long t = (b - M.ub0) * blockDim.x + threadIdx.x;
+  if (t >= M.nent) return;
+  M.F[M.pos[t]] = M.l[t] + I.sc[S_DELTA] * M.q[t];
+}
+void launch_gang_assemble(hipStream_t st, const InstDev* I, const GangAsm* A, int k, long long fblocks,
+                          long long ublocks) {
+  if (fblocks > 0) k_gang_fill<<<(unsigned)fblocks, TILE, 0, st>>>(I, A, k);
+  if (ublocks > 0) k_gang_union<<<(unsigned)ublocks, TILE, 0, st>>>(I, A, k);
+}
+
 // S_ACTIVE = bit i of mask for the occupied slots
 __global__ void k_gang_active(const InstDev* insts, int k, unsigned long long mask, unsigned long long occupied) {
   const int i = threadIdx.x;

@@ -1190,14 +1190,22 @@ double Solver::front_flops() const {
 // device helpers
 // ------------------------------------------------------------------------------------------
 
-void Solver::gang_submit(GOp& op) { gang_->submit(this, op); }
+// The solver whose own launches the calling thread is issuing for a gang (the leader of a tick),
+// so that they launch instead of going to the gang again. Per thread: the member's own thread
+// goes on handing in ops meanwhile.
+static thread_local const Solver* g_issuing = nullptr;
+void Solver::set_issuing(const Solver* s) { g_issuing = s; }
+bool Solver::ganged() const { return gang_ != nullptr && g_issuing != this; }
+
+void Solver::gang_submit(GOp* op) { gang_->submit(this, op); }
 
 void Solver::sync_scalars() {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_READBACK;
+    op.blocking = true;
     op.solo = [this] { raw_sync_scalars(); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   raw_sync_scalars();
@@ -1212,13 +1220,13 @@ void Solver::raw_sync_scalars() {
 void Solver::set_scalar(int slot, double v) {
   hsc_[slot] = v;
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SET2;
     op.a[0] = slot;
     op.a[1] = -1;
     op.v[0] = v;
     op.solo = [this, slot, v] { launch_set_scalar(st_, d_sc_.as<double>(), slot, v); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   // the value travels as a kernel argument: no host buffer outlives the call
@@ -1227,14 +1235,14 @@ void Solver::set_scalar(int slot, double v) {
 
 void Solver::set_scalar2(int s1, double v1, int s2, double v2) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SET2;
     op.a[0] = s1;
     op.a[1] = s2;
     op.v[0] = v1;
     op.v[1] = v2;
     op.solo = [this, s1, v1, s2, v2] { launch_set_scalar2(st_, d_sc_.as<double>(), s1, v1, s2, v2); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_set_scalar2(st_, d_sc_.as<double>(), s1, v1, s2, v2);
@@ -1242,13 +1250,13 @@ void Solver::set_scalar2(int s1, double v1, int s2, double v2) {
 
 void Solver::smu(double nu, double mu, double mut) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SMU;
     op.v[0] = nu;
     op.v[1] = mu;
     op.v[2] = mut;
     op.solo = [this, nu, mu, mut] { launch_smu(st_, d_sc_.as<double>(), nu, mu, mut); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_smu(st_, d_sc_.as<double>(), nu, mu, mut);
@@ -1256,11 +1264,11 @@ void Solver::smu(double nu, double mu, double mut) {
 
 void Solver::cg_scalars(int which) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_CGSC;
     op.a[0] = which;
     op.solo = [this, which] { launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, which); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_cg_scalars(st_, d_inst_.as<InstDev>(), 1, which);
@@ -1268,12 +1276,12 @@ void Solver::cg_scalars(int which) {
 
 void Solver::axpby(int space, int kx, int ky, int kz, Coef a, Coef b) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_AXPBY;
     op.a[0] = space, op.a[1] = kx, op.a[2] = ky, op.a[3] = kz;
     op.c[0] = a, op.c[1] = b;
     op.solo = [=] { axpby(space, kx, ky, kz, a, b); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   const InstDev* I = d_inst_.as<InstDev>();
@@ -1283,12 +1291,12 @@ void Solver::axpby(int space, int kx, int ky, int kz, Coef a, Coef b) {
 
 void Solver::axpbypcz(int space, int kx, int ky, int kw, int kz, Coef a, Coef b, Coef c) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_AXPBYPCZ;
     op.a[0] = space, op.a[1] = kx, op.a[2] = ky, op.a[3] = kw, op.a[4] = kz;
     op.c[0] = a, op.c[1] = b, op.c[2] = c;
     op.solo = [=] { axpbypcz(space, kx, ky, kw, kz, a, b, c); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   const InstDev* I = d_inst_.as<InstDev>();
@@ -1309,12 +1317,12 @@ void Solver::dot(int space, int kx, int ky, int mode, int slot, int op, int kw0,
 // several dot products in one launch while the vectors are short enough for one workgroup each
 void Solver::dots(const DotBatch& b) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_DOTS;
     op.db = b;
     op.solo_only = std::max(n_, m_) > DOTS_LEN_MAX;
     op.solo = [this, b] { dots(b); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   if (std::max(n_, m_) <= DOTS_LEN_MAX) {
@@ -1329,12 +1337,12 @@ void Solver::dots(const DotBatch& b) {
 
 void Solver::lin(const LinSet& N, const LinSet& M) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_LIN;
     op.N = N;
     op.M = M;
     op.solo = [this, N, M] { lin(N, M); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_lin(st_, d_inst_.as<InstDev>(), d_ntiles_.as<Tile>(), ntile_n_, d_mtiles_.as<Tile>(), ntile_m_, N, M);
@@ -1351,7 +1359,7 @@ void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
   else if (N && N->kxq >= 0) ntm = 2;
   else if (N && N->kxt >= 0) ntm = 1;
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SPMVNM;
     op.a[0] = ntm;
     op.a[1] = N ? 1 : 0;
@@ -1363,7 +1371,7 @@ void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
     const bool hn = N != nullptr, hm = M != nullptr;
     op.solo = [=] { launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), hn ? nsptile_[ntm] : 0,
                                    d_sptiles_[0].as<Tile>(), hm ? nsptile_[0] : 0, nn, mm); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_spmv_nm(st_, d_inst_.as<InstDev>(), ntm, d_sptiles_[ntm].as<Tile>(), N ? nsptile_[ntm] : 0,
@@ -1372,12 +1380,12 @@ void Solver::spmv_nm(int ntm, const SpmvN* N, const SpmvM* M) {
 
 void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SPMV;
     op.a[0] = which, op.a[1] = kx, op.a[2] = ky;
     op.c[0] = a, op.c[1] = b;
     op.solo = [=] { spmv(which, kx, ky, a, b); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   const InstDev* I = d_inst_.as<InstDev>();
@@ -1387,11 +1395,11 @@ void Solver::spmv(int which, int kx, int ky, Coef a, Coef b) {
 // scale! (cone.jl:146-247) with the cone blocks' part of ‖H‖² (into S_TMP1)
 void Solver::cone_scale_norm() {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SCALE;
     op.solo_only = nbpsd_ > 0;
     op.solo = [this] { cone_scale_norm(); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_cone_scale(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1,
@@ -1404,12 +1412,12 @@ void Solver::cone_scale_norm() {
 // initpredictor! with μt > 0: the right-hand side r = −d + σμ(−f′(p)) per cone into kind kr
 void Solver::corr0_op(int kr) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_CORR0;
     op.a[0] = kr;
     op.solo_only = nbpsd_ > 0;
     op.solo = [this, kr] { corr0_op(kr); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_cone_corr0(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kr);
@@ -1420,12 +1428,12 @@ void Solver::corr0_op(int kr) {
 // initcorrector! per cone into kind kr from the affine direction (kdp, kdd)
 void Solver::corr_op(int kr, int kdp, int kdd) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_CORR;
     op.a[0] = kr, op.a[1] = kdp, op.a[2] = kdd;
     op.solo_only = nbpsd_ > 0;
     op.solo = [this, kr, kdp, kdd] { corr_op(kr, kdp, kdd); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_cone_corr(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kr, kdp, kdd);
@@ -1435,11 +1443,11 @@ void Solver::corr_op(int kr, int kdp, int kdd) {
 
 void Solver::zerofree(int kx) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_ZEROFREE;
     op.a[0] = kx;
     op.solo = [this, kx] { zerofree(kx); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_zerofree(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, kx);
@@ -1447,33 +1455,35 @@ void Solver::zerofree(int kx) {
 
 void Solver::lsolve(int kv) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_LSOLVE;
     op.park = true;
     op.a[0] = kv;
     op.a[1] = fac_fp32_ ? 1 : 0;
-    op.solo = [this, kv] { raw_lsolve(kv); };
-    gang_submit(op);
+    const bool fp32 = fac_fp32_;
+    op.solo = [this, kv, fp32] { raw_lsolve(kv, fp32); };
+    gang_submit(&op);
     return;
   }
-  raw_lsolve(kv);
+  raw_lsolve(kv, fac_fp32_);
 }
 
 void Solver::usolve(int kv) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_USOLVE;
     op.park = true;
     op.a[0] = kv;
     op.a[1] = fac_fp32_ ? 1 : 0;
-    op.solo = [this, kv] { raw_usolve(kv); };
-    gang_submit(op);
+    const bool fp32 = fac_fp32_;
+    op.solo = [this, kv, fp32] { raw_usolve(kv, fp32); };
+    gang_submit(&op);
     return;
   }
-  raw_usolve(kv);
+  raw_usolve(kv, fac_fp32_);
 }
 
-void Solver::raw_lsolve(int kv) {
+void Solver::raw_lsolve(int kv, bool fp32) {
   for (const Level& L : levels_) {
     int ps = prof_begin(PROF_FSOLVE);
     launch_fsolve(st_, d_fr_.as<FrontDesc>(), d_solve_items_.as<int>() + L.items_off, L.nsmall,
@@ -1484,14 +1494,14 @@ void Solver::raw_lsolve(int kv) {
       BigHost& bh = *big_[L.big[i]];
       epoch_ = next_epoch();
       bh.sbase += launch_big_fsolve(k ? aux_[i % k] : st_, bh.B, d_fr_.as<FrontDesc>(), bh.front,
-                                    d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
+                                    d_childlist_.as<int>(), d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fp32);
     }
     if (k) join_aux(k);
     prof_end(ps, PROF_FSOLVE, 0.0, L.solve_bytes);
   }
 }
 
-void Solver::raw_usolve(int kv) {
+void Solver::raw_usolve(int kv, bool fp32) {
   for (int l = (int)levels_.size() - 1; l >= 0; --l) {
     const Level& L = levels_[l];
     int ps = prof_begin(PROF_BSOLVE);
@@ -1503,7 +1513,7 @@ void Solver::raw_usolve(int kv) {
       BigHost& bh = *big_[L.big[i]];
       epoch_ = next_epoch();
       bh.sbase += launch_big_bsolve(k ? aux_[i % k] : st_, bh.B, d_fr_.as<FrontDesc>(), bh.front,
-                                    d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fac_fp32_);
+                                    d_inst_.as<InstDev>(), kv, epoch_, bh.sbase, fp32);
     }
     if (k) join_aux(k);
     prof_end(ps, PROF_BSOLVE, 0.0, L.solve_bytes);
@@ -1654,13 +1664,13 @@ void Solver::raw_factor_run(bool fp32) {
 
 void Solver::factor_run(bool fp32) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_FACTOR;
     op.park = true;
     op.a[0] = fp32 ? 1 : 0;
     op.solo_only = user_h_ || nbpsd_ > 0;
     op.solo = [this, fp32] { raw_factor_run(fp32); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   raw_factor_run(fp32);
@@ -1704,10 +1714,11 @@ bool Solver::factor(double delta, double& rho) {
     for (double x : part) s += x;
   };
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SOLO;
+    op.blocking = true;
     op.solo = fnorm;
-    gang_submit(op);
+    gang_submit(&op);
   } else {
     fnorm();
   }
@@ -1720,10 +1731,11 @@ bool Solver::factor(double delta, double& rho) {
   };
   fac_fp32_ = false;
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_SOLO;
+    op.blocking = true;
     op.solo = refactor;
-    gang_submit(op);
+    gang_submit(&op);
   } else {
     refactor();
   }
@@ -1839,13 +1851,13 @@ double Solver::cone_maxsteps(const double* dp, const double* dd, double step_fra
 // the step into S_TAU on the device (no read-back)
 void Solver::launch_maxsteps(int kdp, int kdd, double step_frac) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_MAXSTEP;
     op.a[0] = kdp, op.a[1] = kdd;
     op.c[0] = K(step_frac);
     op.solo_only = nbpsd_ > 0;
     op.solo = [this, kdp, kdd, step_frac] { launch_maxsteps(kdp, kdd, step_frac); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   launch_cone_maxstep(st_, d_inst_.as<InstDev>(), d_cones_.as<ConeDesc>(), ncone_, d_cone_ofs_.as<int>(), 1, kdp,
@@ -2213,13 +2225,13 @@ void Solver::dshadow(int kx, int ky) {
 // y += a (H − Q) x: the cone blocks of H
 void Solver::cone_matvec_all(int kx, int ky, Coef a) {
   if (ganged()) {
-    GOp op;
+    GOp& op = *new GOp;
     op.kind = GK_MATVEC;
     op.a[0] = kx, op.a[1] = ky;
     op.c[0] = a;
     op.solo_only = nbpsd_ > 0;
     op.solo = [this, kx, ky, a] { cone_matvec_all(kx, ky, a); };
-    gang_submit(op);
+    gang_submit(&op);
     return;
   }
   const InstDev* I = d_inst_.as<InstDev>();
@@ -2572,18 +2584,7 @@ int Solver::solve() {
   return status;
 }
 
-int Solver::solve_in(Gang& g) {
-  g.join(this);
-  int status = CONTINUE;
-  try {
-    while (status == CONTINUE) status = step();
-  } catch (...) {
-    g.leave(this);
-    throw;
-  }
-  g.leave(this);
-  return status;
-}
+int Solver::solve_in(Gang& g) { return g.run(this); }
 
 // result (solver/ipm.jl:30-74)
 void Solver::result(double* p, double* d, double* y, double* info) {

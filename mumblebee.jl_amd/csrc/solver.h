@@ -173,8 +173,10 @@ class Solver {
   void build_schedule();
   // device ops. Inside a gang each of these is handed to the gang (one shared launch per phase
   // for all members); the raw_ forms launch on the solver's own tables
-  bool ganged() const { return gang_ && !issuing_; }
-  void gang_submit(GOp& op);
+  bool ganged() const;
+  // the gang's leader runs a member's own launches (on the leader's thread) between these
+  static void set_issuing(const Solver* s);
+  void gang_submit(GOp* op);  // the gang owns op from here (a blocking op: until it returns)
   void sync_scalars();
   void raw_sync_scalars();
   void set_scalar(int slot, double v);
@@ -199,8 +201,8 @@ class Solver {
   void spmv(int which, int kx, int ky, Coef a, Coef b);
   void lsolve(int kv);
   void usolve(int kv);
-  void raw_lsolve(int kv);
-  void raw_usolve(int kv);
+  void raw_lsolve(int kv, bool fp32);
+  void raw_usolve(int kv, bool fp32);
   bool factor(double delta, double& rho);
   void run_factor(bool fp32);
   double maxsteps(int kdp, int kdd, double step_frac);
@@ -349,7 +351,6 @@ class Solver {
   HostLists hl_;
   Gang* gang_ = nullptr;
   int gslot_ = -1;
-  bool issuing_ = false;       // the gang's leader is running this solver's own launches
   int gang_prof_ = 0;          // the profile mask to restore on leaving
   hipStream_t own_st_ = nullptr;
 };
