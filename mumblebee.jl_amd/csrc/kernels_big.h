@@ -32,6 +32,14 @@ struct BigFront {
   double* Nt;           // Tr FP64 64×64 blocks N_bᵀ, N_b = L_{b+1,b} L_bb⁻¹ (backward-solve chain step)
   long long* sticket;   // 64-bit monotone ticket of the solve kernels
   int inst;
+  // Write-once factor storage (both set, or both null): L holds the FP64 factor columns (ld × r,
+  // same indexing as F) and stg the staged sub-diagonal tiles (Tr 64×64 tiles of the factor's
+  // element type). Every location the factorization reads after another workgroup wrote it is
+  // then written once per launch and read only after its flag, so those reads go through the
+  // caches (no cache can hold an older copy: nothing read it earlier in the launch). With null,
+  // the FP64 factor overwrites F in place and such reads take the device-coherent path.
+  double* L;
+  double* stg;
 };
 
 // fp32: the factor in FP32 on FP32 MFMA (B.F32 / B.Dinv32 must be allocated); the solves read

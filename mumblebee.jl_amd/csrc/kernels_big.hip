@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <stdexcept>
 
 #include "device.h"
 #include "kernels.h"
@@ -97,7 +98,7 @@ __device__ __forceinline__ int tri_index(int i, int j) { return i * (i + 1) / 2 
 template <class T>
 __device__ __forceinline__ T* factor_of(const BigFront& B);
 template <>
-__device__ __forceinline__ double* factor_of<double>(const BigFront& B) { return B.F; }
+__device__ __forceinline__ double* factor_of<double>(const BigFront& B) { return B.L ? B.L : B.F; }
 template <>
 __device__ __forceinline__ float* factor_of<float>(const BigFront& B) { return B.F32; }
 template <class T>
@@ -107,14 +108,53 @@ __device__ __forceinline__ double* dinv_of<double>(const BigFront& B) { return B
 template <>
 __device__ __forceinline__ float* dinv_of<float>(const BigFront& B) { return B.Dinv32; }
 
+// Makes sure that columns < need of factor rows i and j are final (their tiles' flags hold the
+// epoch), advancing `ready` (uniform). Wave 0 reads the flags of up to 64 columns from `ready` on
+// at once, so a task whose inputs are final already pays one round trip for all of them rather
+// than one per column; a column not final yet is waited for (bounded) and the poll repeats.
+// False on abort or at the wait limit.
+__device__ __forceinline__ bool cols_final(const BigFront& B, int i, int j, int kmax, int need, int epoch,
+                                           int& ready) {
+  __shared__ int ready_s;
+  while (ready < need) {
+    if (threadIdx.x < 64) {
+      const int l = ready + (int)threadIdx.x;
+      bool ok = true;
+      if (l < kmax) {
+        ok = ld_flag(&B.state[tri_index(i, l)]) >= epoch;
+        if (i != j) ok = ok && ld_flag(&B.state[tri_index(j, l)]) >= epoch;
+      }
+      const unsigned long long bad = __ballot(!ok);
+      if (threadIdx.x == 0) ready_s = ready + (bad ? __ffsll((long long)bad) - 1 : 64);
+    }
+    __syncthreads();
+    const int r = min(ready_s, kmax);
+    __syncthreads();
+    if (r >= need) {
+      ready = r;
+      break;
+    }
+    ready = r;
+    // column `ready` is not final: wait for it, then poll on
+    if (!wait_geq(&B.state[tri_index(i, ready)], epoch, B.abortf) ||
+        (i != j && !wait_geq(&B.state[tri_index(j, ready)], epoch, B.abortf)))
+      return false;
+  }
+  return true;
+}
+
 // One task of the tile DAG (tasks[t] of front B), by the whole workgroup. False when a wait
-// ended on the abort flag or at its limit.
-template <class T>
+// ended on the abort flag or at its limit. SEP: write-once factor storage (B.L / B.F32 and B.stg),
+// so the final tiles and inverted diagonal blocks are read through the caches (CONS); otherwise
+// they are read device-coherent.
+template <class T, bool SEP>
 __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, int epoch, int t, T* SA, T* SB, T* SC,
                                          T* SD) {
+  constexpr Coh CONS = SEP ? Coh::None : Coh::Dev;
   double* F = B.F;
   T* Lf = factor_of<T>(B);
   T* Di = dinv_of<T>(B);
+  T* Stg = reinterpret_cast<T*>(B.stg);
   const int LD = B.ld;
   stamp(B.trace, t, 0);
   const int4 task = B.tasks[t];
@@ -131,13 +171,13 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
   // MFMAs of column k run from the other LDS buffer pair
   bool alive = true;
   TileRegs<T> RA, RB;
+  // columns < ready hold final tiles in rows i and j (uniform across the workgroup)
+  int ready = 0;
   if (kmax > 0) {
-    if (!wait_geq(&B.state[tri_index(i, 0)], epoch, B.abortf) ||
-        (i != j && !wait_geq(&B.state[tri_index(j, 0)], epoch, B.abortf)))
-      return false;
+    if (!cols_final(B, i, j, kmax, 1, epoch, ready)) return false;
     const int c00 = B.tb[0], w0 = B.tb[1] - c00;
-    fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c00, hi, w0);
-    if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c00, hj, w0);
+    fetch_tile<CONS>(RA, Lf, LD, r0i, c00, hi, w0);
+    if (i != j) fetch_tile<CONS>(RB, Lf, LD, r0j, c00, hj, w0);
   }
   for (int k = 0; k < kmax; ++k) {
     T* XA = (k & 1) ? SC : SA;
@@ -146,14 +186,13 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
     if (i != j) put_tile(XB, RB);
     __syncthreads();  // also: the buffer pair written next was last read before this point
     if (k + 1 < kmax) {
-      if (!wait_geq(&B.state[tri_index(i, k + 1)], epoch, B.abortf) ||
-          (i != j && !wait_geq(&B.state[tri_index(j, k + 1)], epoch, B.abortf))) {
+      if (k + 1 >= ready && !cols_final(B, i, j, kmax, k + 2, epoch, ready)) {
         alive = false;
         break;
       }
       const int c0n = B.tb[k + 1], wn = B.tb[k + 2] - c0n;
-      fetch_tile<Coh::Dev>(RA, Lf, LD, r0i, c0n, hi, wn);
-      if (i != j) fetch_tile<Coh::Dev>(RB, Lf, LD, r0j, c0n, hj, wn);
+      fetch_tile<CONS>(RA, Lf, LD, r0i, c0n, hi, wn);
+      if (i != j) fetch_tile<CONS>(RB, Lf, LD, r0j, c0n, hj, wn);
     }
     mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
   }
@@ -166,9 +205,10 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
       // ready), then C_jj −= L_{j,j−1} L_{j,j−1}ᵀ
       const int c0p = B.tb[j - 1], wp = B.tb[j] - c0p;
       if (!wait_geq(&B.stage[j - 1], epoch, B.abortf)) return false;
-      load_tile<Coh::Dev>(SA, Lf, LD, r0j, c0p, hj, wp);
+      if (SEP) load_tile<Coh::Dev>(SA, Stg + (size_t)(j - 1) * NB * NB, NB, 0, 0, hj, wp);
+      else load_tile<Coh::Dev>(SA, Lf, LD, r0j, c0p, hj, wp);
       if (!wait_geq(&B.state[tri_index(j - 1, j - 1)], epoch, B.abortf)) return false;
-      load_tile<Coh::Dev>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
+      load_tile<CONS>(SB, Di + (size_t)(j - 1) * NB * NB, NB, 0, 0, NB, NB);
       __syncthreads();
       Acc<T> X;
       acc_zero(X);
@@ -196,14 +236,15 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
     acc_to_lds(C, SA);
     __syncthreads();
     stamp(B.trace, t, 2);
-    store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
+    if (SEP) store_tile<Coh::Dev>(Stg + (size_t)j * NB * NB, NB, 0, 0, hi, hj, SA, false);
+    else store_tile<Coh::Dev>(Lf, LD, r0i, r0j, hi, hj, SA, false);
     publish(&B.stage[j], epoch);
   } else if (type == 1) {
     // L_ij = C_ij L_jj⁻ᵀ
     acc_to_lds(C, SA);
     if (!wait_geq(&B.state[tri_index(j, j)], epoch, B.abortf)) return false;
     stamp(B.trace, t, 2);
-    load_tile<Coh::Dev>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
+    load_tile<CONS>(SB, Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB);
     __syncthreads();
     Acc<T> X;
     acc_zero(X);
@@ -225,7 +266,7 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
   return true;
 }
 
-template <class T>
+template <class T, bool SEP>
 __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* insts, int epoch) {
   __shared__ __align__(16) T SA[NB * LROW];
   __shared__ __align__(16) T SB[NB * LROW];
@@ -244,7 +285,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     const int t = task_s;
     __syncthreads();
     if (t >= B.ntask) break;
-    if (!dag_task<T>(B, I, epoch, t, SA, SB, SC, SD)) break;
+    if (!dag_task<T, SEP>(B, I, epoch, t, SA, SB, SC, SD)) break;
   }
 }
 
@@ -271,7 +312,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor_multi(const BigFront* Bs, con
     const BigFront& B = Bs[o.x];
     const InstDev& I = insts[B.inst];
     if (I.sc[S_ACTIVE] == 0.0 || ld_flag(B.abortf) != 0) continue;
-    (void)dag_task<T>(B, I, epoch, o.y, SA, SB, SC, SD);
+    (void)dag_task<T, true>(B, I, epoch, o.y, SA, SB, SC, SD);
   }
 }
 
@@ -786,8 +827,11 @@ __global__ void __launch_bounds__(NT) k_big_bsolve_multi(const BigFront* Bs, con
 // The ticket starts at 0 (zeroed with the front's buffers, then put back by k_big_mtiles after
 // each factorization, which launch_big_mtiles must follow); the abort flag stays raised once set.
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
-  if (fp32) k_dag_factor<float><<<grid, NT, 0, st>>>(B, I, epoch);
-  else k_dag_factor<double><<<grid, NT, 0, st>>>(B, I, epoch);
+  const bool sep = B.stg != nullptr;
+  if (fp32 && !sep) throw std::invalid_argument("FP32 factorization without staging tiles");
+  if (fp32) k_dag_factor<float, true><<<grid, NT, 0, st>>>(B, I, epoch);
+  else if (sep && B.L) k_dag_factor<double, true><<<grid, NT, 0, st>>>(B, I, epoch);
+  else k_dag_factor<double, false><<<grid, NT, 0, st>>>(B, I, epoch);
 }
 
 void launch_big_mtiles(hipStream_t st, const BigFront& B, const InstDev* I, bool fp32) {

@@ -464,6 +464,8 @@ double Solver::estimate_device_bytes() const {
     bigs.push_back(8.0 * NB * std::max(1.0, Tr) * std::max(1.0, std::ceil(sc / NB)));  // solve partials
     bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // M tiles
     bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // N tiles
+    bigs.push_back(8.0 * ld * std::max(1.0, r));             // FP64 factor columns
+    bigs.push_back(8.0 * NB * NB * std::max(1.0, Tr));       // staged sub-diagonal tiles
   }
   for (double b : bigs) buf(b);
   double dsz = 0.0, csz = 0.0, qsz = 0.0, wsz = 0.0;
@@ -956,6 +958,9 @@ void Solver::build_schedule() {
       bh->spart.alloc(sizeof(double) * NB * std::max(1, Tr) * std::max(1, (Fd.s + NB - 1) / NB));
       bh->mtile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       bh->ntile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
+      // write-once factor storage: the FP64 factor columns beside F, the staged sub-diagonal tiles
+      bh->lsep.alloc(sizeof(double) * (size_t)Fd.ld * std::max(1, Fd.r));
+      bh->stg.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       HIPCHK(hipMemsetAsync(bh->sll.p, 0, bh->sll.bytes, st_));
       BigFront& B = bh->B;
       B.F = Fd.F;
@@ -983,6 +988,8 @@ void Solver::build_schedule() {
       B.Nt = bh->ntile.as<double>();
       B.sticket = d_bigstick_.as<long long>() + big_.size();
       B.inst = Fd.inst;
+      B.L = bh->lsep.as<double>();
+      B.stg = bh->stg.as<double>();
       bh->front = f;
       double r = Fd.r, s = Fd.s;
       L.big_flops += r * r * r / 3.0 + s * r * r + s * s * r;
@@ -2639,6 +2646,11 @@ void Solver::result(double* p, double* d, double* y, double* info) {
 void Solver::dense_factor(double* out) {
   std::vector<double> F(fsize_);
   HIPCHK(hipMemcpyAsync(F.data(), d_F_.p, fsize_ * 8, hipMemcpyDeviceToHost, st_));
+  // a large front's FP64 factor columns live beside its front
+  for (auto& bh : big_)
+    if (bh->B.L)
+      HIPCHK(hipMemcpyAsync(F.data() + fofs_[bh->front], bh->B.L, sizeof(double) * (size_t)bh->B.ld * bh->B.r,
+                            hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   std::memset(out, 0, sizeof(double) * n_ * n_);
   size_t ofs = 0;

@@ -305,7 +305,7 @@ class Solver {
     BigFront B;
     int front;
     long long sbase = 0;
-    DevBuf tb, state, tasks, sll, spart, mtile, ntile, dinv, stage, trace, f32, dinv32;
+    DevBuf tb, state, tasks, sll, spart, mtile, ntile, dinv, stage, trace, f32, dinv32, lsep, stg;
   };
   std::vector<std::unique_ptr<BigHost>> big_;
   // auxiliary streams for the independent large fronts of one level, with a fork event and one
