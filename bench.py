@@ -40,9 +40,11 @@ PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
 METRIC = "solves/s + p99 latency at 1e-8 gap (mixed vs plain FP64); factor TFLOP/s"
 ARGS = None
-# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-# scaled by a 256 MiB device copy (profiles/k_dag_factor_qp2000_fp64.md); keyed by (n, m)
-TRAFFIC_BYTES = {(2000, 1000, 0): 141.5e6}
+# HBM bytes per large front of the dominant kernel's launches, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes scaled by a 256 MiB device copy (FETCH_SIZE ×2, WRITE_SIZE ×1); keyed by (n, m,
+# gang): the single-front kernel (profiles/k_dag_factor_qp2000_fp64.md) and the gang's combined
+# launch, 16 fronts of which read 3.25 GB and write 0.29 GB (profiles/k_dag_factor_multi_qp2000_gang16.md)
+TRAFFIC_PER_FRONT = {(2000, 1000, False): 141.5e6, (2000, 1000, True): 221.1e6}
 
 
 WORKLOADS = {
@@ -398,8 +400,10 @@ def bench_resident(args, mh):
             "peak": PEAK_FP64_MFMA,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_MFMA,
-            "traffic": (TRAFFIC_BYTES.get((args.n, args.m, args.gang)) if dom_name == "dag" and args.workload == "qp"
-                        else None),
+            "traffic": (TRAFFIC_PER_FRONT[(args.n, args.m, gst is not None)] *
+                        (gst["factor_fronts"] / max(1.0, gst["factor_launches"]) if gst is not None else 1.0)
+                        if dom_name == "dag" and args.workload == "qp" and (args.n, args.m, gst is not None)
+                        in TRAFFIC_PER_FRONT else None),
             "kernel": (("k_dag_factor_multi (one persistent left-looking tile FP64 MFMA Cholesky launch over "
                         "the large fronts of every gang member)" if gst is not None else
                         "k_dag_factor (persistent left-looking tile FP64 MFMA Cholesky of the large fronts)")
@@ -459,7 +463,8 @@ def bench_serve(args, mh):
     for k in range(K):
         reqs.append(hip_problem(mh, G.served_request(rng, k, args.serve_nmax)))
     gaps = rng.exponential(1.0 / args.rate, K)
-    server = mh.IPMServer(nslots=args.slots)
+    nslots = args.gang if args.gang else args.slots
+    server = mh.IPMServer(nslots=nslots, gang=bool(args.gang))
     settings = mh.IPMSettings(precision=args.precision)
     t0 = mh.clock()
     arrivals, tickets = [], []
@@ -486,7 +491,8 @@ def bench_serve(args, mh):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"served stream: {K} requests, 60% QP / 30% SOCP / 10% chordal max-cut SDP, n "
                                f"log-uniform in [50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, "
-                               f"{args.slots} slots",
+                               + (f"{nslots} slots as the members of one gang (lockstep, shared launches)"
+                                  if args.gang else f"{nslots} slots, each on its own stream"),
                    "precision": "mixed" if args.precision else "fp64",
                    "optimal": ok, "statuses": statuses, "p50_ms": float(np.percentile(lat_ms, 50)),
                    "p99_ms": float(np.percentile(lat_ms, 99)), "peak_reserved_gb": st["peak_reserved_bytes"] / 2 ** 30},
@@ -510,12 +516,12 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--slots", type=int, default=4,
                     help="the CUs of a large-front factorization are 1/slots of the chip; the resident-loop "
                          "measurement solves this many at a time")
-    ap.add_argument("--threads", type=int, default=24, help="host threads running solve(prob, settings)")
+    ap.add_argument("--threads", type=int, default=32, help="host threads running solve(prob, settings)")
     ap.add_argument("--gang", type=int, default=16,
                     help="members of the lockstep gang the solves run in (0: each solver on its own stream)")
     ap.add_argument("--no-loop", action="store_true", help="skip the resident-loop measurement")

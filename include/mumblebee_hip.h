@@ -23,6 +23,8 @@
  *   mb_cone_corr0            initpredictor! / corr0!                      src/IPM/src/cone/cone.jl:361-444
  *   mb_cone_corr             initcorrector! / corr!                       src/IPM/src/cone/cone.jl:261-352
  *   mb_cone_maxsteps         maxsteps(sched, K, p, d, Δp, Δd, …)          src/IPM/src/cone/cone.jl:455-553
+ *   mb_solver_solve_in,      solve!(solver) for many solvers at once      src/IPM/src/ipm/solver/solver.jl:119-150
+ *   mb_solve_batch           (lockstep gang: no counterpart; each result equals solve!'s)
  *
  * Sparse inputs are CSC with 0-based int64 indices, rows sorted within each column (stored zeros
  * are kept, as in Julia's SparseMatrixCSC). Cone kinds: 0 nonnegative, 1 free ("cofree"),

@@ -28,6 +28,7 @@ struct Gang::BigCombo {
   int* bfront = nullptr;
   int2 *odag = nullptr, *ofs = nullptr, *obs = nullptr;
   int nf = 0, ndag = 0, nfs = 0, nbs = 0, max2tr = 0, maxtr = 0, maxch = 0;
+  DagQueues q{};
   double flops = 0.0;
 };
 
@@ -155,8 +156,6 @@ bool same_op(const GOp& x, const GOp& y) {
 Gang::Gang(int kmax, int dag_grid, bool profile)
     : kmax_(std::max(1, std::min(kmax, GANG_MAX))), dag_grid_(dag_grid), profile_(profile) {
   HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-  HIPCHK(hipEventCreate(&ev0_));
-  HIPCHK(hipEventCreate(&ev1_));
   HIPCHK(hipEventCreateWithFlags(&evleave_, hipEventDisableTiming));
   if (dag_grid_ <= 0) {
     int dev = 0, ncu = 256;
@@ -201,10 +200,24 @@ Gang::~Gang() {
   tab_.reset();
   for (auto& b : pin_free_) (void)hipHostFree(b.first);
   if (h_gather_) (void)hipHostFree(h_gather_);
-  (void)hipEventDestroy(ev0_);
-  (void)hipEventDestroy(ev1_);
+  for (auto& p : timed_) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);
   (void)hipEventDestroy(evleave_);
   (void)hipStreamDestroy(st_);
+}
+
+hipEvent_t Gang::event() {
+  if (evpool_.empty()) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = evpool_.back();
+  evpool_.pop_back();
+  return e;
 }
 
 int Gang::members() {
@@ -585,14 +598,16 @@ void Gang::readbacks(const std::vector<int>& slots) {
   for (int k : slots) std::memcpy(mem_[k]->hsc_, h_gather_ + (size_t)k * NSCALAR, sizeof(double) * NSCALAR);
   st_acc_.readbacks += 1;
   st_acc_.launches += 1;
-  if (profile_ && pending_timing_) {
+  for (auto& p : timed_) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev0_, ev1_) == hipSuccess) {
+    if (hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) {
       st_acc_.factor_ms += ms;
       st_acc_.dag_ms += ms;
     }
-    pending_timing_ = false;
+    evpool_.push_back(p.first);
+    evpool_.push_back(p.second);
   }
+  timed_.clear();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -803,7 +818,32 @@ Gang::BigCombo& Gang::combo(const std::vector<std::pair<int, int>>& fronts) {
     const double r = B.r, sz = B.s;
     c->flops += r * r * r / 3.0 + sz * r * r + sz * sz * r;
   }
-  const std::vector<int2> odag = interleave(cdag), ofs = interleave(cfs), obs = interleave(cbs);
+  // factorization tasks: with at least DAG_QUEUES fronts, one queue per XCD, the fronts dealt to
+  // the queues by work (largest first, to the least loaded queue); fewer fronts share one queue
+  std::vector<int2> odag;
+  const int nf = (int)bs.size();
+  c->q.n = nf >= DAG_QUEUES ? DAG_QUEUES : 1;
+  {
+    std::vector<std::vector<int>> qf(c->q.n);
+    std::vector<double> load(c->q.n, 0.0);
+    std::vector<int> byw(nf);
+    for (int f = 0; f < nf; ++f) byw[f] = f;
+    std::stable_sort(byw.begin(), byw.end(), [&](int a, int b) { return cdag[a] > cdag[b]; });
+    for (int f : byw) {
+      const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      qf[q].push_back(f);
+      load[q] += cdag[f];
+    }
+    c->q.off[0] = 0;
+    for (int q = 0; q < c->q.n; ++q) {
+      std::sort(qf[q].begin(), qf[q].end());
+      std::vector<int> cnt;
+      for (int f : qf[q]) cnt.push_back(cdag[f]);
+      for (const int2& e : interleave(cnt)) odag.push_back(int2{qf[q][e.x], e.y});
+      c->q.off[q + 1] = (int)odag.size();
+    }
+  }
+  const std::vector<int2> ofs = interleave(cfs), obs = interleave(cbs);
   c->nf = (int)bs.size();
   c->ndag = (int)odag.size();
   c->nfs = (int)ofs.size();
@@ -838,7 +878,6 @@ void Gang::combined_factor(const std::vector<int>& slots, bool fp32) {
   launch_assemble_cones(st_, I, T_.cones, T_.cone_front, F, ncone_, cone_nmax_);
   st_acc_.launches += 3;
   const int epoch = next_epoch();
-  if (profile_) HIPCHK(hipEventRecord(ev0_, st_));
   double flops = 0.0;
   int nf = 0;
   for (size_t l = 0; l < lv_.size(); ++l) {
@@ -856,18 +895,20 @@ void Gang::combined_factor(const std::vector<int>& slots, bool fp32) {
       if ((mask >> p.first) & 1ull) big.push_back(p);
     if (!big.empty()) {
       BigCombo& c = combo(big);
-      launch_dag_factor_multi(st_, c.bs, c.nf, c.odag, c.ndag, d_tick_.as<int>(), I,
-                              dag_grid_, epoch, fp32, c.max2tr);
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (profile_) {  // the factorization kernel alone, read at the next read-back
+        e0 = event();
+        e1 = event();
+        timed_.push_back({e0, e1});
+      }
+      launch_dag_factor_multi(st_, c.bs, c.nf, c.odag, c.q, d_tick_.as<int>(), I, dag_grid_, epoch, fp32,
+                              c.max2tr, e0, e1);
       flops += c.flops;
       nf += c.nf;
       st_acc_.dag_launches += 1;
       st_acc_.launches += 2;
     }
     st_acc_.launches += (double)(G.ea_off.size() + 2 * G.pan_off.size());
-  }
-  if (profile_) {
-    HIPCHK(hipEventRecord(ev1_, st_));
-    pending_timing_ = true;
   }
   st_acc_.factor_launches += 1;
   st_acc_.factor_flops += flops;

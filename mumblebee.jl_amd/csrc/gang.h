@@ -143,7 +143,11 @@ class Gang {
   int kmax_, dag_grid_;
   bool profile_;
   hipStream_t st_ = nullptr;
-  hipEvent_t ev0_ = nullptr, ev1_ = nullptr, evleave_ = nullptr;
+  hipEvent_t evleave_ = nullptr;
+  // profile: event pairs around each combined factorization kernel, read at the next read-back
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed_;
+  std::vector<hipEvent_t> evpool_;
+  hipEvent_t event();
   // the scheduler thread runs the members' fibers and the ticks; other threads only hand in jobs
   // and collect them (under mu_)
   std::mutex mu_;
@@ -201,7 +205,6 @@ class Gang {
   InstDev dead_{};
   // launch data of fronts whose member left, freed once the stream has passed its launches
   std::vector<std::unique_ptr<BigCombo>> graveyard_;
-  bool pending_timing_ = false;                        // a timed factorization awaits its read-back
 };
 
 }  // namespace mb

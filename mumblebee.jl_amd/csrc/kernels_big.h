@@ -54,13 +54,24 @@ int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int
 int launch_big_bsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int f, const InstDev* I, int kv,
                       int epoch, long long base, bool fp32);
 
-// The large fronts of a gang (several instances) in one launch each. Bs: the nf fronts (device
-// copies of their BigFront), bfront: their indices in the front table F; order: ntot (front,
-// task or block) pairs, each front's own items in their order. The factorization's ticket must be
-// 0 on entry and is put back to 0 (max2tr: 2·max Tr, the chain-step tile grid); the solves take
-// one ticket of the 64-bit counter per workgroup (the caller advances base by ntot).
-void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const int2* order, int ntot, int* ticket,
-                             const InstDev* I, int grid, int epoch, bool fp32, int max2tr);
+// The large fronts of a gang (several instances) in one launch each (write-once factor storage
+// required). Bs: the nf fronts (device copies of their BigFront), bfront: their indices in the
+// front table F; order: (front, task or block) pairs, each front's own items in their order
+// (max2tr: 2·max Tr, the chain-step tile grid); the solves take one ticket of the 64-bit counter
+// per workgroup (the caller advances base by ntot).
+//
+// The task queues of one launch: queue q is order[off[q], off[q+1]), served by the workgroups
+// ≡ q (mod n); each front's tasks lie in one queue, in their topological order
+constexpr int DAG_QUEUES = 8;
+struct DagQueues {
+  int n;
+  int off[DAG_QUEUES + 1];
+};
+// ticket: DAG_QUEUES counters, 0 on entry and put back to 0; before / after (or null): events
+// recorded around the factorization kernel alone
+void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const int2* order, const DagQueues& Q,
+                             int* ticket, const InstDev* I, int grid, int epoch, bool fp32, int max2tr,
+                             hipEvent_t before = nullptr, hipEvent_t after = nullptr);
 void launch_big_fsolve_multi(hipStream_t st, const BigFront* Bs, const int* bfront, int nf, const int2* order,
                              int ntot, unsigned long long* ticket, const FrontDesc* F, const int* childlist,
                              const InstDev* I, int kv, int epoch, long long base, bool fp32);

@@ -295,20 +295,24 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
 // instance is not taking part, or whose abort flag is raised, has its remaining tasks skipped;
 // the other fronts go on. The ticket goes back to 0 in k_big_mtiles_multi.
 template <class T>
-__global__ void __launch_bounds__(NT) k_dag_factor_multi(const BigFront* Bs, const int2* order, int ntot,
+__global__ void __launch_bounds__(NT) k_dag_factor_multi(const BigFront* Bs, const int2* order, DagQueues Q,
                                                          int* ticket, const InstDev* insts, int epoch) {
   __shared__ __align__(16) T SA[NB * LROW];
   __shared__ __align__(16) T SB[NB * LROW];
   __shared__ __align__(16) T SC[NB * LROW];
   __shared__ __align__(16) T SD[NB * LROW];
   __shared__ int task_s;
+  // queue q is served by the workgroups ≡ q (mod nq): with nq = 8 those of one XCD (the dispatcher
+  // deals workgroup i to XCD i mod 8), so a front's tiles are re-read from that XCD's L2
+  const int q = (int)blockIdx.x % Q.n;
+  const int qn = Q.off[q + 1] - Q.off[q];
   for (;;) {
-    if (threadIdx.x == 0) task_s = atomicAdd(ticket, 1);
+    if (threadIdx.x == 0) task_s = atomicAdd(ticket + q, 1);
     __syncthreads();
     const int g = __builtin_amdgcn_readfirstlane(task_s);
     __syncthreads();
-    if (g >= ntot) break;
-    const int2 o = order[g];
+    if (g >= qn) break;
+    const int2 o = order[Q.off[q] + g];
     const BigFront& B = Bs[o.x];
     const InstDev& I = insts[B.inst];
     if (I.sc[S_ACTIVE] == 0.0 || ld_flag(B.abortf) != 0) continue;
@@ -559,7 +563,7 @@ template <class TF>
 __global__ void __launch_bounds__(NT) k_big_mtiles_multi(const BigFront* Bs, int* ticket, const InstDev* insts) {
   __shared__ __align__(16) double SX[NB * LDT];
   __shared__ __align__(16) double SY[NB * LDT];
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *ticket = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < DAG_QUEUES) ticket[threadIdx.x] = 0;
   const BigFront& B = Bs[blockIdx.y];
   if ((int)blockIdx.x >= 2 * B.Tr) return;
   const InstDev& I = insts[B.inst];
@@ -849,11 +853,20 @@ int launch_big_fsolve(hipStream_t st, const BigFront& B, const FrontDesc* F, int
   return grid;
 }
 
-void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const int2* order, int ntot, int* ticket,
-                             const InstDev* I, int grid, int epoch, bool fp32, int max2tr) {
-  if (ntot <= 0) return;
-  if (fp32) k_dag_factor_multi<float><<<grid, NT, 0, st>>>(Bs, order, ntot, ticket, I, epoch);
-  else k_dag_factor_multi<double><<<grid, NT, 0, st>>>(Bs, order, ntot, ticket, I, epoch);
+#define HIPCHK_K(x)                                                                        \
+  do {                                                                                     \
+    if ((x) != hipSuccess) throw std::runtime_error("HIP error at " #x);                  \
+  } while (0)
+
+void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const int2* order, const DagQueues& Q,
+                             int* ticket, const InstDev* I, int grid, int epoch, bool fp32, int max2tr,
+                             hipEvent_t before, hipEvent_t after) {
+  if (before) HIPCHK_K(hipEventRecord(before, st));
+  if (Q.n < 1 || Q.n > DAG_QUEUES || Q.off[Q.n] <= 0) return;
+  grid = std::max(grid, Q.n);
+  if (fp32) k_dag_factor_multi<float><<<grid, NT, 0, st>>>(Bs, order, Q, ticket, I, epoch);
+  else k_dag_factor_multi<double><<<grid, NT, 0, st>>>(Bs, order, Q, ticket, I, epoch);
+  if (after) HIPCHK_K(hipEventRecord(after, st));
   const dim3 mg(std::max(1, max2tr), nf);
   if (fp32) k_big_mtiles_multi<float><<<mg, NT, 0, st>>>(Bs, ticket, I);
   else k_big_mtiles_multi<double><<<mg, NT, 0, st>>>(Bs, ticket, I);

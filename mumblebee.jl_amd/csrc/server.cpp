@@ -16,8 +16,15 @@ double Server::now() {
 Server::Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis, bool gang)
     : nslots_(std::max(1, nslots)), dag_grid_(dag_grid), budget_(mem_budget_bytes) {
   if (gang) {
+    // one gang per size class (the factorization flops of a request): members of one class take
+    // similar work per step, so lockstep costs little, and the classes overlap on the device
     nslots_ = std::min(nslots_, GANG_MAX);
-    gang_ = std::make_unique<Gang>(nslots_, 0, false);
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+    for (size_t c = 0; c <= GANG_CLASS_FLOPS.size(); ++c)
+      gangs_.push_back(std::make_unique<Gang>(nslots_, ncu, false));
   }
   if (budget_ <= 0.0) {
     size_t free_b = 0, total_b = 0;
@@ -29,13 +36,13 @@ Server::Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis,
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       ncu = 256;
-    // a gang's members run one at a time on its stream: each may take every CU
-    dag_grid_ = gang_ ? ncu : std::max(16, ncu / nslots_);
+    dag_grid_ = std::max(16, ncu / nslots_);
   }
   if (nanalysis <= 0) nanalysis = std::min(8, std::max(2, (int)std::thread::hardware_concurrency() / 2));
-  lookahead_ = (size_t)(2 * nslots_ + nanalysis);
+  lookahead_ = (size_t)(2 * nslots_ * std::max<size_t>(1, gangs_.size()) + nanalysis);
   for (int k = 0; k < nanalysis; ++k) workers_.emplace_back(&Server::analysis_main, this);
-  for (int s = 0; s < nslots_; ++s) slots_.emplace_back(&Server::slot_main, this, s);
+  const int nthreads = gangs_.empty() ? nslots_ : nslots_ * (int)gangs_.size();
+  for (int s = 0; s < nthreads; ++s) slots_.emplace_back(&Server::slot_main, this, s);
 }
 
 Server::~Server() {
@@ -48,7 +55,7 @@ Server::~Server() {
   cv_room_.notify_all();
   for (auto& t : workers_) t.join();
   for (auto& t : slots_) t.join();
-  gang_.reset();
+  gangs_.clear();
 }
 
 int64_t Server::submit(const Problem* prob, const Settings& st) {
@@ -181,7 +188,14 @@ void Server::slot_main(int slot) {
       job->t_admit = now();
       S.device_setup(nullptr, nullptr, nullptr);
       job->bytes = (double)S.device_bytes();
-      job->status = gang_ ? S.solve_in(*gang_) : S.solve();
+      if (gangs_.empty()) {
+        job->status = S.solve();
+      } else {
+        const double fl = S.front_flops();
+        size_t c = 0;
+        while (c < GANG_CLASS_FLOPS.size() && fl >= GANG_CLASS_FLOPS[c]) ++c;
+        job->status = S.solve_in(*gangs_[c]);
+      }
       job->p.resize(S.n());
       job->d.resize(S.n());
       job->y.resize(S.m());

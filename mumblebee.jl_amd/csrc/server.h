@@ -43,10 +43,15 @@ struct ServeJob {
   std::unique_ptr<Solver> solver;  // host-analysed, between analysis and admission
 };
 
+// gang mode: requests whose factorization takes fewer flops than GANG_CLASS_FLOPS[0] go to the
+// first gang, then the next class, and so on
+inline const std::vector<double> GANG_CLASS_FLOPS = {1e8, 2e9};
+
 class Server {
  public:
   // nanalysis ≤ 0: one analysis worker per two hardware threads (at least two); gang: the slots
-  // are the members of one gang (their solves advance in lockstep and share every launch)
+  // are members of gangs, one gang of nslots per size class (their solves advance in lockstep and
+  // share every launch), so nslots × classes requests run at a time
   Server(int nslots, double mem_budget_bytes, int dag_grid, int nanalysis = 0, bool gang = false);
   ~Server();  // finishes the queued and running requests, then joins the threads
   int64_t submit(const Problem* prob, const Settings& st);
@@ -77,7 +82,7 @@ class Server {
   std::map<int64_t, std::shared_ptr<ServeJob>> ready_;    // analysed, by ticket
   std::unordered_map<int64_t, std::shared_ptr<ServeJob>> jobs_;
   std::vector<std::thread> workers_, slots_;
-  std::unique_ptr<Gang> gang_;
+  std::vector<std::unique_ptr<Gang>> gangs_;  // gang mode: one per size class
 };
 
 }  // namespace mb
