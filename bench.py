@@ -491,10 +491,13 @@ def bench_serve(args, mh):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"served stream: {K} requests, 60% QP / 30% SOCP / 10% chordal max-cut SDP, n "
                                f"log-uniform in [50, {args.serve_nmax}], Poisson arrivals at {args.rate}/s, "
-                               + (f"{nslots} slots as the members of one gang (lockstep, shared launches)"
+                               + (f"a gang of {nslots} slots per size class (lockstep, shared launches)"
                                   if args.gang else f"{nslots} slots, each on its own stream"),
                    "precision": "mixed" if args.precision else "fp64",
-                   "optimal": ok, "statuses": statuses, "p50_ms": float(np.percentile(lat_ms, 50)),
+                   "optimal": ok, "statuses": statuses,
+                   "not_optimal": [[k, r.status.name, int(r.niter)] for k, (r, _) in enumerate(out)
+                                   if r.status != mh.IPMStatus.OPTIMAL],
+                   "p50_ms": float(np.percentile(lat_ms, 50)),
                    "p99_ms": float(np.percentile(lat_ms, 99)), "peak_reserved_gb": st["peak_reserved_bytes"] / 2 ** 30},
     }
 

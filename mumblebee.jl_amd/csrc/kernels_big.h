@@ -34,13 +34,14 @@ struct BigFront {
   int inst;
   // Write-once factor storage (both set, or both null): L holds the FP64 factor columns (ld × r,
   // same indexing as F) and stg the staged sub-diagonal tiles (Tr 64×64 tiles of the factor's
-  // element type). Every location the factorization reads after another workgroup wrote it is
-  // then written once per launch and read only after its flag, so those reads go through the
-  // caches (no cache can hold an older copy: nothing read it earlier in the launch). With null,
-  // the FP64 factor overwrites F in place and such reads take the device-coherent path.
+  // element type), so the assembled front stays intact. With null, the FP64 factor overwrites F
+  // in place. Either way, tiles finalized in the running launch are read device-coherent.
   double* L;
   double* stg;
 };
+
+// factorization workgroups resident per CU: launch grids are given in CUs and multiplied by this
+constexpr int DAG_PER_CU = 2;
 
 // fp32: the factor in FP32 on FP32 MFMA (B.F32 / B.Dinv32 must be allocated); the solves read
 // the factor of the same precision and compute in FP64

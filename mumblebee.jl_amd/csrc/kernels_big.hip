@@ -144,13 +144,18 @@ __device__ __forceinline__ bool cols_final(const BigFront& B, int i, int j, int 
 }
 
 // One task of the tile DAG (tasks[t] of front B), by the whole workgroup. False when a wait
-// ended on the abort flag or at its limit. SEP: write-once factor storage (B.L / B.F32 and B.stg),
-// so the final tiles and inverted diagonal blocks are read through the caches (CONS); otherwise
-// they are read device-coherent.
+// ended on the abort flag or at its limit. SEP: write-once factor storage (B.L / B.F32 and B.stg);
+// otherwise the factor overwrites F in place and the staged tile sits at its final place.
+//
+// Tiles and inverted diagonal blocks that other workgroups finalize in this launch are read
+// device-coherent (CONS) in both forms. Cached reads would be wrong even for data written once:
+// when ld is not a multiple of a cache line's elements, one line holds the last rows of tile
+// (i, k) and the first rows of tile (i+1, k) of a column, so a line fetched for a final tile can
+// carry rows of its neighbour from before the neighbour was final, and a later cached read of
+// the neighbour would hit that copy.
 template <class T, bool SEP>
-__device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, int epoch, int t, T* SA, T* SB, T* SC,
-                                         T* SD) {
-  constexpr Coh CONS = SEP ? Coh::None : Coh::Dev;
+__device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, int epoch, int t, T* SA, T* SB, T* W) {
+  constexpr Coh CONS = Coh::Dev;
   double* F = B.F;
   T* Lf = factor_of<T>(B);
   T* Di = dinv_of<T>(B);
@@ -167,8 +172,9 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
   Acc<T> C;
   acc_from_lds(C, SA);
   __syncthreads();
-  // C −= Σ_k L_ik L_jkᵀ, software pipelined: the tiles of column k+1 are in flight while the
-  // MFMAs of column k run from the other LDS buffer pair
+  // C −= Σ_k L_ik L_jkᵀ: the tiles of column k+1 are in flight while the MFMAs of column k run.
+  // One LDS tile pair per workgroup, so two workgroups share a CU and one's staging and barriers
+  // overlap the other's products.
   bool alive = true;
   TileRegs<T> RA, RB;
   // columns < ready hold final tiles in rows i and j (uniform across the workgroup)
@@ -180,11 +186,9 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
     if (i != j) fetch_tile<CONS>(RB, Lf, LD, r0j, c00, hj, w0);
   }
   for (int k = 0; k < kmax; ++k) {
-    T* XA = (k & 1) ? SC : SA;
-    T* XB = (k & 1) ? SD : SB;
-    put_tile(XA, RA);
-    if (i != j) put_tile(XB, RB);
-    __syncthreads();  // also: the buffer pair written next was last read before this point
+    put_tile(SA, RA);
+    if (i != j) put_tile(SB, RB);
+    __syncthreads();
     if (k + 1 < kmax) {
       if (k + 1 >= ready && !cols_final(B, i, j, kmax, k + 2, epoch, ready)) {
         alive = false;
@@ -194,9 +198,9 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
       fetch_tile<CONS>(RA, Lf, LD, r0i, c0n, hi, wn);
       if (i != j) fetch_tile<CONS>(RB, Lf, LD, r0j, c0n, hj, wn);
     }
-    mfma_acc<true>(C, XA, i != j ? XB : XA, B.tb[k + 1] - B.tb[k]);
+    mfma_acc<true>(C, SA, i != j ? SB : SA, B.tb[k + 1] - B.tb[k]);
+    __syncthreads();  // the pair is overwritten next
   }
-  __syncthreads();
   if (!alive) return false;
   stamp(B.trace, t, 1);
   if (type == 0) {
@@ -224,10 +228,10 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
     // factor and invert; L_jj⁻¹ → Dinv and publish, then L_jj → F (read only by later launches)
     acc_to_lds(C, SA);
     __syncthreads();
-    bool ok = chol_inv64(SA, SC, SB, hj, B.trace, t);  // SB is free here and lies below 64 KiB
+    bool ok = chol_inv64(SA, SB, W, hj, B.trace, t);
     stamp(B.trace, t, 6);
     if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-    store_tile<Coh::Dev>(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SC, false);
+    store_tile<Coh::Dev>(Di + (size_t)j * NB * NB, NB, 0, 0, NB, NB, SB, false);
     publish(&B.state[tri_index(j, j)], epoch);
     stamp(B.trace, t, 7);
     store_tile<Coh::Dev>(Lf, LD, r0j, r0j, hj, hj, SA, true);
@@ -266,12 +270,19 @@ __device__ __forceinline__ bool dag_task(const BigFront& B, const InstDev& I, in
   return true;
 }
 
+// LDS of one factorization workgroup: a tile pair and the diagonal factor's scratch (about
+// 68 KiB in FP64), so DAG_PER_CU = 2 workgroups fit a CU's 160 KiB
+template <class T>
+struct DagLds {
+  T W[3 * NB];
+  T A[NB * LROW];
+  T B[NB * LROW];
+};
+
 template <class T, bool SEP>
-__global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* insts, int epoch) {
-  __shared__ __align__(16) T SA[NB * LROW];
-  __shared__ __align__(16) T SB[NB * LROW];
-  __shared__ __align__(16) T SC[NB * LROW];
-  __shared__ __align__(16) T SD[NB * LROW];
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DAG_PER_CU, DAG_PER_CU)))
+k_dag_factor(BigFront B, const InstDev* insts, int epoch) {
+  __shared__ __align__(16) DagLds<T> S;
   __shared__ int task_s;
   const InstDev& I = insts[B.inst];
   if (I.sc[S_ACTIVE] == 0.0) return;
@@ -285,7 +296,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
     const int t = task_s;
     __syncthreads();
     if (t >= B.ntask) break;
-    if (!dag_task<T, SEP>(B, I, epoch, t, SA, SB, SC, SD)) break;
+    if (!dag_task<T, SEP>(B, I, epoch, t, S.A, S.B, S.W)) break;
   }
 }
 
@@ -295,12 +306,10 @@ __global__ void __launch_bounds__(NT) k_dag_factor(BigFront B, const InstDev* in
 // instance is not taking part, or whose abort flag is raised, has its remaining tasks skipped;
 // the other fronts go on. The ticket goes back to 0 in k_big_mtiles_multi.
 template <class T>
-__global__ void __launch_bounds__(NT) k_dag_factor_multi(const BigFront* Bs, const int2* order, DagQueues Q,
-                                                         int* ticket, const InstDev* insts, int epoch) {
-  __shared__ __align__(16) T SA[NB * LROW];
-  __shared__ __align__(16) T SB[NB * LROW];
-  __shared__ __align__(16) T SC[NB * LROW];
-  __shared__ __align__(16) T SD[NB * LROW];
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DAG_PER_CU, DAG_PER_CU)))
+k_dag_factor_multi(const BigFront* Bs, const int2* order, DagQueues Q, int* ticket, const InstDev* insts,
+                   int epoch) {
+  __shared__ __align__(16) DagLds<T> S;
   __shared__ int task_s;
   // queue q is served by the workgroups ≡ q (mod nq): with nq = 8 those of one XCD (the dispatcher
   // deals workgroup i to XCD i mod 8), so a front's tiles are re-read from that XCD's L2
@@ -316,7 +325,7 @@ __global__ void __launch_bounds__(NT) k_dag_factor_multi(const BigFront* Bs, con
     const BigFront& B = Bs[o.x];
     const InstDev& I = insts[B.inst];
     if (I.sc[S_ACTIVE] == 0.0 || ld_flag(B.abortf) != 0) continue;
-    (void)dag_task<T, true>(B, I, epoch, o.y, SA, SB, SC, SD);
+    (void)dag_task<T, true>(B, I, epoch, o.y, S.A, S.B, S.W);
   }
 }
 
@@ -833,6 +842,7 @@ __global__ void __launch_bounds__(NT) k_big_bsolve_multi(const BigFront* Bs, con
 void launch_dag_factor(hipStream_t st, const BigFront& B, const InstDev* I, int grid, int epoch, bool fp32) {
   const bool sep = B.stg != nullptr;
   if (fp32 && !sep) throw std::invalid_argument("FP32 factorization without staging tiles");
+  grid *= DAG_PER_CU;
   if (fp32) k_dag_factor<float, true><<<grid, NT, 0, st>>>(B, I, epoch);
   else if (sep && B.L) k_dag_factor<double, true><<<grid, NT, 0, st>>>(B, I, epoch);
   else k_dag_factor<double, false><<<grid, NT, 0, st>>>(B, I, epoch);
@@ -863,7 +873,7 @@ void launch_dag_factor_multi(hipStream_t st, const BigFront* Bs, int nf, const i
                              hipEvent_t before, hipEvent_t after) {
   if (before) HIPCHK_K(hipEventRecord(before, st));
   if (Q.n < 1 || Q.n > DAG_QUEUES || Q.off[Q.n] <= 0) return;
-  grid = std::max(grid, Q.n);
+  grid = std::max(grid * DAG_PER_CU, Q.n);
   if (fp32) k_dag_factor_multi<float><<<grid, NT, 0, st>>>(Bs, order, Q, ticket, I, epoch);
   else k_dag_factor_multi<double><<<grid, NT, 0, st>>>(Bs, order, Q, ticket, I, epoch);
   if (after) HIPCHK_K(hipEventRecord(after, st));

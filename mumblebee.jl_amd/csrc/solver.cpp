@@ -903,8 +903,8 @@ void Solver::build_schedule() {
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) ncu = 256;
-    // the DAG kernel runs at most one workgroup per CU (register bound); a serving slot may
-    // take a share of the CUs so that several instances' factorizations run side by side
+    // the DAG kernel's grid in CUs (DAG_PER_CU workgroups on each); a serving slot may take a
+    // share of the CUs so that several instances' factorizations run side by side
     dag_grid_ = set_.dag_grid > 0 ? std::min(set_.dag_grid, ncu) : ncu;
   }
   for (int l = 0; l < H; ++l) {

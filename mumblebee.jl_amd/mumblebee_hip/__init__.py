@@ -724,10 +724,13 @@ class IPMServer:
     is a solve(prob, settings), solver/solver.jl:119-150). `nslots` slots, each a host thread with
     its own HIP stream, take requests in arrival order and admit them as slots and device memory
     (`mem_budget_gb`, default 90 % of the free memory) free up; their kernels overlap on the GPU.
-    `dag_grid` workgroups per slot for the persistent factorization (0: the CUs split evenly)."""
+    `dag_grid` workgroups per slot for the persistent factorization (0: the CUs split evenly).
+    With `gang`, requests are routed by the flops of their factorization to one gang (IPMGang) of
+    `nslots` members per size class (below 1e8, below 2e9, above): members advance in lockstep
+    and share every launch."""
 
     def __init__(self, nslots=8, mem_budget_gb=0.0, dag_grid=0, gang=False):
-        if gang:  # the slots are the members of one gang (IPMGang): lockstep, shared launches
+        if gang:  # one gang per size class, nslots members each
             h = lib().mb_server_create_gang(int(nslots), float(mem_budget_gb) * 2.0 ** 30)
         else:
             h = lib().mb_server_create(int(nslots), float(mem_budget_gb) * 2.0 ** 30, int(dag_grid))

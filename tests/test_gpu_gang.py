@@ -58,6 +58,14 @@ def test_gang_large_fronts_bit_exact():
     assert st["factor_fronts"] >= 2 * st["factor_launches"] > 0
 
 
+def test_gang_odd_leading_dimension_bit_exact():
+    # fronts of order 680..735 (a column is not a whole number of cache lines, so a line holds
+    # rows of two tiles) and ≥ 8 of them per combined factorization (per-XCD task queues)
+    data = [G.qp(n=680 + 5 * k, m=340, seed=250 + k) for k in range(12)]
+    _, st = _gang_vs_solo(data, 12, tag="odd ld")
+    assert st["factor_fronts"] >= 4 * st["factor_launches"] > 0
+
+
 def test_gang_mixed_cones_bit_exact():
     data = [G.qp(n=300, m=150, seed=300), G.socp(n=400, m=200, ncone=15, seed=301, adens=0.02),
             G.maxcut(nv=60, band=4, clique=12, extra=0.01, seed=302),
