@@ -17,6 +17,7 @@
 #include "device.h"
 #include "kernels.h"
 #include "tdc.h"
+#include "tile64.h"
 
 namespace mb {
 
@@ -157,7 +158,7 @@ __global__ void k_dot(const InstDev* insts, const Tile* tiles, const int* tile_o
   const double s = block_sum(v, red);
   const int a = tile_ofs[t.inst], b = tile_ofs[t.inst + 1];
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = s;
+    __hip_atomic_store(&partial[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();  // the partial is visible device-wide before the count
     last = atomicAdd(&counter[t.inst], 1) == b - a - 1;
   }
@@ -1014,15 +1015,15 @@ __device__ bool lds_potrf(double* A, int w) {
   return bad == 0;
 }
 
-// Panel step: block 0 of each item factors the diagonal block and writes it; blocks 1..ntile
-// factor it redundantly and solve their 64-row tile below: X = A L⁻ᵀ.
 // One 64-column panel step of every small front of a level. Tile 0 of an item factors the
-// diagonal block in LDS, stores L_jj and publishes the factorization epoch in the item's flag
-// (release, device scope); the row tiles below wait for that flag (acquire) and read L_jj, which
-// they then apply to their rows. The diagonal workgroup precedes its row workgroups in the grid,
-// so it runs before (or beside) them and the wait ends; it is bounded all the same, and raises
-// the abort flag at its limit. (Row tiles that factored the block themselves from F raced with
-// tile 0's store of L_jj into the same place.)
+// diagonal block in LDS, stores L_jj and publishes the factorization epoch in the item's flag;
+// the row tiles below wait for that flag and read L_jj, which they then apply to their rows:
+// X = A L_jj⁻ᵀ. L_jj moves with the device-coherent policy (sc1 buffer stores and loads), as the
+// tiles of the persistent kernels do: after an acquire, ordinary loads of L_jj returned, now and
+// then, lines a CU's cache held from before tile 0's store (results differed from run to run
+// when other work shared the GPU). The diagonal workgroup precedes its row workgroups in the
+// grid, so it runs before (or beside) them and the wait ends; it is bounded all the same, and
+// raises the abort flag at its limit.
 __global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const StepDesc* steps, int nstep,
                                                const InstDev* insts, int* flags, int epoch, int* abortf) {
   __shared__ double Ld[NB * (NB + 1)];
@@ -1046,10 +1047,7 @@ __global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const St
     __syncthreads();
     bool ok = lds_potrf(Ld, w);
     if (!ok && threadIdx.x == 0) atomicAdd(&I.sc[S_FAIL], 1.0);
-    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-      int i = t % NB, j = t / NB;
-      if (i < w && j < w && i >= j) F[(size_t)(kb + i) + (size_t)(kb + j) * LD] = Ld[i + j * ld];
-    }
+    store_tile<Coh::Dev>(F, LD, kb, kb, w, w, Ld, true);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(&flags[it], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1070,10 +1068,7 @@ __global__ void __launch_bounds__(256) k_panel(const FrontDesc* fronts, const St
   }
   __syncthreads();
   if (!okw) return;
-  for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-    int i = t % NB, j = t / NB;
-    Ld[i + j * ld] = (i < w && j < w && i >= j) ? F[(size_t)(kb + i) + (size_t)(kb + j) * LD] : 0.0;
-  }
+  load_tile<Coh::Dev>(Ld, F, LD, kb, kb, w, w);  // its upper part is not read
   __syncthreads();
   int r0 = kb + w + (tile - 1) * NB;
   int nr = min(NB, LD - r0);
