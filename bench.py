@@ -43,8 +43,8 @@ ARGS = None
 # HBM bytes per large front of the dominant kernel's launches, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes scaled by a 256 MiB device copy (FETCH_SIZE ×2, WRITE_SIZE ×1); keyed by (n, m,
 # gang): the single-front kernel (profiles/k_dag_factor_qp2000_fp64.md) and the gang's combined
-# launch, 16 fronts of which read 3.25 GB and write 0.29 GB (profiles/k_dag_factor_multi_qp2000_gang16.md)
-TRAFFIC_PER_FRONT = {(2000, 1000, False): 141.5e6, (2000, 1000, True): 221.1e6}
+# launch, 16 fronts of which read 3.27 GB and write 0.32 GB (profiles/k_dag_factor_multi_qp2000_gang16.md)
+TRAFFIC_PER_FRONT = {(2000, 1000, False): 141.5e6, (2000, 1000, True): 224.4e6}
 
 
 WORKLOADS = {
