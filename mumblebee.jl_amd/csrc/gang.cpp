@@ -738,7 +738,7 @@ void Gang::rebuild() {
     Solver* s = mem_[k];
     if (!s) continue;
     A.F = s->d_F_.as<double>();
-    A.nF = s->fsize_;
+    A.nF = s->fill_free_ ? 0 : s->fsize_;  // a front that needs no clearing gets no fill blocks
     A.pos = s->d_upos_.as<int64_t>();
     A.l = s->d_ul_.as<double>();
     A.q = s->d_uq_.as<double>();

@@ -687,6 +687,23 @@ void Solver::build_device() {
     }
   }
   nunion_ = (i64)upos.size();
+  // Assembly without clearing the front first: one large front (its factor kept beside it, so
+  // the factorization never writes it), no separator and no children (nothing else adds into
+  // it), cone blocks only on the diagonal, and every diagonal entry assigned by the union scatter
+  // (the cones add to the diagonal after it). Every entry the factorization reads is then either
+  // assigned afresh or zero since the setup.
+  {
+    bool ok = NF == 1 && fr_[0].ld >= set_.big_front_ld && fr_[0].s == 0 && fr_[0].nchild == 0;
+    for (i64 v = 0; ok && v < N; ++v) ok = P_.K[v].kind == CONE_POS || P_.K[v].kind == CONE_FREE;
+    if (ok) {
+      std::vector<char> diag(n_, 0);
+      const i64 ld = fr_[0].ld;
+      for (int64_t q : upos)
+        if (q % (ld + 1) == 0) diag[q / (ld + 1)] = 1;
+      for (i64 j = 0; ok && j < n_; ++j) ok = diag[j] != 0;
+    }
+    fill_free_ = ok;
+  }
   bd_parts_[1] = wall_ms() - tb0;
   // ‖Q‖² off the diagonal blocks
   qoff2_ = 0.0;
@@ -742,6 +759,7 @@ void Solver::build_device() {
   HIPCHK(hipMemsetAsync(d_cache_.p, 0, d_cache_.bytes, st_));
   HIPCHK(hipMemsetAsync(d_bws_.p, 0, d_bws_.bytes, st_));
   HIPCHK(hipMemsetAsync(d_D_.p, 0, d_D_.bytes, st_));
+  if (fill_free_) HIPCHK(hipMemsetAsync(d_F_.p, 0, d_F_.bytes, st_));  // once: assembly then only assigns
   d_sepcols_.upload(allsep, st_);
   d_relidx_.upload(relidx, st_);
   d_childlist_.upload(childlist, st_);
@@ -1649,7 +1667,7 @@ void Solver::prof_get(double* out) const {
 // F ← BᵀB + δH (the solver's H: Q and the cone blocks; or the caller's H)
 void Solver::raw_assemble() {
   const InstDev* I = d_inst_.as<InstDev>();
-  launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
+  if (!fill_free_ || user_h_) launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
   if (user_h_) {  // BᵀB + δ (the caller's H)
     launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 0);
     launch_scatter_add(st_, I, 0, d_hpos_.as<int64_t>(), d_hval_.as<double>(), nhent_);

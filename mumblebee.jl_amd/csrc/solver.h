@@ -338,6 +338,9 @@ class Solver {
   i64 qdiag_arc(i64 v) const;
   // a caller's H (kkt_factor_h): CSR rows, row tiles, assembly positions and values
   bool user_h_ = false;
+  // the front needs no clearing before an assembly (one large front with its factor beside it,
+  // no separator or children, diagonal cone blocks, every diagonal entry in the union pattern)
+  bool fill_free_ = false;
   CSRDev hsave_{};  // the CSR not in use (Q's while the caller's H is)
   DevBuf d_Hp_, d_Hc_, d_Hv_, d_htiles_, d_hpos_, d_hval_;
   int nhtiles_ = 0;
