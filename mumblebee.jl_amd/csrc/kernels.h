@@ -174,6 +174,17 @@ void launch_bpsd_dshadow(hipStream_t, const InstDev*, const ConeDesc*, const Big
 // the shadow-solve seed of every listed exponential / power cone (initcache!)
 void launch_td_initcache(hipStream_t, const InstDev*, const ConeDesc*, const int* list, int n);
 void launch_fill(hipStream_t, double* x, size_t n, double v);
+// One piece of a staged setup: bytes at dst (4-byte aligned) from src bytes into the staged image,
+// or zeros (src < 0); the piece is copied by workgroups blk0 .. blk0 + ⌈bytes / STAGE_BLOCK⌉ − 1.
+struct StageSeg {
+  char* dst;
+  long long src;
+  long long bytes;
+  long long blk0;
+};
+constexpr long long STAGE_BLOCK = 64 << 10;
+// every piece of segs[0, nseg) in one launch of nblk workgroups
+void launch_stage_scatter(hipStream_t, const char* image, const StageSeg* segs, int nseg, long long nblk);
 // F[pos] = lval + δ qval (withq) or lval
 void launch_assemble_union(hipStream_t, const InstDev*, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent, int withq);

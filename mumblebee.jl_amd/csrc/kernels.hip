@@ -1400,6 +1400,42 @@ void launch_zerofree(hipStream_t st, const InstDev* I, const ConeDesc* C, int nc
 void launch_fill(hipStream_t st, double* x, size_t n, double v) {
   if (n) k_fill<<<std::max(cdiv((long)(n / 2), TILE), 1), TILE, 0, st>>>(x, n, v);
 }
+
+// The pieces of a staged setup (copies out of one uploaded image, and zero fills) in one launch:
+// workgroup b finds its piece by bisection over blk0 and moves its STAGE_BLOCK bytes of it,
+// 16 bytes per thread and step where both ends are 16-byte aligned, else 4 (and single bytes last).
+__global__ void k_stage_scatter(const char* image, const StageSeg* segs, int nseg) {
+  const long long b = blockIdx.x;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (segs[mid].blk0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const StageSeg S = segs[lo];
+  const long long a = (b - S.blk0) * STAGE_BLOCK, e = min(S.bytes, a + STAGE_BLOCK);
+  if (a >= e) return;
+  char* d = S.dst + a;
+  const char* src = S.src >= 0 ? image + S.src + a : nullptr;
+  const long long n = e - a;
+  const bool v16 = ((reinterpret_cast<uintptr_t>(d) | (src ? reinterpret_cast<uintptr_t>(src) : 0)) & 15) == 0;
+  if (v16) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const long long n16 = n / 16;
+    for (long long i = threadIdx.x; i < n16; i += blockDim.x)
+      reinterpret_cast<u4*>(d)[i] = src ? reinterpret_cast<const u4*>(src)[i] : u4{0, 0, 0, 0};
+    for (long long i = n16 * 4 + threadIdx.x; i < n / 4; i += blockDim.x)
+      reinterpret_cast<unsigned*>(d)[i] = src ? reinterpret_cast<const unsigned*>(src)[i] : 0u;
+  } else {
+    for (long long i = threadIdx.x; i < n / 4; i += blockDim.x)
+      reinterpret_cast<unsigned*>(d)[i] = src ? reinterpret_cast<const unsigned*>(src)[i] : 0u;
+  }
+  if (threadIdx.x == 0)
+    for (long long i = n & ~3ll; i < n; ++i) d[i] = src ? src[i] : 0;  // a piece of odd size
+}
+void launch_stage_scatter(hipStream_t st, const char* image, const StageSeg* segs, int nseg, long long nblk) {
+  if (nseg > 0 && nblk > 0) k_stage_scatter<<<(unsigned)nblk, 256, 0, st>>>(image, segs, nseg);
+}
 void launch_assemble_union(hipStream_t st, const InstDev* I, int inst, const int64_t* pos, const double* lval,
                            const double* qval, int64_t nent, int withq) {
   if (nent) k_assemble_union<<<cdiv(nent, TILE), TILE, 0, st>>>(I, inst, pos, lval, qval, nent, withq);

@@ -36,43 +36,112 @@ static const double FORCING_FRAC = 0.1;  // solver/solver.jl:3
 // device bytes allocated through DevBuf by the solver being built on this thread
 static thread_local size_t* g_dev_counter = nullptr;
 
-// Uploads of the solver being set up on this thread: each is copied once into a pinned host chunk
-// and its asynchronous copy queued at once; the setup waits for them all at its end (an upload
-// from pageable memory waits for its staging copy, and a wait for the stream per buffer costs a
-// host-device round trip: ~50 buffers per solver, with other solvers' kernels on the device).
+// Uploads of the solver being set up on this thread. Each is copied into a pinned host image and
+// recorded; so is each buffer the setup zeroes. flush() moves the image to the device in one copy
+// per pinned chunk and then places every piece (copies and zero fills) with one kernel. A setup
+// used to issue some 50 copies and 20 fills, each a small blit kernel on the solver's stream; while
+// other solvers' persistent kernels held every CU, each of them waited for a free one in turn.
 namespace {
 void* pinned_alloc(size_t c);
 void pinned_free(void* p, size_t c);
 size_t size_class(size_t b);
+void* dev_alloc(size_t b);
+void dev_free(void* p, size_t b);
 struct Stager {
   static constexpr size_t CHUNK = size_t(4) << 20;
   hipStream_t st = nullptr;
   std::vector<std::pair<char*, size_t>> chunks;  // pinned block, its size class
-  size_t used = 0;                                // bytes used in the last chunk
+  std::vector<size_t> base;                      // each chunk's offset in the image
+  size_t used = 0;                               // bytes used in the last chunk
+  std::vector<StageSeg> segs;
+  std::vector<std::pair<void*, size_t>> images;  // device images of earlier flushes (until finish)
   void add(void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
-    if (chunks.empty() || used + bytes > chunks.back().second) {
-      const size_t c = size_class(std::max(CHUNK, bytes));
+    const size_t span = (bytes + 255) / 256 * 256;  // pieces start 256-byte aligned in the image
+    if (chunks.empty() || used + span > chunks.back().second) {
+      const size_t c = size_class(std::max(CHUNK, span));
+      base.push_back(chunks.empty() ? 0 : base.back() + used);
       chunks.emplace_back(static_cast<char*>(pinned_alloc(c)), c);
       used = 0;
     }
-    char* h = chunks.back().first + used;
-    std::memcpy(h, src, bytes);
-    used += (bytes + 255) / 256 * 256;
-    HIPCHK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, st));
+    std::memcpy(chunks.back().first + used, src, bytes);
+    segs.push_back(StageSeg{static_cast<char*>(dst), (long long)(base.back() + used), (long long)bytes, 0});
+    used += span;
   }
-  // wait for the copies, then the chunks go back to the cache
-  void finish() {
-    hipError_t e = chunks.empty() ? hipSuccess : hipStreamSynchronize(st);
-    for (auto& c : chunks) pinned_free(c.first, c.second);
+  void zero(void* dst, size_t bytes) {
+    if (bytes) segs.push_back(StageSeg{static_cast<char*>(dst), -1, (long long)bytes, 0});
+  }
+  // the pieces recorded so far, in stream order before anything queued after this call: the zero
+  // fills in one launch, then the copies in another (a setup zeroes a buffer before it writes
+  // parts of it; the pieces of one launch are placed in no particular order)
+  void flush() {
+    if (segs.empty()) return;
+    std::vector<StageSeg> tab[2];  // copies, zero fills
+    long long nblk[2] = {0, 0};
+    for (const StageSeg& g : segs) {
+      const int z = g.src < 0 ? 1 : 0;
+      tab[z].push_back(g);
+      tab[z].back().blk0 = nblk[z];
+      nblk[z] += (g.bytes + STAGE_BLOCK - 1) / STAGE_BLOCK;
+    }
+    // the piece tables go up with the image, in a chunk of their own
+    const size_t n0 = tab[0].size(), n1 = tab[1].size();
+    const size_t tb = (n0 + n1) * sizeof(StageSeg);
+    const size_t img = chunks.empty() ? 0 : base.back() + used;
+    const size_t tc = size_class(std::max<size_t>(tb, 256));
+    char* tp = static_cast<char*>(pinned_alloc(tc));
+    if (n0) std::memcpy(tp, tab[0].data(), n0 * sizeof(StageSeg));
+    if (n1) std::memcpy(tp + n0 * sizeof(StageSeg), tab[1].data(), n1 * sizeof(StageSeg));
+    const size_t total = img + tb;
+    char* d = static_cast<char*>(dev_alloc(total));
+    images.emplace_back(d, total);
+    for (size_t k = 0; k < chunks.size(); ++k) {
+      const size_t n = k + 1 < chunks.size() ? base[k + 1] - base[k] : used;
+      if (n) HIPCHK(hipMemcpyAsync(d + base[k], chunks[k].first, n, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(d + img, tp, tb, hipMemcpyHostToDevice, st));
+    const StageSeg* dt = reinterpret_cast<const StageSeg*>(d + img);
+    launch_stage_scatter(st, d, dt + n0, (int)n1, nblk[1]);
+    launch_stage_scatter(st, d, dt, (int)n0, nblk[0]);
+    // the pinned blocks are read by the copies: kept until finish() has synchronised
+    chunks.emplace_back(tp, tc);
+    retired.insert(retired.end(), chunks.begin(), chunks.end());
     chunks.clear();
+    base.clear();
     used = 0;
+    segs.clear();
+  }
+  std::vector<std::pair<char*, size_t>> retired;
+  // flush, wait for the device, then the pinned and device blocks go back to the caches
+  void finish() {
+    hipError_t e = hipSuccess;
+    try {
+      flush();
+    } catch (...) {
+      e = hipErrorUnknown;
+    }
+    if (!retired.empty() || !images.empty()) {
+      const hipError_t w = hipStreamSynchronize(st);
+      if (e == hipSuccess) e = w;
+    }
+    release();
     HIPCHK(e);
   }
+  void release() {
+    for (auto& c : retired) pinned_free(c.first, c.second);
+    for (auto& c : chunks) pinned_free(c.first, c.second);
+    for (auto& m : images) dev_free(m.first, m.second);
+    retired.clear();
+    chunks.clear();
+    images.clear();
+    segs.clear();
+    base.clear();
+    used = 0;
+  }
   ~Stager() {
-    if (!chunks.empty()) {
+    if (!retired.empty() || !images.empty() || !chunks.empty()) {
       (void)hipStreamSynchronize(st);
-      for (auto& c : chunks) pinned_free(c.first, c.second);
+      release();
     }
   }
 };
@@ -177,6 +246,10 @@ DevBuf::~DevBuf() {
 }
 void DevBuf::alloc(size_t b) {
   if (p) {
+    if (g_stager) {  // a staged piece may still name the old block: place them all first
+      g_stager->flush();
+      HIPCHK(hipStreamSynchronize(g_stager->st));
+    }
     dev_free(p, bytes);
     p = nullptr;
   }
@@ -189,8 +262,7 @@ void DevBuf::alloc(size_t b) {
 // device has finished them) is not ordered before the solver's kernels.
 template <class T>
 void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
-  if (g_stager) {  // during device setup: the copy is queued, the setup waits for all at its end
-    if (p) HIPCHK(hipStreamSynchronize(st));  // a copy into the old buffer may still be queued
+  if (g_stager) {  // during device setup: staged, placed with the others (alloc flushes first)
     alloc(std::max<size_t>(v.size() * sizeof(T), sizeof(T)));
     g_stager->add(p, v.data(), v.size() * sizeof(T));
     return;
@@ -199,6 +271,13 @@ void DevBuf::upload(const std::vector<T>& v, hipStream_t st) {
   alloc(std::max<size_t>(v.size() * sizeof(T), sizeof(T)));
   if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));  // synchronous: callers pass temporaries
+}
+
+// zeros in the whole buffer, staged during device setup
+void DevBuf::clear(hipStream_t st) {
+  if (!p) return;
+  if (g_stager) g_stager->zero(p, bytes);
+  else HIPCHK(hipMemsetAsync(p, 0, bytes, st));
 }
 
 // the element types other translation units upload (the gang's combined tables)
@@ -753,13 +832,13 @@ void Solver::build_device() {
   d_bws_.alloc(sizeof(double) * std::max<i64>(wofs[N], 1));
   d_qdiag_.upload(qdiag, st_);
   d_uvec_.alloc(sizeof(double) * std::max<i64>(uofs[NF], 1));
-  HIPCHK(hipMemsetAsync(d_nv_.p, 0, d_nv_.bytes, st_));
-  HIPCHK(hipMemsetAsync(d_mv_.p, 0, d_mv_.bytes, st_));
-  HIPCHK(hipMemsetAsync(d_sc_.p, 0, d_sc_.bytes, st_));
-  HIPCHK(hipMemsetAsync(d_cache_.p, 0, d_cache_.bytes, st_));
-  HIPCHK(hipMemsetAsync(d_bws_.p, 0, d_bws_.bytes, st_));
-  HIPCHK(hipMemsetAsync(d_D_.p, 0, d_D_.bytes, st_));
-  if (fill_free_) HIPCHK(hipMemsetAsync(d_F_.p, 0, d_F_.bytes, st_));  // once: assembly then only assigns
+  d_nv_.clear(st_);
+  d_mv_.clear(st_);
+  d_sc_.clear(st_);
+  d_cache_.clear(st_);
+  d_bws_.clear(st_);
+  d_D_.clear(st_);
+  if (fill_free_) d_F_.clear(st_);  // once: assembly then only assigns
   d_sepcols_.upload(allsep, st_);
   d_relidx_.upload(relidx, st_);
   d_childlist_.upload(childlist, st_);
@@ -873,7 +952,7 @@ void Solver::build_device() {
   d_mtofs_.upload(std::vector<int>{0, ntile_m_}, st_);
   d_partial_.alloc(sizeof(double) * (std::max<i64>({(i64)ntile_n_, (i64)ntile_m_, NF, 1}) + 16));
   d_dotcnt_.alloc(sizeof(int) * 2);
-  HIPCHK(hipMemsetAsync(d_dotcnt_.p, 0, d_dotcnt_.bytes, st_));
+  d_dotcnt_.clear(st_);
   d_coneout_.alloc(sizeof(double) * 3 * std::max<i64>(N, 1));
   // instance table
   idev_.nv = d_nv_.as<double>();
@@ -892,7 +971,10 @@ void Solver::build_device() {
   g_stager->add(idev_.nv + (size_t)NV_PSCL * n_, pscl_.data(), n_ * 8);
   g_stager->add(idev_.mv + (size_t)MV_G * m_, P_.g.data(), m_ * 8);
   g_stager->add(idev_.mv + (size_t)MV_YSCL * m_, yscl_.data(), m_ * 8);
-  set_scalar(S_ACTIVE, 1.0);
+  // staged like the rest: a kernel queued now would run before the staged zero fill of the scalars
+  const double one = 1.0;
+  g_stager->add(d_sc_.as<double>() + S_ACTIVE, &one, sizeof(double));
+  hsc_[S_ACTIVE] = 1.0;
 }
 
 void Solver::build_schedule() {
@@ -915,8 +997,8 @@ void Solver::build_schedule() {
     if (fr_[f].ld >= set_.big_front_ld) ++nbig;
   d_bigtick_.alloc(sizeof(int) * 2 * std::max(nbig, 1));
   d_bigstick_.alloc(sizeof(long long) * std::max(nbig, 1));
-  HIPCHK(hipMemsetAsync(d_bigtick_.p, 0, d_bigtick_.bytes, st_));
-  HIPCHK(hipMemsetAsync(d_bigstick_.p, 0, d_bigstick_.bytes, st_));
+  d_bigtick_.clear(st_);
+  d_bigstick_.clear(st_);
   {
     int dev = 0, ncu = 256;
     HIPCHK(hipGetDevice(&dev));
@@ -964,14 +1046,14 @@ void Solver::build_schedule() {
       bh->tb.upload(tb, st_);
       bh->tasks.upload(tasks, st_);
       bh->state.alloc(sizeof(int) * std::max(1, T * (T + 1) / 2));
-      HIPCHK(hipMemsetAsync(bh->state.p, 0, bh->state.bytes, st_));
+      bh->state.clear(st_);
       bh->dinv.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
       if (set_.precision == 1) {  // FP32 factor columns (ld × r) and inverted diagonal blocks
         bh->f32.alloc(sizeof(float) * (size_t)Fd.ld * std::max(1, Fd.r));
         bh->dinv32.alloc(sizeof(float) * NB * NB * std::max(1, Tr));
       }
       bh->stage.alloc(sizeof(int) * std::max(1, Tr));
-      HIPCHK(hipMemsetAsync(bh->stage.p, 0, bh->stage.bytes, st_));
+      bh->stage.clear(st_);
       bh->sll.alloc(sizeof(unsigned long long) * 2 * NB * std::max(1, T));
       bh->spart.alloc(sizeof(double) * NB * std::max(1, Tr) * std::max(1, (Fd.s + NB - 1) / NB));
       bh->mtile.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
@@ -979,7 +1061,7 @@ void Solver::build_schedule() {
       // write-once factor storage: the FP64 factor columns beside F, the staged sub-diagonal tiles
       bh->lsep.alloc(sizeof(double) * (size_t)Fd.ld * std::max(1, Fd.r));
       bh->stg.alloc(sizeof(double) * NB * NB * std::max(1, Tr));
-      HIPCHK(hipMemsetAsync(bh->sll.p, 0, bh->sll.bytes, st_));
+      bh->sll.clear(st_);
       BigFront& B = bh->B;
       B.F = Fd.F;
       B.ld = Fd.ld;
@@ -1078,7 +1160,7 @@ void Solver::build_schedule() {
   }
   d_steps_.upload(steps, st_);
   d_panflag_.alloc(sizeof(int) * std::max<size_t>(steps.size(), 1));
-  HIPCHK(hipMemsetAsync(d_panflag_.p, 0, d_panflag_.bytes, st_));
+  d_panflag_.clear(st_);
   d_ea_items_.upload(ea_items, st_);
   d_ea_tile0_.upload(ea_tile0, st_);
   d_solve_items_.upload(solve_items, st_);

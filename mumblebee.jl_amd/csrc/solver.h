@@ -68,6 +68,7 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
   void alloc(size_t b);
+  void clear(hipStream_t st);
   template <class T>
   T* as() const { return (T*)p; }
   template <class T>
