@@ -315,9 +315,11 @@ int mb_solver_solve_in(mb_solver* s, mb_gang* g);
  * solver's slot taken by the next one; status[i] = IPMStatus of solver i (−1: failed). Returns 0,
  * or −1 if any failed (mb_last_error: the first message). */
 int mb_solve_batch(mb_solver* const* s, int32_t n, mb_gang* g, int32_t* status);
-/* out[14] = ticks, launches, ops, ops in combined launches, read-backs, joins, combined
+/* out[18] = ticks, launches, ops, ops in combined launches, read-backs, joins, combined
  * factorizations, their large fronts, their algorithmic flops, their time in ms (profile ≠ 0),
- * persistent factorization launches, flops in them, their time in ms, members now. */
+ * persistent factorization launches, flops in them, their time in ms, members now; then the
+ * scheduler thread's ms admitting members, running their fibers, issuing ticks (read-backs
+ * included) and waiting for the device in read-backs. */
 int mb_gang_stats(mb_gang* g, double* out);
 
 /* Seconds on the steady clock the server's timestamps use. */

@@ -666,9 +666,10 @@ int mb_gang_stats(mb_gang* g, double* out) {
   return guard([&]() {
     mb::GangStats st;
     g->G->stats(st);
-    const double v[14] = {st.ticks, st.launches, st.ops, st.combined_ops, st.readbacks, st.joins,
+    const double v[18] = {st.ticks, st.launches, st.ops, st.combined_ops, st.readbacks, st.joins,
                           st.factor_launches, st.fronts_per_factor, st.factor_flops, st.factor_ms,
-                          st.dag_launches, st.dag_flops, st.dag_ms, (double)g->G->members()};
+                          st.dag_launches, st.dag_flops, st.dag_ms, (double)g->G->members(),
+                          st.admit_ms, st.fiber_ms, st.tick_ms, st.wait_ms};
     std::memcpy(out, v, sizeof(v));
     return 0;
   });

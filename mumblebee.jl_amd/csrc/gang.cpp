@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <queue>
 #include <stdexcept>
@@ -342,6 +343,25 @@ void Gang::admit() {
   rebuild();
 }
 
+bool Gang::try_admit() {
+  std::string fail;
+  try {
+    admit();
+  } catch (const std::exception& e) {
+    fail = e.what();
+  }
+  if (fail.empty()) return true;
+  // the waiting jobs cannot be admitted: they fail
+  std::lock_guard<std::mutex> lk(mu_);
+  for (Job* j : joining_) {
+    j->err = fail;
+    j->done = true;
+  }
+  joining_.clear();
+  cv_done_.notify_all();
+  return false;
+}
+
 void Gang::sched_main() {
   for (;;) {
     bool fibers = false;
@@ -351,23 +371,11 @@ void Gang::sched_main() {
       cv_.wait(lk, [&] { return stop_ || !joining_.empty(); });
       if (joining_.empty()) return;  // stop_ with nothing left
     }
-    std::string fail;
-    try {
-      admit();
-    } catch (const std::exception& e) {
-      fail = e.what();
-    }
-    if (!fail.empty()) {
-      // the waiting jobs cannot be admitted: they fail
-      std::lock_guard<std::mutex> lk(mu_);
-      for (Job* j : joining_) {
-        j->err = fail;
-        j->done = true;
-      }
-      joining_.clear();
-      cv_done_.notify_all();
-      if (nrun_ == 0) continue;
-    }
+    const auto ta = std::chrono::steady_clock::now();
+    const bool failed = !try_admit();
+    const auto tf = std::chrono::steady_clock::now();
+    st_acc_.admit_ms += std::chrono::duration<double, std::milli>(tf - ta).count();
+    if (failed && nrun_ == 0) continue;
     // every runnable member runs until it waits on an op (or ends)
     for (int k = 0; k < kmax_; ++k) {
       Fiber* f = fib_[k].get();
@@ -383,14 +391,26 @@ void Gang::sched_main() {
         cv_done_.notify_all();
       }
     }
+    const auto tt = std::chrono::steady_clock::now();
+    st_acc_.fiber_ms += std::chrono::duration<double, std::milli>(tt - tf).count();
     // ticks while every member has an op queued
     while (ready()) {
+      factor_issued_ = false;
       try {
         tick();
       } catch (const std::exception& e) {
         fail_fronts(e.what());
       }
+      // solvers waiting to join are admitted while the combined factorization just issued runs
+      // (their tables are rebuilt and uploaded behind it) rather than with the device idle after
+      // a read-back; they start at the next round
+      if (factor_issued_) {
+        const auto ta2 = std::chrono::steady_clock::now();
+        (void)try_admit();
+        st_acc_.admit_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta2).count();
+      }
     }
+    st_acc_.tick_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
     {
       std::lock_guard<std::mutex> lk(mu_);
       st_pub_ = st_acc_;
@@ -478,6 +498,7 @@ void Gang::tick() {
     else rest.push_back(k);
   }
   std::vector<char> used(kmax_, 0);
+  bool fac = false;
   for (size_t i = 0; i < rest.size(); ++i) {
     const int k = rest[i];
     if (used[k]) continue;
@@ -494,6 +515,7 @@ void Gang::tick() {
     st_acc_.ops += (double)g.size();
     const int kind = op_[k]->kind;
     const bool heavy = kind == GK_FACTOR || kind == GK_LSOLVE || kind == GK_USOLVE;
+    fac = fac || kind == GK_FACTOR;
     // a lone cheap op runs as the member's own launch; a lone factorization or solve still takes
     // the combined path, whose persistent kernels use every CU
     if (g.size() == 1 && !(heavy && can_combine(k, *op_[k]))) {
@@ -508,6 +530,7 @@ void Gang::tick() {
     }
   }
   if (!rb.empty()) readbacks(rb);
+  factor_issued_ = fac && rb.empty();
   for (int k : lv) {
     Solver* s = mem_[k];
     set_mask(dev_mask_ & ~(1ull << k));
@@ -593,7 +616,9 @@ void Gang::readbacks(const std::vector<int>& slots) {
   for (int k : slots) mask |= 1ull << k;
   launch_gang_gather(st_, T_.inst, kmax_, mask, d_gather_.as<double>());
   HIPCHK(hipMemcpyAsync(h_gather_, d_gather_.p, sizeof(double) * NSCALAR * kmax_, hipMemcpyDeviceToHost, st_));
+  const auto tw = std::chrono::steady_clock::now();
   HIPCHK(hipStreamSynchronize(st_));
+  st_acc_.wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
   passed();  // the stream has passed every launch that could read them
   for (int k : slots) std::memcpy(mem_[k]->hsc_, h_gather_ + (size_t)k * NSCALAR, sizeof(double) * NSCALAR);
   st_acc_.readbacks += 1;

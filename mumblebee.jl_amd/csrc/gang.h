@@ -98,6 +98,9 @@ struct GangStats {
   double ticks = 0, launches = 0, ops = 0, combined_ops = 0, readbacks = 0, joins = 0, members = 0;
   double factor_ms = 0, factor_flops = 0, factor_launches = 0, fronts_per_factor = 0;
   double dag_ms = 0, dag_launches = 0, dag_flops = 0;
+  // the scheduler thread's time (ms): admitting members (tables rebuilt and uploaded), running the
+  // members' fibers (their host logic), issuing ticks, and waiting in read-backs for the device
+  double admit_ms = 0, fiber_ms = 0, tick_ms = 0, wait_ms = 0;
 };
 
 class Gang {
@@ -128,6 +131,8 @@ class Gang {
   void fail_fronts(const std::string& what);
   void tick();      // the front op of every member: one tick
   void admit();
+  bool try_admit();  // admit(); on failure the waiting jobs fail. False if nothing was waiting
+  bool factor_issued_ = false;  // the last tick issued a combined factorization and no read-back
   void rebuild();
   void set_mask(uint64_t mask);
   void issue_group(const std::vector<int>& slots);

@@ -820,11 +820,12 @@ class IPMGang:
         return [s.result() for s in solvers]
 
     def stats(self):
-        out = np.zeros(14)
+        out = np.zeros(18)
         if lib().mb_gang_stats(self._h, _p(out, ctypes.c_double)) != 0:
             raise _err("gang stats")
         keys = ("ticks", "launches", "ops", "combined_ops", "readbacks", "joins", "factor_launches",
-                "factor_fronts", "factor_flops", "factor_ms", "dag_launches", "dag_flops", "dag_ms", "members")
+                "factor_fronts", "factor_flops", "factor_ms", "dag_launches", "dag_flops", "dag_ms", "members",
+                "admit_ms", "fiber_ms", "tick_ms", "wait_ms")
         return dict(zip(keys, out.tolist()))
 
     def close(self):
