@@ -290,8 +290,6 @@ def _e2e_run(args, mh, precision):
         nxt = [lo]
 
         def worker():
-            if args.host_nice:  # host setup below the gang's scheduler thread
-                os.setpriority(os.PRIO_PROCESS, threading.get_native_id(), args.host_nice)
             while True:
                 with lock:
                     i = nxt[0]
@@ -530,8 +528,6 @@ def main():
     ap.add_argument("--gang", type=int, default=16,
                     help="members of the lockstep gang the solves run in (0: each solver on its own stream)")
     ap.add_argument("--no-loop", action="store_true", help="skip the resident-loop measurement")
-    ap.add_argument("--host-nice", type=int, default=0,
-                    help="niceness of the host threads that run solve(prob, settings) (0: unchanged)")
     ap.add_argument("--loop-only", action="store_true",
                     help="time the IPM loop on instances set up before timing (init excluded and reported)")
     ap.add_argument("--dag-grid", type=int, default=0,
