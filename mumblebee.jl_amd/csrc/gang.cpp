@@ -763,11 +763,12 @@ void Gang::rebuild() {
     Solver* s = mem_[k];
     if (!s) continue;
     A.F = s->d_F_.as<double>();
-    A.nF = s->fill_free_ ? 0 : s->fsize_;  // a front that needs no clearing gets no fill blocks
+    const bool part = s->fill_free_ && s->static_ok_;  // no clearing; only the δ-dependent entries
+    A.nF = part ? 0 : s->fsize_;
     A.pos = s->d_upos_.as<int64_t>();
     A.l = s->d_ul_.as<double>();
     A.q = s->d_uq_.as<double>();
-    A.nent = s->nunion_;
+    A.nent = part ? s->nunion_dyn_ : s->nunion_;
     fb += (A.nF / 2 + TILE_THREADS - 1) / TILE_THREADS + (A.nF & 1 ? 1 : 0);
     ub += (A.nent + TILE_THREADS - 1) / TILE_THREADS;
   }

@@ -603,6 +603,12 @@ void Solver::device_setup(const double* p0, const double* d0, const double* y0) 
     build_schedule();
     g_stager = nullptr;
     stage.finish();
+    if (fill_free_) {  // the front cleared and every entry of the pattern written once
+      launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
+      launch_assemble_union(st_, d_inst_.as<InstDev>(), 0, d_upos_.as<int64_t>(), d_ul_.as<double>(),
+                            d_uq_.as<double>(), nunion_, 1);
+      static_ok_ = true;
+    }
     dev_parts_[2] = wall_ms() - t0 - dev_parts_[0] - dev_parts_[1];
   } catch (...) {
     g_stager = nullptr;
@@ -783,6 +789,30 @@ void Solver::build_device() {
     }
     fill_free_ = ok;
   }
+  // For such a front the entries that do not change with δ (no Q part, off the diagonal) are
+  // written once; the assembly rewrites only the rest, which go first
+  nunion_dyn_ = nunion_;
+  if (fill_free_) {
+    const i64 ld1 = fr_[0].ld + 1;
+    std::vector<int64_t> p2;
+    std::vector<double> l2, q2;
+    p2.reserve(upos.size());
+    l2.reserve(upos.size());
+    q2.reserve(upos.size());
+    for (int pass = 0; pass < 2; ++pass) {
+      for (size_t e = 0; e < upos.size(); ++e) {
+        const bool dyn = uq[e] != 0.0 || upos[e] % ld1 == 0;
+        if (dyn != (pass == 0)) continue;
+        p2.push_back(upos[e]);
+        l2.push_back(ul[e]);
+        q2.push_back(uq[e]);
+      }
+      if (pass == 0) nunion_dyn_ = (i64)p2.size();
+    }
+    upos.swap(p2);
+    ul.swap(l2);
+    uq.swap(q2);
+  }
   bd_parts_[1] = wall_ms() - tb0;
   // ‖Q‖² off the diagonal blocks
   qoff2_ = 0.0;
@@ -838,7 +868,6 @@ void Solver::build_device() {
   d_cache_.clear(st_);
   d_bws_.clear(st_);
   d_D_.clear(st_);
-  if (fill_free_) d_F_.clear(st_);  // once: assembly then only assigns
   d_sepcols_.upload(allsep, st_);
   d_relidx_.upload(relidx, st_);
   d_childlist_.upload(childlist, st_);
@@ -1749,13 +1778,17 @@ void Solver::prof_get(double* out) const {
 // F ← BᵀB + δH (the solver's H: Q and the cone blocks; or the caller's H)
 void Solver::raw_assemble() {
   const InstDev* I = d_inst_.as<InstDev>();
-  if (!fill_free_ || user_h_) launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
+  const bool part = fill_free_ && static_ok_ && !user_h_;  // only the entries that change with δ
+  if (!part) launch_fill(st_, d_F_.as<double>(), (size_t)fsize_, 0.0);
   if (user_h_) {  // BᵀB + δ (the caller's H)
     launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 0);
     launch_scatter_add(st_, I, 0, d_hpos_.as<int64_t>(), d_hval_.as<double>(), nhent_);
+    static_ok_ = false;  // the front holds the caller's H now
     return;
   }
-  launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(), nunion_, 1);
+  launch_assemble_union(st_, I, 0, d_upos_.as<int64_t>(), d_ul_.as<double>(), d_uq_.as<double>(),
+                        part ? nunion_dyn_ : nunion_, 1);
+  if (fill_free_) static_ok_ = true;
   launch_bpsd_assemble(st_, I, d_cones_.as<ConeDesc>(), bpsd(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>());
   launch_assemble_cones(st_, I, d_cones_.as<ConeDesc>(), d_cone_front_.as<int>(), d_fr_.as<FrontDesc>(), ncone_,
                         cone_nmax_);
@@ -1775,7 +1808,7 @@ void Solver::factor_run(bool fp32) {
     op.kind = GK_FACTOR;
     op.park = true;
     op.a[0] = fp32 ? 1 : 0;
-    op.solo_only = user_h_ || nbpsd_ > 0;
+    op.solo_only = user_h_ || nbpsd_ > 0 || (fill_free_ && !static_ok_);
     op.solo = [this, fp32] { raw_factor_run(fp32); };
     gang_submit(&op);
     return;

@@ -342,6 +342,10 @@ class Solver {
   // the front needs no clearing before an assembly (one large front with its factor beside it,
   // no separator or children, diagonal cone blocks, every diagonal entry in the union pattern)
   bool fill_free_ = false;
+  // fill_free_: the entries of the pattern that do not change with δ are in the front (written at
+  // setup, and again by any full assembly); the union's first nunion_dyn_ entries are the rest
+  bool static_ok_ = false;
+  int64_t nunion_dyn_ = 0;
   CSRDev hsave_{};  // the CSR not in use (Q's while the caller's H is)
   DevBuf d_Hp_, d_Hc_, d_Hv_, d_htiles_, d_hpos_, d_hval_;
   int nhtiles_ = 0;
